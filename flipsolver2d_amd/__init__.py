@@ -1,0 +1,296 @@
+"""flipsolver2d_amd -- MI355X-native FLIP substep of FlipSolver2d behind a C-ABI.
+
+This module is a thin ctypes binding of libflipgpu.so (include/flipgpu.h) used
+by the tests and bench.py.  The C++ facade with the reference's class names
+lives in flipsolver2d_amd/cpp/.  There is no CPU fallback: if the HIP library
+is missing or no GPU is visible, every compute entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libflipgpu.so")
+
+FG_OK = 0
+FLUID, SOURCE, SOLID, SINK, EMPTY = 0x40, 0x41, 0x20, 0x12, 0x10
+DENSITY_OFF, DENSITY_GATED, DENSITY_ON = 0, 1, 2
+(STAGE_ADVECT, STAGE_REBIN, STAGE_DENSITY, STAGE_P2G, STAGE_SDF, STAGE_LEVELSET, STAGE_EXTRAPOLATE,
+ STAGE_FORCES, STAGE_PROJECT, STAGE_SOLIDS, STAGE_G2P, STAGE_COUNT) = range(12)
+STAGE_NAMES = ["advect", "rebin", "density", "p2g", "sdf", "levelset", "extrapolate", "forces", "project",
+               "solids", "g2p", "count"]
+
+EXPORTS = [
+    "fg_last_error", "fg_abi_version", "fg_ctx_create", "fg_ctx_destroy", "fg_ctx_bytes_in_use",
+    "fg_ctx_set_timing", "fg_scene_bytes", "fg_scene_create", "fg_scene_destroy", "fg_scene_init_geometry",
+    "fg_scene_set_friction", "fg_scene_set_state", "fg_scene_get_state", "fg_scene_particle_count",
+    "fg_scene_substep", "fg_scene_step_frame", "fg_scene_stage", "fg_batch_step_frame", "fg_poisson_solve",
+]
+
+
+class FgError(RuntimeError):
+    pass
+
+
+class SceneParams(C.Structure):
+    _fields_ = [
+        ("size_i", C.c_int32), ("size_j", C.c_int32),
+        ("dx", C.c_double), ("fluid_density", C.c_double),
+        ("gravity_i", C.c_float), ("gravity_j", C.c_float),
+        ("pic_ratio", C.c_float), ("cfl_number", C.c_float), ("particle_scale", C.c_float),
+        ("fps", C.c_int32), ("max_substeps", C.c_int32), ("particles_per_cell", C.c_int32),
+        ("pcg_iter_limit", C.c_int32), ("density_mode", C.c_int32),
+        ("solver_tolerance", C.c_double), ("max_particles", C.c_int64),
+    ]
+
+
+class SceneState(C.Structure):
+    _fields_ = [
+        ("n_particles", C.c_int64),
+        ("px", C.c_void_p), ("py", C.c_void_p), ("vx", C.c_void_p), ("vy", C.c_void_p), ("pid", C.c_void_p),
+        ("u", C.c_void_p), ("v", C.c_void_p), ("u_valid", C.c_void_p), ("v_valid", C.c_void_p),
+        ("material", C.c_void_p), ("solid_sdf", C.c_void_p), ("fluid_sdf", C.c_void_p),
+        ("solid_id", C.c_void_p), ("particle_counts", C.c_void_p),
+        ("saved_u", C.c_void_p), ("saved_v", C.c_void_p), ("pressure", C.c_void_p), ("rhs", C.c_void_p),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("substeps", C.c_int32), ("pressure_iters", C.c_int32), ("pressure_sweeps", C.c_int32),
+        ("density_iters", C.c_int32), ("density_applied", C.c_int32), ("converged", C.c_int32),
+        ("pressure_rel_resid", C.c_double),
+        ("particles", C.c_int64), ("particle_substeps", C.c_int64),
+        ("particles_killed", C.c_int64), ("particles_erased", C.c_int64),
+        ("last_dt", C.c_float), ("max_velocity", C.c_float),
+        ("stage_ms", C.c_double * 12), ("frame_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "stage_ms"}
+        d["stage_ms"] = {STAGE_NAMES[i]: self.stage_ms[i] for i in range(12)}
+        return d
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("sweeps", C.c_int32), ("converged", C.c_int32),
+                ("rel_residual", C.c_double), ("solve_ms", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libflipgpu.so (built by flipsolver2d_amd.build); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FgError(f"{LIB_PATH} is missing: run `python -m flipsolver2d_amd.build` first")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, f32, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_double
+        sig = {
+            "fg_last_error": (C.c_char_p, []),
+            "fg_abi_version": (i32, []),
+            "fg_ctx_create": (C.c_int, [i32, C.c_uint64, C.POINTER(vp)]),
+            "fg_ctx_destroy": (C.c_int, [vp]),
+            "fg_ctx_bytes_in_use": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
+            "fg_ctx_set_timing": (C.c_int, [vp, i32]),
+            "fg_scene_bytes": (C.c_int, [C.POINTER(SceneParams), C.POINTER(C.c_uint64)]),
+            "fg_scene_create": (C.c_int, [vp, C.POINTER(SceneParams), C.POINTER(vp)]),
+            "fg_scene_destroy": (C.c_int, [vp]),
+            "fg_scene_init_geometry": (C.c_int, [vp, i32, vp, vp, vp, i32, vp, vp, C.c_uint32]),
+            "fg_scene_set_friction": (C.c_int, [vp, i32, vp]),
+            "fg_scene_set_state": (C.c_int, [vp, C.POINTER(SceneState)]),
+            "fg_scene_get_state": (C.c_int, [vp, C.POINTER(SceneState)]),
+            "fg_scene_particle_count": (C.c_int, [vp, C.POINTER(i64)]),
+            "fg_scene_substep": (C.c_int, [vp, f32, C.POINTER(Stats)]),
+            "fg_scene_step_frame": (C.c_int, [vp, C.POINTER(Stats)]),
+            "fg_scene_stage": (C.c_int, [vp, i32, f32]),
+            "fg_batch_step_frame": (C.c_int, [vp, vp, i32, vp]),
+            "fg_poisson_solve": (C.c_int, [vp, i32, i32, vp, f64, vp, vp, f64, i32, C.POINTER(SolveStats)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != FG_OK:
+        raise FgError(f"flipgpu error {rc}: {lib().fg_last_error().decode(errors='replace')}")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def make_params(n_i, n_j=None, *, dx=None, ppc=4, density=1000.0, gravity=(9.81, 0.0), pic=0.03, cfl=5.0,
+                particle_scale=0.8, fps=30, max_substeps=10, pcg_iter_limit=2000, density_mode=DENSITY_OFF,
+                tol=1e-6, max_particles=None):
+    n_j = n_i if n_j is None else n_j
+    p = SceneParams()
+    p.size_i, p.size_j = n_i, n_j
+    p.dx = (1.0 / max(n_i, n_j)) if dx is None else dx
+    p.fluid_density = density
+    p.gravity_i, p.gravity_j = gravity
+    p.pic_ratio, p.cfl_number, p.particle_scale = pic, cfl, particle_scale
+    p.fps, p.max_substeps, p.particles_per_cell = fps, max_substeps, ppc
+    p.pcg_iter_limit, p.density_mode, p.solver_tolerance = pcg_iter_limit, density_mode, tol
+    p.max_particles = n_i * n_j * ppc if max_particles is None else max_particles
+    return p
+
+
+class Context:
+    def __init__(self, device=0, hbm_budget=0):
+        self.h = C.c_void_p()
+        _check(lib().fg_ctx_create(device, hbm_budget, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().fg_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def set_timing(self, on=True):
+        _check(lib().fg_ctx_set_timing(self.h, 1 if on else 0))
+
+    def bytes_in_use(self):
+        v = C.c_uint64()
+        _check(lib().fg_ctx_bytes_in_use(self.h, C.byref(v)))
+        return v.value
+
+    def poisson_solve(self, material, scale, b, tol=1e-6, max_iters=0):
+        material = np.ascontiguousarray(material, np.int8)
+        b = np.ascontiguousarray(b, np.float64)
+        I, J = material.shape
+        x = np.zeros(I * J, np.float64)
+        st = SolveStats()
+        _check(lib().fg_poisson_solve(self.h, I, J, material.ctypes.data, scale, b.ctypes.data, x.ctypes.data, tol,
+                                      max_iters, C.byref(st)))
+        return x, st
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Scene:
+    def __init__(self, ctx: Context, params: SceneParams):
+        self.ctx, self.params = ctx, params
+        self.I, self.J = params.size_i, params.size_j
+        self.h = C.c_void_p()
+        _check(lib().fg_scene_create(ctx.h, C.byref(params), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().fg_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def init_geometry(self, solids, fluids, friction=None, seed=42):
+        snv = np.array([len(w) for w in solids] or [0], np.int32)
+        sv = np.ascontiguousarray(np.concatenate(solids).ravel() if solids else np.zeros(2), np.float32)
+        fnv = np.array([len(f) for f in fluids] or [0], np.int32)
+        fv = np.ascontiguousarray(np.concatenate(fluids).ravel() if fluids else np.zeros(2), np.float32)
+        fr = None if friction is None else np.ascontiguousarray(friction, np.float32)
+        _check(lib().fg_scene_init_geometry(self.h, len(solids), snv.ctypes.data, sv.ctypes.data, _ptr(fr),
+                                            len(fluids), fnv.ctypes.data, fv.ctypes.data, seed))
+
+    def set_friction(self, friction):
+        fr = np.ascontiguousarray(friction, np.float32)
+        _check(lib().fg_scene_set_friction(self.h, len(fr), fr.ctypes.data))
+
+    def set_state(self, st: dict):
+        s = SceneState()
+        keep = {}
+        for name, _ in SceneState._fields_:
+            if name == "n_particles":
+                continue
+            if name in st and st[name] is not None:
+                dt = {"px": np.float32, "py": np.float32, "vx": np.float32, "vy": np.float32, "pid": np.uint32,
+                      "u": np.float32, "v": np.float32, "u_valid": np.uint8, "v_valid": np.uint8,
+                      "material": np.int8, "solid_sdf": np.float32, "fluid_sdf": np.float32,
+                      "solid_id": np.int32}.get(name)
+                if dt is None:
+                    continue
+                a = np.ascontiguousarray(st[name], dt)
+                keep[name] = a
+                setattr(s, name, a.ctypes.data)
+        s.n_particles = len(keep.get("px", ()))
+        _check(lib().fg_scene_set_state(self.h, C.byref(s)))
+
+    def get_state(self):
+        I, J = self.I, self.J
+        n = self.particle_capacity_hint()
+        out = {
+            "px": np.zeros(n, np.float32), "py": np.zeros(n, np.float32), "vx": np.zeros(n, np.float32),
+            "vy": np.zeros(n, np.float32), "pid": np.zeros(n, np.uint32),
+            "u": np.zeros((I + 1) * J, np.float32), "v": np.zeros(I * (J + 1), np.float32),
+            "u_valid": np.zeros((I + 1) * J, np.uint8), "v_valid": np.zeros(I * (J + 1), np.uint8),
+            "material": np.zeros(I * J, np.int8), "solid_sdf": np.zeros(I * J, np.float32),
+            "fluid_sdf": np.zeros(I * J, np.float32), "solid_id": np.zeros(I * J, np.int32),
+            "particle_counts": np.zeros(I * J, np.int32), "saved_u": np.zeros((I + 1) * J, np.float32),
+            "saved_v": np.zeros(I * (J + 1), np.float32), "pressure": np.zeros(I * J, np.float64),
+            "rhs": np.zeros(I * J, np.float64),
+        }
+        s = SceneState()
+        for k, a in out.items():
+            setattr(s, k, a.ctypes.data)
+        _check(lib().fg_scene_get_state(self.h, C.byref(s)))
+        n = s.n_particles
+        for k in ("px", "py", "vx", "vy", "pid"):
+            out[k] = out[k][:n]
+        return out
+
+    def particle_capacity_hint(self):
+        return int(self.params.max_particles)
+
+    def particle_count(self):
+        v = C.c_int64()
+        _check(lib().fg_scene_particle_count(self.h, C.byref(v)))
+        return v.value
+
+    def substep(self, dt):
+        st = Stats()
+        _check(lib().fg_scene_substep(self.h, dt, C.byref(st)))
+        return st
+
+    def step_frame(self):
+        st = Stats()
+        _check(lib().fg_scene_step_frame(self.h, C.byref(st)))
+        return st
+
+    def stage(self, stage, dt=0.0):
+        _check(lib().fg_scene_stage(self.h, stage, dt))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def box(x0, x1, y0, y1):
+    return np.array([[x0, y0], [x0, y1], [x1, y1], [x1, y0]], np.float32)
+
+
+def dam_break_geometry(n, fluid_box=(0.3, 1.0, 0.0, 0.4)):
+    """Dam break in a 1x1 m box (x along I): one-cell solid walls, fluid rectangle clipped to the interior."""
+    dx = 1.0 / n
+    walls = [box(0.0, dx, 0.0, 1.0), box(1.0 - dx, 1.0, 0.0, 1.0), box(0.0, 1.0, 0.0, dx), box(0.0, 1.0, 1.0 - dx, 1.0)]
+    x0, x1, y0, y1 = fluid_box
+    fluid = [box(max(x0, dx), min(x1, 1.0 - dx), max(y0, dx), min(y1, 1.0 - dx))]
+    return walls, fluid
+
+
+def dam_break_scene(ctx, n, ppc=4, seed=42, **kw):
+    p = make_params(n, ppc=ppc, **kw)
+    sc = Scene(ctx, p)
+    walls, fluid = dam_break_geometry(n)
+    sc.init_geometry(walls, fluid, friction=np.zeros(len(walls), np.float32), seed=seed)
+    return sc

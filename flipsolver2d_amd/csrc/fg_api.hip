@@ -1,0 +1,964 @@
+// fg_api.hip -- host orchestration behind the C-ABI in include/flipgpu.h.
+//
+// A scene owns one device arena (a single hipMalloc carved into aligned
+// buffers) and a device copy of its SceneDev descriptor.  A substep is the
+// reference's FlipSolver::step (flipsolver2d.cpp:412-462) as a fixed
+// sequence of kernel launches on the context's stream; the only host round
+// trips are the per-substep read of max |v| (to form dt exactly as
+// FlipSolver::stepFrame does, flipsolver2d.cpp:476-490) and the solver's
+// convergence polls.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fg_kernels.h"
+#include "flipgpu.h"
+
+using namespace fg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(FG_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+#define FGCHK(expr)               \
+    do {                          \
+        int r_ = (expr);          \
+        if (r_ != FG_OK) return r_; \
+    } while (0)
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+struct fg_ctx {
+    int device = 0;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+};
+
+struct fg_scene {
+    fg_ctx* ctx = nullptr;
+    fg_scene_params p{};
+    bool solver_only = false;
+    int I = 0, J = 0;
+    int64_t C = 0, FU = 0, FV = 0, cap = 0;
+    int max_blocks = 0;
+    void* arena = nullptr;
+    uint64_t bytes = 0;
+    SceneDev h{};                 // host copy of the descriptor
+    SceneDev* d = nullptr;        // device descriptor
+    SceneDev* hpin = nullptr;     // pinned staging for descriptor uploads
+    SceneScalars* hsc = nullptr;  // pinned readback of the scalars
+    int64_t n_bound = 0;          // upper bound of live particles (host mirror)
+    int64_t n_live = 0;           // live particles at the last readback
+    bool vmax_ready = false;
+    int next_stage = 0;
+    int frame = 0;
+};
+
+namespace {
+
+// --------------------------------------------------------------------------
+// arena layout
+// --------------------------------------------------------------------------
+struct Carver {
+    uint64_t off = 0;
+    char* base = nullptr;
+    template <typename T>
+    T* take(int64_t count)
+    {
+        off = (off + 255) & ~uint64_t(255);
+        T* ptr = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += sizeof(T) * (uint64_t)std::max<int64_t>(count, 1);
+        return ptr;
+    }
+};
+
+void carve(fg_scene* s, Carver& cv)
+{
+    SceneDev& h = s->h;
+    const int64_t C = s->C, FU = s->FU, FV = s->FV, P = s->cap;
+    const int64_t nb = s->max_blocks;
+    // solver (always)
+    h.mat = cv.take<int8_t>(C);
+    h.stencil = cv.take<uint8_t>(C);
+    h.b = cv.take<double>(C);
+    h.x64 = cv.take<double>(C);
+    h.x64b = nullptr;
+    h.r = cv.take<float>(C);
+    h.r2 = cv.take<float>(C);
+    h.p = cv.take<float>(C);
+    h.p2 = cv.take<float>(C);
+    h.z = cv.take<float>(C);
+    h.q = cv.take<float>(C);
+    h.d = cv.take<float>(C);
+    h.partials = cv.take<double>(kSlots * nb);
+    h.counters = cv.take<int32_t>(kSlots);
+    h.sc = cv.take<SceneScalars>(1);
+    if (s->solver_only) return;
+    h.pos = cv.take<float2>(P);
+    h.vel = cv.take<float2>(P);
+    h.posA = cv.take<float2>(P);
+    h.velA = cv.take<float2>(P);
+    h.pid = cv.take<uint32_t>(P);
+    h.key = cv.take<uint32_t>(P);
+    h.tmpPid = cv.take<uint32_t>(P);
+    h.tmpSrc = cv.take<uint32_t>(P);
+    h.cellCount = cv.take<int32_t>(C);
+    h.cellStart = cv.take<int32_t>(C + 1);
+    h.scanSums = cv.take<int32_t>(cdiv(C, kScanTileCells));
+    h.ssdf = cv.take<float>(C);
+    h.fsdf = cv.take<float>(C);
+    h.sid = cv.take<int32_t>(C);
+    h.pcount = cv.take<int32_t>(C);
+    h.marker = cv.take<int32_t>(C);
+    h.lsFlags = cv.take<int32_t>(kMaxBands);
+    h.u = cv.take<float>(FU);
+    h.su = cv.take<float>(FU);
+    h.uf = cv.take<float>(FU);
+    h.v = cv.take<float>(FV);
+    h.sv = cv.take<float>(FV);
+    h.vf = cv.take<float>(FV);
+    h.uval = cv.take<uint8_t>(FU);
+    h.uvalS = cv.take<uint8_t>(FU);
+    h.uvalF = cv.take<uint8_t>(FU);
+    h.vval = cv.take<uint8_t>(FV);
+    h.vvalS = cv.take<uint8_t>(FV);
+    h.vvalF = cv.take<uint8_t>(FV);
+    h.friction = cv.take<float>(256);
+}
+
+uint64_t arena_bytes(fg_scene* s)
+{
+    Carver cv;
+    SceneDev keep = s->h;
+    carve(s, cv);
+    s->h = keep;
+    return cv.off + 256;
+}
+
+int validate_params(const fg_scene_params* p)
+{
+    if (!p) return fail(FG_E_INVALID, "params is NULL");
+    if (p->size_i < 2 || p->size_j < 2) return fail(FG_E_INVALID, "grid must be at least 2x2");
+    if ((int64_t)p->size_i * p->size_j > (int64_t)1 << 30) return fail(FG_E_INVALID, "grid too large");
+    if (!(p->dx > 0) || !(p->fluid_density > 0)) return fail(FG_E_INVALID, "dx and fluid_density must be > 0");
+    if (p->fps <= 0) return fail(FG_E_INVALID, "fps must be > 0");
+    if (p->max_particles < 0 || p->max_particles > ((int64_t)1 << 31) - 1)
+        return fail(FG_E_INVALID, "max_particles out of range");
+    if (p->density_mode < 0 || p->density_mode > 2) return fail(FG_E_INVALID, "bad density_mode");
+    return FG_OK;
+}
+
+void init_geometry_fields(fg_scene* s)
+{
+    s->I = s->p.size_i;
+    s->J = s->p.size_j;
+    s->C = (int64_t)s->I * s->J;
+    s->FU = (int64_t)(s->I + 1) * s->J;
+    s->FV = (int64_t)s->I * (s->J + 1);
+    s->cap = s->solver_only ? 0 : s->p.max_particles;
+    s->max_blocks = (int)cdiv(s->C, kBlock);
+}
+
+int upload_desc(fg_scene* s)
+{
+    *s->hpin = s->h;
+    HIPCHK(hipMemcpyAsync(s->d, s->hpin, sizeof(SceneDev), hipMemcpyHostToDevice, s->ctx->stream));
+    return FG_OK;
+}
+
+int read_scalars(fg_scene* s)
+{
+    HIPCHK(hipMemcpyAsync(s->hsc, s->h.sc, sizeof(SceneScalars), hipMemcpyDeviceToHost, s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return FG_OK;
+}
+
+int create_scene(fg_ctx* ctx, const fg_scene_params* p, bool solver_only, fg_scene** out)
+{
+    FGCHK(validate_params(p));
+    auto* s = new fg_scene();
+    s->ctx = ctx;
+    s->p = *p;
+    s->solver_only = solver_only;
+    init_geometry_fields(s);
+    s->bytes = arena_bytes(s);
+    if (ctx->budget && ctx->used + s->bytes > ctx->budget) {
+        const uint64_t need = s->bytes;
+        delete s;
+        return fail(FG_E_BUDGET, "scene needs " + std::to_string(need) + " bytes beyond the HBM budget");
+    }
+    hipError_t e = hipMalloc(&s->arena, s->bytes);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(FG_E_DEVICE, std::string("hipMalloc arena: ") + hipGetErrorString(e));
+    }
+    e = hipMalloc(&s->d, sizeof(SceneDev));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->hpin, sizeof(SceneDev), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->hsc, sizeof(SceneScalars), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        hipFree(s->arena);
+        if (s->d) hipFree(s->d);
+        if (s->hpin) hipHostFree(s->hpin);
+        delete s;
+        return fail(FG_E_DEVICE, std::string("descriptor alloc: ") + hipGetErrorString(e));
+    }
+    Carver cv;
+    cv.base = reinterpret_cast<char*>(s->arena);
+    carve(s, cv);
+    SceneDev& h = s->h;
+    h.I = s->I;
+    h.J = s->J;
+    h.ppc = p->particles_per_cell;
+    h.n_friction = 0;
+    h.pic = p->pic_ratio;
+    h.pscale = p->particle_scale;
+    h.g_i = p->gravity_i;
+    h.g_j = p->gravity_j;
+    h.dx = p->dx;
+    h.rho = p->fluid_density;
+    h.tol = p->solver_tolerance > 0 ? p->solver_tolerance : 1e-6;
+    h.max_inner = p->pcg_iter_limit > 0 ? p->pcg_iter_limit : 2000;
+    h.active = 1;
+    h.max_blocks = s->max_blocks;
+    h.cap = s->cap;
+    ctx->used += s->bytes;
+    hipStream_t st = ctx->stream;
+    HIPCHK(hipMemsetAsync(s->arena, 0, s->bytes, st));
+    if (!solver_only) {
+        // empty, all-EMPTY scene until state arrives
+        HIPCHK(hipMemsetAsync(h.mat, (int)(int8_t)FG_EMPTY, s->C, st));
+        HIPCHK(hipMemsetAsync(h.sid, 0xff, sizeof(int32_t) * s->C, st));
+    }
+    FGCHK(upload_desc(s));
+    HIPCHK(hipStreamSynchronize(st));
+    *out = s;
+    return FG_OK;
+}
+
+// --------------------------------------------------------------------------
+// launch geometry
+// --------------------------------------------------------------------------
+struct Geo {
+    dim3 cells, parts, scan, scan1, faces, extrap, ls, one;
+};
+
+Geo geometry(const fg_scene* s)
+{
+    Geo g;
+    g.cells = dim3((unsigned)cdiv(s->C, kBlock), 1);
+    g.parts = dim3((unsigned)std::max<int64_t>(1, cdiv(std::max<int64_t>(s->n_bound, 1), kBlock)), 1);
+    g.scan = dim3((unsigned)cdiv(s->C, kScanTileCells), 1);
+    g.scan1 = dim3(1, 1);
+    g.faces = dim3((unsigned)cdiv(std::max(s->FU, s->FV), kBlock), 1);
+    const int64_t tu = cdiv(s->I + 1, kBfsTile) * cdiv(s->J, kBfsTile);
+    const int64_t tv = cdiv(s->I, kBfsTile) * cdiv(s->J + 1, kBfsTile);
+    g.extrap = dim3((unsigned)std::max(tu, tv), 1, 2);
+    g.ls = dim3((unsigned)(cdiv(s->I, kBfsTile) * cdiv(s->J, kBfsTile)), 1);
+    g.one = dim3(1, 1);
+    return g;
+}
+
+void set_substep_scalars(fg_scene* s, float dt)
+{
+    SceneDev& h = s->h;
+    h.dt = dt;
+    h.force = (float)(dt / s->p.dx);                                   // flipsolver2d.cpp:1224
+    h.sA = dt / (s->p.fluid_density * s->p.dx * s->p.dx);             // flipsolver2d.cpp:799
+    h.sApply = dt / (s->p.fluid_density * s->p.dx);                   // flipsolver2d.cpp:1135
+    h.sRhs = 1.f / s->p.dx;                                            // flipsolver2d.cpp:964
+}
+
+// ---------------------------------------------------------------------------
+// tiny per-scene scalar kernels
+// ---------------------------------------------------------------------------
+__global__ void k_substep_begin(const SceneDev* scenes)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (threadIdx.x != 0 || !s.active) return;
+    s.sc->killed = 0;
+    s.sc->erased = 0;
+}
+__global__ void k_vmax_reset(const SceneDev* scenes)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (threadIdx.x != 0 || !s.active) return;
+    s.sc->max_v2_bits = __float_as_uint(1.17549435e-38f);   // FLT_MIN, flipsolver2d.cpp:1562
+}
+__global__ void k_set_count(const SceneDev* scenes, int n)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (threadIdx.x != 0) return;
+    s.sc->n_particles = n;
+}
+inline void launch_vmax_reset_(dim3 g, hipStream_t st, const SceneDev* d)
+{
+    hipLaunchKernelGGL(k_vmax_reset, g, dim3(64), 0, st, d);
+}
+
+// ---------------------------------------------------------------------------
+// stage timing
+// ---------------------------------------------------------------------------
+struct Timer {
+    fg_ctx* ctx;
+    fg_stats* st;
+    hipEvent_t a = nullptr;
+    Timer(fg_ctx* c, fg_stats* s) : ctx(c), st(s)
+    {
+        if (ctx->timing && st) {
+            hipEventCreate(&a);
+            hipEventRecord(a, ctx->stream);
+        }
+    }
+    void lap(int stage)
+    {
+        if (!(ctx->timing && st)) return;
+        hipEvent_t b;
+        hipEventCreate(&b);
+        hipEventRecord(b, ctx->stream);
+        hipEventSynchronize(b);
+        float ms = 0;
+        hipEventElapsedTime(&ms, a, b);
+        st->stage_ms[stage] += ms;
+        hipEventDestroy(a);
+        a = b;
+    }
+    ~Timer()
+    {
+        if (a) hipEventDestroy(a);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// the solve (shared by project and fg_poisson_solve); b is in place
+// ---------------------------------------------------------------------------
+int run_solve(fg_scene* s, bool rhs_from_grid, int32_t* iters_out, int32_t* sweeps_out)
+{
+    hipStream_t st = s->ctx->stream;
+    const Geo g = geometry(s);
+    launch_solve_reset(g.one, st, s->d);
+    if (rhs_from_grid) launch_rhs(g.cells, st, s->d);
+    else launch_solve_begin(g.cells, st, s->d);
+    constexpr int kPoll = 16;
+    for (int sweep = 0; sweep <= kMaxSweeps; sweep++) {
+        if (sweep > 0) launch_refine_update(g.cells, st, s->d);
+        launch_refine_residual(g.cells, st, s->d);
+        FGCHK(read_scalars(s));
+        if (s->hsc->outer_done) break;
+        launch_inner_init(g.cells, st, s->d);
+        int it = 0;
+        for (;;) {
+            for (int k = 0; k < kPoll; k++, it++) {
+                launch_pcg_a(g.cells, st, s->d, it & 1);
+                launch_pcg_b(g.cells, st, s->d, it & 1);
+            }
+            FGCHK(read_scalars(s));
+            if (s->hsc->inner_done || s->hsc->outer_done) break;
+            if (it > s->h.max_inner + kPoll) return fail(FG_E_STATE, "inner solve did not stop at its cap");
+        }
+    }
+    if (iters_out) *iters_out = s->hsc->total_iters;
+    if (sweeps_out) *sweeps_out = s->hsc->sweeps;
+    if (s->hsc->numeric_bad) return fail(FG_E_NUMERIC, "NaN/Inf in the pressure solve");
+    return FG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// stages of FlipSolver::step
+// ---------------------------------------------------------------------------
+int run_stage(fg_scene* s, int stage, fg_stats* stats)
+{
+    hipStream_t st = s->ctx->stream;
+    const Geo g = geometry(s);
+    const dim3& one = g.one;
+    switch (stage) {
+    case FG_STAGE_ADVECT:
+        launch_advect(g.parts, st, s->d, 1);
+        break;
+    case FG_STAGE_REBIN:
+        launch_stencil(g.cells, st, s->d);          // matrix from the pre-update materials
+        launch_scan_local(g.scan, st, s->d);
+        launch_scan_top(g.scan1, st, s->d);
+        launch_scan_add(g.scan, st, s->d);
+        launch_scatter(g.parts, st, s->d);
+        launch_fixup_gather(g.cells, st, s->d);
+        launch_commit_count(one, st, s->d);
+        break;
+    case FG_STAGE_DENSITY:
+        if (s->p.density_mode != FG_DENSITY_OFF)
+            return fail(FG_E_STATE, "density correction modes other than OFF are not built yet");
+        break;
+    case FG_STAGE_P2G:
+        launch_p2g(g.cells, st, s->d);
+        break;
+    case FG_STAGE_SDF:
+        launch_sdf_materials(g.cells, st, s->d);
+        break;
+    case FG_STAGE_LEVELSET: {
+        launch_levelset_init(g.cells, st, s->d);
+        const int max_bands = (int)std::min<int64_t>(kMaxBands, cdiv(std::max(s->I, s->J), 16) + 1);
+        constexpr int kChunk = 4;
+        for (int band = 0; band < max_bands; band += kChunk) {
+            for (int k = band; k < std::min(max_bands, band + kChunk); k++) launch_levelset_band(g.ls, st, s->d, k);
+            int32_t flag = 0;
+            HIPCHK(hipMemcpyAsync(&flag, s->h.lsFlags + std::min(max_bands, band + kChunk) - 1, sizeof(int32_t),
+                                  hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (!flag) break;
+        }
+        break;
+    }
+    case FG_STAGE_EXTRAPOLATE:
+        launch_extrapolate(g.extrap, st, s->d, 0);  // u -> su (saved)
+        break;
+    case FG_STAGE_FORCES:
+        launch_forces(g.faces, st, s->d);
+        break;
+    case FG_STAGE_PROJECT: {
+        int32_t it = 0, sw = 0;
+        const int r = run_solve(s, true, &it, &sw);
+        if (stats) {
+            stats->pressure_iters = std::max(stats->pressure_iters, it);
+            stats->pressure_sweeps = std::max(stats->pressure_sweeps, sw);
+            const double bb = s->hsc->bb;
+            stats->pressure_rel_resid = bb > 0 ? std::sqrt(s->hsc->rr64 / bb) : 0.0;
+            if (stats->pressure_rel_resid > s->h.tol) stats->converged = 0;
+        }
+        FGCHK(r);
+        launch_apply_solids(g.cells, st, s->d);
+        break;
+    }
+    case FG_STAGE_SOLIDS:
+        // fused into the pressure application kernel (k_apply_solids)
+        break;
+    case FG_STAGE_G2P:
+        launch_extrapolate(g.extrap, st, s->d, 1);  // u -> uf (final)
+        launch_vmax_reset_(one, st, s->d);
+        launch_g2p_count(g.parts, st, s->d);
+        break;
+    case FG_STAGE_COUNT:
+        // countParticles runs inside k_g2p_count; reseeding fills SOURCE
+        // cells only (flipsolver2d.cpp:627-680), which this build has none of
+        break;
+    default:
+        return fail(FG_E_INVALID, "bad stage");
+    }
+    return FG_OK;
+}
+
+int substep(fg_scene* s, float dt, fg_stats* stats)
+{
+    hipStream_t st = s->ctx->stream;
+    set_substep_scalars(s, dt);
+    FGCHK(upload_desc(s));
+    hipLaunchKernelGGL(k_substep_begin, dim3(1), dim3(64), 0, st, s->d);
+    Timer tm(s->ctx, stats);
+    static const int order[] = {FG_STAGE_ADVECT, FG_STAGE_REBIN, FG_STAGE_DENSITY, FG_STAGE_P2G,
+                                FG_STAGE_SDF, FG_STAGE_LEVELSET, FG_STAGE_EXTRAPOLATE, FG_STAGE_FORCES,
+                                FG_STAGE_PROJECT, FG_STAGE_SOLIDS, FG_STAGE_G2P, FG_STAGE_COUNT};
+    const int64_t n_start = s->n_live;
+    for (int stage : order) {
+        FGCHK(run_stage(s, stage, stats));
+        tm.lap(stage);
+    }
+    HIPCHK(hipGetLastError());
+    FGCHK(read_scalars(s));
+    const SceneScalars& sc = *s->hsc;
+    s->n_live = sc.n_particles - sc.erased;
+    s->n_bound = sc.n_particles;
+    s->vmax_ready = true;
+    if (stats) {
+        stats->substeps += 1;
+        stats->particle_substeps += n_start;
+        stats->particles = s->n_live;
+        stats->particles_killed += sc.killed;
+        stats->particles_erased += sc.erased;
+        stats->last_dt = dt;
+    }
+    return FG_OK;
+}
+
+int ensure_vmax(fg_scene* s)
+{
+    if (s->vmax_ready) return FG_OK;
+    const Geo g = geometry(s);
+    launch_vmax_reset_(g.one, s->ctx->stream, s->d);
+    launch_max_v2(g.parts, s->ctx->stream, s->d);
+    FGCHK(read_scalars(s));
+    s->vmax_ready = true;
+    return FG_OK;
+}
+
+void stats_reset(fg_stats* st)
+{
+    if (!st) return;
+    std::memset(st, 0, sizeof(*st));
+    st->converged = 1;
+}
+
+// counter-based uniform in [0,1): 24 bits of a murmur3-finalised hash
+inline uint32_t fmix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+inline float jitter(uint32_t seed, uint32_t cell, uint32_t k, uint32_t axis)
+{
+    const uint32_t h = fmix32(seed ^ fmix32(cell * 0x9e3779b1u ^ fmix32(k * 2u + axis + 0x632be5abu)));
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+
+// Geometry2d::signedDistance, geometry2d.cpp:54-73 (iq polygon SDF).  The
+// zero z term of Vec3::dot is kept; sqrt is taken in double as there.
+float polygon_sdf(const float* vt, int N, float px, float py)
+{
+    auto dot = [](float ax, float ay, float bx, float by) { return (ax * bx + ay * by) + 0.f * 0.f; };
+    const float dx0 = px - vt[0], dy0 = py - vt[1];
+    float sq = dot(dx0, dy0, dx0, dy0);
+    float sgn = 1.0f;
+    for (int i = 0, j = N - 1; i < N; j = i, i++) {
+        const float ex = vt[2 * j] - vt[2 * i], ey = vt[2 * j + 1] - vt[2 * i + 1];
+        const float wx = px - vt[2 * i], wy = py - vt[2 * i + 1];
+        const float h = std::clamp(dot(wx, wy, ex, ey) / dot(ex, ey, ex, ey), 0.0f, 1.0f);
+        const float bx = wx - ex * h, by = wy - ey * h;
+        sq = std::min(sq, dot(bx, by, bx, by));
+        const bool xf = py >= vt[2 * i + 1];
+        const bool yf = py < vt[2 * j + 1];
+        const bool zf = ex * wy > ey * wx;
+        if ((xf && yf && zf) || (!xf && !yf && !zf)) sgn *= -1.0f;
+    }
+    return (float)(sgn * std::sqrt((double)sq));
+}
+
+}  // namespace
+
+// ============================================================================
+// C-ABI
+// ============================================================================
+extern "C" {
+
+const char* fg_last_error(void) { return g_err.c_str(); }
+int32_t fg_abi_version(void) { return FG_ABI_VERSION; }
+
+int fg_ctx_create(int32_t device, uint64_t hbm_budget_bytes, fg_ctx** out)
+{
+    if (!out) return fail(FG_E_INVALID, "out is NULL");
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(FG_E_INVALID, "no such device");
+    HIPCHK(hipSetDevice(device));
+    auto* c = new fg_ctx();
+    c->device = device;
+    c->budget = hbm_budget_bytes;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(FG_E_DEVICE, std::string("stream: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return FG_OK;
+}
+
+int fg_ctx_destroy(fg_ctx* ctx)
+{
+    if (!ctx) return FG_OK;
+    hipStreamSynchronize(ctx->stream);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FG_OK;
+}
+
+int fg_ctx_bytes_in_use(fg_ctx* ctx, uint64_t* out)
+{
+    if (!ctx || !out) return fail(FG_E_INVALID, "NULL argument");
+    *out = ctx->used;
+    return FG_OK;
+}
+
+int fg_ctx_set_timing(fg_ctx* ctx, int32_t enabled)
+{
+    if (!ctx) return fail(FG_E_INVALID, "ctx is NULL");
+    ctx->timing = enabled != 0;
+    return FG_OK;
+}
+
+int fg_scene_bytes(const fg_scene_params* p, uint64_t* out)
+{
+    FGCHK(validate_params(p));
+    if (!out) return fail(FG_E_INVALID, "out is NULL");
+    fg_scene tmp;
+    tmp.p = *p;
+    init_geometry_fields(&tmp);
+    *out = arena_bytes(&tmp);
+    return FG_OK;
+}
+
+int fg_scene_create(fg_ctx* ctx, const fg_scene_params* p, fg_scene** out)
+{
+    if (!ctx || !out) return fail(FG_E_INVALID, "NULL argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    return create_scene(ctx, p, false, out);
+}
+
+int fg_scene_destroy(fg_scene* s)
+{
+    if (!s) return FG_OK;
+    hipStreamSynchronize(s->ctx->stream);
+    hipFree(s->arena);
+    hipFree(s->d);
+    hipHostFree(s->hpin);
+    hipHostFree(s->hsc);
+    s->ctx->used -= s->bytes;
+    delete s;
+    return FG_OK;
+}
+
+int fg_scene_set_friction(fg_scene* s, int32_t n, const float* friction)
+{
+    if (!s || (n > 0 && !friction)) return fail(FG_E_INVALID, "NULL argument");
+    if (n < 0 || n > 256) return fail(FG_E_INVALID, "at most 256 obstacles");
+    if (n > 0)
+        HIPCHK(hipMemcpyAsync(s->h.friction, friction, sizeof(float) * n, hipMemcpyHostToDevice, s->ctx->stream));
+    s->h.n_friction = n;
+    FGCHK(upload_desc(s));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return FG_OK;
+}
+
+int fg_scene_set_state(fg_scene* s, const fg_scene_state* st)
+{
+    if (!s || !st) return fail(FG_E_INVALID, "NULL argument");
+    if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
+    const int64_t n = st->n_particles;
+    if (n < 0 || n > s->cap) return fail(FG_E_CAPACITY, "n_particles exceeds the scene's max_particles");
+    if (n > 0 && (!st->px || !st->py || !st->vx || !st->vy))
+        return fail(FG_E_INVALID, "particle arrays are required");
+    if (!st->u || !st->v || !st->material || !st->solid_sdf)
+        return fail(FG_E_INVALID, "u, v, material and solid_sdf are required");
+    hipStream_t q = s->ctx->stream;
+    const SceneDev& h = s->h;
+    std::vector<float2> pos(n), vel(n);
+    std::vector<uint32_t> pid(n);
+    for (int64_t k = 0; k < n; k++) {
+        pos[k] = make_float2(st->px[k], st->py[k]);
+        vel[k] = make_float2(st->vx[k], st->vy[k]);
+        pid[k] = st->pid ? st->pid[k] : (uint32_t)k;
+    }
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(h.pos, pos.data(), sizeof(float2) * n, hipMemcpyHostToDevice, q));
+        HIPCHK(hipMemcpyAsync(h.vel, vel.data(), sizeof(float2) * n, hipMemcpyHostToDevice, q));
+        HIPCHK(hipMemcpyAsync(h.pid, pid.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, q));
+        HIPCHK(hipMemsetAsync(h.key, 0, sizeof(uint32_t) * n, q));
+    }
+    HIPCHK(hipMemcpyAsync(h.uf, st->u, sizeof(float) * s->FU, hipMemcpyHostToDevice, q));
+    HIPCHK(hipMemcpyAsync(h.vf, st->v, sizeof(float) * s->FV, hipMemcpyHostToDevice, q));
+    if (st->u_valid) HIPCHK(hipMemcpyAsync(h.uvalF, st->u_valid, s->FU, hipMemcpyHostToDevice, q));
+    else HIPCHK(hipMemsetAsync(h.uvalF, 0, s->FU, q));
+    if (st->v_valid) HIPCHK(hipMemcpyAsync(h.vvalF, st->v_valid, s->FV, hipMemcpyHostToDevice, q));
+    else HIPCHK(hipMemsetAsync(h.vvalF, 0, s->FV, q));
+    HIPCHK(hipMemcpyAsync(h.mat, st->material, s->C, hipMemcpyHostToDevice, q));
+    HIPCHK(hipMemcpyAsync(h.ssdf, st->solid_sdf, sizeof(float) * s->C, hipMemcpyHostToDevice, q));
+    if (st->fluid_sdf) HIPCHK(hipMemcpyAsync(h.fsdf, st->fluid_sdf, sizeof(float) * s->C, hipMemcpyHostToDevice, q));
+    if (st->solid_id) HIPCHK(hipMemcpyAsync(h.sid, st->solid_id, sizeof(int32_t) * s->C, hipMemcpyHostToDevice, q));
+    else HIPCHK(hipMemsetAsync(h.sid, 0xff, sizeof(int32_t) * s->C, q));
+    hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, q, s->d, (int)n);
+    HIPCHK(hipStreamSynchronize(q));
+    s->n_bound = n;
+    s->n_live = n;
+    s->vmax_ready = false;
+    s->next_stage = 0;
+    return FG_OK;
+}
+
+int fg_scene_get_state(fg_scene* s, fg_scene_state* st)
+{
+    if (!s || !st) return fail(FG_E_INVALID, "NULL argument");
+    if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
+    hipStream_t q = s->ctx->stream;
+    const SceneDev& h = s->h;
+    FGCHK(read_scalars(s));
+    const int64_t n = s->hsc->n_particles;
+    std::vector<float2> pos(n), vel(n);
+    std::vector<uint32_t> pid(n), key(n);
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(pos.data(), h.pos, sizeof(float2) * n, hipMemcpyDeviceToHost, q));
+        HIPCHK(hipMemcpyAsync(vel.data(), h.vel, sizeof(float2) * n, hipMemcpyDeviceToHost, q));
+        HIPCHK(hipMemcpyAsync(pid.data(), h.pid, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, q));
+        HIPCHK(hipMemcpyAsync(key.data(), h.key, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, q));
+    }
+    auto dl = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (dst) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, q));
+        return FG_OK;
+    };
+    FGCHK(dl(st->u, h.uf, sizeof(float) * s->FU));
+    FGCHK(dl(st->v, h.vf, sizeof(float) * s->FV));
+    FGCHK(dl(st->u_valid, h.uvalF, s->FU));
+    FGCHK(dl(st->v_valid, h.vvalF, s->FV));
+    FGCHK(dl(st->material, h.mat, s->C));
+    FGCHK(dl(st->solid_sdf, h.ssdf, sizeof(float) * s->C));
+    FGCHK(dl(st->fluid_sdf, h.fsdf, sizeof(float) * s->C));
+    FGCHK(dl(st->solid_id, h.sid, sizeof(int32_t) * s->C));
+    FGCHK(dl(st->particle_counts, h.pcount, sizeof(int32_t) * s->C));
+    FGCHK(dl(st->saved_u, h.su, sizeof(float) * s->FU));
+    FGCHK(dl(st->saved_v, h.sv, sizeof(float) * s->FV));
+    FGCHK(dl(st->pressure, h.x64, sizeof(double) * s->C));
+    FGCHK(dl(st->rhs, h.b, sizeof(double) * s->C));
+    HIPCHK(hipStreamSynchronize(q));
+    int64_t w = 0;
+    for (int64_t k = 0; k < n; k++) {
+        if (key[k] == kDead) continue;
+        if (st->px) st->px[w] = pos[k].x;
+        if (st->py) st->py[w] = pos[k].y;
+        if (st->vx) st->vx[w] = vel[k].x;
+        if (st->vy) st->vy[w] = vel[k].y;
+        if (st->pid) st->pid[w] = pid[k];
+        w++;
+    }
+    st->n_particles = w;
+    return FG_OK;
+}
+
+int fg_scene_particle_count(fg_scene* s, int64_t* out)
+{
+    if (!s || !out) return fail(FG_E_INVALID, "NULL argument");
+    *out = s->n_live;
+    return FG_OK;
+}
+
+int fg_scene_init_geometry(fg_scene* s, int32_t n_solids, const int32_t* snv, const float* sv,
+                           const float* friction, int32_t n_fluids, const int32_t* fnv, const float* fv,
+                           uint32_t seed)
+{
+    if (!s || (n_solids > 0 && (!snv || !sv)) || (n_fluids > 0 && (!fnv || !fv)))
+        return fail(FG_E_INVALID, "NULL argument");
+    if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
+    const int I = s->I, J = s->J;
+    const int64_t C = s->C;
+    std::vector<int8_t> mat(C, (int8_t)FG_EMPTY);
+    std::vector<int32_t> sid(C, -1);
+    std::vector<float> ssdf(C, 0.f);
+    const float dxf = (float)s->p.dx;
+    // FlipSolver::updateSolids, flipsolver2d.cpp:502-530
+    for (int i = 0; i < I; i++)
+        for (int j = 0; j < J; j++) {
+            float dist = std::numeric_limits<float>::max();
+            int min_idx = 0;
+            const float* v = sv;
+            for (int k = 0; k < n_solids; k++) {
+                const float x = (float)(((float)i + 0.5) * dxf);
+                const float y = (float)(((float)j + 0.5) * dxf);
+                const float sd = polygon_sdf(v, snv[k], x, y) / dxf;
+                if (sd < dist) {
+                    min_idx = k;
+                    dist = sd;
+                }
+                v += 2 * snv[k];
+            }
+            ssdf[(size_t)i * J + j] = dist;
+            if (dist < 0) {
+                mat[(size_t)i * J + j] = (int8_t)FG_SOLID;
+                sid[(size_t)i * J + j] = min_idx;
+            }
+        }
+    // FlipSolver::updateInitialFluid, flipsolver2d.cpp:573-590
+    for (int i = 0; i < I; i++)
+        for (int j = 0; j < J; j++) {
+            const float* v = fv;
+            for (int k = 0; k < n_fluids; k++) {
+                const float x = ((float)i + 0.5f) * dxf;
+                const float y = ((float)j + 0.5f) * dxf;
+                if (polygon_sdf(v, fnv[k], x, y) <= 0.f) mat[(size_t)i * J + j] = (int8_t)FG_FLUID;
+                v += 2 * fnv[k];
+            }
+        }
+    // seedInitialFluid (flipsolver2d.cpp:682-707): particle k of a strict-fluid
+    // cell is jittered uniformly inside quadrant k%4 of the cell
+    const int ppc = s->p.particles_per_cell;
+    std::vector<float> px, py, zero;
+    std::vector<uint32_t> pid;
+    for (int i = 0; i < I; i++)
+        for (int j = 0; j < J; j++) {
+            if (mat[(size_t)i * J + j] != (int8_t)FG_FLUID) continue;
+            const uint32_t cell = (uint32_t)((size_t)i * J + j);
+            for (int k = 0; k < ppc; k++) {
+                const int q = k & 3;
+                const float qi = (float)(q >> 1), qj = (float)(q & 1);
+                pid.push_back((uint32_t)px.size());
+                px.push_back((float)i + 0.5f * (qi + jitter(seed, cell, (uint32_t)k, 0)));
+                py.push_back((float)j + 0.5f * (qj + jitter(seed, cell, (uint32_t)k, 1)));
+            }
+        }
+    const int64_t n = (int64_t)px.size();
+    if (n > s->cap) return fail(FG_E_CAPACITY, "seeded particles exceed max_particles");
+    zero.assign(n, 0.f);
+    std::vector<float> u(s->FU, 0.f), v(s->FV, 0.f), fsdf(C, 0.f);
+    fg_scene_state st{};
+    st.n_particles = n;
+    st.px = px.data();
+    st.py = py.data();
+    st.vx = zero.data();
+    st.vy = zero.data();
+    st.pid = pid.data();
+    st.u = u.data();
+    st.v = v.data();
+    st.material = mat.data();
+    st.solid_sdf = ssdf.data();
+    st.fluid_sdf = fsdf.data();
+    st.solid_id = sid.data();
+    FGCHK(fg_scene_set_state(s, &st));
+    if (friction && n_solids > 0) FGCHK(fg_scene_set_friction(s, std::min(n_solids, 256), friction));
+    else FGCHK(fg_scene_set_friction(s, 0, nullptr));
+    return FG_OK;
+}
+
+int fg_scene_substep(fg_scene* s, float dt, fg_stats* stats)
+{
+    if (!s) return fail(FG_E_INVALID, "scene is NULL");
+    if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
+    if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FG_E_INVALID, "dt must be positive and finite");
+    stats_reset(stats);
+    return substep(s, dt, stats);
+}
+
+int fg_scene_step_frame(fg_scene* s, fg_stats* stats)
+{
+    if (!s) return fail(FG_E_INVALID, "scene is NULL");
+    if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
+    stats_reset(stats);
+    const auto t0 = std::chrono::steady_clock::now();
+    const float frame_dt = 1.f / s->p.fps;
+    float t = 0.f;
+    int count = 0;
+    bool finished = false;
+    while (!finished) {
+        FGCHK(ensure_vmax(s));
+        // FlipSolver::maxParticleVelocity + the substep split, flipsolver2d.cpp:476-490
+        float v2;
+        std::memcpy(&v2, &s->hsc->max_v2_bits, 4);
+        const float vel = std::sqrt(v2);
+        float mss = s->p.cfl_number / (vel + 1e-15f);
+        if (t + mss >= frame_dt || count == (s->p.max_substeps - 1)) {
+            mss = frame_dt - t;
+            finished = true;
+        } else if (t + 2.f * mss >= frame_dt) {
+            mss = 0.5f * (frame_dt - t);
+        }
+        if (stats) stats->max_velocity = vel;
+        FGCHK(substep(s, mss, stats));
+        t += mss;
+        count++;
+        if (count > 50) break;
+    }
+    s->frame++;
+    if (stats)
+        stats->frame_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return FG_OK;
+}
+
+int fg_scene_stage(fg_scene* s, int32_t stage, float dt)
+{
+    if (!s) return fail(FG_E_INVALID, "scene is NULL");
+    if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
+    if (stage < 0 || stage >= FG_STAGE_NUM) return fail(FG_E_INVALID, "bad stage");
+    if (stage == FG_STAGE_ADVECT) {
+        if (!(dt > 0.f)) return fail(FG_E_INVALID, "dt must be positive");
+        set_substep_scalars(s, dt);
+        FGCHK(upload_desc(s));
+        hipLaunchKernelGGL(k_substep_begin, dim3(1), dim3(64), 0, s->ctx->stream, s->d);
+    }
+    fg_stats tmp;
+    stats_reset(&tmp);
+    FGCHK(run_stage(s, stage, &tmp));
+    HIPCHK(hipGetLastError());
+    FGCHK(read_scalars(s));
+    if (stage == FG_STAGE_G2P) {
+        s->n_live = s->hsc->n_particles - s->hsc->erased;
+        s->n_bound = s->hsc->n_particles;
+        s->vmax_ready = true;
+    }
+    return FG_OK;
+}
+
+int fg_batch_step_frame(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, fg_stats* stats)
+{
+    if (!ctx || (n > 0 && !scenes)) return fail(FG_E_INVALID, "NULL argument");
+    for (int32_t k = 0; k < n; k++) FGCHK(fg_scene_step_frame(scenes[k], stats ? &stats[k] : nullptr));
+    return FG_OK;
+}
+
+int fg_poisson_solve(fg_ctx* ctx, int32_t size_i, int32_t size_j, const int8_t* material, double scale,
+                     const double* b, double* x, double rel_tol, int32_t max_iters, fg_solve_stats* stats)
+{
+    if (!ctx || !material || !b || !x) return fail(FG_E_INVALID, "NULL argument");
+    if (!(scale > 0)) return fail(FG_E_INVALID, "scale must be > 0");
+    fg_scene_params p{};
+    p.size_i = size_i;
+    p.size_j = size_j;
+    p.dx = 1.0;
+    p.fluid_density = 1.0;
+    p.fps = 1;
+    p.solver_tolerance = rel_tol > 0 ? rel_tol : 1e-6;
+    p.pcg_iter_limit = max_iters > 0 ? max_iters : 100000;
+    fg_scene* s = nullptr;
+    FGCHK(create_scene(ctx, &p, true, &s));
+    hipStream_t q = ctx->stream;
+    const int64_t C = s->C;
+    s->h.sA = scale;
+    int rc = upload_desc(s);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (rc == FG_OK) {
+        hipError_t e = hipMemcpyAsync(s->h.mat, material, C, hipMemcpyHostToDevice, q);
+        if (e == hipSuccess) e = hipMemcpyAsync(s->h.b, b, sizeof(double) * C, hipMemcpyHostToDevice, q);
+        if (e != hipSuccess) rc = fail(FG_E_DEVICE, hipGetErrorString(e));
+    }
+    int32_t it = 0, sw = 0;
+    if (rc == FG_OK) {
+        const Geo g = geometry(s);
+        launch_stencil(g.cells, q, s->d);
+        rc = run_solve(s, false, &it, &sw);
+    }
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (rc == FG_OK || rc == FG_E_NUMERIC) {
+        hipError_t e = hipMemcpyAsync(x, s->h.x64, sizeof(double) * C, hipMemcpyDeviceToHost, q);
+        if (e == hipSuccess) e = hipStreamSynchronize(q);
+        if (e != hipSuccess && rc == FG_OK) rc = fail(FG_E_DEVICE, hipGetErrorString(e));
+    }
+    if (stats) {
+        stats->iterations = it;
+        stats->sweeps = sw;
+        stats->rel_residual = s->hsc->bb > 0 ? std::sqrt(s->hsc->rr64 / s->hsc->bb) : 0.0;
+        stats->converged = stats->rel_residual <= p.solver_tolerance;
+        stats->solve_ms = ms;
+    }
+    fg_scene_destroy(s);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,262 @@
+// fg_device.h -- scene descriptor and device helpers shared by the kernels.
+//
+// All kernels are compiled with -ffp-contract=off so each float expression
+// rounds exactly as written; the helpers below keep the operation order of
+// the reference functions they cite (paths relative to the reference repo).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fg {
+
+constexpr int kBlock = 256;             // threads per block (4 wave64s)
+constexpr uint32_t kDead = 0xffffffffu;  // particle key of a removed particle
+constexpr int kMaxLevel = 0x3fffffff;    // "unreached" BFS marker
+constexpr int kMaxSweeps = 8;            // FP64 refinement sweeps per solve
+constexpr int kMaxBands = 512;           // level-set bands of 16 levels
+constexpr int kSlots = 5;                // reduction slots: bb, pq, rz, rr, rr64
+
+// Material codes and tests, FlipSolver2dLib/materialgrid.h:6-20.
+enum : int8_t { M_FLUID = 0x40, M_SOURCE = 0x41, M_SOLID = 0x20, M_SINK = 0x12, M_EMPTY = 0x10 };
+__device__ __forceinline__ bool is_fluid(int8_t m) { return (m & 0x40) != 0; }
+__device__ __forceinline__ bool is_strict_fluid(int8_t m) { return m == M_FLUID; }
+__device__ __forceinline__ bool is_empty(int8_t m) { return (m & 0x10) != 0; }
+__device__ __forceinline__ bool is_solid(int8_t m) { return (m & 0x20) != 0; }
+__device__ __forceinline__ bool is_sink(int8_t m) { return m == M_SINK; }
+
+// Scalars a scene's kernels exchange on the device (reductions, flags).
+struct SceneScalars {
+    uint32_t max_v2_bits;   // max |v|^2 over live particles (float bits, >= 0)
+    int32_t n_particles;    // live particles (sorted prefix of the particle arrays)
+    int32_t killed;         // advect kills this substep
+    int32_t erased;         // countParticles erasures this substep
+    // solver
+    double bb;              // ||b||^2
+    double rr64;            // ||b - A x||^2 (FP64 residual of current x)
+    double rz;              // inner r.z
+    double rz_new;
+    double pq;              // inner p.q
+    double rr0;             // inner ||r||^2 at sweep start
+    double rr;              // inner ||r||^2
+    double alpha, beta;
+    double inner_eps2;      // inner stop: ||r||^2 <= eps2 * ||r0||^2
+    int32_t inner_done;     // inner PCG stopped (converged or capped)
+    int32_t outer_done;     // FP64 residual reached tolerance
+    int32_t inner_iters;    // iterations of the current sweep
+    int32_t total_iters;    // iterations of the current solve
+    int32_t sweeps;
+    int32_t numeric_bad;    // NaN/Inf seen in a reduction
+    int32_t ls_more;        // level-set BFS reached the end of a launch's band
+    int32_t pad;
+};
+
+// Per-scene device descriptor.  One array of these lives in device memory;
+// kernels index it with blockIdx.y (the scene's slot in the launch).
+struct SceneDev {
+    int32_t I, J;           // cells
+    int32_t ppc;
+    int32_t n_friction;
+    float pic, pscale;
+    float g_i, g_j;         // gravity (m/s^2)
+    double dx, rho;
+    double tol;             // FP64 relative residual target
+    int32_t max_inner;      // inner iteration cap per sweep
+    int32_t active;         // 0: skipped by every kernel of a batched launch
+    int32_t max_blocks;     // stride of the partials array (>= cell blocks)
+    int32_t pad0;
+    // per-substep
+    float dt;
+    float force;            // (float)(dt / dx), applyBodyForces
+    double sA;              // dt / (rho dx^2): matrix scale
+    double sApply;          // dt / (rho dx): applyPressure scale
+    double sRhs;            // 1.f / dx: calcPressureRhs scale
+    int64_t cap;            // particle capacity
+    // particles: canonical (sorted) and advected (unsorted) buffers
+    float2 *pos, *vel;
+    uint32_t* pid;
+    float2 *posA, *velA;
+    uint32_t* key;
+    uint32_t *tmpPid, *tmpSrc;
+    // cells
+    int32_t *cellCount, *cellStart;  // C, C+1
+    int32_t* scanSums;      // per scan-tile totals
+    int8_t* mat;
+    uint8_t* stencil;       // matrix snapshot: bits0-3 fluid-nbr mask, 4-6 n, 7 fluid row
+    float *ssdf, *fsdf;
+    int32_t* sid;
+    int32_t* pcount;
+    int32_t* marker;        // level-set BFS levels
+    int32_t* lsFlags;       // [kMaxBands] band reached its last level
+    // faces: working (P2G/project), saved (extrapolated, pre-force), final
+    float *u, *v, *su, *sv, *uf, *vf;
+    uint8_t *uval, *vval, *uvalS, *vvalS, *uvalF, *vvalF;
+    // solver
+    double *b, *x64, *x64b;
+    float *r, *r2, *p, *p2, *z, *q, *d;
+    double* partials;       // per-block partial sums [kSlots * max_blocks]
+    int32_t* counters;      // last-block-done tickets [kSlots]
+    float* friction;
+    SceneScalars* sc;
+};
+
+// ---------------------------------------------------------------------------
+// Arithmetic helpers (FlipSolver2dLib/mathfuncs.cpp, grid2d.h)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float fclamp(float v, float lo, float hi)
+{
+    return (v < lo) ? lo : ((hi < v) ? hi : v);   // std::clamp
+}
+__device__ __forceinline__ int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// mathfuncs.cpp:3-6 frac(v) = v - (long)v  (truncf is exact)
+__device__ __forceinline__ float frac_(float v) { return v - truncf(v); }
+// mathfuncs.cpp:8-11
+__device__ __forceinline__ int integr(float v) { return (int)floorf(v); }
+// mathfuncs.cpp:13-16
+__device__ __forceinline__ float lerp1(float a, float b, float f) { return (a * (1.0f - f)) + (b * f); }
+
+// Grid2d::getAt with OOB_EXTEND (grid2d.h:122-143)
+__device__ __forceinline__ float get_ext(const float* g, int si, int sj, int i, int j)
+{
+    return g[(size_t)iclamp(i, 0, si - 1) * sj + iclamp(j, 0, sj - 1)];
+}
+__device__ __forceinline__ int8_t mat_ext(const int8_t* m, int I, int J, int i, int j)
+{
+    return m[(size_t)iclamp(i, 0, I - 1) * J + iclamp(j, 0, J - 1)];
+}
+
+// Grid2d::lerp, grid2d.h:186-216
+__device__ __forceinline__ float lerp_grid(const float* g, int si, int sj, float oi, float oj, float i, float j)
+{
+    i += oi;
+    j += oj;
+    i = fclamp(i, 0.f, (float)(si - 1));
+    j = fclamp(j, 0.f, (float)(sj - 1));
+    const int ci = integr(i), cj = integr(j);
+    const float fi = frac_(i), fj = frac_(j);
+    const int nj = fj >= 0.5f ? cj + 1 : cj - 1;
+    const int ni = fi >= 0.5f ? ci + 1 : ci - 1;
+    const float iF = fi < 0.5f ? 0.5f - fi : fi - 0.5f;
+    const float jF = fj < 0.5f ? 0.5f - fj : fj - 0.5f;
+    const float v1 = lerp1(get_ext(g, si, sj, ci, cj), get_ext(g, si, sj, ni, cj), iF);
+    const float v2 = lerp1(get_ext(g, si, sj, ci, nj), get_ext(g, si, sj, ni, nj), iF);
+    return lerp1(v1, v2, jF);
+}
+
+// StaggeredVelocityGrid::velocityAt (staggeredvelocitygrid.cpp:158-167)
+__device__ __forceinline__ float2 velocity_at(const float* u, const float* v, int I, int J, float x, float y)
+{
+    return make_float2(lerp_grid(u, I + 1, J, 0.5f, 0.f, x, y), lerp_grid(v, I, J + 1, 0.f, 0.5f, x, y));
+}
+
+// simmath::bSpline / quadraticBSpline, mathfuncs.cpp:32-43
+__device__ __forceinline__ float bspline(float value)
+{
+    value = fabsf(value);
+    const float t1 = (0.75f - value * value) * (float)(value < 0.5f);
+    const float t2 = ((0.5f * (1.5f - value)) * (1.5f - value)) * (float)(value >= 0.5f && value < 1.5f);
+    return t1 + t2;
+}
+__device__ __forceinline__ float qbspline(float x, float y) { return (bspline(x) * bspline(y)) * bspline(0.f); }
+
+// simmath::linearHat / bilinearHat, mathfuncs.cpp:61-77
+__device__ __forceinline__ float linear_hat(float v)
+{
+    if (v >= 0 && v <= 1) return 1 - v;
+    if (v < 0 && v >= -1) return 1 + v;
+    return 0;
+}
+__device__ __forceinline__ float bilinear_hat(float x, float y)
+{
+    return (linear_hat(x) * linear_hat(y)) * linear_hat(0.f);
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+// Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous
+// run of tiles so neighbouring tiles share that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    const int full = nb & ~7;
+    if (b >= full) return b;
+    const int per = full >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+// wave64 + block reduction of doubles; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* lds)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) t += lds[k];
+    return t;
+}
+
+__device__ __forceinline__ float block_maxf(float v, float* lds)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_down(v, o, 64));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    float t = 0.f;
+    if (threadIdx.x == 0) {
+        t = lds[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) t = fmaxf(t, lds[k]);
+    }
+    return t;
+}
+
+__device__ __forceinline__ int block_sumi(int v, int* lds)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    int t = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) t += lds[k];
+    return t;
+}
+
+__device__ __forceinline__ bool finite_d(double x) { return x == x && x < 1e300 && x > -1e300; }
+
+// Finish a block reduction of NV doubles into reduction slot `slot`; the last
+// block of the scene to arrive sums the per-block partials in block order
+// (deterministic) and calls fin(totals) in thread 0.
+template <int NV, typename F>
+__device__ __forceinline__ void reduce_last(const SceneDev& s, double (&v)[NV], int slot, int nb, F fin)
+{
+    __shared__ double red[kBlock / 64];
+    __shared__ int last;
+    double tot[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) tot[q] = block_sum(v[q], red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) s.partials[(size_t)(slot + q) * s.max_blocks + blockIdx.x] = tot[q];
+        __threadfence();
+        last = atomicAdd(&s.counters[slot], 1) == nb - 1;
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        double t[NV];
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            t[q] = 0.0;
+            for (int b = 0; b < nb; b++) t[q] += s.partials[(size_t)(slot + q) * s.max_blocks + b];
+        }
+        s.counters[slot] = 0;
+        fin(t);
+    }
+}
+
+}  // namespace fg

@@ -1,0 +1,440 @@
+// fg_grid.hip -- cell- and face-space kernels of the substep.
+//
+// Layout per scene: cells row-major (i*J + j); U faces (I+1) x J, V faces
+// I x (J+1) (FlipSolver2dLib/staggeredvelocitygrid.cpp:6-14).  One thread
+// per cell; blocks of a launch are remapped so each XCD gets a contiguous
+// band of rows.
+#include "fg_kernels.h"
+
+namespace fg {
+
+__device__ __forceinline__ int cell_index(int nbx)
+{
+    return xcd_tile(blockIdx.x, nbx) * kBlock + threadIdx.x;
+}
+
+// ---------------------------------------------------------------------------
+// Matrix snapshot: FlipSolver::getPressureProjectionMatrix
+// (flipsolver2d.cpp:797-887), stored matrix-free as one byte per cell:
+// bits 0-3 fluid-neighbour mask (i-, i+, j-, j+; set only when the *linear*
+// offset index is inside [0, I*J)), bits 4-6 nonsolid count n, bit 7 fluid.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_stencil(const SceneDev* scenes)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active) return;
+    const int I = s.I, J = s.J, C = I * J;
+    const int c = cell_index(gridDim.x);
+    if (c >= C) return;
+    const int i = c / J, j = c - (c / J) * J;
+    const int8_t m = s.mat[c];
+    uint8_t code = 0;
+    if (is_fluid(m)) {
+        const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+        const int off[4] = {-J, J, -1, 1};
+        int n = 0, mask = 0;
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const int8_t mn = mat_ext(s.mat, I, J, i + di[d], j + dj[d]);
+            if (is_fluid(mn)) {
+                n++;
+                const int ln = c + off[d];
+                if (ln >= 0 && ln < C) mask |= 1 << d;
+            } else if (is_empty(mn)) {
+                n++;
+            }
+        }
+        code = (uint8_t)(0x80 | (n << 4) | mask);
+    }
+    s.stencil[c] = code;
+}
+
+// ---------------------------------------------------------------------------
+// P2G: FlipSolver::particleVelocityToGridThread (flipsolver2d.cpp:1329-1377).
+// A particle reaches face U(i,j)/V(i,j) only if both quadratic B-spline
+// weights exceed 1e-9, which confines it to cells (i-1..i+1, j-1..j+1): three
+// contiguous (cell, id)-ordered particle ranges.  Deterministic gather, no
+// atomics.  U row I and V column J are never written by the reference's
+// cell loop and stay 0 / invalid.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active) return;
+    const int I = s.I, J = s.J, C = I * J;
+    const int c = cell_index(gridDim.x);
+    if (c >= C) return;
+    const int i = c / J, j = c - (c / J) * J;
+    const float fi = (float)i, fj = (float)j;
+    float uw = 1e-10f, vw = 1e-10f, uv = 0.0f, vv = 0.0f;
+    bool uval = false, vval = false;
+    const int j0 = j > 0 ? j - 1 : 0, j1 = j < J - 1 ? j + 1 : J - 1;
+    for (int di = -1; di <= 1; di++) {
+        const int ii = i + di;
+        if (ii < 0 || ii >= I) continue;
+        const int a = s.cellStart[ii * J + j0], e = s.cellStart[ii * J + j1 + 1];
+        for (int k = a; k < e; k++) {
+            const float2 p = s.pos[k];
+            const float wu = qbspline(p.x - fi, p.y - (fj + 0.5f));
+            const float wv = qbspline(p.x - (fi + 0.5f), p.y - fj);
+            if (wu > 1e-9f && wv > 1e-9f) {
+                const float2 vel = s.vel[k];
+                uw += wu;
+                uv += wu * vel.x;
+                uval = true;
+                vw += wv;
+                vv += wv * vel.y;
+                vval = true;
+            }
+        }
+    }
+    s.u[(size_t)i * J + j] = uv / uw;
+    s.uval[(size_t)i * J + j] = uval;
+    s.v[(size_t)i * (J + 1) + j] = vv / vw;
+    s.vval[(size_t)i * (J + 1) + j] = vval;
+    if (i == I - 1) {
+        s.u[(size_t)I * J + j] = 0.f;
+        s.uval[(size_t)I * J + j] = 0;
+    }
+    if (j == J - 1) {
+        s.v[(size_t)i * (J + 1) + J] = 0.f;
+        s.vval[(size_t)i * (J + 1) + J] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fluid SDF: FlipSolver::updateSdfThread (flipsolver2d.cpp:1276-1311) over
+// the reference's window -- the 3x3 bins of 3x3 cells around the cell's bin
+// (markerparticlesystem.cpp:9-20, 109-128) -- then updateMaterials
+// (1053-1075) and the per-cell particle count of countParticles.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_sdf_materials(const SceneDev* scenes)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active) return;
+    const int I = s.I, J = s.J, C = I * J;
+    const int c = cell_index(gridDim.x);
+    if (c >= C) return;
+    const int i = c / J, j = c - (c / J) * J;
+    const int bi = i / 3, bj = j / 3;
+    const int r0 = max(0, 3 * bi - 3), r1 = min(I, 3 * bi + 6);
+    const int c0 = max(0, 3 * bj - 3), c1 = min(J, 3 * bj + 6);
+    const float cx = (float)i + 0.5f, cy = (float)j + 0.5f;
+    float d2 = 3.402823466e+38f;
+    for (int r = r0; r < r1; r++) {
+        const int a = s.cellStart[r * J + c0], e = s.cellStart[r * J + c1];
+        for (int k = a; k < e; k++) {
+            const float2 p = s.pos[k];
+            const float dxp = p.x - cx, dyp = p.y - cy;
+            const float nd = dxp * dxp + dyp * dyp;
+            if (nd < d2) d2 = nd;
+        }
+    }
+    const float sdf = sqrtf(d2) - s.pscale;
+    s.fsdf[c] = sdf;
+    int8_t m = s.mat[c];
+    if (sdf < 0.f) {
+        if (is_empty(m)) m = M_FLUID;
+    } else {
+        if (is_strict_fluid(m)) m = M_EMPTY;
+    }
+    s.mat[c] = m;
+    const int cnt = s.cellStart[c + 1] - s.cellStart[c];
+    s.pcount[c] = min(cnt, 2 * s.ppc);
+}
+
+// ---------------------------------------------------------------------------
+// Breadth-first extrapolation as a level-synchronous LDS computation.
+//
+// The reference's FIFO BFS (simmath::breadthFirstExtrapolate,
+// mathfuncs.cpp:152-215; FlipSolver::extrapolateLevelsetInside,
+// flipsolver2d.cpp:1433-1494) gives every reached cell the level L = its
+// 8-connected distance to the known set, and sets its value from the
+// neighbours of level < L only, all of which are final before any level-L
+// cell is processed.  So levels can be computed band by band: a tile of
+// kTile x kTile cells with an H-cell halo holds every cell a level <= L0+H
+// tile value depends on.  Sums are FP64 in getNeighborhood order
+// (linearindexable2d.h:55-71) exactly as the reference accumulates them.
+// ---------------------------------------------------------------------------
+constexpr int kTile = 32;
+constexpr int16_t kNone = 0x7fff;     // outside the grid
+constexpr int16_t kUnreached = 0x7ffe;
+
+template <int H, bool LEVELSET>
+__device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mkin, float* gout, uint8_t* vout,
+                         int32_t* mkout, int si, int sj, int L0, int* more)
+{
+    constexpr int R = kTile + 2 * H;
+    __shared__ int16_t mk[R * R];
+    __shared__ float val[R * R];
+    const int tilesJ = (sj + kTile - 1) / kTile;
+    const int tile = blockIdx.x;
+    const int ti = tile / tilesJ, tj = tile - (tile / tilesJ) * tilesJ;
+    const int gi0 = ti * kTile - H, gj0 = tj * kTile - H;
+    for (int idx = threadIdx.x; idx < R * R; idx += kBlock) {
+        const int ri = idx / R, rj = idx - (idx / R) * R;
+        const int gi = gi0 + ri, gj = gj0 + rj;
+        int16_t m = kNone;
+        float x = 0.f;
+        if (gi >= 0 && gi < si && gj >= 0 && gj < sj) {
+            const size_t g = (size_t)gi * sj + gj;
+            x = gin[g];
+            if (LEVELSET) {
+                const int lv = mkin[g];
+                m = lv <= L0 ? (int16_t)lv : kUnreached;
+            } else {
+                m = vin[g] ? 0 : kUnreached;
+            }
+        }
+        mk[idx] = m;
+        val[idx] = x;
+    }
+    __syncthreads();
+    // expand levels L0+1 .. L0+H from the level-L0 frontier
+    for (int st = 1; st <= H; st++) {
+        const int16_t prev = (int16_t)(L0 + st - 1), cur = (int16_t)(L0 + st);
+        for (int idx = threadIdx.x; idx < R * R; idx += kBlock) {
+            if (mk[idx] != kUnreached) continue;
+            const int ri = idx / R, rj = idx - (idx / R) * R;
+            bool hit = false;
+            for (int di = -1; di <= 1 && !hit; di++)
+                for (int dj = -1; dj <= 1; dj++) {
+                    const int qi = ri + di, qj = rj + dj;
+                    if ((di == 0 && dj == 0) || qi < 0 || qi >= R || qj < 0 || qj >= R) continue;
+                    if (mk[qi * R + qj] == prev) {
+                        hit = true;
+                        break;
+                    }
+                }
+            if (hit) mk[idx] = cur;
+        }
+        __syncthreads();
+    }
+    // values, level by level
+    for (int st = 1; st <= H; st++) {
+        const int16_t L = (int16_t)(L0 + st);
+        for (int idx = threadIdx.x; idx < R * R; idx += kBlock) {
+            if (mk[idx] != L) continue;
+            const int ri = idx / R, rj = idx - (idx / R) * R;
+            double avg = 0;
+            int count = 0;
+            for (int di = -1; di <= 1; di++)
+                for (int dj = -1; dj <= 1; dj++) {
+                    const int qi = ri + di, qj = rj + dj;
+                    if ((di == 0 && dj == 0) || qi < 0 || qi >= R || qj < 0 || qj >= R) continue;
+                    const int16_t q = mk[qi * R + qj];
+                    if (q < L) {
+                        avg += (double)val[qi * R + qj];
+                        count++;
+                    }
+                }
+            if (LEVELSET) val[idx] = (float)(avg / count - 1.f);
+            else val[idx] = (float)(avg / count);
+        }
+        __syncthreads();
+    }
+    // write the tile
+    int reached_end = 0;
+    for (int idx = threadIdx.x; idx < kTile * kTile; idx += kBlock) {
+        const int li = idx / kTile, lj = idx - (idx / kTile) * kTile;
+        const int gi = ti * kTile + li, gj = tj * kTile + lj;
+        if (gi >= si || gj >= sj) continue;
+        const int r = (li + H) * R + (lj + H);
+        const size_t g = (size_t)gi * sj + gj;
+        const int16_t m = mk[r];
+        if (LEVELSET) {
+            if (m > L0 && m <= L0 + H) {
+                gout[g] = val[r];
+                mkout[g] = m;
+                if (m == L0 + H) reached_end = 1;
+            }
+        } else {
+            gout[g] = val[r];
+            vout[g] = (m != kUnreached) ? 1 : 0;
+        }
+    }
+    if (LEVELSET && reached_end) *more = 1;
+}
+
+// Velocity extrapolation, radius 10 (StaggeredVelocityGrid::extrapolate,
+// staggeredvelocitygrid.cpp:146-156): levels 1..11 in one band.
+// blockIdx.z: 0 = U grid, 1 = V grid.  which: 0 = (u -> su), 1 = (u -> uf).
+__global__ void __launch_bounds__(kBlock) k_extrapolate(const SceneDev* scenes, int which)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active) return;
+    const bool isU = blockIdx.z == 0;
+    const int si = isU ? s.I + 1 : s.I, sj = isU ? s.J : s.J + 1;
+    const int tiles = ((si + kTile - 1) / kTile) * ((sj + kTile - 1) / kTile);
+    if ((int)blockIdx.x >= tiles) return;
+    const float* gin = isU ? s.u : s.v;
+    const uint8_t* vin = isU ? (which == 0 ? s.uval : s.uvalS) : (which == 0 ? s.vval : s.vvalS);
+    float* gout = isU ? (which == 0 ? s.su : s.uf) : (which == 0 ? s.sv : s.vf);
+    uint8_t* vout = isU ? (which == 0 ? s.uvalS : s.uvalF) : (which == 0 ? s.vvalS : s.vvalF);
+    bfs_band<11, false>(gin, vin, nullptr, gout, vout, nullptr, si, sj, 0, nullptr);
+}
+
+__global__ void __launch_bounds__(kBlock) k_levelset_init(const SceneDev* scenes)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active) return;
+    const int C = s.I * s.J;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c < C) s.marker[c] = s.fsdf[c] > 0.f ? 0 : kMaxLevel;
+    if (c < kMaxBands) s.lsFlags[c] = 0;
+}
+
+// One band of extrapolateLevelsetInside: levels L0+1 .. L0+16.  Runs only if
+// the previous band reached its last level somewhere (sc->ls_more).
+__global__ void __launch_bounds__(kBlock) k_levelset_band(const SceneDev* scenes, int band)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active) return;
+    if (band > 0 && s.lsFlags[band - 1] == 0) return;
+    const int tiles = ((s.I + kTile - 1) / kTile) * ((s.J + kTile - 1) / kTile);
+    if ((int)blockIdx.x >= tiles) return;
+    bfs_band<16, true>(s.fsdf, nullptr, s.marker, s.fsdf, nullptr, s.marker, s.I, s.J, band * 16,
+                       &s.lsFlags[band]);
+}
+
+// ---------------------------------------------------------------------------
+// Saved grid + body forces: m_savedFluidVelocityGrid = m_fluidVelocityGrid;
+// applyBodyForces (flipsolver2d.cpp:439-440, 1222-1241).  The extrapolated
+// grid (su/sv) is the saved copy; the working grid gets the force.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_forces(const SceneDev* scenes)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active) return;
+    const int nu = (s.I + 1) * s.J, nv = s.I * (s.J + 1);
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    const float du = s.force * s.g_i, dv = s.force * s.g_j;
+    if (k < nu) s.u[k] = s.su[k] + du;
+    if (k < nv) s.v[k] = s.sv[k] + dv;
+}
+
+// ---------------------------------------------------------------------------
+// Pressure right-hand side: FlipSolver::calcPressureRhs + divergenceAt
+// (flipsolver2d.cpp:759-764, 962-991).  Divergence is formed in float and
+// widened, as there; the divergence-control term is zero because
+// centeredParamsToGrid clears it before this read (flipsolver2d.cpp:1382).
+// Also starts the solve: x = 0, ||b||^2.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_rhs(const SceneDev* scenes)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active) return;
+    const int I = s.I, J = s.J, C = I * J;
+    const int nb = (C + kBlock - 1) / kBlock;
+    if ((int)blockIdx.x >= nb) return;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double bb = 0.0;
+    if (c < C) {
+        const int i = c / J, j = c - (c / J) * J;
+        double val = 0.0;
+        const int8_t m = s.mat[c];
+        if (is_fluid(m)) {
+            const double scale = s.sRhs;
+            const float u0 = s.u[(size_t)i * J + j], u1 = s.u[(size_t)(i + 1) * J + j];
+            const float v0 = s.v[(size_t)i * (J + 1) + j], v1 = s.v[(size_t)i * (J + 1) + j + 1];
+            const float div = (((u1 - u0) + v1) - v0) + 0.f;
+            val = -scale * (double)div;
+            val -= (double)is_solid(mat_ext(s.mat, I, J, i - 1, j)) * scale * (double)(u0 - 0);
+            val += (double)is_solid(mat_ext(s.mat, I, J, i + 1, j)) * scale * (double)(u1 - 0);
+            val -= (double)is_solid(mat_ext(s.mat, I, J, i, j - 1)) * scale * (double)(v0 - 0);
+            val += (double)is_solid(mat_ext(s.mat, I, J, i, j + 1)) * scale * (double)(v1 - 0);
+        }
+        s.b[c] = val;
+        s.x64[c] = 0.0;
+        s.d[c] = 0.f;
+        bb = val * val;
+    }
+    double v[1] = {bb};
+    reduce_last<1>(s, v, 0, nb, [&](const double* t) {
+        s.sc->bb = t[0];
+        s.sc->rr64 = t[0];
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Pressure application + solid friction: applyPressureThreadU/V
+// (flipsolver2d.cpp:1133-1193, FP64 update of the float face values) then
+// updateVelocityFromSolids (1077-1104, materialgrid.cpp:152-166).  Both
+// touch faces U(i,j), V(i,j) of cell (i,j) only, so they fuse per cell.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_apply_solids(const SceneDev* scenes)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active) return;
+    const int I = s.I, J = s.J, C = I * J;
+    const int c = cell_index(gridDim.x);
+    if (c >= C) return;
+    const int i = c / J, j = c - (c / J) * J;
+    const double scale = s.sApply;
+    const size_t fu = (size_t)i * J + j, fv = (size_t)i * (J + 1) + j;
+    const double pc = s.x64[c];
+    const int8_t mb = s.mat[c];
+    float u = s.u[fu], v = s.v[fv];
+    {
+        const double pm = i - 1 < 0 ? 0.0 : s.x64[c - J];
+        const int8_t ma = mat_ext(s.mat, I, J, i - 1, j);
+        if (is_fluid(ma) || is_fluid(mb)) {
+            if (is_solid(ma) || is_solid(mb)) u = 0.f;
+            else u = (float)((double)u - scale * (pc - pm));
+        } else {
+            s.uvalS[fu] = 0;
+        }
+    }
+    {
+        const double pm = j - 1 < 0 ? 0.0 : s.x64[c - 1];
+        const int8_t ma = mat_ext(s.mat, I, J, i, j - 1);
+        if (is_fluid(ma) || is_fluid(mb)) {
+            if (is_solid(ma) || is_solid(mb)) v = 0.f;
+            else v = (float)((double)v - scale * (pc - pm));
+        } else {
+            s.vvalS[fv] = 0;
+        }
+    }
+    // friction of the solids in the 3x3 neighbourhood
+    int n = 0;
+    float avg = 0;
+    for (int di = -1; di < 2; di++)
+        for (int dj = -1; dj < 2; dj++) {
+            const int ii = i + di, jj = j + dj;
+            if (ii >= 0 && ii < I && jj >= 0 && jj < J) {
+                const int id = s.sid[(size_t)ii * J + jj];
+                if (id != -1) {
+                    avg += (id < s.n_friction) ? s.friction[id] : 0.f;
+                    n++;
+                }
+            }
+        }
+    if (n > 0) {
+        avg /= (float)n;
+        auto sol = [&](int a, int b) { return is_solid(mat_ext(s.mat, I, J, a, b)); };
+        if (sol(i, j + 1) || sol(i - 1, j + 1) || sol(i, j) || sol(i - 1, j) || sol(i, j - 1) || sol(i - 1, j - 1))
+            u = u * (1 - avg);
+        if (sol(i + 1, j) || sol(i + 1, j - 1) || sol(i, j) || sol(i, j - 1) || sol(i - 1, j) || sol(i - 1, j - 1))
+            v = v * (1 - avg);
+    }
+    s.u[fu] = u;
+    s.v[fv] = v;
+}
+
+}  // namespace fg
+
+namespace fg {
+#define FG_LAUNCH0(name) \
+    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc) { hipLaunchKernelGGL(k_##name, g, dim3(kBlock), 0, st, sc); }
+#define FG_LAUNCH1(name, T, a) \
+    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc, T a) { hipLaunchKernelGGL(k_##name, g, dim3(kBlock), 0, st, sc, a); }
+FG_LAUNCH0(stencil)
+FG_LAUNCH0(p2g)
+FG_LAUNCH0(sdf_materials)
+FG_LAUNCH1(extrapolate, int, which)
+FG_LAUNCH0(levelset_init)
+FG_LAUNCH1(levelset_band, int, band)
+FG_LAUNCH0(forces)
+FG_LAUNCH0(rhs)
+FG_LAUNCH0(apply_solids)
+}  // namespace fg

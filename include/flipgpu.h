@@ -1,0 +1,194 @@
+/*
+ * flipgpu.h -- C-ABI of the MI355X-native FLIP substep (libflipgpu.so).
+ *
+ * Plain C types only: no C++ or torch types cross this boundary.  Every
+ * function returns FG_OK (0) or a negative FG_E* code; fg_last_error()
+ * returns the calling thread's last message.  No C++ exception crosses.
+ * Host buffers passed in are caller-owned and only read/written during the
+ * call; device state is owned by the context.  A context is driven from one
+ * host thread; all device work is ordered on the scene's stream and every
+ * fg_*_get_* call synchronises that stream.
+ *
+ * Reference interfaces replaced (paths relative to the reference repo):
+ *   FlipSolver::FlipSolver(const FlipSolverParameters*)  FlipSolver2dLib/flipsolver2d.cpp:27-86
+ *       -> fg_scene_create
+ *   FlipSolver::firstFrameInit / updateSolids / updateInitialFluid / seedInitialFluid
+ *       FlipSolver2dLib/flipsolver2d.cpp:502-590, 682-707, 788-795  -> fg_scene_init_geometry
+ *   FlipSolver::stepFrame()          FlipSolver2dLib/flipsolver2d.cpp:464-500 -> fg_scene_step_frame
+ *   FlipSolver::step() (protected)   FlipSolver2dLib/flipsolver2d.cpp:412-462 -> fg_scene_substep
+ *   FlipSolver::advect/particleToGrid/updateSdf/project/particleUpdate/...
+ *       (protected virtual hooks, FlipSolver2dLib/flipsolver2d.h:280-367) -> fg_scene_stage
+ *   FlipSolver accessors markerParticles/materialGrid/fluidVelocityGrid/fluidSdf/solidSdf
+ *       FlipSolver2dLib/flipsolver2d.h:226-236 -> fg_scene_get_state
+ *   FlipSolver::timeStats()          FlipSolver2dLib/flipsolver2d.h:88-181,238 -> fg_stats
+ *   LinearSolver::solve(A, M, x, b, iterLimit, tol)  FlipSolver2dLib/linearsolver.h:15-20
+ *       -> fg_poisson_solve
+ */
+#ifndef FLIPGPU_H
+#define FLIPGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FG_ABI_VERSION 1
+
+enum fg_status {
+    FG_OK = 0,
+    FG_E_INVALID = -1,     /* bad argument */
+    FG_E_DEVICE = -2,      /* HIP runtime error */
+    FG_E_BUDGET = -3,      /* HBM budget would be exceeded */
+    FG_E_CAPACITY = -4,    /* particle capacity exceeded */
+    FG_E_STATE = -5,       /* call not valid in the current state */
+    FG_E_NUMERIC = -6      /* NaN/Inf detected; scene quarantined */
+};
+
+/* Material codes, FlipSolver2dLib/materialgrid.h:6-13 */
+enum fg_material {
+    FG_FLUID = 0x40, FG_SOURCE = 0x41, FG_SOLID = 0x20, FG_SINK = 0x12, FG_EMPTY = 0x10
+};
+
+/* Density correction (FlipSolver::densityCorrection, flipsolver2d.cpp:164-186).
+ * The reference applies it only when its PCG converges within pcgIterLimit,
+ * which its preconditioner never achieves, so FG_DENSITY_OFF reproduces the
+ * reference's results.  GATED applies it when our solve reaches its
+ * tolerance; ON always applies it. */
+enum fg_density_mode { FG_DENSITY_OFF = 0, FG_DENSITY_GATED = 1, FG_DENSITY_ON = 2 };
+
+/* Stages, in FlipSolver::step order (flipsolver2d.cpp:412-462). */
+enum fg_stage {
+    FG_STAGE_ADVECT = 0,          /* advect(): RK4 + solid projection + kill   */
+    FG_STAGE_REBIN = 1,           /* matrix snapshot, prune + sort on cell key */
+    FG_STAGE_DENSITY = 2,         /* densityCorrection()                       */
+    FG_STAGE_P2G = 3,             /* particleVelocityToGrid()                  */
+    FG_STAGE_SDF = 4,             /* updateSdf() + updateMaterials()           */
+    FG_STAGE_LEVELSET = 5,        /* extrapolateLevelsetInside()               */
+    FG_STAGE_EXTRAPOLATE = 6,     /* m_fluidVelocityGrid.extrapolate(10)       */
+    FG_STAGE_FORCES = 7,          /* saved grid copy + applyBodyForces()       */
+    FG_STAGE_PROJECT = 8,         /* calcPressureRhs + solve + apply pressure  */
+    FG_STAGE_SOLIDS = 9,          /* updateVelocityFromSolids()                */
+    FG_STAGE_G2P = 10,            /* particleUpdate()                          */
+    FG_STAGE_COUNT = 11,          /* countParticles() (+ reseed)               */
+    FG_STAGE_NUM = 12
+};
+
+typedef struct fg_scene_params {
+    int32_t size_i, size_j;       /* gridSizeI/J */
+    double  dx;                   /* cell size (m) */
+    double  fluid_density;
+    float   gravity_i, gravity_j; /* globalAcceleration (m/s^2) */
+    float   pic_ratio;
+    float   cfl_number;
+    float   particle_scale;
+    int32_t fps;
+    int32_t max_substeps;
+    int32_t particles_per_cell;
+    int32_t pcg_iter_limit;       /* cap on inner PCG iterations per refinement sweep */
+    int32_t density_mode;         /* fg_density_mode */
+    double  solver_tolerance;     /* FP64 relative residual ||b-Ax||2/||b||2 target */
+    int64_t max_particles;        /* particle capacity */
+} fg_scene_params;
+
+/* Host-side view of a scene's state.  Arrays are row-major, index i*J+j for
+ * cells; U is (I+1) x J, V is I x (J+1) (staggeredvelocitygrid.cpp:6-14).
+ * Particle positions/velocities are in cell units (cells, cells/s).
+ * Any pointer may be NULL on get (skipped).  On set, particles, u/v (+valid),
+ * material, solid_sdf and solid_id are required. */
+typedef struct fg_scene_state {
+    int64_t   n_particles;
+    float    *px, *py, *vx, *vy;  /* [n_particles] */
+    uint32_t *pid;                /* [n_particles] stable particle id */
+    float    *u, *v;
+    uint8_t  *u_valid, *v_valid;
+    int8_t   *material;           /* I*J */
+    float    *solid_sdf;          /* I*J */
+    float    *fluid_sdf;          /* I*J */
+    int32_t  *solid_id;           /* I*J, -1 = none */
+    int32_t  *particle_counts;    /* I*J, get only */
+    float    *saved_u, *saved_v;  /* get only: grid after extrapolation, before forces */
+    double   *pressure;           /* I*J, get only: last pressure solve */
+    double   *rhs;                /* I*J, get only: last pressure right-hand side */
+} fg_scene_state;
+
+typedef struct fg_stats {
+    int32_t substeps;
+    int32_t pressure_iters;       /* max over substeps of total inner PCG iterations */
+    int32_t pressure_sweeps;      /* max over substeps of FP64 refinement sweeps */
+    int32_t density_iters;
+    int32_t density_applied;
+    int32_t converged;            /* 1 if every solve reached solver_tolerance */
+    double  pressure_rel_resid;   /* last solve's FP64 ||b-Ax||2/||b||2 */
+    int64_t particles;            /* live particles after the call */
+    int64_t particle_substeps;    /* sum over substeps of live particles at substep start */
+    int64_t particles_killed;     /* advect: out of bounds / sink */
+    int64_t particles_erased;     /* countParticles: beyond 2*ppc */
+    float   last_dt;
+    float   max_velocity;
+    double  stage_ms[FG_STAGE_NUM]; /* filled when timing is enabled */
+    double  frame_ms;
+} fg_stats;
+
+typedef struct fg_ctx fg_ctx;
+typedef struct fg_scene fg_scene;
+
+const char* fg_last_error(void);
+int32_t fg_abi_version(void);
+
+int fg_ctx_create(int32_t device, uint64_t hbm_budget_bytes, fg_ctx** out);
+int fg_ctx_destroy(fg_ctx* ctx);
+int fg_ctx_bytes_in_use(fg_ctx* ctx, uint64_t* out);
+/* Per-stage HIP-event timing for later calls (adds synchronisation). */
+int fg_ctx_set_timing(fg_ctx* ctx, int32_t enabled);
+
+/* Bytes of device memory a scene with these parameters needs. */
+int fg_scene_bytes(const fg_scene_params* p, uint64_t* out);
+int fg_scene_create(fg_ctx* ctx, const fg_scene_params* p, fg_scene** out);
+int fg_scene_destroy(fg_scene* s);
+
+/* First-frame init from polygons (vertices in metres, x along I): solid SDF,
+ * solid ids and materials, then quadrant-jittered seeding of strict-fluid
+ * cells from a counter-based RNG.  friction[k] belongs to solid k. */
+int fg_scene_init_geometry(fg_scene* s,
+                           int32_t n_solids, const int32_t* solid_nverts, const float* solid_verts,
+                           const float* friction,
+                           int32_t n_fluids, const int32_t* fluid_nverts, const float* fluid_verts,
+                           uint32_t seed);
+/* Obstacle friction table (updateVelocityFromSolids, flipsolver2d.cpp:1077-1104). */
+int fg_scene_set_friction(fg_scene* s, int32_t n, const float* friction);
+int fg_scene_set_state(fg_scene* s, const fg_scene_state* st);
+int fg_scene_get_state(fg_scene* s, fg_scene_state* st);
+int fg_scene_particle_count(fg_scene* s, int64_t* out);
+
+/* One FlipSolver::step() with the given dt. */
+int fg_scene_substep(fg_scene* s, float dt, fg_stats* stats);
+/* One FlipSolver::stepFrame(): CFL-split substeps until the frame is done. */
+int fg_scene_step_frame(fg_scene* s, fg_stats* stats);
+/* Run a single stage (fg_stage) of step() with dt; stages must be called in
+ * order for a full step.  For parity tests of the reference's hooks. */
+int fg_scene_stage(fg_scene* s, int32_t stage, float dt);
+
+/* Step every scene one frame; all scenes advance together, one launch per
+ * phase.  stats may be NULL or an array of n. */
+int fg_batch_step_frame(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, fg_stats* stats);
+
+/* Isolated pressure projection (LinearSolver::solve replacement).  The matrix
+ * is the reference's 5-point operator assembled from `material`
+ * (flipsolver2d.cpp:797-887) with scale = dt/(rho*dx^2); solved to
+ * rel_tol (FP64 relative residual) with FP32 IPP-PCG + FP64 refinement. */
+typedef struct fg_solve_stats {
+    int32_t iterations;           /* total inner PCG iterations */
+    int32_t sweeps;               /* FP64 refinement sweeps */
+    int32_t converged;
+    double  rel_residual;
+    double  solve_ms;
+} fg_solve_stats;
+int fg_poisson_solve(fg_ctx* ctx, int32_t size_i, int32_t size_j, const int8_t* material, double scale,
+                     const double* b, double* x, double rel_tol, int32_t max_iters, fg_solve_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,172 @@
+/*
+ * flip_oracle.h -- CPU restatement of FlipSolver2d's per-substep FLIP step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may
+ * load it.  The product library (flipsolver2d_amd) never links or calls it.
+ *
+ * Every function restates one reference function; the citation names the
+ * reference file and lines it follows (paths relative to the reference repo
+ * root, FlipSolver2dLib/ unless noted).  Floating-point expressions keep the
+ * reference's operation order and are compiled with -ffp-contract=off so the
+ * HIP kernels (built the same way) can be compared bit for bit.
+ *
+ * Deliberate, documented departures from the reference (all forced by the
+ * reference's container artefacts, not by its arithmetic):
+ *   - particles are held as one array sorted by (cell, stable id) instead of
+ *     3x3-cell bins in insertion order; float sums over particles therefore
+ *     run in that canonical order;
+ *   - countParticles keeps, per cell, the 2*ppc particles with the smallest
+ *     ids (the reference keeps the first 2*ppc in its bin storage order);
+ *   - the density-correction solve is selectable (off / gated / on); the
+ *     reference's own solve never converges, so its gate never opens and
+ *     "off" is result-identical to the reference.
+ */
+#ifndef FLIP_ORACLE_H
+#define FLIP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_SOLVER_REF_REPLICA = 0, ORC_SOLVER_CONVERGED = 1 };
+enum { ORC_DENSITY_OFF = 0, ORC_DENSITY_GATED = 1, ORC_DENSITY_ON = 2 };
+
+/* Material codes, materialgrid.h:6-13. */
+enum {
+    ORC_FLUID = 0x40, ORC_SOURCE = 0x41, ORC_SOLID = 0x20,
+    ORC_SINK = 0x12, ORC_EMPTY = 0x10
+};
+
+typedef struct OrcParams {
+    int32_t size_i, size_j;        /* grid cells, I along x (rows), J along y */
+    int32_t particles_per_cell;
+    int32_t pcg_iter_limit;
+    double  dx;                    /* cell size (m) */
+    double  fluid_density;         /* rho */
+    float   gravity_i, gravity_j;  /* globalAcceleration (m/s^2) */
+    float   pic_ratio;
+    float   cfl_number;
+    float   particle_scale;
+    int32_t fps;
+    int32_t max_substeps;
+    int32_t solver_mode;           /* ORC_SOLVER_* */
+    int32_t density_mode;          /* ORC_DENSITY_* */
+    int32_t threads;               /* splitRange thread count (replica reductions, CPU pool) */
+    double  project_tol;           /* replica: max-abs tol; converged: relative L2 residual */
+    int32_t n_obstacles;
+    const float* friction;         /* [n_obstacles] */
+} OrcParams;
+
+typedef struct OrcState {
+    int64_t   n_particles;         /* in/out */
+    int64_t   capacity;
+    float    *px, *py, *vx, *vy;   /* [capacity] */
+    uint32_t *pid;                 /* [capacity] stable particle id */
+    float    *u, *v;               /* (I+1)*J, I*(J+1) */
+    uint8_t  *u_valid, *v_valid;
+    int8_t   *material;            /* I*J */
+    float    *solid_sdf, *fluid_sdf;
+    int32_t  *solid_id;            /* I*J, -1 = none */
+    int32_t  *particle_counts;     /* I*J (out) */
+    float    *saved_u, *saved_v;   /* out: grid after extrapolation, before forces */
+    double   *pressure;            /* I*J out: last pressure solve */
+    double   *rhs;                 /* I*J out: last pressure rhs */
+    float    *density;             /* I*J out: density grid (density mode != off) */
+} OrcState;
+
+typedef struct OrcStats {
+    int32_t pressure_iters;
+    int32_t density_iters;
+    int32_t density_applied;
+    int32_t substeps;
+    double  pressure_rel_resid;    /* ||b-Ax||2/||b||2 of the returned pressure (FP64) */
+    double  density_rel_resid;
+    int64_t particles_killed;      /* advect: out of bounds / sink */
+    int64_t particles_erased;      /* countParticles: beyond 2*ppc */
+    float   last_dt;
+    float   max_velocity;
+    double  stage_ms[12];          /* SolverStage order, flipsolver2d.h:58-72 */
+} OrcStats;
+
+/* ---- interpolation & particle motion (grid2d.h, mathfuncs.cpp, sdfgrid.cpp) ---- */
+float orc_bspline(float v);
+float orc_quadratic_bspline(float x, float y);
+float orc_linear_hat(float v);
+void  orc_lerp(const float* g, int32_t si, int32_t sj, float off_i, float off_j,
+               const float* xi, const float* xj, int64_t n, float* out);
+void  orc_velocity_at(const float* u, const float* v, int32_t I, int32_t J,
+                      const float* xi, const float* xj, int64_t n, float* ou, float* ov);
+void  orc_rk4(const float* u, const float* v, int32_t I, int32_t J, float dt,
+              const float* xi, const float* xj, int64_t n, float* oi, float* oj);
+void  orc_closest_surface_point(const float* sdf, int32_t I, int32_t J,
+                                const float* xi, const float* xj, int64_t n,
+                                float* oi, float* oj);
+/* Polygon signed distance, geometry2d.cpp:54-73 (verts as x,y pairs). */
+float orc_polygon_sdf(const float* verts, int32_t nverts, float x, float y);
+
+/* ---- particles ---- */
+void  orc_advect(const OrcParams* p, OrcState* s, float dt, uint8_t* alive_out);
+void  orc_sort_particles(const OrcParams* p, OrcState* s, const uint8_t* alive, int32_t* cell_start);
+void  orc_cell_start(const OrcParams* p, const OrcState* s, int32_t* cell_start);
+float orc_max_particle_velocity(const OrcState* s);
+int32_t orc_count_particles(const OrcParams* p, OrcState* s, const int32_t* cell_start);
+void  orc_g2p(const OrcParams* p, OrcState* s);
+
+/* ---- grid transfer & rebuild ---- */
+void  orc_p2g(const OrcParams* p, OrcState* s, const int32_t* cell_start);
+void  orc_fluid_sdf(const OrcParams* p, OrcState* s, const int32_t* cell_start);
+void  orc_update_materials(const OrcParams* p, OrcState* s);
+void  orc_extrapolate_levelset_inside(float* sdf, int32_t I, int32_t J);
+void  orc_extrapolate(float* g, uint8_t* valid, int32_t si, int32_t sj, int32_t radius);
+void  orc_body_forces(const OrcParams* p, OrcState* s, float dt);
+void  orc_pressure_rhs(const OrcParams* p, const OrcState* s, double* rhs);
+void  orc_apply_pressure(const OrcParams* p, OrcState* s, float dt, const double* pr);
+void  orc_velocity_from_solids(const OrcParams* p, OrcState* s);
+void  orc_density_grid(const OrcParams* p, OrcState* s, const int32_t* cell_start);
+void  orc_density_rhs(const OrcParams* p, const OrcState* s, float dt, double* rhs);
+void  orc_adjust_particles(const OrcParams* p, OrcState* s, float dt, const double* pr);
+
+/* ---- pressure matrix & solvers (flipsolver2d.cpp:797-945, pressuredata.h, linearsolver.cpp) ---- */
+/* Per cell: nonsolid count n (0..4) and fluid-neighbour mask (bits i-,i+,j-,j+); fluid flag. */
+void  orc_build_matrix(const int8_t* material, int32_t I, int32_t J,
+                       uint8_t* n_out, uint8_t* mask_out, uint8_t* fluid_out);
+void  orc_spmv(const uint8_t* n, const uint8_t* mask, const uint8_t* fluid,
+               int32_t I, int32_t J, double scale, const double* x, double* y);
+void  orc_ipp_ref_apply(const int8_t* material_a, const uint8_t* fluid, int32_t I, int32_t J,
+                        double scale, const double* x, double* y);
+void  orc_ipp_apply(const uint8_t* n, const uint8_t* mask, const uint8_t* fluid,
+                    int32_t I, int32_t J, const double* x, double* y);
+int32_t orc_pcg_replica(const int8_t* material_a, int32_t I, int32_t J, double scale,
+                        const double* b, double* x, int32_t iter_limit, double tol,
+                        int32_t threads, double* err_trace);
+int32_t orc_pcg_converged(const int8_t* material_a, int32_t I, int32_t J, double scale,
+                          const double* b, double* x, int32_t iter_limit, double rel_tol,
+                          double* rel_resid_out);
+double orc_rel_residual(const int8_t* material_a, int32_t I, int32_t J, double scale,
+                        const double* b, const double* x);
+/* splitRange, threading/threadpool.cpp:41-76: fills starts/ends, returns count. */
+int32_t orc_split_range(int64_t length, int64_t min_size, int64_t jobs_per_thread,
+                        int32_t threads, int64_t* starts, int64_t* ends);
+
+/* ---- composition ---- */
+float   orc_frame_dt(float vel, float cfl, float frame_dt, float substep_time,
+                     int32_t substep_count, int32_t max_substeps, int32_t* finished);
+int32_t orc_substep(const OrcParams* p, OrcState* s, float dt, OrcStats* st);
+int32_t orc_step_frame(const OrcParams* p, OrcState* s, OrcStats* st);
+
+/* ---- first-frame init restated for dam-break style scenes ---- */
+/* Solid SDF / id / material from polygons (flipsolver2d.cpp:502-590). */
+void  orc_init_geometry(const OrcParams* p, OrcState* s,
+                        int32_t n_solids, const int32_t* solid_nverts, const float* solid_verts,
+                        int32_t n_fluids, const int32_t* fluid_nverts, const float* fluid_verts);
+/* Quadrant-jittered seeding with the counter RNG shared with the product's host init. */
+int64_t orc_seed_particles(const OrcParams* p, OrcState* s, uint32_t seed);
+float   orc_jitter(uint32_t seed, uint32_t cell, uint32_t k, uint32_t axis);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
