@@ -28,7 +28,9 @@ EXPORTS = [
     "fg_ctx_set_timing", "fg_scene_bytes", "fg_scene_create", "fg_scene_destroy", "fg_scene_init_geometry",
     "fg_scene_set_friction", "fg_scene_set_state", "fg_scene_get_state", "fg_scene_particle_count",
     "fg_scene_substep", "fg_scene_step_frame", "fg_scene_stage", "fg_batch_step_frame", "fg_poisson_solve",
+    "fg_ctx_profile_kernel", "fg_ctx_profile_read", "fg_ctx_bandwidth_probe",
 ]
+(K_ADVECT, K_SORT, K_P2G, K_SDF, K_LEVELSET, K_EXTRAPOLATE, K_PCG_A, K_PCG_B, K_G2P, K_RHS, K_APPLY) = range(11)
 
 
 class FgError(RuntimeError):
@@ -111,6 +113,9 @@ def lib():
             "fg_scene_stage": (C.c_int, [vp, i32, f32]),
             "fg_batch_step_frame": (C.c_int, [vp, vp, i32, vp]),
             "fg_poisson_solve": (C.c_int, [vp, i32, i32, vp, f64, vp, vp, f64, i32, C.POINTER(SolveStats)]),
+            "fg_ctx_profile_kernel": (C.c_int, [vp, i32]),
+            "fg_ctx_profile_read": (C.c_int, [vp, C.POINTER(f64), C.POINTER(i64), C.POINTER(i64)]),
+            "fg_ctx_bandwidth_probe": (C.c_int, [vp, C.c_uint64, i32, C.POINTER(f64), C.POINTER(f64)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -162,6 +167,19 @@ class Context:
         v = C.c_uint64()
         _check(lib().fg_ctx_bytes_in_use(self.h, C.byref(v)))
         return v.value
+
+    def profile_kernel(self, kernel):
+        _check(lib().fg_ctx_profile_kernel(self.h, kernel))
+
+    def profile_read(self):
+        t, n, u = C.c_double(), C.c_int64(), C.c_int64()
+        _check(lib().fg_ctx_profile_read(self.h, C.byref(t), C.byref(n), C.byref(u)))
+        return t.value, n.value, u.value
+
+    def bandwidth_probe(self, nbytes=1 << 30, reps=5):
+        g, ms = C.c_double(), C.c_double()
+        _check(lib().fg_ctx_bandwidth_probe(self.h, nbytes, reps, C.byref(g), C.byref(ms)))
+        return g.value, ms.value
 
     def poisson_solve(self, material, scale, b, tol=1e-6, max_iters=0):
         material = np.ascontiguousarray(material, np.int8)

@@ -15,7 +15,9 @@ constexpr uint32_t kDead = 0xffffffffu;  // particle key of a removed particle
 constexpr int kMaxLevel = 0x3fffffff;    // "unreached" BFS marker
 constexpr int kMaxSweeps = 8;            // FP64 refinement sweeps per solve
 constexpr int kMaxBands = 512;           // level-set bands of 16 levels
-constexpr int kSlots = 5;                // reduction slots: bb, pq, rz, rr, rr64
+constexpr int kSlots = 5;                // FP64 reduction slots
+constexpr int kSlotBB = 0, kSlotPQ = 1, kSlotCG = 2 /* and 3: r.z, r.r */, kSlotRR64 = 4;
+constexpr int kPcgMaxBlocks = 1024;      // solver blocks per scene (contiguous cell chunks)
 
 // Material codes and tests, FlipSolver2dLib/materialgrid.h:6-20.
 enum : int8_t { M_FLUID = 0x40, M_SOURCE = 0x41, M_SOLID = 0x20, M_SINK = 0x12, M_EMPTY = 0x10 };
@@ -39,7 +41,7 @@ struct SceneScalars {
     double pq;              // inner p.q
     double rr0;             // inner ||r||^2 at sweep start
     double rr;              // inner ||r||^2
-    double alpha, beta;
+    double gamma[2];        // r.z of iteration it (parity slots)
     double inner_eps2;      // inner stop: ||r||^2 <= eps2 * ||r0||^2
     int32_t inner_done;     // inner PCG stopped (converged or capped)
     int32_t outer_done;     // FP64 residual reached tolerance
@@ -63,8 +65,10 @@ struct SceneDev {
     double tol;             // FP64 relative residual target
     int32_t max_inner;      // inner iteration cap per sweep
     int32_t active;         // 0: skipped by every kernel of a batched launch
-    int32_t max_blocks;     // stride of the partials array (>= cell blocks)
-    int32_t pad0;
+    int32_t max_blocks;     // stride of the FP64 partials array (>= pcg_blocks)
+    int32_t pcg_blocks;     // solver blocks of this scene
+    int32_t pcg_chunk;      // cells per solver block (multiple of kBlock)
+    int32_t max_pblocks;    // stride of the particle partials arrays
     // per-substep
     float dt;
     float force;            // (float)(dt / dx), applyBodyForces
@@ -92,10 +96,13 @@ struct SceneDev {
     float *u, *v, *su, *sv, *uf, *vf;
     uint8_t *uval, *vval, *uvalS, *vvalS, *uvalF, *vvalF;
     // solver
-    double *b, *x64, *x64b;
-    float *r, *r2, *p, *p2, *z, *q, *d;
+    double *b, *x64;
+    // CG vectors: cr/cr2 residual (double-buffered), cu = z, cp/cs1 direction
+    // (double-buffered), cw = q = A' p, d = correction
+    float *cr, *cr2, *cw, *cs1, *cu, *cp, *d;
     double* partials;       // per-block partial sums [kSlots * max_blocks]
-    int32_t* counters;      // last-block-done tickets [kSlots]
+    int32_t* ipart;         // per-block particle counters [2 * max_pblocks]: killed, erased
+    float* fpart;           // per-block max |v|^2 [max_pblocks]
     float* friction;
     SceneScalars* sc;
 };
@@ -228,34 +235,47 @@ __device__ __forceinline__ int block_sumi(int v, int* lds)
 
 __device__ __forceinline__ bool finite_d(double x) { return x == x && x < 1e300 && x > -1e300; }
 
-// Finish a block reduction of NV doubles into reduction slot `slot`; the last
-// block of the scene to arrive sums the per-block partials in block order
-// (deterministic) and calls fin(totals) in thread 0.
-template <int NV, typename F>
-__device__ __forceinline__ void reduce_last(const SceneDev& s, double (&v)[NV], int slot, int nb, F fin)
+// Solver blocks own contiguous chunks of cells.
+struct Chunk {
+    int begin, end;
+};
+__device__ __forceinline__ Chunk my_chunk(const SceneDev& s)
+{
+    const int C = s.I * s.J;
+    const int beg = (int)blockIdx.x * s.pcg_chunk;
+    return Chunk{beg < C ? beg : C, min(C, beg + s.pcg_chunk)};
+}
+
+// Sum of the nb per-block partials of `slot`, in a fixed order (thread t
+// takes partials t, t+256, ...; then a fixed shuffle/LDS tree), broadcast to
+// every thread.  Every block computing it gets the identical value.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const SceneDev& s, int slot, int nb, double (&out)[NV])
 {
     __shared__ double red[kBlock / 64];
-    __shared__ int last;
-    double tot[NV];
+    __shared__ double bc[NV];
 #pragma unroll
-    for (int q = 0; q < NV; q++) tot[q] = block_sum(v[q], red);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; q++) s.partials[(size_t)(slot + q) * s.max_blocks + blockIdx.x] = tot[q];
-        __threadfence();
-        last = atomicAdd(&s.counters[slot], 1) == nb - 1;
+    for (int q = 0; q < NV; q++) {
+        const double* part = s.partials + (size_t)(slot + q) * s.max_blocks;
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < nb; b += kBlock) acc += part[b];
+        const double t = block_sum(acc, red);
+        if (threadIdx.x == 0) bc[q] = t;
     }
     __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        double t[NV];
 #pragma unroll
-        for (int q = 0; q < NV; q++) {
-            t[q] = 0.0;
-            for (int b = 0; b < nb; b++) t[q] += s.partials[(size_t)(slot + q) * s.max_blocks + b];
-        }
-        s.counters[slot] = 0;
-        fin(t);
+    for (int q = 0; q < NV; q++) out[q] = bc[q];
+}
+
+// Block-reduce NV doubles and store them as this block's partials.
+template <int NV>
+__device__ __forceinline__ void put_partials(const SceneDev& s, int slot, double (&v)[NV])
+{
+    __shared__ double red[kBlock / 64];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        const double t = block_sum(v[q], red);
+        if (threadIdx.x == 0) s.partials[(size_t)(slot + q) * s.max_blocks + blockIdx.x] = t;
     }
 }
 

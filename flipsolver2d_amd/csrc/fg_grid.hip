@@ -106,41 +106,72 @@ __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
 // Fluid SDF: FlipSolver::updateSdfThread (flipsolver2d.cpp:1276-1311) over
 // the reference's window -- the 3x3 bins of 3x3 cells around the cell's bin
 // (markerparticlesystem.cpp:9-20, 109-128) -- then updateMaterials
-// (1053-1075) and the per-cell particle count of countParticles.
+// (1053-1075) and the per-cell particle count of countParticles.  All nine
+// cells of a bin share that window, so one thread per bin streams the
+// window's particles once and keeps nine running minima (min is exact, so
+// the result equals the per-cell loop bit for bit).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_sdf_materials(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
-    const int I = s.I, J = s.J, C = I * J;
-    const int c = cell_index(gridDim.x);
-    if (c >= C) return;
-    const int i = c / J, j = c - (c / J) * J;
-    const int bi = i / 3, bj = j / 3;
+    const int I = s.I, J = s.J;
+    const int nbI = (I + 2) / 3, nbJ = (J + 2) / 3;
+    const int bin = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (bin >= nbI * nbJ) return;
+    const int bi = bin / nbJ, bj = bin - (bin / nbJ) * nbJ;
     const int r0 = max(0, 3 * bi - 3), r1 = min(I, 3 * bi + 6);
     const int c0 = max(0, 3 * bj - 3), c1 = min(J, 3 * bj + 6);
-    const float cx = (float)i + 0.5f, cy = (float)j + 0.5f;
-    float d2 = 3.402823466e+38f;
+    const int i0 = 3 * bi, j0 = 3 * bj;
+    float cx[3], cy[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        cx[q] = (float)(i0 + q) + 0.5f;
+        cy[q] = (float)(j0 + q) + 0.5f;
+    }
+    float d2[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) d2[a][b] = 3.402823466e+38f;
     for (int r = r0; r < r1; r++) {
-        const int a = s.cellStart[r * J + c0], e = s.cellStart[r * J + c1];
-        for (int k = a; k < e; k++) {
+        const int e = s.cellStart[r * J + c1];
+        for (int k = s.cellStart[r * J + c0]; k < e; k++) {
             const float2 p = s.pos[k];
-            const float dxp = p.x - cx, dyp = p.y - cy;
-            const float nd = dxp * dxp + dyp * dyp;
-            if (nd < d2) d2 = nd;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const float dxp = p.x - cx[a];
+                const float dx2 = dxp * dxp;
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    const float dyp = p.y - cy[b];
+                    const float nd = dx2 + dyp * dyp;
+                    if (nd < d2[a][b]) d2[a][b] = nd;
+                }
+            }
         }
     }
-    const float sdf = sqrtf(d2) - s.pscale;
-    s.fsdf[c] = sdf;
-    int8_t m = s.mat[c];
-    if (sdf < 0.f) {
-        if (is_empty(m)) m = M_FLUID;
-    } else {
-        if (is_strict_fluid(m)) m = M_EMPTY;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int i = i0 + a;
+        if (i >= I) continue;
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const int j = j0 + b;
+            if (j >= J) continue;
+            const int c = i * J + j;
+            const float sdf = sqrtf(d2[a][b]) - s.pscale;
+            s.fsdf[c] = sdf;
+            int8_t m = s.mat[c];
+            if (sdf < 0.f) {
+                if (is_empty(m)) m = M_FLUID;
+            } else {
+                if (is_strict_fluid(m)) m = M_EMPTY;
+            }
+            s.mat[c] = m;
+            s.pcount[c] = min(s.cellStart[c + 1] - s.cellStart[c], 2 * s.ppc);
+        }
     }
-    s.mat[c] = m;
-    const int cnt = s.cellStart[c + 1] - s.cellStart[c];
-    s.pcount[c] = min(cnt, 2 * s.ppc);
 }
 
 // ---------------------------------------------------------------------------
@@ -165,94 +196,120 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
                          int32_t* mkout, int si, int sj, int L0, int* more)
 {
     constexpr int R = kTile + 2 * H;
+    constexpr int T = 16;   // threads as a 16 x 16 layout: no index divisions in the passes
     __shared__ int16_t mk[R * R];
     __shared__ float val[R * R];
+    const int tx = threadIdx.x & (T - 1), ty = threadIdx.x >> 4;
     const int tilesJ = (sj + kTile - 1) / kTile;
     const int tile = blockIdx.x;
     const int ti = tile / tilesJ, tj = tile - (tile / tilesJ) * tilesJ;
     const int gi0 = ti * kTile - H, gj0 = tj * kTile - H;
-    for (int idx = threadIdx.x; idx < R * R; idx += kBlock) {
-        const int ri = idx / R, rj = idx - (idx / R) * R;
-        const int gi = gi0 + ri, gj = gj0 + rj;
-        int16_t m = kNone;
-        float x = 0.f;
-        if (gi >= 0 && gi < si && gj >= 0 && gj < sj) {
-            const size_t g = (size_t)gi * sj + gj;
-            x = gin[g];
-            if (LEVELSET) {
-                const int lv = mkin[g];
-                m = lv <= L0 ? (int16_t)lv : kUnreached;
-            } else {
-                m = vin[g] ? 0 : kUnreached;
+    bool has_front = false, has_open = false;
+    for (int ri = ty; ri < R; ri += T)
+        for (int rj = tx; rj < R; rj += T) {
+            const int gi = gi0 + ri, gj = gj0 + rj;
+            int16_t m = kNone;
+            float x = 0.f;
+            if (gi >= 0 && gi < si && gj >= 0 && gj < sj) {
+                const size_t g = (size_t)gi * sj + gj;
+                x = gin[g];
+                if (LEVELSET) {
+                    const int lv = mkin[g];
+                    m = lv <= L0 ? (int16_t)lv : kUnreached;
+                } else {
+                    m = vin[g] ? 0 : kUnreached;
+                }
             }
+            has_front |= m == L0;
+            has_open |= m == kUnreached;
+            mk[ri * R + rj] = m;
+            val[ri * R + rj] = x;
         }
-        mk[idx] = m;
-        val[idx] = x;
+    // a tile cell reaches level L0+s only through a level-L0 cell within
+    // distance s <= H, i.e. inside this region
+    const bool work = __syncthreads_or(has_front) && __syncthreads_or(has_open);
+    if (!work) {
+        if (!LEVELSET) {
+            for (int li = ty; li < kTile; li += T)
+                for (int lj = tx; lj < kTile; lj += T) {
+                    const int gi = ti * kTile + li, gj = tj * kTile + lj;
+                    if (gi >= si || gj >= sj) continue;
+                    const size_t g = (size_t)gi * sj + gj;
+                    const int r = (li + H) * R + (lj + H);
+                    gout[g] = val[r];
+                    vout[g] = mk[r] != kUnreached;
+                }
+        }
+        return;
     }
-    __syncthreads();
     // expand levels L0+1 .. L0+H from the level-L0 frontier
+    int top = L0;
     for (int st = 1; st <= H; st++) {
         const int16_t prev = (int16_t)(L0 + st - 1), cur = (int16_t)(L0 + st);
-        for (int idx = threadIdx.x; idx < R * R; idx += kBlock) {
-            if (mk[idx] != kUnreached) continue;
-            const int ri = idx / R, rj = idx - (idx / R) * R;
-            bool hit = false;
-            for (int di = -1; di <= 1 && !hit; di++)
-                for (int dj = -1; dj <= 1; dj++) {
-                    const int qi = ri + di, qj = rj + dj;
-                    if ((di == 0 && dj == 0) || qi < 0 || qi >= R || qj < 0 || qj >= R) continue;
-                    if (mk[qi * R + qj] == prev) {
-                        hit = true;
-                        break;
+        bool changed = false;
+        for (int ri = ty; ri < R; ri += T)
+            for (int rj = tx; rj < R; rj += T) {
+                if (mk[ri * R + rj] != kUnreached) continue;
+                bool hit = false;
+#pragma unroll
+                for (int di = -1; di <= 1; di++)
+#pragma unroll
+                    for (int dj = -1; dj <= 1; dj++) {
+                        const int qi = ri + di, qj = rj + dj;
+                        if ((di == 0 && dj == 0) || qi < 0 || qi >= R || qj < 0 || qj >= R) continue;
+                        hit |= mk[qi * R + qj] == prev;
                     }
+                if (hit) {
+                    mk[ri * R + rj] = cur;
+                    changed = true;
                 }
-            if (hit) mk[idx] = cur;
-        }
-        __syncthreads();
+            }
+        if (!__syncthreads_or(changed)) break;
+        top = L0 + st;
     }
     // values, level by level
-    for (int st = 1; st <= H; st++) {
-        const int16_t L = (int16_t)(L0 + st);
-        for (int idx = threadIdx.x; idx < R * R; idx += kBlock) {
-            if (mk[idx] != L) continue;
-            const int ri = idx / R, rj = idx - (idx / R) * R;
-            double avg = 0;
-            int count = 0;
-            for (int di = -1; di <= 1; di++)
-                for (int dj = -1; dj <= 1; dj++) {
-                    const int qi = ri + di, qj = rj + dj;
-                    if ((di == 0 && dj == 0) || qi < 0 || qi >= R || qj < 0 || qj >= R) continue;
-                    const int16_t q = mk[qi * R + qj];
-                    if (q < L) {
-                        avg += (double)val[qi * R + qj];
-                        count++;
+    for (int L = L0 + 1; L <= top; L++) {
+        for (int ri = ty; ri < R; ri += T)
+            for (int rj = tx; rj < R; rj += T) {
+                if (mk[ri * R + rj] != L) continue;
+                double avg = 0;
+                int count = 0;
+#pragma unroll
+                for (int di = -1; di <= 1; di++)
+#pragma unroll
+                    for (int dj = -1; dj <= 1; dj++) {
+                        const int qi = ri + di, qj = rj + dj;
+                        if ((di == 0 && dj == 0) || qi < 0 || qi >= R || qj < 0 || qj >= R) continue;
+                        if (mk[qi * R + qj] < L) {
+                            avg += (double)val[qi * R + qj];
+                            count++;
+                        }
                     }
-                }
-            if (LEVELSET) val[idx] = (float)(avg / count - 1.f);
-            else val[idx] = (float)(avg / count);
-        }
+                if (LEVELSET) val[ri * R + rj] = (float)(avg / count - 1.f);
+                else val[ri * R + rj] = (float)(avg / count);
+            }
         __syncthreads();
     }
     // write the tile
-    int reached_end = 0;
-    for (int idx = threadIdx.x; idx < kTile * kTile; idx += kBlock) {
-        const int li = idx / kTile, lj = idx - (idx / kTile) * kTile;
-        const int gi = ti * kTile + li, gj = tj * kTile + lj;
-        if (gi >= si || gj >= sj) continue;
-        const int r = (li + H) * R + (lj + H);
-        const size_t g = (size_t)gi * sj + gj;
-        const int16_t m = mk[r];
-        if (LEVELSET) {
-            if (m > L0 && m <= L0 + H) {
+    bool reached_end = false;
+    for (int li = ty; li < kTile; li += T)
+        for (int lj = tx; lj < kTile; lj += T) {
+            const int gi = ti * kTile + li, gj = tj * kTile + lj;
+            if (gi >= si || gj >= sj) continue;
+            const int r = (li + H) * R + (lj + H);
+            const size_t g = (size_t)gi * sj + gj;
+            const int16_t m = mk[r];
+            if (LEVELSET) {
+                if (m > L0 && m <= L0 + H) {
+                    gout[g] = val[r];
+                    mkout[g] = m;
+                    reached_end |= m == L0 + H;
+                }
+            } else {
                 gout[g] = val[r];
-                mkout[g] = m;
-                if (m == L0 + H) reached_end = 1;
+                vout[g] = (m != kUnreached) ? 1 : 0;
             }
-        } else {
-            gout[g] = val[r];
-            vout[g] = (m != kUnreached) ? 1 : 0;
         }
-    }
     if (LEVELSET && reached_end) *more = 1;
 }
 
@@ -318,18 +375,18 @@ __global__ void __launch_bounds__(kBlock) k_forces(const SceneDev* scenes)
 // (flipsolver2d.cpp:759-764, 962-991).  Divergence is formed in float and
 // widened, as there; the divergence-control term is zero because
 // centeredParamsToGrid clears it before this read (flipsolver2d.cpp:1382).
-// Also starts the solve: x = 0, ||b||^2.
+// Also starts the solve: x = 0, d = 0, partials of ||b||^2.  Runs on the
+// solver's block/chunk layout.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_rhs(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
-    const int I = s.I, J = s.J, C = I * J;
-    const int nb = (C + kBlock - 1) / kBlock;
-    if ((int)blockIdx.x >= nb) return;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if ((int)blockIdx.x >= s.pcg_blocks) return;
+    const int I = s.I, J = s.J;
+    const Chunk ch = my_chunk(s);
     double bb = 0.0;
-    if (c < C) {
+    for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
         const int i = c / J, j = c - (c / J) * J;
         double val = 0.0;
         const int8_t m = s.mat[c];
@@ -347,13 +404,10 @@ __global__ void __launch_bounds__(kBlock) k_rhs(const SceneDev* scenes)
         s.b[c] = val;
         s.x64[c] = 0.0;
         s.d[c] = 0.f;
-        bb = val * val;
+        bb += val * val;
     }
     double v[1] = {bb};
-    reduce_last<1>(s, v, 0, nb, [&](const double* t) {
-        s.sc->bb = t[0];
-        s.sc->rr64 = t[0];
-    });
+    put_partials<1>(s, kSlotBB, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -425,9 +479,9 @@ __global__ void __launch_bounds__(kBlock) k_apply_solids(const SceneDev* scenes)
 
 namespace fg {
 #define FG_LAUNCH0(name) \
-    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc) { hipLaunchKernelGGL(k_##name, g, dim3(kBlock), 0, st, sc); }
+    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc) { fg_launch(k_##name, g, dim3(kBlock), st, sc); }
 #define FG_LAUNCH1(name, T, a) \
-    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc, T a) { hipLaunchKernelGGL(k_##name, g, dim3(kBlock), 0, st, sc, a); }
+    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc, T a) { fg_launch(k_##name, g, dim3(kBlock), st, sc, a); }
 FG_LAUNCH0(stencil)
 FG_LAUNCH0(p2g)
 FG_LAUNCH0(sdf_materials)

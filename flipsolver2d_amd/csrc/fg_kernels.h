@@ -4,9 +4,30 @@
 // beyond a scene's extent exit at once.
 #pragma once
 
+#include <hip/hip_ext.h>
+
 #include "fg_device.h"
 
 namespace fg {
+
+// Events armed here are attached to the next launch (hipExtLaunchKernelGGL),
+// so they time exactly that kernel on its stream.
+struct LaunchEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+LaunchEvents& next_launch_events();
+
+template <typename F, typename... Args>
+inline void fg_launch(F kernel, dim3 grid, dim3 block, hipStream_t st, Args... args)
+{
+    LaunchEvents& ev = next_launch_events();
+    if (ev.start) {
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, st, ev.start, ev.stop, 0, args...);
+        ev = LaunchEvents{};
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+    }
+}
 
 #define FG_DECLARE_LAUNCH(name, ...) void launch_##name(dim3 grid, hipStream_t st, const SceneDev* scenes, ##__VA_ARGS__)
 
@@ -17,9 +38,10 @@ FG_DECLARE_LAUNCH(scan_local);
 FG_DECLARE_LAUNCH(scan_top);
 FG_DECLARE_LAUNCH(scan_add);
 FG_DECLARE_LAUNCH(scatter);
-FG_DECLARE_LAUNCH(fixup_gather);
-FG_DECLARE_LAUNCH(commit_count);
+FG_DECLARE_LAUNCH(fixup);
+FG_DECLARE_LAUNCH(gather);
 FG_DECLARE_LAUNCH(g2p_count);
+FG_DECLARE_LAUNCH(finalize, int nb, int what);
 constexpr int kScanTileCells = kBlock * 16;
 
 // fg_grid.hip
@@ -37,9 +59,9 @@ constexpr int kBfsTile = 32;
 // fg_pcg.hip
 FG_DECLARE_LAUNCH(refine_update);
 FG_DECLARE_LAUNCH(refine_residual);
-FG_DECLARE_LAUNCH(inner_init);
-FG_DECLARE_LAUNCH(pcg_a, int par);
-FG_DECLARE_LAUNCH(pcg_b, int par);
+FG_DECLARE_LAUNCH(cg_init, int sweep);
+FG_DECLARE_LAUNCH(pcg_a, int it);
+FG_DECLARE_LAUNCH(pcg_b, int it);
 FG_DECLARE_LAUNCH(solve_begin);
 FG_DECLARE_LAUNCH(solve_reset);
 

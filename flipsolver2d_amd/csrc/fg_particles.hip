@@ -4,16 +4,40 @@
 // velocity, u32 stable id), kept sorted by (cell, id) after every substep.
 // The reference's 3x3-cell bins and rebin sets
 // (FlipSolver2dLib/markerparticlesystem.cpp:47-58, 260-269, 317-342) become a
-// counting sort on the cell key with a per-cell id fixup, which is
-// deterministic (no float atomics anywhere) and O(particles + cells).
+// counting sort on the cell key with a per-cell id fixup: deterministic (no
+// float atomics anywhere) and O(particles + cells).
+//
+// Counters (kills, erasures, max |v|^2) are per-block partials reduced by a
+// one-block kernel: no two blocks ever update the same address.  The only
+// atomics are the sort's per-cell counters, aggregated over runs of equal
+// keys within a wave (particles arrive sorted by their old cell, so most
+// runs are long).
 #include "fg_kernels.h"
 
 namespace fg {
 
 // ---------------------------------------------------------------------------
+// wave64 runs of equal keys: head lane of this lane's run and the run length
+// ---------------------------------------------------------------------------
+struct Run {
+    int head, len;
+};
+__device__ __forceinline__ Run key_run(uint32_t key)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool is_head = lane == 0 || prev != key;
+    const unsigned long long heads = __ballot(is_head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1);   // bits <= lane
+    const int head = 63 - __clzll(heads & upto);
+    const unsigned long long above = heads & ~upto;
+    const int end = above ? (__ffsll((long long)above) - 1) : 64;
+    return Run{head, end - head};
+}
+
+// ---------------------------------------------------------------------------
 // max |v|^2 over live particles (FlipSolver::maxParticleVelocity,
-// flipsolver2d.cpp:1560-1574).  Max of non-negative floats is exact and
-// order-free, so an integer atomicMax on the float bits is deterministic.
+// flipsolver2d.cpp:1560-1574): per-block maxima, reduced by k_finalize.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_max_v2(const SceneDev* scenes)
 {
@@ -21,14 +45,17 @@ __global__ void __launch_bounds__(kBlock) k_max_v2(const SceneDev* scenes)
     if (!s.active) return;
     __shared__ float red[kBlock / 64];
     const int n = s.sc->n_particles;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
     float m = 0.f;
-    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
-        if (s.key[k] == kDead) continue;
+    if (k < n && s.key[k] != kDead) {
         const float2 v = s.vel[k];
-        m = fmaxf(m, v.x * v.x + v.y * v.y);
+        m = v.x * v.x + v.y * v.y;
     }
     m = block_maxf(m, red);
-    if (threadIdx.x == 0 && m > 0.f) atomicMax(&s.sc->max_v2_bits, __float_as_uint(m));
+    if (threadIdx.x == 0) {
+        s.fpart[blockIdx.x] = m;
+        s.ipart[s.max_pblocks + blockIdx.x] = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -94,36 +121,37 @@ __global__ void __launch_bounds__(kBlock) k_advect(const SceneDev* scenes, int a
     const int I = s.I, J = s.J;
     int killed = 0;
     const int k = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t key = kDead;
     if (k < n) {
         const bool was_alive = !alive_from_key || s.key[k] != kDead;
         float2 p = s.pos[k];
         const float2 vel = s.vel[k];
-        uint32_t key = kDead;
         if (was_alive) {
             p = rk4(s.uf, s.vf, I, J, p, s.dt);
             if (lerp_grid(s.ssdf, I, J, 0.f, 0.f, p.x, p.y) < 0.f) p = closest_surface_point(s.ssdf, I, J, p);
             const int pi = integr(p.x), pj = integr(p.y);
             const bool inb = pi >= 0 && pi < I && pj >= 0 && pj < J;
-            if (!inb || is_sink(s.mat[(size_t)pi * J + pj])) {
-                killed = 1;
-            } else {
-                key = (uint32_t)(pi * J + pj);
-                atomicAdd(&s.cellCount[key], 1);
-            }
+            if (!inb || is_sink(s.mat[(size_t)pi * J + pj])) killed = 1;
+            else key = (uint32_t)(pi * J + pj);
         }
         s.posA[k] = p;
         s.velA[k] = vel;
         s.key[k] = key;
     }
+    // histogram: one atomic per run of equal keys in the wave
+    const Run run = key_run(key);
+    if ((int)(threadIdx.x & 63) == run.head && key != kDead) atomicAdd(&s.cellCount[key], run.len);
     killed = block_sumi(killed, red);
-    if (threadIdx.x == 0 && killed) atomicAdd(&s.sc->killed, killed);
+    if (threadIdx.x == 0) s.ipart[blockIdx.x] = killed;
 }
 
 // ---------------------------------------------------------------------------
 // Counting sort on cell key.
 //   scan: exclusive prefix sum of cellCount -> cellStart (3 passes)
-//   scatter: slot = cellStart[key] + (--cellCount[key]); leaves cellCount 0
-//   fixup: per cell, order the slots by stable id, then gather the payload
+//   scatter: slots from cellStart[key] and a per-run atomic on cellCount
+//            (leaves cellCount at 0 for the next substep)
+//   fixup: per cell, order the (id, source) pairs by id
+//   gather: move the payload into slot order (coalesced writes)
 // ---------------------------------------------------------------------------
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kBlock * kScanItems;   // 4096 cells per block
@@ -210,22 +238,29 @@ __global__ void __launch_bounds__(kBlock) k_scan_add(const SceneDev* scenes)
 }
 
 // Runs over the advected (pre-sort) arrays: sc->n_particles is still the
-// pre-sort count until k_commit_count.
+// pre-sort count until k_finalize commits the new one.
 __global__ void __launch_bounds__(kBlock) k_scatter(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
     const int k = blockIdx.x * kBlock + threadIdx.x;
     const int n = s.sc->n_particles;
-    if (k >= n) return;
-    const uint32_t key = s.key[k];
+    const uint32_t key = k < n ? s.key[k] : kDead;
+    const Run run = key_run(key);
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == run.head && key != kDead) base = atomicSub(&s.cellCount[key], run.len) - run.len;
+    base = __shfl(base, run.head, 64);
     if (key == kDead) return;
-    const int slot = s.cellStart[key] + atomicSub(&s.cellCount[key], 1) - 1;
+    const int slot = s.cellStart[key] + base + (lane - run.head);
     s.tmpPid[slot] = s.pid[k];
     s.tmpSrc[slot] = (uint32_t)k;
 }
 
-__global__ void __launch_bounds__(kBlock) k_fixup_gather(const SceneDev* scenes)
+// Per cell: order the slots by stable id.  Segments of up to 16 entries are
+// sorted in registers by an odd-even transposition network; longer ones
+// (pile-ups) by insertion sort in place.
+__global__ void __launch_bounds__(kBlock) k_fixup(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
@@ -233,7 +268,39 @@ __global__ void __launch_bounds__(kBlock) k_fixup_gather(const SceneDev* scenes)
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= C) return;
     const int a = s.cellStart[c], e = s.cellStart[c + 1];
-    // insertion sort of (pid, src) by pid -- segments hold ~ppc entries
+    const int n = e - a;
+    if (n <= 1) return;
+    if (n <= 16) {
+        uint32_t kp[16], ks[16];
+        bool sorted = true;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            kp[q] = q < n ? s.tmpPid[a + q] : 0xffffffffu;
+            ks[q] = q < n ? s.tmpSrc[a + q] : 0u;
+            if (q > 0 && q < n && kp[q] < kp[q - 1]) sorted = false;
+        }
+        if (sorted) return;
+#pragma unroll
+        for (int round = 0; round < 16; round++) {
+#pragma unroll
+            for (int q = round & 1; q + 1 < 16; q += 2) {
+                const bool sw = kp[q] > kp[q + 1];
+                const uint32_t p0 = sw ? kp[q + 1] : kp[q], p1 = sw ? kp[q] : kp[q + 1];
+                const uint32_t s0 = sw ? ks[q + 1] : ks[q], s1 = sw ? ks[q] : ks[q + 1];
+                kp[q] = p0;
+                kp[q + 1] = p1;
+                ks[q] = s0;
+                ks[q + 1] = s1;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+            if (q < n) {
+                s.tmpPid[a + q] = kp[q];
+                s.tmpSrc[a + q] = ks[q];
+            }
+        return;
+    }
     for (int x = a + 1; x < e; x++) {
         const uint32_t kp = s.tmpPid[x], ks = s.tmpSrc[x];
         int y = x - 1;
@@ -245,20 +312,58 @@ __global__ void __launch_bounds__(kBlock) k_fixup_gather(const SceneDev* scenes)
         s.tmpPid[y + 1] = kp;
         s.tmpSrc[y + 1] = ks;
     }
-    for (int x = a; x < e; x++) {
-        const uint32_t src = s.tmpSrc[x];
-        s.pos[x] = s.posA[src];
-        s.vel[x] = s.velA[src];
-        s.pid[x] = s.tmpPid[x];
-        s.key[x] = (uint32_t)c;
-    }
 }
 
-__global__ void k_commit_count(const SceneDev* scenes)
+__global__ void __launch_bounds__(kBlock) k_gather(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
-    if (!s.active || threadIdx.x != 0) return;
-    s.sc->n_particles = s.cellStart[s.I * s.J];
+    if (!s.active) return;
+    const int total = s.cellStart[s.I * s.J];
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= total) return;
+    const uint32_t src = s.tmpSrc[k];
+    s.pos[k] = s.posA[src];
+    s.vel[k] = s.velA[src];
+    s.pid[k] = s.tmpPid[k];
+    s.key[k] = 0;   // live
+}
+
+// ---------------------------------------------------------------------------
+// One-block reduction of the per-block particle partials.
+//   what & 1: commit the sorted count, sum the advect kills
+//   what & 2: max |v|^2 (flipsolver2d.cpp:1562 starts from FLT_MIN) and
+//             the countParticles erasures
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_finalize(const SceneDev* scenes, int nb, int what)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active) return;
+    __shared__ int redi[kBlock / 64];
+    __shared__ float redf[kBlock / 64];
+    int kills = 0, erased = 0;
+    float m = 0.f;
+    for (int b = threadIdx.x; b < nb; b += kBlock) {
+        if (what & 1) kills += s.ipart[b];
+        if (what & 2) {
+            erased += s.ipart[s.max_pblocks + b];
+            m = fmaxf(m, s.fpart[b]);
+        }
+    }
+    kills = block_sumi(kills, redi);
+    erased = block_sumi(erased, redi);
+    m = block_maxf(m, redf);
+    if (threadIdx.x == 0) {
+        SceneScalars* sc = s.sc;
+        if (what & 1) {
+            sc->n_particles = s.cellStart[s.I * s.J];
+            sc->killed = kills;
+        }
+        if (what & 2) {
+            const float fmin = 1.17549435e-38f;
+            sc->max_v2_bits = __float_as_uint(m > fmin ? m : fmin);
+            sc->erased = erased;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -300,25 +405,31 @@ __global__ void __launch_bounds__(kBlock) k_g2p_count(const SceneDev* scenes)
         }
     }
     m = block_maxf(m, redf);
-    if (threadIdx.x == 0 && m > 0.f) atomicMax(&s.sc->max_v2_bits, __float_as_uint(m));
     erased = block_sumi(erased, redi);
-    if (threadIdx.x == 0 && erased) atomicAdd(&s.sc->erased, erased);
+    if (threadIdx.x == 0) {
+        s.fpart[blockIdx.x] = m;
+        s.ipart[s.max_pblocks + blockIdx.x] = erased;
+    }
 }
 
 }  // namespace fg
 
 namespace fg {
 #define FG_LAUNCH0(name) \
-    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc) { hipLaunchKernelGGL(k_##name, g, dim3(kBlock), 0, st, sc); }
+    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc) { fg_launch(k_##name, g, dim3(kBlock), st, sc); }
 #define FG_LAUNCH1(name, T, a) \
-    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc, T a) { hipLaunchKernelGGL(k_##name, g, dim3(kBlock), 0, st, sc, a); }
+    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc, T a) { fg_launch(k_##name, g, dim3(kBlock), st, sc, a); }
 FG_LAUNCH0(max_v2)
 FG_LAUNCH1(advect, int, alive_from_key)
 FG_LAUNCH0(scan_local)
 FG_LAUNCH0(scan_top)
 FG_LAUNCH0(scan_add)
 FG_LAUNCH0(scatter)
-FG_LAUNCH0(fixup_gather)
-void launch_commit_count(dim3 g, hipStream_t st, const SceneDev* sc) { hipLaunchKernelGGL(k_commit_count, g, dim3(64), 0, st, sc); }
+FG_LAUNCH0(fixup)
+FG_LAUNCH0(gather)
 FG_LAUNCH0(g2p_count)
+void launch_finalize(dim3 g, hipStream_t st, const SceneDev* sc, int nb, int what)
+{
+    hipLaunchKernelGGL(k_finalize, g, dim3(kBlock), 0, st, sc, nb, what);
+}
 }  // namespace fg

@@ -8,9 +8,7 @@
 // iterative refinement: each sweep forms r64 = b - A x in FP64, solves the
 // correction A' d = r' with FP32 PCG (A' = A/s on fluid rows, identity
 // elsewhere, so the stencil has integer coefficients), and adds d into x in
-// FP64.  Dot products are FP64; each is reduced per block and the block
-// partials are summed in block order by the last block to finish, so every
-// result is deterministic.
+// FP64.
 //
 // Preconditioner: the symmetric inverse-Poisson preconditioner the
 // reference defines in inversepoissonpreconditioner.h:45-79,
@@ -20,11 +18,19 @@
 // matrix-free IPP (PressureIPPCoeficients.h:23-40) is mis-scaled and not
 // symmetric and does not converge; it is reproduced only in the oracle.
 //
-// Each PCG iteration is two kernels, each a single pass over the grid:
-//   KA: p = z + beta p_old (recomputed at the 4 neighbours), q = A' p, p.q
-//   KB: r = r_old - alpha q (recomputed at the neighbours), d += alpha p,
-//       z = M r, r.z and r.r
-// so no vector is re-read by a separate axpy kernel.
+// Each PCG iteration is two single-pass kernels:
+//   KA(it): rho, ||r||^2 <- partials of KB(it-1); stop test; beta;
+//           p = z + beta p_old (recomputed at the 4 neighbours), q = A' p, p.q
+//   KB(it): p.q <- partials of KA(it); alpha; r = r_old - alpha q (recomputed
+//           at the neighbours); d += alpha p; z = M r; r.z and r.r
+// A thread issues the loads of its cells before it reduces the previous
+// kernel's partials, so the two memory latencies overlap.
+//
+// Reductions without atomics: every kernel writes one FP64 partial per block
+// (blocks own contiguous cell chunks), and every block of the *next* kernel
+// sums all partials in the same fixed order, so all blocks derive the same
+// scalars (alpha, beta, convergence) deterministically and no block waits on
+// another.
 #include "fg_kernels.h"
 
 namespace fg {
@@ -40,24 +46,29 @@ __global__ void __launch_bounds__(kBlock) k_refine_update(const SceneDev* scenes
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active || s.sc->outer_done) return;
-    const int C = s.I * s.J;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= C) return;
-    s.x64[c] = s.x64[c] + (double)s.d[c];
-    s.d[c] = 0.f;
+    if ((int)blockIdx.x >= s.pcg_blocks) return;
+    const Chunk ch = my_chunk(s);
+    for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
+        s.x64[c] = s.x64[c] + (double)s.d[c];
+        s.d[c] = 0.f;
+    }
 }
 
-// r64 = b - A x64 in FP64; inner rhs r = r64/s on fluid rows; outer test.
+// r64 = b - A x64 in FP64 (partials of ||r64||^2); inner rhs r = r64/s on
+// fluid rows.  Also folds the ||b||^2 partials into sc->bb.
 __global__ void __launch_bounds__(kBlock) k_refine_residual(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active || s.sc->outer_done) return;
-    const int J = s.J, C = s.I * J;
-    const int nb = (C + kBlock - 1) / kBlock;
+    const int nb = s.pcg_blocks;
     if ((int)blockIdx.x >= nb) return;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double bb[1];
+    sum_partials<1>(s, kSlotBB, nb, bb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) s.sc->bb = bb[0];
+    const int J = s.J;
+    const Chunk ch = my_chunk(s);
     double rr = 0.0;
-    if (c < C) {
+    for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
         const uint8_t code = s.stencil[c];
         const double* x = s.x64;
         double ax;
@@ -73,208 +84,274 @@ __global__ void __launch_bounds__(kBlock) k_refine_residual(const SceneDev* scen
             ax = x[c];
         }
         const double r64 = s.b[c] - ax;
-        rr = r64 * r64;
-        s.r[c] = (float)((code & 0x80) ? r64 / s.sA : r64);
+        rr += r64 * r64;
+        s.cr[c] = (float)((code & 0x80) ? r64 / s.sA : r64);
     }
     double v[1] = {rr};
-    reduce_last<1>(s, v, 4, nb, [&](const double* t) {
-        SceneScalars* sc = s.sc;
-        sc->rr64 = t[0];
-        const double bn = sqrt(sc->bb), rn = sqrt(t[0]);
-        if (!finite_d(t[0])) {
-            sc->numeric_bad = 1;
+    put_partials<1>(s, kSlotRR64, v);
+}
+
+constexpr int kItems = 4;                  // cells per thread per pass
+constexpr int kPass = kItems * kBlock;     // cells per block per pass
+
+// Outer test on ||r64||; if not converged: z = M r and the partials of r.z,
+// r.r for the first iteration.
+__global__ void __launch_bounds__(kBlock) k_cg_init(const SceneDev* scenes, int sweep)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active) return;
+    SceneScalars* sc = s.sc;
+    if (sc->outer_done) return;
+    const int nb = s.pcg_blocks;
+    if ((int)blockIdx.x >= nb) return;
+    double rr64[1];
+    sum_partials<1>(s, kSlotRR64, nb, rr64);
+    const double bn = sqrt(sc->bb), rn = sqrt(rr64[0]);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (!finite_d(rr64[0]) || rn <= s.tol * bn || bn == 0.0 || sweep >= kMaxSweeps) {
+        if (lead) {
+            sc->rr64 = rr64[0];
+            if (!finite_d(rr64[0])) sc->numeric_bad = 1;
             sc->outer_done = 1;
-            return;
         }
-        if (rn <= s.tol * bn || bn == 0.0 || sc->sweeps >= kMaxSweeps) {
-            sc->outer_done = 1;
-            return;
-        }
-        sc->sweeps += 1;
+        return;
+    }
+    if (lead) {
+        sc->rr64 = rr64[0];
+        sc->sweeps = sweep + 1;
         sc->inner_done = 0;
         sc->inner_iters = 0;
         // ask the FP32 solve for the reduction still needed (with margin)
-        double f = 0.3 * s.tol * bn / rn;
-        sc->inner_eps2 = (f < 1e-6 ? 1e-6 : f) * (f < 1e-6 ? 1e-6 : f);
-    });
-}
-
-// z = M r, p-direction reset (beta = 0), r.z and r.r
-__global__ void __launch_bounds__(kBlock) k_inner_init(const SceneDev* scenes)
-{
-    const SceneDev& s = scenes[blockIdx.y];
-    if (!s.active || s.sc->outer_done) return;
-    const int J = s.J, C = s.I * J;
-    const int nb = (C + kBlock - 1) / kBlock;
-    if ((int)blockIdx.x >= nb) return;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+        const double f = fmax(0.3 * s.tol * bn / rn, 1e-6);
+        sc->inner_eps2 = f * f;
+    }
+    const int J = s.J;
+    const Chunk ch = my_chunk(s);
+    const float* __restrict__ r = s.cr;
+    const uint8_t* __restrict__ code = s.stencil;
     double rz = 0.0, rr = 0.0;
-    if (c < C) {
-        const uint8_t code = s.stencil[c];
-        const float* r = s.r;
+    for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
+        const uint8_t cc = code[c];
         const float rc = r[c];
         float z = rc;
-        if (code & 0x80) {
+        if (cc & 0x80) {
             float diag = 1.0f, off = 0.0f;
-            if (code & 1) {
-                const float w = inv_n(s.stencil[c - J]);
+            if (cc & 1) {
+                const float w = inv_n(code[c - J]);
                 diag += w * w;
                 off += w * r[c - J];
             }
-            if (code & 4) {
-                const float w = inv_n(s.stencil[c - 1]);
+            if (cc & 4) {
+                const float w = inv_n(code[c - 1]);
                 diag += w * w;
                 off += w * r[c - 1];
             }
-            const float wc = inv_n(code);
-            if (code & 2) off += wc * r[c + J];
-            if (code & 8) off += wc * r[c + 1];
+            const float wc = inv_n(cc);
+            if (cc & 2) off += wc * r[c + J];
+            if (cc & 8) off += wc * r[c + 1];
             z = diag * rc + off;
         }
-        s.z[c] = z;
-        rz = (double)rc * (double)z;
-        rr = (double)rc * (double)rc;
+        s.cu[c] = z;
+        rz += (double)rc * (double)z;
+        rr += (double)rc * (double)rc;
     }
     double v[2] = {rz, rr};
-    reduce_last<2>(s, v, 2, nb, [&](const double* t) {
-        SceneScalars* sc = s.sc;
-        sc->rz = t[0];
-        sc->rr0 = t[1];
-        sc->rr = t[1];
-        sc->beta = 0.0;
-        if (!finite_d(t[0]) || t[1] == 0.0) sc->inner_done = 1;
-    });
+    put_partials<2>(s, kSlotCG, v);
 }
 
-// KA: p = z + beta p_old; q = A' p; p.q
-__global__ void __launch_bounds__(kBlock) k_pcg_a(const SceneDev* scenes, int par)
+// KA: p = z + beta p_old; q = A' p; p.q        (z lives in cu)
+__global__ void __launch_bounds__(kBlock) k_pcg_a(const SceneDev* scenes, int it)
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
     SceneScalars* sc = s.sc;
     if (sc->outer_done || sc->inner_done) return;
-    const int J = s.J, C = s.I * J;
-    const int nb = (C + kBlock - 1) / kBlock;
+    const int nb = s.pcg_blocks;
     if ((int)blockIdx.x >= nb) return;
-    const float* pold = par ? s.p2 : s.p;
-    float* pnew = par ? s.p : s.p2;
-    const float beta = (float)sc->beta;
-    const bool first = sc->inner_iters == 0;
-    const float* z = s.z;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const int J = s.J;
+    const bool first = it == 0;
+    const float* __restrict__ z = s.cu;
+    const float* __restrict__ pold = (it & 1) ? s.cp : s.cs1;
+    float* __restrict__ pnew = (it & 1) ? s.cs1 : s.cp;
+    float* __restrict__ q = s.cw;
+    const uint8_t* __restrict__ code = s.stencil;
+    const Chunk ch = my_chunk(s);
     double pq = 0.0;
-    if (c < C) {
-        auto P = [&](int k) { return first ? z[k] : z[k] + beta * pold[k]; };
-        const uint8_t code = s.stencil[c];
-        const float pc = P(c);
-        float q = pc;
-        if (code & 0x80) {
-            const int n = (code >> 4) & 7;
-            q = (float)n * pc;
-            if (code & 1) q -= P(c - J);
-            if (code & 2) q -= P(c + J);
-            if (code & 4) q -= P(c - 1);
-            if (code & 8) q -= P(c + 1);
+    float beta = 0.f;
+    for (int base = ch.begin; base < ch.end; base += kPass) {
+        // loads first (they do not depend on beta)
+        uint8_t cc[kItems];
+        float zc[kItems][5], pc[kItems][5];
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const int c = base + threadIdx.x + k * kBlock;
+            cc[k] = c < ch.end ? code[c] : 0;
+            const int offs[5] = {0, -J, J, -1, 1};
+#pragma unroll
+            for (int e = 0; e < 5; e++) {
+                const bool need = c < ch.end && (e == 0 || ((cc[k] & 0x80) && (cc[k] & (1 << (e - 1)))));
+                zc[k][e] = need ? z[c + offs[e]] : 0.f;
+                pc[k][e] = (need && !first) ? pold[c + offs[e]] : 0.f;
+            }
         }
-        pnew[c] = pc;
-        s.q[c] = q;
-        pq = (double)pc * (double)q;
+        if (base == ch.begin) {
+            double t[2];
+            sum_partials<2>(s, kSlotCG, nb, t);   // rho = r.z, ||r||^2 of the current residual
+            const double rho = t[0], rr = t[1];
+            const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+            const double rr0 = first ? rr : sc->rr0;
+            if (first && lead) sc->rr0 = rr;
+            bool stop = false;
+            if (!finite_d(rho) || !finite_d(rr)) {
+                stop = true;
+                if (lead) {
+                    sc->numeric_bad = 1;
+                    sc->outer_done = 1;
+                }
+            } else if (rr == 0.0 || (!first && rr <= sc->inner_eps2 * rr0) || it >= s.max_inner) {
+                stop = true;
+            }
+            if (stop) {
+                if (lead) {
+                    sc->inner_done = 1;
+                    sc->rr = rr;
+                }
+                return;
+            }
+            if (lead) sc->gamma[it & 1] = rho;
+            beta = first ? 0.f : (float)(rho / sc->gamma[(it - 1) & 1]);
+        }
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const int c = base + threadIdx.x + k * kBlock;
+            if (c >= ch.end) continue;
+            float P[5];
+#pragma unroll
+            for (int e = 0; e < 5; e++) P[e] = first ? zc[k][e] : zc[k][e] + beta * pc[k][e];
+            float qv = P[0];
+            if (cc[k] & 0x80) {
+                qv = (float)((cc[k] >> 4) & 7) * P[0];
+                if (cc[k] & 1) qv -= P[1];
+                if (cc[k] & 2) qv -= P[2];
+                if (cc[k] & 4) qv -= P[3];
+                if (cc[k] & 8) qv -= P[4];
+            }
+            pnew[c] = P[0];
+            q[c] = qv;
+            pq += (double)P[0] * (double)qv;
+        }
     }
     double v[1] = {pq};
-    reduce_last<1>(s, v, 1, nb, [&](const double* t) {
-        sc->pq = t[0];
-        if (!(t[0] > 0.0) || !finite_d(t[0])) {
-            sc->inner_done = 1;   // breakdown: stop this sweep
-            sc->alpha = 0.0;
-            return;
-        }
-        sc->alpha = sc->rz / t[0];
-    });
+    put_partials<1>(s, kSlotPQ, v);
 }
 
 // KB: r = r_old - alpha q; d += alpha p; z = M r; r.z, r.r
-__global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, int par)
+__global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, int it)
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
     SceneScalars* sc = s.sc;
     if (sc->outer_done || sc->inner_done) return;
-    const int J = s.J, C = s.I * J;
-    const int nb = (C + kBlock - 1) / kBlock;
+    const int nb = s.pcg_blocks;
     if ((int)blockIdx.x >= nb) return;
-    const float* rold = par ? s.r2 : s.r;
-    float* rnew = par ? s.r : s.r2;
-    const float* pnew = par ? s.p : s.p2;
-    const float alpha = (float)sc->alpha;
-    const float* q = s.q;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const int J = s.J;
+    const float* __restrict__ rold = (it & 1) ? s.cr2 : s.cr;
+    float* __restrict__ rnew = (it & 1) ? s.cr : s.cr2;
+    const float* __restrict__ pnew = (it & 1) ? s.cs1 : s.cp;
+    const float* __restrict__ q = s.cw;
+    float* __restrict__ z = s.cu;
+    float* __restrict__ d = s.d;
+    const uint8_t* __restrict__ code = s.stencil;
+    const Chunk ch = my_chunk(s);
     double rz = 0.0, rr = 0.0;
-    if (c < C) {
-        auto R = [&](int k) { return rold[k] - alpha * q[k]; };
-        const uint8_t code = s.stencil[c];
-        const float rc = R(c);
-        float z = rc;
-        if (code & 0x80) {
-            float diag = 1.0f, off = 0.0f;
-            if (code & 1) {
-                const float w = inv_n(s.stencil[c - J]);
-                diag += w * w;
-                off += w * R(c - J);
+    float alpha = 0.f;
+    for (int base = ch.begin; base < ch.end; base += kPass) {
+        uint8_t cc[kItems], cl[kItems][2];
+        float rv[kItems][5], qv[kItems][5], pv[kItems], dv[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const int c = base + threadIdx.x + k * kBlock;
+            const bool in = c < ch.end;
+            cc[k] = in ? code[c] : 0;
+            const int offs[5] = {0, -J, J, -1, 1};
+#pragma unroll
+            for (int e = 0; e < 5; e++) {
+                const bool need = in && (e == 0 || ((cc[k] & 0x80) && (cc[k] & (1 << (e - 1)))));
+                rv[k][e] = need ? rold[c + offs[e]] : 0.f;
+                qv[k][e] = need ? q[c + offs[e]] : 0.f;
             }
-            if (code & 4) {
-                const float w = inv_n(s.stencil[c - 1]);
-                diag += w * w;
-                off += w * R(c - 1);
-            }
-            const float wc = inv_n(code);
-            if (code & 2) off += wc * R(c + J);
-            if (code & 8) off += wc * R(c + 1);
-            z = diag * rc + off;
+            cl[k][0] = (in && (cc[k] & 0x81) == 0x81) ? code[c - J] : 0;
+            cl[k][1] = (in && (cc[k] & 0x84) == 0x84) ? code[c - 1] : 0;
+            pv[k] = in ? pnew[c] : 0.f;
+            dv[k] = in ? d[c] : 0.f;
         }
-        rnew[c] = rc;
-        s.z[c] = z;
-        s.d[c] = s.d[c] + alpha * pnew[c];
-        rz = (double)rc * (double)z;
-        rr = (double)rc * (double)rc;
+        if (base == ch.begin) {
+            double t[1];
+            sum_partials<1>(s, kSlotPQ, nb, t);
+            const double pq = t[0];
+            const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+            if (!(pq > 0.0) || !finite_d(pq)) {   // breakdown: end this sweep
+                if (lead) sc->inner_done = 1;
+                return;
+            }
+            alpha = (float)(sc->gamma[it & 1] / pq);
+            if (lead) {
+                sc->inner_iters = it + 1;
+                sc->total_iters += 1;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const int c = base + threadIdx.x + k * kBlock;
+            if (c >= ch.end) continue;
+            float R[5];
+#pragma unroll
+            for (int e = 0; e < 5; e++) R[e] = rv[k][e] - alpha * qv[k][e];
+            float zz = R[0];
+            if (cc[k] & 0x80) {
+                float diag = 1.0f, off = 0.0f;
+                if (cc[k] & 1) {
+                    const float w = inv_n(cl[k][0]);
+                    diag += w * w;
+                    off += w * R[1];
+                }
+                if (cc[k] & 4) {
+                    const float w = inv_n(cl[k][1]);
+                    diag += w * w;
+                    off += w * R[3];
+                }
+                const float wc = inv_n(cc[k]);
+                if (cc[k] & 2) off += wc * R[2];
+                if (cc[k] & 8) off += wc * R[4];
+                zz = diag * R[0] + off;
+            }
+            rnew[c] = R[0];
+            z[c] = zz;
+            d[c] = dv[k] + alpha * pv[k];
+            rz += (double)R[0] * (double)zz;
+            rr += (double)R[0] * (double)R[0];
+        }
     }
     double v[2] = {rz, rr};
-    reduce_last<2>(s, v, 2, nb, [&](const double* t) {
-        sc->inner_iters += 1;
-        sc->total_iters += 1;
-        sc->rr = t[1];
-        if (!finite_d(t[0]) || !finite_d(t[1])) {
-            sc->numeric_bad = 1;
-            sc->inner_done = 1;
-            sc->outer_done = 1;
-            return;
-        }
-        sc->beta = t[0] / sc->rz;
-        sc->rz = t[0];
-        if (t[1] <= sc->inner_eps2 * sc->rr0 || sc->inner_iters >= s.max_inner) sc->inner_done = 1;
-    });
+    put_partials<2>(s, kSlotCG, v);
 }
 
-// ||b||^2 for an externally supplied right-hand side; x = 0, d = 0.
+// ||b||^2 partials for an externally supplied right-hand side; x = 0, d = 0.
 __global__ void __launch_bounds__(kBlock) k_solve_begin(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
-    const int C = s.I * s.J;
-    const int nb = (C + kBlock - 1) / kBlock;
-    if ((int)blockIdx.x >= nb) return;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if ((int)blockIdx.x >= s.pcg_blocks) return;
+    const Chunk ch = my_chunk(s);
     double bb = 0.0;
-    if (c < C) {
+    for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
         const double b = s.b[c];
-        bb = b * b;
+        bb += b * b;
         s.x64[c] = 0.0;
         s.d[c] = 0.f;
     }
     double v[1] = {bb};
-    reduce_last<1>(s, v, 0, nb, [&](const double* t) {
-        s.sc->bb = t[0];
-        s.sc->rr64 = t[0];
-    });
+    put_partials<1>(s, kSlotBB, v);
 }
 
 // reset the per-solve scalars (one thread per scene)
@@ -289,22 +366,20 @@ __global__ void k_solve_reset(const SceneDev* scenes)
     sc->total_iters = 0;
     sc->sweeps = 0;
     sc->numeric_bad = 0;
-    sc->beta = 0.0;
-    sc->alpha = 0.0;
 }
 
 }  // namespace fg
 
 namespace fg {
 #define FG_LAUNCH0(name) \
-    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc) { hipLaunchKernelGGL(k_##name, g, dim3(kBlock), 0, st, sc); }
+    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc) { fg_launch(k_##name, g, dim3(kBlock), st, sc); }
 #define FG_LAUNCH1(name, T, a) \
-    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc, T a) { hipLaunchKernelGGL(k_##name, g, dim3(kBlock), 0, st, sc, a); }
+    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc, T a) { fg_launch(k_##name, g, dim3(kBlock), st, sc, a); }
 FG_LAUNCH0(refine_update)
 FG_LAUNCH0(refine_residual)
-FG_LAUNCH0(inner_init)
-FG_LAUNCH1(pcg_a, int, par)
-FG_LAUNCH1(pcg_b, int, par)
+FG_LAUNCH1(cg_init, int, sweep)
+FG_LAUNCH1(pcg_a, int, it)
+FG_LAUNCH1(pcg_b, int, it)
 FG_LAUNCH0(solve_begin)
 void launch_solve_reset(dim3 g, hipStream_t st, const SceneDev* sc) { hipLaunchKernelGGL(k_solve_reset, g, dim3(64), 0, st, sc); }
 }  // namespace fg

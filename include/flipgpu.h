@@ -143,6 +143,22 @@ int fg_ctx_bytes_in_use(fg_ctx* ctx, uint64_t* out);
 /* Per-stage HIP-event timing for later calls (adds synchronisation). */
 int fg_ctx_set_timing(fg_ctx* ctx, int32_t enabled);
 
+/* Per-launch HIP-event timing of one kernel family, recorded on the stream
+ * the kernel runs on (the context's stream).  kernel = fg_kernel_id, or -1
+ * to switch off.  fg_ctx_profile_read returns the summed event time, the
+ * number of timed launches and the summed work units (cells for cell
+ * kernels, particles for particle kernels) of launches that did work, then
+ * resets the counters. */
+enum fg_kernel_id {
+    FG_K_ADVECT = 0, FG_K_SORT = 1, FG_K_P2G = 2, FG_K_SDF = 3, FG_K_LEVELSET = 4, FG_K_EXTRAPOLATE = 5,
+    FG_K_PCG_A = 6, FG_K_PCG_B = 7, FG_K_G2P = 8, FG_K_RHS = 9, FG_K_APPLY = 10, FG_K_NUM = 11
+};
+int fg_ctx_profile_kernel(fg_ctx* ctx, int32_t kernel);
+int fg_ctx_profile_read(fg_ctx* ctx, double* total_ms, int64_t* launches, int64_t* work_units);
+/* Device-to-device copy bandwidth (read + write bytes / time) of a plain
+ * copy kernel over `bytes` per buffer, best of `reps`. */
+int fg_ctx_bandwidth_probe(fg_ctx* ctx, uint64_t bytes, int32_t reps, double* gbps, double* ms_per_copy);
+
 /* Bytes of device memory a scene with these parameters needs. */
 int fg_scene_bytes(const fg_scene_params* p, uint64_t* out);
 int fg_scene_create(fg_ctx* ctx, const fg_scene_params* p, fg_scene** out);
