@@ -140,6 +140,10 @@ def _declare_oracle(L):
                                           C.c_double, C.POINTER(C.c_double)]),
         "orc_rel_residual": (C.c_double, [_i8p, C.c_int32, C.c_int32, C.c_double, _f64p, _f64p]),
         "orc_split_range": (C.c_int32, [C.c_int64, C.c_int64, C.c_int64, C.c_int32, _i64p, _i64p]),
+        "orc_material_bits": (C.c_int32, [C.c_int8]),
+        "orc_nonsolid_count_lin": (C.c_int32, [_i8p, C.c_int32, C.c_int32, C.c_int64]),
+        "orc_maxabs_replica": (C.c_double, [_f64p, C.c_int64, C.c_int32]),
+        "orc_dot_replica": (C.c_double, [_f64p, _f64p, C.c_int64, C.c_int32]),
         "orc_frame_dt": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32, C.c_int32,
                                      C.POINTER(C.c_int32)]),
         "orc_substep": (C.c_int32, [P, S, C.c_float, T]),

@@ -372,6 +372,35 @@ float orc_polygon_sdf(const float* vt, int32_t N, float px, float py)
     return static_cast<float>(s * std::sqrt(static_cast<double>(sq)));
 }
 
+int32_t orc_material_bits(int8_t m)
+{
+    return (is_fluid(m) ? 1 : 0) | (is_strict_fluid(m) ? 2 : 0) | (is_empty(m) ? 4 : 0) | (is_solid(m) ? 8 : 0) |
+           (is_source(m) ? 16 : 0) | (is_sink(m) ? 32 : 0);
+}
+
+double orc_maxabs_replica(const double* v, int64_t n, int32_t threads)
+{
+    double mx = 0;
+    for (const Range& r : split_range(n, 1, 1, threads)) {
+        double ma = std::numeric_limits<double>::min();
+        for (int64_t i = r.start; i < r.end; i++)
+            if (std::abs(v[i]) > 0.0) ma = std::abs(v[i]);
+        if (ma > mx) mx = std::abs(ma);
+    }
+    return mx;
+}
+
+double orc_dot_replica(const double* a, const double* b, int64_t n, int32_t threads)
+{
+    double res = 0;
+    for (const Range& r : split_range(n, 1, 1, threads)) {
+        double part = 0.0;
+        for (int64_t i = r.start; i < r.end; i++) part += a[i] * b[i];
+        res += part;
+    }
+    return res;
+}
+
 int32_t orc_split_range(int64_t length, int64_t min_size, int64_t jobs, int32_t threads,
                         int64_t* starts, int64_t* ends)
 {
@@ -939,6 +968,11 @@ static inline int nonsolid_count_lin(const int8_t* m, long I, long J, long lin)
     const long i = lin / J, j = lin - i * J;
     return !is_solid(mat_ext(m, I, J, i - 1, j)) + !is_solid(mat_ext(m, I, J, i + 1, j)) +
            !is_solid(mat_ext(m, I, J, i, j - 1)) + !is_solid(mat_ext(m, I, J, i, j + 1));
+}
+
+int32_t orc_nonsolid_count_lin(const int8_t* m, int32_t I, int32_t J, int64_t lin)
+{
+    return nonsolid_count_lin(m, I, J, (long)lin);
 }
 
 /* Reference IPP: FlipSolver::getIPPCoefficients (flipsolver2d.cpp:889-945) +

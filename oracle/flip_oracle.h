@@ -147,6 +147,14 @@ int32_t orc_pcg_converged(const int8_t* material_a, int32_t I, int32_t J, double
                           double* rel_resid_out);
 double orc_rel_residual(const int8_t* material_a, int32_t I, int32_t J, double scale,
                         const double* b, const double* x);
+/* Material predicates as bits: 1 fluid, 2 strict fluid, 4 empty, 8 solid,
+ * 16 source, 32 sink (materialgrid.h:15-20). */
+int32_t orc_material_bits(int8_t code);
+/* MaterialGrid::nonsolidNeighborCount(ssize_t linIdx), materialgrid.cpp:129-140 */
+int32_t orc_nonsolid_count_lin(const int8_t* material, int32_t I, int32_t J, int64_t lin);
+/* VOps::maxAbs / VOps::dot as the replica solver evaluates them (vmath.cpp:25-45, 100-136) */
+double orc_maxabs_replica(const double* v, int64_t n, int32_t threads);
+double orc_dot_replica(const double* a, const double* b, int64_t n, int32_t threads);
 /* splitRange, threading/threadpool.cpp:41-76: fills starts/ends, returns count. */
 int32_t orc_split_range(int64_t length, int64_t min_size, int64_t jobs_per_thread,
                         int32_t threads, int64_t* starts, int64_t* ends);
