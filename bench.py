@@ -37,6 +37,11 @@ KERNEL_BYTES = {
     "pcg_b": ("K_PCG_B", 33.0, "cells"),
     "pcg_a": ("K_PCG_A", 21.0, "cells"),
 }
+# HBM bytes per cell actually moved, at the 1024^2 dam break: rocprofv3
+# FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) per dispatch, the factors being
+# what the same counters read for a 1 GiB copy kernel
+# (profiles/pcg_dam_break_1024_hbm_traffic.md).
+MEASURED_TRAFFIC_PER_CELL = {1024: {"pcg_b": 34.18, "pcg_a": 21.83}}
 
 
 def parse():
@@ -91,8 +96,8 @@ def cpu_baseline(res, ppc, substeps, threads):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
-              "(whole scenes shard across GPUs; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: scenes are independent and "
+              "would shard whole across GPUs, which this library does not do", file=sys.stderr)
         sys.exit(2)
     import flipsolver2d_amd as fg
 
@@ -127,6 +132,8 @@ def main():
     achieved = (kunits * per_unit) / (ktime * 1e-3) / 1e9 if ktime > 0 else 0.0
     bytes_per_launch = kunits * per_unit / max(klaunch, 1)
     copy_gbs, _ = ctx.bandwidth_probe(1 << 30, 5)
+    tpc = MEASURED_TRAFFIC_PER_CELL.get(a.res, {}).get(a.roofline_kernel)
+    traffic = tpc * kunits / max(klaunch, 1) if tpc else None
 
     stage = None
     if a.stage_timing:
@@ -181,7 +188,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "avg_launch_us": avg_ms * 1e3,
             "launches": klaunch,
             "algorithmic_bytes_per_launch": bytes_per_launch,
