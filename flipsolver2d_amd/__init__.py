@@ -306,9 +306,47 @@ def dam_break_geometry(n, fluid_box=(0.3, 1.0, 0.0, 0.4)):
     return walls, fluid
 
 
-def dam_break_scene(ctx, n, ppc=4, seed=42, **kw):
+def dam_break_scene(ctx, n, ppc=4, seed=42, fluid_box=(0.3, 1.0, 0.0, 0.4), **kw):
     p = make_params(n, ppc=ppc, **kw)
     sc = Scene(ctx, p)
-    walls, fluid = dam_break_geometry(n)
+    walls, fluid = dam_break_geometry(n, fluid_box)
     sc.init_geometry(walls, fluid, friction=np.zeros(len(walls), np.float32), seed=seed)
     return sc
+
+
+def batch_step_frame(ctx, scenes):
+    """One FlipSolver::stepFrame for every scene, advanced together (one
+    launch per phase over all scenes still inside their frame)."""
+    n = len(scenes)
+    arr = (C.c_void_p * n)(*[s.h.value for s in scenes])
+    stats = (Stats * n)()
+    _check(lib().fg_batch_step_frame(ctx.h, arr, n, stats))
+    return list(stats)
+
+
+MIXED_RESOLUTIONS = (64, 96, 128, 192, 256)
+
+
+def mixed_scene_specs(count=256, master_seed=2024, resolutions=MIXED_RESOLUTIONS):
+    """Independent dam-break scenes of uneven size: resolution from
+    `resolutions`, fluid-rectangle corners uniform in [0.1, 0.6] of the box on
+    each axis, gravity 9.81 m/s^2 at an angle uniform in +-30 degrees from +x,
+    4-8 particles per cell, one seed per scene drawn from the master seed."""
+    rng = np.random.default_rng(master_seed)
+    specs = []
+    for k in range(count):
+        n = int(rng.choice(resolutions))
+        xs = np.sort(rng.uniform(0.1, 0.6, 2))
+        ys = np.sort(rng.uniform(0.1, 0.6, 2))
+        ang = np.deg2rad(rng.uniform(-30.0, 30.0))
+        ppc = int(rng.integers(4, 9))
+        seed = int(rng.integers(0, 2**31 - 1))
+        specs.append({"n": n, "fluid_box": (float(xs[0]), float(xs[1]), float(ys[0]), float(ys[1])),
+                      "gravity": (float(9.81 * np.cos(ang)), float(9.81 * np.sin(ang))), "ppc": ppc, "seed": seed})
+    return specs
+
+
+def make_scene(ctx, spec, **kw):
+    """A dam-break scene from one `mixed_scene_specs` entry."""
+    return dam_break_scene(ctx, spec["n"], ppc=spec["ppc"], seed=spec["seed"], fluid_box=spec["fluid_box"],
+                           gravity=spec["gravity"], **kw)

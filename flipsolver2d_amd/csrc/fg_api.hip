@@ -65,6 +65,14 @@ struct fg_ctx {
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     int64_t prof_units = 0;
+    // launch groups of several scenes: descriptor array, scalar gather, flags
+    int bcap = 0;
+    SceneDev* bdesc = nullptr;
+    SceneDev* bdesc_host = nullptr;
+    SceneScalars* bsc = nullptr;
+    SceneScalars* bsc_host = nullptr;
+    int32_t* bflags = nullptr;
+    int32_t* bflags_host = nullptr;
 };
 
 struct fg_scene {
@@ -263,6 +271,7 @@ int create_scene(fg_ctx* ctx, const fg_scene_params* p, bool solver_only, fg_sce
     h.pcg_blocks = s->max_blocks;
     h.pcg_chunk = s->pcg_chunk;
     h.max_pblocks = s->max_pblocks;
+    h.pblocks = 1;
     h.cap = s->cap;
     ctx->used += s->bytes;
     hipStream_t st = ctx->stream;
@@ -278,6 +287,13 @@ int create_scene(fg_ctx* ctx, const fg_scene_params* p, bool solver_only, fg_sce
     return FG_OK;
 }
 
+void stats_reset(fg_stats* st)
+{
+    if (!st) return;
+    std::memset(st, 0, sizeof(*st));
+    st->converged = 1;
+}
+
 // --------------------------------------------------------------------------
 // launch geometry
 // --------------------------------------------------------------------------
@@ -285,22 +301,36 @@ struct Geo {
     dim3 cells, parts, scan, scan1, faces, extrap, ls, one, pcg, bins;
 };
 
+// x extents of one scene's launches (y is set by the launch group)
 Geo geometry(const fg_scene* s)
 {
     Geo g;
-    g.cells = dim3((unsigned)cdiv(s->C, kBlock), 1);
-    g.parts = dim3((unsigned)std::max<int64_t>(1, cdiv(std::max<int64_t>(s->n_bound, 1), kBlock)), 1);
-    g.scan = dim3((unsigned)cdiv(s->C, kScanTileCells), 1);
-    g.scan1 = dim3(1, 1);
-    g.faces = dim3((unsigned)cdiv(std::max(s->FU, s->FV), kBlock), 1);
+    g.cells = dim3((unsigned)cdiv(s->C, kBlock));
+    g.parts = dim3((unsigned)std::max<int64_t>(1, cdiv(std::max<int64_t>(s->n_bound, 1), kBlock)));
+    g.scan = dim3((unsigned)cdiv(s->C, kScanTileCells));
+    g.scan1 = dim3(1);
+    g.faces = dim3((unsigned)cdiv(std::max(s->FU, s->FV), kBlock));
     const int64_t tu = cdiv(s->I + 1, kBfsTile) * cdiv(s->J, kBfsTile);
     const int64_t tv = cdiv(s->I, kBfsTile) * cdiv(s->J + 1, kBfsTile);
     g.extrap = dim3((unsigned)std::max(tu, tv), 1, 2);
-    g.ls = dim3((unsigned)(cdiv(s->I, kBfsTile) * cdiv(s->J, kBfsTile)), 1);
-    g.one = dim3(1, 1);
-    g.pcg = dim3((unsigned)s->max_blocks, 1);
-    g.bins = dim3((unsigned)cdiv(cdiv(s->I, 3) * cdiv(s->J, 3), kBlock), 1);
+    g.ls = dim3((unsigned)(cdiv(s->I, kBfsTile) * cdiv(s->J, kBfsTile)));
+    g.one = dim3(1);
+    g.pcg = dim3((unsigned)s->max_blocks);
+    g.bins = dim3((unsigned)cdiv(cdiv(s->I, 3) * cdiv(s->J, 3), kBlock));
     return g;
+}
+
+void geo_max(Geo& a, const Geo& b)
+{
+    dim3* pa[] = {&a.cells, &a.parts, &a.scan, &a.scan1, &a.faces, &a.extrap, &a.ls, &a.one, &a.pcg, &a.bins};
+    const dim3* pb[] = {&b.cells, &b.parts, &b.scan, &b.scan1, &b.faces, &b.extrap, &b.ls, &b.one, &b.pcg, &b.bins};
+    for (int k = 0; k < 10; k++) pa[k]->x = std::max(pa[k]->x, pb[k]->x);
+}
+
+void geo_set_y(Geo& a, int n)
+{
+    dim3* pa[] = {&a.cells, &a.parts, &a.scan, &a.scan1, &a.faces, &a.extrap, &a.ls, &a.one, &a.pcg, &a.bins};
+    for (dim3* d : pa) d->y = (unsigned)n;
 }
 
 void set_substep_scalars(fg_scene* s, float dt)
@@ -329,32 +359,165 @@ __global__ void k_set_count(const SceneDev* scenes, int n)
     if (threadIdx.x != 0) return;
     s.sc->n_particles = n;
 }
+// Copy every scene's scalar block into one array (one host read per group).
+__global__ void k_gather_scalars(const SceneDev* scenes, int n, SceneScalars* out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = *scenes[k].sc;
+}
+// Group-wide flags for the host's polls:
+//   out[0]: some scene's solve has not met its FP64 tolerance
+//   out[1]: some scene's inner PCG is still iterating
+//   out[2]: some scene's level-set band `band` reached its last level
+__global__ void k_group_flags(const SceneDev* scenes, int n, int band, int32_t* out)
+{
+    __shared__ int any[3];
+    if (threadIdx.x < 3) any[threadIdx.x] = 0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const SceneDev& s = scenes[k];
+        if (!s.active) continue;
+        if (band < 0) {
+            const SceneScalars* sc = s.sc;
+            if (!sc->outer_done) {
+                any[0] = 1;
+                if (!sc->inner_done) any[1] = 1;
+            }
+        } else if (s.lsFlags[band]) {
+            any[2] = 1;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) out[threadIdx.x] = any[threadIdx.x];
+}
 
+// ---------------------------------------------------------------------------
+// launch groups: the scenes of one launch, grid.y = scene slot
+// ---------------------------------------------------------------------------
+struct Group {
+    fg_ctx* ctx = nullptr;
+    std::vector<fg_scene*> sc;
+    std::vector<fg_stats*> st;     // per scene, entries may be NULL
+    const SceneDev* d = nullptr;   // device descriptor array [n]
+    int n = 0;
+    Geo g{};
+    int64_t cells = 0, faces = 0, parts = 0;
+    int max_inner = 0;
+    int max_bands = 0;
+};
+
+int ensure_group_buffers(fg_ctx* ctx, int n)
+{
+    if (n <= ctx->bcap) return FG_OK;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->bdesc) hipFree(ctx->bdesc);
+    if (ctx->bdesc_host) hipHostFree(ctx->bdesc_host);
+    if (ctx->bsc) hipFree(ctx->bsc);
+    if (ctx->bsc_host) hipHostFree(ctx->bsc_host);
+    ctx->bdesc = nullptr;
+    ctx->bdesc_host = nullptr;
+    ctx->bsc = nullptr;
+    ctx->bsc_host = nullptr;
+    ctx->bcap = 0;
+    const int cap = std::max(n, 2 * ctx->bcap);
+    HIPCHK(hipMalloc(&ctx->bdesc, sizeof(SceneDev) * cap));
+    HIPCHK(hipHostMalloc((void**)&ctx->bdesc_host, sizeof(SceneDev) * cap, hipHostMallocDefault));
+    HIPCHK(hipMalloc(&ctx->bsc, sizeof(SceneScalars) * cap));
+    HIPCHK(hipHostMalloc((void**)&ctx->bsc_host, sizeof(SceneScalars) * cap, hipHostMallocDefault));
+    ctx->bcap = cap;
+    return FG_OK;
+}
+
+// Build a group and upload its descriptors.  A single scene uses its own
+// device descriptor; several share the context's descriptor array.
+int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vector<fg_stats*>& stats, Group& G)
+{
+    G = Group{};
+    G.ctx = ctx;
+    G.sc = scenes;
+    G.st = stats;
+    G.st.resize(scenes.size(), nullptr);
+    G.n = (int)scenes.size();
+    if (G.n == 0) return FG_OK;
+    // the pinned staging buffers may still feed an earlier copy
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    bool first = true;
+    for (fg_scene* s : scenes) {
+        s->h.active = 1;
+        s->h.pblocks = (int)std::max<int64_t>(1, cdiv(std::max<int64_t>(s->n_bound, 1), kBlock));
+        const Geo gs = geometry(s);
+        if (first) G.g = gs;
+        else geo_max(G.g, gs);
+        first = false;
+        G.cells += s->C;
+        G.faces += s->FU + s->FV;
+        G.parts += s->n_bound;
+        G.max_inner = std::max(G.max_inner, s->h.max_inner);
+        G.max_bands = std::max(G.max_bands, (int)std::min<int64_t>(kMaxBands, cdiv(std::max(s->I, s->J), 16) + 1));
+    }
+    geo_set_y(G.g, G.n);
+    if (G.n == 1) {
+        fg_scene* s = scenes[0];
+        *s->hpin = s->h;
+        HIPCHK(hipMemcpyAsync(s->d, s->hpin, sizeof(SceneDev), hipMemcpyHostToDevice, ctx->stream));
+        G.d = s->d;
+        return FG_OK;
+    }
+    FGCHK(ensure_group_buffers(ctx, G.n));
+    for (int k = 0; k < G.n; k++) ctx->bdesc_host[k] = scenes[k]->h;
+    HIPCHK(hipMemcpyAsync(ctx->bdesc, ctx->bdesc_host, sizeof(SceneDev) * G.n, hipMemcpyHostToDevice, ctx->stream));
+    G.d = ctx->bdesc;
+    return FG_OK;
+}
+
+// Read every scene's scalar block into its pinned host mirror (synchronises).
+int read_group_scalars(Group& G)
+{
+    fg_ctx* ctx = G.ctx;
+    if (G.n == 1) return read_scalars(G.sc[0]);
+    hipLaunchKernelGGL(k_gather_scalars, dim3((unsigned)cdiv(G.n, 64)), dim3(64), 0, ctx->stream, G.d, G.n, ctx->bsc);
+    HIPCHK(hipMemcpyAsync(ctx->bsc_host, ctx->bsc, sizeof(SceneScalars) * G.n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < G.n; k++) *G.sc[k]->hsc = ctx->bsc_host[k];
+    return FG_OK;
+}
+
+int poll_group(Group& G, int band, int32_t* flags)
+{
+    fg_ctx* ctx = G.ctx;
+    hipLaunchKernelGGL(k_group_flags, dim3(1), dim3(256), 0, ctx->stream, G.d, G.n, band, ctx->bflags);
+    HIPCHK(hipMemcpyAsync(ctx->bflags_host, ctx->bflags, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < 3; k++) flags[k] = ctx->bflags_host[k];
+    return FG_OK;
+}
 
 // ---------------------------------------------------------------------------
 // stage timing
 // ---------------------------------------------------------------------------
 struct Timer {
     fg_ctx* ctx;
-    fg_stats* st;
+    Group* G;
     hipEvent_t a = nullptr;
-    Timer(fg_ctx* c, fg_stats* s) : ctx(c), st(s)
+    Timer(fg_ctx* c, Group* g) : ctx(c), G(g)
     {
-        if (ctx->timing && st) {
+        if (ctx->timing) {
             hipEventCreate(&a);
             hipEventRecord(a, ctx->stream);
         }
     }
+    // the group's time for the stage is charged to every scene in it
     void lap(int stage)
     {
-        if (!(ctx->timing && st)) return;
+        if (!ctx->timing) return;
         hipEvent_t b;
         hipEventCreate(&b);
         hipEventRecord(b, ctx->stream);
         hipEventSynchronize(b);
         float ms = 0;
         hipEventElapsedTime(&ms, a, b);
-        st->stage_ms[stage] += ms;
+        for (fg_stats* st : G->st)
+            if (st) st->stage_ms[stage] += ms;
         hipEventDestroy(a);
         a = b;
     }
@@ -398,136 +561,144 @@ struct KProf {
 };
 
 // ---------------------------------------------------------------------------
-// the solve (shared by project and fg_poisson_solve); b is in place
+// the solve (shared by project and fg_poisson_solve); b is in place.  All
+// scenes of the group run the same sweep and iteration index in lock step;
+// a scene whose solve has stopped skips the launches that remain.
 // ---------------------------------------------------------------------------
-int run_solve(fg_scene* s, bool rhs_from_grid, int32_t* iters_out, int32_t* sweeps_out)
+int run_solve(Group& G, bool rhs_from_grid)
 {
-    hipStream_t st = s->ctx->stream;
-    const Geo g = geometry(s);
-    launch_solve_reset(g.one, st, s->d);
+    fg_ctx* ctx = G.ctx;
+    hipStream_t st = ctx->stream;
+    const Geo& g = G.g;
+    launch_solve_reset(g.one, st, G.d);
     if (rhs_from_grid) {
-        KProf kp(s->ctx, FG_K_RHS, true);
-        launch_rhs(g.pcg, st, s->d);
-        kp.done(s->C);
+        KProf kp(ctx, FG_K_RHS, true);
+        launch_rhs(g.pcg, st, G.d);
+        kp.done(G.cells);
     } else {
-        launch_solve_begin(g.pcg, st, s->d);
+        launch_solve_begin(g.pcg, st, G.d);
     }
     constexpr int kPoll = 16;
+    int32_t flags[3];
+    int launched = 0;
     for (int sweep = 0; sweep <= kMaxSweeps; sweep++) {
-        if (sweep > 0) launch_refine_update(g.pcg, st, s->d);
-        launch_refine_residual(g.pcg, st, s->d);
-        launch_cg_init(g.pcg, st, s->d, sweep);
-        FGCHK(read_scalars(s));
-        if (s->hsc->outer_done) break;
+        if (sweep > 0) launch_refine_update(g.pcg, st, G.d);
+        launch_refine_residual(g.pcg, st, G.d);
+        launch_cg_init(g.pcg, st, G.d, sweep);
+        FGCHK(poll_group(G, -1, flags));
+        if (!flags[0]) break;
         int it = 0;
         for (;;) {
-            const int before = s->hsc->total_iters;
             for (int k = 0; k < kPoll; k++, it++) {
                 {
-                    KProf kp(s->ctx, FG_K_PCG_A, true);
-                    launch_pcg_a(g.pcg, st, s->d, it);
+                    KProf kp(ctx, FG_K_PCG_A, true);
+                    launch_pcg_a(g.pcg, st, G.d, it);
                     kp.done(0);
                 }
-                KProf kp(s->ctx, FG_K_PCG_B, true);
-                launch_pcg_b(g.pcg, st, s->d, it);
+                KProf kp(ctx, FG_K_PCG_B, true);
+                launch_pcg_b(g.pcg, st, G.d, it);
                 kp.done(0);
+                launched++;
             }
-            FGCHK(read_scalars(s));
-            // work units of the launches that iterated (the rest exited early)
-            if (s->ctx->prof_kernel == FG_K_PCG_A || s->ctx->prof_kernel == FG_K_PCG_B) s->ctx->prof_units += (int64_t)(s->hsc->total_iters - before) * s->C;
-            if (s->hsc->inner_done || s->hsc->outer_done) break;
-            if (it > s->h.max_inner + kPoll) return fail(FG_E_STATE, "inner solve did not stop at its cap");
+            FGCHK(poll_group(G, -1, flags));
+            if (!flags[1]) break;
+            if (it > G.max_inner + kPoll) return fail(FG_E_STATE, "inner solve did not stop at its cap");
         }
     }
-    if (iters_out) *iters_out = s->hsc->total_iters;
-    if (sweeps_out) *sweeps_out = s->hsc->sweeps;
-    if (s->hsc->numeric_bad) return fail(FG_E_NUMERIC, "NaN/Inf in the pressure solve");
+    FGCHK(read_group_scalars(G));
+    // work units of the launches that iterated (the rest exited at once)
+    if (ctx->prof_kernel == FG_K_PCG_A || ctx->prof_kernel == FG_K_PCG_B)
+        for (fg_scene* s : G.sc) ctx->prof_units += (int64_t)s->hsc->total_iters * s->C;
+    for (fg_scene* s : G.sc)
+        if (s->hsc->numeric_bad) return fail(FG_E_NUMERIC, "NaN/Inf in the pressure solve");
+    (void)launched;
     return FG_OK;
 }
 
 // ---------------------------------------------------------------------------
 // stages of FlipSolver::step
 // ---------------------------------------------------------------------------
-int run_stage(fg_scene* s, int stage, fg_stats* stats)
+int run_stage(Group& G, int stage)
 {
-    hipStream_t st = s->ctx->stream;
-    const Geo g = geometry(s);
-    const dim3& one = g.one;
+    fg_ctx* ctx = G.ctx;
+    hipStream_t st = ctx->stream;
+    const Geo& g = G.g;
+    const SceneDev* d = G.d;
     switch (stage) {
     case FG_STAGE_ADVECT: {
-        KProf kp(s->ctx, FG_K_ADVECT, true);
-        launch_advect(g.parts, st, s->d, 1);
-        kp.done(s->n_bound);
+        KProf kp(ctx, FG_K_ADVECT, true);
+        launch_advect(g.parts, st, d, 1);
+        kp.done(G.parts);
         break;
     }
     case FG_STAGE_REBIN: {
-        launch_stencil(g.cells, st, s->d);          // matrix from the pre-update materials
-        KProf kp(s->ctx, FG_K_SORT);
-        launch_scan_local(g.scan, st, s->d);
-        launch_scan_top(g.scan1, st, s->d);
-        launch_scan_add(g.scan, st, s->d);
-        launch_scatter(g.parts, st, s->d);
-        launch_fixup(g.cells, st, s->d);
-        launch_gather(g.parts, st, s->d);
-        launch_finalize(one, st, s->d, (int)g.parts.x, 1);
-        kp.done(s->n_bound);
+        launch_stencil(g.cells, st, d);          // matrix from the pre-update materials
+        KProf kp(ctx, FG_K_SORT);
+        launch_scan_local(g.scan, st, d);
+        launch_scan_top(g.scan1, st, d);
+        launch_scan_add(g.scan, st, d);
+        launch_scatter(g.parts, st, d);
+        launch_fixup(g.cells, st, d);
+        launch_gather(g.parts, st, d);
+        launch_finalize(g.one, st, d, 1);
+        kp.done(G.parts);
         break;
     }
     case FG_STAGE_DENSITY:
-        if (s->p.density_mode != FG_DENSITY_OFF)
-            return fail(FG_E_STATE, "density correction modes other than OFF are not built yet");
+        for (fg_scene* s : G.sc)
+            if (s->p.density_mode != FG_DENSITY_OFF)
+                return fail(FG_E_STATE, "density correction modes other than OFF are not built yet");
         break;
     case FG_STAGE_P2G: {
-        KProf kp(s->ctx, FG_K_P2G, true);
-        launch_p2g(g.cells, st, s->d);
-        kp.done(s->C);
+        KProf kp(ctx, FG_K_P2G, true);
+        launch_p2g(g.cells, st, d);
+        kp.done(G.cells);
         break;
     }
     case FG_STAGE_SDF: {
-        KProf kp(s->ctx, FG_K_SDF, true);
-        launch_sdf_materials(g.bins, st, s->d);
-        kp.done(s->C);
+        KProf kp(ctx, FG_K_SDF, true);
+        launch_sdf_materials(g.bins, st, d);
+        kp.done(G.cells);
         break;
     }
     case FG_STAGE_LEVELSET: {
-        KProf kp(s->ctx, FG_K_LEVELSET);
-        launch_levelset_init(g.cells, st, s->d);
-        const int max_bands = (int)std::min<int64_t>(kMaxBands, cdiv(std::max(s->I, s->J), 16) + 1);
+        KProf kp(ctx, FG_K_LEVELSET);
+        launch_levelset_init(g.cells, st, d);
         constexpr int kChunk = 8;
-        for (int band = 0; band < max_bands; band += kChunk) {
-            for (int k = band; k < std::min(max_bands, band + kChunk); k++) launch_levelset_band(g.ls, st, s->d, k);
-            int32_t flag = 0;
-            HIPCHK(hipMemcpyAsync(&flag, s->h.lsFlags + std::min(max_bands, band + kChunk) - 1, sizeof(int32_t),
-                                  hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            if (!flag) break;
+        int32_t flags[3];
+        for (int band = 0; band < G.max_bands; band += kChunk) {
+            const int end = std::min(G.max_bands, band + kChunk);
+            for (int k = band; k < end; k++) launch_levelset_band(g.ls, st, d, k);
+            FGCHK(poll_group(G, end - 1, flags));
+            if (!flags[2]) break;
         }
-        kp.done(s->C);
+        kp.done(G.cells);
         break;
     }
     case FG_STAGE_EXTRAPOLATE: {
-        KProf kp(s->ctx, FG_K_EXTRAPOLATE);
-        launch_extrapolate(g.extrap, st, s->d, 0);  // u -> su (saved)
-        kp.done(s->FU + s->FV);
+        KProf kp(ctx, FG_K_EXTRAPOLATE);
+        launch_extrapolate(g.extrap, st, d, 0);  // u -> su (saved)
+        kp.done(G.faces);
         break;
     }
     case FG_STAGE_FORCES:
-        launch_forces(g.faces, st, s->d);
+        launch_forces(g.faces, st, d);
         break;
     case FG_STAGE_PROJECT: {
-        int32_t it = 0, sw = 0;
-        const int r = run_solve(s, true, &it, &sw);
-        if (stats) {
-            stats->pressure_iters = std::max(stats->pressure_iters, it);
-            stats->pressure_sweeps = std::max(stats->pressure_sweeps, sw);
-            const double bb = s->hsc->bb;
-            stats->pressure_rel_resid = bb > 0 ? std::sqrt(s->hsc->rr64 / bb) : 0.0;
-            if (stats->pressure_rel_resid > s->h.tol) stats->converged = 0;
+        const int r = run_solve(G, true);
+        for (int k = 0; k < G.n; k++) {
+            fg_stats* stats = G.st[k];
+            if (!stats) continue;
+            const SceneScalars& sc = *G.sc[k]->hsc;
+            stats->pressure_iters = std::max(stats->pressure_iters, sc.total_iters);
+            stats->pressure_sweeps = std::max(stats->pressure_sweeps, sc.sweeps);
+            stats->pressure_rel_resid = sc.bb > 0 ? std::sqrt(sc.rr64 / sc.bb) : 0.0;
+            if (stats->pressure_rel_resid > G.sc[k]->h.tol) stats->converged = 0;
         }
         FGCHK(r);
-        KProf kp(s->ctx, FG_K_APPLY, true);
-        launch_apply_solids(g.cells, st, s->d);
-        kp.done(s->C);
+        KProf kp(ctx, FG_K_APPLY, true);
+        launch_apply_solids(g.cells, st, d);
+        kp.done(G.cells);
         break;
     }
     case FG_STAGE_SOLIDS:
@@ -535,14 +706,14 @@ int run_stage(fg_scene* s, int stage, fg_stats* stats)
         break;
     case FG_STAGE_G2P: {
         {
-            KProf kp(s->ctx, FG_K_EXTRAPOLATE);
-            launch_extrapolate(g.extrap, st, s->d, 1);  // u -> uf (final)
-            kp.done(s->FU + s->FV);
+            KProf kp(ctx, FG_K_EXTRAPOLATE);
+            launch_extrapolate(g.extrap, st, d, 1);  // u -> uf (final)
+            kp.done(G.faces);
         }
-        KProf kp(s->ctx, FG_K_G2P, true);
-        launch_g2p_count(g.parts, st, s->d);
-        kp.done(s->n_bound);
-        launch_finalize(one, st, s->d, (int)g.parts.x, 2);
+        KProf kp(ctx, FG_K_G2P, true);
+        launch_g2p_count(g.parts, st, d);
+        kp.done(G.parts);
+        launch_finalize(g.one, st, d, 2);
         break;
     }
     case FG_STAGE_COUNT:
@@ -555,55 +726,125 @@ int run_stage(fg_scene* s, int stage, fg_stats* stats)
     return FG_OK;
 }
 
-int substep(fg_scene* s, float dt, fg_stats* stats)
+// After a substep's last stage: commit the counts the kernels produced.
+void commit_substep(Group& G, float* dts)
 {
-    hipStream_t st = s->ctx->stream;
-    set_substep_scalars(s, dt);
-    FGCHK(upload_desc(s));
-    hipLaunchKernelGGL(k_substep_begin, dim3(1), dim3(64), 0, st, s->d);
-    Timer tm(s->ctx, stats);
+    for (int k = 0; k < G.n; k++) {
+        fg_scene* s = G.sc[k];
+        const SceneScalars& sc = *s->hsc;
+        const int64_t n_start = s->n_live;
+        s->n_live = sc.n_particles - sc.erased;
+        s->n_bound = sc.n_particles;
+        s->vmax_ready = true;
+        if (fg_stats* stats = G.st[k]) {
+            stats->substeps += 1;
+            stats->particle_substeps += n_start;
+            stats->particles = s->n_live;
+            stats->particles_killed += sc.killed;
+            stats->particles_erased += sc.erased;
+            stats->last_dt = dts[k];
+        }
+    }
+}
+
+// One FlipSolver::step for every scene of `scenes`, each with its own dt.
+int substep_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vector<float>& dts,
+                  const std::vector<fg_stats*>& stats)
+{
+    for (size_t k = 0; k < scenes.size(); k++) set_substep_scalars(scenes[k], dts[k]);
+    Group G;
+    FGCHK(make_group(ctx, scenes, stats, G));
+    if (G.n == 0) return FG_OK;
+    hipLaunchKernelGGL(k_substep_begin, G.g.one, dim3(64), 0, ctx->stream, G.d);
+    Timer tm(ctx, &G);
     static const int order[] = {FG_STAGE_ADVECT, FG_STAGE_REBIN, FG_STAGE_DENSITY, FG_STAGE_P2G,
                                 FG_STAGE_SDF, FG_STAGE_LEVELSET, FG_STAGE_EXTRAPOLATE, FG_STAGE_FORCES,
                                 FG_STAGE_PROJECT, FG_STAGE_SOLIDS, FG_STAGE_G2P, FG_STAGE_COUNT};
-    const int64_t n_start = s->n_live;
     for (int stage : order) {
-        FGCHK(run_stage(s, stage, stats));
+        FGCHK(run_stage(G, stage));
         tm.lap(stage);
     }
     HIPCHK(hipGetLastError());
-    FGCHK(read_scalars(s));
-    const SceneScalars& sc = *s->hsc;
-    s->n_live = sc.n_particles - sc.erased;
-    s->n_bound = sc.n_particles;
-    s->vmax_ready = true;
-    if (stats) {
-        stats->substeps += 1;
-        stats->particle_substeps += n_start;
-        stats->particles = s->n_live;
-        stats->particles_killed += sc.killed;
-        stats->particles_erased += sc.erased;
-        stats->last_dt = dt;
+    FGCHK(read_group_scalars(G));
+    std::vector<float> d(dts);
+    commit_substep(G, d.data());
+    return FG_OK;
+}
+
+// max |v|^2 for scenes whose last substep did not leave it on the device
+int ensure_vmax(fg_ctx* ctx, const std::vector<fg_scene*>& scenes)
+{
+    std::vector<fg_scene*> need;
+    for (fg_scene* s : scenes)
+        if (!s->vmax_ready) need.push_back(s);
+    if (need.empty()) return FG_OK;
+    Group G;
+    FGCHK(make_group(ctx, need, {}, G));
+    launch_max_v2(G.g.parts, ctx->stream, G.d);
+    launch_finalize(G.g.one, ctx->stream, G.d, 2);
+    FGCHK(read_group_scalars(G));
+    for (fg_scene* s : need) s->vmax_ready = true;
+    return FG_OK;
+}
+
+// FlipSolver::stepFrame (flipsolver2d.cpp:464-500) for every scene: each
+// scene splits its frame by its own CFL dt; the scenes still inside their
+// frame advance together, one launch per phase.
+int step_frames(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vector<fg_stats*>& stats)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t n = scenes.size();
+    std::vector<float> t(n, 0.f);
+    std::vector<int> count(n, 0);
+    std::vector<char> finished(n, 0);
+    for (size_t k = 0; k < n; k++) stats_reset(stats[k]);
+    for (;;) {
+        std::vector<fg_scene*> act;
+        std::vector<fg_stats*> ast;
+        std::vector<size_t> idx;
+        for (size_t k = 0; k < n; k++)
+            if (!finished[k]) {
+                act.push_back(scenes[k]);
+                ast.push_back(stats[k]);
+                idx.push_back(k);
+            }
+        if (act.empty()) break;
+        FGCHK(ensure_vmax(ctx, act));
+        std::vector<float> dts(act.size());
+        for (size_t a = 0; a < act.size(); a++) {
+            const size_t k = idx[a];
+            fg_scene* s = act[a];
+            // FlipSolver::maxParticleVelocity + the substep split, flipsolver2d.cpp:476-490
+            const float frame_dt = 1.f / s->p.fps;
+            float v2;
+            std::memcpy(&v2, &s->hsc->max_v2_bits, 4);
+            const float vel = std::sqrt(v2);
+            float mss = s->p.cfl_number / (vel + 1e-15f);
+            if (t[k] + mss >= frame_dt || count[k] == (s->p.max_substeps - 1)) {
+                mss = frame_dt - t[k];
+                finished[k] = 1;
+            } else if (t[k] + 2.f * mss >= frame_dt) {
+                mss = 0.5f * (frame_dt - t[k]);
+            }
+            if (stats[k]) stats[k]->max_velocity = vel;
+            dts[a] = mss;
+        }
+        FGCHK(substep_group(ctx, act, dts, ast));
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (size_t a = 0; a < act.size(); a++) {
+            const size_t k = idx[a];
+            t[k] += dts[a];
+            count[k]++;
+            if (count[k] > 50) finished[k] = 1;
+            if (finished[k]) {
+                act[a]->frame++;
+                if (stats[k]) stats[k]->frame_ms = ms;
+            }
+        }
     }
     return FG_OK;
 }
 
-int ensure_vmax(fg_scene* s)
-{
-    if (s->vmax_ready) return FG_OK;
-    const Geo g = geometry(s);
-    launch_max_v2(g.parts, s->ctx->stream, s->d);
-    launch_finalize(g.one, s->ctx->stream, s->d, (int)g.parts.x, 2);
-    FGCHK(read_scalars(s));
-    s->vmax_ready = true;
-    return FG_OK;
-}
-
-void stats_reset(fg_stats* st)
-{
-    if (!st) return;
-    std::memset(st, 0, sizeof(*st));
-    st->converged = 1;
-}
 
 // counter-based uniform in [0,1): 24 bits of a murmur3-finalised hash
 inline uint32_t fmix32(uint32_t h)
@@ -666,9 +907,11 @@ int fg_ctx_create(int32_t device, uint64_t hbm_budget_bytes, fg_ctx** out)
     c->device = device;
     c->budget = hbm_budget_bytes;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&c->bflags, sizeof(int32_t) * 4);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->bflags_host, sizeof(int32_t) * 4, hipHostMallocDefault);
     if (e != hipSuccess) {
-        delete c;
-        return fail(FG_E_DEVICE, std::string("stream: ") + hipGetErrorString(e));
+        fg_ctx_destroy(c);
+        return fail(FG_E_DEVICE, std::string("context: ") + hipGetErrorString(e));
     }
     *out = c;
     return FG_OK;
@@ -677,8 +920,17 @@ int fg_ctx_create(int32_t device, uint64_t hbm_budget_bytes, fg_ctx** out)
 int fg_ctx_destroy(fg_ctx* ctx)
 {
     if (!ctx) return FG_OK;
-    hipStreamSynchronize(ctx->stream);
-    hipStreamDestroy(ctx->stream);
+    if (ctx->stream) {
+        hipStreamSynchronize(ctx->stream);
+        hipStreamDestroy(ctx->stream);
+    }
+    for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
+    if (ctx->bdesc) hipFree(ctx->bdesc);
+    if (ctx->bdesc_host) hipHostFree(ctx->bdesc_host);
+    if (ctx->bsc) hipFree(ctx->bsc);
+    if (ctx->bsc_host) hipHostFree(ctx->bsc_host);
+    if (ctx->bflags) hipFree(ctx->bflags);
+    if (ctx->bflags_host) hipHostFree(ctx->bflags_host);
     delete ctx;
     return FG_OK;
 }
@@ -1001,42 +1253,14 @@ int fg_scene_substep(fg_scene* s, float dt, fg_stats* stats)
     if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
     if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FG_E_INVALID, "dt must be positive and finite");
     stats_reset(stats);
-    return substep(s, dt, stats);
+    return substep_group(s->ctx, {s}, {dt}, {stats});
 }
 
 int fg_scene_step_frame(fg_scene* s, fg_stats* stats)
 {
     if (!s) return fail(FG_E_INVALID, "scene is NULL");
     if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
-    stats_reset(stats);
-    const auto t0 = std::chrono::steady_clock::now();
-    const float frame_dt = 1.f / s->p.fps;
-    float t = 0.f;
-    int count = 0;
-    bool finished = false;
-    while (!finished) {
-        FGCHK(ensure_vmax(s));
-        // FlipSolver::maxParticleVelocity + the substep split, flipsolver2d.cpp:476-490
-        float v2;
-        std::memcpy(&v2, &s->hsc->max_v2_bits, 4);
-        const float vel = std::sqrt(v2);
-        float mss = s->p.cfl_number / (vel + 1e-15f);
-        if (t + mss >= frame_dt || count == (s->p.max_substeps - 1)) {
-            mss = frame_dt - t;
-            finished = true;
-        } else if (t + 2.f * mss >= frame_dt) {
-            mss = 0.5f * (frame_dt - t);
-        }
-        if (stats) stats->max_velocity = vel;
-        FGCHK(substep(s, mss, stats));
-        t += mss;
-        count++;
-        if (count > 50) break;
-    }
-    s->frame++;
-    if (stats)
-        stats->frame_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return FG_OK;
+    return step_frames(s->ctx, {s}, {stats});
 }
 
 int fg_scene_stage(fg_scene* s, int32_t stage, float dt)
@@ -1047,12 +1271,13 @@ int fg_scene_stage(fg_scene* s, int32_t stage, float dt)
     if (stage == FG_STAGE_ADVECT) {
         if (!(dt > 0.f)) return fail(FG_E_INVALID, "dt must be positive");
         set_substep_scalars(s, dt);
-        FGCHK(upload_desc(s));
-        hipLaunchKernelGGL(k_substep_begin, dim3(1), dim3(64), 0, s->ctx->stream, s->d);
     }
     fg_stats tmp;
     stats_reset(&tmp);
-    FGCHK(run_stage(s, stage, &tmp));
+    Group G;
+    FGCHK(make_group(s->ctx, {s}, {&tmp}, G));
+    if (stage == FG_STAGE_ADVECT) hipLaunchKernelGGL(k_substep_begin, G.g.one, dim3(64), 0, s->ctx->stream, G.d);
+    FGCHK(run_stage(G, stage));
     HIPCHK(hipGetLastError());
     FGCHK(read_scalars(s));
     if (stage == FG_STAGE_G2P) {
@@ -1065,9 +1290,19 @@ int fg_scene_stage(fg_scene* s, int32_t stage, float dt)
 
 int fg_batch_step_frame(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, fg_stats* stats)
 {
-    if (!ctx || (n > 0 && !scenes)) return fail(FG_E_INVALID, "NULL argument");
-    for (int32_t k = 0; k < n; k++) FGCHK(fg_scene_step_frame(scenes[k], stats ? &stats[k] : nullptr));
-    return FG_OK;
+    if (!ctx || n < 0 || (n > 0 && !scenes)) return fail(FG_E_INVALID, "NULL argument");
+    std::vector<fg_scene*> v(scenes, scenes + n);
+    std::vector<fg_stats*> st(n, nullptr);
+    for (int32_t k = 0; k < n; k++) {
+        fg_scene* s = scenes[k];
+        if (!s) return fail(FG_E_INVALID, "scene is NULL");
+        if (s->ctx != ctx) return fail(FG_E_INVALID, "scene belongs to another context");
+        if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
+        for (int32_t q = 0; q < k; q++)
+            if (scenes[q] == s) return fail(FG_E_INVALID, "a scene appears twice in the batch");
+        if (stats) st[k] = &stats[k];
+    }
+    return step_frames(ctx, v, st);
 }
 
 int fg_poisson_solve(fg_ctx* ctx, int32_t size_i, int32_t size_j, const int8_t* material, double scale,
@@ -1097,9 +1332,14 @@ int fg_poisson_solve(fg_ctx* ctx, int32_t size_i, int32_t size_j, const int8_t* 
     }
     int32_t it = 0, sw = 0;
     if (rc == FG_OK) {
-        const Geo g = geometry(s);
-        launch_stencil(g.cells, q, s->d);
-        rc = run_solve(s, false, &it, &sw);
+        Group G;
+        rc = make_group(ctx, {s}, {}, G);
+        if (rc == FG_OK) {
+            launch_stencil(G.g.cells, q, G.d);
+            rc = run_solve(G, false);
+            it = s->hsc->total_iters;
+            sw = s->hsc->sweeps;
+        }
     }
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (rc == FG_OK || rc == FG_E_NUMERIC) {

@@ -69,6 +69,7 @@ struct SceneDev {
     int32_t pcg_blocks;     // solver blocks of this scene
     int32_t pcg_chunk;      // cells per solver block (multiple of kBlock)
     int32_t max_pblocks;    // stride of the particle partials arrays
+    int32_t pblocks;        // particle blocks of this scene's current launches (<= max_pblocks)
     // per-substep
     float dt;
     float force;            // (float)(dt / dx), applyBodyForces

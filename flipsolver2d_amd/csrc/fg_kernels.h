@@ -41,7 +41,7 @@ FG_DECLARE_LAUNCH(scatter);
 FG_DECLARE_LAUNCH(fixup);
 FG_DECLARE_LAUNCH(gather);
 FG_DECLARE_LAUNCH(g2p_count);
-FG_DECLARE_LAUNCH(finalize, int nb, int what);
+FG_DECLARE_LAUNCH(finalize, int what);
 constexpr int kScanTileCells = kBlock * 16;
 
 // fg_grid.hip

@@ -42,7 +42,7 @@ __device__ __forceinline__ Run key_run(uint32_t key)
 __global__ void __launch_bounds__(kBlock) k_max_v2(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
-    if (!s.active) return;
+    if (!s.active || (int)blockIdx.x >= s.pblocks) return;
     __shared__ float red[kBlock / 64];
     const int n = s.sc->n_particles;
     const int k = blockIdx.x * kBlock + threadIdx.x;
@@ -115,7 +115,7 @@ __device__ float2 closest_surface_point(const float* sdf, int I, int J, float2 p
 __global__ void __launch_bounds__(kBlock) k_advect(const SceneDev* scenes, int alive_from_key)
 {
     const SceneDev& s = scenes[blockIdx.y];
-    if (!s.active) return;
+    if (!s.active || (int)blockIdx.x >= s.pblocks) return;
     __shared__ int red[kBlock / 64];
     const int n = s.sc->n_particles;
     const int I = s.I, J = s.J;
@@ -329,15 +329,16 @@ __global__ void __launch_bounds__(kBlock) k_gather(const SceneDev* scenes)
 }
 
 // ---------------------------------------------------------------------------
-// One-block reduction of the per-block particle partials.
+// One-block reduction of the s.pblocks per-block particle partials.
 //   what & 1: commit the sorted count, sum the advect kills
 //   what & 2: max |v|^2 (flipsolver2d.cpp:1562 starts from FLT_MIN) and
 //             the countParticles erasures
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_finalize(const SceneDev* scenes, int nb, int what)
+__global__ void __launch_bounds__(kBlock) k_finalize(const SceneDev* scenes, int what)
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
+    const int nb = s.pblocks;
     __shared__ int redi[kBlock / 64];
     __shared__ float redf[kBlock / 64];
     int kills = 0, erased = 0;
@@ -374,7 +375,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const SceneDev* scenes, int
 __global__ void __launch_bounds__(kBlock) k_g2p_count(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
-    if (!s.active) return;
+    if (!s.active || (int)blockIdx.x >= s.pblocks) return;
     __shared__ float redf[kBlock / 64];
     __shared__ int redi[kBlock / 64];
     const int n = s.sc->n_particles;
@@ -428,8 +429,8 @@ FG_LAUNCH0(scatter)
 FG_LAUNCH0(fixup)
 FG_LAUNCH0(gather)
 FG_LAUNCH0(g2p_count)
-void launch_finalize(dim3 g, hipStream_t st, const SceneDev* sc, int nb, int what)
+void launch_finalize(dim3 g, hipStream_t st, const SceneDev* sc, int what)
 {
-    hipLaunchKernelGGL(k_finalize, g, dim3(kBlock), 0, st, sc, nb, what);
+    hipLaunchKernelGGL(k_finalize, g, dim3(kBlock), 0, st, sc, what);
 }
 }  // namespace fg
