@@ -1,18 +1,22 @@
 #!/usr/bin/env python3
-"""bench.py -- particle-substeps/s of the FLIP step on one MI355X.
+"""bench.py -- particle-substeps/s of the FLIP step on one MI355X, batched vs plain.
 
-Workload: the dam break at 1024x1024 cells, 8 jittered particles per cell
-(2,342,752 FP32 particles), FP32 grids/SpMV/IPP with FP64 PCG reductions and
-FP64 iterative refinement to relative residual 1e-6.  A "step" is one
-FlipSolver::stepFrame() (a 1/30 s frame, CFL-split into substeps).  The scene
-state stays resident in HBM; the timed region covers K frames after W warmup
-frames, bracketed by device synchronisation.
+Headline workload (`value`): 256 independent dam-break scenes of uneven size
+(resolution per scene from {64, 96, 128, 192, 256}, fluid-rectangle corners
+uniform in [0.1, 0.6] of the box, gravity 9.81 m/s^2 at +-30 degrees, 4-8
+particles per cell), FP32 state, FP64 PCG reductions + FP64 refinement to
+relative residual 1e-6.  A "step" is one frame (1/30 s, CFL-split) of every
+scene, all advanced together by fg_batch_step_frame (one launch per phase).
+The same scenes, stepped one at a time by fg_scene_step_frame on the same
+frames, give the plain-GPU number; both sets do identical work (their
+results are bit-identical).  State stays resident in HBM.
 
-Prints one JSON line (rank 0).  `roofline` is for the dominant kernel,
-measured with HIP events around each of its launches on the stream it runs
-on; `cpu_baseline` times the CPU restatement of the reference path (its own
-PCG, IPP and density solve, the reference's thread-pool decomposition) on a
-bounded sample of the same workload.
+Also reported on the same line: the plain single-scene 1024^2 dam break
+(8 particles per cell), a served stream (Poisson arrivals, HBM-budget
+admission, continuous batching), `roofline` for the dominant kernel measured
+with HIP events on its own stream, and `cpu_baseline`: the CPU restatement
+of the reference path (its own PCG, IPP and gated density solve, the
+reference's thread-pool decomposition) on a bounded sample of the workload.
 """
 from __future__ import annotations
 
@@ -20,7 +24,6 @@ import argparse
 import ctypes as C
 import json
 import os
-import statistics
 import sys
 import time
 
@@ -30,66 +33,272 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-substeps/s + p99 frame latency, batched vs plain GPU @1e-6 resid"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per work unit (per cell).  One PCG iteration is 54 B/cell:
+# Algorithmic bytes per cell of the PCG kernels.  One iteration is 54 B/cell:
 # SpMV 9 + direction update 12 (k_pcg_a: 21) and axpy pair 24 + IPP 9
 # (k_pcg_b: 33).
 KERNEL_BYTES = {
     "pcg_b": ("K_PCG_B", 33.0, "cells"),
     "pcg_a": ("K_PCG_A", 21.0, "cells"),
 }
-# HBM bytes per cell actually moved, at the 1024^2 dam break: rocprofv3
-# FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) per dispatch, the factors being
-# what the same counters read for a 1 GiB copy kernel
-# (profiles/pcg_dam_break_1024_hbm_traffic.md).
-MEASURED_TRAFFIC_PER_CELL = {1024: {"pcg_b": 34.18, "pcg_a": 21.83}}
+# HBM bytes per cell actually moved, per workload: rocprofv3 FETCH_SIZE
+# (x2.000) + WRITE_SIZE (x1.000) per dispatch, the factors being what the same
+# counters read for a 1 GiB copy kernel (profiles/pcg_*_hbm_traffic.md).
+MEASURED_TRAFFIC_PER_CELL = {"dam1024": {"pcg_b": 34.18, "pcg_a": 21.83}, "mixed256": {"pcg_b": 31.34, "pcg_a": 19.44}}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed frames")
+    ap.add_argument("--steps", type=int, default=8, help="timed frames of the batched workload")
     ap.add_argument("--warmup", type=int, default=3, help="untimed frames")
-    ap.add_argument("--res", type=int, default=1024)
-    ap.add_argument("--ppc", type=int, default=8)
+    ap.add_argument("--scenes", type=int, default=256)
+    ap.add_argument("--master-seed", type=int, default=2024)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--roofline-kernel", default="pcg_b", choices=sorted(KERNEL_BYTES))
-    ap.add_argument("--cpu-substeps", type=int, default=4, help="substeps in the CPU-baseline sample (0: skip)")
+    ap.add_argument("--skip-plain", action="store_true", help="batched leg only (profiling runs)")
+    ap.add_argument("--dam-frames", type=int, default=10, help="timed frames of the 1024^2 leg (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="served-stream window (0: skip)")
+    ap.add_argument("--serve-max-resident", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline sample budget (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
-    ap.add_argument("--stage-timing", action="store_true", help="extra run with per-stage event timing")
     return ap.parse_args()
 
 
-def cpu_baseline(res, ppc, substeps, threads):
-    """The oracle's reference-replica mode (reference PCG + IPP + max-abs test,
-    cap 200, density solve gated as in the reference) on the same dam break."""
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+def pctl(v, q):
+    if not v:
+        return 0.0
+    s = sorted(v)
+    pos = q * (len(s) - 1)
+    lo, hi = int(pos), min(len(s) - 1, int(pos) + 1)
+    return s[lo] + (s[hi] - s[lo]) * (pos - lo)
+
+
+class KernelTimer:
+    """Per-launch HIP-event timing of one kernel family (events attached to
+    each launch on the solver's stream), accumulated over chosen regions."""
+
+    def __init__(self, ctx, fg, name):
+        self.ctx, self.fg = ctx, fg
+        self.kid = getattr(fg, KERNEL_BYTES[name][0])
+        self.per_unit = KERNEL_BYTES[name][1]
+        self.ms = 0.0
+        self.launches = 0
+        self.units = 0
+
+    def __enter__(self):
+        self.ctx.profile_kernel(self.kid)
+        return self
+
+    def __exit__(self, *a):
+        t, n, u = self.ctx.profile_read()
+        self.ctx.profile_kernel(-1)
+        self.ms += t
+        self.launches += n
+        self.units += u
+
+    def roofline(self, traffic_per_cell=None, copy_gbs=None):
+        achieved = self.units * self.per_unit / (self.ms * 1e-3) / 1e9 if self.ms > 0 else 0.0
+        per_launch = self.units / max(self.launches, 1)
+        return {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_per_cell * per_launch if traffic_per_cell else None,
+            "avg_launch_us": 1e3 * self.ms / max(self.launches, 1),
+            "launches": self.launches,
+            "algorithmic_bytes_per_launch": per_launch * self.per_unit,
+            "bytes_per_unit": self.per_unit,
+            "unit_of_work": "cells",
+            "measured_copy_gbs": copy_gbs,
+        }
+
+
+def batched_leg(fg, ctx, a):
+    specs = fg.mixed_scene_specs(a.scenes, master_seed=a.master_seed)
+    t0 = time.perf_counter()
+    batch = [fg.make_scene(ctx, sp, tol=a.tol, pcg_iter_limit=4000) for sp in specs]
+    plain = [] if a.skip_plain else [fg.make_scene(ctx, sp, tol=a.tol, pcg_iter_limit=4000) for sp in specs]
+    n0 = sum(s.particle_count() for s in batch)
+    cells = sum(sp["n"] ** 2 for sp in specs)
+    log(f"{a.scenes} mixed scenes: {cells} cells, {n0} particles, set up in {time.perf_counter() - t0:.1f} s")
+    for w in range(a.warmup):
+        st = fg.batch_step_frame(ctx, batch)
+        for sc in plain:
+            sc.step_frame()
+        log(f"warmup frame {w}: max substeps {max(s.substeps for s in st)}")
+    kt = KernelTimer(ctx, fg, a.roofline_kernel)
+    el_b = el_p = 0.0
+    ps_b = ps_p = 0
+    lat_b, lat_p, frame_b, frame_p = [], [], [], []
+    conv, resid, iters, subs = True, 0.0, 0, 0
+    for k in range(a.steps):
+        with kt:
+            t0 = time.perf_counter()
+            st = fg.batch_step_frame(ctx, batch)
+            dt = time.perf_counter() - t0
+        el_b += dt
+        frame_b.append(dt * 1e3)
+        for s in st:
+            ps_b += s.particle_substeps
+            lat_b.append(s.frame_ms)
+            conv = conv and bool(s.converged)
+            resid = max(resid, s.pressure_rel_resid)
+            iters = max(iters, s.pressure_iters)
+            subs += s.substeps
+        if plain:
+            t0 = time.perf_counter()
+            for sc in plain:
+                s = sc.step_frame()
+                ps_p += s.particle_substeps
+                lat_p.append(s.frame_ms)
+            dt = time.perf_counter() - t0
+            el_p += dt
+            frame_p.append(dt * 1e3)
+        log(f"frame {k}: batched {frame_b[-1]:.1f} ms" + (f", plain {frame_p[-1]:.1f} ms" if plain else ""))
+    for sc in batch + plain:
+        sc.close()
+    out = {
+        "value": ps_b / el_b,
+        "ms_per_step": 1e3 * el_b / a.steps,
+        "particles": n0,
+        "cells": cells,
+        "substeps": subs,
+        "p50_frame_ms": pctl(lat_b, 0.5),
+        "p99_frame_ms": pctl(lat_b, 0.99),
+        "all_converged": conv,
+        "max_rel_residual": resid,
+        "pcg_iters_per_substep_max": iters,
+        "kt": kt,
+    }
+    if plain:
+        assert ps_p == ps_b, "plain and batched sets diverged"
+        out["plain"] = {
+            "value": ps_p / el_p,
+            "ms_per_step": 1e3 * el_p / a.steps,
+            "p50_frame_ms": pctl(lat_p, 0.5),
+            "p99_frame_ms": pctl(lat_p, 0.99),
+        }
+        out["batched_over_plain"] = out["value"] / out["plain"]["value"]
+    return out
+
+
+def dam_leg(fg, ctx, a, copy_gbs):
+    res, ppc = 1024, 8
+    scene = fg.dam_break_scene(ctx, res, ppc=ppc, tol=a.tol, pcg_iter_limit=4000)
+    n0 = scene.particle_count()
+    for _ in range(a.warmup):
+        scene.step_frame()
+    kt = KernelTimer(ctx, fg, a.roofline_kernel)
+    lat, psub, subs, resid, conv = [], 0, 0, 0.0, True
+    with kt:
+        t0 = time.perf_counter()
+        for _ in range(a.dam_frames):
+            s = scene.step_frame()
+            lat.append(s.frame_ms)
+            psub += s.particle_substeps
+            subs += s.substeps
+            resid = max(resid, s.pressure_rel_resid)
+            conv = conv and bool(s.converged)
+        el = time.perf_counter() - t0
+    scene.close()
+    log(f"dam break {res}^2: {a.dam_frames} frames, {subs} substeps, {el:.2f} s")
+    return {
+        "workload": f"plain single-scene dam break {res}x{res}, {ppc} ppc ({n0} particles)",
+        "value": psub / el,
+        "ms_per_step": 1e3 * el / a.dam_frames,
+        "frames": a.dam_frames,
+        "substeps": subs,
+        "p50_frame_ms": pctl(lat, 0.5),
+        "p99_frame_ms": pctl(lat, 0.99),
+        "max_rel_residual": resid,
+        "all_converged": conv,
+        "roofline": kt.roofline(MEASURED_TRAFFIC_PER_CELL["dam1024"].get(a.roofline_kernel), copy_gbs),
+    }
+
+
+def served_leg(fg, ctx, a):
+    rate, budget = 200.0, 200 << 30
+    reqs = fg.served_request_specs(rate=rate, duration=a.serve_seconds)
+    srv = fg.Server(ctx, hbm_budget=budget, max_resident=a.serve_max_resident)
+    t0 = time.perf_counter()
+    nxt, psub, ticks = 0, 0, 0
+    while True:
+        now = time.perf_counter() - t0
+        if now >= a.serve_seconds:
+            break
+        while nxt < len(reqs) and reqs[nxt]["t"] <= now:
+            srv.submit_spec(reqs[nxt], reqs[nxt]["frames"], tol=a.tol, pcg_iter_limit=4000)
+            nxt += 1
+        t = srv.tick()
+        ticks += 1
+        psub += t.particle_substeps
+        if t.resident == 0 and t.queued == 0 and nxt < len(reqs):
+            time.sleep(max(0.0, min(reqs[nxt]["t"] - (time.perf_counter() - t0), 0.01)))
+    el = time.perf_counter() - t0
+    s = srv.summary()
+    srv.close()
+    log(f"served {a.serve_seconds:.0f} s: {s.frames} scene frames, {psub / el / 1e6:.2f} M particle-substeps/s")
+    return {
+        "workload": f"Poisson arrivals {rate:g}/s for {a.serve_seconds:g} s (the full stream runs 120 s), "
+                    "resolution log-uniform [64, 1024], 4-8 ppc, 30-300 frames per request, 200 GB HBM budget, "
+                    f"at most {a.serve_max_resident} resident scenes",
+        "value": psub / el,
+        "unit": "particle-substeps/s",
+        "p50_frame_ms": s.p50_frame_ms,
+        "p99_frame_ms": s.p99_frame_ms,
+        "p50_admit_wait_ms": s.p50_admit_wait_ms,
+        "p99_admit_wait_ms": s.p99_admit_wait_ms,
+        "submitted": s.submitted,
+        "completed": s.completed,
+        "scene_frames": s.frames,
+        "peak_resident": s.peak_resident,
+        "peak_resident_gb": s.peak_resident_bytes / 1e9,
+    }
+
+
+def cpu_baseline(fg, a):
+    """The oracle's reference-replica mode (reference PCG + IPP + max-abs
+    test, cap 200, gated density solve) on frame 0 of the same mixed scenes,
+    in order, until the time budget is spent."""
     import oracle as orc
 
-    p, st = orc.dam_break(res, ppc=ppc, solver=orc.SOLVER_REF_REPLICA, density_mode=orc.DENSITY_GATED,
-                          threads=threads, tol=1e-2)
+    threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+    specs = fg.mixed_scene_specs(a.scenes, master_seed=a.master_seed)
     L = orc.lib()
-    stats = orc.OrcStats()
-    frame_dt = 1.0 / p.fps
-    t_frame, count, done = 0.0, 0, C.c_int32(0)
-    psub = 0
-    t0 = time.perf_counter()
-    for _ in range(substeps):
-        vel = L.orc_max_particle_velocity(st.ref())
-        dt = L.orc_frame_dt(vel, p.cfl_number, frame_dt, t_frame, count, p.max_substeps, C.byref(done))
-        psub += st.n
-        L.orc_substep(C.byref(p), st.ref(), dt, C.byref(stats))
-        t_frame += dt
-        count += 1
-        if done.value:
-            t_frame, count = 0.0, 0
-    el = time.perf_counter() - t0
+    psub, el, done = 0, 0.0, 0
+    for sp in specs:
+        p = orc.make_params(sp["n"], ppc=sp["ppc"], gravity=sp["gravity"], solver=orc.SOLVER_REF_REPLICA,
+                            density_mode=orc.DENSITY_GATED, threads=threads, tol=1e-2)
+        walls, fluids = orc.dam_break_geometry(sp["n"], sp["fluid_box"])
+        st = orc.init_scene(p, walls, fluids, seed=sp["seed"])
+        frame_dt = 1.0 / p.fps
+        t_frame, count, fin = 0.0, 0, C.c_int32(0)
+        t0 = time.perf_counter()
+        while not fin.value:
+            vel = L.orc_max_particle_velocity(st.ref())
+            dt = L.orc_frame_dt(vel, p.cfl_number, frame_dt, t_frame, count, p.max_substeps, C.byref(fin))
+            psub += st.n
+            L.orc_substep(C.byref(p), st.ref(), dt, C.byref(orc.OrcStats()))
+            t_frame += dt
+            count += 1
+        el += time.perf_counter() - t0
+        done += 1
+        if el >= a.cpu_seconds:
+            break
     return {
         "value": psub / el,
         "unit": "particle-substeps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {substeps} substeps of the {res}x{res} dam break ({st.n} particles), "
-                  f"reference PCG/IPP replica incl. the gated density solve, {threads} threads; {el:.1f} s",
-        "seconds": el,
+        "sample": f"frame 0 of the first {done} of the {a.scenes} mixed scenes, CPU restatement in "
+                  f"reference-replica mode (reference PCG/IPP, cap 200, gated density solve), {threads} threads; "
+                  f"{el:.1f} s",
     }
 
 
@@ -102,104 +311,49 @@ def main():
     import flipsolver2d_amd as fg
 
     ctx = fg.Context(0)
-    scene = fg.dam_break_scene(ctx, a.res, ppc=a.ppc, tol=a.tol, pcg_iter_limit=4000)
-    n0 = scene.particle_count()
-    print(f"# dam break {a.res}^2 ppc {a.ppc}: {n0} particles", file=sys.stderr, flush=True)
-    for w in range(a.warmup):
-        s = scene.step_frame()
-        print(f"# warmup frame {w}: {s.substeps} substeps, {s.pressure_iters} PCG iters, "
-              f"{s.frame_ms:.1f} ms", file=sys.stderr, flush=True)
-
-    kname, per_unit, unit_name = KERNEL_BYTES[a.roofline_kernel]
-    ctx.profile_kernel(getattr(fg, kname))
-    fg.lib().fg_ctx_profile_read(ctx.h, None, None, None)  # sync + reset
-    frame_ms, psub, subs, iters, resid, conv = [], 0, 0, [], 0.0, True
-    t0 = time.perf_counter()
-    for k in range(a.steps):
-        s = scene.step_frame()
-        frame_ms.append(s.frame_ms)
-        psub += s.particle_substeps
-        subs += s.substeps
-        iters.append(s.pressure_iters)
-        resid = max(resid, s.pressure_rel_resid)
-        conv = conv and bool(s.converged)
-    ktime, klaunch, kunits = ctx.profile_read()   # synchronises the stream
-    el = time.perf_counter() - t0
-    ctx.profile_kernel(-1)
-    print(f"# timed {a.steps} frames: {subs} substeps, {el:.3f} s", file=sys.stderr, flush=True)
-
-    avg_ms = ktime / max(klaunch, 1)
-    achieved = (kunits * per_unit) / (ktime * 1e-3) / 1e9 if ktime > 0 else 0.0
-    bytes_per_launch = kunits * per_unit / max(klaunch, 1)
     copy_gbs, _ = ctx.bandwidth_probe(1 << 30, 5)
-    tpc = MEASURED_TRAFFIC_PER_CELL.get(a.res, {}).get(a.roofline_kernel)
-    traffic = tpc * kunits / max(klaunch, 1) if tpc else None
-
-    stage = None
-    if a.stage_timing:
-        ctx.set_timing(True)
-        s = scene.step_frame()
-        ctx.set_timing(False)
-        stage = {k: round(v, 3) for k, v in s.as_dict()["stage_ms"].items()}
-
-    fs = sorted(frame_ms)
-    p50 = statistics.median(fs)
-    p99 = fs[min(len(fs) - 1, int(round(0.99 * (len(fs) - 1))))]
-    scene.close()
+    b = batched_leg(fg, ctx, a)
+    dam = dam_leg(fg, ctx, a, copy_gbs) if a.dam_frames > 0 else None
+    served = served_leg(fg, ctx, a) if a.serve_seconds > 0 else None
     ctx.close()
+    cpu = cpu_baseline(fg, a) if a.cpu_seconds > 0 else None
+    if cpu:
+        log(f"cpu baseline: {cpu['value']:.3e} particle-substeps/s")
 
-    cpu = None
-    if a.cpu_substeps > 0:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(a.res, a.ppc, a.cpu_substeps, threads)
-        print(f"# cpu baseline: {cpu['value']:.3e} particle-substeps/s ({cpu['seconds']:.1f} s)",
-              file=sys.stderr, flush=True)
-        cpu.pop("seconds")
-
-    value = psub / el
+    kt = b.pop("kt")
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": b["value"],
         "unit": "particle-substeps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * el / a.steps,
+        "ms_per_step": b["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"plain single-scene dam break {a.res}x{a.res}, {a.ppc} ppc ({n0} particles), "
-                        f"FP32 state, FP64 PCG reductions + refinement to rel resid {a.tol:g}",
-            "resolution": a.res, "particles_per_cell": a.ppc, "particles": n0, "step": "one stepFrame (1/30 s)",
-            "substeps": subs, "solver_tol": a.tol,
+            "workload": f"batched: {a.scenes} independent dam-break scenes, resolution from {{64,96,128,192,256}}, "
+                        f"4-8 ppc ({b['particles']} particles, {b['cells']} cells), FP32 state, FP64 PCG "
+                        f"reductions + refinement to rel resid {a.tol:g}; one step = one frame of every scene",
+            "scenes": a.scenes, "master_seed": a.master_seed, "solver_tol": a.tol,
+            "substeps": b["substeps"],
         },
-        "p50_frame_ms": p50,
-        "p99_frame_ms": p99,
-        "pcg_iters_per_substep_max": max(iters) if iters else 0,
-        "max_rel_residual": resid,
-        "all_converged": conv,
-        "roofline": {
-            "bound": "hbm",
-            "kernel": f"k_{a.roofline_kernel}",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "avg_launch_us": avg_ms * 1e3,
-            "launches": klaunch,
-            "algorithmic_bytes_per_launch": bytes_per_launch,
-            "bytes_per_unit": per_unit,
-            "unit_of_work": unit_name,
-            "measured_copy_gbs": copy_gbs,
-        },
+        "p50_frame_ms": b["p50_frame_ms"],
+        "p99_frame_ms": b["p99_frame_ms"],
+        "all_converged": b["all_converged"],
+        "max_rel_residual": b["max_rel_residual"],
+        "pcg_iters_per_substep_max": b["pcg_iters_per_substep_max"],
+        "plain": b.get("plain"),
+        "batched_over_plain": b.get("batched_over_plain"),
+        "roofline": dict(kt.roofline(MEASURED_TRAFFIC_PER_CELL.get("mixed256", {}).get(a.roofline_kernel),
+                                     copy_gbs), kernel=f"k_{a.roofline_kernel}"),
         "cpu_baseline": cpu,
+        "dam_break_1024": dam,
+        "served": served,
     }
-    if stage:
-        out["stage_ms_one_frame"] = stage
     print(json.dumps(out), flush=True)
 
 

@@ -29,7 +29,10 @@ EXPORTS = [
     "fg_scene_set_friction", "fg_scene_set_state", "fg_scene_get_state", "fg_scene_particle_count",
     "fg_scene_substep", "fg_scene_step_frame", "fg_scene_stage", "fg_batch_step_frame", "fg_poisson_solve",
     "fg_ctx_profile_kernel", "fg_ctx_profile_read", "fg_ctx_bandwidth_probe",
+    "fg_server_create", "fg_server_destroy", "fg_server_submit", "fg_server_tick", "fg_server_status",
+    "fg_server_scene", "fg_server_release", "fg_server_summary_get",
 ]
+REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_REJECTED, REQ_RELEASED = range(5)
 (K_ADVECT, K_SORT, K_P2G, K_SDF, K_LEVELSET, K_EXTRAPOLATE, K_PCG_A, K_PCG_B, K_G2P, K_RHS, K_APPLY) = range(11)
 
 
@@ -77,6 +80,36 @@ class Stats(C.Structure):
         return d
 
 
+class Request(C.Structure):
+    _fields_ = [("params", SceneParams), ("n_solids", C.c_int32), ("solid_nverts", C.c_void_p),
+                ("solid_verts", C.c_void_p), ("friction", C.c_void_p), ("n_fluids", C.c_int32),
+                ("fluid_nverts", C.c_void_p), ("fluid_verts", C.c_void_p), ("seed", C.c_uint32),
+                ("frames", C.c_int32), ("keep", C.c_int32)]
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("frames_done", C.c_int32), ("particles", C.c_int64),
+                ("particle_substeps", C.c_int64), ("submit_ms", C.c_double), ("admit_ms", C.c_double),
+                ("finish_ms", C.c_double), ("max_frame_ms", C.c_double)]
+
+
+class TickStats(C.Structure):
+    _fields_ = [("admitted", C.c_int32), ("stepped", C.c_int32), ("retired", C.c_int32), ("queued", C.c_int32),
+                ("resident", C.c_int32), ("particle_substeps", C.c_int64), ("tick_ms", C.c_double),
+                ("admit_ms", C.c_double), ("frame_ms_max", C.c_double), ("resident_bytes", C.c_uint64)]
+
+
+class ServerSummary(C.Structure):
+    _fields_ = [("frames", C.c_int64), ("particle_substeps", C.c_int64), ("busy_ms", C.c_double),
+                ("p50_frame_ms", C.c_double), ("p99_frame_ms", C.c_double), ("max_frame_ms", C.c_double),
+                ("p50_admit_wait_ms", C.c_double), ("p99_admit_wait_ms", C.c_double),
+                ("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
+                ("peak_resident_bytes", C.c_uint64), ("peak_resident", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("sweeps", C.c_int32), ("converged", C.c_int32),
                 ("rel_residual", C.c_double), ("solve_ms", C.c_double)]
@@ -116,6 +149,14 @@ def lib():
             "fg_ctx_profile_kernel": (C.c_int, [vp, i32]),
             "fg_ctx_profile_read": (C.c_int, [vp, C.POINTER(f64), C.POINTER(i64), C.POINTER(i64)]),
             "fg_ctx_bandwidth_probe": (C.c_int, [vp, C.c_uint64, i32, C.POINTER(f64), C.POINTER(f64)]),
+            "fg_server_create": (C.c_int, [vp, C.c_uint64, i32, C.POINTER(vp)]),
+            "fg_server_destroy": (C.c_int, [vp]),
+            "fg_server_submit": (C.c_int, [vp, C.POINTER(Request), C.POINTER(i64)]),
+            "fg_server_tick": (C.c_int, [vp, C.POINTER(TickStats)]),
+            "fg_server_status": (C.c_int, [vp, i64, C.POINTER(RequestStatus)]),
+            "fg_server_scene": (C.c_int, [vp, i64, C.POINTER(vp)]),
+            "fg_server_release": (C.c_int, [vp, i64]),
+            "fg_server_summary_get": (C.c_int, [vp, C.POINTER(ServerSummary)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -344,6 +385,110 @@ def mixed_scene_specs(count=256, master_seed=2024, resolutions=MIXED_RESOLUTIONS
         specs.append({"n": n, "fluid_box": (float(xs[0]), float(xs[1]), float(ys[0]), float(ys[1])),
                       "gravity": (float(9.81 * np.cos(ang)), float(9.81 * np.sin(ang))), "ppc": ppc, "seed": seed})
     return specs
+
+
+class _SceneView(Scene):
+    """A scene owned by a server (not destroyed by close())."""
+
+    def __init__(self, handle, params):
+        self.ctx, self.params = None, params
+        self.I, self.J = params.size_i, params.size_j
+        self.h = C.c_void_p(handle)
+
+    def close(self):
+        self.h = C.c_void_p()
+
+
+class Server:
+    """Continuous batching of scene simulations (fg_server_*): requests queue
+    in arrival order, are admitted while their memory fits `hbm_budget` and a
+    slot is free, and every tick steps one frame of all resident scenes in
+    one batched call."""
+
+    def __init__(self, ctx: Context, hbm_budget: int, max_resident: int = 256):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        self._params = {}
+        _check(lib().fg_server_create(ctx.h, hbm_budget, max_resident, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().fg_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def submit(self, params, solids, fluids, frames, seed=42, friction=None, keep=False):
+        snv = np.array([len(w) for w in solids] or [0], np.int32)
+        sv = np.ascontiguousarray(np.concatenate(solids).ravel() if solids else np.zeros(2), np.float32)
+        fnv = np.array([len(f) for f in fluids] or [0], np.int32)
+        fv = np.ascontiguousarray(np.concatenate(fluids).ravel() if fluids else np.zeros(2), np.float32)
+        fr = None if friction is None else np.ascontiguousarray(friction, np.float32)
+        r = Request()
+        r.params = params
+        r.n_solids, r.solid_nverts, r.solid_verts, r.friction = len(solids), snv.ctypes.data, sv.ctypes.data, _ptr(fr)
+        r.n_fluids, r.fluid_nverts, r.fluid_verts = len(fluids), fnv.ctypes.data, fv.ctypes.data
+        r.seed, r.frames, r.keep = seed, frames, 1 if keep else 0
+        rid = C.c_int64()
+        _check(lib().fg_server_submit(self.h, C.byref(r), C.byref(rid)))
+        self._params[rid.value] = params
+        return rid.value
+
+    def submit_spec(self, spec, frames, keep=False, **kw):
+        p = make_params(spec["n"], ppc=spec["ppc"], gravity=spec["gravity"], **kw)
+        walls, fluid = dam_break_geometry(spec["n"], spec["fluid_box"])
+        return self.submit(p, walls, fluid, frames, seed=spec["seed"], friction=np.zeros(len(walls), np.float32),
+                           keep=keep)
+
+    def tick(self):
+        t = TickStats()
+        _check(lib().fg_server_tick(self.h, C.byref(t)))
+        return t
+
+    def status(self, rid):
+        s = RequestStatus()
+        _check(lib().fg_server_status(self.h, rid, C.byref(s)))
+        return s
+
+    def scene(self, rid):
+        h = C.c_void_p()
+        _check(lib().fg_server_scene(self.h, rid, C.byref(h)))
+        return _SceneView(h.value, self._params[rid])
+
+    def release(self, rid):
+        _check(lib().fg_server_release(self.h, rid))
+
+    def summary(self):
+        s = ServerSummary()
+        _check(lib().fg_server_summary_get(self.h, C.byref(s)))
+        return s
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def served_request_specs(rate=200.0, duration=120.0, master_seed=77, res_range=(64, 1024),
+                         frame_range=(30, 300)):
+    """A Poisson stream of scene requests: arrival times at `rate` per second
+    over `duration` seconds; per request a resolution log-uniform in
+    `res_range`, the dam-break geometry, gravity angle and 4-8 particles per
+    cell of `mixed_scene_specs`, and a frame count uniform in `frame_range`."""
+    rng = np.random.default_rng(master_seed)
+    out, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            break
+        n = int(round(np.exp(rng.uniform(np.log(res_range[0]), np.log(res_range[1])))))
+        xs = np.sort(rng.uniform(0.1, 0.6, 2))
+        ys = np.sort(rng.uniform(0.1, 0.6, 2))
+        ang = np.deg2rad(rng.uniform(-30.0, 30.0))
+        out.append({"t": t, "n": n, "fluid_box": (float(xs[0]), float(xs[1]), float(ys[0]), float(ys[1])),
+                    "gravity": (float(9.81 * np.cos(ang)), float(9.81 * np.sin(ang))),
+                    "ppc": int(rng.integers(4, 9)), "seed": int(rng.integers(0, 2**31 - 1)),
+                    "frames": int(rng.integers(frame_range[0], frame_range[1] + 1))})
+    return out
 
 
 def make_scene(ctx, spec, **kw):

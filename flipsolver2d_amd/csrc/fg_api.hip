@@ -887,6 +887,7 @@ float polygon_sdf(const float* vt, int N, float px, float py)
 }  // namespace
 
 fg::LaunchEvents& fg::next_launch_events() { return g_next_events; }
+int fg::report_error(int code, const std::string& msg) { return fail(code, msg); }
 
 // ============================================================================
 // C-ABI

@@ -6,6 +6,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <string>
+
 #include "fg_device.h"
 
 namespace fg {
@@ -16,6 +18,9 @@ struct LaunchEvents {
     hipEvent_t start = nullptr, stop = nullptr;
 };
 LaunchEvents& next_launch_events();
+
+// Sets the message fg_last_error() returns and passes `code` through.
+int report_error(int code, const std::string& msg);
 
 template <typename F, typename... Args>
 inline void fg_launch(F kernel, dim3 grid, dim3 block, hipStream_t st, Args... args)

@@ -204,6 +204,80 @@ typedef struct fg_solve_stats {
 int fg_poisson_solve(fg_ctx* ctx, int32_t size_i, int32_t size_j, const int8_t* material, double scale,
                      const double* b, double* x, double rel_tol, int32_t max_iters, fg_solve_stats* stats);
 
+/* ------------------------------------------------------------------------
+ * Serving: continuous batching of scene simulations on one device.
+ *
+ * Replaces the reference's application loop that owns one FlipSolver per
+ * scene and calls stepFrame() on it (AutoBench/benchmarkrunnerapplication.cpp:88-92,
+ * Liquid2dRender/liquidrenderapp.cpp:250-253), for many scenes at once.
+ * Requests queue in arrival order; each tick admits queued requests while
+ * their scene memory (fg_scene_bytes) fits the server's HBM budget and a
+ * resident slot is free, steps one frame of every resident scene in one
+ * batched call (fg_batch_step_frame), and retires scenes that finished their
+ * frames.  State stays on the device between frames.  One host thread per
+ * server; calls on one server must not overlap.
+ * ---------------------------------------------------------------------- */
+typedef struct fg_server fg_server;
+
+typedef struct fg_request {
+    fg_scene_params params;
+    int32_t n_solids;             /* solids as in fg_scene_init_geometry */
+    const int32_t* solid_nverts;
+    const float* solid_verts;
+    const float* friction;        /* [n_solids] or NULL */
+    int32_t n_fluids;
+    const int32_t* fluid_nverts;
+    const float* fluid_verts;
+    uint32_t seed;
+    int32_t frames;               /* frames to simulate (>= 1) */
+    int32_t keep;                 /* 1: keep the scene after its last frame until fg_server_release */
+} fg_request;
+
+enum fg_request_state {
+    FG_REQ_QUEUED = 0, FG_REQ_RESIDENT = 1, FG_REQ_DONE = 2, FG_REQ_REJECTED = 3, FG_REQ_RELEASED = 4
+};
+
+typedef struct fg_request_status {
+    int32_t state;                /* fg_request_state */
+    int32_t frames_done;
+    int64_t particles;            /* live particles after the last frame */
+    int64_t particle_substeps;
+    double  submit_ms, admit_ms, finish_ms;  /* server clock: ms since fg_server_create */
+    double  max_frame_ms;         /* slowest frame of this request */
+} fg_request_status;
+
+typedef struct fg_tick_stats {
+    int32_t admitted, stepped, retired, queued, resident;
+    int64_t particle_substeps;
+    double  tick_ms;              /* admission + batched frame + retirement */
+    double  admit_ms;             /* of which: scene creation and first-frame init */
+    double  frame_ms_max;         /* slowest scene frame of the tick */
+    uint64_t resident_bytes;
+} fg_tick_stats;
+
+typedef struct fg_server_summary {
+    int64_t frames;               /* scene frames stepped */
+    int64_t particle_substeps;
+    double  busy_ms;              /* sum of tick durations */
+    double  p50_frame_ms, p99_frame_ms, max_frame_ms;   /* per scene frame */
+    double  p50_admit_wait_ms, p99_admit_wait_ms;       /* submit -> admit */
+    int64_t submitted, completed, rejected;
+    uint64_t peak_resident_bytes;
+    int32_t peak_resident;
+} fg_server_summary;
+
+int fg_server_create(fg_ctx* ctx, uint64_t hbm_budget_bytes, int32_t max_resident, fg_server** out);
+int fg_server_destroy(fg_server* srv);
+/* Queue a request (geometry is copied).  A request whose scene alone
+ * exceeds the budget is marked FG_REQ_REJECTED at once. */
+int fg_server_submit(fg_server* srv, const fg_request* req, int64_t* id_out);
+int fg_server_tick(fg_server* srv, fg_tick_stats* st);
+int fg_server_status(fg_server* srv, int64_t id, fg_request_status* out);
+/* The scene of a resident or kept request (for fg_scene_get_state). */
+int fg_server_scene(fg_server* srv, int64_t id, fg_scene** out);
+int fg_server_release(fg_server* srv, int64_t id);
+int fg_server_summary_get(fg_server* srv, fg_server_summary* out);
+
 #ifdef __cplusplus
 }
 #endif
