@@ -73,6 +73,7 @@ struct fg_ctx {
     SceneScalars* bsc_host = nullptr;
     int32_t* bflags = nullptr;
     int32_t* bflags_host = nullptr;
+    SceneDev* bpack = nullptr;      // iterating scenes of a solve, packed
 };
 
 struct fg_scene {
@@ -202,7 +203,7 @@ void init_geometry_fields(fg_scene* s)
     // solver blocks own contiguous chunks of whole 256-cell groups
     const int64_t groups = cdiv(s->C, kBlock);
     const int64_t want = std::min<int64_t>(groups, kPcgMaxBlocks);
-    s->pcg_chunk = (int)(cdiv(groups, want) * kBlock);
+    s->pcg_chunk = (int)std::max<int64_t>(kPcgMinChunk, cdiv(groups, want) * kBlock);
     s->max_blocks = (int)cdiv(s->C, s->pcg_chunk);
     s->max_pblocks = (int)std::max<int64_t>(1, cdiv(s->cap, kBlock));
 }
@@ -391,6 +392,34 @@ __global__ void k_group_flags(const SceneDev* scenes, int n, int band, int32_t* 
     if (threadIdx.x < 3) out[threadIdx.x] = any[threadIdx.x];
 }
 
+// The descriptors of the scenes whose inner PCG still iterates, packed in
+// slot order into `out` (one wave); info = {count, max solver blocks}.
+__global__ void k_pack_iterating(const SceneDev* in, int n, SceneDev* out, int32_t* info)
+{
+    const int lane = threadIdx.x;
+    int base = 0, maxblk = 0;
+    for (int b0 = 0; b0 < n; b0 += 64) {
+        const int k = b0 + lane;
+        bool it = false;
+        if (k < n) {
+            const SceneDev& s = in[k];
+            it = s.active && !s.sc->outer_done && !s.sc->inner_done;
+        }
+        const unsigned long long mask = __ballot(it);
+        const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+        if (it) {
+            out[base + pos] = in[k];
+            maxblk = max(maxblk, in[k].pcg_blocks);
+        }
+        base += __popcll(mask);
+    }
+    for (int o = 32; o > 0; o >>= 1) maxblk = max(maxblk, __shfl_xor(maxblk, o, 64));
+    if (lane == 0) {
+        info[0] = base;
+        info[1] = maxblk;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launch groups: the scenes of one launch, grid.y = scene slot
 // ---------------------------------------------------------------------------
@@ -409,20 +438,23 @@ struct Group {
 int ensure_group_buffers(fg_ctx* ctx, int n)
 {
     if (n <= ctx->bcap) return FG_OK;
+    const int cap = std::max(n, 2 * ctx->bcap);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (ctx->bdesc) hipFree(ctx->bdesc);
     if (ctx->bdesc_host) hipHostFree(ctx->bdesc_host);
     if (ctx->bsc) hipFree(ctx->bsc);
     if (ctx->bsc_host) hipHostFree(ctx->bsc_host);
+    if (ctx->bpack) hipFree(ctx->bpack);
+    ctx->bpack = nullptr;
     ctx->bdesc = nullptr;
     ctx->bdesc_host = nullptr;
     ctx->bsc = nullptr;
     ctx->bsc_host = nullptr;
     ctx->bcap = 0;
-    const int cap = std::max(n, 2 * ctx->bcap);
     HIPCHK(hipMalloc(&ctx->bdesc, sizeof(SceneDev) * cap));
     HIPCHK(hipHostMalloc((void**)&ctx->bdesc_host, sizeof(SceneDev) * cap, hipHostMallocDefault));
     HIPCHK(hipMalloc(&ctx->bsc, sizeof(SceneScalars) * cap));
+    HIPCHK(hipMalloc(&ctx->bpack, sizeof(SceneDev) * cap));
     HIPCHK(hipHostMalloc((void**)&ctx->bsc_host, sizeof(SceneScalars) * cap, hipHostMallocDefault));
     ctx->bcap = cap;
     return FG_OK;
@@ -489,6 +521,20 @@ int poll_group(Group& G, int band, int32_t* flags)
     HIPCHK(hipMemcpyAsync(ctx->bflags_host, ctx->bflags, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (int k = 0; k < 3; k++) flags[k] = ctx->bflags_host[k];
+    return FG_OK;
+}
+
+// Pack the scenes whose inner PCG still iterates; returns their count and
+// the launch grid over them (synchronises).
+int pack_iterating(Group& G, const SceneDev** d, dim3* grid, int* count)
+{
+    fg_ctx* ctx = G.ctx;
+    hipLaunchKernelGGL(k_pack_iterating, dim3(1), dim3(64), 0, ctx->stream, G.d, G.n, ctx->bpack, ctx->bflags);
+    HIPCHK(hipMemcpyAsync(ctx->bflags_host, ctx->bflags, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *count = ctx->bflags_host[0];
+    *d = ctx->bpack;
+    *grid = dim3((unsigned)std::max(1, ctx->bflags_host[1]), (unsigned)std::max(1, *count));
     return FG_OK;
 }
 
@@ -587,22 +633,36 @@ int run_solve(Group& G, bool rhs_from_grid)
         launch_cg_init(g.pcg, st, G.d, sweep);
         FGCHK(poll_group(G, -1, flags));
         if (!flags[0]) break;
+        // several scenes: launch over the packed iterating ones only, and
+        // re-pack at every poll as scenes stop
+        const SceneDev* dit = G.d;
+        dim3 git = g.pcg;
+        int count = G.n;
+        if (G.n > 1) {
+            FGCHK(pack_iterating(G, &dit, &git, &count));
+        } else {
+            count = flags[1];
+        }
         int it = 0;
-        for (;;) {
+        while (count > 0) {
             for (int k = 0; k < kPoll; k++, it++) {
                 {
                     KProf kp(ctx, FG_K_PCG_A, true);
-                    launch_pcg_a(g.pcg, st, G.d, it);
+                    launch_pcg_a(git, st, dit, it);
                     kp.done(0);
                 }
                 KProf kp(ctx, FG_K_PCG_B, true);
-                launch_pcg_b(g.pcg, st, G.d, it);
+                launch_pcg_b(git, st, dit, it);
                 kp.done(0);
                 launched++;
             }
-            FGCHK(poll_group(G, -1, flags));
-            if (!flags[1]) break;
-            if (it > G.max_inner + kPoll) return fail(FG_E_STATE, "inner solve did not stop at its cap");
+            if (G.n > 1) {
+                FGCHK(pack_iterating(G, &dit, &git, &count));
+            } else {
+                FGCHK(poll_group(G, -1, flags));
+                count = flags[1];
+            }
+            if (count > 0 && it > G.max_inner + kPoll) return fail(FG_E_STATE, "inner solve did not stop at its cap");
         }
     }
     FGCHK(read_group_scalars(G));
@@ -930,6 +990,7 @@ int fg_ctx_destroy(fg_ctx* ctx)
     if (ctx->bdesc_host) hipHostFree(ctx->bdesc_host);
     if (ctx->bsc) hipFree(ctx->bsc);
     if (ctx->bsc_host) hipHostFree(ctx->bsc_host);
+    if (ctx->bpack) hipFree(ctx->bpack);
     if (ctx->bflags) hipFree(ctx->bflags);
     if (ctx->bflags_host) hipHostFree(ctx->bflags_host);
     delete ctx;
