@@ -43,7 +43,7 @@ KERNEL_BYTES = {
 # HBM bytes per cell actually moved, per workload: rocprofv3 FETCH_SIZE
 # (x2.000) + WRITE_SIZE (x1.000) per dispatch, the factors being what the same
 # counters read for a 1 GiB copy kernel (profiles/pcg_*_hbm_traffic.md).
-MEASURED_TRAFFIC_PER_CELL = {"dam1024": {"pcg_b": 34.18, "pcg_a": 21.83}, "mixed256": {"pcg_b": 31.34, "pcg_a": 19.44}}
+MEASURED_TRAFFIC_PER_CELL = {"dam1024": {"pcg_b": 34.18, "pcg_a": 21.83}, "mixed256": {"pcg_b": 29.70, "pcg_a": 17.80}}
 
 
 def parse():
