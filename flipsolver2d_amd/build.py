@@ -48,9 +48,33 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    _build_facade()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB}, {FACADE}, {DEMO}")
     return LIB
+
+
+CPP = os.path.join(PKG, "cpp")
+FACADE = os.path.join(LIBDIR, "libflipsolver2d_hip.so")
+DEMO = os.path.join(LIBDIR, "facade_demo")
+CXX = os.environ.get("CXX", "g++")
+
+
+def _build_facade() -> None:
+    """The reference-named C++ API (host-only C++ over the C-ABI) and its
+    demo driver, linked against libflipgpu.so next to them."""
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", CPP]
+    link = ["-L", LIBDIR, "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    steps = [
+        [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall"] + inc + [os.path.join(CPP, "flipsolver2d_hip.cpp")]
+        + link + ["-lflipgpu", "-o", FACADE],
+        [CXX, "-O2", "-std=c++17", "-Wall"] + inc + [os.path.join(CPP, "facade_demo.cpp")]
+        + link + ["-lflipsolver2d_hip", "-lflipgpu", "-o", DEMO],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"C++ API build failed:\n{' '.join(cmd)}\n{r.stderr}")
 
 
 if __name__ == "__main__":

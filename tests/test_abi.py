@@ -66,3 +66,15 @@ def test_product_does_not_link_the_oracle():
             if f.endswith((".py", ".hip", ".h", ".cpp")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in text and "flip_oracle" not in text, f
+
+
+def test_cpp_api_builds_and_links():
+    """The reference-named C++ API library and its driver are built and
+    resolve their libraries (the driver's usage path makes no HIP call)."""
+    import subprocess
+
+    from flipsolver2d_amd import build as b
+
+    assert os.path.exists(b.FACADE) and os.path.exists(b.DEMO)
+    r = subprocess.run([b.DEMO], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
