@@ -74,6 +74,8 @@ struct fg_ctx {
     int32_t* bflags = nullptr;
     int32_t* bflags_host = nullptr;
     SceneDev* bpack = nullptr;      // iterating scenes of a solve, packed
+    SceneDev* bsub = nullptr;       // a sub-group's descriptors (density correction)
+    SceneDev* bsub_host = nullptr;
 };
 
 struct fg_scene {
@@ -268,6 +270,10 @@ int create_scene(fg_ctx* ctx, const fg_scene_params* p, bool solver_only, fg_sce
     h.tol = p->solver_tolerance > 0 ? p->solver_tolerance : 1e-6;
     h.max_inner = p->pcg_iter_limit > 0 ? p->pcg_iter_limit : 2000;
     h.active = 1;
+    h.density_mode = p->density_mode;
+    // flipsolver2d.cpp:207-208
+    h.cell_vol = (float)(p->dx * p->dx * p->dx);
+    h.pmass = (float)((p->fluid_density * h.cell_vol) / (float)p->particles_per_cell);
     h.max_blocks = s->max_blocks;
     h.pcg_blocks = s->max_blocks;
     h.pcg_chunk = s->pcg_chunk;
@@ -342,6 +348,8 @@ void set_substep_scalars(fg_scene* s, float dt)
     h.sA = dt / (s->p.fluid_density * s->p.dx * s->p.dx);             // flipsolver2d.cpp:799
     h.sApply = dt / (s->p.fluid_density * s->p.dx);                   // flipsolver2d.cpp:1135
     h.sRhs = 1.f / s->p.dx;                                            // flipsolver2d.cpp:964
+    h.dens_scale = (float)((dt * dt) / (s->p.fluid_density * s->p.dx * s->p.dx));  // flipsolver2d.cpp:263
+    h.dens_rhs_scale = 1.0 / dt;                                       // flipsolver2d.cpp:995
 }
 
 // ---------------------------------------------------------------------------
@@ -353,6 +361,7 @@ __global__ void k_substep_begin(const SceneDev* scenes)
     if (threadIdx.x != 0 || !s.active) return;
     s.sc->killed = 0;
     s.sc->erased = 0;
+    s.sc->density_applied = 0;
 }
 __global__ void k_set_count(const SceneDev* scenes, int n)
 {
@@ -445,7 +454,11 @@ int ensure_group_buffers(fg_ctx* ctx, int n)
     if (ctx->bsc) hipFree(ctx->bsc);
     if (ctx->bsc_host) hipHostFree(ctx->bsc_host);
     if (ctx->bpack) hipFree(ctx->bpack);
+    if (ctx->bsub) hipFree(ctx->bsub);
+    if (ctx->bsub_host) hipHostFree(ctx->bsub_host);
     ctx->bpack = nullptr;
+    ctx->bsub = nullptr;
+    ctx->bsub_host = nullptr;
     ctx->bdesc = nullptr;
     ctx->bdesc_host = nullptr;
     ctx->bsc = nullptr;
@@ -455,6 +468,8 @@ int ensure_group_buffers(fg_ctx* ctx, int n)
     HIPCHK(hipHostMalloc((void**)&ctx->bdesc_host, sizeof(SceneDev) * cap, hipHostMallocDefault));
     HIPCHK(hipMalloc(&ctx->bsc, sizeof(SceneScalars) * cap));
     HIPCHK(hipMalloc(&ctx->bpack, sizeof(SceneDev) * cap));
+    HIPCHK(hipMalloc(&ctx->bsub, sizeof(SceneDev) * cap));
+    HIPCHK(hipHostMalloc((void**)&ctx->bsub_host, sizeof(SceneDev) * cap, hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&ctx->bsc_host, sizeof(SceneScalars) * cap, hipHostMallocDefault));
     ctx->bcap = cap;
     return FG_OK;
@@ -462,7 +477,8 @@ int ensure_group_buffers(fg_ctx* ctx, int n)
 
 // Build a group and upload its descriptors.  A single scene uses its own
 // device descriptor; several share the context's descriptor array.
-int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vector<fg_stats*>& stats, Group& G)
+int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vector<fg_stats*>& stats, Group& G,
+               bool sub = false)
 {
     G = Group{};
     G.ctx = ctx;
@@ -496,9 +512,11 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
         return FG_OK;
     }
     FGCHK(ensure_group_buffers(ctx, G.n));
-    for (int k = 0; k < G.n; k++) ctx->bdesc_host[k] = scenes[k]->h;
-    HIPCHK(hipMemcpyAsync(ctx->bdesc, ctx->bdesc_host, sizeof(SceneDev) * G.n, hipMemcpyHostToDevice, ctx->stream));
-    G.d = ctx->bdesc;
+    SceneDev* dev = sub ? ctx->bsub : ctx->bdesc;
+    SceneDev* host = sub ? ctx->bsub_host : ctx->bdesc_host;
+    for (int k = 0; k < G.n; k++) host[k] = scenes[k]->h;
+    HIPCHK(hipMemcpyAsync(dev, host, sizeof(SceneDev) * G.n, hipMemcpyHostToDevice, ctx->stream));
+    G.d = dev;
     return FG_OK;
 }
 
@@ -704,11 +722,34 @@ int run_stage(Group& G, int stage)
         kp.done(G.parts);
         break;
     }
-    case FG_STAGE_DENSITY:
-        for (fg_scene* s : G.sc)
-            if (s->p.density_mode != FG_DENSITY_OFF)
-                return fail(FG_E_STATE, "density correction modes other than OFF are not built yet");
+    case FG_STAGE_DENSITY: {
+        // FlipSolver::densityCorrection (flipsolver2d.cpp:164-186) for the
+        // scenes that run it, as a sub-group with its own descriptors
+        std::vector<fg_scene*> ds;
+        std::vector<fg_stats*> dst;
+        for (int k = 0; k < G.n; k++)
+            if (G.sc[k]->p.density_mode != FG_DENSITY_OFF) {
+                ds.push_back(G.sc[k]);
+                dst.push_back(G.st[k]);
+            }
+        if (ds.empty()) break;
+        Group D;
+        FGCHK(make_group(ctx, ds, dst, D, true));
+        launch_density_rhs(D.g.cells, st, D.d);
+        FGCHK(run_solve(D, false));
+        for (int k = 0; k < D.n; k++)
+            if (D.st[k]) D.st[k]->density_iters = std::max(D.st[k]->density_iters, D.sc[k]->hsc->total_iters);
+        // move the particles (where the gate opens) and restore (cell, id) order
+        launch_density_adjust(D.g.parts, st, D.d);
+        launch_scan_local(D.g.scan, st, D.d);
+        launch_scan_top(D.g.scan1, st, D.d);
+        launch_scan_add(D.g.scan, st, D.d);
+        launch_scatter(D.g.parts, st, D.d);
+        launch_fixup(D.g.cells, st, D.d);
+        launch_gather(D.g.parts, st, D.d);
+        launch_finalize(D.g.one, st, D.d, 4);
         break;
+    }
     case FG_STAGE_P2G: {
         KProf kp(ctx, FG_K_P2G, true);
         launch_p2g(g.cells, st, d);
@@ -802,6 +843,7 @@ void commit_substep(Group& G, float* dts)
             stats->particles = s->n_live;
             stats->particles_killed += sc.killed;
             stats->particles_erased += sc.erased;
+            stats->density_applied += sc.density_applied;
             stats->last_dt = dts[k];
         }
     }
@@ -991,6 +1033,8 @@ int fg_ctx_destroy(fg_ctx* ctx)
     if (ctx->bsc) hipFree(ctx->bsc);
     if (ctx->bsc_host) hipHostFree(ctx->bsc_host);
     if (ctx->bpack) hipFree(ctx->bpack);
+    if (ctx->bsub) hipFree(ctx->bsub);
+    if (ctx->bsub_host) hipHostFree(ctx->bsub_host);
     if (ctx->bflags) hipFree(ctx->bflags);
     if (ctx->bflags_host) hipHostFree(ctx->bflags_host);
     delete ctx;

@@ -51,7 +51,7 @@ struct SceneScalars {
     int32_t sweeps;
     int32_t numeric_bad;    // NaN/Inf seen in a reduction
     int32_t ls_more;        // level-set BFS reached the end of a launch's band
-    int32_t pad;
+    int32_t density_applied;  // density correction moved the particles this substep
 };
 
 // Per-scene device descriptor.  One array of these lives in device memory;
@@ -66,6 +66,11 @@ struct SceneDev {
     double tol;             // FP64 relative residual target
     int32_t max_inner;      // inner iteration cap per sweep
     int32_t active;         // 0: skipped by every kernel of a batched launch
+    int32_t density_mode;   // 0 off, 1 gated on convergence, 2 on
+    float cell_vol;         // (float)(dx^3): updateDensityGridThread's cell volume
+    float pmass;            // rho * cell_vol / ppc as a float
+    float dens_scale;       // (float)(dt*dt / (rho dx^2)): adjustParticlesByDensity
+    double dens_rhs_scale;  // 1.0 / dt: calcDensityCorrectionRhs
     int32_t max_blocks;     // stride of the FP64 partials array (>= pcg_blocks)
     int32_t pcg_blocks;     // solver blocks of this scene
     int32_t pcg_chunk;      // cells per solver block (multiple of kBlock)

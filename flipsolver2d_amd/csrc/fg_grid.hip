@@ -355,6 +355,101 @@ __global__ void __launch_bounds__(kBlock) k_levelset_band(const SceneDev* scenes
 }
 
 // ---------------------------------------------------------------------------
+// Density correction, part 1: FlipSolver::updateDensityGridThread
+// (flipsolver2d.cpp:199-249) and calcDensityCorrectionRhs (993-1011).  The
+// centred bilinear hat reaches a cell from particles of the 3x3 cells around
+// it only, gathered in (cell, id) order; the right-hand side goes to b.
+// Materials are the pre-update ones (the step runs this before gridUpdate).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_density_rhs(const SceneDev* scenes)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active || s.density_mode == 0) return;
+    const int I = s.I, J = s.J, C = I * J;
+    const int c = cell_index(gridDim.x);
+    if (c >= C) return;
+    const int i = c / J, j = c - (c / J) * J;
+    const float fi = (float)i, fj = (float)j;
+    const float mass = s.pmass;
+    float d = 0.f;
+    const int j0 = j > 0 ? j - 1 : 0, j1 = j < J - 1 ? j + 1 : J - 1;
+    for (int di = -1; di <= 1; di++) {
+        const int ii = i + di;
+        if (ii < 0 || ii >= I) continue;
+        const int a = s.cellStart[ii * J + j0], e = s.cellStart[ii * J + j1 + 1];
+        for (int k = a; k < e; k++) {
+            const float2 p = s.pos[k];
+            const float w = bilinear_hat(p.x - fi - 0.5f, p.y - fj - 0.5f);
+            if (fabsf(w) > 1e-6f) d += w * mass;
+        }
+    }
+    d /= s.cell_vol;
+    const int8_t m = s.mat[c];
+    if (is_fluid(m) && (is_empty(mat_ext(s.mat, I, J, i + 1, j)) || is_empty(mat_ext(s.mat, I, J, i - 1, j)) ||
+                        is_empty(mat_ext(s.mat, I, J, i, j + 1)) || is_empty(mat_ext(s.mat, I, J, i, j - 1))))
+        d = fclamp(d, (float)s.rho, 3.40282347e+38f);
+    if (is_fluid(m)) {
+        double r = (double)d / s.rho;
+        r = r < 0.5 ? 0.5 : (1.5 < r ? 1.5 : r);
+        s.b[c] = s.dens_rhs_scale * (1.0 - r);
+    } else {
+        s.b[c] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Density correction, part 2: adjustParticlesByDensityThread
+// (flipsolver2d.cpp:261-303) when the scene's gate is open (mode on, or
+// gated and the solve reached its tolerance), then re-key for the counting
+// sort: particles keep (cell, id) order for every later gather.  The
+// reference's index past the last row reads outside its vector; it reads 0
+// here.  A particle pushed outside the grid is dropped.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_density_adjust(const SceneDev* scenes)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active || s.density_mode == 0 || (int)blockIdx.x >= s.pblocks) return;
+    const SceneScalars* sc = s.sc;
+    const double bn = sqrt(sc->bb), rn = sqrt(sc->rr64);
+    const bool conv = !sc->numeric_bad && (bn == 0.0 || rn <= s.tol * bn);
+    const bool apply = s.density_mode == 2 || conv;
+    const int n = sc->n_particles;
+    const int I = s.I, J = s.J;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t key = kDead;
+    if (k < n) {
+        float2 p = s.pos[k];
+        if (apply) {
+            const float x = p.x, y = p.y;
+            const int ic = iclamp((int)(x - 0.5f), 0, I), jc = iclamp((int)(y - 0.5f), 0, J);
+            const int i = iclamp((int)x, 0, I), j = iclamp((int)y, 0, J);
+            auto P = [&](int a, int b) -> float {
+                return (a < 0 || a >= I || b < 0 || b >= J) ? 0.f : (float)s.x64[(size_t)a * J + b];
+            };
+            const float pci = P(ic, j), pcj = P(i, jc);
+            float pi = P(ic + 1, j), pj = P(i, jc + 1);
+            if (is_solid(mat_ext(s.mat, I, J, ic + 1, j)) || is_solid(mat_ext(s.mat, I, J, ic, j))) pi = pci;
+            if (is_solid(mat_ext(s.mat, I, J, i, jc + 1)) || is_solid(mat_ext(s.mat, I, J, i, jc))) pj = pcj;
+            p.x = x + (pi - pci) * s.dens_scale;
+            p.y = y + (pj - pcj) * s.dens_scale;
+        }
+        const int pi_ = integr(p.x), pj_ = integr(p.y);
+        if (pi_ >= 0 && pi_ < I && pj_ >= 0 && pj_ < J) key = (uint32_t)(pi_ * J + pj_);
+        s.posA[k] = p;
+        s.velA[k] = s.vel[k];
+        s.key[k] = key;
+    }
+    const int lane = threadIdx.x & 63;
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || prev != key;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long above = heads & ~((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+    const int end = above ? (__ffsll((long long)above) - 1) : 64;
+    if (head && key != kDead) atomicAdd(&s.cellCount[key], end - lane);
+    if (blockIdx.x == 0 && threadIdx.x == 0) s.sc->density_applied = apply ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
 // Saved grid + body forces: m_savedFluidVelocityGrid = m_fluidVelocityGrid;
 // applyBodyForces (flipsolver2d.cpp:439-440, 1222-1241).  The extrapolated
 // grid (su/sv) is the saved copy; the working grid gets the force.
@@ -491,4 +586,6 @@ FG_LAUNCH1(levelset_band, int, band)
 FG_LAUNCH0(forces)
 FG_LAUNCH0(rhs)
 FG_LAUNCH0(apply_solids)
+FG_LAUNCH0(density_rhs)
+FG_LAUNCH0(density_adjust)
 }  // namespace fg

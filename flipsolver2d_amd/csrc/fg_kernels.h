@@ -59,6 +59,8 @@ FG_DECLARE_LAUNCH(levelset_band, int band);
 FG_DECLARE_LAUNCH(forces);
 FG_DECLARE_LAUNCH(rhs);
 FG_DECLARE_LAUNCH(apply_solids);
+FG_DECLARE_LAUNCH(density_rhs);
+FG_DECLARE_LAUNCH(density_adjust);
 constexpr int kBfsTile = 32;
 
 // fg_pcg.hip

@@ -333,6 +333,7 @@ __global__ void __launch_bounds__(kBlock) k_gather(const SceneDev* scenes)
 //   what & 1: commit the sorted count, sum the advect kills
 //   what & 2: max |v|^2 (flipsolver2d.cpp:1562 starts from FLT_MIN) and
 //             the countParticles erasures
+//   what & 4: commit the sorted count only (re-sort after density correction)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_finalize(const SceneDev* scenes, int what)
 {
@@ -359,6 +360,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const SceneDev* scenes, int
             sc->n_particles = s.cellStart[s.I * s.J];
             sc->killed = kills;
         }
+        if (what & 4) sc->n_particles = s.cellStart[s.I * s.J];
         if (what & 2) {
             const float fmin = 1.17549435e-38f;
             sc->max_v2_bits = __float_as_uint(m > fmin ? m : fmin);
