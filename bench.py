@@ -12,7 +12,8 @@ frames, give the plain-GPU number; both sets do identical work (their
 results are bit-identical).  State stays resident in HBM.
 
 Also reported on the same line: the plain single-scene 1024^2 dam break
-(8 particles per cell), a served stream (Poisson arrivals, HBM-budget
+(8 particles per cell), the isolated 4096^2 pressure projection, a served
+stream (Poisson arrivals, HBM-budget
 admission, continuous batching), `roofline` for the dominant kernel measured
 with HIP events on its own stream, and `cpu_baseline`: the CPU restatement
 of the reference path (its own PCG, IPP and gated density solve, the
@@ -57,6 +58,7 @@ def parse():
     ap.add_argument("--roofline-kernel", default="pcg_b", choices=sorted(KERNEL_BYTES))
     ap.add_argument("--skip-plain", action="store_true", help="batched leg only (profiling runs)")
     ap.add_argument("--dam-frames", type=int, default=10, help="timed frames of the 1024^2 leg (0: skip)")
+    ap.add_argument("--proj-res", type=int, default=4096, help="isolated projection grid (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="served-stream window (0: skip)")
     ap.add_argument("--serve-max-resident", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline sample budget (0: skip)")
@@ -222,6 +224,27 @@ def dam_leg(fg, ctx, a, copy_gbs):
     }
 
 
+def projection_leg(fg, ctx, a, copy_gbs):
+    n = a.proj_res
+    mat, _, _, b, scale = fg.isolated_projection_system(n, 1)
+    kt = KernelTimer(ctx, fg, a.roofline_kernel)
+    ctx.poisson_solve(mat, scale, b, tol=a.tol)          # warm-up (allocation, code load)
+    with kt:
+        x, st = ctx.poisson_solve(mat, scale, b, tol=a.tol)
+    liquid = int(((mat & fg.FLUID) != 0).sum())
+    log(f"isolated projection {n}^2: {st.iterations} iterations, {st.solve_ms:.1f} ms, resid {st.rel_residual:.2e}")
+    return {
+        "workload": f"isolated pressure projection {n}x{n}: smoothed-noise liquid mask ({liquid} unknowns), "
+                    "one-cell solid border + 2% random solids, rhs = divergence of U[-1,1] faces",
+        "iterations": st.iterations,
+        "sweeps": st.sweeps,
+        "rel_residual": st.rel_residual,
+        "converged": bool(st.converged),
+        "solve_ms_incl_rhs_upload": st.solve_ms,
+        "roofline": kt.roofline(None, copy_gbs),
+    }
+
+
 def served_leg(fg, ctx, a):
     rate, budget = 200.0, 200 << 30
     reqs = fg.served_request_specs(rate=rate, duration=a.serve_seconds)
@@ -314,6 +337,7 @@ def main():
     copy_gbs, _ = ctx.bandwidth_probe(1 << 30, 5)
     b = batched_leg(fg, ctx, a)
     dam = dam_leg(fg, ctx, a, copy_gbs) if a.dam_frames > 0 else None
+    proj = projection_leg(fg, ctx, a, copy_gbs) if a.proj_res > 0 else None
     served = served_leg(fg, ctx, a) if a.serve_seconds > 0 else None
     ctx.close()
     cpu = cpu_baseline(fg, a) if a.cpu_seconds > 0 else None
@@ -352,6 +376,7 @@ def main():
                                      copy_gbs), kernel=f"k_{a.roofline_kernel}"),
         "cpu_baseline": cpu,
         "dam_break_1024": dam,
+        "isolated_projection": proj,
         "served": served,
     }
     print(json.dumps(out), flush=True)

@@ -491,6 +491,48 @@ def served_request_specs(rate=200.0, duration=120.0, master_seed=77, res_range=(
     return out
 
 
+def isolated_projection_system(n=4096, seed=1, dt=1.0 / 30, density=1000.0):
+    """The isolated pressure projection: an n x n MAC grid whose liquid mask
+    is uniform noise smoothed by a Gaussian (sigma 8 cells, reflecting edges)
+    and thresholded at its 40th percentile (~60 % liquid, the rest air), plus
+    a one-cell solid border and 2 % random solid cells; face velocities
+    uniform in [-1, 1] (FP32); the right-hand side is calcPressureRhs
+    (flipsolver2d.cpp:962-991) of those faces.  Returns (material[n, n] int8,
+    u, v, b, scale) with scale = dt / (rho dx^2), dx = 1/n."""
+    from scipy.ndimage import gaussian_filter
+
+    rng = np.random.default_rng(seed)
+    smooth = gaussian_filter(rng.random((n, n), dtype=np.float32), 8.0)
+    mat = np.where(smooth > np.percentile(smooth, 40), FLUID, EMPTY).astype(np.int8)
+    mat[0, :] = mat[-1, :] = mat[:, 0] = mat[:, -1] = SOLID
+    mat[rng.random((n, n)) < 0.02] = SOLID
+    u = rng.uniform(-1.0, 1.0, (n + 1, n)).astype(np.float32)
+    v = rng.uniform(-1.0, 1.0, (n, n + 1)).astype(np.float32)
+    dx = 1.0 / n
+    return mat, u, v, pressure_rhs(mat, u, v, dx), dt / (density * dx * dx)
+
+
+def pressure_rhs(mat, u, v, dx):
+    """FlipSolver::calcPressureRhs + divergenceAt (flipsolver2d.cpp:759-764,
+    962-991) on host arrays: the divergence in float, widened, with the solid
+    face terms (solid velocity 0) in the same order; neighbour materials
+    clamp at the edges."""
+    scale = 1.0 / dx
+    u0, u1 = u[:-1, :], u[1:, :]
+    v0, v1 = v[:, :-1], v[:, 1:]
+    div = (((u1 - u0) + v1) - v0) + np.float32(0.0)
+    ext = np.pad(mat, 1, mode="edge")
+    solid = (ext & SOLID) != 0
+    s_im, s_ip = solid[:-2, 1:-1], solid[2:, 1:-1]
+    s_jm, s_jp = solid[1:-1, :-2], solid[1:-1, 2:]
+    val = -scale * div.astype(np.float64)
+    val = val - s_im * scale * u0.astype(np.float64)
+    val = val + s_ip * scale * u1.astype(np.float64)
+    val = val - s_jm * scale * v0.astype(np.float64)
+    val = val + s_jp * scale * v1.astype(np.float64)
+    return np.where((mat & FLUID) != 0, val, 0.0).ravel()
+
+
 def make_scene(ctx, spec, **kw):
     """A dam-break scene from one `mixed_scene_specs` entry."""
     return dam_break_scene(ctx, spec["n"], ppc=spec["ppc"], seed=spec["seed"], fluid_box=spec["fluid_box"],
