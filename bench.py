@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--roofline-kernel", default="pcg_b", choices=sorted(KERNEL_BYTES))
     ap.add_argument("--skip-plain", action="store_true", help="batched leg only (profiling runs)")
+    ap.add_argument("--tune-rounds", type=int, default=1, help="autotuner rounds (each steps 4 frames)")
     ap.add_argument("--dam-frames", type=int, default=10, help="timed frames of the 1024^2 leg (0: skip)")
     ap.add_argument("--proj-res", type=int, default=4096, help="isolated projection grid (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="served-stream window (0: skip)")
@@ -88,6 +89,7 @@ class KernelTimer:
         self.kid = getattr(fg, KERNEL_BYTES[name][0])
         self.per_unit = KERNEL_BYTES[name][1]
         self.ms = 0.0
+        self.busy = 0.0
         self.launches = 0
         self.units = 0
 
@@ -96,9 +98,10 @@ class KernelTimer:
         return self
 
     def __exit__(self, *a):
-        t, n, u = self.ctx.profile_read()
+        t, b, n, u = self.ctx.profile_read()
         self.ctx.profile_kernel(-1)
         self.ms += t
+        self.busy += b
         self.launches += n
         self.units += u
 
@@ -118,6 +121,10 @@ class KernelTimer:
             "bytes_per_unit": self.per_unit,
             "unit_of_work": "cells",
             "measured_copy_gbs": copy_gbs,
+            # launches of several lanes overlap: algorithmic bytes over the
+            # time at least one launch of the kernel was running
+            "busy_ms": self.busy,
+            "busy_achieved": self.units * self.per_unit / (self.busy * 1e-3) / 1e9 if self.busy > 0 else 0.0,
         }
 
 
@@ -134,6 +141,13 @@ def batched_leg(fg, ctx, a):
         for sc in plain:
             sc.step_frame()
         log(f"warmup frame {w}: max substeps {max(s.substeps for s in st)}")
+    # the lane schedule (size classes on separate streams) is picked by
+    # measurement on live frames; the plain set advances the same frames
+    lanes, rates = fg.batch_autotune(ctx, batch, rounds=a.tune_rounds)
+    for _ in range(a.tune_rounds * fg.MAX_LANES):
+        for sc in plain:
+            sc.step_frame()
+    log(f"autotuned lanes: {lanes} (rates {[round(r / 1e6, 1) for r in rates]} M)")
     kt = KernelTimer(ctx, fg, a.roofline_kernel)
     el_b = el_p = 0.0
     ps_b = ps_p = 0
@@ -177,6 +191,8 @@ def batched_leg(fg, ctx, a):
         "max_rel_residual": resid,
         "pcg_iters_per_substep_max": iters,
         "kt": kt,
+        "lanes": lanes,
+        "autotune_rates": [float(r) for r in rates],
     }
     if plain:
         assert ps_p == ps_b, "plain and batched sets diverged"
@@ -282,6 +298,7 @@ def served_leg(fg, ctx, a):
         "scene_frames": s.frames,
         "peak_resident": s.peak_resident,
         "peak_resident_gb": s.peak_resident_bytes / 1e9,
+        "lanes": ctx.batch_lanes(),
     }
 
 
@@ -363,7 +380,8 @@ def main():
                         f"4-8 ppc ({b['particles']} particles, {b['cells']} cells), FP32 state, FP64 PCG "
                         f"reductions + refinement to rel resid {a.tol:g}; one step = one frame of every scene",
             "scenes": a.scenes, "master_seed": a.master_seed, "solver_tol": a.tol,
-            "substeps": b["substeps"],
+            "substeps": b["substeps"], "lanes": b["lanes"],
+            "autotune_rates": b["autotune_rates"],
         },
         "p50_frame_ms": b["p50_frame_ms"],
         "p99_frame_ms": b["p99_frame_ms"],

@@ -30,8 +30,10 @@ EXPORTS = [
     "fg_scene_substep", "fg_scene_step_frame", "fg_scene_stage", "fg_batch_step_frame", "fg_poisson_solve",
     "fg_ctx_profile_kernel", "fg_ctx_profile_read", "fg_ctx_bandwidth_probe",
     "fg_server_create", "fg_server_destroy", "fg_server_submit", "fg_server_tick", "fg_server_status",
-    "fg_server_scene", "fg_server_release", "fg_server_summary_get",
+    "fg_server_scene", "fg_server_release", "fg_server_summary_get", "fg_ctx_set_batch_lanes",
+    "fg_ctx_batch_lanes", "fg_batch_autotune",
 ]
+MAX_LANES = 4
 REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_REJECTED, REQ_RELEASED = range(5)
 (K_ADVECT, K_SORT, K_P2G, K_SDF, K_LEVELSET, K_EXTRAPOLATE, K_PCG_A, K_PCG_B, K_G2P, K_RHS, K_APPLY) = range(11)
 
@@ -147,8 +149,11 @@ def lib():
             "fg_batch_step_frame": (C.c_int, [vp, vp, i32, vp]),
             "fg_poisson_solve": (C.c_int, [vp, i32, i32, vp, f64, vp, vp, f64, i32, C.POINTER(SolveStats)]),
             "fg_ctx_profile_kernel": (C.c_int, [vp, i32]),
-            "fg_ctx_profile_read": (C.c_int, [vp, C.POINTER(f64), C.POINTER(i64), C.POINTER(i64)]),
+            "fg_ctx_profile_read": (C.c_int, [vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64), C.POINTER(i64)]),
             "fg_ctx_bandwidth_probe": (C.c_int, [vp, C.c_uint64, i32, C.POINTER(f64), C.POINTER(f64)]),
+            "fg_ctx_set_batch_lanes": (C.c_int, [vp, i32]),
+            "fg_ctx_batch_lanes": (C.c_int, [vp, C.POINTER(i32)]),
+            "fg_batch_autotune": (C.c_int, [vp, vp, i32, i32, C.POINTER(i32), vp]),
             "fg_server_create": (C.c_int, [vp, C.c_uint64, i32, C.POINTER(vp)]),
             "fg_server_destroy": (C.c_int, [vp]),
             "fg_server_submit": (C.c_int, [vp, C.POINTER(Request), C.POINTER(i64)]),
@@ -204,6 +209,15 @@ class Context:
     def set_timing(self, on=True):
         _check(lib().fg_ctx_set_timing(self.h, 1 if on else 0))
 
+    def set_batch_lanes(self, lanes):
+        """Split batched frames over `lanes` streams (size classes), one host thread each."""
+        _check(lib().fg_ctx_set_batch_lanes(self.h, lanes))
+
+    def batch_lanes(self):
+        v = C.c_int32()
+        _check(lib().fg_ctx_batch_lanes(self.h, C.byref(v)))
+        return v.value
+
     def bytes_in_use(self):
         v = C.c_uint64()
         _check(lib().fg_ctx_bytes_in_use(self.h, C.byref(v)))
@@ -213,9 +227,10 @@ class Context:
         _check(lib().fg_ctx_profile_kernel(self.h, kernel))
 
     def profile_read(self):
-        t, n, u = C.c_double(), C.c_int64(), C.c_int64()
-        _check(lib().fg_ctx_profile_read(self.h, C.byref(t), C.byref(n), C.byref(u)))
-        return t.value, n.value, u.value
+        """(summed launch ms, busy ms = union of launch intervals, launches, work units)"""
+        t, b, n, u = C.c_double(), C.c_double(), C.c_int64(), C.c_int64()
+        _check(lib().fg_ctx_profile_read(self.h, C.byref(t), C.byref(b), C.byref(n), C.byref(u)))
+        return t.value, b.value, n.value, u.value
 
     def bandwidth_probe(self, nbytes=1 << 30, reps=5):
         g, ms = C.c_double(), C.c_double()
@@ -363,6 +378,17 @@ def batch_step_frame(ctx, scenes):
     stats = (Stats * n)()
     _check(lib().fg_batch_step_frame(ctx.h, arr, n, stats))
     return list(stats)
+
+
+def batch_autotune(ctx, scenes, rounds=1):
+    """Measure each lane count on `rounds` frames of the batch (the scenes
+    advance), set the fastest on the context; returns (lanes, rates)."""
+    n = len(scenes)
+    arr = (C.c_void_p * n)(*[s.h.value for s in scenes])
+    best = C.c_int32()
+    rates = np.zeros(MAX_LANES, np.float64)
+    _check(lib().fg_batch_autotune(ctx.h, arr, n, rounds, C.byref(best), rates.ctypes.data))
+    return best.value, rates
 
 
 MIXED_RESOLUTIONS = (64, 96, 128, 192, 256)

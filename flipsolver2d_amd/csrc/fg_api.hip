@@ -15,6 +15,7 @@
 #include <limits>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fg_kernels.h"
@@ -54,28 +55,39 @@ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 }  // namespace
 
+// One HIP stream with the buffers its launch groups use.  A context owns
+// lane 0 (its own stream); a batch may be split over more lanes, each
+// stepped by its own host thread, so the scenes of one lane overlap on the
+// device with those of the others.
+struct Lane {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int bcap = 0;
+    SceneDev* bdesc = nullptr;      // a group's descriptor array
+    SceneDev* bdesc_host = nullptr;
+    SceneDev* bsub = nullptr;       // a sub-group's descriptors (density correction)
+    SceneDev* bsub_host = nullptr;
+    SceneDev* bpack = nullptr;      // iterating scenes of a solve, packed
+    SceneScalars* bsc = nullptr;    // gathered scalar blocks
+    SceneScalars* bsc_host = nullptr;
+    int32_t* bflags = nullptr;
+    int32_t* bflags_host = nullptr;
+    // per-launch kernel timing
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    int64_t prof_units = 0;
+};
+
 struct fg_ctx {
     int device = 0;
     uint64_t budget = 0;
     uint64_t used = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // lane 0's stream
     bool timing = false;
-    // per-launch kernel timing
-    int prof_kernel = -1;
-    std::vector<hipEvent_t> ev_pool;
-    size_t ev_used = 0;
-    int64_t prof_units = 0;
-    // launch groups of several scenes: descriptor array, scalar gather, flags
-    int bcap = 0;
-    SceneDev* bdesc = nullptr;
-    SceneDev* bdesc_host = nullptr;
-    SceneScalars* bsc = nullptr;
-    SceneScalars* bsc_host = nullptr;
-    int32_t* bflags = nullptr;
-    int32_t* bflags_host = nullptr;
-    SceneDev* bpack = nullptr;      // iterating scenes of a solve, packed
-    SceneDev* bsub = nullptr;       // a sub-group's descriptors (density correction)
-    SceneDev* bsub_host = nullptr;
+    int prof_kernel = -1;           // kernel family timed per launch, -1: none
+    hipEvent_t prof_ref = nullptr;  // recorded when timing starts, before any timed launch
+    int nlanes = 1;                 // lanes a batch is split over
+    std::vector<Lane*> lanes;
 };
 
 struct fg_scene {
@@ -434,6 +446,7 @@ __global__ void k_pack_iterating(const SceneDev* in, int n, SceneDev* out, int32
 // ---------------------------------------------------------------------------
 struct Group {
     fg_ctx* ctx = nullptr;
+    Lane* L = nullptr;
     std::vector<fg_scene*> sc;
     std::vector<fg_stats*> st;     // per scene, entries may be NULL
     const SceneDev* d = nullptr;   // device descriptor array [n]
@@ -444,51 +457,85 @@ struct Group {
     int max_bands = 0;
 };
 
-int ensure_group_buffers(fg_ctx* ctx, int n)
+void lane_free_buffers(Lane* L)
 {
-    if (n <= ctx->bcap) return FG_OK;
-    const int cap = std::max(n, 2 * ctx->bcap);
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (ctx->bdesc) hipFree(ctx->bdesc);
-    if (ctx->bdesc_host) hipHostFree(ctx->bdesc_host);
-    if (ctx->bsc) hipFree(ctx->bsc);
-    if (ctx->bsc_host) hipHostFree(ctx->bsc_host);
-    if (ctx->bpack) hipFree(ctx->bpack);
-    if (ctx->bsub) hipFree(ctx->bsub);
-    if (ctx->bsub_host) hipHostFree(ctx->bsub_host);
-    ctx->bpack = nullptr;
-    ctx->bsub = nullptr;
-    ctx->bsub_host = nullptr;
-    ctx->bdesc = nullptr;
-    ctx->bdesc_host = nullptr;
-    ctx->bsc = nullptr;
-    ctx->bsc_host = nullptr;
-    ctx->bcap = 0;
-    HIPCHK(hipMalloc(&ctx->bdesc, sizeof(SceneDev) * cap));
-    HIPCHK(hipHostMalloc((void**)&ctx->bdesc_host, sizeof(SceneDev) * cap, hipHostMallocDefault));
-    HIPCHK(hipMalloc(&ctx->bsc, sizeof(SceneScalars) * cap));
-    HIPCHK(hipMalloc(&ctx->bpack, sizeof(SceneDev) * cap));
-    HIPCHK(hipMalloc(&ctx->bsub, sizeof(SceneDev) * cap));
-    HIPCHK(hipHostMalloc((void**)&ctx->bsub_host, sizeof(SceneDev) * cap, hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void**)&ctx->bsc_host, sizeof(SceneScalars) * cap, hipHostMallocDefault));
-    ctx->bcap = cap;
+    if (L->bdesc) hipFree(L->bdesc);
+    if (L->bdesc_host) hipHostFree(L->bdesc_host);
+    if (L->bsub) hipFree(L->bsub);
+    if (L->bsub_host) hipHostFree(L->bsub_host);
+    if (L->bpack) hipFree(L->bpack);
+    if (L->bsc) hipFree(L->bsc);
+    if (L->bsc_host) hipHostFree(L->bsc_host);
+    L->bdesc = L->bdesc_host = L->bsub = L->bsub_host = L->bpack = nullptr;
+    L->bsc = L->bsc_host = nullptr;
+    L->bcap = 0;
+}
+
+void lane_destroy(Lane* L)
+{
+    if (!L) return;
+    if (L->stream) hipStreamSynchronize(L->stream);
+    lane_free_buffers(L);
+    if (L->bflags) hipFree(L->bflags);
+    if (L->bflags_host) hipHostFree(L->bflags_host);
+    for (hipEvent_t e : L->ev_pool) hipEventDestroy(e);
+    if (L->own_stream && L->stream) hipStreamDestroy(L->stream);
+    delete L;
+}
+
+int lane_create(hipStream_t stream, Lane** out)
+{
+    auto* L = new Lane();
+    hipError_t e = hipSuccess;
+    if (stream) {
+        L->stream = stream;
+    } else {
+        e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking);
+        L->own_stream = e == hipSuccess;
+    }
+    if (e == hipSuccess) e = hipMalloc(&L->bflags, sizeof(int32_t) * 4);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&L->bflags_host, sizeof(int32_t) * 4, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        lane_destroy(L);
+        return fail(FG_E_DEVICE, std::string("lane: ") + hipGetErrorString(e));
+    }
+    *out = L;
+    return FG_OK;
+}
+
+int ensure_group_buffers(Lane* L, int n)
+{
+    if (n <= L->bcap) return FG_OK;
+    const int cap = std::max(n, 2 * L->bcap);
+    HIPCHK(hipStreamSynchronize(L->stream));
+    lane_free_buffers(L);
+    HIPCHK(hipMalloc(&L->bdesc, sizeof(SceneDev) * cap));
+    HIPCHK(hipHostMalloc((void**)&L->bdesc_host, sizeof(SceneDev) * cap, hipHostMallocDefault));
+    HIPCHK(hipMalloc(&L->bsub, sizeof(SceneDev) * cap));
+    HIPCHK(hipHostMalloc((void**)&L->bsub_host, sizeof(SceneDev) * cap, hipHostMallocDefault));
+    HIPCHK(hipMalloc(&L->bpack, sizeof(SceneDev) * cap));
+    HIPCHK(hipMalloc(&L->bsc, sizeof(SceneScalars) * cap));
+    HIPCHK(hipHostMalloc((void**)&L->bsc_host, sizeof(SceneScalars) * cap, hipHostMallocDefault));
+    L->bcap = cap;
     return FG_OK;
 }
 
 // Build a group and upload its descriptors.  A single scene uses its own
 // device descriptor; several share the context's descriptor array.
 int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vector<fg_stats*>& stats, Group& G,
-               bool sub = false)
+               bool sub = false, Lane* lane = nullptr)
 {
     G = Group{};
     G.ctx = ctx;
+    G.L = lane ? lane : ctx->lanes[0];
+    Lane* L = G.L;
     G.sc = scenes;
     G.st = stats;
     G.st.resize(scenes.size(), nullptr);
     G.n = (int)scenes.size();
     if (G.n == 0) return FG_OK;
     // the pinned staging buffers may still feed an earlier copy
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
     bool first = true;
     for (fg_scene* s : scenes) {
         s->h.active = 1;
@@ -507,15 +554,15 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
     if (G.n == 1) {
         fg_scene* s = scenes[0];
         *s->hpin = s->h;
-        HIPCHK(hipMemcpyAsync(s->d, s->hpin, sizeof(SceneDev), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(s->d, s->hpin, sizeof(SceneDev), hipMemcpyHostToDevice, L->stream));
         G.d = s->d;
         return FG_OK;
     }
-    FGCHK(ensure_group_buffers(ctx, G.n));
-    SceneDev* dev = sub ? ctx->bsub : ctx->bdesc;
-    SceneDev* host = sub ? ctx->bsub_host : ctx->bdesc_host;
+    FGCHK(ensure_group_buffers(L, G.n));
+    SceneDev* dev = sub ? L->bsub : L->bdesc;
+    SceneDev* host = sub ? L->bsub_host : L->bdesc_host;
     for (int k = 0; k < G.n; k++) host[k] = scenes[k]->h;
-    HIPCHK(hipMemcpyAsync(dev, host, sizeof(SceneDev) * G.n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dev, host, sizeof(SceneDev) * G.n, hipMemcpyHostToDevice, L->stream));
     G.d = dev;
     return FG_OK;
 }
@@ -523,22 +570,27 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
 // Read every scene's scalar block into its pinned host mirror (synchronises).
 int read_group_scalars(Group& G)
 {
-    fg_ctx* ctx = G.ctx;
-    if (G.n == 1) return read_scalars(G.sc[0]);
-    hipLaunchKernelGGL(k_gather_scalars, dim3((unsigned)cdiv(G.n, 64)), dim3(64), 0, ctx->stream, G.d, G.n, ctx->bsc);
-    HIPCHK(hipMemcpyAsync(ctx->bsc_host, ctx->bsc, sizeof(SceneScalars) * G.n, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    for (int k = 0; k < G.n; k++) *G.sc[k]->hsc = ctx->bsc_host[k];
+    Lane* L = G.L;
+    if (G.n == 1) {
+        fg_scene* s = G.sc[0];
+        HIPCHK(hipMemcpyAsync(s->hsc, s->h.sc, sizeof(SceneScalars), hipMemcpyDeviceToHost, L->stream));
+        HIPCHK(hipStreamSynchronize(L->stream));
+        return FG_OK;
+    }
+    hipLaunchKernelGGL(k_gather_scalars, dim3((unsigned)cdiv(G.n, 64)), dim3(64), 0, L->stream, G.d, G.n, L->bsc);
+    HIPCHK(hipMemcpyAsync(L->bsc_host, L->bsc, sizeof(SceneScalars) * G.n, hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    for (int k = 0; k < G.n; k++) *G.sc[k]->hsc = L->bsc_host[k];
     return FG_OK;
 }
 
 int poll_group(Group& G, int band, int32_t* flags)
 {
-    fg_ctx* ctx = G.ctx;
-    hipLaunchKernelGGL(k_group_flags, dim3(1), dim3(256), 0, ctx->stream, G.d, G.n, band, ctx->bflags);
-    HIPCHK(hipMemcpyAsync(ctx->bflags_host, ctx->bflags, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    for (int k = 0; k < 3; k++) flags[k] = ctx->bflags_host[k];
+    Lane* L = G.L;
+    hipLaunchKernelGGL(k_group_flags, dim3(1), dim3(256), 0, L->stream, G.d, G.n, band, L->bflags);
+    HIPCHK(hipMemcpyAsync(L->bflags_host, L->bflags, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    for (int k = 0; k < 3; k++) flags[k] = L->bflags_host[k];
     return FG_OK;
 }
 
@@ -546,13 +598,13 @@ int poll_group(Group& G, int band, int32_t* flags)
 // the launch grid over them (synchronises).
 int pack_iterating(Group& G, const SceneDev** d, dim3* grid, int* count)
 {
-    fg_ctx* ctx = G.ctx;
-    hipLaunchKernelGGL(k_pack_iterating, dim3(1), dim3(64), 0, ctx->stream, G.d, G.n, ctx->bpack, ctx->bflags);
-    HIPCHK(hipMemcpyAsync(ctx->bflags_host, ctx->bflags, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    *count = ctx->bflags_host[0];
-    *d = ctx->bpack;
-    *grid = dim3((unsigned)std::max(1, ctx->bflags_host[1]), (unsigned)std::max(1, *count));
+    Lane* L = G.L;
+    hipLaunchKernelGGL(k_pack_iterating, dim3(1), dim3(64), 0, L->stream, G.d, G.n, L->bpack, L->bflags);
+    HIPCHK(hipMemcpyAsync(L->bflags_host, L->bflags, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    *count = L->bflags_host[0];
+    *d = L->bpack;
+    *grid = dim3((unsigned)std::max(1, L->bflags_host[1]), (unsigned)std::max(1, *count));
     return FG_OK;
 }
 
@@ -567,7 +619,7 @@ struct Timer {
     {
         if (ctx->timing) {
             hipEventCreate(&a);
-            hipEventRecord(a, ctx->stream);
+            hipEventRecord(a, G->L->stream);
         }
     }
     // the group's time for the stage is charged to every scene in it
@@ -576,7 +628,7 @@ struct Timer {
         if (!ctx->timing) return;
         hipEvent_t b;
         hipEventCreate(&b);
-        hipEventRecord(b, ctx->stream);
+        hipEventRecord(b, G->L->stream);
         hipEventSynchronize(b);
         float ms = 0;
         hipEventElapsedTime(&ms, a, b);
@@ -595,32 +647,33 @@ struct Timer {
 // per-launch kernel timing (events bracket each launch on its own stream)
 // ---------------------------------------------------------------------------
 struct KProf {
-    fg_ctx* ctx;
+    Lane* L;
     bool on, single;
     // single: the events are attached to the next kernel launch itself
     // (hipExtLaunchKernelGGL) and bracket exactly that kernel; otherwise they
     // are recorded around the whole multi-kernel stage.
-    KProf(fg_ctx* c, int kernel, bool single_kernel = false) : ctx(c), on(c->prof_kernel == kernel), single(single_kernel)
+    KProf(const Group& G, int kernel, bool single_kernel = false)
+        : L(G.L), on(G.ctx->prof_kernel == kernel), single(single_kernel)
     {
         if (!on) return;
         hipEvent_t e0 = take(), e1 = take();
         if (single) next_launch_events() = LaunchEvents{e0, e1};
-        else hipEventRecord(e0, ctx->stream);
+        else hipEventRecord(e0, L->stream);
     }
     hipEvent_t take()
     {
-        if (ctx->ev_used == ctx->ev_pool.size()) {
+        if (L->ev_used == L->ev_pool.size()) {
             hipEvent_t e;
             hipEventCreate(&e);
-            ctx->ev_pool.push_back(e);
+            L->ev_pool.push_back(e);
         }
-        return ctx->ev_pool[ctx->ev_used++];
+        return L->ev_pool[L->ev_used++];
     }
     void done(int64_t units)
     {
         if (!on) return;
-        if (!single) hipEventRecord(ctx->ev_pool[ctx->ev_used - 1], ctx->stream);
-        ctx->prof_units += units;
+        if (!single) hipEventRecord(L->ev_pool[L->ev_used - 1], L->stream);
+        L->prof_units += units;
     }
 };
 
@@ -632,11 +685,11 @@ struct KProf {
 int run_solve(Group& G, bool rhs_from_grid)
 {
     fg_ctx* ctx = G.ctx;
-    hipStream_t st = ctx->stream;
+    hipStream_t st = G.L->stream;
     const Geo& g = G.g;
     launch_solve_reset(g.one, st, G.d);
     if (rhs_from_grid) {
-        KProf kp(ctx, FG_K_RHS, true);
+        KProf kp(G, FG_K_RHS, true);
         launch_rhs(g.pcg, st, G.d);
         kp.done(G.cells);
     } else {
@@ -665,11 +718,11 @@ int run_solve(Group& G, bool rhs_from_grid)
         while (count > 0) {
             for (int k = 0; k < kPoll; k++, it++) {
                 {
-                    KProf kp(ctx, FG_K_PCG_A, true);
+                    KProf kp(G, FG_K_PCG_A, true);
                     launch_pcg_a(git, st, dit, it);
                     kp.done(0);
                 }
-                KProf kp(ctx, FG_K_PCG_B, true);
+                KProf kp(G, FG_K_PCG_B, true);
                 launch_pcg_b(git, st, dit, it);
                 kp.done(0);
                 launched++;
@@ -686,7 +739,7 @@ int run_solve(Group& G, bool rhs_from_grid)
     FGCHK(read_group_scalars(G));
     // work units of the launches that iterated (the rest exited at once)
     if (ctx->prof_kernel == FG_K_PCG_A || ctx->prof_kernel == FG_K_PCG_B)
-        for (fg_scene* s : G.sc) ctx->prof_units += (int64_t)s->hsc->total_iters * s->C;
+        for (fg_scene* s : G.sc) G.L->prof_units += (int64_t)s->hsc->total_iters * s->C;
     for (fg_scene* s : G.sc)
         if (s->hsc->numeric_bad) return fail(FG_E_NUMERIC, "NaN/Inf in the pressure solve");
     (void)launched;
@@ -699,19 +752,19 @@ int run_solve(Group& G, bool rhs_from_grid)
 int run_stage(Group& G, int stage)
 {
     fg_ctx* ctx = G.ctx;
-    hipStream_t st = ctx->stream;
+    hipStream_t st = G.L->stream;
     const Geo& g = G.g;
     const SceneDev* d = G.d;
     switch (stage) {
     case FG_STAGE_ADVECT: {
-        KProf kp(ctx, FG_K_ADVECT, true);
+        KProf kp(G, FG_K_ADVECT, true);
         launch_advect(g.parts, st, d, 1);
         kp.done(G.parts);
         break;
     }
     case FG_STAGE_REBIN: {
         launch_stencil(g.cells, st, d);          // matrix from the pre-update materials
-        KProf kp(ctx, FG_K_SORT);
+        KProf kp(G, FG_K_SORT);
         launch_scan_local(g.scan, st, d);
         launch_scan_top(g.scan1, st, d);
         launch_scan_add(g.scan, st, d);
@@ -734,7 +787,7 @@ int run_stage(Group& G, int stage)
             }
         if (ds.empty()) break;
         Group D;
-        FGCHK(make_group(ctx, ds, dst, D, true));
+        FGCHK(make_group(ctx, ds, dst, D, true, G.L));
         launch_density_rhs(D.g.cells, st, D.d);
         FGCHK(run_solve(D, false));
         for (int k = 0; k < D.n; k++)
@@ -751,19 +804,19 @@ int run_stage(Group& G, int stage)
         break;
     }
     case FG_STAGE_P2G: {
-        KProf kp(ctx, FG_K_P2G, true);
+        KProf kp(G, FG_K_P2G, true);
         launch_p2g(g.cells, st, d);
         kp.done(G.cells);
         break;
     }
     case FG_STAGE_SDF: {
-        KProf kp(ctx, FG_K_SDF, true);
+        KProf kp(G, FG_K_SDF, true);
         launch_sdf_materials(g.bins, st, d);
         kp.done(G.cells);
         break;
     }
     case FG_STAGE_LEVELSET: {
-        KProf kp(ctx, FG_K_LEVELSET);
+        KProf kp(G, FG_K_LEVELSET);
         launch_levelset_init(g.cells, st, d);
         constexpr int kChunk = 8;
         int32_t flags[3];
@@ -777,7 +830,7 @@ int run_stage(Group& G, int stage)
         break;
     }
     case FG_STAGE_EXTRAPOLATE: {
-        KProf kp(ctx, FG_K_EXTRAPOLATE);
+        KProf kp(G, FG_K_EXTRAPOLATE);
         launch_extrapolate(g.extrap, st, d, 0);  // u -> su (saved)
         kp.done(G.faces);
         break;
@@ -797,7 +850,7 @@ int run_stage(Group& G, int stage)
             if (stats->pressure_rel_resid > G.sc[k]->h.tol) stats->converged = 0;
         }
         FGCHK(r);
-        KProf kp(ctx, FG_K_APPLY, true);
+        KProf kp(G, FG_K_APPLY, true);
         launch_apply_solids(g.cells, st, d);
         kp.done(G.cells);
         break;
@@ -807,11 +860,11 @@ int run_stage(Group& G, int stage)
         break;
     case FG_STAGE_G2P: {
         {
-            KProf kp(ctx, FG_K_EXTRAPOLATE);
+            KProf kp(G, FG_K_EXTRAPOLATE);
             launch_extrapolate(g.extrap, st, d, 1);  // u -> uf (final)
             kp.done(G.faces);
         }
-        KProf kp(ctx, FG_K_G2P, true);
+        KProf kp(G, FG_K_G2P, true);
         launch_g2p_count(g.parts, st, d);
         kp.done(G.parts);
         launch_finalize(g.one, st, d, 2);
@@ -850,14 +903,14 @@ void commit_substep(Group& G, float* dts)
 }
 
 // One FlipSolver::step for every scene of `scenes`, each with its own dt.
-int substep_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vector<float>& dts,
+int substep_group(fg_ctx* ctx, Lane* L, const std::vector<fg_scene*>& scenes, const std::vector<float>& dts,
                   const std::vector<fg_stats*>& stats)
 {
     for (size_t k = 0; k < scenes.size(); k++) set_substep_scalars(scenes[k], dts[k]);
     Group G;
-    FGCHK(make_group(ctx, scenes, stats, G));
+    FGCHK(make_group(ctx, scenes, stats, G, false, L));
     if (G.n == 0) return FG_OK;
-    hipLaunchKernelGGL(k_substep_begin, G.g.one, dim3(64), 0, ctx->stream, G.d);
+    hipLaunchKernelGGL(k_substep_begin, G.g.one, dim3(64), 0, G.L->stream, G.d);
     Timer tm(ctx, &G);
     static const int order[] = {FG_STAGE_ADVECT, FG_STAGE_REBIN, FG_STAGE_DENSITY, FG_STAGE_P2G,
                                 FG_STAGE_SDF, FG_STAGE_LEVELSET, FG_STAGE_EXTRAPOLATE, FG_STAGE_FORCES,
@@ -874,16 +927,16 @@ int substep_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::
 }
 
 // max |v|^2 for scenes whose last substep did not leave it on the device
-int ensure_vmax(fg_ctx* ctx, const std::vector<fg_scene*>& scenes)
+int ensure_vmax(fg_ctx* ctx, Lane* L, const std::vector<fg_scene*>& scenes)
 {
     std::vector<fg_scene*> need;
     for (fg_scene* s : scenes)
         if (!s->vmax_ready) need.push_back(s);
     if (need.empty()) return FG_OK;
     Group G;
-    FGCHK(make_group(ctx, need, {}, G));
-    launch_max_v2(G.g.parts, ctx->stream, G.d);
-    launch_finalize(G.g.one, ctx->stream, G.d, 2);
+    FGCHK(make_group(ctx, need, {}, G, false, L));
+    launch_max_v2(G.g.parts, G.L->stream, G.d);
+    launch_finalize(G.g.one, G.L->stream, G.d, 2);
     FGCHK(read_group_scalars(G));
     for (fg_scene* s : need) s->vmax_ready = true;
     return FG_OK;
@@ -891,15 +944,14 @@ int ensure_vmax(fg_ctx* ctx, const std::vector<fg_scene*>& scenes)
 
 // FlipSolver::stepFrame (flipsolver2d.cpp:464-500) for every scene: each
 // scene splits its frame by its own CFL dt; the scenes still inside their
-// frame advance together, one launch per phase.
-int step_frames(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vector<fg_stats*>& stats)
+// frame advance together on lane L, one launch per phase.
+int step_frames_lane(fg_ctx* ctx, Lane* L, const std::vector<fg_scene*>& scenes, const std::vector<fg_stats*>& stats,
+                     std::chrono::steady_clock::time_point t0)
 {
-    const auto t0 = std::chrono::steady_clock::now();
     const size_t n = scenes.size();
     std::vector<float> t(n, 0.f);
     std::vector<int> count(n, 0);
     std::vector<char> finished(n, 0);
-    for (size_t k = 0; k < n; k++) stats_reset(stats[k]);
     for (;;) {
         std::vector<fg_scene*> act;
         std::vector<fg_stats*> ast;
@@ -911,7 +963,7 @@ int step_frames(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::ve
                 idx.push_back(k);
             }
         if (act.empty()) break;
-        FGCHK(ensure_vmax(ctx, act));
+        FGCHK(ensure_vmax(ctx, L, act));
         std::vector<float> dts(act.size());
         for (size_t a = 0; a < act.size(); a++) {
             const size_t k = idx[a];
@@ -931,7 +983,7 @@ int step_frames(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::ve
             if (stats[k]) stats[k]->max_velocity = vel;
             dts[a] = mss;
         }
-        FGCHK(substep_group(ctx, act, dts, ast));
+        FGCHK(substep_group(ctx, L, act, dts, ast));
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         for (size_t a = 0; a < act.size(); a++) {
             const size_t k = idx[a];
@@ -944,6 +996,76 @@ int step_frames(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::ve
             }
         }
     }
+    return FG_OK;
+}
+
+int ensure_lanes(fg_ctx* ctx, int n)
+{
+    while ((int)ctx->lanes.size() < n) {
+        Lane* L = nullptr;
+        FGCHK(lane_create(nullptr, &L));
+        ctx->lanes.push_back(L);
+    }
+    return FG_OK;
+}
+
+// Split a batch into size classes, one lane each: scenes sorted by cell
+// count, cut into contiguous runs of about equal work (cells^1.5 per scene,
+// the PCG cost of a 2-D Poisson solve with this preconditioner).  The
+// largest scenes get a lane of their own, so the smaller ones no longer wait
+// on their solves' latency-bound tails.
+std::vector<std::vector<size_t>> lane_plan(const std::vector<fg_scene*>& scenes, int lanes)
+{
+    const size_t n = scenes.size();
+    std::vector<size_t> order(n);
+    for (size_t k = 0; k < n; k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return scenes[a]->C > scenes[b]->C; });
+    double total = 0;
+    std::vector<double> cost(n);
+    for (size_t k = 0; k < n; k++) total += (cost[k] = std::pow((double)scenes[k]->C, 1.5));
+    std::vector<std::vector<size_t>> plan(1);
+    double acc = 0;
+    for (size_t r = 0; r < n; r++) {
+        const size_t k = order[r];
+        const int lane = (int)plan.size() - 1;
+        if (!plan.back().empty() && lane + 1 < lanes && acc >= total * (lane + 1) / lanes &&
+            n - r >= (size_t)(lanes - 1 - lane))
+            plan.emplace_back();
+        plan.back().push_back(k);
+        acc += cost[k];
+    }
+    return plan;
+}
+
+// Every scene one frame: on lane 0 alone, or split over the context's lanes
+// (one host thread each).  Results do not depend on the split.
+int step_frames(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vector<fg_stats*>& stats)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    for (fg_stats* st : stats) stats_reset(st);
+    const int lanes = std::min<int>(ctx->nlanes, (int)scenes.size());
+    if (lanes <= 1) return step_frames_lane(ctx, ctx->lanes[0], scenes, stats, t0);
+    FGCHK(ensure_lanes(ctx, lanes));
+    const auto plan = lane_plan(scenes, lanes);
+    std::vector<int> rc(plan.size(), FG_OK);
+    std::vector<std::string> msg(plan.size());
+    auto run = [&](size_t li) {
+        hipSetDevice(ctx->device);
+        std::vector<fg_scene*> sc;
+        std::vector<fg_stats*> st;
+        for (size_t k : plan[li]) {
+            sc.push_back(scenes[k]);
+            st.push_back(stats[k]);
+        }
+        rc[li] = step_frames_lane(ctx, ctx->lanes[li], sc, st, t0);
+        if (rc[li] != FG_OK) msg[li] = g_err;
+    };
+    std::vector<std::thread> workers;
+    for (size_t li = 1; li < plan.size(); li++) workers.emplace_back(run, li);
+    run(0);
+    for (std::thread& w : workers) w.join();
+    for (size_t li = 0; li < plan.size(); li++)
+        if (rc[li] != FG_OK) return fail(rc[li], msg[li]);
     return FG_OK;
 }
 
@@ -1009,13 +1131,14 @@ int fg_ctx_create(int32_t device, uint64_t hbm_budget_bytes, fg_ctx** out)
     auto* c = new fg_ctx();
     c->device = device;
     c->budget = hbm_budget_bytes;
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&c->bflags, sizeof(int32_t) * 4);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&c->bflags_host, sizeof(int32_t) * 4, hipHostMallocDefault);
-    if (e != hipSuccess) {
-        fg_ctx_destroy(c);
-        return fail(FG_E_DEVICE, std::string("context: ") + hipGetErrorString(e));
+    Lane* L = nullptr;
+    const int rc = lane_create(nullptr, &L);
+    if (rc != FG_OK) {
+        delete c;
+        return rc;
     }
+    c->lanes.push_back(L);
+    c->stream = L->stream;
     *out = c;
     return FG_OK;
 }
@@ -1023,20 +1146,8 @@ int fg_ctx_create(int32_t device, uint64_t hbm_budget_bytes, fg_ctx** out)
 int fg_ctx_destroy(fg_ctx* ctx)
 {
     if (!ctx) return FG_OK;
-    if (ctx->stream) {
-        hipStreamSynchronize(ctx->stream);
-        hipStreamDestroy(ctx->stream);
-    }
-    for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
-    if (ctx->bdesc) hipFree(ctx->bdesc);
-    if (ctx->bdesc_host) hipHostFree(ctx->bdesc_host);
-    if (ctx->bsc) hipFree(ctx->bsc);
-    if (ctx->bsc_host) hipHostFree(ctx->bsc_host);
-    if (ctx->bpack) hipFree(ctx->bpack);
-    if (ctx->bsub) hipFree(ctx->bsub);
-    if (ctx->bsub_host) hipHostFree(ctx->bsub_host);
-    if (ctx->bflags) hipFree(ctx->bflags);
-    if (ctx->bflags_host) hipHostFree(ctx->bflags_host);
+    for (Lane* L : ctx->lanes) lane_destroy(L);
+    if (ctx->prof_ref) hipEventDestroy(ctx->prof_ref);
     delete ctx;
     return FG_OK;
 }
@@ -1059,28 +1170,68 @@ int fg_ctx_profile_kernel(fg_ctx* ctx, int32_t kernel)
 {
     if (!ctx) return fail(FG_E_INVALID, "ctx is NULL");
     if (kernel < -1 || kernel >= FG_K_NUM) return fail(FG_E_INVALID, "bad kernel id");
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (Lane* L : ctx->lanes) {
+        HIPCHK(hipStreamSynchronize(L->stream));
+        L->ev_used = 0;
+        L->prof_units = 0;
+    }
+    if (!ctx->prof_ref) HIPCHK(hipEventCreate(&ctx->prof_ref));
+    HIPCHK(hipEventRecord(ctx->prof_ref, ctx->lanes[0]->stream));
+    HIPCHK(hipStreamSynchronize(ctx->lanes[0]->stream));
     ctx->prof_kernel = kernel;
-    ctx->ev_used = 0;
-    ctx->prof_units = 0;
     return FG_OK;
 }
 
-int fg_ctx_profile_read(fg_ctx* ctx, double* total_ms, int64_t* launches, int64_t* work_units)
+int fg_ctx_set_batch_lanes(fg_ctx* ctx, int32_t lanes)
 {
     if (!ctx) return fail(FG_E_INVALID, "ctx is NULL");
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (lanes < 1 || lanes > FG_MAX_LANES) return fail(FG_E_INVALID, "lanes must be in [1, FG_MAX_LANES]");
+    FGCHK(ensure_lanes(ctx, lanes));
+    ctx->nlanes = lanes;
+    return FG_OK;
+}
+
+int fg_ctx_profile_read(fg_ctx* ctx, double* total_ms, double* busy_ms, int64_t* launches, int64_t* work_units)
+{
+    if (!ctx) return fail(FG_E_INVALID, "ctx is NULL");
     double tot = 0;
-    for (size_t k = 0; k + 1 < ctx->ev_used; k += 2) {
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, ctx->ev_pool[k], ctx->ev_pool[k + 1]));
-        tot += ms;
+    int64_t n = 0, units = 0;
+    std::vector<std::pair<double, double>> iv;   // launch intervals after prof_ref
+    for (Lane* L : ctx->lanes) {
+        HIPCHK(hipStreamSynchronize(L->stream));
+        for (size_t k = 0; k + 1 < L->ev_used; k += 2) {
+            float ms = 0, a = 0, b = 0;
+            HIPCHK(hipEventElapsedTime(&ms, L->ev_pool[k], L->ev_pool[k + 1]));
+            tot += ms;
+            if (ctx->prof_ref) {
+                HIPCHK(hipEventElapsedTime(&a, ctx->prof_ref, L->ev_pool[k]));
+                HIPCHK(hipEventElapsedTime(&b, ctx->prof_ref, L->ev_pool[k + 1]));
+                iv.emplace_back(a, b);
+            }
+        }
+        n += (int64_t)(L->ev_used / 2);
+        units += L->prof_units;
+        L->ev_used = 0;
+        L->prof_units = 0;
     }
+    // time during which at least one timed launch ran (launches of several
+    // lanes overlap)
+    std::sort(iv.begin(), iv.end());
+    double busy = 0, cs = 0, ce = -1;
+    for (const auto& p : iv) {
+        if (p.first > ce) {
+            if (ce > cs) busy += ce - cs;
+            cs = p.first;
+            ce = p.second;
+        } else {
+            ce = std::max(ce, p.second);
+        }
+    }
+    if (ce > cs) busy += ce - cs;
     if (total_ms) *total_ms = tot;
-    if (launches) *launches = (int64_t)(ctx->ev_used / 2);
-    if (work_units) *work_units = ctx->prof_units;
-    ctx->ev_used = 0;
-    ctx->prof_units = 0;
+    if (busy_ms) *busy_ms = busy;
+    if (launches) *launches = n;
+    if (work_units) *work_units = units;
     return FG_OK;
 }
 
@@ -1359,7 +1510,7 @@ int fg_scene_substep(fg_scene* s, float dt, fg_stats* stats)
     if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
     if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FG_E_INVALID, "dt must be positive and finite");
     stats_reset(stats);
-    return substep_group(s->ctx, {s}, {dt}, {stats});
+    return substep_group(s->ctx, s->ctx->lanes[0], {s}, {dt}, {stats});
 }
 
 int fg_scene_step_frame(fg_scene* s, fg_stats* stats)
@@ -1409,6 +1560,44 @@ int fg_batch_step_frame(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, fg_stat
         if (stats) st[k] = &stats[k];
     }
     return step_frames(ctx, v, st);
+}
+
+int fg_ctx_batch_lanes(fg_ctx* ctx, int32_t* lanes)
+{
+    if (!ctx || !lanes) return fail(FG_E_INVALID, "NULL argument");
+    *lanes = ctx->nlanes;
+    return FG_OK;
+}
+
+int fg_batch_autotune(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, int32_t rounds, int32_t* lanes_out,
+                      double* rates_out)
+{
+    if (!ctx || !scenes || n < 1 || rounds < 1) return fail(FG_E_INVALID, "bad argument");
+    std::vector<double> ms(FG_MAX_LANES + 1, 0.0), work(FG_MAX_LANES + 1, 0.0);
+    std::vector<fg_stats> st(n);
+    const int saved = ctx->nlanes;
+    for (int r = 0; r < rounds; r++)
+        for (int q = 0; q < FG_MAX_LANES; q++) {
+            const int lanes = (r & 1) ? FG_MAX_LANES - q : q + 1;   // alternate the order
+            int rc = fg_ctx_set_batch_lanes(ctx, lanes);
+            const auto t0 = std::chrono::steady_clock::now();
+            if (rc == FG_OK) rc = fg_batch_step_frame(ctx, scenes, n, st.data());
+            if (rc != FG_OK) {
+                ctx->nlanes = saved;
+                return rc;
+            }
+            ms[lanes] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            for (const fg_stats& s : st) work[lanes] += (double)s.particle_substeps;
+        }
+    int best = 1;
+    for (int l = 1; l <= FG_MAX_LANES; l++) {
+        const double rate = ms[l] > 0 ? work[l] / (ms[l] * 1e-3) : 0.0;
+        if (rates_out) rates_out[l - 1] = rate;
+        if (rate > work[best] / (ms[best] * 1e-3)) best = l;
+    }
+    ctx->nlanes = best;
+    if (lanes_out) *lanes_out = best;
+    return FG_OK;
 }
 
 int fg_poisson_solve(fg_ctx* ctx, int32_t size_i, int32_t size_j, const int8_t* material, double scale,

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FG_ABI_VERSION 1
+#define FG_ABI_VERSION 2
 
 enum fg_status {
     FG_OK = 0,
@@ -143,18 +143,36 @@ int fg_ctx_bytes_in_use(fg_ctx* ctx, uint64_t* out);
 /* Per-stage HIP-event timing for later calls (adds synchronisation). */
 int fg_ctx_set_timing(fg_ctx* ctx, int32_t enabled);
 
+/* fg_batch_step_frame may split a batch over up to FG_MAX_LANES lanes:
+ * streams of this context, each stepped by its own host thread, with the
+ * scenes cut by size into contiguous classes of about equal solver work.
+ * Small scenes then overlap the latency-bound solve tails of large ones.
+ * Results do not depend on the number of lanes.  Default 1.  No counterpart
+ * in the reference (its thread pool steps one solver at a time). */
+#define FG_MAX_LANES 4
+int fg_ctx_set_batch_lanes(fg_ctx* ctx, int32_t lanes);
+int fg_ctx_batch_lanes(fg_ctx* ctx, int32_t* lanes);
+/* Autotuner for that schedule: steps `rounds` frames of the batch with each
+ * lane count 1..FG_MAX_LANES (order alternating per round; the simulation
+ * advances normally), measures particle-substeps per second for each, and
+ * sets the fastest on the context.  *lanes_out gets it; rates_out (may be
+ * NULL) gets FG_MAX_LANES measured rates. */
+int fg_batch_autotune(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, int32_t rounds, int32_t* lanes_out,
+                      double* rates_out);
+
 /* Per-launch HIP-event timing of one kernel family, recorded on the stream
  * the kernel runs on (the context's stream).  kernel = fg_kernel_id, or -1
  * to switch off.  fg_ctx_profile_read returns the summed event time, the
  * number of timed launches and the summed work units (cells for cell
  * kernels, particles for particle kernels) of launches that did work, then
- * resets the counters. */
+ * resets the counters.  busy_ms is the union of the launch intervals: the
+ * time some timed launch was running (launches on several lanes overlap). */
 enum fg_kernel_id {
     FG_K_ADVECT = 0, FG_K_SORT = 1, FG_K_P2G = 2, FG_K_SDF = 3, FG_K_LEVELSET = 4, FG_K_EXTRAPOLATE = 5,
     FG_K_PCG_A = 6, FG_K_PCG_B = 7, FG_K_G2P = 8, FG_K_RHS = 9, FG_K_APPLY = 10, FG_K_NUM = 11
 };
 int fg_ctx_profile_kernel(fg_ctx* ctx, int32_t kernel);
-int fg_ctx_profile_read(fg_ctx* ctx, double* total_ms, int64_t* launches, int64_t* work_units);
+int fg_ctx_profile_read(fg_ctx* ctx, double* total_ms, double* busy_ms, int64_t* launches, int64_t* work_units);
 /* Device-to-device copy bandwidth (read + write bytes / time) of a plain
  * copy kernel over `bytes` per buffer, best of `reps`. */
 int fg_ctx_bandwidth_probe(fg_ctx* ctx, uint64_t bytes, int32_t reps, double* gbps, double* ms_per_copy);

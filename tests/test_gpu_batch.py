@@ -44,20 +44,24 @@ def test_mixed_specs_are_deterministic(fg):
         assert abs(np.hypot(*sp["gravity"]) - 9.81) < 1e-9
 
 
-@pytest.mark.parametrize("count,frames", [(6, 2), (13, 1)])
-def test_batch_equals_solo(fg, ctx, count, frames):
+@pytest.mark.parametrize("count,frames,lanes", [(6, 2, 1), (13, 1, 1), (13, 2, 2), (9, 2, 3), (7, 1, 4)])
+def test_batch_equals_solo(fg, ctx, count, frames, lanes):
     specs = fg.mixed_scene_specs(count, master_seed=11 + count, resolutions=(32, 48, 64, 80))
-    _log(f"batch vs solo: {count} scenes {[s['n'] for s in specs]}, {frames} frames")
+    _log(f"batch vs solo: {count} scenes {[s['n'] for s in specs]}, {frames} frames, {lanes} lanes")
     solo = _scenes(fg, ctx, specs)
     batch = _scenes(fg, ctx, specs)
-    for f in range(frames):
-        s_stats = [sc.step_frame() for sc in solo]
-        b_stats = fg.batch_step_frame(ctx, batch)
-        for k in range(count):
-            assert s_stats[k].substeps == b_stats[k].substeps, (f, k)
-            assert s_stats[k].pressure_iters == b_stats[k].pressure_iters, (f, k)
-            assert s_stats[k].particle_substeps == b_stats[k].particle_substeps, (f, k)
-            assert b_stats[k].converged == 1
+    ctx.set_batch_lanes(lanes)
+    try:
+        for f in range(frames):
+            s_stats = [sc.step_frame() for sc in solo]
+            b_stats = fg.batch_step_frame(ctx, batch)
+            for k in range(count):
+                assert s_stats[k].substeps == b_stats[k].substeps, (f, k)
+                assert s_stats[k].pressure_iters == b_stats[k].pressure_iters, (f, k)
+                assert s_stats[k].particle_substeps == b_stats[k].particle_substeps, (f, k)
+                assert b_stats[k].converged == 1
+    finally:
+        ctx.set_batch_lanes(1)
     for k in range(count):
         _same(solo[k].get_state(), batch[k].get_state(), k)
     for sc in solo + batch:
@@ -86,3 +90,23 @@ def test_batch_rejects_duplicates(fg, ctx):
     with pytest.raises(fg.FgError):
         fg.batch_step_frame(ctx, [sc, sc])
     sc.close()
+
+
+def test_autotune_keeps_results(fg, ctx):
+    """The lane autotuner steps real frames; the scenes stay equal to solo."""
+    specs = fg.mixed_scene_specs(8, master_seed=77, resolutions=(32, 48, 64, 96))
+    solo = _scenes(fg, ctx, specs)
+    batch = _scenes(fg, ctx, specs)
+    try:
+        lanes, rates = fg.batch_autotune(ctx, batch, rounds=1)
+        assert 1 <= lanes <= fg.MAX_LANES and ctx.batch_lanes() == lanes
+        assert np.all(rates > 0)
+        for _ in range(fg.MAX_LANES):
+            for sc in solo:
+                sc.step_frame()
+        for k in range(len(specs)):
+            _same(solo[k].get_state(), batch[k].get_state(), k)
+    finally:
+        ctx.set_batch_lanes(1)
+    for sc in solo + batch:
+        sc.close()
