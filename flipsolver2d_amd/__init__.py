@@ -31,7 +31,7 @@ EXPORTS = [
     "fg_ctx_profile_kernel", "fg_ctx_profile_read", "fg_ctx_bandwidth_probe",
     "fg_server_create", "fg_server_destroy", "fg_server_submit", "fg_server_tick", "fg_server_status",
     "fg_server_scene", "fg_server_release", "fg_server_summary_get", "fg_ctx_set_batch_lanes",
-    "fg_ctx_batch_lanes", "fg_batch_autotune",
+    "fg_ctx_batch_lanes", "fg_batch_autotune", "fg_ctx_set_solver_tiling",
 ]
 MAX_LANES = 4
 REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_REJECTED, REQ_RELEASED = range(5)
@@ -153,6 +153,7 @@ def lib():
             "fg_ctx_bandwidth_probe": (C.c_int, [vp, C.c_uint64, i32, C.POINTER(f64), C.POINTER(f64)]),
             "fg_ctx_set_batch_lanes": (C.c_int, [vp, i32]),
             "fg_ctx_batch_lanes": (C.c_int, [vp, C.POINTER(i32)]),
+            "fg_ctx_set_solver_tiling": (C.c_int, [vp, i32, i32]),
             "fg_batch_autotune": (C.c_int, [vp, vp, i32, i32, C.POINTER(i32), vp]),
             "fg_server_create": (C.c_int, [vp, C.c_uint64, i32, C.POINTER(vp)]),
             "fg_server_destroy": (C.c_int, [vp]),
@@ -212,6 +213,9 @@ class Context:
     def set_batch_lanes(self, lanes):
         """Split batched frames over `lanes` streams (size classes), one host thread each."""
         _check(lib().fg_ctx_set_batch_lanes(self.h, lanes))
+
+    def set_solver_tiling(self, target_blocks=1024, min_chunk=1024):
+        _check(lib().fg_ctx_set_solver_tiling(self.h, target_blocks, min_chunk))
 
     def batch_lanes(self):
         v = C.c_int32()

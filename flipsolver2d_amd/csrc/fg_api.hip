@@ -87,6 +87,8 @@ struct fg_ctx {
     int prof_kernel = -1;           // kernel family timed per launch, -1: none
     hipEvent_t prof_ref = nullptr;  // recorded when timing starts, before any timed launch
     int nlanes = 1;                 // lanes a batch is split over
+    int pcg_target_blocks = kPcgMaxBlocks;   // solver tiling: blocks per scene at most
+    int pcg_min_chunk = kPcgMinChunk;        // and cells per block at least
     std::vector<Lane*> lanes;
 };
 
@@ -96,8 +98,9 @@ struct fg_scene {
     bool solver_only = false;
     int I = 0, J = 0;
     int64_t C = 0, FU = 0, FV = 0, cap = 0;
-    int max_blocks = 0;     // solver blocks (= FP64 partials stride)
-    int pcg_chunk = 0;
+    int max_blocks = 0;     // FP64 partials stride: solver blocks at the finest tiling
+    int pcg_chunk = 0;      // current tiling (set per launch group from the context)
+    int pcg_blocks = 0;
     int max_pblocks = 0;    // particle blocks at capacity
     void* arena = nullptr;
     uint64_t bytes = 0;
@@ -216,9 +219,12 @@ void init_geometry_fields(fg_scene* s)
     s->cap = s->solver_only ? 0 : s->p.max_particles;
     // solver blocks own contiguous chunks of whole 256-cell groups
     const int64_t groups = cdiv(s->C, kBlock);
-    const int64_t want = std::min<int64_t>(groups, kPcgMaxBlocks);
-    s->pcg_chunk = (int)std::max<int64_t>(kPcgMinChunk, cdiv(groups, want) * kBlock);
-    s->max_blocks = (int)cdiv(s->C, s->pcg_chunk);
+    // partials for the finest tiling the context may choose
+    const int64_t want = std::min<int64_t>(groups, kPcgTilingMaxBlocks);
+    const int64_t finest = std::max<int64_t>(kPcgTilingMinChunk, cdiv(groups, want) * kBlock);
+    s->max_blocks = (int)cdiv(s->C, finest);
+    s->pcg_chunk = (int)finest;
+    s->pcg_blocks = s->max_blocks;
     s->max_pblocks = (int)std::max<int64_t>(1, cdiv(s->cap, kBlock));
 }
 
@@ -287,7 +293,7 @@ int create_scene(fg_ctx* ctx, const fg_scene_params* p, bool solver_only, fg_sce
     h.cell_vol = (float)(p->dx * p->dx * p->dx);
     h.pmass = (float)((p->fluid_density * h.cell_vol) / (float)p->particles_per_cell);
     h.max_blocks = s->max_blocks;
-    h.pcg_blocks = s->max_blocks;
+    h.pcg_blocks = s->pcg_blocks;
     h.pcg_chunk = s->pcg_chunk;
     h.max_pblocks = s->max_pblocks;
     h.pblocks = 1;
@@ -334,7 +340,7 @@ Geo geometry(const fg_scene* s)
     g.extrap = dim3((unsigned)std::max(tu, tv), 1, 2);
     g.ls = dim3((unsigned)(cdiv(s->I, kBfsTile) * cdiv(s->J, kBfsTile)));
     g.one = dim3(1);
-    g.pcg = dim3((unsigned)s->max_blocks);
+    g.pcg = dim3((unsigned)s->pcg_blocks);
     g.bins = dim3((unsigned)cdiv(cdiv(s->I, 3) * cdiv(s->J, 3), kBlock));
     return g;
 }
@@ -503,6 +509,25 @@ int lane_create(hipStream_t stream, Lane** out)
     return FG_OK;
 }
 
+// Solver blocks own contiguous chunks of whole 256-cell groups: at most
+// pcg_target_blocks per scene, pcg_min_chunk cells each at least.
+void apply_tiling(const fg_ctx* ctx, fg_scene* s)
+{
+    const int64_t groups = cdiv(s->C, kBlock);
+    const int64_t want = std::max<int64_t>(1, std::min<int64_t>(groups, ctx->pcg_target_blocks));
+    int64_t chunk = std::max<int64_t>(ctx->pcg_min_chunk, cdiv(groups, want) * kBlock);
+    int64_t blocks = cdiv(s->C, chunk);
+    if (blocks > s->max_blocks) {   // finer than the partials allow
+        chunk = cdiv(s->C, s->max_blocks);
+        chunk = cdiv(chunk, kBlock) * kBlock;
+        blocks = cdiv(s->C, chunk);
+    }
+    s->pcg_chunk = (int)chunk;
+    s->pcg_blocks = (int)blocks;
+    s->h.pcg_chunk = s->pcg_chunk;
+    s->h.pcg_blocks = s->pcg_blocks;
+}
+
 int ensure_group_buffers(Lane* L, int n)
 {
     if (n <= L->bcap) return FG_OK;
@@ -538,6 +563,7 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
     HIPCHK(hipStreamSynchronize(L->stream));
     bool first = true;
     for (fg_scene* s : scenes) {
+        apply_tiling(ctx, s);
         s->h.active = 1;
         s->h.pblocks = (int)std::max<int64_t>(1, cdiv(std::max<int64_t>(s->n_bound, 1), kBlock));
         const Geo gs = geometry(s);
@@ -1560,6 +1586,16 @@ int fg_batch_step_frame(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, fg_stat
         if (stats) st[k] = &stats[k];
     }
     return step_frames(ctx, v, st);
+}
+
+int fg_ctx_set_solver_tiling(fg_ctx* ctx, int32_t target_blocks, int32_t min_chunk)
+{
+    if (!ctx) return fail(FG_E_INVALID, "ctx is NULL");
+    if (target_blocks < 1 || target_blocks > kPcgTilingMaxBlocks) return fail(FG_E_INVALID, "bad target_blocks");
+    if (min_chunk < kPcgTilingMinChunk || min_chunk % kBlock) return fail(FG_E_INVALID, "bad min_chunk");
+    ctx->pcg_target_blocks = target_blocks;
+    ctx->pcg_min_chunk = min_chunk;
+    return FG_OK;
 }
 
 int fg_ctx_batch_lanes(fg_ctx* ctx, int32_t* lanes)

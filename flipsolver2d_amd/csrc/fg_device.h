@@ -17,8 +17,10 @@ constexpr int kMaxSweeps = 8;            // FP64 refinement sweeps per solve
 constexpr int kMaxBands = 512;           // level-set bands of 16 levels
 constexpr int kSlots = 5;                // FP64 reduction slots
 constexpr int kSlotBB = 0, kSlotPQ = 1, kSlotCG = 2 /* and 3: r.z, r.r */, kSlotRR64 = 4;
-constexpr int kPcgMaxBlocks = 1024;      // solver blocks per scene (contiguous cell chunks)
-constexpr int kPcgMinChunk = 1024;       // cells per solver block at least (4 per thread)
+constexpr int kPcgMaxBlocks = 1024;      // default solver blocks per scene (contiguous cell chunks)
+constexpr int kPcgMinChunk = 1024;       // default cells per solver block at least (4 per thread)
+constexpr int kPcgTilingMaxBlocks = 4096;  // finest tiling a context may select
+constexpr int kPcgTilingMinChunk = 256;
 
 // Material codes and tests, FlipSolver2dLib/materialgrid.h:6-20.
 enum : int8_t { M_FLUID = 0x40, M_SOURCE = 0x41, M_SOLID = 0x20, M_SINK = 0x12, M_EMPTY = 0x10 };

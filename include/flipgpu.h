@@ -152,6 +152,11 @@ int fg_ctx_set_timing(fg_ctx* ctx, int32_t enabled);
 #define FG_MAX_LANES 4
 int fg_ctx_set_batch_lanes(fg_ctx* ctx, int32_t lanes);
 int fg_ctx_batch_lanes(fg_ctx* ctx, int32_t* lanes);
+/* Solver tiling: each scene's solver kernels use at most target_blocks
+ * blocks (1..4096) of at least min_chunk cells (a multiple of 256, >= 256).
+ * Default 1024 / 1024.  Results are deterministic for a given tiling; the
+ * FP64 reductions' summation order depends on it. */
+int fg_ctx_set_solver_tiling(fg_ctx* ctx, int32_t target_blocks, int32_t min_chunk);
 /* Autotuner for that schedule: steps `rounds` frames of the batch with each
  * lane count 1..FG_MAX_LANES (order alternating per round; the simulation
  * advances normally), measures particle-substeps per second for each, and
