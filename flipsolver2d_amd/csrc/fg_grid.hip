@@ -191,39 +191,62 @@ constexpr int kTile = 32;
 constexpr int16_t kNone = 0x7fff;     // outside the grid
 constexpr int16_t kUnreached = 0x7ffe;
 
+// Levels are expanded as row bitmasks (the region is at most 64 cells
+// wide): one wave holds one region row per lane, dilates the frontier of
+// level L with shifts and lane shuffles, and removes the reached set, so the
+// expansion costs a few instructions per row and level and needs no
+// barrier.  The value pass then visits only the cells of each level.
 template <int H, bool LEVELSET>
 __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mkin, float* gout, uint8_t* vout,
                          int32_t* mkout, int si, int sj, int L0, int* more)
 {
     constexpr int R = kTile + 2 * H;
-    constexpr int T = 16;   // threads as a 16 x 16 layout: no index divisions in the passes
+    static_assert(R <= 64, "a region row must fit one 64-bit mask");
+    constexpr int T = 16;   // threads as a 16 x 16 layout for the load and store passes
     __shared__ int16_t mk[R * R];
     __shared__ float val[R * R];
+    __shared__ unsigned long long front[H + 1][R];   // front[s]: region cells at level L0 + s
+    __shared__ unsigned long long open_s[R];
+    __shared__ int16_t pre[H + 1][64];                // row offsets of level L0+s's cells
+    __shared__ int16_t tot[H + 1];
+    __shared__ int top_s;
     const int tx = threadIdx.x & (T - 1), ty = threadIdx.x >> 4;
     const int tilesJ = (sj + kTile - 1) / kTile;
     const int tile = blockIdx.x;
     const int ti = tile / tilesJ, tj = tile - (tile / tilesJ) * tilesJ;
     const int gi0 = ti * kTile - H, gj0 = tj * kTile - H;
-    bool has_front = false, has_open = false;
-    for (int ri = ty; ri < R; ri += T)
-        for (int rj = tx; rj < R; rj += T) {
+    // the region: every load of the thread issued before any is used
+    constexpr int NA = (R + T - 1) / T;
+    float xr[NA][NA];
+    int lr[NA][NA];
+#pragma unroll
+    for (int a = 0; a < NA; a++)
+#pragma unroll
+        for (int b = 0; b < NA; b++) {
+            const int ri = ty + a * T, rj = tx + b * T;
             const int gi = gi0 + ri, gj = gj0 + rj;
+            const bool ok = ri < R && rj < R && gi >= 0 && gi < si && gj >= 0 && gj < sj;
+            const size_t g = ok ? (size_t)gi * sj + gj : 0;
+            xr[a][b] = ok ? gin[g] : 0.f;
+            if (LEVELSET) lr[a][b] = ok ? mkin[g] : -1;
+            else lr[a][b] = ok ? (int)vin[g] : -1;
+        }
+    bool has_front = false, has_open = false;
+#pragma unroll
+    for (int a = 0; a < NA; a++)
+#pragma unroll
+        for (int b = 0; b < NA; b++) {
+            const int ri = ty + a * T, rj = tx + b * T;
+            if (ri >= R || rj >= R) continue;
             int16_t m = kNone;
-            float x = 0.f;
-            if (gi >= 0 && gi < si && gj >= 0 && gj < sj) {
-                const size_t g = (size_t)gi * sj + gj;
-                x = gin[g];
-                if (LEVELSET) {
-                    const int lv = mkin[g];
-                    m = lv <= L0 ? (int16_t)lv : kUnreached;
-                } else {
-                    m = vin[g] ? 0 : kUnreached;
-                }
+            if (lr[a][b] >= 0) {
+                if (LEVELSET) m = lr[a][b] <= L0 ? (int16_t)lr[a][b] : kUnreached;
+                else m = lr[a][b] ? 0 : kUnreached;
             }
             has_front |= m == L0;
             has_open |= m == kUnreached;
             mk[ri * R + rj] = m;
-            val[ri * R + rj] = x;
+            val[ri * R + rj] = xr[a][b];
         }
     // a tile cell reaches level L0+s only through a level-L0 cell within
     // distance s <= H, i.e. inside this region
@@ -242,52 +265,107 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
         }
         return;
     }
-    // expand levels L0+1 .. L0+H from the level-L0 frontier
-    int top = L0;
-    for (int st = 1; st <= H; st++) {
-        const int16_t prev = (int16_t)(L0 + st - 1), cur = (int16_t)(L0 + st);
-        bool changed = false;
-        for (int ri = ty; ri < R; ri += T)
-            for (int rj = tx; rj < R; rj += T) {
-                if (mk[ri * R + rj] != kUnreached) continue;
-                bool hit = false;
+    // row masks of the open (unreached) cells and of level L0, one ballot
+    // per region row (4 waves, consecutive LDS addresses)
+    {
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int r = w; r < R; r += kBlock / 64) {
+            const int16_t m = lane < R ? mk[r * R + lane] : kNone;
+            const unsigned long long o = __ballot(m == kUnreached);
+            const unsigned long long f = __ballot(m == (int16_t)L0);
+            if (lane == 0) {
+                open_s[r] = o;
+                front[0][r] = f;
+            }
+        }
+    }
+    __syncthreads();
+    // expand levels L0+1 .. L0+H: wave 0, lane = region row; per level also
+    // the row offsets of its cells (exclusive scan of the row counts)
+    if (threadIdx.x < 64) {
+        const int r = threadIdx.x;
+        unsigned long long open = r < R ? open_s[r] : 0ull, f = r < R ? front[0][r] : 0ull;
+        int top = 0;
+        for (int st = 1; st <= H; st++) {
+            const unsigned long long d = f | (f << 1) | (f >> 1);
+            const unsigned long long up = __shfl_up(d, 1, 64), dn = __shfl_down(d, 1, 64);
+            unsigned long long nf = d;
+            if (r > 0) nf |= up;
+            if (r < R - 1) nf |= dn;
+            nf &= open;
+            if (r >= R) nf = 0;
+            open &= ~nf;
+            if (r < R) front[st][r] = nf;
+            const int cnt = __popcll(nf);
+            int incl = cnt;
 #pragma unroll
-                for (int di = -1; di <= 1; di++)
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(incl, o, 64);
+                if (r >= o) incl += t;
+            }
+            pre[st][r] = (int16_t)(incl - cnt);
+            if (r == 63) tot[st] = (int16_t)incl;
+            f = nf;
+            if (!__any(nf != 0ull)) break;
+            top = st;
+        }
+        if (r == 0) top_s = top;
+    }
+    __syncthreads();
+    const int top = top_s;
+    // values, level by level: the cells of a level spread over all threads
+    // (cell t of the level: the row whose offset range holds t, then the
+    // (t - offset)-th set bit of that row's mask)
+    for (int st = 1; st <= top; st++) {
+        const int L = L0 + st;
+        const int n = tot[st];
+        for (int t = threadIdx.x; t < n; t += kBlock) {
+            int lo = 0, hi = 63;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pre[st][mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            const int ri = lo;
+            int k = t - pre[st][ri];
+            unsigned long long m = front[st][ri];
+            int rj = 0;
 #pragma unroll
-                    for (int dj = -1; dj <= 1; dj++) {
-                        const int qi = ri + di, qj = rj + dj;
-                        if ((di == 0 && dj == 0) || qi < 0 || qi >= R || qj < 0 || qj >= R) continue;
-                        hit |= mk[qi * R + qj] == prev;
-                    }
-                if (hit) {
-                    mk[ri * R + rj] = cur;
-                    changed = true;
+            for (int w = 32; w >= 1; w >>= 1) {
+                const unsigned long long low = m & ((1ull << w) - 1ull);
+                const int c = __popcll(low);
+                if (k >= c) {
+                    k -= c;
+                    m >>= w;
+                    rj += w;
+                } else {
+                    m = low;
                 }
             }
-        if (!__syncthreads_or(changed)) break;
-        top = L0 + st;
-    }
-    // values, level by level
-    for (int L = L0 + 1; L <= top; L++) {
-        for (int ri = ty; ri < R; ri += T)
-            for (int rj = tx; rj < R; rj += T) {
-                if (mk[ri * R + rj] != L) continue;
-                double avg = 0;
-                int count = 0;
+            // the 8 neighbours in getNeighborhood order, loads first
+            int16_t qm[8];
+            float qv[8];
 #pragma unroll
-                for (int di = -1; di <= 1; di++)
-#pragma unroll
-                    for (int dj = -1; dj <= 1; dj++) {
-                        const int qi = ri + di, qj = rj + dj;
-                        if ((di == 0 && dj == 0) || qi < 0 || qi >= R || qj < 0 || qj >= R) continue;
-                        if (mk[qi * R + qj] < L) {
-                            avg += (double)val[qi * R + qj];
-                            count++;
-                        }
-                    }
-                if (LEVELSET) val[ri * R + rj] = (float)(avg / count - 1.f);
-                else val[ri * R + rj] = (float)(avg / count);
+            for (int e = 0; e < 8; e++) {
+                const int di = (e < 3) ? -1 : (e < 5 ? 0 : 1);
+                const int dj = (e < 3) ? e - 1 : (e < 5 ? (e == 3 ? -1 : 1) : e - 6);
+                const int qi = ri + di, qj = rj + dj;
+                const bool ok = qi >= 0 && qi < R && qj >= 0 && qj < R;
+                qm[e] = ok ? mk[qi * R + qj] : kNone;
+                qv[e] = ok ? val[qi * R + qj] : 0.f;
             }
+            double avg = 0;
+            int count = 0;
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                if (qm[e] < L) {
+                    avg += (double)qv[e];
+                    count++;
+                }
+            if (LEVELSET) val[ri * R + rj] = (float)(avg / count - 1.f);
+            else val[ri * R + rj] = (float)(avg / count);
+            mk[ri * R + rj] = (int16_t)L;   // same-level neighbours read L or unreached: not below L either way
+        }
         __syncthreads();
     }
     // write the tile
