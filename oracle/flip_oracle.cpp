@@ -1004,11 +1004,16 @@ void orc_ipp_ref_apply(const int8_t* m, const uint8_t* fluid, int32_t I, int32_t
     ipp_ref_mt(1, m, fluid, I, J, scale, x, y);
 }
 
-/* Symmetric IPP, the preconditioner the reference defines in
- * inversepoissonpreconditioner.h:45-79 -- M^-1 = (I - L D^-1)(I - L D^-1)^T --
- * restricted to A's 5-point pattern (no fill-in).  For a fluid row c:
+/* The inverse-Poisson preconditioner as inversepoissonpreconditioner.h:45-79
+ * defines it: M = K K^T with K = I - L D^-1 (L the strictly lower part of A,
+ * D its diagonal), the full sparse product.  K has 1 on the diagonal and
+ * 1/n_k at a fluid row's lower fluid neighbours k (mask bits, lower =
+ * smaller linear index).  For a fluid row c:
  *   z_c = (1 + sum_lower (1/n_k)^2) r_c + sum_lower r_k / n_k + sum_upper r_u / n_c
- * over fluid neighbours (mask bits); lower = smaller linear index. */
+ *       + [i-(c) and j-(c-J+1)] r_{c-J+1} / n_{c-J}^2
+ *       + [j-(c) and i-(c+J-1)] r_{c+J-1} / n_{c-1}^2
+ * the last two being the fill-in at the anti-diagonal neighbours (rows c
+ * and d share the lower neighbour e). */
 void orc_ipp_apply(const uint8_t* n, const uint8_t* mask, const uint8_t* fluid, int32_t I, int32_t J,
                    const double* x, double* y)
 {
@@ -1032,6 +1037,15 @@ void orc_ipp_apply(const uint8_t* n, const uint8_t* mask, const uint8_t* fluid, 
             }
         for (int d = 0; d < 2; d++)
             if (mask[l] & upb[d]) off += wc * x[up[d]];
+        // fill-in: (c-J+1) shares c-J, (c+J-1) shares c-1
+        if ((mask[l] & 1) && l - J + 1 < C && (mask[l - J + 1] & 4)) {
+            const double we = n[l - J] ? 1.0 / n[l - J] : 0.0;
+            off += we * we * x[l - J + 1];
+        }
+        if ((mask[l] & 4) && l + J - 1 < C && (mask[l + J - 1] & 1)) {
+            const double we = n[l - 1] ? 1.0 / n[l - 1] : 0.0;
+            off += we * we * x[l + J - 1];
+        }
         y[l] = diag * x[l] + off;
     }
 }

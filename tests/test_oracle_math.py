@@ -315,8 +315,9 @@ def test_matrix_times_ones_counts_empty_neighbours():
                 assert y[i * J + j] == pytest.approx(s * empties)
 
 
-def test_symmetric_ipp_is_the_reference_definition():
-    """M = (I - L D^-1)(I - L D^-1)^T (inversepoissonpreconditioner.h:45-79) on A's pattern."""
+def test_ipp_is_the_reference_definition():
+    """M = (I - L D^-1)(I - L D^-1)^T (inversepoissonpreconditioner.h:45-79), the
+    full sparse product, against dense linear algebra; also SPD."""
     rng = np.random.default_rng(5)
     I, J, s = 8, 9, 0.81
     for _ in range(4):
@@ -327,8 +328,6 @@ def test_symmetric_ipp_is_the_reference_definition():
         Lo = np.tril(Af, -1)
         K = np.eye(len(Af)) - Lo @ np.diag(1.0 / np.diag(Af))
         Mfull = K @ K.T
-        pattern = (Af != 0)
-        Mpat = np.where(pattern, Mfull, 0.0)
         Mo = np.zeros((I * J, I * J))
         e = np.zeros(I * J)
         y = np.zeros(I * J)
@@ -337,8 +336,10 @@ def test_symmetric_ipp_is_the_reference_definition():
             e[k] = 1
             L.orc_ipp_apply(n, mask, fl, I, J, e, y)
             Mo[:, k] = y
-        assert np.allclose(Mo[np.ix_(f, f)], Mpat, atol=1e-12)
+        assert np.allclose(Mo[np.ix_(f, f)], Mfull, atol=1e-12)
+        assert np.abs(Mfull - np.where(Af != 0, Mfull, 0)).max() > 0   # fill-in present
         assert np.allclose(Mo[np.ix_(~f, ~f)], np.eye(int((~f).sum())))
+        assert np.linalg.eigvalsh(Mo[np.ix_(f, f)]).min() > 0
 
 
 def test_converged_solver_vs_dense():
