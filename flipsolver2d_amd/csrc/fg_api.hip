@@ -323,7 +323,7 @@ void stats_reset(fg_stats* st)
 // launch geometry
 // --------------------------------------------------------------------------
 struct Geo {
-    dim3 cells, parts, scan, scan1, faces, extrap, ls, one, pcg, bins;
+    dim3 cells, parts, scan, scan1, faces, extrap, ls, one, pcg, bins, p2g;
 };
 
 // x extents of one scene's launches (y is set by the launch group)
@@ -342,19 +342,21 @@ Geo geometry(const fg_scene* s)
     g.one = dim3(1);
     g.pcg = dim3((unsigned)s->pcg_blocks);
     g.bins = dim3((unsigned)cdiv(cdiv(s->I, 3) * cdiv(s->J, 3), kBlock));
+    g.p2g = dim3((unsigned)(cdiv(s->I, 16) * cdiv(s->J, 16)));
     return g;
 }
 
 void geo_max(Geo& a, const Geo& b)
 {
-    dim3* pa[] = {&a.cells, &a.parts, &a.scan, &a.scan1, &a.faces, &a.extrap, &a.ls, &a.one, &a.pcg, &a.bins};
-    const dim3* pb[] = {&b.cells, &b.parts, &b.scan, &b.scan1, &b.faces, &b.extrap, &b.ls, &b.one, &b.pcg, &b.bins};
-    for (int k = 0; k < 10; k++) pa[k]->x = std::max(pa[k]->x, pb[k]->x);
+    dim3* pa[] = {&a.cells, &a.parts, &a.scan, &a.scan1, &a.faces, &a.extrap, &a.ls, &a.one, &a.pcg, &a.bins, &a.p2g};
+    const dim3* pb[] = {&b.cells, &b.parts, &b.scan, &b.scan1, &b.faces, &b.extrap, &b.ls, &b.one, &b.pcg, &b.bins,
+                        &b.p2g};
+    for (int k = 0; k < 11; k++) pa[k]->x = std::max(pa[k]->x, pb[k]->x);
 }
 
 void geo_set_y(Geo& a, int n)
 {
-    dim3* pa[] = {&a.cells, &a.parts, &a.scan, &a.scan1, &a.faces, &a.extrap, &a.ls, &a.one, &a.pcg, &a.bins};
+    dim3* pa[] = {&a.cells, &a.parts, &a.scan, &a.scan1, &a.faces, &a.extrap, &a.ls, &a.one, &a.pcg, &a.bins, &a.p2g};
     for (dim3* d : pa) d->y = (unsigned)n;
 }
 
@@ -831,7 +833,7 @@ int run_stage(Group& G, int stage)
     }
     case FG_STAGE_P2G: {
         KProf kp(G, FG_K_P2G, true);
-        launch_p2g(g.cells, st, d);
+        launch_p2g(g.p2g, st, d);
         kp.done(G.cells);
         break;
     }

@@ -165,15 +165,20 @@ __device__ __forceinline__ float2 velocity_at(const float* u, const float* v, in
     return make_float2(lerp_grid(u, I + 1, J, 0.5f, 0.f, x, y), lerp_grid(v, I, J + 1, 0.f, 0.5f, x, y));
 }
 
-// simmath::bSpline / quadraticBSpline, mathfuncs.cpp:32-43
+// simmath::bSpline / quadraticBSpline, mathfuncs.cpp:32-43.  The reference
+// multiplies each piece by a 0/1 factor and adds them; exactly one piece
+// survives (beyond 1.5 both are zero and the sum is +0), so selecting the
+// piece gives the same float bit for bit for every |value| < 2^64 (checked
+// exhaustively on the host; beyond, value^2 overflows and the reference's
+// form gives NaN) at a third of the instructions.  bSpline(0) = 0.75.
 __device__ __forceinline__ float bspline(float value)
 {
     value = fabsf(value);
-    const float t1 = (0.75f - value * value) * (float)(value < 0.5f);
-    const float t2 = ((0.5f * (1.5f - value)) * (1.5f - value)) * (float)(value >= 0.5f && value < 1.5f);
-    return t1 + t2;
+    const float t1 = 0.75f - value * value;
+    const float t2 = (0.5f * (1.5f - value)) * (1.5f - value);
+    return value < 0.5f ? t1 : (value < 1.5f ? t2 : 0.0f);
 }
-__device__ __forceinline__ float qbspline(float x, float y) { return (bspline(x) * bspline(y)) * bspline(0.f); }
+__device__ __forceinline__ float qbspline(float x, float y) { return (bspline(x) * bspline(y)) * 0.75f; }
 
 // simmath::linearHat / bilinearHat, mathfuncs.cpp:61-77
 __device__ __forceinline__ float linear_hat(float v)

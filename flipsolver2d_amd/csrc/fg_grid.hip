@@ -56,35 +56,116 @@ __global__ void __launch_bounds__(kBlock) k_stencil(const SceneDev* scenes)
 // contiguous (cell, id)-ordered particle ranges.  Deterministic gather, no
 // atomics.  U row I and V column J are never written by the reference's
 // cell loop and stay 0 / invalid.
+//
+// A block owns a 16 x 16 tile of cells and first stages the positions of
+// the particles of the 18 x 18-cell window around it in LDS: in (cell, id)
+// order each window row is one contiguous segment, so the loads are
+// coalesced and every position is read from HBM about (18/16)^2 times
+// instead of nine.  Each cell then sums over its 3 x 3 cells from LDS in the
+// same order as before (bit-exact); velocities are read only for particles
+// that pass the weight gate.  The capacity holds 2*ppc = 16 particles per
+// window cell at 8 ppc; a fuller window reads positions from global memory.
 // ---------------------------------------------------------------------------
+constexpr int kP2gTile = 16;
+constexpr int kP2gCap = (kP2gTile + 2) * (kP2gTile + 2) * 16;   // 5184 positions (41.5 KiB)
+
 __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
-    const int I = s.I, J = s.J, C = I * J;
-    const int c = cell_index(gridDim.x);
-    if (c >= C) return;
-    const int i = c / J, j = c - (c / J) * J;
+    const int I = s.I, J = s.J;
+    const int tilesJ = (J + kP2gTile - 1) / kP2gTile;
+    const int tiles = ((I + kP2gTile - 1) / kP2gTile) * tilesJ;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= tiles) return;
+    const int ti = tile / tilesJ, tj = tile - (tile / tilesJ) * tilesJ;
+    const int i0 = ti * kP2gTile, j0 = tj * kP2gTile;
+    const int wr0 = i0 > 0 ? i0 - 1 : 0, wr1 = min(i0 + kP2gTile, I - 1);
+    const int wc0 = j0 > 0 ? j0 - 1 : 0, wc1 = min(j0 + kP2gTile, J - 1);
+    const int nrows = wr1 - wr0 + 1;   // <= 18
+    __shared__ float2 spos[kP2gCap];
+    __shared__ int seg_beg[kP2gTile + 2], seg_off[kP2gTile + 3];
+    if (threadIdx.x < 64) {
+        const int r = threadIdx.x;
+        int beg = 0, cnt = 0;
+        if (r < nrows) {
+            const size_t row = (size_t)(wr0 + r) * J;
+            beg = s.cellStart[row + wc0];
+            cnt = s.cellStart[row + wc1 + 1] - beg;
+        }
+        int incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (r >= o) incl += t;
+        }
+        if (r < nrows) {
+            seg_beg[r] = beg;
+            seg_off[r + 1] = incl;
+        }
+        if (r == 0) seg_off[0] = 0;
+    }
+    __syncthreads();
+    const int total = seg_off[nrows];
+    const bool staged = total <= kP2gCap;
+    if (staged) {
+        constexpr int U = 4;   // loads in flight per thread
+        for (int t0 = threadIdx.x; t0 < total; t0 += U * kBlock) {
+            float2 v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = t0 + u * kBlock;
+                if (t >= total) continue;
+                int lo = 0, hi = nrows - 1;   // the row whose offset range holds t
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (seg_off[mid] <= t) lo = mid;
+                    else hi = mid - 1;
+                }
+                v[u] = s.pos[seg_beg[lo] + (t - seg_off[lo])];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (t0 + u * kBlock < total) spos[t0 + u * kBlock] = v[u];
+        }
+    }
+    __syncthreads();
+    const int i = i0 + (threadIdx.x >> 4), j = j0 + (threadIdx.x & 15);
+    if (i >= I || j >= J) return;
     const float fi = (float)i, fj = (float)j;
     float uw = 1e-10f, vw = 1e-10f, uv = 0.0f, vv = 0.0f;
     bool uval = false, vval = false;
-    const int j0 = j > 0 ? j - 1 : 0, j1 = j < J - 1 ? j + 1 : J - 1;
+    const int jl = j > 0 ? j - 1 : 0, jh = j < J - 1 ? j + 1 : J - 1;
     for (int di = -1; di <= 1; di++) {
         const int ii = i + di;
         if (ii < 0 || ii >= I) continue;
-        const int a = s.cellStart[ii * J + j0], e = s.cellStart[ii * J + j1 + 1];
-        for (int k = a; k < e; k++) {
-            const float2 p = s.pos[k];
-            const float wu = qbspline(p.x - fi, p.y - (fj + 0.5f));
-            const float wv = qbspline(p.x - (fi + 0.5f), p.y - fj);
-            if (wu > 1e-9f && wv > 1e-9f) {
-                const float2 vel = s.vel[k];
-                uw += wu;
-                uv += wu * vel.x;
-                uval = true;
-                vw += wv;
-                vv += wv * vel.y;
-                vval = true;
+        const int a = s.cellStart[ii * J + jl], e = s.cellStart[ii * J + jh + 1];
+        const int shift = seg_off[ii - wr0] - seg_beg[ii - wr0];
+        // chunks of 8 particles: all loads first, then the sums in order
+        constexpr int U = 8;
+        for (int k0 = a; k0 < e; k0 += U) {
+            float2 pp[U], vel[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = k0 + u;
+                if (k >= e) continue;
+                pp[u] = staged ? spos[k + shift] : s.pos[k];
+                vel[u] = s.vel[k];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (k0 + u >= e) continue;
+                const float2 p = pp[u];
+                const float wu = qbspline(p.x - fi, p.y - (fj + 0.5f));
+                const float wv = qbspline(p.x - (fi + 0.5f), p.y - fj);
+                if (wu > 1e-9f && wv > 1e-9f) {
+                    uw += wu;
+                    uv += wu * vel[u].x;
+                    uval = true;
+                    vw += wv;
+                    vv += wv * vel[u].y;
+                    vval = true;
+                }
             }
         }
     }
