@@ -341,7 +341,7 @@ Geo geometry(const fg_scene* s)
     g.ls = dim3((unsigned)(cdiv(s->I, kBfsTile) * cdiv(s->J, kBfsTile)));
     g.one = dim3(1);
     g.pcg = dim3((unsigned)s->pcg_blocks);
-    g.bins = dim3((unsigned)cdiv(cdiv(s->I, 3) * cdiv(s->J, 3), kBlock));
+    g.bins = dim3((unsigned)cdiv(cdiv(s->I, 3) * cdiv(s->J, 3), kBlock / kSdfLanes));
     g.p2g = dim3((unsigned)(cdiv(s->I, 16) * cdiv(s->J, 16)));
     return g;
 }

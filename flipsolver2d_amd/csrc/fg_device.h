@@ -21,6 +21,8 @@ constexpr int kPcgMaxBlocks = 1024;      // default solver blocks per scene (con
 constexpr int kPcgMinChunk = 1024;       // default cells per solver block at least (4 per thread)
 constexpr int kPcgTilingMaxBlocks = 4096;  // finest tiling a context may select
 constexpr int kPcgTilingMinChunk = 256;
+constexpr int kSdfLanes = 4;             // lanes per 3x3-cell bin in the fluid SDF
+constexpr int kFixupLongCap = 512;       // longest cell the wave-cooperative id sort takes
 
 // Material codes and tests, FlipSolver2dLib/materialgrid.h:6-20.
 enum : int8_t { M_FLUID = 0x40, M_SOURCE = 0x41, M_SOLID = 0x20, M_SINK = 0x12, M_EMPTY = 0x10 };

@@ -188,18 +188,21 @@ __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
 // the reference's window -- the 3x3 bins of 3x3 cells around the cell's bin
 // (markerparticlesystem.cpp:9-20, 109-128) -- then updateMaterials
 // (1053-1075) and the per-cell particle count of countParticles.  All nine
-// cells of a bin share that window, so one thread per bin streams the
-// window's particles once and keeps nine running minima (min is exact, so
+// cells of a bin share that window, so the bin's kSdfLanes lanes split the
+// window's rows, stream their particles once with nine running minima each,
+// and combine the minima by lane shuffles (min is exact and order-free, so
 // the result equals the per-cell loop bit for bit).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_sdf_materials(const SceneDev* scenes)
 {
+    static_assert(kSdfLanes == 4, "the shuffle reduction assumes 4 lanes per bin");
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
     const int I = s.I, J = s.J;
     const int nbI = (I + 2) / 3, nbJ = (J + 2) / 3;
-    const int bin = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (bin >= nbI * nbJ) return;
+    const int sub = threadIdx.x & (kSdfLanes - 1);
+    const int bin = xcd_tile(blockIdx.x, gridDim.x) * (kBlock / kSdfLanes) + threadIdx.x / kSdfLanes;
+    if (bin >= nbI * nbJ) return;   // all lanes of a bin leave together
     const int bi = bin / nbJ, bj = bin - (bin / nbJ) * nbJ;
     const int r0 = max(0, 3 * bi - 3), r1 = min(I, 3 * bi + 6);
     const int c0 = max(0, 3 * bj - 3), c1 = min(J, 3 * bj + 6);
@@ -215,43 +218,69 @@ __global__ void __launch_bounds__(kBlock) k_sdf_materials(const SceneDev* scenes
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int b = 0; b < 3; b++) d2[a][b] = 3.402823466e+38f;
-    for (int r = r0; r < r1; r++) {
-        const int e = s.cellStart[r * J + c1];
-        for (int k = s.cellStart[r * J + c0]; k < e; k++) {
-            const float2 p = s.pos[k];
+    // this lane's rows r0+sub, r0+sub+4, r0+sub+8: bounds loaded together
+    constexpr int NR = (9 + kSdfLanes - 1) / kSdfLanes;
+    int rb[NR], re[NR];
 #pragma unroll
-            for (int a = 0; a < 3; a++) {
-                const float dxp = p.x - cx[a];
-                const float dx2 = dxp * dxp;
+    for (int q = 0; q < NR; q++) {
+        const int r = r0 + sub + q * kSdfLanes;
+        rb[q] = r < r1 ? s.cellStart[r * J + c0] : 0;
+        re[q] = r < r1 ? s.cellStart[r * J + c1] : 0;
+    }
 #pragma unroll
-                for (int b = 0; b < 3; b++) {
-                    const float dyp = p.y - cy[b];
-                    const float nd = dx2 + dyp * dyp;
-                    if (nd < d2[a][b]) d2[a][b] = nd;
+    for (int q = 0; q < NR; q++) {
+        const int e = re[q];
+        constexpr int U = 8;   // loads in flight per lane
+        for (int k0 = rb[q]; k0 < e; k0 += U) {
+            float2 pp[U];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (k0 + u < e) pp[u] = s.pos[k0 + u];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (k0 + u >= e) continue;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    const float dxp = pp[u].x - cx[a];
+                    const float dx2 = dxp * dxp;
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        const float dyp = pp[u].y - cy[b];
+                        const float nd = dx2 + dyp * dyp;
+                        if (nd < d2[a][b]) d2[a][b] = nd;
+                    }
                 }
             }
         }
     }
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        const int i = i0 + a;
-        if (i >= I) continue;
+    for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int b = 0; b < 3; b++) {
-            const int j = j0 + b;
-            if (j >= J) continue;
-            const int c = i * J + j;
-            const float sdf = sqrtf(d2[a][b]) - s.pscale;
-            s.fsdf[c] = sdf;
-            int8_t m = s.mat[c];
-            if (sdf < 0.f) {
-                if (is_empty(m)) m = M_FLUID;
-            } else {
-                if (is_strict_fluid(m)) m = M_EMPTY;
+#pragma unroll
+            for (int o = 1; o < kSdfLanes; o <<= 1) {
+                const float t = __shfl_xor(d2[a][b], o, 64);
+                if (t < d2[a][b]) d2[a][b] = t;
             }
-            s.mat[c] = m;
-            s.pcount[c] = min(s.cellStart[c + 1] - s.cellStart[c], 2 * s.ppc);
         }
+    // cell q of the bin is written by lane q mod kSdfLanes
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        if ((q & (kSdfLanes - 1)) != sub) continue;
+        const int a = q / 3, b = q - (q / 3) * 3;
+        const int i = i0 + a, j = j0 + b;
+        if (i >= I || j >= J) continue;
+        const int c = i * J + j;
+        const float sdf = sqrtf(d2[a][b]) - s.pscale;
+        s.fsdf[c] = sdf;
+        int8_t m = s.mat[c];
+        if (sdf < 0.f) {
+            if (is_empty(m)) m = M_FLUID;
+        } else {
+            if (is_strict_fluid(m)) m = M_EMPTY;
+        }
+        s.mat[c] = m;
+        s.pcount[c] = min(s.cellStart[c + 1] - s.cellStart[c], 2 * s.ppc);
     }
 }
 

@@ -257,60 +257,133 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const SceneDev* scenes)
     s.tmpSrc[slot] = (uint32_t)k;
 }
 
-// Per cell: order the slots by stable id.  Segments of up to 16 entries are
-// sorted in registers by an odd-even transposition network; longer ones
-// (pile-ups) by insertion sort in place.
+// Per cell: order the slots by (stable id, source slot).  The source slot
+// is unique, so the order is total even if a caller repeats an id, and the
+// padding pair (0xffffffff, 0xffffffff) sorts after every real entry (a
+// source slot is below the particle capacity).
+__device__ __forceinline__ bool pair_gt(uint32_t p0, uint32_t s0, uint32_t p1, uint32_t s1)
+{
+    return p0 > p1 || (p0 == p1 && s0 > s1);
+}
+
+// A cell of 2..N entries, sorted in registers by a bitonic network.
+template <int N>
+__device__ __forceinline__ void sort_cell_regs(uint32_t* tp, uint32_t* ts, int a, int n)
+{
+    uint32_t kp[N], ks[N];
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        kp[q] = q < n ? tp[a + q] : 0xffffffffu;
+        ks[q] = q < n ? ts[a + q] : 0xffffffffu;
+    }
+    bool sorted = true;
+#pragma unroll
+    for (int q = 1; q < N; q++)
+        if (q < n && pair_gt(kp[q - 1], ks[q - 1], kp[q], ks[q])) sorted = false;
+    if (sorted) return;
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const int l = i ^ j;
+                if (l <= i) continue;
+                const bool gt = pair_gt(kp[i], ks[i], kp[l], ks[l]);
+                const bool sw = (i & k) == 0 ? gt : !gt;
+                const uint32_t p0 = sw ? kp[l] : kp[i], p1 = sw ? kp[i] : kp[l];
+                const uint32_t s0 = sw ? ks[l] : ks[i], s1 = sw ? ks[i] : ks[l];
+                kp[i] = p0;
+                kp[l] = p1;
+                ks[i] = s0;
+                ks[l] = s1;
+            }
+#pragma unroll
+    for (int q = 0; q < N; q++)
+        if (q < n) {
+            tp[a + q] = kp[q];
+            ts[a + q] = ks[q];
+        }
+}
+
+// Cells of up to 32 entries: one thread each, in registers.  Longer cells
+// (pile-ups): the whole wave ranks the cell's entries through LDS, one cell
+// at a time; beyond kFixupLongCap entries, insertion sort by the owning lane.
 __global__ void __launch_bounds__(kBlock) k_fixup(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
+    __shared__ uint32_t lp[kBlock / 64][kFixupLongCap], ls[kBlock / 64][kFixupLongCap];
     const int C = s.I * s.J;
     const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= C) return;
-    const int a = s.cellStart[c], e = s.cellStart[c + 1];
-    const int n = e - a;
-    if (n <= 1) return;
-    if (n <= 16) {
-        uint32_t kp[16], ks[16];
-        bool sorted = true;
-#pragma unroll
-        for (int q = 0; q < 16; q++) {
-            kp[q] = q < n ? s.tmpPid[a + q] : 0xffffffffu;
-            ks[q] = q < n ? s.tmpSrc[a + q] : 0u;
-            if (q > 0 && q < n && kp[q] < kp[q - 1]) sorted = false;
-        }
-        if (sorted) return;
-#pragma unroll
-        for (int round = 0; round < 16; round++) {
-#pragma unroll
-            for (int q = round & 1; q + 1 < 16; q += 2) {
-                const bool sw = kp[q] > kp[q + 1];
-                const uint32_t p0 = sw ? kp[q + 1] : kp[q], p1 = sw ? kp[q] : kp[q + 1];
-                const uint32_t s0 = sw ? ks[q + 1] : ks[q], s1 = sw ? ks[q] : ks[q + 1];
-                kp[q] = p0;
-                kp[q + 1] = p1;
-                ks[q] = s0;
-                ks[q + 1] = s1;
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 16; q++)
-            if (q < n) {
-                s.tmpPid[a + q] = kp[q];
-                s.tmpSrc[a + q] = ks[q];
-            }
-        return;
+    int a = 0, n = 0;
+    if (c < C) {
+        a = s.cellStart[c];
+        n = s.cellStart[c + 1] - a;
     }
-    for (int x = a + 1; x < e; x++) {
-        const uint32_t kp = s.tmpPid[x], ks = s.tmpSrc[x];
-        int y = x - 1;
-        while (y >= a && s.tmpPid[y] > kp) {
-            s.tmpPid[y + 1] = s.tmpPid[y];
-            s.tmpSrc[y + 1] = s.tmpSrc[y];
-            y--;
+    uint32_t* tp = s.tmpPid;
+    uint32_t* ts = s.tmpSrc;
+    if (n > 1 && n <= 16) sort_cell_regs<16>(tp, ts, a, n);
+    else if (n > 16 && n <= 32) sort_cell_regs<32>(tp, ts, a, n);
+    unsigned long long longs = __ballot(n > 32);   // every lane of the wave is here
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    while (longs) {
+        const int owner = __ffsll((long long)longs) - 1;
+        longs &= longs - 1;
+        const int la = __shfl(a, owner, 64), ln = __shfl(n, owner, 64);
+        if (ln > kFixupLongCap) {
+            if (lane == owner) {
+                for (int x = la + 1; x < la + ln; x++) {
+                    const uint32_t kp = tp[x], ks = ts[x];
+                    int y = x - 1;
+                    while (y >= la && pair_gt(tp[y], ts[y], kp, ks)) {
+                        tp[y + 1] = tp[y];
+                        ts[y + 1] = ts[y];
+                        y--;
+                    }
+                    tp[y + 1] = kp;
+                    ts[y + 1] = ks;
+                }
+            }
+            continue;
         }
-        s.tmpPid[y + 1] = kp;
-        s.tmpSrc[y + 1] = ks;
+        constexpr int Q = kFixupLongCap / 64;
+        uint32_t kp[Q], ks[Q];
+        int rank[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int j = lane + 64 * q;
+            kp[q] = j < ln ? tp[la + j] : 0u;
+            ks[q] = j < ln ? ts[la + j] : 0u;
+            rank[q] = 0;
+        }
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int j = lane + 64 * q;
+            if (j < ln) {
+                lp[w][j] = kp[q];
+                ls[w][j] = ks[q];
+            }
+        }
+        // one wave writes and reads its own LDS rows: its LDS operations
+        // complete in issue order, the barrier keeps the compiler's order
+        __builtin_amdgcn_wave_barrier();
+        for (int m = 0; m < ln; m++) {
+            const uint32_t pm = lp[w][m], sm = ls[w][m];
+#pragma unroll
+            for (int q = 0; q < Q; q++) rank[q] += pair_gt(kp[q], ks[q], pm, sm) ? 1 : 0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // every entry of the cell is in registers (the LDS stores consumed
+        // all loads), so the in-place scatter reads nothing it overwrites
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int j = lane + 64 * q;
+            if (j < ln) {
+                tp[la + rank[q]] = kp[q];
+                ts[la + rank[q]] = ks[q];
+            }
+        }
     }
 }
 

@@ -436,7 +436,7 @@ void orc_advect(const OrcParams* p, OrcState* s, float dt, uint8_t* alive)
 }
 
 /* Prune dead particles (flipsolver2d.cpp:353-359) and order the survivors by
- * (cell, stable id).  cell_start gets I*J+1 offsets. */
+ * (cell, stable id, input index).  cell_start gets I*J+1 offsets. */
 void orc_sort_particles(const OrcParams* p, OrcState* s, const uint8_t* alive, int32_t* cell_start)
 {
     const long J = p->size_j;
@@ -453,7 +453,8 @@ void orc_sort_particles(const OrcParams* p, OrcState* s, const uint8_t* alive, i
     for (int64_t k : idx) key[k] = cell_of(s->px[k], s->py[k], J);
     std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
         if (key[a] != key[b]) return key[a] < key[b];
-        return s->pid[a] < s->pid[b];
+        if (s->pid[a] != s->pid[b]) return s->pid[a] < s->pid[b];
+        return a < b; /* a caller may repeat an id: input order breaks the tie */
     });
     const int64_t n = (int64_t)idx.size();
     std::vector<float> px(n), py(n), vx(n), vy(n);
