@@ -34,17 +34,17 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-substeps/s + p99 frame latency, batched vs plain GPU @1e-6 resid"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per cell of the PCG kernels.  One iteration is 54 B/cell:
-# SpMV 9 + direction update 12 (k_pcg_a: 21) and axpy pair 24 + IPP 9
-# (k_pcg_b: 33).
+# Algorithmic bytes per cell of the PCG kernels (multigrid preconditioner):
+# k_pcg_a = SpMV 9 + direction update 12 = 21; k_pcg_b = the axpy pair
+# r -= alpha q, d += alpha p = 24 (the V-cycle forms z).
 KERNEL_BYTES = {
-    "pcg_b": ("K_PCG_B", 33.0, "cells"),
+    "pcg_b": ("K_PCG_B", 24.0, "cells"),
     "pcg_a": ("K_PCG_A", 21.0, "cells"),
 }
 # HBM bytes per cell actually moved, per workload: rocprofv3 FETCH_SIZE
 # (x2.000) + WRITE_SIZE (x1.000) per dispatch, the factors being what the same
-# counters read for a 1 GiB copy kernel (profiles/pcg_*_hbm_traffic.md).
-MEASURED_TRAFFIC_PER_CELL = {"dam1024": {"pcg_b": 34.18, "pcg_a": 21.83}, "mixed256": {"pcg_b": 29.70, "pcg_a": 17.80}}
+# counters read for a 1 GiB copy kernel.
+MEASURED_TRAFFIC_PER_CELL = {}
 
 
 def parse():
@@ -236,7 +236,7 @@ def dam_leg(fg, ctx, a, copy_gbs):
         "p99_frame_ms": pctl(lat, 0.99),
         "max_rel_residual": resid,
         "all_converged": conv,
-        "roofline": kt.roofline(MEASURED_TRAFFIC_PER_CELL["dam1024"].get(a.roofline_kernel), copy_gbs),
+        "roofline": kt.roofline(MEASURED_TRAFFIC_PER_CELL.get("dam1024", {}).get(a.roofline_kernel), copy_gbs),
     }
 
 

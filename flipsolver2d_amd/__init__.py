@@ -31,11 +31,14 @@ EXPORTS = [
     "fg_ctx_profile_kernel", "fg_ctx_profile_read", "fg_ctx_bandwidth_probe",
     "fg_server_create", "fg_server_destroy", "fg_server_submit", "fg_server_tick", "fg_server_status",
     "fg_server_scene", "fg_server_release", "fg_server_summary_get", "fg_ctx_set_batch_lanes",
-    "fg_ctx_batch_lanes", "fg_batch_autotune", "fg_ctx_set_solver_tiling",
+    "fg_ctx_batch_lanes", "fg_batch_autotune", "fg_ctx_set_solver_tiling", "fg_ctx_set_preconditioner",
+    "fg_ctx_preconditioner",
 ]
 MAX_LANES = 4
 REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_REJECTED, REQ_RELEASED = range(5)
-(K_ADVECT, K_SORT, K_P2G, K_SDF, K_LEVELSET, K_EXTRAPOLATE, K_PCG_A, K_PCG_B, K_G2P, K_RHS, K_APPLY) = range(11)
+(K_ADVECT, K_SORT, K_P2G, K_SDF, K_LEVELSET, K_EXTRAPOLATE, K_PCG_A, K_PCG_B, K_G2P, K_RHS, K_APPLY,
+ K_VCYCLE) = range(12)
+PRECOND_MG, PRECOND_IPP = 0, 1
 
 
 class FgError(RuntimeError):
@@ -154,6 +157,8 @@ def lib():
             "fg_ctx_set_batch_lanes": (C.c_int, [vp, i32]),
             "fg_ctx_batch_lanes": (C.c_int, [vp, C.POINTER(i32)]),
             "fg_ctx_set_solver_tiling": (C.c_int, [vp, i32, i32]),
+            "fg_ctx_set_preconditioner": (C.c_int, [vp, i32]),
+            "fg_ctx_preconditioner": (C.c_int, [vp, C.POINTER(i32)]),
             "fg_batch_autotune": (C.c_int, [vp, vp, i32, i32, C.POINTER(i32), vp]),
             "fg_server_create": (C.c_int, [vp, C.c_uint64, i32, C.POINTER(vp)]),
             "fg_server_destroy": (C.c_int, [vp]),
@@ -216,6 +221,15 @@ class Context:
 
     def set_solver_tiling(self, target_blocks=1024, min_chunk=1024):
         _check(lib().fg_ctx_set_solver_tiling(self.h, target_blocks, min_chunk))
+
+    def set_preconditioner(self, kind):
+        """PRECOND_MG (multigrid V-cycle, default) or PRECOND_IPP for later pressure solves."""
+        _check(lib().fg_ctx_set_preconditioner(self.h, kind))
+
+    def preconditioner(self):
+        v = C.c_int32()
+        _check(lib().fg_ctx_preconditioner(self.h, C.byref(v)))
+        return v.value
 
     def batch_lanes(self):
         v = C.c_int32()

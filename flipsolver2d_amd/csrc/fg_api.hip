@@ -89,6 +89,7 @@ struct fg_ctx {
     int nlanes = 1;                 // lanes a batch is split over
     int pcg_target_blocks = kPcgMaxBlocks;   // solver tiling: blocks per scene at most
     int pcg_min_chunk = kPcgMinChunk;        // and cells per block at least
+    int precond = kPrecondMG;                // pressure PCG preconditioner
     std::vector<Lane*> lanes;
 };
 
@@ -105,6 +106,8 @@ struct fg_scene {
     void* arena = nullptr;
     uint64_t bytes = 0;
     SceneDev h{};                 // host copy of the descriptor
+    MgLevels mgh{};               // host copy of the multigrid hierarchy (device pointers)
+    bool mg_ok = false;           // the hierarchy fits kMgMaxLevels
     SceneDev* d = nullptr;        // device descriptor
     SceneDev* hpin = nullptr;     // pinned staging for descriptor uploads
     SceneScalars* hsc = nullptr;  // pinned readback of the scalars
@@ -152,6 +155,33 @@ void carve(fg_scene* s, Carver& cv)
     h.d = cv.take<float>(C);
     h.partials = cv.take<double>(kSlots * nb);
     h.sc = cv.take<SceneScalars>(1);
+    // multigrid: halve until the bottom fits kMgBottomMax per side (at
+    // least one coarse level)
+    {
+        MgLevels& M = s->mgh;
+        M = MgLevels{};
+        M.lv[0].I = s->I;
+        M.lv[0].J = s->J;
+        int n = 1;
+        while (n < kMgMaxLevels && (n == 1 || std::max(M.lv[n - 1].I, M.lv[n - 1].J) > kMgBottomMax)) {
+            M.lv[n].I = (M.lv[n - 1].I + 1) / 2;
+            M.lv[n].J = (M.lv[n - 1].J + 1) / 2;
+            n++;
+        }
+        s->mg_ok = std::max(M.lv[n - 1].I, M.lv[n - 1].J) <= kMgBottomMax;
+        M.n = n;
+        h.mg = cv.take<MgLevels>(1);
+        for (int l = 1; l < n; l++) {
+            MgLevel& L = M.lv[l];
+            const int64_t CL = (int64_t)L.I * L.J;
+            L.act = cv.take<uint8_t>(CL);
+            L.coef = cv.take<float>(kMgStencil * CL);
+            L.dinv = cv.take<float>(CL);
+            L.r = cv.take<float>(CL);
+            L.x = cv.take<float>(CL);
+            L.y = cv.take<float>(CL);
+        }
+    }
     if (s->solver_only) return;
     h.pos = cv.take<float2>(P);
     h.vel = cv.take<float2>(P);
@@ -306,6 +336,7 @@ int create_scene(fg_ctx* ctx, const fg_scene_params* p, bool solver_only, fg_sce
         HIPCHK(hipMemsetAsync(h.mat, (int)(int8_t)FG_EMPTY, s->C, st));
         HIPCHK(hipMemsetAsync(h.sid, 0xff, sizeof(int32_t) * s->C, st));
     }
+    HIPCHK(hipMemcpyAsync(h.mg, &s->mgh, sizeof(MgLevels), hipMemcpyHostToDevice, st));
     FGCHK(upload_desc(s));
     HIPCHK(hipStreamSynchronize(st));
     *out = s;
@@ -463,6 +494,8 @@ struct Group {
     int64_t cells = 0, faces = 0, parts = 0;
     int max_inner = 0;
     int max_bands = 0;
+    int mg_levels = 0;                  // most multigrid levels of a scene using the V-cycle
+    unsigned mg_grid[kMgMaxLevels]{};   // blocks per scene for one thread per level-l cell
 };
 
 void lane_free_buffers(Lane* L)
@@ -567,6 +600,12 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
     for (fg_scene* s : scenes) {
         apply_tiling(ctx, s);
         s->h.active = 1;
+        s->h.precond = (ctx->precond == kPrecondMG && s->mg_ok) ? kPrecondMG : kPrecondIPP;
+        if (s->h.precond == kPrecondMG) {
+            G.mg_levels = std::max(G.mg_levels, s->mgh.n);
+            for (int l = 1; l < s->mgh.n; l++)
+                G.mg_grid[l] = std::max(G.mg_grid[l], (unsigned)cdiv((int64_t)s->mgh.lv[l].I * s->mgh.lv[l].J, kBlock));
+        }
         s->h.pblocks = (int)std::max<int64_t>(1, cdiv(std::max<int64_t>(s->n_bound, 1), kBlock));
         const Geo gs = geometry(s);
         if (first) G.g = gs;
@@ -710,6 +749,25 @@ struct KProf {
 // scenes of the group run the same sweep and iteration index in lock step;
 // a scene whose solve has stopped skips the launches that remain.
 // ---------------------------------------------------------------------------
+// One V-cycle z = M r over the scenes of `d` (ny of them) whose solve uses
+// it; which: the residual buffer (0: cr, 1: cr2).  Level-l launches cover
+// the largest scene's level l; scenes with fewer levels skip them.
+void launch_vcycle(const Group& G, hipStream_t st, const SceneDev* d, unsigned ny, dim3 pcg_grid, int which)
+{
+    const int n = G.mg_levels;
+    launch_mg_down0(dim3(G.mg_grid[1], ny), st, d, which);
+    for (int l = 1; l <= n - 2; l++) {
+        launch_mg_resid(dim3(G.mg_grid[l], ny), st, d, l);
+        launch_mg_restrict(dim3(G.mg_grid[l + 1], ny), st, d, l + 1);
+    }
+    launch_mg_bottom(dim3(1, ny), st, d);
+    for (int l = n - 2; l >= 1; l--) {
+        launch_mg_prolong(dim3(G.mg_grid[l], ny), st, d, l);
+        launch_mg_post(dim3(G.mg_grid[l], ny), st, d, l);
+    }
+    launch_mg_up0(pcg_grid, st, d, which);
+}
+
 int run_solve(Group& G, bool rhs_from_grid)
 {
     fg_ctx* ctx = G.ctx;
@@ -723,6 +781,13 @@ int run_solve(Group& G, bool rhs_from_grid)
     } else {
         launch_solve_begin(g.pcg, st, G.d);
     }
+    // multigrid hierarchy of the matrix snapshot (fixed for the whole solve)
+    for (int l = 1; l < G.mg_levels; l++) {
+        const dim3 gl(G.mg_grid[l], (unsigned)G.n);
+        launch_mg_mark(gl, st, G.d, l);
+        launch_mg_galerkin(gl, st, G.d, l);
+        launch_mg_sym(gl, st, G.d, l);
+    }
     constexpr int kPoll = 16;
     int32_t flags[3];
     int launched = 0;
@@ -730,6 +795,11 @@ int run_solve(Group& G, bool rhs_from_grid)
         if (sweep > 0) launch_refine_update(g.pcg, st, G.d);
         launch_refine_residual(g.pcg, st, G.d);
         launch_cg_init(g.pcg, st, G.d, sweep);
+        if (G.mg_levels) {
+            KProf kp(G, FG_K_VCYCLE);
+            launch_vcycle(G, st, G.d, (unsigned)G.n, g.pcg, 0);
+            kp.done(0);
+        }
         FGCHK(poll_group(G, -1, flags));
         if (!flags[0]) break;
         // several scenes: launch over the packed iterating ones only, and
@@ -750,9 +820,17 @@ int run_solve(Group& G, bool rhs_from_grid)
                     launch_pcg_a(git, st, dit, it);
                     kp.done(0);
                 }
-                KProf kp(G, FG_K_PCG_B, true);
-                launch_pcg_b(git, st, dit, it);
-                kp.done(0);
+                {
+                    KProf kp(G, FG_K_PCG_B, true);
+                    launch_pcg_b(git, st, dit, it);
+                    kp.done(0);
+                }
+                if (G.mg_levels) {
+                    // the new residual: pcg_b(it) wrote cr2 for even it, cr for odd
+                    KProf kp(G, FG_K_VCYCLE);
+                    launch_vcycle(G, st, dit, git.y, git, (it & 1) ? 0 : 1);
+                    kp.done(0);
+                }
                 launched++;
             }
             if (G.n > 1) {
@@ -768,6 +846,9 @@ int run_solve(Group& G, bool rhs_from_grid)
     // work units of the launches that iterated (the rest exited at once)
     if (ctx->prof_kernel == FG_K_PCG_A || ctx->prof_kernel == FG_K_PCG_B)
         for (fg_scene* s : G.sc) G.L->prof_units += (int64_t)s->hsc->total_iters * s->C;
+    if (ctx->prof_kernel == FG_K_VCYCLE)   // one V-cycle per iteration and per sweep start
+        for (fg_scene* s : G.sc)
+            if (s->h.precond == kPrecondMG) G.L->prof_units += (int64_t)(s->hsc->total_iters + s->hsc->sweeps) * s->C;
     for (fg_scene* s : G.sc)
         if (s->hsc->numeric_bad) return fail(FG_E_NUMERIC, "NaN/Inf in the pressure solve");
     (void)launched;
@@ -1597,6 +1678,21 @@ int fg_ctx_set_solver_tiling(fg_ctx* ctx, int32_t target_blocks, int32_t min_chu
     if (min_chunk < kPcgTilingMinChunk || min_chunk % kBlock) return fail(FG_E_INVALID, "bad min_chunk");
     ctx->pcg_target_blocks = target_blocks;
     ctx->pcg_min_chunk = min_chunk;
+    return FG_OK;
+}
+
+int fg_ctx_set_preconditioner(fg_ctx* ctx, int32_t kind)
+{
+    if (!ctx) return fail(FG_E_INVALID, "ctx is NULL");
+    if (kind != FG_PRECOND_MG && kind != FG_PRECOND_IPP) return fail(FG_E_INVALID, "bad preconditioner");
+    ctx->precond = kind == FG_PRECOND_MG ? kPrecondMG : kPrecondIPP;
+    return FG_OK;
+}
+
+int fg_ctx_preconditioner(fg_ctx* ctx, int32_t* kind)
+{
+    if (!ctx || !kind) return fail(FG_E_INVALID, "NULL argument");
+    *kind = ctx->precond == kPrecondMG ? FG_PRECOND_MG : FG_PRECOND_IPP;
     return FG_OK;
 }
 

@@ -22,6 +22,12 @@ constexpr int kPcgMinChunk = 1024;       // default cells per solver block at le
 constexpr int kPcgTilingMaxBlocks = 4096;  // finest tiling a context may select
 constexpr int kPcgTilingMinChunk = 256;
 constexpr int kSdfLanes = 4;             // lanes per 3x3-cell bin in the fluid SDF
+// multigrid preconditioner
+constexpr int kPrecondMG = 0, kPrecondIPP = 1;
+constexpr int kMgMaxLevels = 12;
+constexpr int kMgBottomMax = 16;         // the bottom level has at most this many cells per side
+constexpr int kMgBottomSweeps = 8;       // L1-Jacobi sweeps on the bottom level
+constexpr int kMgStencil = 25;           // 5x5 Galerkin stencil of the coarse levels
 constexpr int kFixupLongCap = 512;       // longest cell the wave-cooperative id sort takes
 
 // Material codes and tests, FlipSolver2dLib/materialgrid.h:6-20.
@@ -58,6 +64,22 @@ struct SceneScalars {
     int32_t density_applied;  // density correction moved the particles this substep
 };
 
+// One level of the multigrid hierarchy (levels >= 1; level 0 is the
+// pressure matrix itself, read from SceneDev::stencil).
+struct MgLevel {
+    int32_t I, J;
+    uint8_t* act;     // the cell takes part: one of its 2x2 finer cells does
+    float* coef;      // [kMgStencil][I*J]: entry k couples the cell to (i + k/5 - 2, j + k%5 - 2)
+    float* dinv;      // 1 / sum_k |coef_k| on active cells, 0 elsewhere
+    float* r;         // restricted residual
+    float* x;         // pre-smoothed iterate plus the coarse correction
+    float* y;         // residual on the way down, smoothed result on the way up
+};
+struct MgLevels {
+    int32_t n;        // levels; level n-1 is the bottom
+    MgLevel lv[kMgMaxLevels];
+};
+
 // Per-scene device descriptor.  One array of these lives in device memory;
 // kernels index it with blockIdx.y (the scene's slot in the launch).
 struct SceneDev {
@@ -80,6 +102,8 @@ struct SceneDev {
     int32_t pcg_chunk;      // cells per solver block (multiple of kBlock)
     int32_t max_pblocks;    // stride of the particle partials arrays
     int32_t pblocks;        // particle blocks of this scene's current launches (<= max_pblocks)
+    int32_t precond;        // kPrecondMG or kPrecondIPP
+    MgLevels* mg;           // the scene's multigrid hierarchy (device)
     // per-substep
     float dt;
     float force;            // (float)(dt / dx), applyBodyForces

@@ -37,6 +37,18 @@ inline void fg_launch(F kernel, dim3 grid, dim3 block, hipStream_t st, Args... a
 #define FG_DECLARE_LAUNCH(name, ...) void launch_##name(dim3 grid, hipStream_t st, const SceneDev* scenes, ##__VA_ARGS__)
 
 // fg_particles.hip
+// fg_mg.hip
+FG_DECLARE_LAUNCH(mg_mark, int l);
+FG_DECLARE_LAUNCH(mg_galerkin, int l);
+FG_DECLARE_LAUNCH(mg_sym, int l);
+FG_DECLARE_LAUNCH(mg_down0, int which);
+FG_DECLARE_LAUNCH(mg_resid, int l);
+FG_DECLARE_LAUNCH(mg_restrict, int l);
+FG_DECLARE_LAUNCH(mg_bottom);
+FG_DECLARE_LAUNCH(mg_prolong, int l);
+FG_DECLARE_LAUNCH(mg_post, int l);
+FG_DECLARE_LAUNCH(mg_up0, int which);
+
 FG_DECLARE_LAUNCH(max_v2);
 FG_DECLARE_LAUNCH(advect, int alive_from_key);
 FG_DECLARE_LAUNCH(scan_local);

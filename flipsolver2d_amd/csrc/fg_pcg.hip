@@ -127,6 +127,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const SceneDev* scenes, int 
         const double f = fmax(0.3 * s.tol * bn / rn, 1e-6);
         sc->inner_eps2 = f * f;
     }
+    if (s.precond == kPrecondMG) return;   // the V-cycle forms z and the partials
     const int J = s.J;
     const Chunk ch = my_chunk(s);
     const float* __restrict__ r = s.cr;
@@ -247,7 +248,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_a(const SceneDev* scenes, int it
     put_partials<1>(s, kSlotPQ, v);
 }
 
-// KB: r = r_old - alpha q; d += alpha p; z = M r; r.z, r.r
+// KB: r = r_old - alpha q; d += alpha p; z = M r; r.z, r.r (with the IPP;
+// with the V-cycle, r and d only)
 __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, int it)
 {
     const SceneDev& s = scenes[blockIdx.y];
@@ -265,6 +267,27 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, int it
     float* __restrict__ d = s.d;
     const uint8_t* __restrict__ code = s.stencil;
     const Chunk ch = my_chunk(s);
+    if (s.precond == kPrecondMG) {
+        // r and d only; the V-cycle that follows forms z, r.z and r.r
+        double t[1];
+        sum_partials<1>(s, kSlotPQ, nb, t);
+        const double pq = t[0];
+        const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+        if (!(pq > 0.0) || !finite_d(pq)) {
+            if (lead) sc->inner_done = 1;
+            return;
+        }
+        const float a = (float)(sc->gamma[it & 1] / pq);
+        if (lead) {
+            sc->inner_iters = it + 1;
+            sc->total_iters += 1;
+        }
+        for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
+            rnew[c] = rold[c] - a * q[c];
+            d[c] = d[c] + a * pnew[c];
+        }
+        return;
+    }
     double rz = 0.0, rr = 0.0;
     float alpha = 0.f;
     for (int base = ch.begin; base < ch.end; base += kPass) {
