@@ -53,6 +53,13 @@ int fail(int code, const std::string& msg)
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Block maps of the solve's per-iteration launches, by kind: 0 = solver
+// chunks, 1 = level-1 cells one per thread (k_mg_down0), 1 + l = level-l
+// cells one per thread (l >= 1).
+constexpr int kMapKinds = 1 + kMgMaxLevels;
+constexpr int kMapMaxScenes = 4096;   // scenes one map build handles (LDS prefix)
+constexpr int kFlagWords = 4 + kMapKinds;
+
 }  // namespace
 
 // One HIP stream with the buffers its launch groups use.  A context owns
@@ -70,8 +77,10 @@ struct Lane {
     SceneDev* bpack = nullptr;      // iterating scenes of a solve, packed
     SceneScalars* bsc = nullptr;    // gathered scalar blocks
     SceneScalars* bsc_host = nullptr;
-    int32_t* bflags = nullptr;
+    int32_t* bflags = nullptr;      // [kFlagWords]: poll flags / pack info, then map totals
     int32_t* bflags_host = nullptr;
+    int2* bmap = nullptr;           // block maps of the packed scenes
+    int64_t bmap_cap = 0;
     // per-launch kernel timing
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -136,6 +145,50 @@ struct Carver {
     }
 };
 
+// The Galerkin operators of an all-liquid grid, level by level: the same
+// products, in the same order, as k_mg_galerkin forms for a cell whose whole
+// neighbourhood is active (level 0: the 5-point operator, n = 4).
+const float (&mg_canonical())[kMgMaxLevels][kMgStencil]
+{
+    static float canon[kMgMaxLevels][kMgStencil];
+    static bool done = false;
+    if (done) return canon;
+    auto half_floor = [](int t) { return t >= 0 ? t / 2 : -((1 - t) / 2); };
+    auto w1 = [](int a) { return (a == 1 || a == 2) ? 0.75 : 0.25; };
+    for (int k = 0; k < kMgStencil; k++) canon[0][k] = 0.f;
+    canon[0][12] = 4.f;
+    canon[0][7] = canon[0][17] = canon[0][11] = canon[0][13] = -1.f;
+    for (int l = 1; l < kMgMaxLevels; l++) {
+        double acc[kMgStencil] = {};
+        for (int ab = 0; ab < 16; ab++) {
+            const int a = ab / 4, b = ab % 4;
+            const double wc = w1(a) * w1(b);
+            auto spread = [&](int ei, int ej, double v) {
+                const int ti = -1 + a + ei, tj = -1 + b + ej;
+                const int pi0 = half_floor(ti), pj0 = half_floor(tj);
+                const int pi1 = (ti & 1) ? pi0 + 1 : pi0 - 1, pj1 = (tj & 1) ? pj0 + 1 : pj0 - 1;
+                const int pis[4] = {pi0, pi0, pi1, pi1}, pjs[4] = {pj0, pj1, pj0, pj1};
+                const double ws[4] = {0.5625, 0.1875, 0.1875, 0.0625};
+                for (int q = 0; q < 4; q++)
+                    if (pis[q] >= -2 && pis[q] <= 2 && pjs[q] >= -2 && pjs[q] <= 2)
+                        acc[(pis[q] + 2) * 5 + (pjs[q] + 2)] += (wc * ws[q]) * v;
+            };
+            if (l == 1) {
+                spread(0, 0, 4.0);
+                spread(-1, 0, -1.0);
+                spread(1, 0, -1.0);
+                spread(0, -1, -1.0);
+                spread(0, 1, -1.0);
+            } else {
+                for (int e = 0; e < kMgStencil; e++) spread(e / 5 - 2, e % 5 - 2, (double)canon[l - 1][e]);
+            }
+        }
+        for (int k = 0; k < kMgStencil; k++) canon[l][k] = (float)(0.25 * acc[k]);
+    }
+    done = true;
+    return canon;
+}
+
 void carve(fg_scene* s, Carver& cv)
 {
     SceneDev& h = s->h;
@@ -170,6 +223,7 @@ void carve(fg_scene* s, Carver& cv)
         }
         s->mg_ok = std::max(M.lv[n - 1].I, M.lv[n - 1].J) <= kMgBottomMax;
         M.n = n;
+        std::memcpy(M.canon, mg_canonical(), sizeof(M.canon));
         h.mg = cv.take<MgLevels>(1);
         for (int l = 1; l < n; l++) {
             MgLevel& L = M.lv[l];
@@ -480,6 +534,74 @@ __global__ void k_pack_iterating(const SceneDev* in, int n, SceneDev* out, int32
     }
 }
 
+// Blocks of kind `kind` (see kMapKinds) a packed scene launches.
+__device__ int map_blocks(const SceneDev& s, int kind)
+{
+    if (kind == 0) return s.pcg_blocks;
+    if (s.precond != kPrecondMG) return 0;
+    const MgLevels& M = *s.mg;
+    if (kind == 1) return (M.lv[1].I * M.lv[1].J + kBlock - 1) / kBlock;
+    const int l = kind - 1;
+    if (l >= M.n) return 0;
+    return (M.lv[l].I * M.lv[l].J + kBlock - 1) / kBlock;
+}
+
+struct MapOffsets {
+    int64_t off[kMapKinds];
+};
+
+// One block per map kind: the (slot, block) pairs of every packed scene's
+// blocks, stored so that flat block e takes entry xcd_tile(e, total) of the
+// natural order (consecutive tiles of a scene on one XCD).  info[0] = the
+// packed count; info[4 + kind] gets the kind's total.
+__global__ void __launch_bounds__(1024) k_build_maps(const SceneDev* packed, int32_t* info, int2* maps, MapOffsets offs)
+{
+    __shared__ int pre[kMapMaxScenes + 1];
+    __shared__ int wsum[1024 / 64];
+    const int kind = blockIdx.x;
+    const int n = info[0];
+    const int t = threadIdx.x;
+    // each thread sums a contiguous run of scenes, then a block scan of the runs
+    const int per = (n + 1023) / 1024;
+    const int b0 = min(n, t * per), b1 = min(n, b0 + per);
+    int run = 0;
+    for (int i = b0; i < b1; i++) {
+        const int c = map_blocks(packed[i], kind);
+        pre[i + 1] = c;
+        run += c;
+    }
+    int incl = run;
+    const int lane = t & 63, w = t >> 6;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int woff = 0;
+    for (int k = 0; k < w; k++) woff += wsum[k];
+    int acc = woff + incl - run;   // exclusive prefix of this thread's run
+    if (t == 0) pre[0] = 0;
+    for (int i = b0; i < b1; i++) {
+        acc += pre[i + 1];
+        pre[i + 1] = acc;
+    }
+    __syncthreads();
+    const int total = pre[n];
+    int2* out = maps + offs.off[kind];
+    for (int e = t; e < total; e += 1024) {
+        const int nat = xcd_tile(e, total);
+        int lo = 0, hi = n - 1;   // the last scene whose blocks start at or before nat
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= nat) lo = mid;
+            else hi = mid - 1;
+        }
+        out[e] = make_int2(lo, nat - pre[lo]);
+    }
+    if (t == 0) info[4 + kind] = total;
+}
+
 // ---------------------------------------------------------------------------
 // launch groups: the scenes of one launch, grid.y = scene slot
 // ---------------------------------------------------------------------------
@@ -495,7 +617,10 @@ struct Group {
     int max_inner = 0;
     int max_bands = 0;
     int mg_levels = 0;                  // most multigrid levels of a scene using the V-cycle
-    unsigned mg_grid[kMgMaxLevels]{};   // blocks per scene for one thread per level-l cell
+    int map_kinds = 0;                  // block-map kinds the solve uses (0 when maps are off)
+    int64_t map_off[kMapKinds]{};       // offset of each kind's map in the lane's map buffer
+    unsigned mg_grid[kMgMaxLevels]{};   // blocks per scene over level l, one cell per thread
+    unsigned mg_grid_d0 = 0;            // blocks per scene of k_mg_down0 (one level-1 cell per thread)
 };
 
 void lane_free_buffers(Lane* L)
@@ -519,6 +644,7 @@ void lane_destroy(Lane* L)
     lane_free_buffers(L);
     if (L->bflags) hipFree(L->bflags);
     if (L->bflags_host) hipHostFree(L->bflags_host);
+    if (L->bmap) hipFree(L->bmap);
     for (hipEvent_t e : L->ev_pool) hipEventDestroy(e);
     if (L->own_stream && L->stream) hipStreamDestroy(L->stream);
     delete L;
@@ -534,8 +660,8 @@ int lane_create(hipStream_t stream, Lane** out)
         e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking);
         L->own_stream = e == hipSuccess;
     }
-    if (e == hipSuccess) e = hipMalloc(&L->bflags, sizeof(int32_t) * 4);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&L->bflags_host, sizeof(int32_t) * 4, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&L->bflags, sizeof(int32_t) * kFlagWords);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&L->bflags_host, sizeof(int32_t) * kFlagWords, hipHostMallocDefault);
     if (e != hipSuccess) {
         lane_destroy(L);
         return fail(FG_E_DEVICE, std::string("lane: ") + hipGetErrorString(e));
@@ -604,7 +730,9 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
         if (s->h.precond == kPrecondMG) {
             G.mg_levels = std::max(G.mg_levels, s->mgh.n);
             for (int l = 1; l < s->mgh.n; l++)
-                G.mg_grid[l] = std::max(G.mg_grid[l], (unsigned)cdiv((int64_t)s->mgh.lv[l].I * s->mgh.lv[l].J, kBlock));
+                G.mg_grid[l] = std::max(G.mg_grid[l],
+                                        (unsigned)cdiv((int64_t)s->mgh.lv[l].I * s->mgh.lv[l].J, kBlock));
+            G.mg_grid_d0 = std::max(G.mg_grid_d0, (unsigned)cdiv((int64_t)s->mgh.lv[1].I * s->mgh.lv[1].J, kBlock));
         }
         s->h.pblocks = (int)std::max<int64_t>(1, cdiv(std::max<int64_t>(s->n_bound, 1), kBlock));
         const Geo gs = geometry(s);
@@ -618,6 +746,32 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
         G.max_bands = std::max(G.max_bands, (int)std::min<int64_t>(kMaxBands, cdiv(std::max(s->I, s->J), 16) + 1));
     }
     geo_set_y(G.g, G.n);
+    // block maps for the solve's per-iteration launches (several scenes):
+    // capacity = every scene's blocks, the bound for any packed subset
+    if (G.n > 1 && G.n <= kMapMaxScenes) {
+        G.map_kinds = G.mg_levels + 1;
+        int64_t cap[kMapKinds] = {};
+        for (fg_scene* s : scenes) {
+            cap[0] += s->pcg_blocks;
+            if (s->h.precond != kPrecondMG) continue;
+            const MgLevels& M = s->mgh;
+            cap[1] += cdiv((int64_t)M.lv[1].I * M.lv[1].J, kBlock);
+            for (int l = 1; l < M.n; l++) cap[1 + l] += cdiv((int64_t)M.lv[l].I * M.lv[l].J, kBlock);
+        }
+        int64_t total = 0;
+        for (int k = 0; k < G.map_kinds; k++) {
+            G.map_off[k] = total;
+            total += cap[k];
+        }
+        if (total > L->bmap_cap) {
+            HIPCHK(hipStreamSynchronize(L->stream));
+            if (L->bmap) hipFree(L->bmap);
+            L->bmap = nullptr;
+            L->bmap_cap = 0;
+            HIPCHK(hipMalloc(&L->bmap, sizeof(int2) * std::max<int64_t>(total, 1)));
+            L->bmap_cap = total;
+        }
+    }
     if (G.n == 1) {
         fg_scene* s = scenes[0];
         *s->hpin = s->h;
@@ -661,17 +815,57 @@ int poll_group(Group& G, int band, int32_t* flags)
     return FG_OK;
 }
 
-// Pack the scenes whose inner PCG still iterates; returns their count and
-// the launch grid over them (synchronises).
-int pack_iterating(Group& G, const SceneDev** d, dim3* grid, int* count)
+// How the solve's per-iteration kernels launch: over a descriptor array on
+// (blocks, scenes) grids, or flat over block maps (kinds as in kMapKinds).
+struct SolveLaunch {
+    const SceneDev* d = nullptr;
+    unsigned ny = 0;                    // scenes in d
+    bool flat = false;
+    const int2* map[kMapKinds] = {};
+    unsigned blocks[kMapKinds] = {};    // flat: total blocks of the kind; else blocks per scene
+    dim3 grid(int kind) const { return flat ? dim3(blocks[kind]) : dim3(blocks[kind], ny); }
+};
+
+// A group's own descriptors on (blocks, scenes) grids.
+SolveLaunch group_launch(const Group& G, const SceneDev* d, unsigned ny)
+{
+    SolveLaunch S;
+    S.d = d;
+    S.ny = ny;
+    S.blocks[0] = G.g.pcg.x;
+    S.blocks[1] = G.mg_grid_d0;
+    for (int l = 1; l < kMgMaxLevels; l++) S.blocks[1 + l] = G.mg_grid[l];
+    return S;
+}
+
+// Pack the scenes whose inner PCG still iterates and build their block maps;
+// returns their count and the launch description (synchronises).
+int pack_iterating(Group& G, SolveLaunch* S, int* count)
 {
     Lane* L = G.L;
     hipLaunchKernelGGL(k_pack_iterating, dim3(1), dim3(64), 0, L->stream, G.d, G.n, L->bpack, L->bflags);
-    HIPCHK(hipMemcpyAsync(L->bflags_host, L->bflags, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, L->stream));
+    const int kinds = G.map_kinds;
+    if (kinds > 0) {
+        MapOffsets offs{};
+        for (int k = 0; k < kinds; k++) offs.off[k] = G.map_off[k];
+        hipLaunchKernelGGL(k_build_maps, dim3(kinds), dim3(1024), 0, L->stream, L->bpack, L->bflags, L->bmap, offs);
+    }
+    HIPCHK(hipMemcpyAsync(L->bflags_host, L->bflags, sizeof(int32_t) * (4 + kinds), hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
     *count = L->bflags_host[0];
-    *d = L->bpack;
-    *grid = dim3((unsigned)std::max(1, L->bflags_host[1]), (unsigned)std::max(1, *count));
+    if (kinds > 0) {
+        *S = SolveLaunch{};
+        S->d = L->bpack;
+        S->ny = (unsigned)std::max(1, *count);
+        S->flat = true;
+        for (int k = 0; k < kinds; k++) {
+            S->map[k] = L->bmap + G.map_off[k];
+            S->blocks[k] = (unsigned)L->bflags_host[4 + k];
+        }
+    } else {
+        *S = group_launch(G, L->bpack, (unsigned)std::max(1, *count));
+        S->blocks[0] = (unsigned)std::max(1, L->bflags_host[1]);
+    }
     return FG_OK;
 }
 
@@ -749,23 +943,24 @@ struct KProf {
 // scenes of the group run the same sweep and iteration index in lock step;
 // a scene whose solve has stopped skips the launches that remain.
 // ---------------------------------------------------------------------------
-// One V-cycle z = M r over the scenes of `d` (ny of them) whose solve uses
-// it; which: the residual buffer (0: cr, 1: cr2).  Level-l launches cover
-// the largest scene's level l; scenes with fewer levels skip them.
-void launch_vcycle(const Group& G, hipStream_t st, const SceneDev* d, unsigned ny, dim3 pcg_grid, int which)
+// One V-cycle z = M r over the scenes of S whose solve uses it; which: the
+// residual buffer (0: cr, 1: cr2).  Level-l launches cover the scenes that
+// have level l; a kind with no blocks is not launched.
+void launch_vcycle(const Group& G, hipStream_t st, const SolveLaunch& S, int which)
 {
     const int n = G.mg_levels;
-    launch_mg_down0(dim3(G.mg_grid[1], ny), st, d, which);
+    if (S.blocks[1]) launch_mg_down0(S.grid(1), st, S.d, S.map[1], which);
     for (int l = 1; l <= n - 2; l++) {
-        launch_mg_resid(dim3(G.mg_grid[l], ny), st, d, l);
-        launch_mg_restrict(dim3(G.mg_grid[l + 1], ny), st, d, l + 1);
+        if (S.blocks[1 + l]) launch_mg_resid(S.grid(1 + l), st, S.d, S.map[1 + l], l);
+        if (S.blocks[2 + l]) launch_mg_restrict(S.grid(2 + l), st, S.d, S.map[2 + l], l + 1);
     }
-    launch_mg_bottom(dim3(1, ny), st, d);
+    launch_mg_bottom(dim3(1, S.ny), st, S.d);
     for (int l = n - 2; l >= 1; l--) {
-        launch_mg_prolong(dim3(G.mg_grid[l], ny), st, d, l);
-        launch_mg_post(dim3(G.mg_grid[l], ny), st, d, l);
+        if (!S.blocks[1 + l]) continue;
+        launch_mg_prolong(S.grid(1 + l), st, S.d, S.map[1 + l], l);
+        launch_mg_post(S.grid(1 + l), st, S.d, S.map[1 + l], l);
     }
-    launch_mg_up0(pcg_grid, st, d, which);
+    launch_mg_up0(S.grid(0), st, S.d, S.map[0], which);
 }
 
 int run_solve(Group& G, bool rhs_from_grid)
@@ -790,51 +985,49 @@ int run_solve(Group& G, bool rhs_from_grid)
     }
     constexpr int kPoll = 16;
     int32_t flags[3];
-    int launched = 0;
     for (int sweep = 0; sweep <= kMaxSweeps; sweep++) {
         if (sweep > 0) launch_refine_update(g.pcg, st, G.d);
         launch_refine_residual(g.pcg, st, G.d);
         launch_cg_init(g.pcg, st, G.d, sweep);
+        // the scenes whose solve goes on: several are packed (and re-packed
+        // at every poll as they stop); one keeps its own descriptor
+        SolveLaunch S;
+        int count = 0;
+        if (G.n > 1) {
+            FGCHK(pack_iterating(G, &S, &count));
+        } else {
+            FGCHK(poll_group(G, -1, flags));
+            count = flags[1];
+            S = group_launch(G, G.d, 1);
+        }
+        if (count == 0) break;
         if (G.mg_levels) {
             KProf kp(G, FG_K_VCYCLE);
-            launch_vcycle(G, st, G.d, (unsigned)G.n, g.pcg, 0);
+            launch_vcycle(G, st, S, 0);
             kp.done(0);
-        }
-        FGCHK(poll_group(G, -1, flags));
-        if (!flags[0]) break;
-        // several scenes: launch over the packed iterating ones only, and
-        // re-pack at every poll as scenes stop
-        const SceneDev* dit = G.d;
-        dim3 git = g.pcg;
-        int count = G.n;
-        if (G.n > 1) {
-            FGCHK(pack_iterating(G, &dit, &git, &count));
-        } else {
-            count = flags[1];
         }
         int it = 0;
         while (count > 0) {
             for (int k = 0; k < kPoll; k++, it++) {
                 {
                     KProf kp(G, FG_K_PCG_A, true);
-                    launch_pcg_a(git, st, dit, it);
+                    launch_pcg_a(S.grid(0), st, S.d, S.map[0], it);
                     kp.done(0);
                 }
                 {
                     KProf kp(G, FG_K_PCG_B, true);
-                    launch_pcg_b(git, st, dit, it);
+                    launch_pcg_b(S.grid(0), st, S.d, S.map[0], it);
                     kp.done(0);
                 }
                 if (G.mg_levels) {
-                    // the new residual: pcg_b(it) wrote cr2 for even it, cr for odd
+                    // the new residual: iteration it writes cr2 for even it, cr for odd
                     KProf kp(G, FG_K_VCYCLE);
-                    launch_vcycle(G, st, dit, git.y, git, (it & 1) ? 0 : 1);
+                    launch_vcycle(G, st, S, (it & 1) ? 0 : 1);
                     kp.done(0);
                 }
-                launched++;
             }
             if (G.n > 1) {
-                FGCHK(pack_iterating(G, &dit, &git, &count));
+                FGCHK(pack_iterating(G, &S, &count));
             } else {
                 FGCHK(poll_group(G, -1, flags));
                 count = flags[1];
@@ -851,7 +1044,6 @@ int run_solve(Group& G, bool rhs_from_grid)
             if (s->h.precond == kPrecondMG) G.L->prof_units += (int64_t)(s->hsc->total_iters + s->hsc->sweeps) * s->C;
     for (fg_scene* s : G.sc)
         if (s->hsc->numeric_bad) return fail(FG_E_NUMERIC, "NaN/Inf in the pressure solve");
-    (void)launched;
     return FG_OK;
 }
 

@@ -70,7 +70,8 @@ struct MgLevel {
     int32_t I, J;
     uint8_t* act;     // the cell takes part: one of its 2x2 finer cells does
     float* coef;      // [kMgStencil][I*J]: entry k couples the cell to (i + k/5 - 2, j + k%5 - 2)
-    float* dinv;      // 1 / sum_k |coef_k| on active cells, 0 elsewhere
+    float* dinv;      // 1 / sum_k |coef_k| on active cells, 0 elsewhere; negative on
+                      // rows equal to the level's canonical interior stencil
     float* r;         // restricted residual
     float* x;         // pre-smoothed iterate plus the coarse correction
     float* y;         // residual on the way down, smoothed result on the way up
@@ -78,6 +79,10 @@ struct MgLevel {
 struct MgLevels {
     int32_t n;        // levels; level n-1 is the bottom
     MgLevel lv[kMgMaxLevels];
+    // the Galerkin row of a cell deep inside the liquid (all of its fine
+    // support and coarse neighbourhood active), per level: such rows read
+    // these instead of their stored coefficients
+    float canon[kMgMaxLevels][kMgStencil];
 };
 
 // Per-scene device descriptor.  One array of these lives in device memory;
@@ -218,6 +223,17 @@ __device__ __forceinline__ float bilinear_hat(float x, float y)
     return (linear_hat(x) * linear_hat(y)) * linear_hat(0.f);
 }
 
+// Array pointers the kernels read from descriptors are generic (flat) to the
+// compiler; cast to the global address space they load and store with
+// global instructions (shorter latency, no LDS-counter waits).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> gbl(T* p)
+{
+    return (gptr<T>)p;
+}
+
 // ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------
@@ -229,6 +245,23 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
     if (b >= full) return b;
     const int per = full >> 3;
     return (b & 7) * per + (b >> 3);
+}
+
+// The (scene slot, block within the scene) a block works on.  A launch with
+// a block map runs a flat grid over only the blocks that have work: entry
+// blockIdx.x of the map names them (the map is already in XCD-tiled order).
+// Without one, the grid is (blocks, scenes) and the block index is remapped
+// by xcd_tile when `tiled`.
+struct SceneBlock {
+    int scene, block;
+};
+__device__ __forceinline__ SceneBlock scene_block(const int2* map, bool tiled)
+{
+    if (map) {
+        const int2 e = map[blockIdx.x];
+        return SceneBlock{e.x, e.y};
+    }
+    return SceneBlock{(int)blockIdx.y, tiled ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x};
 }
 
 // wave64 + block reduction of doubles; result valid in thread 0.
@@ -279,10 +312,10 @@ __device__ __forceinline__ bool finite_d(double x) { return x == x && x < 1e300 
 struct Chunk {
     int begin, end;
 };
-__device__ __forceinline__ Chunk my_chunk(const SceneDev& s)
+__device__ __forceinline__ Chunk my_chunk(const SceneDev& s, int blk)
 {
     const int C = s.I * s.J;
-    const int beg = (int)blockIdx.x * s.pcg_chunk;
+    const int beg = blk * s.pcg_chunk;
     return Chunk{beg < C ? beg : C, min(C, beg + s.pcg_chunk)};
 }
 
@@ -307,15 +340,15 @@ __device__ __forceinline__ void sum_partials(const SceneDev& s, int slot, int nb
     for (int q = 0; q < NV; q++) out[q] = bc[q];
 }
 
-// Block-reduce NV doubles and store them as this block's partials.
+// Block-reduce NV doubles and store them as the partials of solver block blk.
 template <int NV>
-__device__ __forceinline__ void put_partials(const SceneDev& s, int slot, double (&v)[NV])
+__device__ __forceinline__ void put_partials(const SceneDev& s, int slot, double (&v)[NV], int blk)
 {
     __shared__ double red[kBlock / 64];
 #pragma unroll
     for (int q = 0; q < NV; q++) {
         const double t = block_sum(v[q], red);
-        if (threadIdx.x == 0) s.partials[(size_t)(slot + q) * s.max_blocks + blockIdx.x] = t;
+        if (threadIdx.x == 0) s.partials[(size_t)(slot + q) * s.max_blocks + blk] = t;
     }
 }
 

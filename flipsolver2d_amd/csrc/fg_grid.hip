@@ -667,7 +667,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(const SceneDev* scenes)
     if (!s.active) return;
     if ((int)blockIdx.x >= s.pcg_blocks) return;
     const int I = s.I, J = s.J;
-    const Chunk ch = my_chunk(s);
+    const Chunk ch = my_chunk(s, blockIdx.x);
     double bb = 0.0;
     for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
         const int i = c / J, j = c - (c / J) * J;
@@ -690,7 +690,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(const SceneDev* scenes)
         bb += val * val;
     }
     double v[1] = {bb};
-    put_partials<1>(s, kSlotBB, v);
+    put_partials<1>(s, kSlotBB, v, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------

@@ -41,13 +41,13 @@ inline void fg_launch(F kernel, dim3 grid, dim3 block, hipStream_t st, Args... a
 FG_DECLARE_LAUNCH(mg_mark, int l);
 FG_DECLARE_LAUNCH(mg_galerkin, int l);
 FG_DECLARE_LAUNCH(mg_sym, int l);
-FG_DECLARE_LAUNCH(mg_down0, int which);
-FG_DECLARE_LAUNCH(mg_resid, int l);
-FG_DECLARE_LAUNCH(mg_restrict, int l);
+FG_DECLARE_LAUNCH(mg_down0, const int2* map, int which);
+FG_DECLARE_LAUNCH(mg_resid, const int2* map, int l);
+FG_DECLARE_LAUNCH(mg_restrict, const int2* map, int l);
 FG_DECLARE_LAUNCH(mg_bottom);
-FG_DECLARE_LAUNCH(mg_prolong, int l);
-FG_DECLARE_LAUNCH(mg_post, int l);
-FG_DECLARE_LAUNCH(mg_up0, int which);
+FG_DECLARE_LAUNCH(mg_prolong, const int2* map, int l);
+FG_DECLARE_LAUNCH(mg_post, const int2* map, int l);
+FG_DECLARE_LAUNCH(mg_up0, const int2* map, int which);
 
 FG_DECLARE_LAUNCH(max_v2);
 FG_DECLARE_LAUNCH(advect, int alive_from_key);
@@ -79,8 +79,8 @@ constexpr int kBfsTile = 32;
 FG_DECLARE_LAUNCH(refine_update);
 FG_DECLARE_LAUNCH(refine_residual);
 FG_DECLARE_LAUNCH(cg_init, int sweep);
-FG_DECLARE_LAUNCH(pcg_a, int it);
-FG_DECLARE_LAUNCH(pcg_b, int it);
+FG_DECLARE_LAUNCH(pcg_a, const int2* map, int it);
+FG_DECLARE_LAUNCH(pcg_b, const int2* map, int it);
 FG_DECLARE_LAUNCH(solve_begin);
 FG_DECLARE_LAUNCH(solve_reset);
 

@@ -54,30 +54,64 @@ __device__ __forceinline__ float l1_diag0(uint8_t code)
 // 1-D restriction weights of the fine cells 2c-1 .. 2c+2 (times 4)
 __device__ __forceinline__ float w1(int a) { return (a == 1 || a == 2) ? 0.75f : 0.25f; }
 
-// Coarse correction at fine cell (fi, fj) of a level whose coarser level is
-// C: sum over the (up to) four nearest active coarse cells of weight * y.
-__device__ __forceinline__ float prolong_at(const MgLevel& Cl, int fi, int fj)
+// Coarse correction at fine cell (fi, fj) from the coarser level's y (I x J):
+// the (up to) four nearest coarse cells, weights 9/16, 3/16, 3/16, 1/16.
+// The four loads are unconditional (clamped index, weight 0 outside), so
+// they issue together.  Inactive coarse cells hold y = 0.
+__device__ __forceinline__ float prolong_at(gptr<const float> y, int I, int J, int fi, int fj)
 {
     const int p0 = fi >> 1, q0 = fj >> 1;
     const int p1 = (fi & 1) ? p0 + 1 : p0 - 1, q1 = (fj & 1) ? q0 + 1 : q0 - 1;
     const int pi[2] = {p0, p1}, qj[2] = {q0, q1};
     const float wi[2] = {0.75f, 0.25f};
+    float v[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int i = min(max(pi[a], 0), I - 1), j = min(max(qj[b], 0), J - 1);
+            v[a][b] = y[i * J + j];
+        }
     float acc = 0.f;
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) {
-            const int i = pi[a], j = qj[b];
-            if (i < 0 || i >= Cl.I || j < 0 || j >= Cl.J) continue;
-            acc += (wi[a] * wi[b]) * Cl.y[i * Cl.J + j];   // inactive coarse cells hold y = 0
+            const bool ok = pi[a] >= 0 && pi[a] < I && qj[b] >= 0 && qj[b] < J;
+            acc += ((wi[a] * wi[b]) * v[a][b]) * (ok ? 1.f : 0.f);
         }
     return acc;
+}
+
+// sum_k a_k v_k over a level-l row with the level's 25 stencil entries: the
+// canonical stencil when every lane of the wave has a canonical row (the
+// branch is wave-uniform, so their coefficient loads are skipped), else the
+// stored coefficients (all 25 loads issued together).
+__device__ __forceinline__ float row_dot(const MgLevels& M, int l, gptr<const float> coef, int CL, int c,
+                                         bool canon_row, const float (&v)[kMgStencil])
+{
+    const float* cn = M.canon[l];
+    float ax = 0.f;
+    if (__all(canon_row)) {
+#pragma unroll
+        for (int k = 0; k < kMgStencil; k++) ax += cn[k] * v[k];
+    } else {
+        float a[kMgStencil];
+#pragma unroll
+        for (int k = 0; k < kMgStencil; k++) a[k] = coef[(size_t)k * CL + c];
+        // 0/1 factors rather than a select, so the loads stay unconditional
+        const float mc = canon_row ? 1.f : 0.f, ms = 1.f - mc;
+#pragma unroll
+        for (int k = 0; k < kMgStencil; k++) ax += (cn[k] * mc + a[k] * ms) * v[k];
+    }
+    return ax;
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// set-up, per level l >= 1 in increasing order
+// set-up, per level l >= 1 in increasing order (one launch per level for all
+// scenes of the group; a thread per level-l cell)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_mg_mark(const SceneDev* scenes, int l)
 {
@@ -88,19 +122,20 @@ __global__ void __launch_bounds__(kBlock) k_mg_mark(const SceneDev* scenes, int 
     const MgLevel& L = M.lv[l];
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= L.I * L.J) return;
-    const int ci = c / L.J, cj = c - (c / L.J) * L.J;
+    const int ci = c / L.J, cj = c - ci * L.J;
     const int FI = l == 1 ? s.I : M.lv[l - 1].I, FJ = l == 1 ? s.J : M.lv[l - 1].J;
+    gptr<const uint8_t> fact = gbl<const uint8_t>(l == 1 ? s.stencil : M.lv[l - 1].act);
     uint8_t act = 0;
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) {
             const int fi = 2 * ci + a, fj = 2 * cj + b;
-            if (fi >= FI || fj >= FJ) continue;
-            const int f = fi * FJ + fj;
-            act |= l == 1 ? (uint8_t)(s.stencil[f] >> 7) : M.lv[l - 1].act[f];
+            const bool ok = fi < FI && fj < FJ;
+            const uint8_t v = fact[ok ? fi * FJ + fj : 0];
+            if (ok) act |= l == 1 ? (uint8_t)(v >> 7) : v;
         }
-    L.act[c] = act;
+    gbl(L.act)[c] = act;
 }
 
 // floor(t / 2) for small t of either sign
@@ -204,13 +239,16 @@ __global__ void __launch_bounds__(kBlock) k_mg_galerkin(const SceneDev* scenes, 
         for (int k = 0; k < kMgStencil; k++) L.coef[(size_t)k * CL + c] = 0.f;
         return;
     }
-    const int ci = c / L.J, cj = c - (c / L.J) * L.J;
+    const int ci = c / L.J, cj = c - ci * L.J;
     if (l == 1) galerkin_row<true>(s, M, l, c, ci, cj);
     else galerkin_row<false>(s, M, l, c, ci, cj);
 }
 
 // Mirror the upper entries into the lower ones (exact symmetry) and form the
 // L1 diagonal.  Reads only entries k > 12, writes only k < 12 of its own row.
+// Rows equal to the level's canonical interior stencil (every entry within
+// 1e-5 of the canonical diagonal; other rows differ by more than 1 %) are
+// flagged by a negative dinv and read the canonical stencil later.
 __global__ void __launch_bounds__(kBlock) k_mg_sym(const SceneDev* scenes, int l)
 {
     const SceneDev& s = scenes[blockIdx.y];
@@ -218,47 +256,62 @@ __global__ void __launch_bounds__(kBlock) k_mg_sym(const SceneDev* scenes, int l
     const MgLevels& M = *s.mg;
     if (l >= M.n) return;
     const MgLevel& L = M.lv[l];
-    const int CL = L.I * L.J;
+    const int LI = L.I, LJ = L.J, CL = LI * LJ;
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= CL) return;
-    const int ci = c / L.J, cj = c - (c / L.J) * L.J;
-    float sum = 0.f;
+    const int ci = c / LJ, cj = c - ci * LJ;
+    gptr<float> coef = gbl(L.coef);
+    float v[kMgStencil];
 #pragma unroll
     for (int k = 0; k < kMgStencil; k++) {
-        float v;
         if (k < 12) {
             const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
-            v = (ni >= 0 && ni < L.I && nj >= 0 && nj < L.J) ? L.coef[(size_t)(24 - k) * CL + ni * L.J + nj] : 0.f;
-            L.coef[(size_t)k * CL + c] = v;
+            const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
+            const float m = coef[(size_t)(24 - k) * CL + (ok ? ni * LJ + nj : c)];
+            v[k] = ok ? m : 0.f;
         } else {
-            v = L.coef[(size_t)k * CL + c];
+            v[k] = coef[(size_t)k * CL + c];
         }
-        sum += fabsf(v);
     }
-    L.dinv[c] = (L.act[c] && sum > 0.f) ? 1.0f / sum : 0.f;
+    const float* cn = M.canon[l];
+    const float tol = 1e-5f * cn[12];
+    float sum = 0.f;
+    bool canon_row = true;
+#pragma unroll
+    for (int k = 0; k < kMgStencil; k++) {
+        if (k < 12) coef[(size_t)k * CL + c] = v[k];
+        sum += fabsf(v[k]);
+        canon_row = canon_row && fabsf(v[k] - cn[k]) <= tol;
+    }
+    const float di = (L.act[c] && sum > 0.f) ? 1.0f / sum : 0.f;
+    gbl(L.dinv)[c] = (canon_row && di > 0.f) ? -di : di;
 }
 
 // ---------------------------------------------------------------------------
-// the V-cycle: z = M r, r the current PCG residual (which: 0 = cr, 1 = cr2)
+// the V-cycle: z = M r, r the current PCG residual (which: 0 = cr, 1 = cr2).
+// Each kernel takes its scenes and blocks from a block map (flat launch over
+// the packed iterating scenes) or from a (blocks, scenes) grid.  All loads
+// of a thread are unconditional (clamped indices, masked afterwards) so they
+// issue together; the kernels are latency-bound.
 // ---------------------------------------------------------------------------
 
 // Level 0 -> 1: x0 = r / D1 on the fine level, res0 = r - A' x0, and
 // r1 = R res0, one level-1 cell per thread over its 4x4 fine support (a 6x6
 // fine patch with the stencil's reach).
-__global__ void __launch_bounds__(kBlock) k_mg_down0(const SceneDev* scenes, int which)
+__global__ void __launch_bounds__(kBlock) k_mg_down0(const SceneDev* scenes, const int2* map, int which)
 {
-    const SceneDev& s = scenes[blockIdx.y];
+    const SceneBlock sb = scene_block(map, true);
+    const SceneDev& s = scenes[sb.scene];
     if (!mg_iterating(s)) return;
     const MgLevel& L1 = s.mg->lv[1];
-    const int c = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (c >= L1.I * L1.J) return;
-    if (!L1.act[c]) {
-        L1.r[c] = 0.f;
-        return;
-    }
-    const int ci = c / L1.J, cj = c - (c / L1.J) * L1.J;
+    const int J1 = L1.J;
+    const int c = sb.block * kBlock + threadIdx.x;
+    if (c >= L1.I * J1) return;
     const int I = s.I, J = s.J;
-    const float* __restrict__ r = which ? s.cr2 : s.cr;
+    gptr<const float> r = gbl<const float>(which ? s.cr2 : s.cr);
+    gptr<const uint8_t> code = gbl<const uint8_t>(s.stencil);
+    const int ci = c / J1, cj = c - ci * J1;
+    const bool act = gbl<const uint8_t>(L1.act)[c];
     uint8_t cd[6][6];
     float rv[6][6];
 #pragma unroll
@@ -267,8 +320,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_down0(const SceneDev* scenes, int
         for (int b = 0; b < 6; b++) {
             const int fi = 2 * ci - 2 + a, fj = 2 * cj - 2 + b;
             const bool in = fi >= 0 && fi < I && fj >= 0 && fj < J;
-            cd[a][b] = in ? s.stencil[fi * J + fj] : 0;
-            rv[a][b] = in ? r[fi * J + fj] : 0.f;
+            const int f = min(max(fi, 0), I - 1) * J + min(max(fj, 0), J - 1);   // clamped, not selected
+            const uint8_t k = code[f];
+            const float v = r[f];
+            cd[a][b] = in ? k : 0;
+            rv[a][b] = v * (in ? 1.f : 0.f);
         }
     float x0[6][6];
 #pragma unroll
@@ -284,73 +340,79 @@ __global__ void __launch_bounds__(kBlock) k_mg_down0(const SceneDev* scenes, int
 #pragma unroll
         for (int b = 1; b < 5; b++) {
             const uint8_t k = cd[a][b];
-            if (!(k & 0x80)) continue;
             float ax = (float)((k >> 4) & 7) * x0[a][b];
-            if (k & 1) ax -= x0[a - 1][b];
-            if (k & 2) ax -= x0[a + 1][b];
-            if (k & 4) ax -= x0[a][b - 1];
-            if (k & 8) ax -= x0[a][b + 1];
-            acc += (w1(a - 1) * w1(b - 1)) * (rv[a][b] - ax);
+            ax -= (k & 1) ? x0[a - 1][b] : 0.f;
+            ax -= (k & 2) ? x0[a + 1][b] : 0.f;
+            ax -= (k & 4) ? x0[a][b - 1] : 0.f;
+            ax -= (k & 8) ? x0[a][b + 1] : 0.f;
+            acc += (k & 0x80) ? (w1(a - 1) * w1(b - 1)) * (rv[a][b] - ax) : 0.f;
         }
-    L1.r[c] = 0.25f * acc;
+    gbl(L1.r)[c] = act ? 0.25f * acc : 0.f;
 }
 
 // Level l (1 <= l <= n-2): x = r * dinv, y = res = r - A_l x.
-__global__ void __launch_bounds__(kBlock) k_mg_resid(const SceneDev* scenes, int l)
+__global__ void __launch_bounds__(kBlock) k_mg_resid(const SceneDev* scenes, const int2* map, int l)
 {
-    const SceneDev& s = scenes[blockIdx.y];
+    const SceneBlock sb = scene_block(map, true);
+    const SceneDev& s = scenes[sb.scene];
     if (!mg_iterating(s)) return;
     const MgLevels& M = *s.mg;
     if (l > M.n - 2) return;
     const MgLevel& L = M.lv[l];
-    const int CL = L.I * L.J;
-    const int c = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const int LI = L.I, LJ = L.J, CL = LI * LJ;
+    const int c = sb.block * kBlock + threadIdx.x;
     if (c >= CL) return;
-    if (!L.act[c]) {
-        L.y[c] = 0.f;
-        return;
-    }
-    const int ci = c / L.J, cj = c - (c / L.J) * L.J;
-    float ax = 0.f;
+    gptr<const float> r = gbl<const float>(L.r), dinv = gbl<const float>(L.dinv);
+    const int ci = c / LJ, cj = c - ci * LJ;
+    float xv[kMgStencil];
 #pragma unroll
     for (int k = 0; k < kMgStencil; k++) {
         const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
-        if (ni < 0 || ni >= L.I || nj < 0 || nj >= L.J) continue;
-        const int n = ni * L.J + nj;
-        ax += L.coef[(size_t)k * CL + c] * (L.r[n] * L.dinv[n]);
+        const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
+        const int n = min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1);   // clamped, not selected
+        const float rn = r[n], dn = dinv[n];
+        xv[k] = (rn * fabsf(dn)) * (ok ? 1.f : 0.f);   // a 0/1 factor, not a select: the load stays unconditional
     }
-    L.y[c] = L.r[c] - ax;
+    const float rc = r[c];
+    const bool canon_row = dinv[c] < 0.f;
+    const float ax = row_dot(M, l, gbl<const float>(L.coef), CL, c, canon_row, xv);
+    gbl(L.y)[c] = rc - ax;   // inactive rows: r = 0 and no coefficients
 }
 
 // Level l-1 -> l (2 <= l <= n-1): r_l = R y_{l-1}.
-__global__ void __launch_bounds__(kBlock) k_mg_restrict(const SceneDev* scenes, int l)
+__global__ void __launch_bounds__(kBlock) k_mg_restrict(const SceneDev* scenes, const int2* map, int l)
 {
-    const SceneDev& s = scenes[blockIdx.y];
+    const SceneBlock sb = scene_block(map, true);
+    const SceneDev& s = scenes[sb.scene];
     if (!mg_iterating(s)) return;
     const MgLevels& M = *s.mg;
     if (l > M.n - 1) return;
     const MgLevel& L = M.lv[l];
     const MgLevel& F = M.lv[l - 1];
-    const int c = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (c >= L.I * L.J) return;
-    if (!L.act[c]) {
-        L.r[c] = 0.f;
-        return;
-    }
-    const int ci = c / L.J, cj = c - (c / L.J) * L.J;
-    float acc = 0.f;
+    const int LJ = L.J, FI = F.I, FJ = F.J;
+    const int c = sb.block * kBlock + threadIdx.x;
+    if (c >= L.I * LJ) return;
+    gptr<const float> fy = gbl<const float>(F.y);
+    const int ci = c / LJ, cj = c - ci * LJ;
+    const bool act = gbl<const uint8_t>(L.act)[c];
+    float v[4][4];
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
-        const int fi = 2 * ci - 1 + a;
-        if (fi < 0 || fi >= F.I) continue;
+    for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) {
-            const int fj = 2 * cj - 1 + b;
-            if (fj < 0 || fj >= F.J) continue;
-            acc += (w1(a) * w1(b)) * F.y[fi * F.J + fj];   // inactive fine cells hold y = 0
+            const int fi = min(max(2 * ci - 1 + a, 0), FI - 1), fj = min(max(2 * cj - 1 + b, 0), FJ - 1);
+            v[a][b] = fy[fi * FJ + fj];
         }
-    }
-    L.r[c] = 0.25f * acc;
+    float acc = 0.f;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int fi = 2 * ci - 1 + a, fj = 2 * cj - 1 + b;
+            const bool ok = fi >= 0 && fi < FI && fj >= 0 && fj < FJ;
+            acc += ((w1(a) * w1(b)) * v[a][b]) * (ok ? 1.f : 0.f);   // inactive fine cells hold y = 0
+        }
+    gbl(L.r)[c] = act ? 0.25f * acc : 0.f;
 }
 
 // Bottom level: kMgBottomSweeps L1-Jacobi sweeps from zero, one block per
@@ -369,10 +431,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_bottom(const SceneDev* scenes)
     const int c = threadIdx.x;
     const bool in = c < CL;
     const int ci = in ? c / L.J : 0, cj = in ? c - ci * L.J : 0;
+    gptr<const float> coef = gbl<const float>(L.coef);
     float cf[kMgStencil];
 #pragma unroll
-    for (int k = 0; k < kMgStencil; k++) cf[k] = in ? L.coef[(size_t)k * CL + c] : 0.f;
-    const float dv = in ? L.dinv[c] : 0.f, rv = in ? L.r[c] : 0.f;
+    for (int k = 0; k < kMgStencil; k++) cf[k] = coef[(size_t)k * CL + (in ? c : 0)];
+    const float dv = fabsf(gbl<const float>(L.dinv)[in ? c : 0]), rv = gbl<const float>(L.r)[in ? c : 0];
     for (int t = threadIdx.x; t < 2 * W * W; t += kBlock) (&xs[0][0])[t] = 0.f;
     __syncthreads();
     const int home = (ci + 2) * W + (cj + 2);
@@ -387,99 +450,114 @@ __global__ void __launch_bounds__(kBlock) k_mg_bottom(const SceneDev* scenes)
         __syncthreads();
         cur ^= 1;
     }
-    if (in) L.y[c] = xs[cur][home];
+    if (in) gbl(L.y)[c] = xs[cur][home];
 }
 
 // Level l (1 <= l <= n-2), up: x = r * dinv + P y_{l+1}.
-__global__ void __launch_bounds__(kBlock) k_mg_prolong(const SceneDev* scenes, int l)
+__global__ void __launch_bounds__(kBlock) k_mg_prolong(const SceneDev* scenes, const int2* map, int l)
 {
-    const SceneDev& s = scenes[blockIdx.y];
+    const SceneBlock sb = scene_block(map, true);
+    const SceneDev& s = scenes[sb.scene];
     if (!mg_iterating(s)) return;
     const MgLevels& M = *s.mg;
     if (l > M.n - 2) return;
     const MgLevel& L = M.lv[l];
-    const int c = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (c >= L.I * L.J) return;
-    if (!L.act[c]) {
-        L.x[c] = 0.f;
-        return;
-    }
-    const int ci = c / L.J, cj = c - (c / L.J) * L.J;
-    L.x[c] = L.r[c] * L.dinv[c] + prolong_at(M.lv[l + 1], ci, cj);
+    const MgLevel& Cl = M.lv[l + 1];
+    const int LJ = L.J;
+    const int c = sb.block * kBlock + threadIdx.x;
+    if (c >= L.I * LJ) return;
+    const int ci = c / LJ, cj = c - ci * LJ;
+    const bool act = gbl<const uint8_t>(L.act)[c];
+    const float rc = gbl<const float>(L.r)[c], dc = gbl<const float>(L.dinv)[c];
+    const float pc = prolong_at(gbl<const float>(Cl.y), Cl.I, Cl.J, ci, cj);
+    gbl(L.x)[c] = act ? rc * fabsf(dc) + pc : 0.f;
 }
 
 // Level l (1 <= l <= n-2), up: y = x + dinv (r - A_l x).
-__global__ void __launch_bounds__(kBlock) k_mg_post(const SceneDev* scenes, int l)
+__global__ void __launch_bounds__(kBlock) k_mg_post(const SceneDev* scenes, const int2* map, int l)
 {
-    const SceneDev& s = scenes[blockIdx.y];
+    const SceneBlock sb = scene_block(map, true);
+    const SceneDev& s = scenes[sb.scene];
     if (!mg_iterating(s)) return;
     const MgLevels& M = *s.mg;
     if (l > M.n - 2) return;
     const MgLevel& L = M.lv[l];
-    const int CL = L.I * L.J;
-    const int c = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const int LI = L.I, LJ = L.J, CL = LI * LJ;
+    const int c = sb.block * kBlock + threadIdx.x;
     if (c >= CL) return;
-    if (!L.act[c]) {
-        L.y[c] = 0.f;
-        return;
-    }
-    const int ci = c / L.J, cj = c - (c / L.J) * L.J;
-    float ax = 0.f;
+    gptr<const float> x = gbl<const float>(L.x);
+    const int ci = c / LJ, cj = c - ci * LJ;
+    float xv[kMgStencil];
 #pragma unroll
     for (int k = 0; k < kMgStencil; k++) {
         const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
-        if (ni < 0 || ni >= L.I || nj < 0 || nj >= L.J) continue;
-        ax += L.coef[(size_t)k * CL + c] * L.x[ni * L.J + nj];
+        const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
+        const float v = x[min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1)];
+        xv[k] = v * (ok ? 1.f : 0.f);
     }
-    L.y[c] = L.x[c] + L.dinv[c] * (L.r[c] - ax);
+    const float dc = gbl<const float>(L.dinv)[c], rc = gbl<const float>(L.r)[c];
+    const float ax = row_dot(M, l, gbl<const float>(L.coef), CL, c, dc < 0.f, xv);
+    gbl(L.y)[c] = xv[12] + fabsf(dc) * (rc - ax);   // inactive rows: x = 0, dinv = 0
 }
 
 // Level 0, up: z0 = r / D1 + P y1 at the cell and its fluid neighbours, then
 // z = z0 + (r - A' z0) / D1; z = r on non-fluid rows.  Writes z and the
 // partials of r.z and r.r the next PCG iteration reduces (solver chunks).
-__global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, int which)
+__global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, const int2* map, int which)
 {
-    const SceneDev& s = scenes[blockIdx.y];
+    const SceneBlock sb = scene_block(map, false);
+    const SceneDev& s = scenes[sb.scene];
     if (!mg_iterating(s)) return;
-    if ((int)blockIdx.x >= s.pcg_blocks) return;
+    if (sb.block >= s.pcg_blocks) return;
     const int J = s.J;
     const MgLevel& L1 = s.mg->lv[1];
-    const float* __restrict__ r = which ? s.cr2 : s.cr;
-    const uint8_t* __restrict__ code = s.stencil;
-    const Chunk ch = my_chunk(s);
+    gptr<const float> y1 = gbl<const float>(L1.y);
+    const int I1 = L1.I, J1 = L1.J;
+    gptr<const float> r = gbl<const float>(which ? s.cr2 : s.cr);
+    gptr<const uint8_t> code = gbl<const uint8_t>(s.stencil);
+    gptr<float> z = gbl(s.cu);
+    const Chunk ch = my_chunk(s, sb.block);
     double rz = 0.0, rr = 0.0;
     for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
+        const int ci = c / J, cj = c - ci * J;
         const uint8_t k = code[c];
         const float rc = r[c];
-        float z = rc;
-        const float dc = l1_diag0(k);
-        if ((k & 0x80) && dc > 0.f) {
-            const int ci = c / J, cj = c - (c / J) * J;
-            const int offs[5] = {0, -J, J, -1, 1};
-            const int di[5] = {0, -1, 1, 0, 0}, dj[5] = {0, 0, 0, -1, 1};
-            float z0[5];
+        const int di[5] = {0, -1, 1, 0, 0}, dj[5] = {0, 0, 0, -1, 1};
+        // the fluid neighbours that enter A' (not the row-wrapping coupling)
+        bool use[5];
+        use[0] = true;
 #pragma unroll
-            for (int e = 0; e < 5; e++) {
-                z0[e] = 0.f;
-                if (e > 0 && !(k & (1 << (e - 1)))) continue;   // only fluid neighbours enter A'
-                if ((e == 3 && cj == 0) || (e == 4 && cj == J - 1)) continue;   // row-wrapping coupling
-                const int g = c + offs[e];
-                const uint8_t kg = e == 0 ? k : code[g];
-                const float dg = l1_diag0(kg);
-                const float rg = e == 0 ? rc : r[g];
-                z0[e] = (dg > 0.f ? rg / dg : 0.f) + prolong_at(L1, ci + di[e], cj + dj[e]);
-            }
-            float ax = (float)((k >> 4) & 7) * z0[0];
+        for (int e = 1; e < 5; e++) use[e] = (k & (1 << (e - 1))) != 0;
+        use[3] = use[3] && cj > 0;
+        use[4] = use[4] && cj < J - 1;
+        uint8_t kg[5];
+        float rg[5], pg[5];
+        const int I = s.I;
 #pragma unroll
-            for (int e = 1; e < 5; e++) ax -= z0[e];   // zero where no coupling
-            z = z0[0] + (rc - ax) / dc;
+        for (int e = 0; e < 5; e++) {
+            // clamped coordinates (not a select of c), so every load issues
+            const int gi = min(max(ci + di[e], 0), I - 1), gj = min(max(cj + dj[e], 0), J - 1);
+            kg[e] = code[gi * J + gj];
+            rg[e] = r[gi * J + gj];
+            pg[e] = prolong_at(y1, I1, J1, gi, gj);
         }
-        s.cu[c] = z;
-        rz += (double)rc * (double)z;
+        float z0[5];
+#pragma unroll
+        for (int e = 0; e < 5; e++) {
+            const float dg = l1_diag0(kg[e]);
+            z0[e] = use[e] ? (dg > 0.f ? rg[e] / dg : 0.f) + pg[e] : 0.f;
+        }
+        const float dc = l1_diag0(k);
+        float ax = (float)((k >> 4) & 7) * z0[0];
+#pragma unroll
+        for (int e = 1; e < 5; e++) ax -= z0[e];
+        const float zc = ((k & 0x80) && dc > 0.f) ? z0[0] + (rc - ax) / dc : rc;
+        z[c] = zc;
+        rz += (double)rc * (double)zc;
         rr += (double)rc * (double)rc;
     }
     double v[2] = {rz, rr};
-    put_partials<2>(s, kSlotCG, v);
+    put_partials<2>(s, kSlotCG, v, sb.block);
 }
 
 }  // namespace fg
@@ -487,14 +565,19 @@ __global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, int w
 namespace fg {
 #define FG_MG_LAUNCH1(name, T, a) \
     void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc, T a) { fg_launch(k_##name, g, dim3(kBlock), st, sc, a); }
+#define FG_MG_LAUNCH_MAP(name, T, a) \
+    void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc, const int2* map, T a) \
+    { \
+        fg_launch(k_##name, g, dim3(kBlock), st, sc, map, a); \
+    }
 FG_MG_LAUNCH1(mg_mark, int, l)
 FG_MG_LAUNCH1(mg_galerkin, int, l)
 FG_MG_LAUNCH1(mg_sym, int, l)
-FG_MG_LAUNCH1(mg_down0, int, which)
-FG_MG_LAUNCH1(mg_resid, int, l)
-FG_MG_LAUNCH1(mg_restrict, int, l)
-FG_MG_LAUNCH1(mg_prolong, int, l)
-FG_MG_LAUNCH1(mg_post, int, l)
-FG_MG_LAUNCH1(mg_up0, int, which)
+FG_MG_LAUNCH_MAP(mg_down0, int, which)
+FG_MG_LAUNCH_MAP(mg_resid, int, l)
+FG_MG_LAUNCH_MAP(mg_restrict, int, l)
+FG_MG_LAUNCH_MAP(mg_prolong, int, l)
+FG_MG_LAUNCH_MAP(mg_post, int, l)
+FG_MG_LAUNCH_MAP(mg_up0, int, which)
 void launch_mg_bottom(dim3 g, hipStream_t st, const SceneDev* sc) { fg_launch(k_mg_bottom, g, dim3(kBlock), st, sc); }
 }  // namespace fg

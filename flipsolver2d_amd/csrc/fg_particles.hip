@@ -88,21 +88,85 @@ __device__ __forceinline__ float2 grad_trunc(const float* g, int I, int J, float
     return make_float2((ip1 - im1) / 2.f, (jp1 - jm1) / 2.f);
 }
 
+// The 3x3 block of clamped SDF values (get_ext) around a centre cell.  The
+// search below moves by fractions of a cell per step, so most of its lookups
+// hit the block and need no memory access; a lookup outside reloads the
+// block around its own cell (nine independent loads).  Values and
+// arithmetic are those of lerp_grid / grad_trunc, so results are unchanged.
+struct SdfBlock {
+    int ci, cj;
+    float v[3][3];
+};
+
+__device__ __forceinline__ void sdf_block_load(SdfBlock& B, const float* g, int I, int J, int ci, int cj)
+{
+    B.ci = ci;
+    B.cj = cj;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) B.v[a][b] = get_ext(g, I, J, ci - 1 + a, cj - 1 + b);
+}
+
+// value at logical cell (i, j), |i - ci| <= 1 and |j - cj| <= 1
+__device__ __forceinline__ float sdf_block_at(const SdfBlock& B, int i, int j)
+{
+    const int a = i - B.ci, b = j - B.cj;
+    float r[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) r[q] = b < 0 ? B.v[q][0] : (b == 0 ? B.v[q][1] : B.v[q][2]);
+    return a < 0 ? r[0] : (a == 0 ? r[1] : r[2]);
+}
+
+__device__ __forceinline__ bool sdf_block_has(const SdfBlock& B, int i, int j)
+{
+    return i >= B.ci - 1 && i <= B.ci + 1 && j >= B.cj - 1 && j <= B.cj + 1;
+}
+
+// lerp_grid (grid2d.h:186-216) of the solid SDF through the block
+__device__ __forceinline__ float sdf_lerp_cached(SdfBlock& B, const float* g, int I, int J, float i, float j)
+{
+    i = fclamp(i, 0.f, (float)(I - 1));
+    j = fclamp(j, 0.f, (float)(J - 1));
+    const int ci = integr(i), cj = integr(j);
+    const float fi = frac_(i), fj = frac_(j);
+    const int nj = fj >= 0.5f ? cj + 1 : cj - 1;
+    const int ni = fi >= 0.5f ? ci + 1 : ci - 1;
+    const float iF = fi < 0.5f ? 0.5f - fi : fi - 0.5f;
+    const float jF = fj < 0.5f ? 0.5f - fj : fj - 0.5f;
+    if (!sdf_block_has(B, ci, cj) || !sdf_block_has(B, ni, nj)) sdf_block_load(B, g, I, J, ci, cj);
+    const float v1 = lerp1(sdf_block_at(B, ci, cj), sdf_block_at(B, ni, cj), iF);
+    const float v2 = lerp1(sdf_block_at(B, ci, nj), sdf_block_at(B, ni, nj), iF);
+    return lerp1(v1, v2, jF);
+}
+
+// grad_trunc through the block
+__device__ __forceinline__ float2 sdf_grad_cached(SdfBlock& B, const float* g, int I, int J, float x, float y)
+{
+    const int i = (int)x, j = (int)y;
+    if (i != B.ci || j != B.cj) sdf_block_load(B, g, I, J, i, j);
+    const float ip1 = sdf_block_at(B, i + 1, j), im1 = sdf_block_at(B, i - 1, j);
+    const float jp1 = sdf_block_at(B, i, j + 1), jm1 = sdf_block_at(B, i, j - 1);
+    return make_float2((ip1 - im1) / 2.f, (jp1 - jm1) / 2.f);
+}
+
 __device__ float2 closest_surface_point(const float* sdf, int I, int J, float2 pos)
 {
+    SdfBlock B;
+    sdf_block_load(B, sdf, I, J, (int)pos.x, (int)pos.y);
     float2 cp = pos;
-    float value = lerp_grid(sdf, I, J, 0.f, 0.f, pos.x, pos.y);
-    float2 grad = grad_trunc(sdf, I, J, pos.x, pos.y);
+    float value = sdf_lerp_cached(B, sdf, I, J, pos.x, pos.y);
+    float2 grad = sdf_grad_cached(B, sdf, I, J, pos.x, pos.y);
     for (int it = 0; it < 100; it++) {
         float alpha = 1;
         for (int k = 0; k < 10; k++) {
             const float t = alpha * value;
             const float2 q = make_float2(cp.x - grad.x * t, cp.y - grad.y * t);
-            const float vq = lerp_grid(sdf, I, J, 0.f, 0.f, q.x, q.y);
+            const float vq = sdf_lerp_cached(B, sdf, I, J, q.x, q.y);
             if (fabsf(vq) < fabsf(value)) {
                 cp = q;
                 value = vq;
-                grad = grad_trunc(sdf, I, J, q.x, q.y);
+                grad = sdf_grad_cached(B, sdf, I, J, q.x, q.y);
                 if (fabsf(value) < 1e-5f) return cp;
             } else {
                 alpha *= 0.7f;

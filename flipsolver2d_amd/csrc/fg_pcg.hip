@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(kBlock) k_refine_update(const SceneDev* scenes
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active || s.sc->outer_done) return;
     if ((int)blockIdx.x >= s.pcg_blocks) return;
-    const Chunk ch = my_chunk(s);
+    const Chunk ch = my_chunk(s, blockIdx.x);
     for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
         s.x64[c] = s.x64[c] + (double)s.d[c];
         s.d[c] = 0.f;
@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(kBlock) k_refine_residual(const SceneDev* scen
     sum_partials<1>(s, kSlotBB, nb, bb);
     if (blockIdx.x == 0 && threadIdx.x == 0) s.sc->bb = bb[0];
     const int J = s.J;
-    const Chunk ch = my_chunk(s);
+    const Chunk ch = my_chunk(s, blockIdx.x);
     double rr = 0.0;
     for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
         const uint8_t code = s.stencil[c];
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(kBlock) k_refine_residual(const SceneDev* scen
         s.cr[c] = (float)((code & 0x80) ? r64 / s.sA : r64);
     }
     double v[1] = {rr};
-    put_partials<1>(s, kSlotRR64, v);
+    put_partials<1>(s, kSlotRR64, v, blockIdx.x);
 }
 
 constexpr int kItems = 4;                  // cells per thread per pass
@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const SceneDev* scenes, int 
     }
     if (s.precond == kPrecondMG) return;   // the V-cycle forms z and the partials
     const int J = s.J;
-    const Chunk ch = my_chunk(s);
+    const Chunk ch = my_chunk(s, blockIdx.x);
     const float* __restrict__ r = s.cr;
     const uint8_t* __restrict__ code = s.stencil;
     double rz = 0.0, rr = 0.0;
@@ -159,49 +159,63 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const SceneDev* scenes, int 
         rr += (double)rc * (double)rc;
     }
     double v[2] = {rz, rr};
-    put_partials<2>(s, kSlotCG, v);
+    put_partials<2>(s, kSlotCG, v, blockIdx.x);
 }
 
 // KA: p = z + beta p_old; q = A' p; p.q        (z lives in cu)
-__global__ void __launch_bounds__(kBlock) k_pcg_a(const SceneDev* scenes, int it)
+__global__ void __launch_bounds__(kBlock) k_pcg_a(const SceneDev* scenes, const int2* map, int it)
 {
-    const SceneDev& s = scenes[blockIdx.y];
+    const SceneBlock sb = scene_block(map, false);
+    const SceneDev& s = scenes[sb.scene];
     if (!s.active) return;
     SceneScalars* sc = s.sc;
     if (sc->outer_done || sc->inner_done) return;
     const int nb = s.pcg_blocks;
-    if ((int)blockIdx.x >= nb) return;
-    const int J = s.J;
+    if (sb.block >= nb) return;
+    const int J = s.J, C = s.I * s.J;
     const bool first = it == 0;
-    const float* __restrict__ z = s.cu;
-    const float* __restrict__ pold = (it & 1) ? s.cp : s.cs1;
-    float* __restrict__ pnew = (it & 1) ? s.cs1 : s.cp;
-    float* __restrict__ q = s.cw;
-    const uint8_t* __restrict__ code = s.stencil;
-    const Chunk ch = my_chunk(s);
+    gptr<const float> z = gbl<const float>(s.cu);
+    gptr<const float> pold = gbl<const float>((it & 1) ? s.cp : s.cs1);
+    gptr<float> pnew = gbl((it & 1) ? s.cs1 : s.cp);
+    gptr<float> q = gbl(s.cw);
+    gptr<const uint8_t> code = gbl<const uint8_t>(s.stencil);
+    const Chunk ch = my_chunk(s, sb.block);
     double pq = 0.0;
     float beta = 0.f;
     for (int base = ch.begin; base < ch.end; base += kPass) {
-        // loads first (they do not depend on beta)
+        // loads first (they depend neither on beta nor on the stencil code:
+        // clamped indices, masked afterwards), so they issue together
         uint8_t cc[kItems];
         float zc[kItems][5], pc[kItems][5];
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
             const int c = base + threadIdx.x + k * kBlock;
-            cc[k] = c < ch.end ? code[c] : 0;
+            const bool in = c < ch.end;
+            const int c0 = in ? c : ch.begin;
+            const uint8_t code_c = code[c0];
             const int offs[5] = {0, -J, J, -1, 1};
+            float zl[5], pl[5];
 #pragma unroll
             for (int e = 0; e < 5; e++) {
-                const bool need = c < ch.end && (e == 0 || ((cc[k] & 0x80) && (cc[k] & (1 << (e - 1)))));
-                zc[k][e] = need ? z[c + offs[e]] : 0.f;
-                pc[k][e] = (need && !first) ? pold[c + offs[e]] : 0.f;
+                const int idx = min(max(c0 + offs[e], 0), C - 1);
+                zl[e] = z[idx];
+                pl[e] = first ? 0.f : pold[idx];
+            }
+            cc[k] = code_c;   // lanes past the chunk store nothing
+#pragma unroll
+            for (int e = 0; e < 5; e++) {
+                // 0/1 factors rather than selects keep the loads unconditional
+                const bool need = in && (e == 0 || ((cc[k] & 0x80) && (cc[k] & (1 << (e - 1)))));
+                const float m = need ? 1.f : 0.f;
+                zc[k][e] = zl[e] * m;
+                pc[k][e] = pl[e] * m;
             }
         }
         if (base == ch.begin) {
             double t[2];
             sum_partials<2>(s, kSlotCG, nb, t);   // rho = r.z, ||r||^2 of the current residual
             const double rho = t[0], rr = t[1];
-            const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+            const bool lead = sb.block == 0 && threadIdx.x == 0;
             const double rr0 = first ? rr : sc->rr0;
             if (first && lead) sc->rr0 = rr;
             bool stop = false;
@@ -245,34 +259,35 @@ __global__ void __launch_bounds__(kBlock) k_pcg_a(const SceneDev* scenes, int it
         }
     }
     double v[1] = {pq};
-    put_partials<1>(s, kSlotPQ, v);
+    put_partials<1>(s, kSlotPQ, v, sb.block);
 }
 
 // KB: r = r_old - alpha q; d += alpha p; z = M r; r.z, r.r (with the IPP;
 // with the V-cycle, r and d only)
-__global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, int it)
+__global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const int2* map, int it)
 {
-    const SceneDev& s = scenes[blockIdx.y];
+    const SceneBlock sb = scene_block(map, false);
+    const SceneDev& s = scenes[sb.scene];
     if (!s.active) return;
     SceneScalars* sc = s.sc;
     if (sc->outer_done || sc->inner_done) return;
     const int nb = s.pcg_blocks;
-    if ((int)blockIdx.x >= nb) return;
-    const int J = s.J;
-    const float* __restrict__ rold = (it & 1) ? s.cr2 : s.cr;
-    float* __restrict__ rnew = (it & 1) ? s.cr : s.cr2;
-    const float* __restrict__ pnew = (it & 1) ? s.cs1 : s.cp;
-    const float* __restrict__ q = s.cw;
-    float* __restrict__ z = s.cu;
-    float* __restrict__ d = s.d;
-    const uint8_t* __restrict__ code = s.stencil;
-    const Chunk ch = my_chunk(s);
+    if (sb.block >= nb) return;
+    const int J = s.J, C = s.I * s.J;
+    gptr<const float> rold = gbl<const float>((it & 1) ? s.cr2 : s.cr);
+    gptr<float> rnew = gbl((it & 1) ? s.cr : s.cr2);
+    gptr<const float> pnew = gbl<const float>((it & 1) ? s.cs1 : s.cp);
+    gptr<const float> q = gbl<const float>(s.cw);
+    gptr<float> z = gbl(s.cu);
+    gptr<float> d = gbl(s.d);
+    gptr<const uint8_t> code = gbl<const uint8_t>(s.stencil);
+    const Chunk ch = my_chunk(s, sb.block);
     if (s.precond == kPrecondMG) {
         // r and d only; the V-cycle that follows forms z, r.z and r.r
         double t[1];
         sum_partials<1>(s, kSlotPQ, nb, t);
         const double pq = t[0];
-        const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+        const bool lead = sb.block == 0 && threadIdx.x == 0;
         if (!(pq > 0.0) || !finite_d(pq)) {
             if (lead) sc->inner_done = 1;
             return;
@@ -283,8 +298,9 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, int it
             sc->total_iters += 1;
         }
         for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
-            rnew[c] = rold[c] - a * q[c];
-            d[c] = d[c] + a * pnew[c];
+            const float ro = rold[c], qc = q[c], dc = d[c], pc = pnew[c];
+            rnew[c] = ro - a * qc;
+            d[c] = dc + a * pc;
         }
         return;
     }
@@ -297,24 +313,36 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, int it
         for (int k = 0; k < kItems; k++) {
             const int c = base + threadIdx.x + k * kBlock;
             const bool in = c < ch.end;
-            cc[k] = in ? code[c] : 0;
+            const int c0 = in ? c : ch.begin;
+            const uint8_t code_c = code[c0];
             const int offs[5] = {0, -J, J, -1, 1};
+            float rl[5], ql[5];
+#pragma unroll
+            for (int e = 0; e < 5; e++) {
+                const int idx = min(max(c0 + offs[e], 0), C - 1);
+                rl[e] = rold[idx];
+                ql[e] = q[idx];
+            }
+            const uint8_t code_im = code[max(c0 - J, 0)], code_jm = code[max(c0 - 1, 0)];
+            const float pl = pnew[c0], dl = d[c0];
+            cc[k] = code_c;   // lanes past the chunk store nothing
 #pragma unroll
             for (int e = 0; e < 5; e++) {
                 const bool need = in && (e == 0 || ((cc[k] & 0x80) && (cc[k] & (1 << (e - 1)))));
-                rv[k][e] = need ? rold[c + offs[e]] : 0.f;
-                qv[k][e] = need ? q[c + offs[e]] : 0.f;
+                const float m = need ? 1.f : 0.f;
+                rv[k][e] = rl[e] * m;
+                qv[k][e] = ql[e] * m;
             }
-            cl[k][0] = (in && (cc[k] & 0x81) == 0x81) ? code[c - J] : 0;
-            cl[k][1] = (in && (cc[k] & 0x84) == 0x84) ? code[c - 1] : 0;
-            pv[k] = in ? pnew[c] : 0.f;
-            dv[k] = in ? d[c] : 0.f;
+            cl[k][0] = ((cc[k] & 0x81) == 0x81) ? code_im : 0;
+            cl[k][1] = ((cc[k] & 0x84) == 0x84) ? code_jm : 0;
+            pv[k] = pl;
+            dv[k] = dl;
         }
         if (base == ch.begin) {
             double t[1];
             sum_partials<1>(s, kSlotPQ, nb, t);
             const double pq = t[0];
-            const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+            const bool lead = sb.block == 0 && threadIdx.x == 0;
             if (!(pq > 0.0) || !finite_d(pq)) {   // breakdown: end this sweep
                 if (lead) sc->inner_done = 1;
                 return;
@@ -358,7 +386,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, int it
         }
     }
     double v[2] = {rz, rr};
-    put_partials<2>(s, kSlotCG, v);
+    put_partials<2>(s, kSlotCG, v, sb.block);
 }
 
 // ||b||^2 partials for an externally supplied right-hand side; x = 0, d = 0.
@@ -367,7 +395,7 @@ __global__ void __launch_bounds__(kBlock) k_solve_begin(const SceneDev* scenes)
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
     if ((int)blockIdx.x >= s.pcg_blocks) return;
-    const Chunk ch = my_chunk(s);
+    const Chunk ch = my_chunk(s, blockIdx.x);
     double bb = 0.0;
     for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
         const double b = s.b[c];
@@ -376,7 +404,7 @@ __global__ void __launch_bounds__(kBlock) k_solve_begin(const SceneDev* scenes)
         s.d[c] = 0.f;
     }
     double v[1] = {bb};
-    put_partials<1>(s, kSlotBB, v);
+    put_partials<1>(s, kSlotBB, v, blockIdx.x);
 }
 
 // reset the per-solve scalars (one thread per scene)
@@ -403,8 +431,14 @@ namespace fg {
 FG_LAUNCH0(refine_update)
 FG_LAUNCH0(refine_residual)
 FG_LAUNCH1(cg_init, int, sweep)
-FG_LAUNCH1(pcg_a, int, it)
-FG_LAUNCH1(pcg_b, int, it)
+void launch_pcg_a(dim3 g, hipStream_t st, const SceneDev* sc, const int2* map, int it)
+{
+    fg_launch(k_pcg_a, g, dim3(kBlock), st, sc, map, it);
+}
+void launch_pcg_b(dim3 g, hipStream_t st, const SceneDev* sc, const int2* map, int it)
+{
+    fg_launch(k_pcg_b, g, dim3(kBlock), st, sc, map, it);
+}
 FG_LAUNCH0(solve_begin)
 void launch_solve_reset(dim3 g, hipStream_t st, const SceneDev* sc) { hipLaunchKernelGGL(k_solve_reset, g, dim3(64), 0, st, sc); }
 }  // namespace fg
