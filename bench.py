@@ -15,7 +15,10 @@ Also reported on the same line: the plain single-scene 1024^2 dam break
 (8 particles per cell), the isolated 4096^2 pressure projection, a served
 stream (Poisson arrivals, HBM-budget
 admission, continuous batching), `roofline` for the dominant kernel measured
-with HIP events on its own stream, and `cpu_baseline`: the CPU restatement
+with HIP events attached to its launches on the stream they run on (over as
+many frames of the same workload right after the timed ones: events on every
+launch slow the launch pipeline by ~7 %, so the timed frames run without
+them), and `cpu_baseline`: the CPU restatement
 of the reference path (its own PCG, IPP and gated density solve, the
 reference's thread-pool decomposition) on a bounded sample of the workload.
 """
@@ -34,19 +37,25 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-substeps/s + p99 frame latency, batched vs plain GPU @1e-6 resid"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per cell of the PCG kernels (multigrid preconditioner):
+# Algorithmic bytes per work unit of the kernels bench.py can report a
+# roofline for.  k_mg_resid, the largest share of GPU time in the batched
+# workload, timed at its level-1 launch (one per V-cycle: timing every level's
+# launch perturbs the launch pipeline by ~14 %); the library counts the bytes
+# itself (unit = 1 byte): per level-1 cell r, |dinv| and y (12 B), plus the
+# 25 stored coefficients (100 B) of every active row that is not the level's
+# canonical interior stencil.
 # k_pcg_a reads the stencil code (1), z (4) and the old direction (4) and
-# writes p (4) and q = A' p (4): 17.  k_pcg_b (the axpy pair r -= alpha q,
-# d += alpha p: 24) runs only for groups with IPP scenes; with the V-cycle
-# its work is done inside the V-cycle's first launch.
+# writes p (4) and q = A' p (4): 17 B per cell; k_pcg_b (with the V-cycle,
+# the axpy pair r -= alpha q, d += alpha p) 24 B per cell.
 KERNEL_BYTES = {
+    "mg_resid": ("K_MG_RESID", 1.0, "bytes"),
     "pcg_a": ("K_PCG_A", 17.0, "cells"),
     "pcg_b": ("K_PCG_B", 24.0, "cells"),
 }
-# HBM bytes per cell actually moved, per workload: rocprofv3 FETCH_SIZE
+# HBM bytes actually moved per work unit, per workload: rocprofv3 FETCH_SIZE
 # (x2.000) + WRITE_SIZE (x1.000) per dispatch, the factors being what the same
 # counters read for a 1 GiB copy kernel.
-MEASURED_TRAFFIC_PER_CELL = {}
+MEASURED_TRAFFIC_PER_UNIT = {"mixed256": {"mg_resid": 1.397}}
 
 
 def parse():
@@ -57,7 +66,7 @@ def parse():
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--master-seed", type=int, default=2024)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--roofline-kernel", default="pcg_a", choices=sorted(KERNEL_BYTES))
+    ap.add_argument("--roofline-kernel", default="mg_resid", choices=sorted(KERNEL_BYTES))
     ap.add_argument("--skip-plain", action="store_true", help="batched leg only (profiling runs)")
     ap.add_argument("--tune-rounds", type=int, default=1, help="autotuner rounds (each steps 4 frames)")
     ap.add_argument("--dam-frames", type=int, default=10, help="timed frames of the 1024^2 leg (0: skip)")
@@ -90,6 +99,7 @@ class KernelTimer:
         self.ctx, self.fg = ctx, fg
         self.kid = getattr(fg, KERNEL_BYTES[name][0])
         self.per_unit = KERNEL_BYTES[name][1]
+        self.unit_of_work = KERNEL_BYTES[name][2]
         self.ms = 0.0
         self.busy = 0.0
         self.launches = 0
@@ -107,7 +117,7 @@ class KernelTimer:
         self.launches += n
         self.units += u
 
-    def roofline(self, traffic_per_cell=None, copy_gbs=None):
+    def roofline(self, traffic_per_unit=None, copy_gbs=None):
         achieved = self.units * self.per_unit / (self.ms * 1e-3) / 1e9 if self.ms > 0 else 0.0
         per_launch = self.units / max(self.launches, 1)
         return {
@@ -116,12 +126,12 @@ class KernelTimer:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_per_cell * per_launch if traffic_per_cell else None,
+            "traffic": traffic_per_unit * per_launch if traffic_per_unit else None,
             "avg_launch_us": 1e3 * self.ms / max(self.launches, 1),
             "launches": self.launches,
             "algorithmic_bytes_per_launch": per_launch * self.per_unit,
             "bytes_per_unit": self.per_unit,
-            "unit_of_work": "cells",
+            "unit_of_work": self.unit_of_work,
             "measured_copy_gbs": copy_gbs,
             # launches of several lanes overlap: algorithmic bytes over the
             # time at least one launch of the kernel was running
@@ -156,10 +166,9 @@ def batched_leg(fg, ctx, a):
     lat_b, lat_p, frame_b, frame_p = [], [], [], []
     conv, resid, iters, subs = True, 0.0, 0, 0
     for k in range(a.steps):
-        with kt:
-            t0 = time.perf_counter()
-            st = fg.batch_step_frame(ctx, batch)
-            dt = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        st = fg.batch_step_frame(ctx, batch)
+        dt = time.perf_counter() - t0
         el_b += dt
         frame_b.append(dt * 1e3)
         for s in st:
@@ -179,6 +188,13 @@ def batched_leg(fg, ctx, a):
             el_p += dt
             frame_p.append(dt * 1e3)
         log(f"frame {k}: batched {frame_b[-1]:.1f} ms" + (f", plain {frame_p[-1]:.1f} ms" if plain else ""))
+    # the roofline kernel: HIP events attached to each of its launches on the
+    # stream it runs on, over as many further frames of the same batch (events
+    # on every launch slow the launch pipeline by ~7 %, so the timed frames
+    # above run without them)
+    for k in range(a.steps):
+        with kt:
+            fg.batch_step_frame(ctx, batch)
     for sc in batch + plain:
         sc.close()
     out = {
@@ -216,16 +232,19 @@ def dam_leg(fg, ctx, a, copy_gbs):
         scene.step_frame()
     kt = KernelTimer(ctx, fg, a.roofline_kernel)
     lat, psub, subs, resid, conv = [], 0, 0, 0.0, True
+    t0 = time.perf_counter()
+    for _ in range(a.dam_frames):
+        s = scene.step_frame()
+        lat.append(s.frame_ms)
+        psub += s.particle_substeps
+        subs += s.substeps
+        resid = max(resid, s.pressure_rel_resid)
+        conv = conv and bool(s.converged)
+    el = time.perf_counter() - t0
+    roof_frames = min(a.dam_frames, 3)   # the roofline kernel, on further frames (see batched_leg)
     with kt:
-        t0 = time.perf_counter()
-        for _ in range(a.dam_frames):
-            s = scene.step_frame()
-            lat.append(s.frame_ms)
-            psub += s.particle_substeps
-            subs += s.substeps
-            resid = max(resid, s.pressure_rel_resid)
-            conv = conv and bool(s.converged)
-        el = time.perf_counter() - t0
+        for _ in range(roof_frames):
+            scene.step_frame()
     scene.close()
     log(f"dam break {res}^2: {a.dam_frames} frames, {subs} substeps, {el:.2f} s")
     return {
@@ -238,7 +257,9 @@ def dam_leg(fg, ctx, a, copy_gbs):
         "p99_frame_ms": pctl(lat, 0.99),
         "max_rel_residual": resid,
         "all_converged": conv,
-        "roofline": kt.roofline(MEASURED_TRAFFIC_PER_CELL.get("dam1024", {}).get(a.roofline_kernel), copy_gbs),
+        "roofline": dict(kt.roofline(MEASURED_TRAFFIC_PER_UNIT.get("dam1024", {}).get(a.roofline_kernel), copy_gbs),
+                         kernel=f"k_{a.roofline_kernel}",
+                         measured_on=f"{roof_frames} frames right after the timed ones"),
     }
 
 
@@ -247,8 +268,9 @@ def projection_leg(fg, ctx, a, copy_gbs):
     mat, _, _, b, scale = fg.isolated_projection_system(n, 1)
     kt = KernelTimer(ctx, fg, a.roofline_kernel)
     ctx.poisson_solve(mat, scale, b, tol=a.tol)          # warm-up (allocation, code load)
-    with kt:
-        x, st = ctx.poisson_solve(mat, scale, b, tol=a.tol)
+    x, st = ctx.poisson_solve(mat, scale, b, tol=a.tol)
+    with kt:                                              # the roofline kernel, on a second solve
+        ctx.poisson_solve(mat, scale, b, tol=a.tol)
     liquid = int(((mat & fg.FLUID) != 0).sum())
     log(f"isolated projection {n}^2: {st.iterations} iterations, {st.solve_ms:.1f} ms, resid {st.rel_residual:.2e}")
     return {
@@ -259,7 +281,8 @@ def projection_leg(fg, ctx, a, copy_gbs):
         "rel_residual": st.rel_residual,
         "converged": bool(st.converged),
         "solve_ms_incl_rhs_upload": st.solve_ms,
-        "roofline": kt.roofline(None, copy_gbs),
+        "roofline": dict(kt.roofline(None, copy_gbs), kernel=f"k_{a.roofline_kernel}",
+                         measured_on="a second solve of the same system"),
     }
 
 
@@ -392,8 +415,9 @@ def main():
         "pcg_iters_per_substep_max": b["pcg_iters_per_substep_max"],
         "plain": b.get("plain"),
         "batched_over_plain": b.get("batched_over_plain"),
-        "roofline": dict(kt.roofline(MEASURED_TRAFFIC_PER_CELL.get("mixed256", {}).get(a.roofline_kernel),
-                                     copy_gbs), kernel=f"k_{a.roofline_kernel}"),
+        "roofline": dict(kt.roofline(MEASURED_TRAFFIC_PER_UNIT.get("mixed256", {}).get(a.roofline_kernel),
+                                     copy_gbs), kernel=f"k_{a.roofline_kernel}",
+                         measured_on=f"{a.steps} frames of the same batch right after the timed ones"),
         "cpu_baseline": cpu,
         "dam_break_1024": dam,
         "isolated_projection": proj,

@@ -951,7 +951,12 @@ void launch_vcycle(const Group& G, hipStream_t st, const SolveLaunch& S, int whi
     const int n = G.mg_levels;
     if (S.blocks[1]) launch_mg_down0(S.grid(1), st, S.d, S.map[1], which);
     for (int l = 1; l <= n - 2; l++) {
-        if (S.blocks[1 + l]) launch_mg_resid(S.grid(1 + l), st, S.d, S.map[1 + l], l);
+        if (S.blocks[1 + l]) {
+            // the profiling family times the largest residual launch (level 1)
+            KProf kp(G, l == 1 ? FG_K_MG_RESID : -2, true);
+            launch_mg_resid(S.grid(1 + l), st, S.d, S.map[1 + l], l);
+            kp.done(0);
+        }
         if (S.blocks[2 + l]) launch_mg_restrict(S.grid(2 + l), st, S.d, S.map[2 + l], l + 1);
     }
     launch_mg_bottom(dim3(1, S.ny), st, S.d);
@@ -1042,6 +1047,13 @@ int run_solve(Group& G, bool rhs_from_grid)
     if (ctx->prof_kernel == FG_K_VCYCLE)   // one V-cycle per iteration and per sweep start
         for (fg_scene* s : G.sc)
             if (s->h.precond == kPrecondMG) G.L->prof_units += (int64_t)(s->hsc->total_iters + s->hsc->sweeps) * s->C;
+    if (ctx->prof_kernel == FG_K_MG_RESID)   // level-1 bytes: r, dinv, y per cell + stored rows' coefficients
+        for (fg_scene* s : G.sc) {
+            if (s->h.precond != kPrecondMG || s->mgh.n < 3) continue;
+            const MgLevel& L1 = s->mgh.lv[1];
+            const int64_t per_cycle = 12 * (int64_t)L1.I * L1.J + 4 * kMgStencil * (int64_t)s->hsc->mg_stored[1];
+            G.L->prof_units += (int64_t)(s->hsc->total_iters + s->hsc->sweeps) * per_cycle;
+        }
     for (fg_scene* s : G.sc)
         if (s->hsc->numeric_bad) return fail(FG_E_NUMERIC, "NaN/Inf in the pressure solve");
     return FG_OK;

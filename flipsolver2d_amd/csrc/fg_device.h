@@ -62,6 +62,7 @@ struct SceneScalars {
     int32_t numeric_bad;    // NaN/Inf seen in a reduction
     int32_t ls_more;        // level-set BFS reached the end of a launch's band
     int32_t density_applied;  // density correction moved the particles this substep
+    int32_t mg_stored[kMgMaxLevels];  // active rows per level that read stored coefficients
 };
 
 // One level of the multigrid hierarchy (levels >= 1; level 0 is the

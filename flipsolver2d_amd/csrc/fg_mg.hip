@@ -83,27 +83,24 @@ __device__ __forceinline__ float prolong_at(gptr<const float> y, int I, int J, i
     return acc;
 }
 
-// sum_k a_k v_k over a level-l row with the level's 25 stencil entries: the
-// canonical stencil when every lane of the wave has a canonical row (the
-// branch is wave-uniform, so their coefficient loads are skipped), else the
-// stored coefficients (all 25 loads issued together).
+// sum_k a_k v_k over an active level-l row (dinv != 0) with the level's 25
+// stencil entries: the canonical stencil on canonical rows (dinv < 0), else
+// the stored coefficients, loaded (all 25 together, one wait) only by the
+// lanes whose rows need them.
 __device__ __forceinline__ float row_dot(const MgLevels& M, int l, gptr<const float> coef, int CL, int c,
-                                         bool canon_row, const float (&v)[kMgStencil])
+                                         float dinv_c, const float (&v)[kMgStencil])
 {
     const float* cn = M.canon[l];
-    float ax = 0.f;
-    if (__all(canon_row)) {
+    float a[kMgStencil];
 #pragma unroll
-        for (int k = 0; k < kMgStencil; k++) ax += cn[k] * v[k];
-    } else {
-        float a[kMgStencil];
+    for (int k = 0; k < kMgStencil; k++) a[k] = cn[k];
+    if (dinv_c > 0.f) {
 #pragma unroll
         for (int k = 0; k < kMgStencil; k++) a[k] = coef[(size_t)k * CL + c];
-        // 0/1 factors rather than a select, so the loads stay unconditional
-        const float mc = canon_row ? 1.f : 0.f, ms = 1.f - mc;
-#pragma unroll
-        for (int k = 0; k < kMgStencil; k++) ax += (cn[k] * mc + a[k] * ms) * v[k];
     }
+    float ax = 0.f;
+#pragma unroll
+    for (int k = 0; k < kMgStencil; k++) ax += a[k] * v[k];
     return ax;
 }
 
@@ -257,8 +254,10 @@ __global__ void __launch_bounds__(kBlock) k_mg_sym(const SceneDev* scenes, int l
     if (l >= M.n) return;
     const MgLevel& L = M.lv[l];
     const int LI = L.I, LJ = L.J, CL = LI * LJ;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= CL) return;
+    if ((int)blockIdx.x * kBlock >= CL) return;   // the whole block is past the level
+    const int c0 = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = c0 < CL;
+    const int c = valid ? c0 : CL - 1;   // past the level: recompute the last row, store nothing
     const int ci = c / LJ, cj = c - ci * LJ;
     gptr<float> coef = gbl(L.coef);
     float v[kMgStencil];
@@ -279,12 +278,16 @@ __global__ void __launch_bounds__(kBlock) k_mg_sym(const SceneDev* scenes, int l
     bool canon_row = true;
 #pragma unroll
     for (int k = 0; k < kMgStencil; k++) {
-        if (k < 12) coef[(size_t)k * CL + c] = v[k];
+        if (k < 12 && valid) coef[(size_t)k * CL + c] = v[k];
         sum += fabsf(v[k]);
         canon_row = canon_row && fabsf(v[k] - cn[k]) <= tol;
     }
     const float di = (L.act[c] && sum > 0.f) ? 1.0f / sum : 0.f;
-    gbl(L.dinv)[c] = (canon_row && di > 0.f) ? -di : di;
+    if (valid) gbl(L.dinv)[c] = (canon_row && di > 0.f) ? -di : di;
+    // active rows that will read their stored coefficients (for the kernels'
+    // byte counts): one atomic per block, once per solve
+    const int stored = __syncthreads_count(valid && di > 0.f && !canon_row);
+    if (threadIdx.x == 0 && stored) atomicAdd(&s.sc->mg_stored[l], stored);
 }
 
 // ---------------------------------------------------------------------------
@@ -310,8 +313,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_down0(const SceneDev* scenes, con
     const int I = s.I, J = s.J;
     gptr<const float> r = gbl<const float>(which ? s.cr2 : s.cr);
     gptr<const uint8_t> code = gbl<const uint8_t>(s.stencil);
+    if (!gbl<const uint8_t>(L1.act)[c]) {   // inactive coarse cell: nothing to restrict
+        gbl(L1.r)[c] = 0.f;
+        return;
+    }
     const int ci = c / J1, cj = c - ci * J1;
-    const bool act = gbl<const uint8_t>(L1.act)[c];
     uint8_t cd[6][6];
     float rv[6][6];
 #pragma unroll
@@ -347,7 +353,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_down0(const SceneDev* scenes, con
             ax -= (k & 8) ? x0[a][b + 1] : 0.f;
             acc += (k & 0x80) ? (w1(a - 1) * w1(b - 1)) * (rv[a][b] - ax) : 0.f;
         }
-    gbl(L1.r)[c] = act ? 0.25f * acc : 0.f;
+    gbl(L1.r)[c] = 0.25f * acc;
 }
 
 // Level l (1 <= l <= n-2): x = r * dinv, y = res = r - A_l x.
@@ -363,6 +369,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_resid(const SceneDev* scenes, con
     const int c = sb.block * kBlock + threadIdx.x;
     if (c >= CL) return;
     gptr<const float> r = gbl<const float>(L.r), dinv = gbl<const float>(L.dinv);
+    const float dc = dinv[c];
+    if (dc == 0.f) {   // inactive row: r = 0 and no coefficients
+        gbl(L.y)[c] = 0.f;
+        return;
+    }
     const int ci = c / LJ, cj = c - ci * LJ;
     float xv[kMgStencil];
 #pragma unroll
@@ -374,9 +385,8 @@ __global__ void __launch_bounds__(kBlock) k_mg_resid(const SceneDev* scenes, con
         xv[k] = (rn * fabsf(dn)) * (ok ? 1.f : 0.f);   // a 0/1 factor, not a select: the load stays unconditional
     }
     const float rc = r[c];
-    const bool canon_row = dinv[c] < 0.f;
-    const float ax = row_dot(M, l, gbl<const float>(L.coef), CL, c, canon_row, xv);
-    gbl(L.y)[c] = rc - ax;   // inactive rows: r = 0 and no coefficients
+    const float ax = row_dot(M, l, gbl<const float>(L.coef), CL, c, dc, xv);
+    gbl(L.y)[c] = rc - ax;
 }
 
 // Level l-1 -> l (2 <= l <= n-1): r_l = R y_{l-1}.
@@ -393,8 +403,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(const SceneDev* scenes, 
     const int c = sb.block * kBlock + threadIdx.x;
     if (c >= L.I * LJ) return;
     gptr<const float> fy = gbl<const float>(F.y);
+    if (!gbl<const uint8_t>(L.act)[c]) {
+        gbl(L.r)[c] = 0.f;
+        return;
+    }
     const int ci = c / LJ, cj = c - ci * LJ;
-    const bool act = gbl<const uint8_t>(L.act)[c];
     float v[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -412,7 +425,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(const SceneDev* scenes, 
             const bool ok = fi >= 0 && fi < FI && fj >= 0 && fj < FJ;
             acc += ((w1(a) * w1(b)) * v[a][b]) * (ok ? 1.f : 0.f);   // inactive fine cells hold y = 0
         }
-    gbl(L.r)[c] = act ? 0.25f * acc : 0.f;
+    gbl(L.r)[c] = 0.25f * acc;
 }
 
 // Bottom level: kMgBottomSweeps L1-Jacobi sweeps from zero, one block per
@@ -466,11 +479,14 @@ __global__ void __launch_bounds__(kBlock) k_mg_prolong(const SceneDev* scenes, c
     const int LJ = L.J;
     const int c = sb.block * kBlock + threadIdx.x;
     if (c >= L.I * LJ) return;
+    if (!gbl<const uint8_t>(L.act)[c]) {
+        gbl(L.x)[c] = 0.f;
+        return;
+    }
     const int ci = c / LJ, cj = c - ci * LJ;
-    const bool act = gbl<const uint8_t>(L.act)[c];
     const float rc = gbl<const float>(L.r)[c], dc = gbl<const float>(L.dinv)[c];
     const float pc = prolong_at(gbl<const float>(Cl.y), Cl.I, Cl.J, ci, cj);
-    gbl(L.x)[c] = act ? rc * fabsf(dc) + pc : 0.f;
+    gbl(L.x)[c] = rc * fabsf(dc) + pc;
 }
 
 // Level l (1 <= l <= n-2), up: y = x + dinv (r - A_l x).
@@ -486,6 +502,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_post(const SceneDev* scenes, cons
     const int c = sb.block * kBlock + threadIdx.x;
     if (c >= CL) return;
     gptr<const float> x = gbl<const float>(L.x);
+    const float dc = gbl<const float>(L.dinv)[c];
+    if (dc == 0.f) {   // inactive row: x = 0
+        gbl(L.y)[c] = 0.f;
+        return;
+    }
     const int ci = c / LJ, cj = c - ci * LJ;
     float xv[kMgStencil];
 #pragma unroll
@@ -495,9 +516,9 @@ __global__ void __launch_bounds__(kBlock) k_mg_post(const SceneDev* scenes, cons
         const float v = x[min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1)];
         xv[k] = v * (ok ? 1.f : 0.f);
     }
-    const float dc = gbl<const float>(L.dinv)[c], rc = gbl<const float>(L.r)[c];
-    const float ax = row_dot(M, l, gbl<const float>(L.coef), CL, c, dc < 0.f, xv);
-    gbl(L.y)[c] = xv[12] + fabsf(dc) * (rc - ax);   // inactive rows: x = 0, dinv = 0
+    const float rc = gbl<const float>(L.r)[c];
+    const float ax = row_dot(M, l, gbl<const float>(L.coef), CL, c, dc, xv);
+    gbl(L.y)[c] = xv[12] + fabsf(dc) * (rc - ax);
 }
 
 // Level 0, up: z0 = r / D1 + P y1 at the cell and its fluid neighbours, then
@@ -519,9 +540,15 @@ __global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, const
     const Chunk ch = my_chunk(s, sb.block);
     double rz = 0.0, rr = 0.0;
     for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
-        const int ci = c / J, cj = c - ci * J;
         const uint8_t k = code[c];
         const float rc = r[c];
+        if (!(k & 0x80)) {   // identity row: z = r, no neighbourhood
+            z[c] = rc;
+            rz += (double)rc * (double)rc;
+            rr += (double)rc * (double)rc;
+            continue;
+        }
+        const int ci = c / J, cj = c - ci * J;
         const int di[5] = {0, -1, 1, 0, 0}, dj[5] = {0, 0, 0, -1, 1};
         // the fluid neighbours that enter A' (not the row-wrapping coupling)
         bool use[5];

@@ -419,6 +419,7 @@ __global__ void k_solve_reset(const SceneDev* scenes)
     sc->total_iters = 0;
     sc->sweeps = 0;
     sc->numeric_bad = 0;
+    for (int l = 0; l < kMgMaxLevels; l++) sc->mg_stored[l] = 0;
 }
 
 }  // namespace fg
