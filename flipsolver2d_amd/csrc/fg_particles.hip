@@ -159,6 +159,7 @@ __device__ float2 closest_surface_point(const float* sdf, int I, int J, float2 p
     float2 grad = sdf_grad_cached(B, sdf, I, J, pos.x, pos.y);
     for (int it = 0; it < 100; it++) {
         float alpha = 1;
+        bool moved = false;
         for (int k = 0; k < 10; k++) {
             const float t = alpha * value;
             const float2 q = make_float2(cp.x - grad.x * t, cp.y - grad.y * t);
@@ -167,11 +168,16 @@ __device__ float2 closest_surface_point(const float* sdf, int I, int J, float2 p
                 cp = q;
                 value = vq;
                 grad = sdf_grad_cached(B, sdf, I, J, q.x, q.y);
+                moved = true;
                 if (fabsf(value) < 1e-5f) return cp;
             } else {
                 alpha *= 0.7f;
             }
         }
+        // an outer iteration that accepted no step leaves (cp, value, grad)
+        // as they were and alpha restarts at 1, so every later one repeats
+        // it exactly: the reference's remaining iterations change nothing
+        if (!moved) break;
     }
     return cp;
 }
