@@ -363,6 +363,7 @@ int create_scene(fg_ctx* ctx, const fg_scene_params* p, bool solver_only, fg_sce
     h.J = s->J;
     h.ppc = p->particles_per_cell;
     h.n_friction = 0;
+    h.friction_any = 0;
     h.pic = p->pic_ratio;
     h.pscale = p->particle_scale;
     h.g_i = p->gravity_i;
@@ -1626,6 +1627,9 @@ int fg_scene_set_friction(fg_scene* s, int32_t n, const float* friction)
     if (n > 0)
         HIPCHK(hipMemcpyAsync(s->h.friction, friction, sizeof(float) * n, hipMemcpyHostToDevice, s->ctx->stream));
     s->h.n_friction = n;
+    s->h.friction_any = 0;
+    for (int k = 0; k < n; k++)
+        if (friction[k] != 0.f) s->h.friction_any = 1;
     FGCHK(upload_desc(s));
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
     return FG_OK;

@@ -92,6 +92,7 @@ struct SceneDev {
     int32_t I, J;           // cells
     int32_t ppc;
     int32_t n_friction;
+    int32_t friction_any;   // some friction value is nonzero (else the friction factor is exactly 1)
     float pic, pscale;
     float g_i, g_j;         // gravity (m/s^2)
     double dx, rho;

@@ -732,20 +732,23 @@ __global__ void __launch_bounds__(kBlock) k_apply_solids(const SceneDev* scenes)
             s.vvalS[fv] = 0;
         }
     }
-    // friction of the solids in the 3x3 neighbourhood
+    // friction of the solids in the 3x3 neighbourhood; with no nonzero
+    // friction the factor (1 - avg) is exactly 1 and the loads are skipped
     int n = 0;
     float avg = 0;
-    for (int di = -1; di < 2; di++)
-        for (int dj = -1; dj < 2; dj++) {
-            const int ii = i + di, jj = j + dj;
-            if (ii >= 0 && ii < I && jj >= 0 && jj < J) {
-                const int id = s.sid[(size_t)ii * J + jj];
-                if (id != -1) {
-                    avg += (id < s.n_friction) ? s.friction[id] : 0.f;
-                    n++;
+    if (s.friction_any) {
+        for (int di = -1; di < 2; di++)
+            for (int dj = -1; dj < 2; dj++) {
+                const int ii = i + di, jj = j + dj;
+                if (ii >= 0 && ii < I && jj >= 0 && jj < J) {
+                    const int id = s.sid[(size_t)ii * J + jj];
+                    if (id != -1) {
+                        avg += (id < s.n_friction) ? s.friction[id] : 0.f;
+                        n++;
+                    }
                 }
             }
-        }
+    }
     if (n > 0) {
         avg /= (float)n;
         auto sol = [&](int a, int b) { return is_solid(mat_ext(s.mat, I, J, a, b)); };

@@ -145,3 +145,30 @@ def test_frames_run(fg, orc, ctx):
     assert subs >= 3
     assert np.all(np.isfinite(g["vx"])) and np.all(np.isfinite(g["u"]))
     assert len(g["px"]) > 17000
+
+
+def test_substep_parity_with_friction(fg, orc, ctx):
+    """Nonzero wall friction (updateVelocityFromSolids, flipsolver2d.cpp:1077-1104):
+    the GPU's friction branch against the oracle's, after frames that ran
+    with it."""
+    n, ppc = 64, 4
+    fr = np.array([0.3, 0.1, 0.5, 0.2], np.float32)
+    p, st = orc.dam_break(n, ppc=ppc, tol=1e-12, friction=tuple(fr))
+    run_oracle_frames(orc, p, st, 4)
+    vel = orc.lib().orc_max_particle_velocity(st.ref())
+    dt = np.float32(min(5.0 / (vel + 1e-15), 1.0 / 30))
+    sc = fg.Scene(ctx, fg.make_params(n, ppc=ppc, tol=1e-10))
+    sc.set_friction(fr)
+    sc.set_state(oracle_state_dict(st))
+    gs = sc.substep(float(dt))
+    g = sc.get_state()
+    sc.close()
+    ostats = orc.OrcStats()
+    orc.lib().orc_substep(C.byref(p), st.ref(), C.c_float(dt), C.byref(ostats))
+    N = st.n
+    _log(f"friction substep: {gs.pressure_iters} iters")
+    assert len(g["px"]) == N
+    assert np.array_equal(g["px"].view(np.uint32), st.px[:N].view(np.uint32))
+    assert rel_l2(g["pressure"], st.pressure) <= 1e-5
+    assert max(rel_l2(g["u"], st.u), rel_l2(g["v"], st.v)) <= 1e-5
+    assert max(rel_l2(g["vx"], st.vx[:N]), rel_l2(g["vy"], st.vy[:N])) <= 1e-5
