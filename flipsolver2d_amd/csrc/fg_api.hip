@@ -223,14 +223,38 @@ void carve(fg_scene* s, Carver& cv)
         }
         s->mg_ok = std::max(M.lv[n - 1].I, M.lv[n - 1].J) <= kMgBottomMax;
         M.n = n;
+        // the levels k_mg_coarse runs in one block: from the first level
+        // that fits, with its coarser levels, in LDS (the bottom always
+        // does: at most kMgBottomMax^2 cells)
+        auto cells = [&](int l) { return (int64_t)M.lv[l].I * M.lv[l].J; };
+        M.t = std::max(1, n - 1);
+        for (int t = 1; t < n - 1; t++) {
+            int64_t sum = 0;
+            for (int l = t; l < n; l++) sum += cells(l);
+            if (cells(t) <= kMgCoarseTop && sum <= kMgCoarseCells) {
+                M.t = t;
+                break;
+            }
+        }
+        int o = 0;
+        for (int l = M.t; l < n; l++) {
+            M.roff[l] = o;
+            o += (int)cells(l);
+        }
+        M.tcells = o;
         std::memcpy(M.canon, mg_canonical(), sizeof(M.canon));
         h.mg = cv.take<MgLevels>(1);
+        int64_t all = 0;
+        for (int l = 1; l < n; l++) all += cells(l);
+        float* const dinv = cv.take<float>(all);   // every level's dinv, contiguous in level order
+        int64_t doff = 0;
         for (int l = 1; l < n; l++) {
             MgLevel& L = M.lv[l];
             const int64_t CL = (int64_t)L.I * L.J;
             L.act = cv.take<uint8_t>(CL);
             L.coef = cv.take<float>(kMgStencil * CL);
-            L.dinv = cv.take<float>(CL);
+            L.dinv = dinv ? dinv + doff : nullptr;
+            doff += CL;
             L.r = cv.take<float>(CL);
             L.x = cv.take<float>(CL);
             L.y = cv.take<float>(CL);
@@ -543,7 +567,7 @@ __device__ int map_blocks(const SceneDev& s, int kind)
     const MgLevels& M = *s.mg;
     if (kind == 1) return (M.lv[1].I * M.lv[1].J + kBlock - 1) / kBlock;
     const int l = kind - 1;
-    if (l >= M.n) return 0;
+    if (l >= M.t) return 0;   // level l runs inside k_mg_coarse
     return (M.lv[l].I * M.lv[l].J + kBlock - 1) / kBlock;
 }
 
@@ -620,7 +644,9 @@ struct Group {
     int mg_levels = 0;                  // most multigrid levels of a scene using the V-cycle
     int map_kinds = 0;                  // block-map kinds the solve uses (0 when maps are off)
     int64_t map_off[kMapKinds]{};       // offset of each kind's map in the lane's map buffer
+    int mg_top = 0;                     // largest first level of k_mg_coarse (levels below: one launch each)
     unsigned mg_grid[kMgMaxLevels]{};   // blocks per scene over level l, one cell per thread
+    unsigned vc_grid[kMgMaxLevels]{};   // the same over the scenes whose V-cycle launches level l (l < t)
     unsigned mg_grid_d0 = 0;            // blocks per scene of k_mg_down0 (one level-1 cell per thread)
 };
 
@@ -730,9 +756,12 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
         s->h.precond = (ctx->precond == kPrecondMG && s->mg_ok) ? kPrecondMG : kPrecondIPP;
         if (s->h.precond == kPrecondMG) {
             G.mg_levels = std::max(G.mg_levels, s->mgh.n);
-            for (int l = 1; l < s->mgh.n; l++)
-                G.mg_grid[l] = std::max(G.mg_grid[l],
-                                        (unsigned)cdiv((int64_t)s->mgh.lv[l].I * s->mgh.lv[l].J, kBlock));
+            G.mg_top = std::max(G.mg_top, s->mgh.t);
+            for (int l = 1; l < s->mgh.n; l++) {
+                const unsigned b = (unsigned)cdiv((int64_t)s->mgh.lv[l].I * s->mgh.lv[l].J, kBlock);
+                G.mg_grid[l] = std::max(G.mg_grid[l], b);
+                if (l < s->mgh.t) G.vc_grid[l] = std::max(G.vc_grid[l], b);
+            }
             G.mg_grid_d0 = std::max(G.mg_grid_d0, (unsigned)cdiv((int64_t)s->mgh.lv[1].I * s->mgh.lv[1].J, kBlock));
         }
         s->h.pblocks = (int)std::max<int64_t>(1, cdiv(std::max<int64_t>(s->n_bound, 1), kBlock));
@@ -750,14 +779,14 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
     // block maps for the solve's per-iteration launches (several scenes):
     // capacity = every scene's blocks, the bound for any packed subset
     if (G.n > 1 && G.n <= kMapMaxScenes) {
-        G.map_kinds = G.mg_levels + 1;
+        G.map_kinds = G.mg_top + 1;   // pcg, down0, and levels 1 .. t-1
         int64_t cap[kMapKinds] = {};
         for (fg_scene* s : scenes) {
             cap[0] += s->pcg_blocks;
             if (s->h.precond != kPrecondMG) continue;
             const MgLevels& M = s->mgh;
             cap[1] += cdiv((int64_t)M.lv[1].I * M.lv[1].J, kBlock);
-            for (int l = 1; l < M.n; l++) cap[1 + l] += cdiv((int64_t)M.lv[l].I * M.lv[l].J, kBlock);
+            for (int l = 1; l < M.t; l++) cap[1 + l] += cdiv((int64_t)M.lv[l].I * M.lv[l].J, kBlock);
         }
         int64_t total = 0;
         for (int k = 0; k < G.map_kinds; k++) {
@@ -835,7 +864,7 @@ SolveLaunch group_launch(const Group& G, const SceneDev* d, unsigned ny)
     S.ny = ny;
     S.blocks[0] = G.g.pcg.x;
     S.blocks[1] = G.mg_grid_d0;
-    for (int l = 1; l < kMgMaxLevels; l++) S.blocks[1 + l] = G.mg_grid[l];
+    for (int l = 1; l < kMgMaxLevels; l++) S.blocks[1 + l] = G.vc_grid[l];
     return S;
 }
 
@@ -949,19 +978,22 @@ struct KProf {
 // have level l; a kind with no blocks is not launched.
 void launch_vcycle(const Group& G, hipStream_t st, const SolveLaunch& S, int which)
 {
-    const int n = G.mg_levels;
+    // levels 1 .. t-1 one launch per step, levels t .. n-1 in k_mg_coarse
+    // (t per scene; the maps and vc_grid hold a level's blocks only for the
+    // scenes with l < t)
+    const int top = G.mg_top;
     if (S.blocks[1]) launch_mg_down0(S.grid(1), st, S.d, S.map[1], which);
-    for (int l = 1; l <= n - 2; l++) {
+    for (int l = 1; l < top; l++) {
         if (S.blocks[1 + l]) {
             // the profiling family times the largest residual launch (level 1)
             KProf kp(G, l == 1 ? FG_K_MG_RESID : -2, true);
             launch_mg_resid(S.grid(1 + l), st, S.d, S.map[1 + l], l);
             kp.done(0);
         }
-        if (S.blocks[2 + l]) launch_mg_restrict(S.grid(2 + l), st, S.d, S.map[2 + l], l + 1);
+        if (l + 1 < top && S.blocks[2 + l]) launch_mg_restrict(S.grid(2 + l), st, S.d, S.map[2 + l], l + 1);
     }
-    launch_mg_bottom(dim3(1, S.ny), st, S.d);
-    for (int l = n - 2; l >= 1; l--) {
+    launch_mg_coarse(dim3(1, S.ny), st, S.d);
+    for (int l = top - 1; l >= 1; l--) {
         if (!S.blocks[1 + l]) continue;
         launch_mg_prolong(S.grid(1 + l), st, S.d, S.map[1 + l], l);
         launch_mg_post(S.grid(1 + l), st, S.d, S.map[1 + l], l);
@@ -1050,7 +1082,7 @@ int run_solve(Group& G, bool rhs_from_grid)
             if (s->h.precond == kPrecondMG) G.L->prof_units += (int64_t)(s->hsc->total_iters + s->hsc->sweeps) * s->C;
     if (ctx->prof_kernel == FG_K_MG_RESID)   // level-1 bytes: r, dinv, y per cell + stored rows' coefficients
         for (fg_scene* s : G.sc) {
-            if (s->h.precond != kPrecondMG || s->mgh.n < 3) continue;
+            if (s->h.precond != kPrecondMG || s->mgh.t < 2) continue;   // no level-1 launch
             const MgLevel& L1 = s->mgh.lv[1];
             const int64_t per_cycle = 12 * (int64_t)L1.I * L1.J + 4 * kMgStencil * (int64_t)s->hsc->mg_stored[1];
             G.L->prof_units += (int64_t)(s->hsc->total_iters + s->hsc->sweeps) * per_cycle;

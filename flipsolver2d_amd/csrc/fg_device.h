@@ -28,6 +28,12 @@ constexpr int kMgMaxLevels = 12;
 constexpr int kMgBottomMax = 16;         // the bottom level has at most this many cells per side
 constexpr int kMgBottomSweeps = 8;       // L1-Jacobi sweeps on the bottom level
 constexpr int kMgStencil = 25;           // 5x5 Galerkin stencil of the coarse levels
+// k_mg_coarse runs levels t .. n-1 of a scene in one block: t is the first
+// level >= 1 with at most kMgCoarseTop cells whose levels t .. n-1 hold at
+// most kMgCoarseCells cells together (their r stays in LDS)
+constexpr int kMgCoarseThreads = 1024;
+constexpr int kMgCoarseTop = 4096;
+constexpr int kMgCoarseCells = 5632;
 constexpr int kFixupLongCap = 512;       // longest cell the wave-cooperative id sort takes
 
 // Material codes and tests, FlipSolver2dLib/materialgrid.h:6-20.
@@ -71,14 +77,22 @@ struct MgLevel {
     int32_t I, J;
     uint8_t* act;     // the cell takes part: one of its 2x2 finer cells does
     float* coef;      // [kMgStencil][I*J]: entry k couples the cell to (i + k/5 - 2, j + k%5 - 2)
-    float* dinv;      // 1 / sum_k |coef_k| on active cells, 0 elsewhere; negative on
-                      // rows equal to the level's canonical interior stencil
+    float* dinv;      // 1 / sum_k |coef_k| on active cells, +0 on inactive ones (-0 on an
+                      // active cell whose row is all zero, so that a nonzero bit
+                      // pattern marks exactly the active cells); negative on rows
+                      // equal to the level's canonical interior stencil
     float* r;         // restricted residual
     float* x;         // pre-smoothed iterate plus the coarse correction
     float* y;         // residual on the way down, smoothed result on the way up
 };
 struct MgLevels {
     int32_t n;        // levels; level n-1 is the bottom
+    int32_t t;        // first level of k_mg_coarse (1 <= t <= n-1)
+    int32_t tcells;   // cells of levels t .. n-1 together
+    // offset of level l >= t among the cells of levels t .. n-1: in
+    // k_mg_coarse's LDS, and of its dinv from lv[t].dinv (the levels' dinv
+    // arrays are contiguous in level order)
+    int32_t roff[kMgMaxLevels];
     MgLevel lv[kMgMaxLevels];
     // the Galerkin row of a cell deep inside the liquid (all of its fine
     // support and coarse neighbourhood active), per level: such rows read

@@ -44,7 +44,7 @@ FG_DECLARE_LAUNCH(mg_sym, int l);
 FG_DECLARE_LAUNCH(mg_down0, const int2* map, int which);
 FG_DECLARE_LAUNCH(mg_resid, const int2* map, int l);
 FG_DECLARE_LAUNCH(mg_restrict, const int2* map, int l);
-FG_DECLARE_LAUNCH(mg_bottom);
+FG_DECLARE_LAUNCH(mg_coarse);
 FG_DECLARE_LAUNCH(mg_prolong, const int2* map, int l);
 FG_DECLARE_LAUNCH(mg_post, const int2* map, int l);
 FG_DECLARE_LAUNCH(mg_up0, const int2* map, int which);

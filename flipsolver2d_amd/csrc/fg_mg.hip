@@ -58,7 +58,8 @@ __device__ __forceinline__ float w1(int a) { return (a == 1 || a == 2) ? 0.75f :
 // the (up to) four nearest coarse cells, weights 9/16, 3/16, 3/16, 1/16.
 // The four loads are unconditional (clamped index, weight 0 outside), so
 // they issue together.  Inactive coarse cells hold y = 0.
-__device__ __forceinline__ float prolong_at(gptr<const float> y, int I, int J, int fi, int fj)
+template <typename P>
+__device__ __forceinline__ float prolong_at(P y, int I, int J, int fi, int fj)
 {
     const int p0 = fi >> 1, q0 = fj >> 1;
     const int p1 = (fi & 1) ? p0 + 1 : p0 - 1, q1 = (fj & 1) ? q0 + 1 : q0 - 1;
@@ -102,6 +103,47 @@ __device__ __forceinline__ float row_dot(const MgLevels& M, int l, gptr<const fl
 #pragma unroll
     for (int k = 0; k < kMgStencil; k++) ax += a[k] * v[k];
     return ax;
+}
+
+// r at coarse cell (ci, cj) from the finer level's y (FI x FJ): 1-D weights
+// 1/4, 3/4, 3/4, 1/4 over fine cells 2c-1 .. 2c+2, times 1/4.  Inactive fine
+// cells hold y = 0; the loads are unconditional (clamped index, weight 0
+// outside the grid).
+template <typename P>
+__device__ __forceinline__ float restrict_at(P fy, int FI, int FJ, int ci, int cj)
+{
+    float v[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int fi = min(max(2 * ci - 1 + a, 0), FI - 1), fj = min(max(2 * cj - 1 + b, 0), FJ - 1);
+            v[a][b] = fy[fi * FJ + fj];
+        }
+    float acc = 0.f;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int fi = 2 * ci - 1 + a, fj = 2 * cj - 1 + b;
+            const bool ok = fi >= 0 && fi < FI && fj >= 0 && fj < FJ;
+            acc += ((w1(a) * w1(b)) * v[a][b]) * (ok ? 1.f : 0.f);
+        }
+    return 0.25f * acc;
+}
+
+// The 5x5 neighbourhood of (ci, cj) in x (LI x LJ), 0 outside the grid
+// (clamped loads times a 0/1 factor, so every load issues).
+template <typename P>
+__device__ __forceinline__ void gather25(P x, int LI, int LJ, int ci, int cj, float (&xv)[kMgStencil])
+{
+#pragma unroll
+    for (int k = 0; k < kMgStencil; k++) {
+        const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
+        const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
+        const float v = x[min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1)];
+        xv[k] = v * (ok ? 1.f : 0.f);
+    }
 }
 
 }  // namespace
@@ -282,8 +324,9 @@ __global__ void __launch_bounds__(kBlock) k_mg_sym(const SceneDev* scenes, int l
         sum += fabsf(v[k]);
         canon_row = canon_row && fabsf(v[k] - cn[k]) <= tol;
     }
-    const float di = (L.act[c] && sum > 0.f) ? 1.0f / sum : 0.f;
-    if (valid) gbl(L.dinv)[c] = (canon_row && di > 0.f) ? -di : di;
+    const bool act = L.act[c] != 0;
+    const float di = (act && sum > 0.f) ? 1.0f / sum : 0.f;
+    if (valid) gbl(L.dinv)[c] = (canon_row && di > 0.f) ? -di : (act && di == 0.f ? -0.f : di);
     // active rows that will read their stored coefficients (for the kernels'
     // byte counts): one atomic per block, once per solve
     const int stored = __syncthreads_count(valid && di > 0.f && !canon_row);
@@ -356,14 +399,14 @@ __global__ void __launch_bounds__(kBlock) k_mg_down0(const SceneDev* scenes, con
     gbl(L1.r)[c] = 0.25f * acc;
 }
 
-// Level l (1 <= l <= n-2): x = r * dinv, y = res = r - A_l x.
+// Level l (1 <= l < t): x = r * dinv, y = res = r - A_l x.
 __global__ void __launch_bounds__(kBlock) k_mg_resid(const SceneDev* scenes, const int2* map, int l)
 {
     const SceneBlock sb = scene_block(map, true);
     const SceneDev& s = scenes[sb.scene];
     if (!mg_iterating(s)) return;
     const MgLevels& M = *s.mg;
-    if (l > M.n - 2) return;
+    if (l >= M.t) return;   // in k_mg_coarse
     const MgLevel& L = M.lv[l];
     const int LI = L.I, LJ = L.J, CL = LI * LJ;
     const int c = sb.block * kBlock + threadIdx.x;
@@ -389,14 +432,14 @@ __global__ void __launch_bounds__(kBlock) k_mg_resid(const SceneDev* scenes, con
     gbl(L.y)[c] = rc - ax;
 }
 
-// Level l-1 -> l (2 <= l <= n-1): r_l = R y_{l-1}.
+// Level l-1 -> l (2 <= l < t): r_l = R y_{l-1}.
 __global__ void __launch_bounds__(kBlock) k_mg_restrict(const SceneDev* scenes, const int2* map, int l)
 {
     const SceneBlock sb = scene_block(map, true);
     const SceneDev& s = scenes[sb.scene];
     if (!mg_iterating(s)) return;
     const MgLevels& M = *s.mg;
-    if (l > M.n - 1) return;
+    if (l >= M.t) return;   // in k_mg_coarse
     const MgLevel& L = M.lv[l];
     const MgLevel& F = M.lv[l - 1];
     const int LJ = L.J, FI = F.I, FJ = F.J;
@@ -408,72 +451,195 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(const SceneDev* scenes, 
         return;
     }
     const int ci = c / LJ, cj = c - ci * LJ;
-    float v[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const int fi = min(max(2 * ci - 1 + a, 0), FI - 1), fj = min(max(2 * cj - 1 + b, 0), FJ - 1);
-            v[a][b] = fy[fi * FJ + fj];
-        }
-    float acc = 0.f;
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const int fi = 2 * ci - 1 + a, fj = 2 * cj - 1 + b;
-            const bool ok = fi >= 0 && fi < FI && fj >= 0 && fj < FJ;
-            acc += ((w1(a) * w1(b)) * v[a][b]) * (ok ? 1.f : 0.f);   // inactive fine cells hold y = 0
-        }
-    gbl(L.r)[c] = 0.25f * acc;
+    gbl(L.r)[c] = restrict_at(fy, FI, FJ, ci, cj);
 }
 
-// Bottom level: kMgBottomSweeps L1-Jacobi sweeps from zero, one block per
-// scene, one cell per thread, the iterate in LDS with a two-cell zero halo
-// (so the stencil needs no bounds tests); result in y.
-__global__ void __launch_bounds__(kBlock) k_mg_bottom(const SceneDev* scenes)
+// Levels t .. n-1 of one scene in one block (t = MgLevels::t, chosen so
+// that they fit in LDS): r_t = R y_{t-1} (r_1 comes from k_mg_down0 when
+// t = 1), the down steps of k_mg_resid and k_mg_restrict to the bottom,
+// kMgBottomSweeps L1-Jacobi sweeps from zero there, then the up steps of
+// k_mg_prolong and k_mg_post back to level t, whose y it writes for the
+// level above.  The arithmetic and its order are those of the per-level
+// kernels.  LDS: every level's r and dinv (a cell is active exactly where
+// its dinv's bit pattern is nonzero), and two arrays for the x and y of the
+// level at hand (the finer level's y is dead once the coarser level's r, or
+// the finer level's x, is made from it).  The bottom's iterate has its own
+// array with a two-cell zero halo, so its stencil needs no bounds tests.
+__device__ __forceinline__ bool mg_active(float dinv) { return __float_as_uint(dinv) != 0u; }
+
+// row_dot with the canonical stencil in LDS and v_k = V(k) made as it is
+// summed (same terms, same order).  The stored coefficients are addressed by
+// 32-bit byte offsets from the level's uniform base.
+template <typename F>
+__device__ __forceinline__ float coarse_row_dot(const float* cn, gptr<const float> coef, int CL, int c, float dinv_c,
+                                                F V)
 {
-    static_assert(kMgBottomMax * kMgBottomMax <= kBlock, "one bottom cell per thread");
+    float a[kMgStencil];
+    if (dinv_c > 0.f) {
+        gptr<const char> base = (gptr<const char>)coef;
+#pragma unroll
+        for (int k = 0; k < kMgStencil; k++)
+            a[k] = *(gptr<const float>)(base + (uint32_t)(k * CL + c) * 4u);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kMgStencil; k++) a[k] = cn[k];
+    }
+    float ax = 0.f;
+#pragma unroll
+    for (int k = 0; k < kMgStencil; k++) ax += a[k] * V(k);
+    return ax;
+}
+
+__global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* scenes)
+{
+    constexpr int NT = kMgCoarseThreads;
     constexpr int W = kMgBottomMax + 4;
-    __shared__ float xs[2][W * W];
+    static_assert(kMgBottomMax * kMgBottomMax <= NT, "one bottom cell per thread");
+    __shared__ float rs[kMgCoarseCells], ds[kMgCoarseCells];
+    __shared__ float tx[kMgCoarseTop], ty[kMgCoarseTop];
+    __shared__ float xb[2][W * W];
+    __shared__ float cns[kMgMaxLevels][kMgStencil];
     const SceneDev& s = scenes[blockIdx.y];
     if (!mg_iterating(s)) return;
     const MgLevels& M = *s.mg;
-    const MgLevel& L = M.lv[M.n - 1];
-    const int CL = L.I * L.J;   // <= kBlock (the hierarchy stops there)
-    const int c = threadIdx.x;
-    const bool in = c < CL;
-    const int ci = in ? c / L.J : 0, cj = in ? c - ci * L.J : 0;
-    gptr<const float> coef = gbl<const float>(L.coef);
-    float cf[kMgStencil];
+    const int t = M.t, nb = M.n - 1;
+    const int tid = threadIdx.x;
+    if (tid < kMgMaxLevels * kMgStencil) (&cns[0][0])[tid] = (&M.canon[0][0])[tid];
+    {   // dinv of levels t .. n-1 (contiguous), r_1 when t = 1
+        gptr<const float> dg = gbl<const float>(M.lv[t].dinv);
+        const int TC = M.tcells;
+        constexpr int Q = (kMgCoarseCells + NT - 1) / NT;
+        float dv[Q];
 #pragma unroll
-    for (int k = 0; k < kMgStencil; k++) cf[k] = coef[(size_t)k * CL + (in ? c : 0)];
-    const float dv = fabsf(gbl<const float>(L.dinv)[in ? c : 0]), rv = gbl<const float>(L.r)[in ? c : 0];
-    for (int t = threadIdx.x; t < 2 * W * W; t += kBlock) (&xs[0][0])[t] = 0.f;
+        for (int q = 0; q < Q; q++) dv[q] = dg[min(tid + q * NT, TC - 1)];
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+            if (tid + q * NT < TC) ds[tid + q * NT] = dv[q];
+        if (t == 1) {
+            const int CL = M.lv[1].I * M.lv[1].J;
+            gptr<const float> r1 = gbl<const float>(M.lv[1].r);
+            constexpr int Q1 = kMgCoarseTop / NT;
+            float rv[Q1];
+#pragma unroll
+            for (int q = 0; q < Q1; q++) rv[q] = r1[min(tid + q * NT, CL - 1)];
+#pragma unroll
+            for (int q = 0; q < Q1; q++)
+                if (tid + q * NT < CL) rs[tid + q * NT] = rv[q];
+        }
+    }
     __syncthreads();
-    const int home = (ci + 2) * W + (cj + 2);
-    int cur = 0;
-    for (int sw = 0; sw < kMgBottomSweeps; sw++) {
-        if (in) {
-            float ax = 0.f;
-#pragma unroll
-            for (int k = 0; k < kMgStencil; k++) ax += cf[k] * xs[cur][home + (k / 5 - 2) * W + (k % 5 - 2)];
-            xs[cur ^ 1][home] = xs[cur][home] + dv * (rv - ax);
+    if (t > 1) {   // r_t = R y_{t-1} from the launch before
+        const MgLevel& L = M.lv[t];
+        const MgLevel& F = M.lv[t - 1];
+        const int LJ = L.J, CL = L.I * LJ;
+        gptr<const float> fy = gbl<const float>(F.y);
+        for (int c = tid; c < CL; c += NT) {
+            const int ci = c / LJ, cj = c - ci * LJ;
+            rs[c] = mg_active(ds[c]) ? restrict_at(fy, F.I, F.J, ci, cj) : 0.f;
         }
         __syncthreads();
-        cur ^= 1;
     }
-    if (in) gbl(L.y)[c] = xs[cur][home];
+    for (int l = t; l < nb; l++) {   // down: x = r dinv, y = r - A x, r_{l+1} = R y
+        const MgLevel& L = M.lv[l];
+        const int LI = L.I, LJ = L.J, CL = LI * LJ, o = M.roff[l];
+        for (int c = tid; c < CL; c += NT) {
+            const float dc = ds[o + c];
+            float y = 0.f;   // inactive row: r = 0 and no coefficients
+            if (dc != 0.f) {
+                const int ci = c / LJ, cj = c - ci * LJ;
+                const float ax = coarse_row_dot(cns[l], gbl<const float>(L.coef), CL, c, dc, [&](int k) {
+                    const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
+                    const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
+                    const int n = o + min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1);
+                    return (rs[n] * fabsf(ds[n])) * (ok ? 1.f : 0.f);
+                });
+                y = rs[o + c] - ax;
+            }
+            ty[c] = y;
+        }
+        __syncthreads();
+        const MgLevel& N = M.lv[l + 1];
+        const int NJ = N.J, NC = N.I * NJ, on = M.roff[l + 1];
+        for (int c = tid; c < NC; c += NT) {
+            const int ci = c / NJ, cj = c - ci * NJ;
+            rs[on + c] = mg_active(ds[on + c]) ? restrict_at(static_cast<const float*>(ty), LI, LJ, ci, cj) : 0.f;
+        }
+        __syncthreads();
+    }
+    {   // bottom: y = kMgBottomSweeps sweeps of x += dinv (r - A x) from 0
+        const MgLevel& L = M.lv[nb];
+        const int CL = L.I * L.J, o = M.roff[nb];   // CL <= kMgBottomMax^2 (the hierarchy stops there)
+        const bool in = tid < CL;
+        const int c = in ? tid : 0;
+        const int ci = c / L.J, cj = c - ci * L.J;
+        gptr<const float> coef = gbl<const float>(L.coef);
+        float cf[kMgStencil];
+#pragma unroll
+        for (int k = 0; k < kMgStencil; k++) cf[k] = coef[(size_t)k * CL + c];
+        const float dv = fabsf(ds[o + c]), rv = rs[o + c];
+        for (int k = tid; k < 2 * W * W; k += NT) (&xb[0][0])[k] = 0.f;
+        __syncthreads();
+        const int home = (ci + 2) * W + (cj + 2);
+        int cur = 0;
+        for (int sw = 0; sw < kMgBottomSweeps; sw++) {
+            if (in) {
+                float ax = 0.f;
+#pragma unroll
+                for (int k = 0; k < kMgStencil; k++) ax += cf[k] * xb[cur][home + (k / 5 - 2) * W + (k % 5 - 2)];
+                xb[cur ^ 1][home] = xb[cur][home] + dv * (rv - ax);
+            }
+            __syncthreads();
+            cur ^= 1;
+        }
+        if (in) {
+            const float y = xb[cur][home];
+            ty[c] = y;
+            if (nb == t) gbl(L.y)[c] = y;
+        }
+        __syncthreads();
+    }
+    for (int l = nb - 1; l >= t; l--) {   // up: x = r dinv + P y_{l+1}, y = x + dinv (r - A x)
+        const MgLevel& L = M.lv[l];
+        const MgLevel& N = M.lv[l + 1];
+        const int LI = L.I, LJ = L.J, CL = LI * LJ, o = M.roff[l];
+        for (int c = tid; c < CL; c += NT) {
+            const float dc = ds[o + c];
+            float x = 0.f;
+            if (mg_active(dc)) {
+                const int ci = c / LJ, cj = c - ci * LJ;
+                x = rs[o + c] * fabsf(dc) + prolong_at(static_cast<const float*>(ty), N.I, N.J, ci, cj);
+            }
+            tx[c] = x;
+        }
+        __syncthreads();
+        gptr<float> yg = gbl(L.y);
+        for (int c = tid; c < CL; c += NT) {
+            const float dc = ds[o + c];
+            float y = 0.f;   // inactive row: x = 0
+            if (dc != 0.f) {
+                const int ci = c / LJ, cj = c - ci * LJ;
+                const float ax = coarse_row_dot(cns[l], gbl<const float>(L.coef), CL, c, dc, [&](int k) {
+                    const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
+                    const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
+                    return tx[min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1)] * (ok ? 1.f : 0.f);
+                });
+                y = tx[c] + fabsf(dc) * (rs[o + c] - ax);   // tx[c] = the centre term x_12
+            }
+            ty[c] = y;
+            if (l == t) yg[c] = y;
+        }
+        __syncthreads();
+    }
 }
 
-// Level l (1 <= l <= n-2), up: x = r * dinv + P y_{l+1}.
+// Level l (1 <= l < t), up: x = r * dinv + P y_{l+1}.
 __global__ void __launch_bounds__(kBlock) k_mg_prolong(const SceneDev* scenes, const int2* map, int l)
 {
     const SceneBlock sb = scene_block(map, true);
     const SceneDev& s = scenes[sb.scene];
     if (!mg_iterating(s)) return;
     const MgLevels& M = *s.mg;
-    if (l > M.n - 2) return;
+    if (l >= M.t) return;   // in k_mg_coarse
     const MgLevel& L = M.lv[l];
     const MgLevel& Cl = M.lv[l + 1];
     const int LJ = L.J;
@@ -489,14 +655,14 @@ __global__ void __launch_bounds__(kBlock) k_mg_prolong(const SceneDev* scenes, c
     gbl(L.x)[c] = rc * fabsf(dc) + pc;
 }
 
-// Level l (1 <= l <= n-2), up: y = x + dinv (r - A_l x).
+// Level l (1 <= l < t), up: y = x + dinv (r - A_l x).
 __global__ void __launch_bounds__(kBlock) k_mg_post(const SceneDev* scenes, const int2* map, int l)
 {
     const SceneBlock sb = scene_block(map, true);
     const SceneDev& s = scenes[sb.scene];
     if (!mg_iterating(s)) return;
     const MgLevels& M = *s.mg;
-    if (l > M.n - 2) return;
+    if (l >= M.t) return;   // in k_mg_coarse
     const MgLevel& L = M.lv[l];
     const int LI = L.I, LJ = L.J, CL = LI * LJ;
     const int c = sb.block * kBlock + threadIdx.x;
@@ -509,13 +675,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_post(const SceneDev* scenes, cons
     }
     const int ci = c / LJ, cj = c - ci * LJ;
     float xv[kMgStencil];
-#pragma unroll
-    for (int k = 0; k < kMgStencil; k++) {
-        const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
-        const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
-        const float v = x[min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1)];
-        xv[k] = v * (ok ? 1.f : 0.f);
-    }
+    gather25(x, LI, LJ, ci, cj, xv);
     const float rc = gbl<const float>(L.r)[c];
     const float ax = row_dot(M, l, gbl<const float>(L.coef), CL, c, dc, xv);
     gbl(L.y)[c] = xv[12] + fabsf(dc) * (rc - ax);
@@ -606,5 +766,8 @@ FG_MG_LAUNCH_MAP(mg_restrict, int, l)
 FG_MG_LAUNCH_MAP(mg_prolong, int, l)
 FG_MG_LAUNCH_MAP(mg_post, int, l)
 FG_MG_LAUNCH_MAP(mg_up0, int, which)
-void launch_mg_bottom(dim3 g, hipStream_t st, const SceneDev* sc) { fg_launch(k_mg_bottom, g, dim3(kBlock), st, sc); }
+void launch_mg_coarse(dim3 g, hipStream_t st, const SceneDev* sc)
+{
+    fg_launch(k_mg_coarse, g, dim3(kMgCoarseThreads), st, sc);
+}
 }  // namespace fg
