@@ -120,6 +120,7 @@ struct fg_scene {
     SceneDev* d = nullptr;        // device descriptor
     SceneDev* hpin = nullptr;     // pinned staging for descriptor uploads
     SceneScalars* hsc = nullptr;  // pinned readback of the scalars
+    int iters_hint[2] = {0, 0};   // first-sweep PCG iterations of the last solve (pressure, density)
     int64_t n_bound = 0;          // upper bound of live particles (host mirror)
     int64_t n_live = 0;           // live particles at the last readback
     bool vmax_ready = false;
@@ -1021,7 +1022,14 @@ int run_solve(Group& G, bool rhs_from_grid)
         launch_mg_galerkin(gl, st, G.d, l);
         launch_mg_sym(gl, st, G.d, l);
     }
-    constexpr int kPoll = 16;
+    // A poll is a host round trip; launches after every scene of the group
+    // has stopped run empty.  The first poll of a solve comes after as many
+    // iterations as its scenes' first sweeps took the last time (plus the
+    // launch that sees the stop), later ones every kPollNext.
+    constexpr int kPollFirst = 16, kPollNext = 4, kPollMax = 64;
+    const int kind = rhs_from_grid ? 0 : 1;
+    int hint = 0;
+    for (fg_scene* s : G.sc) hint = std::max(hint, s->iters_hint[kind]);
     int32_t flags[3];
     for (int sweep = 0; sweep <= kMaxSweeps; sweep++) {
         if (sweep > 0) launch_refine_update(g.pcg, st, G.d);
@@ -1045,8 +1053,9 @@ int run_solve(Group& G, bool rhs_from_grid)
             kp.done(0);
         }
         int it = 0;
+        int chunk = sweep > 0 ? kPollNext : hint > 0 ? std::min(hint + 1, kPollMax) : kPollFirst;
         while (count > 0) {
-            for (int k = 0; k < kPoll; k++, it++) {
+            for (int k = 0; k < chunk; k++, it++) {
                 {
                     KProf kp(G, FG_K_PCG_A, true);
                     launch_pcg_a(S.grid(0), st, S.d, S.map[0], it);
@@ -1070,10 +1079,15 @@ int run_solve(Group& G, bool rhs_from_grid)
                 FGCHK(poll_group(G, -1, flags));
                 count = flags[1];
             }
-            if (count > 0 && it > G.max_inner + kPoll) return fail(FG_E_STATE, "inner solve did not stop at its cap");
+            if (count > 0 && it > G.max_inner + kPollMax) return fail(FG_E_STATE, "inner solve did not stop at its cap");
+            chunk = kPollNext;
         }
     }
     FGCHK(read_group_scalars(G));
+    for (fg_scene* s : G.sc) {
+        const SceneScalars& h = *s->hsc;
+        s->iters_hint[kind] = h.sweeps > 1 ? h.total_iters - h.inner_iters : h.total_iters;
+    }
     // work units of the launches that iterated (the rest exited at once)
     if (ctx->prof_kernel == FG_K_PCG_A || ctx->prof_kernel == FG_K_PCG_B)
         for (fg_scene* s : G.sc) G.L->prof_units += (int64_t)s->hsc->total_iters * s->C;

@@ -32,8 +32,8 @@ constexpr int kMgStencil = 25;           // 5x5 Galerkin stencil of the coarse l
 // level >= 1 with at most kMgCoarseTop cells whose levels t .. n-1 hold at
 // most kMgCoarseCells cells together (their r stays in LDS)
 constexpr int kMgCoarseThreads = 1024;
-constexpr int kMgCoarseTop = 4096;
-constexpr int kMgCoarseCells = 5632;
+constexpr int kMgCoarseTop = 9216;
+constexpr int kMgCoarseCells = 12288;
 constexpr int kFixupLongCap = 512;       // longest cell the wave-cooperative id sort takes
 
 // Material codes and tests, FlipSolver2dLib/materialgrid.h:6-20.

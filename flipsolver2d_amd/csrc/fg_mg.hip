@@ -461,10 +461,12 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(const SceneDev* scenes, 
 // k_mg_prolong and k_mg_post back to level t, whose y it writes for the
 // level above.  The arithmetic and its order are those of the per-level
 // kernels.  LDS: every level's r and dinv (a cell is active exactly where
-// its dinv's bit pattern is nonzero), and two arrays for the x and y of the
-// level at hand (the finer level's y is dead once the coarser level's r, or
-// the finer level's x, is made from it).  The bottom's iterate has its own
-// array with a two-cell zero halo, so its stencil needs no bounds tests.
+// its dinv's bit pattern is nonzero) and one work array for the level at
+// hand: its residual y on the way down (dead once the coarser r is made),
+// its x on the way up.  On the way up each level's y overwrites its r in
+// place (a thread reads only its own cell's r there), where the finer level
+// reads it.  The bottom's iterate has its own array with a two-cell zero
+// halo, so its stencil needs no bounds tests.
 __device__ __forceinline__ bool mg_active(float dinv) { return __float_as_uint(dinv) != 0u; }
 
 // row_dot with the canonical stencil in LDS and v_k = V(k) made as it is
@@ -496,7 +498,7 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
     constexpr int W = kMgBottomMax + 4;
     static_assert(kMgBottomMax * kMgBottomMax <= NT, "one bottom cell per thread");
     __shared__ float rs[kMgCoarseCells], ds[kMgCoarseCells];
-    __shared__ float tx[kMgCoarseTop], ty[kMgCoarseTop];
+    __shared__ float tx[kMgCoarseTop];
     __shared__ float xb[2][W * W];
     __shared__ float cns[kMgMaxLevels][kMgStencil];
     const SceneDev& s = scenes[blockIdx.y];
@@ -518,7 +520,7 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
         if (t == 1) {
             const int CL = M.lv[1].I * M.lv[1].J;
             gptr<const float> r1 = gbl<const float>(M.lv[1].r);
-            constexpr int Q1 = kMgCoarseTop / NT;
+            constexpr int Q1 = (kMgCoarseTop + NT - 1) / NT;
             float rv[Q1];
 #pragma unroll
             for (int q = 0; q < Q1; q++) rv[q] = r1[min(tid + q * NT, CL - 1)];
@@ -555,14 +557,14 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
                 });
                 y = rs[o + c] - ax;
             }
-            ty[c] = y;
+            tx[c] = y;
         }
         __syncthreads();
         const MgLevel& N = M.lv[l + 1];
         const int NJ = N.J, NC = N.I * NJ, on = M.roff[l + 1];
         for (int c = tid; c < NC; c += NT) {
             const int ci = c / NJ, cj = c - ci * NJ;
-            rs[on + c] = mg_active(ds[on + c]) ? restrict_at(static_cast<const float*>(ty), LI, LJ, ci, cj) : 0.f;
+            rs[on + c] = mg_active(ds[on + c]) ? restrict_at(static_cast<const float*>(tx), LI, LJ, ci, cj) : 0.f;
         }
         __syncthreads();
     }
@@ -593,7 +595,7 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
         }
         if (in) {
             const float y = xb[cur][home];
-            ty[c] = y;
+            rs[o + c] = y;
             if (nb == t) gbl(L.y)[c] = y;
         }
         __syncthreads();
@@ -602,12 +604,13 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
         const MgLevel& L = M.lv[l];
         const MgLevel& N = M.lv[l + 1];
         const int LI = L.I, LJ = L.J, CL = LI * LJ, o = M.roff[l];
+        const float* yn = rs + M.roff[l + 1];   // y_{l+1}, in place of r_{l+1}
         for (int c = tid; c < CL; c += NT) {
             const float dc = ds[o + c];
             float x = 0.f;
             if (mg_active(dc)) {
                 const int ci = c / LJ, cj = c - ci * LJ;
-                x = rs[o + c] * fabsf(dc) + prolong_at(static_cast<const float*>(ty), N.I, N.J, ci, cj);
+                x = rs[o + c] * fabsf(dc) + prolong_at(yn, N.I, N.J, ci, cj);
             }
             tx[c] = x;
         }
@@ -625,7 +628,7 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
                 });
                 y = tx[c] + fabsf(dc) * (rs[o + c] - ax);   // tx[c] = the centre term x_12
             }
-            ty[c] = y;
+            rs[o + c] = y;
             if (l == t) yg[c] = y;
         }
         __syncthreads();
