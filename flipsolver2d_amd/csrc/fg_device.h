@@ -268,6 +268,16 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 // blockIdx.x of the map names them (the map is already in XCD-tiled order).
 // Without one, the grid is (blocks, scenes) and the block index is remapped
 // by xcd_tile when `tiled`.
+// 1.0f or 0.0f, opaque to the optimizer.  A loaded value times mask01(ok)
+// keeps its load unconditional: with a plain (ok ? 1 : 0) the compiler
+// specialises the code on ok and guards each load by a branch and a wait.
+__device__ __forceinline__ float mask01(bool ok)
+{
+    float m = ok ? 1.f : 0.f;
+    asm volatile("" : "+v"(m));
+    return m;
+}
+
 struct SceneBlock {
     int scene, block;
 };

@@ -79,7 +79,7 @@ __device__ __forceinline__ float prolong_at(P y, int I, int J, int fi, int fj)
 #pragma unroll
         for (int b = 0; b < 2; b++) {
             const bool ok = pi[a] >= 0 && pi[a] < I && qj[b] >= 0 && qj[b] < J;
-            acc += ((wi[a] * wi[b]) * v[a][b]) * (ok ? 1.f : 0.f);
+            acc += ((wi[a] * wi[b]) * v[a][b]) * mask01(ok);
         }
     return acc;
 }
@@ -127,7 +127,7 @@ __device__ __forceinline__ float restrict_at(P fy, int FI, int FJ, int ci, int c
         for (int b = 0; b < 4; b++) {
             const int fi = 2 * ci - 1 + a, fj = 2 * cj - 1 + b;
             const bool ok = fi >= 0 && fi < FI && fj >= 0 && fj < FJ;
-            acc += ((w1(a) * w1(b)) * v[a][b]) * (ok ? 1.f : 0.f);
+            acc += ((w1(a) * w1(b)) * v[a][b]) * mask01(ok);
         }
     return 0.25f * acc;
 }
@@ -142,7 +142,7 @@ __device__ __forceinline__ void gather25(P x, int LI, int LJ, int ci, int cj, fl
         const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
         const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
         const float v = x[min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1)];
-        xv[k] = v * (ok ? 1.f : 0.f);
+        xv[k] = v * mask01(ok);
     }
 }
 
@@ -198,7 +198,7 @@ __device__ __forceinline__ void galerkin_add(double (&acc)[kMgStencil], uint32_t
 {
     if constexpr (DI >= -2 && DI <= 2 && DJ >= -2 && DJ <= 2) {
         constexpr int k = (DI + 2) * 5 + (DJ + 2);
-        if ((am >> k) & 1u) acc[k] += w;
+        acc[k] += ((am >> k) & 1u) ? w : 0.0;   // a select, not a branch
     }
 }
 
@@ -227,38 +227,70 @@ __device__ __forceinline__ void galerkin_row(const SceneDev& s, const MgLevels& 
     const int CL = L.I * L.J;
     // activity of the 5x5 coarse neighbourhood, bit (di+2)*5 + (dj+2)
     uint32_t am = 0;
+    gptr<const uint8_t> act = gbl<const uint8_t>(L.act);
 #pragma unroll
-    for (int k = 0; k < kMgStencil; k++) {
+    for (int k = 0; k < kMgStencil; k++) {   // clamped loads, all issued together
         const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
-        if (ni >= 0 && ni < L.I && nj >= 0 && nj < L.J && L.act[ni * L.J + nj]) am |= 1u << k;
+        const bool in = ni >= 0 && ni < L.I && nj >= 0 && nj < L.J;
+        const uint8_t v = act[min(max(ni, 0), L.I - 1) * L.J + min(max(nj, 0), L.J - 1)];
+        am |= (in && v) ? 1u << k : 0u;
     }
     double acc[kMgStencil];
 #pragma unroll
     for (int k = 0; k < kMgStencil; k++) acc[k] = 0.0;
     const int FI = FINE0 ? s.I : M.lv[l - 1].I, FJ = FINE0 ? s.J : M.lv[l - 1].J;
     const int CF = FI * FJ;
-    static_for<16>([&](auto ab) {
-        constexpr int a = decltype(ab)::value / 4, b = decltype(ab)::value % 4;
-        const int fi = 2 * ci - 1 + a, fj = 2 * cj - 1 + b;
-        if (fi < 0 || fi >= FI || fj < 0 || fj >= FJ) return;
-        const int f = fi * FJ + fj;
-        if constexpr (FINE0) {
-            const uint8_t code = s.stencil[f];
-            if (!(code & 0x80)) return;
-            galerkin_spread<a, b, 0, 0>(acc, am, (double)((code >> 4) & 7));
-            galerkin_spread<a, b, -1, 0>(acc, am, (code & 1) ? -1.0 : 0.0);
-            galerkin_spread<a, b, 1, 0>(acc, am, (code & 2) ? -1.0 : 0.0);
-            galerkin_spread<a, b, 0, -1>(acc, am, ((code & 4) && fj > 0) ? -1.0 : 0.0);   // not the row-wrapping coupling
-            galerkin_spread<a, b, 0, 1>(acc, am, ((code & 8) && fj < FJ - 1) ? -1.0 : 0.0);
-        } else {
-            if (!M.lv[l - 1].act[f]) return;
-            const float* fc = M.lv[l - 1].coef;
-            static_for<kMgStencil>([&](auto ee) {
-                constexpr int e = decltype(ee)::value;
-                galerkin_spread<a, b, e / 5 - 2, e % 5 - 2>(acc, am, (double)fc[(size_t)e * CF + f]);
+    // Every load is unconditional (clamped fine index; a fine cell outside
+    // the grid contributes zero), so the loads of all 16 fine rows can be in
+    // flight together instead of one branch-guarded batch after another.
+    // Inactive fine rows hold all-zero coefficients (level >= 1) or a
+    // non-fluid code (level 0): they add nothing.
+    if constexpr (FINE0) {
+        gptr<const uint8_t> code = gbl<const uint8_t>(s.stencil);
+        uint8_t cd[16];
+        static_for<16>([&](auto ab) {
+            constexpr int a = decltype(ab)::value / 4, b = decltype(ab)::value % 4;
+            const int fi = 2 * ci - 1 + a, fj = 2 * cj - 1 + b;
+            const bool in = fi >= 0 && fi < FI && fj >= 0 && fj < FJ;
+            const uint8_t k = code[min(max(fi, 0), FI - 1) * FJ + min(max(fj, 0), FJ - 1)];
+            cd[decltype(ab)::value] = in ? k : 0;
+        });
+        static_for<16>([&](auto ab) {
+            constexpr int a = decltype(ab)::value / 4, b = decltype(ab)::value % 4;
+            const int fj = 2 * cj - 1 + b;
+            const uint8_t k = cd[decltype(ab)::value];
+            const double on = (k & 0x80) ? 1.0 : 0.0;
+            galerkin_spread<a, b, 0, 0>(acc, am, on * (double)((k >> 4) & 7));
+            galerkin_spread<a, b, -1, 0>(acc, am, (k & 1) ? -on : 0.0);
+            galerkin_spread<a, b, 1, 0>(acc, am, (k & 2) ? -on : 0.0);
+            galerkin_spread<a, b, 0, -1>(acc, am, ((k & 4) && fj > 0) ? -on : 0.0);   // not the row-wrapping coupling
+            galerkin_spread<a, b, 0, 1>(acc, am, ((k & 8) && fj < FJ - 1) ? -on : 0.0);
+        });
+    } else {
+        gptr<const float> fc = gbl<const float>(M.lv[l - 1].coef);
+        gptr<const char> fb = (gptr<const char>)fc;
+        static_for<8>([&](auto hh) {   // two fine cells at a time: 50 loads in flight
+            constexpr int a = decltype(hh)::value / 2, b0 = 2 * (decltype(hh)::value % 2);
+            float v[2][kMgStencil];
+            bool in[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int fi = 2 * ci - 1 + a, fj = 2 * cj - 1 + b0 + h;
+                in[h] = fi >= 0 && fi < FI && fj >= 0 && fj < FJ;
+                const uint32_t f = (uint32_t)(min(max(fi, 0), FI - 1) * FJ + min(max(fj, 0), FJ - 1));
+#pragma unroll
+                for (int e = 0; e < kMgStencil; e++)
+                    v[h][e] = *(gptr<const float>)(fb + ((uint32_t)e * (uint32_t)CF + f) * 4u);
+            }
+            static_for<2>([&](auto h2) {
+                constexpr int h = decltype(h2)::value, b = b0 + h;
+                static_for<kMgStencil>([&](auto ee) {
+                    constexpr int e = decltype(ee)::value;
+                    galerkin_spread<a, b, e / 5 - 2, e % 5 - 2>(acc, am, in[h] ? (double)v[h][e] : 0.0);
+                });
             });
-        }
-    });
+        });
+    }
 #pragma unroll
     for (int k = 0; k < kMgStencil; k++) L.coef[(size_t)k * CL + c] = (float)(0.25 * acc[k]);
 }
@@ -302,29 +334,36 @@ __global__ void __launch_bounds__(kBlock) k_mg_sym(const SceneDev* scenes, int l
     const int c = valid ? c0 : CL - 1;   // past the level: recompute the last row, store nothing
     const int ci = c / LJ, cj = c - ci * LJ;
     gptr<float> coef = gbl(L.coef);
+    gptr<const char> cb = (gptr<const char>)coef;
     float v[kMgStencil];
+    const uint8_t av = gbl<const uint8_t>(L.act)[c];
 #pragma unroll
-    for (int k = 0; k < kMgStencil; k++) {
+    for (int k = 0; k < kMgStencil; k++) {   // every load unconditional: clamped neighbour, 0/1 factor
         if (k < 12) {
             const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
             const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
-            const float m = coef[(size_t)(24 - k) * CL + (ok ? ni * LJ + nj : c)];
-            v[k] = ok ? m : 0.f;
+            const uint32_t n = (uint32_t)(min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1));
+            const float m = *(gptr<const float>)(cb + ((uint32_t)(24 - k) * (uint32_t)CL + n) * 4u);
+            v[k] = m * mask01(ok);
         } else {
-            v[k] = coef[(size_t)k * CL + c];
+            v[k] = *(gptr<const float>)(cb + ((uint32_t)k * (uint32_t)CL + (uint32_t)c) * 4u);
         }
     }
-    const float* cn = M.canon[l];
+    float cn[kMgStencil];   // read before the stores below (which the compiler cannot tell apart from them)
+#pragma unroll
+    for (int k = 0; k < kMgStencil; k++) cn[k] = M.canon[l][k];
     const float tol = 1e-5f * cn[12];
     float sum = 0.f;
     bool canon_row = true;
 #pragma unroll
     for (int k = 0; k < kMgStencil; k++) {
-        if (k < 12 && valid) coef[(size_t)k * CL + c] = v[k];
         sum += fabsf(v[k]);
         canon_row = canon_row && fabsf(v[k] - cn[k]) <= tol;
     }
-    const bool act = L.act[c] != 0;
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+        if (valid) coef[(size_t)k * CL + c] = v[k];
+    const bool act = av != 0;
     const float di = (act && sum > 0.f) ? 1.0f / sum : 0.f;
     if (valid) gbl(L.dinv)[c] = (canon_row && di > 0.f) ? -di : (act && di == 0.f ? -0.f : di);
     // active rows that will read their stored coefficients (for the kernels'
@@ -373,7 +412,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_down0(const SceneDev* scenes, con
             const uint8_t k = code[f];
             const float v = r[f];
             cd[a][b] = in ? k : 0;
-            rv[a][b] = v * (in ? 1.f : 0.f);
+            rv[a][b] = v * mask01(in);
         }
     float x0[6][6];
 #pragma unroll
@@ -425,7 +464,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_resid(const SceneDev* scenes, con
         const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
         const int n = min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1);   // clamped, not selected
         const float rn = r[n], dn = dinv[n];
-        xv[k] = (rn * fabsf(dn)) * (ok ? 1.f : 0.f);   // a 0/1 factor, not a select: the load stays unconditional
+        xv[k] = (rn * fabsf(dn)) * mask01(ok);   // a 0/1 factor, not a select: the load stays unconditional
     }
     const float rc = r[c];
     const float ax = row_dot(M, l, gbl<const float>(L.coef), CL, c, dc, xv);
@@ -553,7 +592,7 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
                     const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
                     const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
                     const int n = o + min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1);
-                    return (rs[n] * fabsf(ds[n])) * (ok ? 1.f : 0.f);
+                    return (rs[n] * fabsf(ds[n])) * mask01(ok);
                 });
                 y = rs[o + c] - ax;
             }
@@ -624,7 +663,7 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
                 const float ax = coarse_row_dot(cns[l], gbl<const float>(L.coef), CL, c, dc, [&](int k) {
                     const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
                     const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
-                    return tx[min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1)] * (ok ? 1.f : 0.f);
+                    return tx[min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1)] * mask01(ok);
                 });
                 y = tx[c] + fabsf(dc) * (rs[o + c] - ax);   // tx[c] = the centre term x_12
             }
