@@ -580,12 +580,13 @@ struct MapOffsets {
 // blocks, stored so that flat block e takes entry xcd_tile(e, total) of the
 // natural order (consecutive tiles of a scene on one XCD).  info[0] = the
 // packed count; info[4 + kind] gets the kind's total.
-__global__ void __launch_bounds__(1024) k_build_maps(const SceneDev* packed, int32_t* info, int2* maps, MapOffsets offs)
+__global__ void __launch_bounds__(1024) k_build_maps(const SceneDev* packed, int32_t* info, int2* maps, MapOffsets offs,
+                                                     int n_all)
 {
     __shared__ int pre[kMapMaxScenes + 1];
     __shared__ int wsum[1024 / 64];
     const int kind = blockIdx.x;
-    const int n = info[0];
+    const int n = n_all >= 0 ? n_all : info[0];   // n_all: every scene of the array
     const int t = threadIdx.x;
     // each thread sums a contiguous run of scenes, then a block scan of the runs
     const int per = (n + 1023) / 1024;
@@ -879,7 +880,7 @@ int pack_iterating(Group& G, SolveLaunch* S, int* count)
     if (kinds > 0) {
         MapOffsets offs{};
         for (int k = 0; k < kinds; k++) offs.off[k] = G.map_off[k];
-        hipLaunchKernelGGL(k_build_maps, dim3(kinds), dim3(1024), 0, L->stream, L->bpack, L->bflags, L->bmap, offs);
+        hipLaunchKernelGGL(k_build_maps, dim3(kinds), dim3(1024), 0, L->stream, L->bpack, L->bflags, L->bmap, offs, -1);
     }
     HIPCHK(hipMemcpyAsync(L->bflags_host, L->bflags, sizeof(int32_t) * (4 + kinds), hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
@@ -898,6 +899,39 @@ int pack_iterating(Group& G, SolveLaunch* S, int* count)
         S->blocks[0] = (unsigned)std::max(1, L->bflags_host[1]);
     }
     return FG_OK;
+}
+
+// Blocks of map kind `kind` for one scene (the host's copy of map_blocks).
+int64_t map_blocks_host(const fg_scene* s, int kind)
+{
+    if (kind == 0) return s->pcg_blocks;
+    if (s->h.precond != kPrecondMG) return 0;
+    const MgLevels& M = s->mgh;
+    const int l = kind == 1 ? 1 : kind - 1;
+    if (kind > 1 && l >= M.t) return 0;
+    return cdiv((int64_t)M.lv[l].I * M.lv[l].J, kBlock);
+}
+
+// Flat launches over every scene of the group (a solve's first sweep, where
+// all of them iterate): the maps are built on the device from the group's
+// own descriptors and their totals are known here, so nothing synchronises.
+void all_scene_launch(Group& G, SolveLaunch* S)
+{
+    Lane* L = G.L;
+    const int kinds = G.map_kinds;
+    MapOffsets offs{};
+    for (int k = 0; k < kinds; k++) offs.off[k] = G.map_off[k];
+    hipLaunchKernelGGL(k_build_maps, dim3(kinds), dim3(1024), 0, L->stream, G.d, L->bflags, L->bmap, offs, G.n);
+    *S = SolveLaunch{};
+    S->d = G.d;
+    S->ny = (unsigned)G.n;
+    S->flat = true;
+    for (int k = 0; k < kinds; k++) {
+        int64_t total = 0;
+        for (const fg_scene* s : G.sc) total += map_blocks_host(s, k);
+        S->map[k] = L->bmap + G.map_off[k];
+        S->blocks[k] = (unsigned)total;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1039,7 +1073,12 @@ int run_solve(Group& G, bool rhs_from_grid)
         // at every poll as they stop); one keeps its own descriptor
         SolveLaunch S;
         int count = 0;
-        if (G.n > 1) {
+        if (G.n > 1 && sweep == 0 && G.map_kinds > 0) {
+            // every scene takes part (one whose FP64 test already passed
+            // exits each kernel at once); the first poll repacks
+            all_scene_launch(G, &S);
+            count = G.n;
+        } else if (G.n > 1) {
             FGCHK(pack_iterating(G, &S, &count));
         } else {
             FGCHK(poll_group(G, -1, flags));
