@@ -38,16 +38,19 @@ METRIC = "particle-substeps/s + p99 frame latency, batched vs plain GPU @1e-6 re
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per work unit of the kernels bench.py can report a
-# roofline for.  k_mg_resid, the largest share of GPU time in the batched
-# workload, timed at its level-1 launch (one per V-cycle: timing every level's
-# launch perturbs the launch pipeline by ~14 %); the library counts the bytes
-# itself (unit = 1 byte): per level-1 cell r, |dinv| and y (12 B), plus the
-# 25 stored coefficients (100 B) of every active row that is not the level's
-# canonical interior stencil.
+# roofline for; the library counts the bytes itself where the unit is 1 byte.
+# k_mg_coarse, the largest share of GPU time in the batched workload (the
+# V-cycle's coarse levels, one block per scene): per scene and V-cycle the
+# dinv of its levels and its input and output vectors (4 B a cell) and once
+# each the 25 coefficients (100 B) of every row that is not its level's
+# canonical interior stencil.  k_mg_resid, timed at its level-1 launch (one
+# per V-cycle where the scene has a level below the coarse block's): per
+# level-1 cell r, |dinv| and y (12 B) plus 100 B per stored row.
 # k_pcg_a reads the stencil code (1), z (4) and the old direction (4) and
 # writes p (4) and q = A' p (4): 17 B per cell; k_pcg_b (with the V-cycle,
 # the axpy pair r -= alpha q, d += alpha p) 24 B per cell.
 KERNEL_BYTES = {
+    "mg_coarse": ("K_MG_COARSE", 1.0, "bytes"),
     "mg_resid": ("K_MG_RESID", 1.0, "bytes"),
     "pcg_a": ("K_PCG_A", 17.0, "cells"),
     "pcg_b": ("K_PCG_B", 24.0, "cells"),
@@ -55,7 +58,7 @@ KERNEL_BYTES = {
 # HBM bytes actually moved per work unit, per workload: rocprofv3 FETCH_SIZE
 # (x2.000) + WRITE_SIZE (x1.000) per dispatch, the factors being what the same
 # counters read for a 1 GiB copy kernel.
-MEASURED_TRAFFIC_PER_UNIT = {"mixed256": {"mg_resid": 1.397}, "dam1024": {"mg_resid": 2.034}}
+MEASURED_TRAFFIC_PER_UNIT = {"mixed256": {"mg_coarse": 1.772}, "dam1024": {"mg_coarse": 1.951}}
 
 
 def parse():
@@ -66,8 +69,9 @@ def parse():
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--master-seed", type=int, default=2024)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--roofline-kernel", default="mg_resid", choices=sorted(KERNEL_BYTES))
+    ap.add_argument("--roofline-kernel", default="mg_coarse", choices=sorted(KERNEL_BYTES))
     ap.add_argument("--skip-plain", action="store_true", help="batched leg only (profiling runs)")
+    ap.add_argument("--lanes", type=int, default=0, help="lanes of the batched leg (0: the autotuner's choice)")
     ap.add_argument("--tune-rounds", type=int, default=1, help="autotuner rounds (each steps 4 frames)")
     ap.add_argument("--dam-frames", type=int, default=10, help="timed frames of the 1024^2 leg (0: skip)")
     ap.add_argument("--proj-res", type=int, default=4096, help="isolated projection grid (0: skip)")
@@ -160,6 +164,10 @@ def batched_leg(fg, ctx, a):
         for sc in plain:
             sc.step_frame()
     log(f"autotuned lanes: {lanes} (rates {[round(r / 1e6, 1) for r in rates]} M)")
+    if a.lanes > 0:   # a fixed lane count (profiling runs compare like with like)
+        lanes = a.lanes
+        ctx.set_batch_lanes(lanes)
+        log(f"lanes fixed at {lanes}")
     kt = KernelTimer(ctx, fg, a.roofline_kernel)
     el_b = el_p = 0.0
     ps_b = ps_p = 0

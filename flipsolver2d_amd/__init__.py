@@ -37,7 +37,7 @@ EXPORTS = [
 MAX_LANES = 4
 REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_REJECTED, REQ_RELEASED = range(5)
 (K_ADVECT, K_SORT, K_P2G, K_SDF, K_LEVELSET, K_EXTRAPOLATE, K_PCG_A, K_PCG_B, K_G2P, K_RHS, K_APPLY,
- K_VCYCLE, K_MG_RESID) = range(13)
+ K_VCYCLE, K_MG_RESID, K_MG_COARSE) = range(14)
 PRECOND_MG, PRECOND_IPP = 0, 1
 
 

@@ -1027,13 +1027,31 @@ void launch_vcycle(const Group& G, hipStream_t st, const SolveLaunch& S, int whi
         }
         if (l + 1 < top && S.blocks[2 + l]) launch_mg_restrict(S.grid(2 + l), st, S.d, S.map[2 + l], l + 1);
     }
-    launch_mg_coarse(dim3(1, S.ny), st, S.d);
+    {
+        KProf kp(G, FG_K_MG_COARSE, true);
+        launch_mg_coarse(dim3(1, S.ny), st, S.d);
+        kp.done(0);
+    }
     for (int l = top - 1; l >= 1; l--) {
         if (!S.blocks[1 + l]) continue;
         launch_mg_prolong(S.grid(1 + l), st, S.d, S.map[1 + l], l);
         launch_mg_post(S.grid(1 + l), st, S.d, S.map[1 + l], l);
     }
     launch_mg_up0(S.grid(0), st, S.d, S.map[0], which);
+}
+
+// Algorithmic bytes of one k_mg_coarse block (one scene, one V-cycle): the
+// dinv of its levels (4 B a cell), its input (r_1, or the finer level's y it
+// restricts), the y it writes (4 B a level-t cell), and once each the
+// 25 coefficients of the rows that read stored coefficients (100 B a row).
+int64_t mg_coarse_bytes(const fg_scene* s)
+{
+    const MgLevels& M = s->mgh;
+    const int t = M.t;
+    auto cells = [&](int l) { return (int64_t)M.lv[l].I * M.lv[l].J; };
+    int64_t b = 4 * (int64_t)M.tcells + 4 * cells(t) + 4 * cells(t == 1 ? 1 : t - 1);
+    for (int l = t; l < M.n; l++) b += 4 * kMgStencil * (int64_t)s->hsc->mg_stored[l];
+    return b;
 }
 
 int run_solve(Group& G, bool rhs_from_grid)
@@ -1140,6 +1158,10 @@ int run_solve(Group& G, bool rhs_from_grid)
             const int64_t per_cycle = 12 * (int64_t)L1.I * L1.J + 4 * kMgStencil * (int64_t)s->hsc->mg_stored[1];
             G.L->prof_units += (int64_t)(s->hsc->total_iters + s->hsc->sweeps) * per_cycle;
         }
+    if (ctx->prof_kernel == FG_K_MG_COARSE)   // its algorithmic bytes per V-cycle, see mg_coarse_bytes
+        for (fg_scene* s : G.sc)
+            if (s->h.precond == kPrecondMG)
+                G.L->prof_units += (int64_t)(s->hsc->total_iters + s->hsc->sweeps) * mg_coarse_bytes(s);
     for (fg_scene* s : G.sc)
         if (s->hsc->numeric_bad) return fail(FG_E_NUMERIC, "NaN/Inf in the pressure solve");
     return FG_OK;

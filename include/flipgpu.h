@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FG_ABI_VERSION 3
+#define FG_ABI_VERSION 4
 
 enum fg_status {
     FG_OK = 0,
@@ -189,7 +189,8 @@ enum fg_kernel_id {
     FG_K_PCG_A = 6, FG_K_PCG_B = 7, FG_K_G2P = 8, FG_K_RHS = 9, FG_K_APPLY = 10,
     FG_K_VCYCLE = 11,   /* one multigrid V-cycle (all its launches) */
     FG_K_MG_RESID = 12, /* the V-cycle's level-1 residual launch; work units are its algorithmic bytes */
-    FG_K_NUM = 13
+    FG_K_MG_COARSE = 13, /* the V-cycle's coarse-levels launch (one block per scene); units: algorithmic bytes */
+    FG_K_NUM = 14
 };
 int fg_ctx_profile_kernel(fg_ctx* ctx, int32_t kernel);
 int fg_ctx_profile_read(fg_ctx* ctx, double* total_ms, double* busy_ms, int64_t* launches, int64_t* work_units);
