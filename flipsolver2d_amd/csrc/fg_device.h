@@ -250,6 +250,17 @@ __device__ __forceinline__ gptr<T> gbl(T* p)
     return (gptr<T>)p;
 }
 
+// Read-only data of a launch (descriptors, the multigrid level table, the
+// scalars earlier kernels wrote) read through the constant address space:
+// uniform addresses become scalar loads, which issue together.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ cptr<T> cst(const T* p)
+{
+    return (cptr<T>)p;
+}
+
 // ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------
@@ -355,7 +366,7 @@ __device__ __forceinline__ void sum_partials(const SceneDev& s, int slot, int nb
     __shared__ double bc[NV];
 #pragma unroll
     for (int q = 0; q < NV; q++) {
-        const double* part = s.partials + (size_t)(slot + q) * s.max_blocks;
+        gptr<const double> part = gbl<const double>(s.partials + (size_t)(slot + q) * s.max_blocks);
         double acc = 0.0;
         for (int b = threadIdx.x; b < nb; b += kBlock) acc += part[b];
         const double t = block_sum(acc, red);

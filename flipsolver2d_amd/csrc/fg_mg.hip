@@ -40,9 +40,13 @@ namespace fg {
 
 namespace {
 
+// Both stop flags are read unconditionally (scalar loads issued together
+// with the caller's other uniform loads, no branch between them).
 __device__ __forceinline__ bool mg_iterating(const SceneDev& s)
 {
-    return s.active && s.precond == kPrecondMG && !s.sc->outer_done && !s.sc->inner_done;
+    cptr<SceneScalars> sc = cst(s.sc);
+    const int od = sc->outer_done, id = sc->inner_done;
+    return (s.active != 0) & (s.precond == kPrecondMG) & (od == 0) & (id == 0);
 }
 
 // L1 diagonal of a level-0 row: n + (number of fluid neighbours)
@@ -88,10 +92,11 @@ __device__ __forceinline__ float prolong_at(P y, int I, int J, int fi, int fj)
 // stencil entries: the canonical stencil on canonical rows (dinv < 0), else
 // the stored coefficients, loaded (all 25 together, one wait) only by the
 // lanes whose rows need them.
-__device__ __forceinline__ float row_dot(const MgLevels& M, int l, gptr<const float> coef, int CL, int c,
+template <typename MT>
+__device__ __forceinline__ float row_dot(const MT& M, int l, gptr<const float> coef, int CL, int c,
                                          float dinv_c, const float (&v)[kMgStencil])
 {
-    const float* cn = M.canon[l];
+    const auto& cn = M.canon[l];
     float a[kMgStencil];
 #pragma unroll
     for (int k = 0; k < kMgStencil; k++) a[k] = cn[k];
@@ -156,9 +161,9 @@ __global__ void __launch_bounds__(kBlock) k_mg_mark(const SceneDev* scenes, int 
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active || s.precond != kPrecondMG) return;
-    const MgLevels& M = *s.mg;
+    const auto& M = *cst(s.mg);
     if (l >= M.n) return;
-    const MgLevel& L = M.lv[l];
+    const auto& L = M.lv[l];
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= L.I * L.J) return;
     const int ci = c / L.J, cj = c - ci * L.J;
@@ -220,10 +225,10 @@ __device__ __forceinline__ void galerkin_spread(double (&acc)[kMgStencil], uint3
 // A_l = R A_{l-1} P, one coarse row C = (ci, cj) per thread: the fine cells
 // f, their stencil entries g and g's coarse parents sit at compile-time
 // offsets from C, so the 25 FP64 accumulators stay in registers.
-template <bool FINE0>
-__device__ __forceinline__ void galerkin_row(const SceneDev& s, const MgLevels& M, int l, int c, int ci, int cj)
+template <bool FINE0, typename MT>
+__device__ __forceinline__ void galerkin_row(const SceneDev& s, const MT& M, int l, int c, int ci, int cj)
 {
-    const MgLevel& L = M.lv[l];
+    const auto& L = M.lv[l];
     const int CL = L.I * L.J;
     // activity of the 5x5 coarse neighbourhood, bit (di+2)*5 + (dj+2)
     uint32_t am = 0;
@@ -299,9 +304,9 @@ __global__ void __launch_bounds__(kBlock) k_mg_galerkin(const SceneDev* scenes, 
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active || s.precond != kPrecondMG) return;
-    const MgLevels& M = *s.mg;
+    const auto& M = *cst(s.mg);
     if (l >= M.n) return;
-    const MgLevel& L = M.lv[l];
+    const auto& L = M.lv[l];
     const int CL = L.I * L.J;
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= CL) return;
@@ -324,9 +329,9 @@ __global__ void __launch_bounds__(kBlock) k_mg_sym(const SceneDev* scenes, int l
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active || s.precond != kPrecondMG) return;
-    const MgLevels& M = *s.mg;
+    const auto& M = *cst(s.mg);
     if (l >= M.n) return;
-    const MgLevel& L = M.lv[l];
+    const auto& L = M.lv[l];
     const int LI = L.I, LJ = L.J, CL = LI * LJ;
     if ((int)blockIdx.x * kBlock >= CL) return;   // the whole block is past the level
     const int c0 = blockIdx.x * kBlock + threadIdx.x;
@@ -387,11 +392,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_down0(const SceneDev* scenes, con
 {
     const SceneBlock sb = scene_block(map, true);
     const SceneDev& s = scenes[sb.scene];
-    if (!mg_iterating(s)) return;
-    const MgLevel& L1 = s.mg->lv[1];
+    const auto& L1 = cst(s.mg)->lv[1];
     const int J1 = L1.J;
     const int c = sb.block * kBlock + threadIdx.x;
-    if (c >= L1.I * J1) return;
+    // one test after the uniform loads, so that they issue together
+    if (!mg_iterating(s) | (c >= L1.I * J1)) return;
     const int I = s.I, J = s.J;
     gptr<const float> r = gbl<const float>(which ? s.cr2 : s.cr);
     gptr<const uint8_t> code = gbl<const uint8_t>(s.stencil);
@@ -443,13 +448,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_resid(const SceneDev* scenes, con
 {
     const SceneBlock sb = scene_block(map, true);
     const SceneDev& s = scenes[sb.scene];
-    if (!mg_iterating(s)) return;
-    const MgLevels& M = *s.mg;
-    if (l >= M.t) return;   // in k_mg_coarse
-    const MgLevel& L = M.lv[l];
+    const auto& M = *cst(s.mg);
+    const auto& L = M.lv[l];
     const int LI = L.I, LJ = L.J, CL = LI * LJ;
     const int c = sb.block * kBlock + threadIdx.x;
-    if (c >= CL) return;
+    if (!mg_iterating(s) | (l >= M.t) | (c >= CL)) return;   // l >= t: in k_mg_coarse
     gptr<const float> r = gbl<const float>(L.r), dinv = gbl<const float>(L.dinv);
     const float dc = dinv[c];
     if (dc == 0.f) {   // inactive row: r = 0 and no coefficients
@@ -476,14 +479,12 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(const SceneDev* scenes, 
 {
     const SceneBlock sb = scene_block(map, true);
     const SceneDev& s = scenes[sb.scene];
-    if (!mg_iterating(s)) return;
-    const MgLevels& M = *s.mg;
-    if (l >= M.t) return;   // in k_mg_coarse
-    const MgLevel& L = M.lv[l];
-    const MgLevel& F = M.lv[l - 1];
+    const auto& M = *cst(s.mg);
+    const auto& L = M.lv[l];
+    const auto& F = M.lv[l - 1];
     const int LJ = L.J, FI = F.I, FJ = F.J;
     const int c = sb.block * kBlock + threadIdx.x;
-    if (c >= L.I * LJ) return;
+    if (!mg_iterating(s) | (l >= M.t) | (c >= L.I * LJ)) return;   // l >= t: in k_mg_coarse
     gptr<const float> fy = gbl<const float>(F.y);
     if (!gbl<const uint8_t>(L.act)[c]) {
         gbl(L.r)[c] = 0.f;
@@ -541,9 +542,9 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
     __shared__ float xb[2][W * W];
     __shared__ float cns[kMgMaxLevels][kMgStencil];
     const SceneDev& s = scenes[blockIdx.y];
-    if (!mg_iterating(s)) return;
-    const MgLevels& M = *s.mg;
+    const auto& M = *cst(s.mg);
     const int t = M.t, nb = M.n - 1;
+    if (!mg_iterating(s)) return;
     const int tid = threadIdx.x;
     if (tid < kMgMaxLevels * kMgStencil) (&cns[0][0])[tid] = (&M.canon[0][0])[tid];
     {   // dinv of levels t .. n-1 (contiguous), r_1 when t = 1
@@ -570,8 +571,8 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
     }
     __syncthreads();
     if (t > 1) {   // r_t = R y_{t-1} from the launch before
-        const MgLevel& L = M.lv[t];
-        const MgLevel& F = M.lv[t - 1];
+        const auto& L = M.lv[t];
+        const auto& F = M.lv[t - 1];
         const int LJ = L.J, CL = L.I * LJ;
         gptr<const float> fy = gbl<const float>(F.y);
         for (int c = tid; c < CL; c += NT) {
@@ -581,7 +582,7 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
         __syncthreads();
     }
     for (int l = t; l < nb; l++) {   // down: x = r dinv, y = r - A x, r_{l+1} = R y
-        const MgLevel& L = M.lv[l];
+        const auto& L = M.lv[l];
         const int LI = L.I, LJ = L.J, CL = LI * LJ, o = M.roff[l];
         for (int c = tid; c < CL; c += NT) {
             const float dc = ds[o + c];
@@ -599,7 +600,7 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
             tx[c] = y;
         }
         __syncthreads();
-        const MgLevel& N = M.lv[l + 1];
+        const auto& N = M.lv[l + 1];
         const int NJ = N.J, NC = N.I * NJ, on = M.roff[l + 1];
         for (int c = tid; c < NC; c += NT) {
             const int ci = c / NJ, cj = c - ci * NJ;
@@ -608,7 +609,7 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
         __syncthreads();
     }
     {   // bottom: y = kMgBottomSweeps sweeps of x += dinv (r - A x) from 0
-        const MgLevel& L = M.lv[nb];
+        const auto& L = M.lv[nb];
         const int CL = L.I * L.J, o = M.roff[nb];   // CL <= kMgBottomMax^2 (the hierarchy stops there)
         const bool in = tid < CL;
         const int c = in ? tid : 0;
@@ -640,8 +641,8 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
         __syncthreads();
     }
     for (int l = nb - 1; l >= t; l--) {   // up: x = r dinv + P y_{l+1}, y = x + dinv (r - A x)
-        const MgLevel& L = M.lv[l];
-        const MgLevel& N = M.lv[l + 1];
+        const auto& L = M.lv[l];
+        const auto& N = M.lv[l + 1];
         const int LI = L.I, LJ = L.J, CL = LI * LJ, o = M.roff[l];
         const float* yn = rs + M.roff[l + 1];   // y_{l+1}, in place of r_{l+1}
         for (int c = tid; c < CL; c += NT) {
@@ -679,14 +680,12 @@ __global__ void __launch_bounds__(kBlock) k_mg_prolong(const SceneDev* scenes, c
 {
     const SceneBlock sb = scene_block(map, true);
     const SceneDev& s = scenes[sb.scene];
-    if (!mg_iterating(s)) return;
-    const MgLevels& M = *s.mg;
-    if (l >= M.t) return;   // in k_mg_coarse
-    const MgLevel& L = M.lv[l];
-    const MgLevel& Cl = M.lv[l + 1];
+    const auto& M = *cst(s.mg);
+    const auto& L = M.lv[l];
+    const auto& Cl = M.lv[l + 1];
     const int LJ = L.J;
     const int c = sb.block * kBlock + threadIdx.x;
-    if (c >= L.I * LJ) return;
+    if (!mg_iterating(s) | (l >= M.t) | (c >= L.I * LJ)) return;   // l >= t: in k_mg_coarse
     if (!gbl<const uint8_t>(L.act)[c]) {
         gbl(L.x)[c] = 0.f;
         return;
@@ -702,13 +701,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_post(const SceneDev* scenes, cons
 {
     const SceneBlock sb = scene_block(map, true);
     const SceneDev& s = scenes[sb.scene];
-    if (!mg_iterating(s)) return;
-    const MgLevels& M = *s.mg;
-    if (l >= M.t) return;   // in k_mg_coarse
-    const MgLevel& L = M.lv[l];
+    const auto& M = *cst(s.mg);
+    const auto& L = M.lv[l];
     const int LI = L.I, LJ = L.J, CL = LI * LJ;
     const int c = sb.block * kBlock + threadIdx.x;
-    if (c >= CL) return;
+    if (!mg_iterating(s) | (l >= M.t) | (c >= CL)) return;   // l >= t: in k_mg_coarse
     gptr<const float> x = gbl<const float>(L.x);
     const float dc = gbl<const float>(L.dinv)[c];
     if (dc == 0.f) {   // inactive row: x = 0
@@ -730,10 +727,9 @@ __global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, const
 {
     const SceneBlock sb = scene_block(map, false);
     const SceneDev& s = scenes[sb.scene];
-    if (!mg_iterating(s)) return;
-    if (sb.block >= s.pcg_blocks) return;
+    const auto& L1 = cst(s.mg)->lv[1];
+    if (!mg_iterating(s) | (sb.block >= s.pcg_blocks)) return;
     const int J = s.J;
-    const MgLevel& L1 = s.mg->lv[1];
     gptr<const float> y1 = gbl<const float>(L1.y);
     const int I1 = L1.I, J1 = L1.J;
     gptr<const float> r = gbl<const float>(which ? s.cr2 : s.cr);

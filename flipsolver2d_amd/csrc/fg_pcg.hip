@@ -47,7 +47,7 @@ __device__ __forceinline__ float inv_n(uint8_t code)
 __global__ void __launch_bounds__(kBlock) k_refine_update(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
-    if (!s.active || s.sc->outer_done) return;
+    if (!s.active | (cst(s.sc)->outer_done != 0)) return;
     if ((int)blockIdx.x >= s.pcg_blocks) return;
     const Chunk ch = my_chunk(s, blockIdx.x);
     for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(kBlock) k_refine_update(const SceneDev* scenes
 __global__ void __launch_bounds__(kBlock) k_refine_residual(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
-    if (!s.active || s.sc->outer_done) return;
+    if (!s.active | (cst(s.sc)->outer_done != 0)) return;
     const int nb = s.pcg_blocks;
     if ((int)blockIdx.x >= nb) return;
     double bb[1];
@@ -101,11 +101,9 @@ constexpr int kPass = kItems * kBlock;     // cells per block per pass
 __global__ void __launch_bounds__(kBlock) k_cg_init(const SceneDev* scenes, int sweep)
 {
     const SceneDev& s = scenes[blockIdx.y];
-    if (!s.active) return;
     SceneScalars* sc = s.sc;
-    if (sc->outer_done) return;
     const int nb = s.pcg_blocks;
-    if ((int)blockIdx.x >= nb) return;
+    if (!s.active | (cst(sc)->outer_done != 0) | ((int)blockIdx.x >= nb)) return;
     double rr64[1];
     sum_partials<1>(s, kSlotRR64, nb, rr64);
     const double bn = sqrt(sc->bb), rn = sqrt(rr64[0]);
@@ -167,11 +165,11 @@ __global__ void __launch_bounds__(kBlock) k_pcg_a(const SceneDev* scenes, const 
 {
     const SceneBlock sb = scene_block(map, false);
     const SceneDev& s = scenes[sb.scene];
-    if (!s.active) return;
     SceneScalars* sc = s.sc;
-    if (sc->outer_done || sc->inner_done) return;
+    cptr<SceneScalars> scr = cst(sc);   // values earlier kernels wrote: scalar loads
     const int nb = s.pcg_blocks;
-    if (sb.block >= nb) return;
+    // one test after the uniform loads, so that they issue together
+    if (!s.active | (scr->outer_done != 0) | (scr->inner_done != 0) | (sb.block >= nb)) return;
     const int J = s.J, C = s.I * s.J;
     const bool first = it == 0;
     gptr<const float> z = gbl<const float>(s.cu);
@@ -216,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_a(const SceneDev* scenes, const 
             sum_partials<2>(s, kSlotCG, nb, t);   // rho = r.z, ||r||^2 of the current residual
             const double rho = t[0], rr = t[1];
             const bool lead = sb.block == 0 && threadIdx.x == 0;
-            const double rr0 = first ? rr : sc->rr0;
+            const double rr0 = first ? rr : scr->rr0;
             if (first && lead) sc->rr0 = rr;
             bool stop = false;
             if (!finite_d(rho) || !finite_d(rr)) {
@@ -225,7 +223,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_a(const SceneDev* scenes, const 
                     sc->numeric_bad = 1;
                     sc->outer_done = 1;
                 }
-            } else if (rr == 0.0 || (!first && rr <= sc->inner_eps2 * rr0) || it >= s.max_inner) {
+            } else if (rr == 0.0 || (!first && rr <= scr->inner_eps2 * rr0) || it >= s.max_inner) {
                 stop = true;
             }
             if (stop) {
@@ -236,7 +234,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_a(const SceneDev* scenes, const 
                 return;
             }
             if (lead) sc->gamma[it & 1] = rho;
-            beta = first ? 0.f : (float)(rho / sc->gamma[(it - 1) & 1]);
+            beta = first ? 0.f : (float)(rho / scr->gamma[(it - 1) & 1]);
         }
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
@@ -268,11 +266,11 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const 
 {
     const SceneBlock sb = scene_block(map, false);
     const SceneDev& s = scenes[sb.scene];
-    if (!s.active) return;
     SceneScalars* sc = s.sc;
-    if (sc->outer_done || sc->inner_done) return;
+    cptr<SceneScalars> scr = cst(sc);   // values earlier kernels wrote: scalar loads
     const int nb = s.pcg_blocks;
-    if (sb.block >= nb) return;
+    // one test after the uniform loads, so that they issue together
+    if (!s.active | (scr->outer_done != 0) | (scr->inner_done != 0) | (sb.block >= nb)) return;
     const int J = s.J, C = s.I * s.J;
     gptr<const float> rold = gbl<const float>((it & 1) ? s.cr2 : s.cr);
     gptr<float> rnew = gbl((it & 1) ? s.cr : s.cr2);
@@ -292,7 +290,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const 
             if (lead) sc->inner_done = 1;
             return;
         }
-        const float a = (float)(sc->gamma[it & 1] / pq);
+        const float a = (float)(scr->gamma[it & 1] / pq);
         if (lead) {
             sc->inner_iters = it + 1;
             sc->total_iters += 1;
@@ -347,7 +345,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const 
                 if (lead) sc->inner_done = 1;
                 return;
             }
-            alpha = (float)(sc->gamma[it & 1] / pq);
+            alpha = (float)(scr->gamma[it & 1] / pq);
             if (lead) {
                 sc->inner_iters = it + 1;
                 sc->total_iters += 1;
