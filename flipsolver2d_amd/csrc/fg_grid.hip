@@ -136,22 +136,41 @@ __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
     float uw = 1e-10f, vw = 1e-10f, uv = 0.0f, vv = 0.0f;
     bool uval = false, vval = false;
     const int jl = j > 0 ? j - 1 : 0, jh = j < J - 1 ? j + 1 : J - 1;
+    // the three rows' particle ranges, loaded together (clamped rows; a row
+    // outside the grid gets an empty range)
+    gptr<const int32_t> cs = gbl<const int32_t>(s.cellStart);
+    int ra[3], re[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int ii = i + q - 1, ic = min(max(ii, 0), I - 1);
+        const int a = cs[ic * J + jl], e = cs[ic * J + jh + 1];
+        const bool in = ii >= 0 && ii < I;
+        ra[q] = in ? a : 0;
+        re[q] = in ? e : 0;
+    }
     for (int di = -1; di <= 1; di++) {
         const int ii = i + di;
         if (ii < 0 || ii >= I) continue;
-        const int a = s.cellStart[ii * J + jl], e = s.cellStart[ii * J + jh + 1];
+        const int a = ra[di + 1], e = re[di + 1];
         const int shift = seg_off[ii - wr0] - seg_beg[ii - wr0];
-        // chunks of 8 particles: all loads first, then the sums in order
+        // chunks of 8 particles: all loads first (clamped to the row's last
+        // particle, unconditional, pinned so that none is sunk into the
+        // guarded sums), then the sums in order
         constexpr int U = 8;
+        gptr<const float2> pos = gbl<const float2>(s.pos), velg = gbl<const float2>(s.vel);
         for (int k0 = a; k0 < e; k0 += U) {
             float2 pp[U], vel[U];
+            if (staged) {   // uniform over the block
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int k = k0 + u;
-                if (k >= e) continue;
-                pp[u] = staged ? spos[k + shift] : s.pos[k];
-                vel[u] = s.vel[k];
+                for (int u = 0; u < U; u++) pp[u] = spos[min(k0 + u, e - 1) + shift];
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; u++) pp[u] = pos[min(k0 + u, e - 1)];
             }
+#pragma unroll
+            for (int u = 0; u < U; u++) vel[u] = velg[min(k0 + u, e - 1)];
+#pragma unroll
+            for (int u = 0; u < U; u++) asm volatile("" : "+v"(pp[u].x), "+v"(pp[u].y), "+v"(vel[u].x), "+v"(vel[u].y));
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 if (k0 + u >= e) continue;
@@ -220,25 +239,28 @@ __global__ void __launch_bounds__(kBlock) k_sdf_materials(const SceneDev* scenes
         for (int b = 0; b < 3; b++) d2[a][b] = 3.402823466e+38f;
     // this lane's rows r0+sub, r0+sub+4, r0+sub+8: bounds loaded together
     constexpr int NR = (9 + kSdfLanes - 1) / kSdfLanes;
+    gptr<const int32_t> cs = gbl<const int32_t>(s.cellStart);
+    gptr<const float2> pos = gbl<const float2>(s.pos);
     int rb[NR], re[NR];
 #pragma unroll
-    for (int q = 0; q < NR; q++) {
-        const int r = r0 + sub + q * kSdfLanes;
-        rb[q] = r < r1 ? s.cellStart[r * J + c0] : 0;
-        re[q] = r < r1 ? s.cellStart[r * J + c1] : 0;
+    for (int q = 0; q < NR; q++) {   // clamped rows: every load unconditional
+        const int r = r0 + sub + q * kSdfLanes, rc = min(r, r1 - 1);
+        const int b = cs[rc * J + c0], e = cs[rc * J + c1];
+        rb[q] = r < r1 ? b : 0;
+        re[q] = r < r1 ? e : 0;
     }
 #pragma unroll
     for (int q = 0; q < NR; q++) {
         const int e = re[q];
-        constexpr int U = 8;   // loads in flight per lane
+        constexpr int U = 24;   // loads in flight per lane (a row segment holds ~9 ppc)
         for (int k0 = rb[q]; k0 < e; k0 += U) {
+            // past the row's end a lane re-reads its last particle: the
+            // minimum of a value with itself changes nothing, so no branch
             float2 pp[U];
 #pragma unroll
-            for (int u = 0; u < U; u++)
-                if (k0 + u < e) pp[u] = s.pos[k0 + u];
+            for (int u = 0; u < U; u++) pp[u] = pos[min(k0 + u, e - 1)];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                if (k0 + u >= e) continue;
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
                     const float dxp = pp[u].x - cx[a];
@@ -326,20 +348,31 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
     const int ti = tile / tilesJ, tj = tile - (tile / tilesJ) * tilesJ;
     const int gi0 = ti * kTile - H, gj0 = tj * kTile - H;
     // the region: every load of the thread issued before any is used
+    // (clamped indices, the loads unconditional; each value is pinned by an
+    // empty asm after all of them, so none is sunk into a later branch)
     constexpr int NA = (R + T - 1) / T;
     float xr[NA][NA];
     int lr[NA][NA];
+    bool okr[NA][NA];
 #pragma unroll
     for (int a = 0; a < NA; a++)
 #pragma unroll
         for (int b = 0; b < NA; b++) {
             const int ri = ty + a * T, rj = tx + b * T;
             const int gi = gi0 + ri, gj = gj0 + rj;
-            const bool ok = ri < R && rj < R && gi >= 0 && gi < si && gj >= 0 && gj < sj;
-            const size_t g = ok ? (size_t)gi * sj + gj : 0;
-            xr[a][b] = ok ? gin[g] : 0.f;
-            if (LEVELSET) lr[a][b] = ok ? mkin[g] : -1;
-            else lr[a][b] = ok ? (int)vin[g] : -1;
+            okr[a][b] = ri < R && rj < R && gi >= 0 && gi < si && gj >= 0 && gj < sj;
+            const size_t g = (size_t)min(max(gi, 0), si - 1) * sj + min(max(gj, 0), sj - 1);
+            xr[a][b] = gbl<const float>(gin)[g];
+            if (LEVELSET) lr[a][b] = gbl<const int32_t>(mkin)[g];
+            else lr[a][b] = (int)gbl<const uint8_t>(vin)[g];
+        }
+#pragma unroll
+    for (int a = 0; a < NA; a++)
+#pragma unroll
+        for (int b = 0; b < NA; b++) {
+            asm volatile("" : "+v"(xr[a][b]), "+v"(lr[a][b]));
+            xr[a][b] = okr[a][b] ? xr[a][b] : 0.f;
+            lr[a][b] = okr[a][b] ? lr[a][b] : -1;
         }
     bool has_front = false, has_open = false;
 #pragma unroll
@@ -535,7 +568,7 @@ __global__ void __launch_bounds__(kBlock) k_levelset_band(const SceneDev* scenes
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
-    if (band > 0 && s.lsFlags[band - 1] == 0) return;
+    if (band > 0 && cst(s.lsFlags)[band - 1] == 0) return;   // written by the previous band's launch
     const int tiles = ((s.I + kTile - 1) / kTile) * ((s.J + kTile - 1) / kTile);
     if ((int)blockIdx.x >= tiles) return;
     bfs_band<16, true>(s.fsdf, nullptr, s.marker, s.fsdf, nullptr, s.marker, s.I, s.J, band * 16,
