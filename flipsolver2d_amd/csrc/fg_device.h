@@ -250,6 +250,14 @@ __device__ __forceinline__ gptr<T> gbl(T* p)
     return (gptr<T>)p;
 }
 
+// p[i] of a float2 array as one 64-bit global load (a vector-type object
+// cannot be copied out of the global address space directly)
+__device__ __forceinline__ float2 ldg2(const float2* p, int i)
+{
+    const unsigned long long v = gbl<const unsigned long long>(reinterpret_cast<const unsigned long long*>(p))[i];
+    return make_float2(__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32)));
+}
+
 // Read-only data of a launch (descriptors, the multigrid level table, the
 // scalars earlier kernels wrote) read through the constant address space:
 // uniform addresses become scalar loads, which issue together.

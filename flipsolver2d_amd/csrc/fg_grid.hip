@@ -157,7 +157,6 @@ __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
         // particle, unconditional, pinned so that none is sunk into the
         // guarded sums), then the sums in order
         constexpr int U = 8;
-        gptr<const float2> pos = gbl<const float2>(s.pos), velg = gbl<const float2>(s.vel);
         for (int k0 = a; k0 < e; k0 += U) {
             float2 pp[U], vel[U];
             if (staged) {   // uniform over the block
@@ -165,10 +164,10 @@ __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
                 for (int u = 0; u < U; u++) pp[u] = spos[min(k0 + u, e - 1) + shift];
             } else {
 #pragma unroll
-                for (int u = 0; u < U; u++) pp[u] = pos[min(k0 + u, e - 1)];
+                for (int u = 0; u < U; u++) pp[u] = ldg2(s.pos, min(k0 + u, e - 1));
             }
 #pragma unroll
-            for (int u = 0; u < U; u++) vel[u] = velg[min(k0 + u, e - 1)];
+            for (int u = 0; u < U; u++) vel[u] = ldg2(s.vel, min(k0 + u, e - 1));
 #pragma unroll
             for (int u = 0; u < U; u++) asm volatile("" : "+v"(pp[u].x), "+v"(pp[u].y), "+v"(vel[u].x), "+v"(vel[u].y));
 #pragma unroll
@@ -240,7 +239,6 @@ __global__ void __launch_bounds__(kBlock) k_sdf_materials(const SceneDev* scenes
     // this lane's rows r0+sub, r0+sub+4, r0+sub+8: bounds loaded together
     constexpr int NR = (9 + kSdfLanes - 1) / kSdfLanes;
     gptr<const int32_t> cs = gbl<const int32_t>(s.cellStart);
-    gptr<const float2> pos = gbl<const float2>(s.pos);
     int rb[NR], re[NR];
 #pragma unroll
     for (int q = 0; q < NR; q++) {   // clamped rows: every load unconditional
@@ -258,7 +256,7 @@ __global__ void __launch_bounds__(kBlock) k_sdf_materials(const SceneDev* scenes
             // minimum of a value with itself changes nothing, so no branch
             float2 pp[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) pp[u] = pos[min(k0 + u, e - 1)];
+            for (int u = 0; u < U; u++) pp[u] = ldg2(s.pos, min(k0 + u, e - 1));
 #pragma unroll
             for (int u = 0; u < U; u++) {
 #pragma unroll
