@@ -1068,11 +1068,16 @@ int run_solve(Group& G, bool rhs_from_grid)
         launch_solve_begin(g.pcg, st, G.d);
     }
     // multigrid hierarchy of the matrix snapshot (fixed for the whole solve)
+    // per level: Galerkin product, then its symmetrisation together with
+    // the next level's activity marks
+    if (G.mg_levels > 1) launch_mg_mark(dim3(G.mg_grid[1], (unsigned)G.n), st, G.d, 1);
     for (int l = 1; l < G.mg_levels; l++) {
         const dim3 gl(G.mg_grid[l], (unsigned)G.n);
-        launch_mg_mark(gl, st, G.d, l);
         launch_mg_galerkin(gl, st, G.d, l);
-        launch_mg_sym(gl, st, G.d, l);
+        if (l + 1 < G.mg_levels)
+            launch_mg_symmark(dim3(G.mg_grid[l] + G.mg_grid[l + 1], (unsigned)G.n), st, G.d, l, (int)G.mg_grid[l]);
+        else
+            launch_mg_sym(gl, st, G.d, l);
     }
     // A poll is a host round trip; launches after every scene of the group
     // has stopped run empty.  The first poll of a solve comes after as many

@@ -41,6 +41,7 @@ inline void fg_launch(F kernel, dim3 grid, dim3 block, hipStream_t st, Args... a
 FG_DECLARE_LAUNCH(mg_mark, int l);
 FG_DECLARE_LAUNCH(mg_galerkin, int l);
 FG_DECLARE_LAUNCH(mg_sym, int l);
+FG_DECLARE_LAUNCH(mg_symmark, int l, int nsym);
 FG_DECLARE_LAUNCH(mg_down0, const int2* map, int which);
 FG_DECLARE_LAUNCH(mg_resid, const int2* map, int l);
 FG_DECLARE_LAUNCH(mg_restrict, const int2* map, int l);

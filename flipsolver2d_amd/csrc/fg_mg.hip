@@ -157,14 +157,13 @@ __device__ __forceinline__ void gather25(P x, int LI, int LJ, int ci, int cj, fl
 // set-up, per level l >= 1 in increasing order (one launch per level for all
 // scenes of the group; a thread per level-l cell)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_mg_mark(const SceneDev* scenes, int l)
+__device__ __forceinline__ void mg_mark_block(const SceneDev& s, int l, int bx)
 {
-    const SceneDev& s = scenes[blockIdx.y];
     if (!s.active || s.precond != kPrecondMG) return;
     const auto& M = *cst(s.mg);
     if (l >= M.n) return;
     const auto& L = M.lv[l];
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const int c = bx * kBlock + threadIdx.x;
     if (c >= L.I * L.J) return;
     const int ci = c / L.J, cj = c - ci * L.J;
     const int FI = l == 1 ? s.I : M.lv[l - 1].I, FJ = l == 1 ? s.J : M.lv[l - 1].J;
@@ -180,6 +179,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_mark(const SceneDev* scenes, int 
             if (ok) act |= l == 1 ? (uint8_t)(v >> 7) : v;
         }
     gbl(L.act)[c] = act;
+}
+
+__global__ void __launch_bounds__(kBlock) k_mg_mark(const SceneDev* scenes, int l)
+{
+    mg_mark_block(scenes[blockIdx.y], l, blockIdx.x);
 }
 
 // floor(t / 2) for small t of either sign
@@ -325,16 +329,15 @@ __global__ void __launch_bounds__(kBlock) k_mg_galerkin(const SceneDev* scenes, 
 // Rows equal to the level's canonical interior stencil (every entry within
 // 1e-5 of the canonical diagonal; other rows differ by more than 1 %) are
 // flagged by a negative dinv and read the canonical stencil later.
-__global__ void __launch_bounds__(kBlock) k_mg_sym(const SceneDev* scenes, int l)
+__device__ __forceinline__ void mg_sym_block(const SceneDev& s, int l, int bx)
 {
-    const SceneDev& s = scenes[blockIdx.y];
     if (!s.active || s.precond != kPrecondMG) return;
     const auto& M = *cst(s.mg);
     if (l >= M.n) return;
     const auto& L = M.lv[l];
     const int LI = L.I, LJ = L.J, CL = LI * LJ;
-    if ((int)blockIdx.x * kBlock >= CL) return;   // the whole block is past the level
-    const int c0 = blockIdx.x * kBlock + threadIdx.x;
+    if (bx * kBlock >= CL) return;   // the whole block is past the level
+    const int c0 = bx * kBlock + threadIdx.x;
     const bool valid = c0 < CL;
     const int c = valid ? c0 : CL - 1;   // past the level: recompute the last row, store nothing
     const int ci = c / LJ, cj = c - ci * LJ;
@@ -375,6 +378,21 @@ __global__ void __launch_bounds__(kBlock) k_mg_sym(const SceneDev* scenes, int l
     // byte counts): one atomic per block, once per solve
     const int stored = __syncthreads_count(valid && di > 0.f && !canon_row);
     if (threadIdx.x == 0 && stored) atomicAdd(&s.sc->mg_stored[l], stored);
+}
+
+__global__ void __launch_bounds__(kBlock) k_mg_sym(const SceneDev* scenes, int l)
+{
+    mg_sym_block(scenes[blockIdx.y], l, blockIdx.x);
+}
+
+// k_mg_sym of level l (blocks [0, nsym)) and k_mg_mark of level l+1 (the
+// rest) in one launch: they touch different arrays, and both must be done
+// before the Galerkin product of level l+1.
+__global__ void __launch_bounds__(kBlock) k_mg_symmark(const SceneDev* scenes, int l, int nsym)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if ((int)blockIdx.x < nsym) mg_sym_block(s, l, blockIdx.x);
+    else mg_mark_block(s, l + 1, blockIdx.x - nsym);
 }
 
 // ---------------------------------------------------------------------------
@@ -798,6 +816,10 @@ namespace fg {
 FG_MG_LAUNCH1(mg_mark, int, l)
 FG_MG_LAUNCH1(mg_galerkin, int, l)
 FG_MG_LAUNCH1(mg_sym, int, l)
+void launch_mg_symmark(dim3 g, hipStream_t st, const SceneDev* sc, int l, int nsym)
+{
+    fg_launch(k_mg_symmark, g, dim3(kBlock), st, sc, l, nsym);
+}
 FG_MG_LAUNCH_MAP(mg_down0, int, which)
 FG_MG_LAUNCH_MAP(mg_resid, int, l)
 FG_MG_LAUNCH_MAP(mg_restrict, int, l)
