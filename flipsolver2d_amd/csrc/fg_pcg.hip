@@ -281,24 +281,45 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const 
     gptr<const uint8_t> code = gbl<const uint8_t>(s.stencil);
     const Chunk ch = my_chunk(s, sb.block);
     if (s.precond == kPrecondMG) {
-        // r and d only; the V-cycle that follows forms z, r.z and r.r
-        double t[1];
-        sum_partials<1>(s, kSlotPQ, nb, t);
-        const double pq = t[0];
-        const bool lead = sb.block == 0 && threadIdx.x == 0;
-        if (!(pq > 0.0) || !finite_d(pq)) {
-            if (lead) sc->inner_done = 1;
-            return;
-        }
-        const float a = (float)(scr->gamma[it & 1] / pq);
-        if (lead) {
-            sc->inner_iters = it + 1;
-            sc->total_iters += 1;
-        }
-        for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
-            const float ro = rold[c], qc = q[c], dc = d[c], pc = pnew[c];
-            rnew[c] = ro - a * qc;
-            d[c] = dc + a * pc;
+        // r and d only; the V-cycle that follows forms z, r.z and r.r.  A
+        // pass's loads (kItems cells a thread, clamped to the chunk) are
+        // issued before its stores -- and the first pass's before the p.q
+        // reduction -- since the compiler cannot move a load of r_old above
+        // a store to r_new.
+        float a = 0.f;
+        for (int base = ch.begin; base < ch.end; base += kPass) {
+            float ro[kItems], qc[kItems], dc[kItems], pc[kItems];
+#pragma unroll
+            for (int k = 0; k < kItems; k++) {
+                const int c = min(base + (int)threadIdx.x + k * kBlock, ch.end - 1);
+                ro[k] = rold[c];
+                qc[k] = q[c];
+                dc[k] = d[c];
+                pc[k] = pnew[c];
+            }
+            if (base == ch.begin) {
+                double t[1];
+                sum_partials<1>(s, kSlotPQ, nb, t);
+                const double pq = t[0];
+                const bool lead = sb.block == 0 && threadIdx.x == 0;
+                if (!(pq > 0.0) || !finite_d(pq)) {
+                    if (lead) sc->inner_done = 1;
+                    return;
+                }
+                a = (float)(scr->gamma[it & 1] / pq);
+                if (lead) {
+                    sc->inner_iters = it + 1;
+                    sc->total_iters += 1;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kItems; k++) {
+                const int c = base + (int)threadIdx.x + k * kBlock;
+                if (c < ch.end) {
+                    rnew[c] = ro[k] - a * qc[k];
+                    d[c] = dc[k] + a * pc[k];
+                }
+            }
         }
         return;
     }
