@@ -716,7 +716,11 @@ __global__ void __launch_bounds__(kBlock) k_rhs(const SceneDev* scenes)
             val += (double)is_solid(mat_ext(s.mat, I, J, i, j + 1)) * scale * (double)(v1 - 0);
         }
         s.b[c] = val;
-        s.x64[c] = 0.0;
+        // with the V-cycle the solve starts from the previous substep's
+        // pressure on fluid cells (zero elsewhere; k_cg_init falls back to
+        // zero where that start is no better): the pressure changes little
+        // between substeps, so the first residual is a fraction of ||b||
+        if (!is_fluid(m) || s.precond != kPrecondMG) s.x64[c] = 0.0;
         s.d[c] = 0.f;
         bb += val * val;
     }

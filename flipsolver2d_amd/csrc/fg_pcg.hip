@@ -106,7 +106,20 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const SceneDev* scenes, int 
     if (!s.active | (cst(sc)->outer_done != 0) | ((int)blockIdx.x >= nb)) return;
     double rr64[1];
     sum_partials<1>(s, kSlotRR64, nb, rr64);
-    const double bn = sqrt(sc->bb), rn = sqrt(rr64[0]);
+    const double bb = sc->bb;
+    if (sweep == 0 && s.precond == kPrecondMG && !(rr64[0] <= 25.0 * bb)) {
+        // the warm start (see k_rhs) is far off (residual above 5 ||b||, or
+        // not finite): start from zero, r = b (every block resets its
+        // chunk; all see the same test)
+        const Chunk ch = my_chunk(s, blockIdx.x);
+        for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
+            const double b = s.b[c];
+            s.x64[c] = 0.0;
+            s.cr[c] = (float)((s.stencil[c] & 0x80) ? b / s.sA : b);
+        }
+        rr64[0] = bb;
+    }
+    const double bn = sqrt(bb), rn = sqrt(rr64[0]);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (!finite_d(rr64[0]) || rn <= s.tol * bn || bn == 0.0 || sweep >= kMaxSweeps) {
         if (lead) {
@@ -121,8 +134,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const SceneDev* scenes, int 
         sc->sweeps = sweep + 1;
         sc->inner_done = 0;
         sc->inner_iters = 0;
-        // ask the FP32 solve for the reduction still needed (with margin)
-        const double f = fmax(0.3 * s.tol * bn / rn, 1e-6);
+        // ask the FP32 solve for the reduction still needed (with margin),
+        // at least 1e-6 of ||b|| (relative to this sweep's start: a warm
+        // start's residual may exceed ||b||)
+        const double f = fmax(0.3 * s.tol * bn / rn, 1e-6 * fmin(1.0, bn / rn));
         sc->inner_eps2 = f * f;
     }
     if (s.precond == kPrecondMG) return;   // the V-cycle forms z and the partials
