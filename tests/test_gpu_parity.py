@@ -112,6 +112,36 @@ def test_substep_parity_at_1e6(fg, orc, ctx):
     assert max(e_vx, e_vy) <= 1e-5
 
 
+def test_warm_started_second_substep(fg, orc, ctx):
+    """The second of two substeps starts its pressure solve from the first
+    substep's pressure (the oracle, like the reference, starts from zero);
+    both solves converge to the same solution of the same system."""
+    n, ppc = 64, 4
+    _log("two substeps, the second warm-started")
+    p, st = _prepared(orc, n, ppc, 4)
+    sc = fg.Scene(ctx, fg.make_params(n, ppc=ppc, tol=1e-10))
+    walls, _ = fg.dam_break_geometry(n)
+    sc.set_friction(np.zeros(len(walls), np.float32))
+    sc.set_state(oracle_state_dict(st))
+    for k in range(2):
+        vel = orc.lib().orc_max_particle_velocity(st.ref())
+        dt = np.float32(min(5.0 / (vel + 1e-15), 1.0 / 30))
+        gs = sc.substep(float(dt))
+        orc.lib().orc_substep(C.byref(p), st.ref(), C.c_float(dt), C.byref(orc.OrcStats()))
+        _log(f"  substep {k}: iters={gs.pressure_iters} sweeps={gs.pressure_sweeps} resid={gs.pressure_rel_resid:.2e}")
+        assert gs.pressure_rel_resid <= 1e-10
+    g = sc.get_state()
+    sc.close()
+    N = st.n
+    assert len(g["px"]) == N
+    e_p = rel_l2(g["pressure"], st.pressure)
+    e_vx = rel_l2(g["vx"], st.vx[:N])
+    e_vy = rel_l2(g["vy"], st.vy[:N])
+    _log(f"  rel L2 after two substeps: p={e_p:.2e} vx={e_vx:.2e} vy={e_vy:.2e}")
+    assert e_p <= 1e-5
+    assert max(e_vx, e_vy) <= 1e-5
+
+
 def test_poisson_solve_vs_oracle(fg, orc, ctx):
     _log("poisson solve 192x160")
     rng = np.random.default_rng(7)
