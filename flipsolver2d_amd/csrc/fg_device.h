@@ -27,6 +27,7 @@ constexpr int kPrecondMG = 0, kPrecondIPP = 1;
 constexpr int kMgMaxLevels = 12;
 constexpr int kMgBottomMax = 16;         // the bottom level has at most this many cells per side
 constexpr int kMgBottomSweeps = 8;       // L1-Jacobi sweeps on the bottom level
+constexpr float kMgOmega = 1.6f;         // L1-Jacobi weight: x += w (r - A x) / sum_k |a_k| (SPD for w < 2)
 constexpr int kMgStencil = 25;           // 5x5 Galerkin stencil of the coarse levels
 // k_mg_coarse runs levels t .. n-1 of a scene in one block: t is the first
 // level >= 1 with at most kMgCoarseTop cells whose levels t .. n-1 hold at
@@ -77,7 +78,7 @@ struct MgLevel {
     int32_t I, J;
     uint8_t* act;     // the cell takes part: one of its 2x2 finer cells does
     float* coef;      // [kMgStencil][I*J]: entry k couples the cell to (i + k/5 - 2, j + k%5 - 2)
-    float* dinv;      // 1 / sum_k |coef_k| on active cells, +0 on inactive ones (-0 on an
+    float* dinv;      // kMgOmega / sum_k |coef_k| on active cells, +0 on inactive ones (-0 on an
                       // active cell whose row is all zero, so that a nonzero bit
                       // pattern marks exactly the active cells); negative on rows
                       // equal to the level's canonical interior stencil

@@ -5,7 +5,7 @@
 // count grows linearly with the grid side (about 140 / 280 / 565 iterations
 // to 1e-6 on dam-break systems of 128^2 / 256^2 / 512^2 cells).  This V-cycle
 // is a fixed symmetric positive definite linear operator, so PCG converges
-// to the same solution of the same system; it takes 11 / 13 / 17.
+// to the same solution of the same system; it takes 9 / 11 / 14.
 //
 // Hierarchy (rebuilt for every solve, since the fluid region changes):
 //   level 0      the pressure matrix A' itself: 5-point, integer
@@ -20,10 +20,15 @@
 //   bottom       the first level with at most kMgBottomMax cells per side
 //                (at least level 1).
 // Smoother: one L1-Jacobi sweep before and after the coarse correction,
-// x += (r - A x) / sum_k |a_k|, which converges for every symmetric
-// positive definite A (a damped plain Jacobi sweep does not on all of these
-// Galerkin operators); the bottom level gets kMgBottomSweeps sweeps from
-// zero.  Every step is the same before and after, so the cycle is symmetric.
+// x += w (r - A x) / sum_k |a_k| with w = kMgOmega = 1.6.  The L1 diagonal
+// dominates A, so the eigenvalues of A / sum_k |a_k| lie in (0, 1] and the
+// sweep converges for every symmetric positive definite A and every w < 2
+// (a damped plain Jacobi sweep does not on all of these Galerkin
+// operators); w = 1.6 takes the fewest PCG iterations on dam-break and
+// fragmented systems of 128^2 .. 512^2 cells (1.0: 11 / 13 / 17 on the
+// dam break, 1.6: 9 / 11 / 14; 1.8 and up take more again).  The bottom
+// level gets kMgBottomSweeps sweeps from zero.  Every step is the same
+// before and after, so the cycle is symmetric.
 //
 // Non-fluid rows of A' are identity rows: z = r there, as in the IPP.
 // Level 0 as the V-cycle sees it is A' without the row-wrapping couplings
@@ -372,7 +377,7 @@ __device__ __forceinline__ void mg_sym_block(const SceneDev& s, int l, int bx)
     for (int k = 0; k < 12; k++)
         if (valid) coef[(size_t)k * CL + c] = v[k];
     const bool act = av != 0;
-    const float di = (act && sum > 0.f) ? 1.0f / sum : 0.f;
+    const float di = (act && sum > 0.f) ? kMgOmega / sum : 0.f;
     if (valid) gbl(L.dinv)[c] = (canon_row && di > 0.f) ? -di : (act && di == 0.f ? -0.f : di);
     // active rows that will read their stored coefficients (for the kernels'
     // byte counts): one atomic per block, once per solve
@@ -403,7 +408,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_symmark(const SceneDev* scenes, i
 // issue together; the kernels are latency-bound.
 // ---------------------------------------------------------------------------
 
-// Level 0 -> 1: x0 = r / D1 on the fine level, res0 = r - A' x0, and
+// Level 0 -> 1: x0 = w r / D1 on the fine level, res0 = r - A' x0, and
 // r1 = R res0, one level-1 cell per thread over its 4x4 fine support (a 6x6
 // fine patch with the stencil's reach).
 __global__ void __launch_bounds__(kBlock) k_mg_down0(const SceneDev* scenes, const int2* map, int which)
@@ -443,7 +448,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_down0(const SceneDev* scenes, con
 #pragma unroll
         for (int b = 0; b < 6; b++) {
             const float d = l1_diag0(cd[a][b]);
-            x0[a][b] = ((cd[a][b] & 0x80) && d > 0.f) ? rv[a][b] / d : 0.f;
+            x0[a][b] = ((cd[a][b] & 0x80) && d > 0.f) ? kMgOmega * rv[a][b] / d : 0.f;
         }
     float acc = 0.f;
 #pragma unroll
@@ -738,8 +743,8 @@ __global__ void __launch_bounds__(kBlock) k_mg_post(const SceneDev* scenes, cons
     gbl(L.y)[c] = xv[12] + fabsf(dc) * (rc - ax);
 }
 
-// Level 0, up: z0 = r / D1 + P y1 at the cell and its fluid neighbours, then
-// z = z0 + (r - A' z0) / D1; z = r on non-fluid rows.  Writes z and the
+// Level 0, up: z0 = w r / D1 + P y1 at the cell and its fluid neighbours, then
+// z = z0 + w (r - A' z0) / D1; z = r on non-fluid rows.  Writes z and the
 // partials of r.z and r.r the next PCG iteration reduces (solver chunks).
 __global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, const int2* map, int which)
 {
@@ -788,13 +793,13 @@ __global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, const
 #pragma unroll
         for (int e = 0; e < 5; e++) {
             const float dg = l1_diag0(kg[e]);
-            z0[e] = use[e] ? (dg > 0.f ? rg[e] / dg : 0.f) + pg[e] : 0.f;
+            z0[e] = use[e] ? (dg > 0.f ? kMgOmega * rg[e] / dg : 0.f) + pg[e] : 0.f;
         }
         const float dc = l1_diag0(k);
         float ax = (float)((k >> 4) & 7) * z0[0];
 #pragma unroll
         for (int e = 1; e < 5; e++) ax -= z0[e];
-        const float zc = ((k & 0x80) && dc > 0.f) ? z0[0] + (rc - ax) / dc : rc;
+        const float zc = ((k & 0x80) && dc > 0.f) ? z0[0] + kMgOmega * (rc - ax) / dc : rc;
         z[c] = zc;
         rz += (double)rc * (double)zc;
         rr += (double)rc * (double)rc;
