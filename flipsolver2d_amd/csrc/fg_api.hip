@@ -206,7 +206,6 @@ void carve(fg_scene* s, Carver& cv)
     h.cs1 = cv.take<float>(C);
     h.cu = cv.take<float>(C);
     h.cp = cv.take<float>(C);
-    h.d = cv.take<float>(C);
     h.partials = cv.take<double>(kSlots * nb);
     h.sc = cv.take<SceneScalars>(1);
     // multigrid: halve until the bottom fits kMgBottomMax per side (at
@@ -1089,7 +1088,6 @@ int run_solve(Group& G, bool rhs_from_grid)
     for (fg_scene* s : G.sc) hint = std::max(hint, s->iters_hint[kind]);
     int32_t flags[3];
     for (int sweep = 0; sweep <= kMaxSweeps; sweep++) {
-        if (sweep > 0) launch_refine_update(g.pcg, st, G.d);
         launch_refine_residual(g.pcg, st, G.d);
         launch_cg_init(g.pcg, st, G.d, sweep);
         // the scenes whose solve goes on: several are packed (and re-packed

@@ -155,8 +155,8 @@ struct SceneDev {
     // solver
     double *b, *x64;
     // CG vectors: cr/cr2 residual (double-buffered), cu = z, cp/cs1 direction
-    // (double-buffered), cw = q = A' p, d = correction
-    float *cr, *cr2, *cw, *cs1, *cu, *cp, *d;
+    // (double-buffered), cw = q = A' p; the iterate is x64
+    float *cr, *cr2, *cw, *cs1, *cu, *cp;
     double* partials;       // per-block partial sums [kSlots * max_blocks]
     int32_t* ipart;         // per-block particle counters [2 * max_pblocks]: killed, erased
     float* fpart;           // per-block max |v|^2 [max_pblocks]

@@ -721,7 +721,6 @@ __global__ void __launch_bounds__(kBlock) k_rhs(const SceneDev* scenes)
         // zero where that start is no better): the pressure changes little
         // between substeps, so the first residual is a fraction of ||b||
         if (!is_fluid(m) || s.precond != kPrecondMG) s.x64[c] = 0.0;
-        s.d[c] = 0.f;
         bb += val * val;
     }
     double v[1] = {bb};

@@ -77,7 +77,6 @@ FG_DECLARE_LAUNCH(density_adjust);
 constexpr int kBfsTile = 32;
 
 // fg_pcg.hip
-FG_DECLARE_LAUNCH(refine_update);
 FG_DECLARE_LAUNCH(refine_residual);
 FG_DECLARE_LAUNCH(cg_init, int sweep);
 FG_DECLARE_LAUNCH(pcg_a, const int2* map, int it);

@@ -5,10 +5,15 @@
 //   fluid row c:  (A x)_c = s * (n_c x_c - sum_{fluid nbrs} x_k),  s = dt/(rho dx^2)
 //   other rows:   (A x)_c = x_c
 // solved for the FP64 relative residual ||b - A x||2 / ||b||2 <= tol by
-// iterative refinement: each sweep forms r64 = b - A x in FP64, solves the
-// correction A' d = r' with FP32 PCG (A' = A/s on fluid rows, identity
-// elsewhere, so the stencil has integer coefficients), and adds d into x in
-// FP64.
+// iterative refinement: each sweep forms r64 = b - A x in FP64 and solves
+// the correction A' d = r' with PCG in FP32 (A' = A/s on fluid rows,
+// identity elsewhere, so the stencil has integer coefficients), except the
+// iterate: each step's alpha p goes straight into x in FP64.  An FP32
+// correction d would limit the true residual to about eps32 ||A'|| ||d||,
+// above 1e-6 ||b|| for the smooth corrections of a developed flow, and cost
+// a second sweep whatever the inner target; with x in FP64 only the FP32
+// recurrence's drift (a few eps32 ||r'||) separates the inner residual from
+// the true one.
 //
 // Preconditioner: the inverse-Poisson preconditioner the reference defines
 // in inversepoissonpreconditioner.h:45-79, M = (I - L D^-1)(I - L D^-1)^T,
@@ -41,19 +46,6 @@ __device__ __forceinline__ float inv_n(uint8_t code)
 {
     const int n = (code >> 4) & 7;
     return n == 1 ? 1.0f : n == 2 ? 0.5f : n == 3 ? (1.0f / 3.0f) : n == 4 ? 0.25f : 0.0f;
-}
-
-// x64 += d; d = 0 (start of a refinement sweep)
-__global__ void __launch_bounds__(kBlock) k_refine_update(const SceneDev* scenes)
-{
-    const SceneDev& s = scenes[blockIdx.y];
-    if (!s.active | (cst(s.sc)->outer_done != 0)) return;
-    if ((int)blockIdx.x >= s.pcg_blocks) return;
-    const Chunk ch = my_chunk(s, blockIdx.x);
-    for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
-        s.x64[c] = s.x64[c] + (double)s.d[c];
-        s.d[c] = 0.f;
-    }
 }
 
 // r64 = b - A x64 in FP64 (partials of ||r64||^2); inner rhs r = r64/s on
@@ -275,8 +267,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_a(const SceneDev* scenes, const 
     put_partials<1>(s, kSlotPQ, v, sb.block);
 }
 
-// KB: r = r_old - alpha q; d += alpha p; z = M r; r.z, r.r (with the IPP;
-// with the V-cycle, r and d only)
+// KB: r = r_old - alpha q; x += alpha p (FP64); z = M r; r.z, r.r (with the
+// IPP; with the V-cycle, r and x only)
 __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const int2* map, int it)
 {
     const SceneBlock sb = scene_block(map, false);
@@ -292,24 +284,25 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const 
     gptr<const float> pnew = gbl<const float>((it & 1) ? s.cs1 : s.cp);
     gptr<const float> q = gbl<const float>(s.cw);
     gptr<float> z = gbl(s.cu);
-    gptr<float> d = gbl(s.d);
+    gptr<double> x = gbl(s.x64);
     gptr<const uint8_t> code = gbl<const uint8_t>(s.stencil);
     const Chunk ch = my_chunk(s, sb.block);
     if (s.precond == kPrecondMG) {
-        // r and d only; the V-cycle that follows forms z, r.z and r.r.  A
+        // r and x only; the V-cycle that follows forms z, r.z and r.r.  A
         // pass's loads (kItems cells a thread, clamped to the chunk) are
         // issued before its stores -- and the first pass's before the p.q
         // reduction -- since the compiler cannot move a load of r_old above
         // a store to r_new.
         float a = 0.f;
         for (int base = ch.begin; base < ch.end; base += kPass) {
-            float ro[kItems], qc[kItems], dc[kItems], pc[kItems];
+            float ro[kItems], qc[kItems], pc[kItems];
+            double xc[kItems];
 #pragma unroll
             for (int k = 0; k < kItems; k++) {
                 const int c = min(base + (int)threadIdx.x + k * kBlock, ch.end - 1);
                 ro[k] = rold[c];
                 qc[k] = q[c];
-                dc[k] = d[c];
+                xc[k] = x[c];
                 pc[k] = pnew[c];
             }
             if (base == ch.begin) {
@@ -332,7 +325,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const 
                 const int c = base + (int)threadIdx.x + k * kBlock;
                 if (c < ch.end) {
                     rnew[c] = ro[k] - a * qc[k];
-                    d[c] = dc[k] + a * pc[k];
+                    x[c] = xc[k] + (double)a * (double)pc[k];
                 }
             }
         }
@@ -342,7 +335,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const 
     float alpha = 0.f;
     for (int base = ch.begin; base < ch.end; base += kPass) {
         uint8_t cc[kItems], cl[kItems][2];
-        float rv[kItems][5], qv[kItems][5], pv[kItems], dv[kItems];
+        float rv[kItems][5], qv[kItems][5], pv[kItems];
+        double xv[kItems];
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
             const int c = base + threadIdx.x + k * kBlock;
@@ -358,7 +352,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const 
                 ql[e] = q[idx];
             }
             const uint8_t code_im = code[max(c0 - J, 0)], code_jm = code[max(c0 - 1, 0)];
-            const float pl = pnew[c0], dl = d[c0];
+            const float pl = pnew[c0];
+            const double xl = x[c0];
             cc[k] = code_c;   // lanes past the chunk store nothing
 #pragma unroll
             for (int e = 0; e < 5; e++) {
@@ -370,7 +365,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const 
             cl[k][0] = ((cc[k] & 0x81) == 0x81) ? code_im : 0;
             cl[k][1] = ((cc[k] & 0x84) == 0x84) ? code_jm : 0;
             pv[k] = pl;
-            dv[k] = dl;
+            xv[k] = xl;
         }
         if (base == ch.begin) {
             double t[1];
@@ -414,7 +409,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const 
             }
             rnew[c] = R[0];
             z[c] = zz;
-            d[c] = dv[k] + alpha * pv[k];
+            x[c] = xv[k] + (double)alpha * (double)pv[k];
             rz += (double)R[0] * (double)zz;
             rr += (double)R[0] * (double)R[0];
         }
@@ -423,7 +418,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const 
     put_partials<2>(s, kSlotCG, v, sb.block);
 }
 
-// ||b||^2 partials for an externally supplied right-hand side; x = 0, d = 0.
+// ||b||^2 partials for an externally supplied right-hand side; x = 0.
 __global__ void __launch_bounds__(kBlock) k_solve_begin(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
@@ -435,7 +430,6 @@ __global__ void __launch_bounds__(kBlock) k_solve_begin(const SceneDev* scenes)
         const double b = s.b[c];
         bb += b * b;
         s.x64[c] = 0.0;
-        s.d[c] = 0.f;
     }
     double v[1] = {bb};
     put_partials<1>(s, kSlotBB, v, blockIdx.x);
@@ -463,7 +457,6 @@ namespace fg {
     void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc) { fg_launch(k_##name, g, dim3(kBlock), st, sc); }
 #define FG_LAUNCH1(name, T, a) \
     void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc, T a) { fg_launch(k_##name, g, dim3(kBlock), st, sc, a); }
-FG_LAUNCH0(refine_update)
 FG_LAUNCH0(refine_residual)
 FG_LAUNCH1(cg_init, int, sweep)
 void launch_pcg_a(dim3 g, hipStream_t st, const SceneDev* sc, const int2* map, int it)
