@@ -48,12 +48,12 @@ HBM_PEAK_GBS = 8000.0
 # level-1 cell r, |dinv| and y (12 B) plus 100 B per stored row.
 # k_pcg_a reads the stencil code (1), z (4) and the old direction (4) and
 # writes p (4) and q = A' p (4): 17 B per cell; k_pcg_b (with the V-cycle,
-# the axpy pair r -= alpha q, d += alpha p) 24 B per cell.
+# the pair r -= alpha q in FP32, x += alpha p in FP64) 32 B per cell.
 KERNEL_BYTES = {
     "mg_coarse": ("K_MG_COARSE", 1.0, "bytes"),
     "mg_resid": ("K_MG_RESID", 1.0, "bytes"),
     "pcg_a": ("K_PCG_A", 17.0, "cells"),
-    "pcg_b": ("K_PCG_B", 24.0, "cells"),
+    "pcg_b": ("K_PCG_B", 32.0, "cells"),
 }
 # HBM bytes actually moved per work unit, per workload: rocprofv3 FETCH_SIZE
 # (x2.000) + WRITE_SIZE (x1.000) per dispatch, the factors being what the same

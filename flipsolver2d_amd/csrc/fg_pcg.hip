@@ -250,17 +250,22 @@ __global__ void __launch_bounds__(kBlock) k_pcg_a(const SceneDev* scenes, const 
             float P[5];
 #pragma unroll
             for (int e = 0; e < 5; e++) P[e] = first ? zc[k][e] : zc[k][e] + beta * pc[k][e];
-            float qv = P[0];
+            // q = A' p summed in FP64 (integer weights times FP32 values:
+            // exact but for the rare exponent spread), rounded once: for a
+            // smooth p, n p_c - sum p_k cancels to far below |p|, and an
+            // FP32 sum's error (about eps32 * 8 |p|) would enter the
+            // residual recurrence and separate it from the true residual
+            double qd = (double)P[0];
             if (cc[k] & 0x80) {
-                qv = (float)((cc[k] >> 4) & 7) * P[0];
-                if (cc[k] & 1) qv -= P[1];
-                if (cc[k] & 2) qv -= P[2];
-                if (cc[k] & 4) qv -= P[3];
-                if (cc[k] & 8) qv -= P[4];
+                qd = (double)((cc[k] >> 4) & 7) * (double)P[0];
+                if (cc[k] & 1) qd -= (double)P[1];
+                if (cc[k] & 2) qd -= (double)P[2];
+                if (cc[k] & 4) qd -= (double)P[3];
+                if (cc[k] & 8) qd -= (double)P[4];
             }
             pnew[c] = P[0];
-            q[c] = qv;
-            pq += (double)P[0] * (double)qv;
+            q[c] = (float)qd;
+            pq += (double)P[0] * qd;
         }
     }
     double v[1] = {pq};
