@@ -2041,12 +2041,20 @@ int fg_batch_autotune(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, int32_t r
             ms[lanes] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             for (const fg_stats& s : st) work[lanes] += (double)s.particle_substeps;
         }
-    int best = 1;
+    // each count is measured on other frames of the advancing simulation,
+    // so rates within kTie of the fastest count as equal, and the most lanes
+    // among them win (more lanes keep more independent launches in flight)
+    constexpr double kTie = 0.97;
+    std::vector<double> rate(FG_MAX_LANES + 1, 0.0);
+    double top = 0.0;
     for (int l = 1; l <= FG_MAX_LANES; l++) {
-        const double rate = ms[l] > 0 ? work[l] / (ms[l] * 1e-3) : 0.0;
-        if (rates_out) rates_out[l - 1] = rate;
-        if (rate > work[best] / (ms[best] * 1e-3)) best = l;
+        rate[l] = ms[l] > 0 ? work[l] / (ms[l] * 1e-3) : 0.0;
+        if (rates_out) rates_out[l - 1] = rate[l];
+        top = std::max(top, rate[l]);
     }
+    int best = 1;
+    for (int l = 1; l <= FG_MAX_LANES; l++)
+        if (rate[l] >= kTie * top) best = l;
     ctx->nlanes = best;
     if (lanes_out) *lanes_out = best;
     return FG_OK;

@@ -172,7 +172,8 @@ int fg_ctx_preconditioner(fg_ctx* ctx, int32_t* kind);
 /* Autotuner for that schedule: steps `rounds` frames of the batch with each
  * lane count 1..FG_MAX_LANES (order alternating per round; the simulation
  * advances normally), measures particle-substeps per second for each, and
- * sets the fastest on the context.  *lanes_out gets it; rates_out (may be
+ * sets the fastest on the context (the most lanes among those within 3 % of
+ * the fastest rate: each count runs on different frames).  *lanes_out gets it; rates_out (may be
  * NULL) gets FG_MAX_LANES measured rates. */
 int fg_batch_autotune(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, int32_t rounds, int32_t* lanes_out,
                       double* rates_out);

@@ -101,6 +101,9 @@ def test_autotune_keeps_results(fg, ctx):
         lanes, rates = fg.batch_autotune(ctx, batch, rounds=1)
         assert 1 <= lanes <= fg.MAX_LANES and ctx.batch_lanes() == lanes
         assert np.all(rates > 0)
+        # the most lanes among the rates within 3 % of the fastest
+        assert rates[lanes - 1] >= 0.97 * rates.max()
+        assert all(rates[l - 1] < 0.97 * rates.max() for l in range(lanes + 1, fg.MAX_LANES + 1))
         for _ in range(fg.MAX_LANES):
             for sc in solo:
                 sc.step_frame()
