@@ -532,18 +532,21 @@ __global__ void k_group_flags(const SceneDev* scenes, int n, int band, int32_t* 
 }
 
 // The descriptors of the scenes whose inner PCG still iterates, packed in
-// slot order into `out` (one wave); info = {count, max solver blocks}.
+// slot order into `out` (one wave); info = {count, max solver blocks,
+// scenes whose FP64 test has not passed}.
 __global__ void k_pack_iterating(const SceneDev* in, int n, SceneDev* out, int32_t* info)
 {
     const int lane = threadIdx.x;
-    int base = 0, maxblk = 0;
+    int base = 0, maxblk = 0, open = 0;
     for (int b0 = 0; b0 < n; b0 += 64) {
         const int k = b0 + lane;
-        bool it = false;
+        bool it = false, op = false;
         if (k < n) {
             const SceneDev& s = in[k];
-            it = s.active && !s.sc->outer_done && !s.sc->inner_done;
+            op = s.active && !s.sc->outer_done;
+            it = op && !s.sc->inner_done;
         }
+        open += __popcll(__ballot(op));
         const unsigned long long mask = __ballot(it);
         const int pos = __popcll(mask & ((1ull << lane) - 1ull));
         if (it) {
@@ -556,6 +559,7 @@ __global__ void k_pack_iterating(const SceneDev* in, int n, SceneDev* out, int32
     if (lane == 0) {
         info[0] = base;
         info[1] = maxblk;
+        info[2] = open;
     }
 }
 
@@ -819,30 +823,46 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
     return FG_OK;
 }
 
-// Read every scene's scalar block into its pinned host mirror (synchronises).
-int read_group_scalars(Group& G)
+// Queue the copy of every scene's scalar block to the host; the caller
+// synchronises and then calls scalars_arrived.
+int queue_group_scalars(Group& G)
 {
     Lane* L = G.L;
     if (G.n == 1) {
         fg_scene* s = G.sc[0];
         HIPCHK(hipMemcpyAsync(s->hsc, s->h.sc, sizeof(SceneScalars), hipMemcpyDeviceToHost, L->stream));
-        HIPCHK(hipStreamSynchronize(L->stream));
         return FG_OK;
     }
     hipLaunchKernelGGL(k_gather_scalars, dim3((unsigned)cdiv(G.n, 64)), dim3(64), 0, L->stream, G.d, G.n, L->bsc);
     HIPCHK(hipMemcpyAsync(L->bsc_host, L->bsc, sizeof(SceneScalars) * G.n, hipMemcpyDeviceToHost, L->stream));
-    HIPCHK(hipStreamSynchronize(L->stream));
-    for (int k = 0; k < G.n; k++) *G.sc[k]->hsc = L->bsc_host[k];
+    return FG_OK;
+}
+void scalars_arrived(Group& G)
+{
+    if (G.n > 1)
+        for (int k = 0; k < G.n; k++) *G.sc[k]->hsc = G.L->bsc_host[k];
+}
+
+// Read every scene's scalar block into its pinned host mirror (synchronises).
+int read_group_scalars(Group& G)
+{
+    FGCHK(queue_group_scalars(G));
+    HIPCHK(hipStreamSynchronize(G.L->stream));
+    scalars_arrived(G);
     return FG_OK;
 }
 
-int poll_group(Group& G, int band, int32_t* flags)
+// with_scalars: the scenes' scalar blocks come back with the flags (one
+// synchronisation for both)
+int poll_group(Group& G, int band, int32_t* flags, bool with_scalars = false)
 {
     Lane* L = G.L;
     hipLaunchKernelGGL(k_group_flags, dim3(1), dim3(256), 0, L->stream, G.d, G.n, band, L->bflags);
     HIPCHK(hipMemcpyAsync(L->bflags_host, L->bflags, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, L->stream));
+    if (with_scalars) FGCHK(queue_group_scalars(G));
     HIPCHK(hipStreamSynchronize(L->stream));
     for (int k = 0; k < 3; k++) flags[k] = L->bflags_host[k];
+    if (with_scalars) scalars_arrived(G);
     return FG_OK;
 }
 
@@ -870,8 +890,10 @@ SolveLaunch group_launch(const Group& G, const SceneDev* d, unsigned ny)
 }
 
 // Pack the scenes whose inner PCG still iterates and build their block maps;
-// returns their count and the launch description (synchronises).
-int pack_iterating(Group& G, SolveLaunch* S, int* count)
+// returns their count, the count of scenes whose FP64 test has not passed,
+// and the launch description; the scalar blocks come back with them
+// (synchronises once).
+int pack_iterating(Group& G, SolveLaunch* S, int* count, int* open)
 {
     Lane* L = G.L;
     hipLaunchKernelGGL(k_pack_iterating, dim3(1), dim3(64), 0, L->stream, G.d, G.n, L->bpack, L->bflags);
@@ -882,8 +904,11 @@ int pack_iterating(Group& G, SolveLaunch* S, int* count)
         hipLaunchKernelGGL(k_build_maps, dim3(kinds), dim3(1024), 0, L->stream, L->bpack, L->bflags, L->bmap, offs, -1);
     }
     HIPCHK(hipMemcpyAsync(L->bflags_host, L->bflags, sizeof(int32_t) * (4 + kinds), hipMemcpyDeviceToHost, L->stream));
+    FGCHK(queue_group_scalars(G));
     HIPCHK(hipStreamSynchronize(L->stream));
+    scalars_arrived(G);
     *count = L->bflags_host[0];
+    *open = L->bflags_host[2];
     if (kinds > 0) {
         *S = SolveLaunch{};
         S->d = L->bpack;
@@ -1086,23 +1111,31 @@ int run_solve(Group& G, bool rhs_from_grid)
     const int kind = rhs_from_grid ? 0 : 1;
     int hint = 0;
     for (fg_scene* s : G.sc) hint = std::max(hint, s->iters_hint[kind]);
+    // Polls bring the scalar blocks back with their flags.  After each chunk
+    // of iterations the FP64 test runs on the scenes whose inner solve has
+    // stopped (spec passes), so the poll that sees the last one stop also
+    // sees whether every scene met its tolerance, and the solve ends there.
     int32_t flags[3];
+    bool scalars_fresh = false;   // the last poll read the scalars, nothing ran since
     for (int sweep = 0; sweep <= kMaxSweeps; sweep++) {
-        launch_refine_residual(g.pcg, st, G.d);
-        launch_cg_init(g.pcg, st, G.d, sweep);
+        launch_refine_residual(g.pcg, st, G.d, 0);
+        launch_cg_init(g.pcg, st, G.d, sweep, 0);
+        scalars_fresh = false;
         // the scenes whose solve goes on: several are packed (and re-packed
         // at every poll as they stop); one keeps its own descriptor
         SolveLaunch S;
-        int count = 0;
+        int count = 0, open = 0;
         if (G.n > 1 && sweep == 0 && G.map_kinds > 0) {
             // every scene takes part (one whose FP64 test already passed
             // exits each kernel at once); the first poll repacks
             all_scene_launch(G, &S);
             count = G.n;
         } else if (G.n > 1) {
-            FGCHK(pack_iterating(G, &S, &count));
+            FGCHK(pack_iterating(G, &S, &count, &open));
+            scalars_fresh = true;
         } else {
-            FGCHK(poll_group(G, -1, flags));
+            FGCHK(poll_group(G, -1, flags, true));
+            scalars_fresh = true;
             count = flags[1];
             S = group_launch(G, G.d, 1);
         }
@@ -1133,17 +1166,22 @@ int run_solve(Group& G, bool rhs_from_grid)
                     kp.done(0);
                 }
             }
+            launch_refine_residual(g.pcg, st, G.d, 1);
+            launch_cg_init(g.pcg, st, G.d, sweep + 1, 1);
             if (G.n > 1) {
-                FGCHK(pack_iterating(G, &S, &count));
+                FGCHK(pack_iterating(G, &S, &count, &open));
             } else {
-                FGCHK(poll_group(G, -1, flags));
+                FGCHK(poll_group(G, -1, flags, true));
                 count = flags[1];
+                open = flags[0];
             }
+            scalars_fresh = true;
             if (count > 0 && it > G.max_inner + kPollMax) return fail(FG_E_STATE, "inner solve did not stop at its cap");
             chunk = kPollNext;
         }
+        if (open == 0) break;   // every scene passed its FP64 test
     }
-    FGCHK(read_group_scalars(G));
+    if (!scalars_fresh) FGCHK(read_group_scalars(G));
     for (fg_scene* s : G.sc) {
         const SceneScalars& h = *s->hsc;
         s->iters_hint[kind] = h.sweeps > 1 ? h.total_iters - h.inner_iters : h.total_iters;

@@ -77,8 +77,8 @@ FG_DECLARE_LAUNCH(density_adjust);
 constexpr int kBfsTile = 32;
 
 // fg_pcg.hip
-FG_DECLARE_LAUNCH(refine_residual);
-FG_DECLARE_LAUNCH(cg_init, int sweep);
+FG_DECLARE_LAUNCH(refine_residual, int spec);
+FG_DECLARE_LAUNCH(cg_init, int sweep, int spec);
 FG_DECLARE_LAUNCH(pcg_a, const int2* map, int it);
 FG_DECLARE_LAUNCH(pcg_b, const int2* map, int it);
 FG_DECLARE_LAUNCH(solve_begin);

@@ -49,13 +49,16 @@ __device__ __forceinline__ float inv_n(uint8_t code)
 }
 
 // r64 = b - A x64 in FP64 (partials of ||r64||^2); inner rhs r = r64/s on
-// fluid rows.  Also folds the ||b||^2 partials into sc->bb.
-__global__ void __launch_bounds__(kBlock) k_refine_residual(const SceneDev* scenes)
+// fluid rows.  Also folds the ||b||^2 partials into sc->bb.  spec: only the
+// scenes whose inner PCG has stopped (the FP64 test ahead of a poll, while
+// other scenes may still iterate; a stopped sweep's r is dead).
+__global__ void __launch_bounds__(kBlock) k_refine_residual(const SceneDev* scenes, int spec)
 {
     const SceneDev& s = scenes[blockIdx.y];
-    if (!s.active | (cst(s.sc)->outer_done != 0)) return;
+    cptr<SceneScalars> scr = cst(s.sc);
+    const int od = scr->outer_done, idn = scr->inner_done;
     const int nb = s.pcg_blocks;
-    if ((int)blockIdx.x >= nb) return;
+    if (!s.active | (od != 0) | ((spec != 0) & (idn == 0)) | ((int)blockIdx.x >= nb)) return;
     double bb[1];
     sum_partials<1>(s, kSlotBB, nb, bb);
     if (blockIdx.x == 0 && threadIdx.x == 0) s.sc->bb = bb[0];
@@ -89,16 +92,29 @@ constexpr int kItems = 4;                  // cells per thread per pass
 constexpr int kPass = kItems * kBlock;     // cells per block per pass
 
 // Outer test on ||r64||; if not converged: z = M r and the partials of r.z,
-// r.r for the first iteration.
-__global__ void __launch_bounds__(kBlock) k_cg_init(const SceneDev* scenes, int sweep)
+// r.r for the first iteration.  spec: the test alone, on the scenes whose
+// inner PCG has stopped (after k_refine_residual's spec pass): a scene that
+// passes is done, one that does not waits for the next sweep's k_cg_init.
+__global__ void __launch_bounds__(kBlock) k_cg_init(const SceneDev* scenes, int sweep, int spec)
 {
     const SceneDev& s = scenes[blockIdx.y];
     SceneScalars* sc = s.sc;
+    cptr<SceneScalars> scr = cst(sc);
+    const int od = scr->outer_done, idn = scr->inner_done;
     const int nb = s.pcg_blocks;
-    if (!s.active | (cst(sc)->outer_done != 0) | ((int)blockIdx.x >= nb)) return;
+    if (!s.active | (od != 0) | ((spec != 0) & (idn == 0)) | ((int)blockIdx.x >= nb)) return;
     double rr64[1];
     sum_partials<1>(s, kSlotRR64, nb, rr64);
     const double bb = sc->bb;
+    if (spec) {
+        const double bn = sqrt(bb), rn = sqrt(rr64[0]);
+        if (blockIdx.x == 0 && threadIdx.x == 0 && (!finite_d(rr64[0]) || rn <= s.tol * bn || bn == 0.0)) {
+            sc->rr64 = rr64[0];
+            if (!finite_d(rr64[0])) sc->numeric_bad = 1;
+            sc->outer_done = 1;
+        }
+        return;
+    }
     if (sweep == 0 && s.precond == kPrecondMG && !(rr64[0] <= 25.0 * bb)) {
         // the warm start (see k_rhs) is far off (residual above 5 ||b||, or
         // not finite): start from zero, r = b (every block resets its
@@ -462,8 +478,11 @@ namespace fg {
     void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc) { fg_launch(k_##name, g, dim3(kBlock), st, sc); }
 #define FG_LAUNCH1(name, T, a) \
     void launch_##name(dim3 g, hipStream_t st, const SceneDev* sc, T a) { fg_launch(k_##name, g, dim3(kBlock), st, sc, a); }
-FG_LAUNCH0(refine_residual)
-FG_LAUNCH1(cg_init, int, sweep)
+FG_LAUNCH1(refine_residual, int, spec)
+void launch_cg_init(dim3 g, hipStream_t st, const SceneDev* sc, int sweep, int spec)
+{
+    fg_launch(k_cg_init, g, dim3(kBlock), st, sc, sweep, spec);
+}
 void launch_pcg_a(dim3 g, hipStream_t st, const SceneDev* sc, const int2* map, int it)
 {
     fg_launch(k_pcg_a, g, dim3(kBlock), st, sc, map, it);
