@@ -309,6 +309,10 @@ __device__ __forceinline__ void galerkin_row(const SceneDev& s, const MT& M, int
     for (int k = 0; k < kMgStencil; k++) L.coef[(size_t)k * CL + c] = (float)(0.25 * acc[k]);
 }
 
+// An inactive row must hold zeros; its dinv, not yet rewritten for this
+// hierarchy, says whether it held any: +0 (bit pattern 0) only on a row that
+// was inactive before (or never set: the arena starts zeroed), whose
+// coefficients are zero already -- most of the level, and most of its writes.
 __global__ void __launch_bounds__(kBlock) k_mg_galerkin(const SceneDev* scenes, int l)
 {
     const SceneDev& s = scenes[blockIdx.y];
@@ -320,8 +324,10 @@ __global__ void __launch_bounds__(kBlock) k_mg_galerkin(const SceneDev* scenes, 
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= CL) return;
     if (!L.act[c]) {
+        if (__float_as_uint(gbl<const float>(L.dinv)[c]) != 0u) {
 #pragma unroll
-        for (int k = 0; k < kMgStencil; k++) L.coef[(size_t)k * CL + c] = 0.f;
+            for (int k = 0; k < kMgStencil; k++) L.coef[(size_t)k * CL + c] = 0.f;
+        }
         return;
     }
     const int ci = c / L.J, cj = c - ci * L.J;
@@ -373,10 +379,11 @@ __device__ __forceinline__ void mg_sym_block(const SceneDev& s, int l, int bx)
         sum += fabsf(v[k]);
         canon_row = canon_row && fabsf(v[k] - cn[k]) <= tol;
     }
+    const bool act = av != 0;
+    // an inactive row is all zero already (k_mg_galerkin), mirrored entries included
 #pragma unroll
     for (int k = 0; k < 12; k++)
-        if (valid) coef[(size_t)k * CL + c] = v[k];
-    const bool act = av != 0;
+        if (valid && act) coef[(size_t)k * CL + c] = v[k];
     const float di = (act && sum > 0.f) ? kMgOmega / sum : 0.f;
     if (valid) gbl(L.dinv)[c] = (canon_row && di > 0.f) ? -di : (act && di == 0.f ? -0.f : di);
     // active rows that will read their stored coefficients (for the kernels'
