@@ -67,7 +67,13 @@ __global__ void __launch_bounds__(kBlock) k_stencil(const SceneDev* scenes)
 // window cell at 8 ppc; a fuller window reads positions from global memory.
 // ---------------------------------------------------------------------------
 constexpr int kP2gTile = 16;
-constexpr int kP2gCap = (kP2gTile + 2) * (kP2gTile + 2) * 16;   // 5184 positions (41.5 KiB)
+// A tile's window (18 x 18 cells) is staged in LDS, positions and
+// velocities, when it holds at most 8 particles a cell on average (41.5 KiB,
+// three blocks per CU); a denser window reads them through the cache.
+// Staging both at up to 16 a cell (83 KiB, one block per CU) was 5 % slower
+// on the 256 mixed scenes, at 10 or 12 no better than at 8.
+constexpr int kP2gPerCell = 8;
+constexpr int kP2gCap = (kP2gTile + 2) * (kP2gTile + 2) * kP2gPerCell;
 
 __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
 {
@@ -83,7 +89,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
     const int wr0 = i0 > 0 ? i0 - 1 : 0, wr1 = min(i0 + kP2gTile, I - 1);
     const int wc0 = j0 > 0 ? j0 - 1 : 0, wc1 = min(j0 + kP2gTile, J - 1);
     const int nrows = wr1 - wr0 + 1;   // <= 18
-    __shared__ float2 spos[kP2gCap];
+    __shared__ float2 spos[kP2gCap], svel[kP2gCap];
     __shared__ int seg_beg[kP2gTile + 2], seg_off[kP2gTile + 3];
     if (threadIdx.x < 64) {
         const int r = threadIdx.x;
@@ -111,7 +117,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
     if (staged) {
         constexpr int U = 4;   // loads in flight per thread
         for (int t0 = threadIdx.x; t0 < total; t0 += U * kBlock) {
-            float2 v[U];
+            float2 v[U], w[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int t = t0 + u * kBlock;
@@ -123,10 +129,14 @@ __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
                     else hi = mid - 1;
                 }
                 v[u] = s.pos[seg_beg[lo] + (t - seg_off[lo])];
+                w[u] = s.vel[seg_beg[lo] + (t - seg_off[lo])];
             }
 #pragma unroll
             for (int u = 0; u < U; u++)
-                if (t0 + u * kBlock < total) spos[t0 + u * kBlock] = v[u];
+                if (t0 + u * kBlock < total) {
+                    spos[t0 + u * kBlock] = v[u];
+                    svel[t0 + u * kBlock] = w[u];
+                }
         }
     }
     __syncthreads();
@@ -161,13 +171,17 @@ __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
             float2 pp[U], vel[U];
             if (staged) {   // uniform over the block
 #pragma unroll
-                for (int u = 0; u < U; u++) pp[u] = spos[min(k0 + u, e - 1) + shift];
+                for (int u = 0; u < U; u++) {
+                    pp[u] = spos[min(k0 + u, e - 1) + shift];
+                    vel[u] = svel[min(k0 + u, e - 1) + shift];
+                }
             } else {
 #pragma unroll
-                for (int u = 0; u < U; u++) pp[u] = ldg2(s.pos, min(k0 + u, e - 1));
+                for (int u = 0; u < U; u++) {
+                    pp[u] = ldg2(s.pos, min(k0 + u, e - 1));
+                    vel[u] = ldg2(s.vel, min(k0 + u, e - 1));
+                }
             }
-#pragma unroll
-            for (int u = 0; u < U; u++) vel[u] = ldg2(s.vel, min(k0 + u, e - 1));
 #pragma unroll
             for (int u = 0; u < U; u++) asm volatile("" : "+v"(pp[u].x), "+v"(pp[u].y), "+v"(vel[u].x), "+v"(vel[u].y));
 #pragma unroll
