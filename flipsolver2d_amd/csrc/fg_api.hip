@@ -121,6 +121,7 @@ struct fg_scene {
     SceneDev* hpin = nullptr;     // pinned staging for descriptor uploads
     SceneScalars* hsc = nullptr;  // pinned readback of the scalars
     int iters_hint[2] = {0, 0};   // first-sweep PCG iterations of the last solve (pressure, density)
+    int ls_hint = 0;              // level-set bands the last substep ran
     int64_t n_bound = 0;          // upper bound of live particles (host mirror)
     int64_t n_live = 0;           // live particles at the last readback
     bool vmax_ready = false;
@@ -1280,10 +1281,16 @@ int run_stage(Group& G, int stage)
     case FG_STAGE_LEVELSET: {
         KProf kp(G, FG_K_LEVELSET);
         launch_levelset_init(g.cells, st, d);
+        // a band launch after the last one needed exits at once but still
+        // dispatches its blocks: the first poll comes after one band more
+        // than the group's scenes ran last substep, later ones every kChunk
         constexpr int kChunk = 8;
+        int hint = 0;
+        for (fg_scene* s : G.sc) hint = std::max(hint, s->ls_hint);
+        int chunk = hint > 0 ? hint + 1 : kChunk;
         int32_t flags[3];
-        for (int band = 0; band < G.max_bands; band += kChunk) {
-            const int end = std::min(G.max_bands, band + kChunk);
+        for (int band = 0; band < G.max_bands; band += chunk, chunk = kChunk) {
+            const int end = std::min(G.max_bands, band + chunk);
             for (int k = band; k < end; k++) launch_levelset_band(g.ls, st, d, k);
             FGCHK(poll_group(G, end - 1, flags));
             if (!flags[2]) break;
@@ -1349,6 +1356,7 @@ void commit_substep(Group& G, float* dts)
         fg_scene* s = G.sc[k];
         const SceneScalars& sc = *s->hsc;
         const int64_t n_start = s->n_live;
+        s->ls_hint = sc.ls_bands;
         s->n_live = sc.n_particles - sc.erased;
         s->n_bound = sc.n_particles;
         s->vmax_ready = true;

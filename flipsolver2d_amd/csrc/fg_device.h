@@ -67,7 +67,7 @@ struct SceneScalars {
     int32_t total_iters;    // iterations of the current solve
     int32_t sweeps;
     int32_t numeric_bad;    // NaN/Inf seen in a reduction
-    int32_t ls_more;        // level-set BFS reached the end of a launch's band
+    int32_t ls_bands;       // level-set BFS bands that ran this substep
     int32_t density_applied;  // density correction moved the particles this substep
     int32_t mg_stored[kMgMaxLevels];  // active rows per level that read stored coefficients
 };

@@ -575,7 +575,8 @@ __global__ void __launch_bounds__(kBlock) k_levelset_init(const SceneDev* scenes
 }
 
 // One band of extrapolateLevelsetInside: levels L0+1 .. L0+16.  Runs only if
-// the previous band reached its last level somewhere (sc->ls_more).
+// the previous band reached its last level somewhere (lsFlags[band - 1]);
+// records the bands that ran in sc->ls_bands.
 __global__ void __launch_bounds__(kBlock) k_levelset_band(const SceneDev* scenes, int band)
 {
     const SceneDev& s = scenes[blockIdx.y];
@@ -583,6 +584,7 @@ __global__ void __launch_bounds__(kBlock) k_levelset_band(const SceneDev* scenes
     if (band > 0 && cst(s.lsFlags)[band - 1] == 0) return;   // written by the previous band's launch
     const int tiles = ((s.I + kTile - 1) / kTile) * ((s.J + kTile - 1) / kTile);
     if ((int)blockIdx.x >= tiles) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) s.sc->ls_bands = band + 1;
     bfs_band<16, true>(s.fsdf, nullptr, s.marker, s.fsdf, nullptr, s.marker, s.I, s.J, band * 16,
                        &s.lsFlags[band]);
 }
