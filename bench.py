@@ -58,7 +58,7 @@ KERNEL_BYTES = {
 # HBM bytes actually moved per work unit, per workload: rocprofv3 FETCH_SIZE
 # (x2.000) + WRITE_SIZE (x1.000) per dispatch, the factors being what the same
 # counters read for a 1 GiB copy kernel.
-MEASURED_TRAFFIC_PER_UNIT = {"mixed256": {"mg_coarse": 1.772}, "dam1024": {"mg_coarse": 1.951}}
+MEASURED_TRAFFIC_PER_UNIT = {"mixed256": {"mg_coarse": 1.773}, "dam1024": {"mg_coarse": 1.894}}
 
 
 def parse():
