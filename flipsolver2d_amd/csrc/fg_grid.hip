@@ -362,6 +362,9 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
     // the region: every load of the thread issued before any is used
     // (clamped indices, the loads unconditional; each value is pinned by an
     // empty asm after all of them, so none is sunk into a later branch)
+    // Extrapolation (!LEVELSET) loads the halo's values only if the tile has
+    // work: a tile with no unknown cell, or no known one within reach, is
+    // copied, and needs the region's validity bytes and its own values alone.
     constexpr int NA = (R + T - 1) / T;
     float xr[NA][NA];
     int lr[NA][NA];
@@ -374,7 +377,8 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
             const int gi = gi0 + ri, gj = gj0 + rj;
             okr[a][b] = ri < R && rj < R && gi >= 0 && gi < si && gj >= 0 && gj < sj;
             const size_t g = (size_t)min(max(gi, 0), si - 1) * sj + min(max(gj, 0), sj - 1);
-            xr[a][b] = gbl<const float>(gin)[g];
+            const bool inner = ri >= H && ri < H + kTile && rj >= H && rj < H + kTile;
+            xr[a][b] = (LEVELSET || inner) ? gbl<const float>(gin)[g] : 0.f;
             if (LEVELSET) lr[a][b] = gbl<const int32_t>(mkin)[g];
             else lr[a][b] = (int)gbl<const uint8_t>(vin)[g];
         }
@@ -406,6 +410,27 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
     // a tile cell reaches level L0+s only through a level-L0 cell within
     // distance s <= H, i.e. inside this region
     const bool work = __syncthreads_or(has_front) && __syncthreads_or(has_open);
+    if (!LEVELSET && work) {   // the halo's values (the barriers below order these stores)
+        float hv[NA][NA];
+#pragma unroll
+        for (int a = 0; a < NA; a++)
+#pragma unroll
+            for (int b = 0; b < NA; b++) {
+                const int ri = ty + a * T, rj = tx + b * T;
+                const bool inner = ri >= H && ri < H + kTile && rj >= H && rj < H + kTile;
+                const int gi = gi0 + ri, gj = gj0 + rj;
+                const size_t g = (size_t)min(max(gi, 0), si - 1) * sj + min(max(gj, 0), sj - 1);
+                hv[a][b] = (okr[a][b] && !inner) ? gbl<const float>(gin)[g] : 0.f;
+            }
+#pragma unroll
+        for (int a = 0; a < NA; a++)
+#pragma unroll
+            for (int b = 0; b < NA; b++) {
+                const int ri = ty + a * T, rj = tx + b * T;
+                const bool inner = ri >= H && ri < H + kTile && rj >= H && rj < H + kTile;
+                if (ri < R && rj < R && !inner) val[ri * R + rj] = hv[a][b];
+            }
+    }
     if (!work) {
         if (!LEVELSET) {
             for (int li = ty; li < kTile; li += T)
