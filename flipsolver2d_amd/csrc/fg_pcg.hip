@@ -29,7 +29,7 @@
 //   KA(it): rho, ||r||^2 <- partials of KB(it-1); stop test; beta;
 //           p = z + beta p_old (recomputed at the 4 neighbours), q = A' p, p.q
 //   KB(it): p.q <- partials of KA(it); alpha; r = r_old - alpha q (recomputed
-//           at the neighbours); d += alpha p; z = M r; r.z and r.r
+//           at the neighbours); x += alpha p (FP64); z = M r; r.z and r.r
 // A thread issues the loads of its cells before it reduces the previous
 // kernel's partials, so the two memory latencies overlap.
 //
