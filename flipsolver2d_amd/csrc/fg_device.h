@@ -27,7 +27,7 @@ constexpr int kPrecondMG = 0, kPrecondIPP = 1;
 constexpr int kMgMaxLevels = 12;
 constexpr int kMgBottomMax = 16;         // the bottom level has at most this many cells per side
 constexpr int kMgBottomSweeps = 8;       // L1-Jacobi sweeps on the bottom level
-constexpr float kMgOmega = 1.6f;         // L1-Jacobi weight: x += w (r - A x) / sum_k |a_k| (SPD for w < 2)
+constexpr float kMgOmega = 1.5f;         // L1-Jacobi weight: x += w (r - A x) / sum_k |a_k| (SPD for w < 2)
 constexpr int kMgStencil = 25;           // 5x5 Galerkin stencil of the coarse levels
 // k_mg_coarse runs levels t .. n-1 of a scene in one block: t is the first
 // level >= 1 with at most kMgCoarseTop cells whose levels t .. n-1 hold at

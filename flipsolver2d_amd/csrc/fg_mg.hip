@@ -20,13 +20,15 @@
 //   bottom       the first level with at most kMgBottomMax cells per side
 //                (at least level 1).
 // Smoother: one L1-Jacobi sweep before and after the coarse correction,
-// x += w (r - A x) / sum_k |a_k| with w = kMgOmega = 1.6.  The L1 diagonal
+// x += w (r - A x) / sum_k |a_k| with w = kMgOmega = 1.5.  The L1 diagonal
 // dominates A, so the eigenvalues of A / sum_k |a_k| lie in (0, 1] and the
 // sweep converges for every symmetric positive definite A and every w < 2
 // (a damped plain Jacobi sweep does not on all of these Galerkin
-// operators); w = 1.6 takes the fewest PCG iterations on dam-break and
-// fragmented systems of 128^2 .. 512^2 cells (1.0: 11 / 13 / 17 on the
-// dam break, 1.6: 9 / 11 / 14; 1.8 and up take more again).  The bottom
+// operators).  Over-relaxed weights take fewer PCG iterations: from zero
+// on dam-break systems of 128^2 / 256^2 / 512^2 cells 11 / 13 / 17 with
+// w = 1, 9 / 11 / 14 with 1.5 or 1.6, more again from 1.8; on the
+// warm-started solves of 256 mixed scenes 6.2 a solve on average with 1.5
+// (1.3: 6.6, 1.4: 6.4, 1.6: 6.5, 1.7: 7.7).  The bottom
 // level gets kMgBottomSweeps sweeps from zero.  Every step is the same
 // before and after, so the cycle is symmetric.
 //
