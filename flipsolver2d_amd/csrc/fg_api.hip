@@ -1134,6 +1134,11 @@ int run_solve(Group& G, bool rhs_from_grid)
         } else if (G.n > 1) {
             FGCHK(pack_iterating(G, &S, &count, &open));
             scalars_fresh = true;
+        } else if (sweep == 0) {
+            // one scene: it takes part (its kernels exit at once if the FP64
+            // test already passed); the first poll tells
+            count = 1;
+            S = group_launch(G, G.d, 1);
         } else {
             FGCHK(poll_group(G, -1, flags, true));
             scalars_fresh = true;
