@@ -1108,7 +1108,7 @@ int run_solve(Group& G, bool rhs_from_grid)
     // has stopped run empty.  The first poll of a solve comes after as many
     // iterations as its scenes' first sweeps took the last time (plus the
     // launch that sees the stop), later ones every kPollNext.
-    constexpr int kPollFirst = 16, kPollNext = 4, kPollMax = 64;
+    constexpr int kPollFirst = 16, kPollNext = 2, kPollMax = 64;
     const int kind = rhs_from_grid ? 0 : 1;
     int hint = 0;
     for (fg_scene* s : G.sc) hint = std::max(hint, s->iters_hint[kind]);
