@@ -37,28 +37,46 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-substeps/s + p99 frame latency, batched vs plain GPU @1e-6 resid"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per work unit of the kernels bench.py can report a
-# roofline for; the library counts the bytes itself where the unit is 1 byte.
-# k_mg_coarse, the largest share of GPU time in the batched workload (the
-# V-cycle's coarse levels, one block per scene): per scene and V-cycle the
-# dinv of its levels and its input and output vectors (4 B a cell) and once
-# each the 25 coefficients (100 B) of every row that is not its level's
-# canonical interior stencil.  k_mg_resid, timed at its level-1 launch (one
-# per V-cycle where the scene has a level below the coarse block's): per
-# level-1 cell r, |dinv| and y (12 B) plus 100 B per stored row.
-# k_pcg_a reads the stencil code (1), z (4) and the old direction (4) and
-# writes p (4) and q = A' p (4): 17 B per cell; k_pcg_b (with the V-cycle,
-# the pair r -= alpha q in FP32, x += alpha p in FP64) 32 B per cell.
+# Kernel families bench.py reports a roofline for: (library family id,
+# algorithmic bytes per work unit, unit, kernels of the family).  The library
+# counts the bytes itself where the unit is 1 byte (P particles, C cells,
+# F = F_U + F_V faces, C1 level-1 cells):
+#   p2g        16 P + 5 F per substep (positions, velocities; faces + validity)
+#   particles  36 P + 8 F + 4 C per substep: G2P + advection (k_g2p_count and
+#              k_advect: positions/velocities read and written, keys; U, V,
+#              saved U, saved V; solid SDF)
+#   sdf        8 P + 6 C per substep (positions; fluid SDF + material)
+#   pcg_a      17 B per cell and iteration (stencil code 1, z 4, old
+#              direction 4; p 4, q = A' p 4)
+#   pcg_b      32 B per cell and iteration (r -= alpha q in FP32, x += alpha p
+#              in FP64)
+#   mg_down0   5 C + 5 C1 per V-cycle (code, r; level-1 activity, r1)
+#   mg_up0     9 C + 4 C1 per V-cycle (code, r, z; level-1 y)
+#   mg_resid   12 B per level-1 cell + 100 B per stored row, per V-cycle
+#              (timed at level 1, present where a scene has a level below
+#              the coarse block's)
+#   mg_coarse  per scene and V-cycle: 4 B per cell of the levels' dinv, of its
+#              input and of its output, once each the 100 B of every row that
+#              is not its level's canonical interior stencil
 KERNEL_BYTES = {
-    "mg_coarse": ("K_MG_COARSE", 1.0, "bytes"),
-    "mg_resid": ("K_MG_RESID", 1.0, "bytes"),
-    "pcg_a": ("K_PCG_A", 17.0, "cells"),
-    "pcg_b": ("K_PCG_B", 32.0, "cells"),
+    "p2g": ("K_P2G", 1.0, "bytes", ["k_p2g"]),
+    "particles": ("K_PARTICLES", 1.0, "bytes", ["k_advect", "k_g2p_count"]),
+    "sdf": ("K_SDF", 1.0, "bytes", ["k_sdf_materials"]),
+    "pcg_a": ("K_PCG_A", 17.0, "cells", ["k_pcg_a"]),
+    "pcg_b": ("K_PCG_B", 32.0, "cells", ["k_pcg_b"]),
+    "mg_down0": ("K_MG_DOWN0", 1.0, "bytes", ["k_mg_down0"]),
+    "mg_up0": ("K_MG_UP0", 1.0, "bytes", ["k_mg_up0"]),
+    "mg_resid": ("K_MG_RESID", 1.0, "bytes", ["k_mg_resid"]),
+    "mg_coarse": ("K_MG_COARSE", 1.0, "bytes", ["k_mg_coarse"]),
 }
-# HBM bytes actually moved per work unit, per workload: rocprofv3 FETCH_SIZE
-# (x2.000) + WRITE_SIZE (x1.000) per dispatch, the factors being what the same
-# counters read for a 1 GiB copy kernel.
-MEASURED_TRAFFIC_PER_UNIT = {"mixed256": {"mg_coarse": 1.773}, "dam1024": {"mg_coarse": 1.894}}
+STEP_FAMILIES = ["p2g", "particles", "sdf", "pcg_a", "pcg_b", "mg_down0", "mg_up0", "mg_resid", "mg_coarse"]
+SOLVE_FAMILIES = ["pcg_a", "pcg_b", "mg_down0", "mg_up0", "mg_resid", "mg_coarse"]
+# HBM bytes moved per algorithmic byte, per workload and family: rocprofv3
+# FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) summed over every dispatch of the
+# family's kernels in a run of that workload alone, divided by the library's
+# algorithmic bytes over the same launches (profiles/hbm_traffic.md; the
+# factors are what the same counters read for a 1 GiB copy kernel).
+MEASURED_TRAFFIC_PER_UNIT = {"mixed256": {}, "dam1024": {}, "proj4096": {}}
 
 
 def parse():
@@ -69,7 +87,10 @@ def parse():
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--master-seed", type=int, default=2024)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--roofline-kernel", default="mg_coarse", choices=sorted(KERNEL_BYTES))
+    ap.add_argument("--legs", default="batched,plain,dam,proj,served,cpu",
+                    help="comma list of batched, plain, dam, proj, served, cpu (profiling runs pick one)")
+    ap.add_argument("--kernel-frames", type=int, default=2,
+                    help="frames (solves) each kernel family is timed over, after the timed ones (0: skip)")
     ap.add_argument("--skip-plain", action="store_true", help="batched leg only (profiling runs)")
     ap.add_argument("--lanes", type=int, default=0, help="lanes of the batched leg (0: the autotuner's choice)")
     ap.add_argument("--tune-rounds", type=int, default=1, help="autotuner rounds (each steps 4 frames)")
@@ -97,10 +118,10 @@ def pctl(v, q):
 
 class KernelTimer:
     """Per-launch HIP-event timing of one kernel family (events attached to
-    each launch on the solver's stream), accumulated over chosen regions."""
+    each launch on the stream it runs on), accumulated over chosen regions."""
 
     def __init__(self, ctx, fg, name):
-        self.ctx, self.fg = ctx, fg
+        self.ctx, self.fg, self.name = ctx, fg, name
         self.kid = getattr(fg, KERNEL_BYTES[name][0])
         self.per_unit = KERNEL_BYTES[name][1]
         self.unit_of_work = KERNEL_BYTES[name][2]
@@ -121,18 +142,22 @@ class KernelTimer:
         self.launches += n
         self.units += u
 
-    def roofline(self, traffic_per_unit=None, copy_gbs=None):
+    def roofline(self, traffic_per_unit=None, copy_gbs=None, per=1):
+        """per: the frames (solves) the timer covered, for the per-frame time."""
         achieved = self.units * self.per_unit / (self.ms * 1e-3) / 1e9 if self.ms > 0 else 0.0
         per_launch = self.units / max(self.launches, 1)
         return {
+            "kernel": "+".join(KERNEL_BYTES[self.name][3]),
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_per_unit * per_launch if traffic_per_unit else None,
+            "traffic": traffic_per_unit * per_launch * self.per_unit if traffic_per_unit else None,
+            "traffic_per_algorithmic_byte": traffic_per_unit,
             "avg_launch_us": 1e3 * self.ms / max(self.launches, 1),
             "launches": self.launches,
+            "ms_per_frame": self.ms / max(per, 1),
             "algorithmic_bytes_per_launch": per_launch * self.per_unit,
             "bytes_per_unit": self.per_unit,
             "unit_of_work": self.unit_of_work,
@@ -144,11 +169,39 @@ class KernelTimer:
         }
 
 
-def batched_leg(fg, ctx, a):
+def kernel_table(fg, ctx, families, step, reps, workload, copy_gbs, where):
+    """Each family timed with HIP events on its launches over `reps` further
+    frames (solves) of the workload, one family at a time; returns the
+    per-family rooflines and the dominant family (most GPU time per frame)."""
+    table = {}
+    if reps <= 0:
+        return table, None
+    for name in families:
+        kt = KernelTimer(ctx, fg, name)
+        with kt:
+            for _ in range(reps):
+                step()
+        if kt.launches == 0 or kt.units == 0:   # the workload has no such launch
+            continue
+        table[name] = dict(kt.roofline(MEASURED_TRAFFIC_PER_UNIT[workload].get(name), copy_gbs, per=reps),
+                           measured_on=f"{reps} {where} after the timed ones")
+    dom = max(table, key=lambda k: table[k]["ms_per_frame"]) if table else None
+    return table, dom
+
+
+def work_totals(fg, ctx):
+    """Algorithmic bytes of every reported family over the leg (all launches)."""
+    w = ctx.work_read(reset=True)
+    return {name: int(w[getattr(fg, kid)] * per) for name, (kid, per, _, _) in KERNEL_BYTES.items()}
+
+
+def batched_leg(fg, ctx, a, copy_gbs):
     specs = fg.mixed_scene_specs(a.scenes, master_seed=a.master_seed)
     t0 = time.perf_counter()
+    ctx.work_read(reset=True)
     batch = [fg.make_scene(ctx, sp, tol=a.tol, pcg_iter_limit=4000) for sp in specs]
-    plain = [] if a.skip_plain else [fg.make_scene(ctx, sp, tol=a.tol, pcg_iter_limit=4000) for sp in specs]
+    plain = [] if "plain" not in a.legs else [fg.make_scene(ctx, sp, tol=a.tol, pcg_iter_limit=4000)
+                                              for sp in specs]
     n0 = sum(s.particle_count() for s in batch)
     cells = sum(sp["n"] ** 2 for sp in specs)
     log(f"{a.scenes} mixed scenes: {cells} cells, {n0} particles, set up in {time.perf_counter() - t0:.1f} s")
@@ -168,7 +221,6 @@ def batched_leg(fg, ctx, a):
         lanes = a.lanes
         ctx.set_batch_lanes(lanes)
         log(f"lanes fixed at {lanes}")
-    kt = KernelTimer(ctx, fg, a.roofline_kernel)
     el_b = el_p = 0.0
     ps_b = ps_p = 0
     lat_b, lat_p, frame_b, frame_p = [], [], [], []
@@ -196,13 +248,12 @@ def batched_leg(fg, ctx, a):
             el_p += dt
             frame_p.append(dt * 1e3)
         log(f"frame {k}: batched {frame_b[-1]:.1f} ms" + (f", plain {frame_p[-1]:.1f} ms" if plain else ""))
-    # the roofline kernel: HIP events attached to each of its launches on the
-    # stream it runs on, over as many further frames of the same batch (events
-    # on every launch slow the launch pipeline by ~7 %, so the timed frames
-    # above run without them)
-    for k in range(a.steps):
-        with kt:
-            fg.batch_step_frame(ctx, batch)
+    # the named kernels: HIP events attached to each of their launches on the
+    # stream they run on, over further frames of the same batch (events on
+    # every launch slow the launch pipeline by ~7 %, so the timed frames above
+    # run without them)
+    table, dom = kernel_table(fg, ctx, STEP_FAMILIES, lambda: fg.batch_step_frame(ctx, batch), a.kernel_frames,
+                              "mixed256", copy_gbs, "frames of the same batch")
     for sc in batch + plain:
         sc.close()
     out = {
@@ -216,7 +267,9 @@ def batched_leg(fg, ctx, a):
         "all_converged": conv,
         "max_rel_residual": resid,
         "pcg_iters_per_substep_max": iters,
-        "kt": kt,
+        "kernels": table,
+        "dominant": dom,
+        "algorithmic_bytes_total": work_totals(fg, ctx),
         "lanes": lanes,
         "autotune_rates": [float(r) for r in rates],
     }
@@ -234,11 +287,11 @@ def batched_leg(fg, ctx, a):
 
 def dam_leg(fg, ctx, a, copy_gbs):
     res, ppc = 1024, 8
+    ctx.work_read(reset=True)
     scene = fg.dam_break_scene(ctx, res, ppc=ppc, tol=a.tol, pcg_iter_limit=4000)
     n0 = scene.particle_count()
     for _ in range(a.warmup):
         scene.step_frame()
-    kt = KernelTimer(ctx, fg, a.roofline_kernel)
     lat, psub, subs, resid, conv = [], 0, 0, 0.0, True
     t0 = time.perf_counter()
     for _ in range(a.dam_frames):
@@ -249,10 +302,8 @@ def dam_leg(fg, ctx, a, copy_gbs):
         resid = max(resid, s.pressure_rel_resid)
         conv = conv and bool(s.converged)
     el = time.perf_counter() - t0
-    roof_frames = min(a.dam_frames, 3)   # the roofline kernel, on further frames (see batched_leg)
-    with kt:
-        for _ in range(roof_frames):
-            scene.step_frame()
+    table, dom = kernel_table(fg, ctx, STEP_FAMILIES, scene.step_frame, a.kernel_frames, "dam1024", copy_gbs,
+                              "frames")
     scene.close()
     log(f"dam break {res}^2: {a.dam_frames} frames, {subs} substeps, {el:.2f} s")
     return {
@@ -265,20 +316,20 @@ def dam_leg(fg, ctx, a, copy_gbs):
         "p99_frame_ms": pctl(lat, 0.99),
         "max_rel_residual": resid,
         "all_converged": conv,
-        "roofline": dict(kt.roofline(MEASURED_TRAFFIC_PER_UNIT.get("dam1024", {}).get(a.roofline_kernel), copy_gbs),
-                         kernel=f"k_{a.roofline_kernel}",
-                         measured_on=f"{roof_frames} frames right after the timed ones"),
+        "roofline": table.get(dom),
+        "kernels": table,
+        "algorithmic_bytes_total": work_totals(fg, ctx),
     }
 
 
 def projection_leg(fg, ctx, a, copy_gbs):
     n = a.proj_res
     mat, _, _, b, scale = fg.isolated_projection_system(n, 1)
-    kt = KernelTimer(ctx, fg, a.roofline_kernel)
+    ctx.work_read(reset=True)
     ctx.poisson_solve(mat, scale, b, tol=a.tol)          # warm-up (allocation, code load)
     x, st = ctx.poisson_solve(mat, scale, b, tol=a.tol)
-    with kt:                                              # the roofline kernel, on a second solve
-        ctx.poisson_solve(mat, scale, b, tol=a.tol)
+    table, dom = kernel_table(fg, ctx, SOLVE_FAMILIES, lambda: ctx.poisson_solve(mat, scale, b, tol=a.tol),
+                              max(1, a.kernel_frames), "proj4096", copy_gbs, "solves of the same system")
     liquid = int(((mat & fg.FLUID) != 0).sum())
     log(f"isolated projection {n}^2: {st.iterations} iterations, {st.solve_ms:.1f} ms, resid {st.rel_residual:.2e}")
     return {
@@ -289,8 +340,9 @@ def projection_leg(fg, ctx, a, copy_gbs):
         "rel_residual": st.rel_residual,
         "converged": bool(st.converged),
         "solve_ms_incl_rhs_upload": st.solve_ms,
-        "roofline": dict(kt.roofline(None, copy_gbs), kernel=f"k_{a.roofline_kernel}",
-                         measured_on="a second solve of the same system"),
+        "roofline": table.get(dom),
+        "kernels": table,
+        "algorithmic_bytes_total": work_totals(fg, ctx),
     }
 
 
@@ -383,18 +435,25 @@ def main():
         sys.exit(2)
     import flipsolver2d_amd as fg
 
+    a.legs = set(a.legs.split(","))
+    if a.skip_plain:
+        a.legs.discard("plain")
     ctx = fg.Context(0)
     copy_gbs, _ = ctx.bandwidth_probe(1 << 30, 5)
-    b = batched_leg(fg, ctx, a)
-    dam = dam_leg(fg, ctx, a, copy_gbs) if a.dam_frames > 0 else None
-    proj = projection_leg(fg, ctx, a, copy_gbs) if a.proj_res > 0 else None
-    served = served_leg(fg, ctx, a) if a.serve_seconds > 0 else None
+    b = batched_leg(fg, ctx, a, copy_gbs) if "batched" in a.legs else None
+    dam = dam_leg(fg, ctx, a, copy_gbs) if a.dam_frames > 0 and "dam" in a.legs else None
+    proj = projection_leg(fg, ctx, a, copy_gbs) if a.proj_res > 0 and "proj" in a.legs else None
+    served = served_leg(fg, ctx, a) if a.serve_seconds > 0 and "served" in a.legs else None
     ctx.close()
-    cpu = cpu_baseline(fg, a) if a.cpu_seconds > 0 else None
+    cpu = cpu_baseline(fg, a) if a.cpu_seconds > 0 and "cpu" in a.legs else None
     if cpu:
         log(f"cpu baseline: {cpu['value']:.3e} particle-substeps/s")
+    if b is None:   # a profiling run of other legs: their results alone
+        print(json.dumps({"dam_break_1024": dam, "isolated_projection": proj, "served": served,
+                          "cpu_baseline": cpu}), flush=True)
+        return
 
-    kt = b.pop("kt")
+    dom = b.pop("dominant")
     out = {
         "metric": METRIC,
         "value": b["value"],
@@ -423,9 +482,9 @@ def main():
         "pcg_iters_per_substep_max": b["pcg_iters_per_substep_max"],
         "plain": b.get("plain"),
         "batched_over_plain": b.get("batched_over_plain"),
-        "roofline": dict(kt.roofline(MEASURED_TRAFFIC_PER_UNIT.get("mixed256", {}).get(a.roofline_kernel),
-                                     copy_gbs), kernel=f"k_{a.roofline_kernel}",
-                         measured_on=f"{a.steps} frames of the same batch right after the timed ones"),
+        "roofline": b["kernels"].get(dom),
+        "kernels": b["kernels"],
+        "algorithmic_bytes_total": b["algorithmic_bytes_total"],
         "cpu_baseline": cpu,
         "dam_break_1024": dam,
         "isolated_projection": proj,

@@ -32,17 +32,21 @@ EXPORTS = [
     "fg_server_create", "fg_server_destroy", "fg_server_submit", "fg_server_tick", "fg_server_status",
     "fg_server_scene", "fg_server_release", "fg_server_summary_get", "fg_ctx_set_batch_lanes",
     "fg_ctx_batch_lanes", "fg_batch_autotune", "fg_ctx_set_solver_tiling", "fg_ctx_set_preconditioner",
-    "fg_ctx_preconditioner",
+    "fg_ctx_preconditioner", "fg_batch_substep", "fg_server_cancel", "fg_ctx_work_read",
 ]
 MAX_LANES = 4
-REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_REJECTED, REQ_RELEASED = range(5)
+REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_REJECTED, REQ_RELEASED, REQ_FAILED, REQ_CANCELLED = range(7)
+E_INVALID, E_DEVICE, E_BUDGET, E_CAPACITY, E_STATE, E_NUMERIC, E_CANCELLED = -1, -2, -3, -4, -5, -6, -7
 (K_ADVECT, K_SORT, K_P2G, K_SDF, K_LEVELSET, K_EXTRAPOLATE, K_PCG_A, K_PCG_B, K_G2P, K_RHS, K_APPLY,
- K_VCYCLE, K_MG_RESID, K_MG_COARSE) = range(14)
+ K_VCYCLE, K_MG_RESID, K_MG_COARSE, K_MG_DOWN0, K_MG_UP0, K_PARTICLES) = range(17)
+K_NUM = 17
 PRECOND_MG, PRECOND_IPP = 0, 1
 
 
 class FgError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 class SceneParams(C.Structure):
@@ -65,6 +69,7 @@ class SceneState(C.Structure):
         ("material", C.c_void_p), ("solid_sdf", C.c_void_p), ("fluid_sdf", C.c_void_p),
         ("solid_id", C.c_void_p), ("particle_counts", C.c_void_p),
         ("saved_u", C.c_void_p), ("saved_v", C.c_void_p), ("pressure", C.c_void_p), ("rhs", C.c_void_p),
+        ("stencil", C.c_void_p),
     ]
 
 
@@ -77,6 +82,7 @@ class Stats(C.Structure):
         ("particles_killed", C.c_int64), ("particles_erased", C.c_int64),
         ("last_dt", C.c_float), ("max_velocity", C.c_float),
         ("stage_ms", C.c_double * 12), ("frame_ms", C.c_double),
+        ("status", C.c_int32), ("reserved", C.c_int32),
     ]
 
     def as_dict(self):
@@ -95,7 +101,8 @@ class Request(C.Structure):
 class RequestStatus(C.Structure):
     _fields_ = [("state", C.c_int32), ("frames_done", C.c_int32), ("particles", C.c_int64),
                 ("particle_substeps", C.c_int64), ("submit_ms", C.c_double), ("admit_ms", C.c_double),
-                ("finish_ms", C.c_double), ("max_frame_ms", C.c_double)]
+                ("finish_ms", C.c_double), ("max_frame_ms", C.c_double), ("error", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class TickStats(C.Structure):
@@ -109,7 +116,8 @@ class ServerSummary(C.Structure):
                 ("p50_frame_ms", C.c_double), ("p99_frame_ms", C.c_double), ("max_frame_ms", C.c_double),
                 ("p50_admit_wait_ms", C.c_double), ("p99_admit_wait_ms", C.c_double),
                 ("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
-                ("peak_resident_bytes", C.c_uint64), ("peak_resident", C.c_int32)]
+                ("peak_resident_bytes", C.c_uint64), ("peak_resident", C.c_int32), ("reserved", C.c_int32),
+                ("failed", C.c_int64), ("cancelled", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -150,8 +158,10 @@ def lib():
             "fg_scene_step_frame": (C.c_int, [vp, C.POINTER(Stats)]),
             "fg_scene_stage": (C.c_int, [vp, i32, f32]),
             "fg_batch_step_frame": (C.c_int, [vp, vp, i32, vp]),
+            "fg_batch_substep": (C.c_int, [vp, vp, i32, vp, vp]),
             "fg_poisson_solve": (C.c_int, [vp, i32, i32, vp, f64, vp, vp, f64, i32, C.POINTER(SolveStats)]),
             "fg_ctx_profile_kernel": (C.c_int, [vp, i32]),
+            "fg_ctx_work_read": (C.c_int, [vp, vp, i32]),
             "fg_ctx_profile_read": (C.c_int, [vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64), C.POINTER(i64)]),
             "fg_ctx_bandwidth_probe": (C.c_int, [vp, C.c_uint64, i32, C.POINTER(f64), C.POINTER(f64)]),
             "fg_ctx_set_batch_lanes": (C.c_int, [vp, i32]),
@@ -167,6 +177,7 @@ def lib():
             "fg_server_status": (C.c_int, [vp, i64, C.POINTER(RequestStatus)]),
             "fg_server_scene": (C.c_int, [vp, i64, C.POINTER(vp)]),
             "fg_server_release": (C.c_int, [vp, i64]),
+            "fg_server_cancel": (C.c_int, [vp, i64]),
             "fg_server_summary_get": (C.c_int, [vp, C.POINTER(ServerSummary)]),
         }
         for name, (res, args) in sig.items():
@@ -179,7 +190,7 @@ def lib():
 
 def _check(rc):
     if rc != FG_OK:
-        raise FgError(f"flipgpu error {rc}: {lib().fg_last_error().decode(errors='replace')}")
+        raise FgError(f"flipgpu error {rc}: {lib().fg_last_error().decode(errors='replace')}", rc)
 
 
 def _ptr(a):
@@ -249,6 +260,13 @@ class Context:
         t, b, n, u = C.c_double(), C.c_double(), C.c_int64(), C.c_int64()
         _check(lib().fg_ctx_profile_read(self.h, C.byref(t), C.byref(b), C.byref(n), C.byref(u)))
         return t.value, b.value, n.value, u.value
+
+    def work_read(self, reset=False):
+        """Work units of every kernel family (fg_kernel_id order) since the
+        last reset, counted for every launch."""
+        out = np.zeros(K_NUM, np.int64)
+        _check(lib().fg_ctx_work_read(self.h, out.ctypes.data, 1 if reset else 0))
+        return out
 
     def bandwidth_probe(self, nbytes=1 << 30, reps=5):
         g, ms = C.c_double(), C.c_double()
@@ -328,7 +346,7 @@ class Scene:
             "fluid_sdf": np.zeros(I * J, np.float32), "solid_id": np.zeros(I * J, np.int32),
             "particle_counts": np.zeros(I * J, np.int32), "saved_u": np.zeros((I + 1) * J, np.float32),
             "saved_v": np.zeros(I * (J + 1), np.float32), "pressure": np.zeros(I * J, np.float64),
-            "rhs": np.zeros(I * J, np.float64),
+            "rhs": np.zeros(I * J, np.float64), "stencil": np.zeros(I * J, np.uint8),
         }
         s = SceneState()
         for k, a in out.items():
@@ -388,14 +406,38 @@ def dam_break_scene(ctx, n, ppc=4, seed=42, fluid_box=(0.3, 1.0, 0.0, 0.4), **kw
     return sc
 
 
-def batch_step_frame(ctx, scenes):
+def batch_step_frame(ctx, scenes, allow_quarantine=False):
     """One FlipSolver::stepFrame for every scene, advanced together (one
-    launch per phase over all scenes still inside their frame)."""
+    launch per phase over all scenes still inside their frame).  With
+    allow_quarantine, a quarantined scene (stats[k].status) does not raise."""
     n = len(scenes)
     arr = (C.c_void_p * n)(*[s.h.value for s in scenes])
     stats = (Stats * n)()
-    _check(lib().fg_batch_step_frame(ctx.h, arr, n, stats))
+    rc = lib().fg_batch_step_frame(ctx.h, arr, n, stats)
+    if not (allow_quarantine and rc == E_NUMERIC):
+        _check(rc)
     return list(stats)
+
+
+def batch_substep(ctx, scenes, dts, allow_quarantine=False):
+    """One FlipSolver::step of every scene, scene k with dts[k], advanced
+    together (one launch per phase)."""
+    n = len(scenes)
+    arr = (C.c_void_p * n)(*[s.h.value for s in scenes])
+    d = np.ascontiguousarray(dts, np.float32)
+    assert len(d) == n
+    stats = (Stats * n)()
+    rc = lib().fg_batch_substep(ctx.h, arr, n, d.ctypes.data, stats)
+    if not (allow_quarantine and rc == E_NUMERIC):
+        _check(rc)
+    return list(stats)
+
+
+def stencil_matrix(stencil):
+    """Split the stencil bytes of get_state() into the (n, mask, fluid)
+    arrays of the pressure matrix (bits 4-6, bits 0-3, bit 7)."""
+    st = np.asarray(stencil, np.uint8)
+    return ((st >> 4) & 7).astype(np.uint8), (st & 15).astype(np.uint8), (st >> 7).astype(np.uint8)
 
 
 def batch_autotune(ctx, scenes, rounds=1):
@@ -499,6 +541,9 @@ class Server:
 
     def release(self, rid):
         _check(lib().fg_server_release(self.h, rid))
+
+    def cancel(self, rid):
+        _check(lib().fg_server_cancel(self.h, rid))
 
     def summary(self):
         s = ServerSummary()

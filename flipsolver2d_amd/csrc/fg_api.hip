@@ -85,6 +85,8 @@ struct Lane {
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     int64_t prof_units = 0;
+    // work units of every family since the last fg_ctx_work_read reset
+    int64_t work[FG_K_NUM] = {};
 };
 
 struct fg_ctx {
@@ -127,6 +129,7 @@ struct fg_scene {
     bool vmax_ready = false;
     int next_stage = 0;
     int frame = 0;
+    int status = FG_OK;           // FG_E_NUMERIC: quarantined until the next set_state
 };
 
 namespace {
@@ -351,6 +354,55 @@ int read_scalars(fg_scene* s)
     return FG_OK;
 }
 
+// Carve a scene's arena (s->arena, s->bytes) and fill its descriptor; clear
+// the arena and upload the hierarchy table and the descriptor (s->d) on the
+// context's stream.  Allocates nothing.
+int setup_scene_memory(fg_scene* s)
+{
+    fg_ctx* ctx = s->ctx;
+    const fg_scene_params* p = &s->p;
+    const bool solver_only = s->solver_only;
+    Carver cv;
+    cv.base = reinterpret_cast<char*>(s->arena);
+    carve(s, cv);
+    SceneDev& h = s->h;
+    h.I = s->I;
+    h.J = s->J;
+    h.ppc = p->particles_per_cell;
+    h.n_friction = 0;
+    h.friction_any = 0;
+    h.pic = p->pic_ratio;
+    h.pscale = p->particle_scale;
+    h.g_i = p->gravity_i;
+    h.g_j = p->gravity_j;
+    h.dx = p->dx;
+    h.rho = p->fluid_density;
+    h.tol = p->solver_tolerance > 0 ? p->solver_tolerance : 1e-6;
+    h.max_inner = p->pcg_iter_limit > 0 ? p->pcg_iter_limit : 2000;
+    h.active = 1;
+    h.density_mode = p->density_mode;
+    // flipsolver2d.cpp:207-208
+    h.cell_vol = (float)(p->dx * p->dx * p->dx);
+    h.pmass = (float)((p->fluid_density * h.cell_vol) / (float)p->particles_per_cell);
+    h.max_blocks = s->max_blocks;
+    h.pcg_blocks = s->pcg_blocks;
+    h.pcg_chunk = s->pcg_chunk;
+    h.max_pblocks = s->max_pblocks;
+    h.pblocks = 1;
+    h.cap = s->cap;
+    hipStream_t st = ctx->stream;
+    HIPCHK(hipMemsetAsync(s->arena, 0, s->bytes, st));
+    if (!solver_only) {
+        // empty, all-EMPTY scene until state arrives
+        HIPCHK(hipMemsetAsync(h.mat, (int)(int8_t)FG_EMPTY, s->C, st));
+        HIPCHK(hipMemsetAsync(h.sid, 0xff, sizeof(int32_t) * s->C, st));
+    }
+    HIPCHK(hipMemcpyAsync(h.mg, &s->mgh, sizeof(MgLevels), hipMemcpyHostToDevice, st));
+    FGCHK(upload_desc(s));
+    HIPCHK(hipStreamSynchronize(st));
+    return FG_OK;
+}
+
 int create_scene(fg_ctx* ctx, const fg_scene_params* p, bool solver_only, fg_scene** out)
 {
     FGCHK(validate_params(p));
@@ -380,45 +432,12 @@ int create_scene(fg_ctx* ctx, const fg_scene_params* p, bool solver_only, fg_sce
         delete s;
         return fail(FG_E_DEVICE, std::string("descriptor alloc: ") + hipGetErrorString(e));
     }
-    Carver cv;
-    cv.base = reinterpret_cast<char*>(s->arena);
-    carve(s, cv);
-    SceneDev& h = s->h;
-    h.I = s->I;
-    h.J = s->J;
-    h.ppc = p->particles_per_cell;
-    h.n_friction = 0;
-    h.friction_any = 0;
-    h.pic = p->pic_ratio;
-    h.pscale = p->particle_scale;
-    h.g_i = p->gravity_i;
-    h.g_j = p->gravity_j;
-    h.dx = p->dx;
-    h.rho = p->fluid_density;
-    h.tol = p->solver_tolerance > 0 ? p->solver_tolerance : 1e-6;
-    h.max_inner = p->pcg_iter_limit > 0 ? p->pcg_iter_limit : 2000;
-    h.active = 1;
-    h.density_mode = p->density_mode;
-    // flipsolver2d.cpp:207-208
-    h.cell_vol = (float)(p->dx * p->dx * p->dx);
-    h.pmass = (float)((p->fluid_density * h.cell_vol) / (float)p->particles_per_cell);
-    h.max_blocks = s->max_blocks;
-    h.pcg_blocks = s->pcg_blocks;
-    h.pcg_chunk = s->pcg_chunk;
-    h.max_pblocks = s->max_pblocks;
-    h.pblocks = 1;
-    h.cap = s->cap;
     ctx->used += s->bytes;
-    hipStream_t st = ctx->stream;
-    HIPCHK(hipMemsetAsync(s->arena, 0, s->bytes, st));
-    if (!solver_only) {
-        // empty, all-EMPTY scene until state arrives
-        HIPCHK(hipMemsetAsync(h.mat, (int)(int8_t)FG_EMPTY, s->C, st));
-        HIPCHK(hipMemsetAsync(h.sid, 0xff, sizeof(int32_t) * s->C, st));
+    const int rc = setup_scene_memory(s);
+    if (rc != FG_OK) {
+        fg_scene_destroy(s);
+        return rc;
     }
-    HIPCHK(hipMemcpyAsync(h.mg, &s->mgh, sizeof(MgLevels), hipMemcpyHostToDevice, st));
-    FGCHK(upload_desc(s));
-    HIPCHK(hipStreamSynchronize(st));
     *out = s;
     return FG_OK;
 }
@@ -997,14 +1016,32 @@ struct Timer {
 // ---------------------------------------------------------------------------
 // per-launch kernel timing (events bracket each launch on its own stream)
 // ---------------------------------------------------------------------------
+// Work units of a kernel family: always counted; also the timed family's
+// units when it is the one being timed.
+inline void add_work(Lane* L, int timed, int family, int64_t units)
+{
+    L->work[family] += units;
+    if (timed == family) L->prof_units += units;
+}
+
+// The timed family takes in the kernel: itself, or a family made of it.
+inline bool prof_family_has(int family, int kernel)
+{
+    return family == kernel || (family == FG_K_PARTICLES && (kernel == FG_K_ADVECT || kernel == FG_K_G2P));
+}
+
 struct KProf {
     Lane* L;
+    int timed, kernel;
     bool on, single;
     // single: the events are attached to the next kernel launch itself
     // (hipExtLaunchKernelGGL) and bracket exactly that kernel; otherwise they
-    // are recorded around the whole multi-kernel stage.
-    KProf(const Group& G, int kernel, bool single_kernel = false)
-        : L(G.L), on(G.ctx->prof_kernel == kernel), single(single_kernel)
+    // are recorded around the whole multi-kernel stage.  Work units are
+    // counted only when the kernel is the timed family itself (a composite
+    // family counts its bytes once per substep).
+    KProf(const Group& G, int kernel_, bool single_kernel = false)
+        : L(G.L), timed(G.ctx->prof_kernel), kernel(kernel_), on(prof_family_has(timed, kernel_)),
+          single(single_kernel)
     {
         if (!on) return;
         hipEvent_t e0 = take(), e1 = take();
@@ -1022,9 +1059,9 @@ struct KProf {
     }
     void done(int64_t units)
     {
+        if (kernel >= 0) add_work(L, timed, kernel, units);
         if (!on) return;
         if (!single) hipEventRecord(L->ev_pool[L->ev_used - 1], L->stream);
-        L->prof_units += units;
     }
 };
 
@@ -1042,7 +1079,11 @@ void launch_vcycle(const Group& G, hipStream_t st, const SolveLaunch& S, int whi
     // (t per scene; the maps and vc_grid hold a level's blocks only for the
     // scenes with l < t)
     const int top = G.mg_top;
-    if (S.blocks[1]) launch_mg_down0(S.grid(1), st, S.d, S.map[1], which);
+    if (S.blocks[1]) {
+        KProf kp(G, FG_K_MG_DOWN0, true);
+        launch_mg_down0(S.grid(1), st, S.d, S.map[1], which);
+        kp.done(0);
+    }
     for (int l = 1; l < top; l++) {
         if (S.blocks[1 + l]) {
             // the profiling family times the largest residual launch (level 1)
@@ -1062,7 +1103,9 @@ void launch_vcycle(const Group& G, hipStream_t st, const SolveLaunch& S, int whi
         launch_mg_prolong(S.grid(1 + l), st, S.d, S.map[1 + l], l);
         launch_mg_post(S.grid(1 + l), st, S.d, S.map[1 + l], l);
     }
+    KProf kp(G, FG_K_MG_UP0, true);
     launch_mg_up0(S.grid(0), st, S.d, S.map[0], which);
+    kp.done(0);
 }
 
 // Algorithmic bytes of one k_mg_coarse block (one scene, one V-cycle): the
@@ -1158,7 +1201,7 @@ int run_solve(Group& G, bool rhs_from_grid)
                 {
                     KProf kp(G, FG_K_PCG_A, true);
                     launch_pcg_a(S.grid(0), st, S.d, S.map[0], it);
-                    kp.done(0);
+                    kp.done(0);   // units: at the end of the solve
                 }
                 {
                     KProf kp(G, FG_K_PCG_B, true);
@@ -1193,24 +1236,25 @@ int run_solve(Group& G, bool rhs_from_grid)
         s->iters_hint[kind] = h.sweeps > 1 ? h.total_iters - h.inner_iters : h.total_iters;
     }
     // work units of the launches that iterated (the rest exited at once)
-    if (ctx->prof_kernel == FG_K_PCG_A || ctx->prof_kernel == FG_K_PCG_B)
-        for (fg_scene* s : G.sc) G.L->prof_units += (int64_t)s->hsc->total_iters * s->C;
-    if (ctx->prof_kernel == FG_K_VCYCLE)   // one V-cycle per iteration and per sweep start
-        for (fg_scene* s : G.sc)
-            if (s->h.precond == kPrecondMG) G.L->prof_units += (int64_t)(s->hsc->total_iters + s->hsc->sweeps) * s->C;
-    if (ctx->prof_kernel == FG_K_MG_RESID)   // level-1 bytes: r, dinv, y per cell + stored rows' coefficients
-        for (fg_scene* s : G.sc) {
-            if (s->h.precond != kPrecondMG || s->mgh.t < 2) continue;   // no level-1 launch
-            const MgLevel& L1 = s->mgh.lv[1];
-            const int64_t per_cycle = 12 * (int64_t)L1.I * L1.J + 4 * kMgStencil * (int64_t)s->hsc->mg_stored[1];
-            G.L->prof_units += (int64_t)(s->hsc->total_iters + s->hsc->sweeps) * per_cycle;
-        }
-    if (ctx->prof_kernel == FG_K_MG_COARSE)   // its algorithmic bytes per V-cycle, see mg_coarse_bytes
-        for (fg_scene* s : G.sc)
-            if (s->h.precond == kPrecondMG)
-                G.L->prof_units += (int64_t)(s->hsc->total_iters + s->hsc->sweeps) * mg_coarse_bytes(s);
-    for (fg_scene* s : G.sc)
-        if (s->hsc->numeric_bad) return fail(FG_E_NUMERIC, "NaN/Inf in the pressure solve");
+    const int timed = ctx->prof_kernel;
+    for (fg_scene* s : G.sc) {
+        const SceneScalars& h = *s->hsc;
+        add_work(G.L, timed, FG_K_PCG_A, (int64_t)h.total_iters * s->C);
+        add_work(G.L, timed, FG_K_PCG_B, (int64_t)h.total_iters * s->C);
+        if (s->h.precond != kPrecondMG) continue;
+        const int64_t cycles = (int64_t)h.total_iters + h.sweeps;   // one per iteration and per sweep start
+        add_work(G.L, timed, FG_K_VCYCLE, cycles * s->C);
+        const MgLevel& L1 = s->mgh.lv[1];
+        const int64_t C1 = (int64_t)L1.I * L1.J;
+        // level-1 residual launch (scenes with a level below the coarse
+        // block's): r, dinv, y per cell + stored rows' coefficients
+        if (s->mgh.t >= 2) add_work(G.L, timed, FG_K_MG_RESID, cycles * (12 * C1 + 4 * kMgStencil * (int64_t)h.mg_stored[1]));
+        add_work(G.L, timed, FG_K_MG_COARSE, cycles * mg_coarse_bytes(s));
+        add_work(G.L, timed, FG_K_MG_DOWN0, cycles * (5 * s->C + 5 * C1));
+        add_work(G.L, timed, FG_K_MG_UP0, cycles * (9 * s->C + 4 * C1));
+    }
+    // a scene whose solve saw NaN/Inf stopped iterating (its flags say so);
+    // the caller quarantines it, the others are unaffected
     return FG_OK;
 }
 
@@ -1274,13 +1318,13 @@ int run_stage(Group& G, int stage)
     case FG_STAGE_P2G: {
         KProf kp(G, FG_K_P2G, true);
         launch_p2g(g.p2g, st, d);
-        kp.done(G.cells);
+        kp.done(0);   // bytes: stage_bytes, once the substep's counts are back
         break;
     }
     case FG_STAGE_SDF: {
         KProf kp(G, FG_K_SDF, true);
         launch_sdf_materials(g.bins, st, d);
-        kp.done(G.cells);
+        kp.done(0);
         break;
     }
     case FG_STAGE_LEVELSET: {
@@ -1354,12 +1398,39 @@ int run_stage(Group& G, int stage)
     return FG_OK;
 }
 
-// After a substep's last stage: commit the counts the kernels produced.
+// Algorithmic bytes of the byte-counted particle-stage families for one
+// substep of every scene of G (P = the particles this substep sorted, the
+// input of P2G, the fluid SDF and G2P; see fg_kernel_id).
+void stage_bytes(Group& G)
+{
+    const int timed = G.ctx->prof_kernel;
+    for (fg_scene* s : G.sc) {
+        const int64_t P = s->hsc->n_particles, F = s->FU + s->FV, C = s->C;
+        add_work(G.L, timed, FG_K_P2G, 16 * P + 5 * F);
+        add_work(G.L, timed, FG_K_SDF, 8 * P + 6 * C);
+        add_work(G.L, timed, FG_K_PARTICLES, 36 * P + 8 * F + 4 * C);
+    }
+}
+
+// Take a scene out of stepping until set_state gives it a new state.
+void quarantine(fg_scene* s, fg_stats* st)
+{
+    s->status = FG_E_NUMERIC;
+    s->vmax_ready = false;
+    if (st) {
+        st->status = FG_E_NUMERIC;
+        st->converged = 0;
+    }
+}
+
+// After a substep's last stage: commit the counts the kernels produced;
+// a scene whose solve saw NaN/Inf is quarantined.
 void commit_substep(Group& G, float* dts)
 {
     for (int k = 0; k < G.n; k++) {
         fg_scene* s = G.sc[k];
         const SceneScalars& sc = *s->hsc;
+        if (sc.numeric_bad) quarantine(s, G.st[k]);
         const int64_t n_start = s->n_live;
         s->ls_hint = sc.ls_bands;
         s->n_live = sc.n_particles - sc.erased;
@@ -1396,6 +1467,7 @@ int substep_group(fg_ctx* ctx, Lane* L, const std::vector<fg_scene*>& scenes, co
     }
     HIPCHK(hipGetLastError());
     FGCHK(read_group_scalars(G));
+    stage_bytes(G);
     std::vector<float> d(dts);
     commit_substep(G, d.data());
     return FG_OK;
@@ -1427,27 +1499,37 @@ int step_frames_lane(fg_ctx* ctx, Lane* L, const std::vector<fg_scene*>& scenes,
     std::vector<float> t(n, 0.f);
     std::vector<int> count(n, 0);
     std::vector<char> finished(n, 0);
+    for (size_t k = 0; k < n; k++)
+        if (scenes[k]->status != FG_OK) {   // quarantined earlier: skipped
+            finished[k] = 1;
+            if (stats[k]) {
+                stats[k]->status = scenes[k]->status;
+                stats[k]->converged = 0;
+            }
+        }
     for (;;) {
+        std::vector<fg_scene*> need;
+        for (size_t k = 0; k < n; k++)
+            if (!finished[k]) need.push_back(scenes[k]);
+        if (need.empty()) break;
+        FGCHK(ensure_vmax(ctx, L, need));
         std::vector<fg_scene*> act;
         std::vector<fg_stats*> ast;
         std::vector<size_t> idx;
-        for (size_t k = 0; k < n; k++)
-            if (!finished[k]) {
-                act.push_back(scenes[k]);
-                ast.push_back(stats[k]);
-                idx.push_back(k);
-            }
-        if (act.empty()) break;
-        FGCHK(ensure_vmax(ctx, L, act));
-        std::vector<float> dts(act.size());
-        for (size_t a = 0; a < act.size(); a++) {
-            const size_t k = idx[a];
-            fg_scene* s = act[a];
+        std::vector<float> dts;
+        for (size_t k = 0; k < n; k++) {
+            if (finished[k]) continue;
+            fg_scene* s = scenes[k];
             // FlipSolver::maxParticleVelocity + the substep split, flipsolver2d.cpp:476-490
             const float frame_dt = 1.f / s->p.fps;
             float v2;
             std::memcpy(&v2, &s->hsc->max_v2_bits, 4);
             const float vel = std::sqrt(v2);
+            if (!std::isfinite(vel)) {   // a NaN/Inf particle velocity: no dt can be formed
+                quarantine(s, stats[k]);
+                finished[k] = 1;
+                continue;
+            }
             float mss = s->p.cfl_number / (vel + 1e-15f);
             if (t[k] + mss >= frame_dt || count[k] == (s->p.max_substeps - 1)) {
                 mss = frame_dt - t[k];
@@ -1456,8 +1538,12 @@ int step_frames_lane(fg_ctx* ctx, Lane* L, const std::vector<fg_scene*>& scenes,
                 mss = 0.5f * (frame_dt - t[k]);
             }
             if (stats[k]) stats[k]->max_velocity = vel;
-            dts[a] = mss;
+            act.push_back(s);
+            ast.push_back(stats[k]);
+            idx.push_back(k);
+            dts.push_back(mss);
         }
+        if (act.empty()) continue;
         FGCHK(substep_group(ctx, L, act, dts, ast));
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         for (size_t a = 0; a < act.size(); a++) {
@@ -1465,6 +1551,10 @@ int step_frames_lane(fg_ctx* ctx, Lane* L, const std::vector<fg_scene*>& scenes,
             t[k] += dts[a];
             count[k]++;
             if (count[k] > 50) finished[k] = 1;
+            if (act[a]->status != FG_OK) {   // quarantined by this substep
+                finished[k] = 1;
+                continue;
+            }
             if (finished[k]) {
                 act[a]->frame++;
                 if (stats[k]) stats[k]->frame_ms = ms;
@@ -1512,12 +1602,32 @@ std::vector<std::vector<size_t>> lane_plan(const std::vector<fg_scene*>& scenes,
     return plan;
 }
 
+int quarantine_result(const std::vector<fg_scene*>& scenes);
+int step_frames_split(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vector<fg_stats*>& stats,
+                      std::chrono::steady_clock::time_point t0);
+
 // Every scene one frame: on lane 0 alone, or split over the context's lanes
 // (one host thread each).  Results do not depend on the split.
 int step_frames(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vector<fg_stats*>& stats)
 {
     const auto t0 = std::chrono::steady_clock::now();
     for (fg_stats* st : stats) stats_reset(st);
+    FGCHK(step_frames_split(ctx, scenes, stats, t0));
+    return quarantine_result(scenes);
+}
+
+// FG_E_NUMERIC (with a message) when a scene of the call is quarantined.
+int quarantine_result(const std::vector<fg_scene*>& scenes)
+{
+    int bad = 0;
+    for (const fg_scene* s : scenes) bad += s->status != FG_OK;
+    if (bad) return fail(FG_E_NUMERIC, std::to_string(bad) + " scene(s) quarantined (NaN/Inf); the others completed");
+    return FG_OK;
+}
+
+int step_frames_split(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vector<fg_stats*>& stats,
+                      std::chrono::steady_clock::time_point t0)
+{
     const int lanes = std::min<int>(ctx->nlanes, (int)scenes.size());
     if (lanes <= 1) return step_frames_lane(ctx, ctx->lanes[0], scenes, stats, t0);
     FGCHK(ensure_lanes(ctx, lanes));
@@ -1581,6 +1691,83 @@ float polygon_sdf(const float* vt, int N, float px, float py)
         if ((xf && yf && zf) || (!xf && !yf && !zf)) sgn *= -1.0f;
     }
     return (float)(sgn * std::sqrt((double)sq));
+}
+
+// The solver-only scene of an isolated solve (LinearSolver::solve replacement).
+fg_scene_params solve_params(int32_t size_i, int32_t size_j, double rel_tol, int32_t max_iters)
+{
+    fg_scene_params p{};
+    p.size_i = size_i;
+    p.size_j = size_j;
+    p.dx = 1.0;
+    p.fluid_density = 1.0;
+    p.fps = 1;
+    p.solver_tolerance = rel_tol > 0 ? rel_tol : 1e-6;
+    p.pcg_iter_limit = max_iters > 0 ? max_iters : 100000;
+    return p;
+}
+
+// Upload the system into a solver-only scene, solve, download x.
+int solve_on_scene(fg_scene* s, const int8_t* material, double scale, const double* b, double* x,
+                   fg_solve_stats* stats)
+{
+    fg_ctx* ctx = s->ctx;
+    hipStream_t q = ctx->stream;
+    const int64_t C = s->C;
+    s->h.sA = scale;
+    int rc = upload_desc(s);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (rc == FG_OK) {
+        hipError_t e = hipMemcpyAsync(s->h.mat, material, C, hipMemcpyHostToDevice, q);
+        if (e == hipSuccess) e = hipMemcpyAsync(s->h.b, b, sizeof(double) * C, hipMemcpyHostToDevice, q);
+        if (e != hipSuccess) rc = fail(FG_E_DEVICE, hipGetErrorString(e));
+    }
+    int32_t it = 0, sw = 0;
+    if (rc == FG_OK) {
+        Group G;
+        rc = make_group(ctx, {s}, {}, G);
+        if (rc == FG_OK) {
+            launch_stencil(G.g.cells, q, G.d);
+            rc = run_solve(G, false);
+            it = s->hsc->total_iters;
+            sw = s->hsc->sweeps;
+            if (rc == FG_OK && s->hsc->numeric_bad) rc = fail(FG_E_NUMERIC, "NaN/Inf in the solve");
+        }
+    }
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (rc == FG_OK || rc == FG_E_NUMERIC) {
+        hipError_t e = hipMemcpyAsync(x, s->h.x64, sizeof(double) * C, hipMemcpyDeviceToHost, q);
+        if (e == hipSuccess) e = hipStreamSynchronize(q);
+        if (e != hipSuccess && rc == FG_OK) rc = fail(FG_E_DEVICE, hipGetErrorString(e));
+    }
+    if (stats) {
+        stats->iterations = it;
+        stats->sweeps = sw;
+        stats->rel_residual = s->hsc->bb > 0 ? std::sqrt(s->hsc->rr64 / s->hsc->bb) : 0.0;
+        stats->converged = stats->rel_residual <= s->p.solver_tolerance;
+        stats->solve_ms = ms;
+    }
+    return rc;
+}
+
+// Arguments of the batched entry points: the scenes (distinct, of this
+// context, full scenes) and per-scene stats pointers (NULL when stats is).
+int batch_args(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, fg_stats* stats, std::vector<fg_scene*>& v,
+               std::vector<fg_stats*>& st)
+{
+    if (!ctx || n < 0 || (n > 0 && !scenes)) return fail(FG_E_INVALID, "NULL argument");
+    v.assign(scenes, scenes + n);
+    st.assign(n, nullptr);
+    for (int32_t k = 0; k < n; k++) {
+        fg_scene* s = scenes[k];
+        if (!s) return fail(FG_E_INVALID, "scene is NULL");
+        if (s->ctx != ctx) return fail(FG_E_INVALID, "scene belongs to another context");
+        if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
+        for (int32_t q = 0; q < k; q++)
+            if (scenes[q] == s) return fail(FG_E_INVALID, "a scene appears twice in the batch");
+        if (stats) st[k] = &stats[k];
+    }
+    return FG_OK;
 }
 
 }  // namespace
@@ -1654,6 +1841,19 @@ int fg_ctx_profile_kernel(fg_ctx* ctx, int32_t kernel)
     HIPCHK(hipEventRecord(ctx->prof_ref, ctx->lanes[0]->stream));
     HIPCHK(hipStreamSynchronize(ctx->lanes[0]->stream));
     ctx->prof_kernel = kernel;
+    return FG_OK;
+}
+
+int fg_ctx_work_read(fg_ctx* ctx, int64_t* units, int32_t reset)
+{
+    if (!ctx || !units) return fail(FG_E_INVALID, "NULL argument");
+    for (int k = 0; k < FG_K_NUM; k++) units[k] = 0;
+    for (Lane* L : ctx->lanes) {
+        for (int k = 0; k < FG_K_NUM; k++) {
+            units[k] += L->work[k];
+            if (reset) L->work[k] = 0;
+        }
+    }
     return FG_OK;
 }
 
@@ -1832,12 +2032,15 @@ int fg_scene_set_state(fg_scene* s, const fg_scene_state* st)
     if (st->fluid_sdf) HIPCHK(hipMemcpyAsync(h.fsdf, st->fluid_sdf, sizeof(float) * s->C, hipMemcpyHostToDevice, q));
     if (st->solid_id) HIPCHK(hipMemcpyAsync(h.sid, st->solid_id, sizeof(int32_t) * s->C, hipMemcpyHostToDevice, q));
     else HIPCHK(hipMemsetAsync(h.sid, 0xff, sizeof(int32_t) * s->C, q));
+    // the pressure iterate (the next solve's warm start) belongs to the old state
+    HIPCHK(hipMemsetAsync(h.x64, 0, sizeof(double) * s->C, q));
     hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, q, s->d, (int)n);
     HIPCHK(hipStreamSynchronize(q));
     s->n_bound = n;
     s->n_live = n;
     s->vmax_ready = false;
     s->next_stage = 0;
+    s->status = FG_OK;
     return FG_OK;
 }
 
@@ -1874,6 +2077,7 @@ int fg_scene_get_state(fg_scene* s, fg_scene_state* st)
     FGCHK(dl(st->saved_v, h.sv, sizeof(float) * s->FV));
     FGCHK(dl(st->pressure, h.x64, sizeof(double) * s->C));
     FGCHK(dl(st->rhs, h.b, sizeof(double) * s->C));
+    FGCHK(dl(st->stencil, h.stencil, s->C));
     HIPCHK(hipStreamSynchronize(q));
     int64_t w = 0;
     for (int64_t k = 0; k < n; k++) {
@@ -1988,7 +2192,12 @@ int fg_scene_substep(fg_scene* s, float dt, fg_stats* stats)
     if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
     if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FG_E_INVALID, "dt must be positive and finite");
     stats_reset(stats);
-    return substep_group(s->ctx, s->ctx->lanes[0], {s}, {dt}, {stats});
+    if (s->status != FG_OK) {
+        if (stats) stats->status = s->status;
+        return fail(s->status, "scene is quarantined (set_state gives it a new state)");
+    }
+    FGCHK(substep_group(s->ctx, s->ctx->lanes[0], {s}, {dt}, {stats}));
+    return quarantine_result({s});
 }
 
 int fg_scene_step_frame(fg_scene* s, fg_stats* stats)
@@ -2003,6 +2212,7 @@ int fg_scene_stage(fg_scene* s, int32_t stage, float dt)
     if (!s) return fail(FG_E_INVALID, "scene is NULL");
     if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
     if (stage < 0 || stage >= FG_STAGE_NUM) return fail(FG_E_INVALID, "bad stage");
+    if (s->status != FG_OK) return fail(s->status, "scene is quarantined (set_state gives it a new state)");
     if (stage == FG_STAGE_ADVECT) {
         if (!(dt > 0.f)) return fail(FG_E_INVALID, "dt must be positive");
         set_substep_scalars(s, dt);
@@ -2015,6 +2225,10 @@ int fg_scene_stage(fg_scene* s, int32_t stage, float dt)
     FGCHK(run_stage(G, stage));
     HIPCHK(hipGetLastError());
     FGCHK(read_scalars(s));
+    if (stage == FG_STAGE_PROJECT && s->hsc->numeric_bad) {
+        quarantine(s, nullptr);
+        return fail(FG_E_NUMERIC, "NaN/Inf in the pressure solve: scene quarantined");
+    }
     if (stage == FG_STAGE_G2P) {
         s->n_live = s->hsc->n_particles - s->hsc->erased;
         s->n_bound = s->hsc->n_particles;
@@ -2025,19 +2239,38 @@ int fg_scene_stage(fg_scene* s, int32_t stage, float dt)
 
 int fg_batch_step_frame(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, fg_stats* stats)
 {
-    if (!ctx || n < 0 || (n > 0 && !scenes)) return fail(FG_E_INVALID, "NULL argument");
-    std::vector<fg_scene*> v(scenes, scenes + n);
-    std::vector<fg_stats*> st(n, nullptr);
-    for (int32_t k = 0; k < n; k++) {
-        fg_scene* s = scenes[k];
-        if (!s) return fail(FG_E_INVALID, "scene is NULL");
-        if (s->ctx != ctx) return fail(FG_E_INVALID, "scene belongs to another context");
-        if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
-        for (int32_t q = 0; q < k; q++)
-            if (scenes[q] == s) return fail(FG_E_INVALID, "a scene appears twice in the batch");
-        if (stats) st[k] = &stats[k];
-    }
+    std::vector<fg_scene*> v;
+    std::vector<fg_stats*> st;
+    FGCHK(batch_args(ctx, scenes, n, stats, v, st));
     return step_frames(ctx, v, st);
+}
+
+int fg_batch_substep(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, const float* dts, fg_stats* stats)
+{
+    std::vector<fg_scene*> v;
+    std::vector<fg_stats*> st;
+    FGCHK(batch_args(ctx, scenes, n, stats, v, st));
+    if (n > 0 && !dts) return fail(FG_E_INVALID, "dts is NULL");
+    for (int32_t k = 0; k < n; k++)
+        if (!(dts[k] > 0.f) || !std::isfinite(dts[k])) return fail(FG_E_INVALID, "dt must be positive and finite");
+    std::vector<fg_scene*> act;
+    std::vector<fg_stats*> ast;
+    std::vector<float> adt;
+    for (int32_t k = 0; k < n; k++) {
+        stats_reset(st[k]);
+        if (v[k]->status != FG_OK) {   // quarantined earlier: skipped
+            if (st[k]) {
+                st[k]->status = v[k]->status;
+                st[k]->converged = 0;
+            }
+            continue;
+        }
+        act.push_back(v[k]);
+        ast.push_back(st[k]);
+        adt.push_back(dts[k]);
+    }
+    if (!act.empty()) FGCHK(substep_group(ctx, ctx->lanes[0], act, adt, ast));
+    return quarantine_result(v);
 }
 
 int fg_ctx_set_solver_tiling(fg_ctx* ctx, int32_t target_blocks, int32_t min_chunk)
@@ -2116,50 +2349,10 @@ int fg_poisson_solve(fg_ctx* ctx, int32_t size_i, int32_t size_j, const int8_t* 
 {
     if (!ctx || !material || !b || !x) return fail(FG_E_INVALID, "NULL argument");
     if (!(scale > 0)) return fail(FG_E_INVALID, "scale must be > 0");
-    fg_scene_params p{};
-    p.size_i = size_i;
-    p.size_j = size_j;
-    p.dx = 1.0;
-    p.fluid_density = 1.0;
-    p.fps = 1;
-    p.solver_tolerance = rel_tol > 0 ? rel_tol : 1e-6;
-    p.pcg_iter_limit = max_iters > 0 ? max_iters : 100000;
+    const fg_scene_params p = solve_params(size_i, size_j, rel_tol, max_iters);
     fg_scene* s = nullptr;
     FGCHK(create_scene(ctx, &p, true, &s));
-    hipStream_t q = ctx->stream;
-    const int64_t C = s->C;
-    s->h.sA = scale;
-    int rc = upload_desc(s);
-    const auto t0 = std::chrono::steady_clock::now();
-    if (rc == FG_OK) {
-        hipError_t e = hipMemcpyAsync(s->h.mat, material, C, hipMemcpyHostToDevice, q);
-        if (e == hipSuccess) e = hipMemcpyAsync(s->h.b, b, sizeof(double) * C, hipMemcpyHostToDevice, q);
-        if (e != hipSuccess) rc = fail(FG_E_DEVICE, hipGetErrorString(e));
-    }
-    int32_t it = 0, sw = 0;
-    if (rc == FG_OK) {
-        Group G;
-        rc = make_group(ctx, {s}, {}, G);
-        if (rc == FG_OK) {
-            launch_stencil(G.g.cells, q, G.d);
-            rc = run_solve(G, false);
-            it = s->hsc->total_iters;
-            sw = s->hsc->sweeps;
-        }
-    }
-    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    if (rc == FG_OK || rc == FG_E_NUMERIC) {
-        hipError_t e = hipMemcpyAsync(x, s->h.x64, sizeof(double) * C, hipMemcpyDeviceToHost, q);
-        if (e == hipSuccess) e = hipStreamSynchronize(q);
-        if (e != hipSuccess && rc == FG_OK) rc = fail(FG_E_DEVICE, hipGetErrorString(e));
-    }
-    if (stats) {
-        stats->iterations = it;
-        stats->sweeps = sw;
-        stats->rel_residual = s->hsc->bb > 0 ? std::sqrt(s->hsc->rr64 / s->hsc->bb) : 0.0;
-        stats->converged = stats->rel_residual <= p.solver_tolerance;
-        stats->solve_ms = ms;
-    }
+    const int rc = solve_on_scene(s, material, scale, b, x, stats);
     fg_scene_destroy(s);
     return rc;
 }

@@ -693,7 +693,10 @@ __global__ void __launch_bounds__(kBlock) k_density_adjust(const SceneDev* scene
             p.x = x + (pi - pci) * s.dens_scale;
             p.y = y + (pj - pcj) * s.dens_scale;
         }
-        const int pi_ = integr(p.x), pj_ = integr(p.y);
+        // a non-finite position (NaN pressure) is dropped before the
+        // conversion to a cell index, like one outside the grid
+        const bool fin = isfinite(p.x) && isfinite(p.y);
+        const int pi_ = fin ? integr(p.x) : -1, pj_ = fin ? integr(p.y) : -1;
         if (pi_ >= 0 && pi_ < I && pj_ >= 0 && pj_ < J) key = (uint32_t)(pi_ * J + pj_);
         s.posA[k] = p;
         s.velA[k] = s.vel[k];
@@ -760,8 +763,10 @@ __global__ void __launch_bounds__(kBlock) k_rhs(const SceneDev* scenes)
         // with the V-cycle the solve starts from the previous substep's
         // pressure on fluid cells (zero elsewhere; k_cg_init falls back to
         // zero where that start is no better): the pressure changes little
-        // between substeps, so the first residual is a fraction of ||b||
-        if (!is_fluid(m) || s.precond != kPrecondMG) s.x64[c] = 0.0;
+        // between substeps, so the first residual is a fraction of ||b||.
+        // With the density correction on, x holds this substep's density
+        // solve instead, so the pressure solve starts from zero.
+        if (!is_fluid(m) || s.precond != kPrecondMG || s.density_mode != 0) s.x64[c] = 0.0;
         bb += val * val;
     }
     double v[1] = {bb};

@@ -199,7 +199,10 @@ __global__ void __launch_bounds__(kBlock) k_advect(const SceneDev* scenes, int a
         if (was_alive) {
             p = rk4(s.uf, s.vf, I, J, p, s.dt);
             if (lerp_grid(s.ssdf, I, J, 0.f, 0.f, p.x, p.y) < 0.f) p = closest_surface_point(s.ssdf, I, J, p);
-            const int pi = integr(p.x), pj = integr(p.y);
+            // a non-finite position (from a NaN velocity field) has no cell:
+            // it is killed before the conversion to a cell index
+            const bool fin = isfinite(p.x) && isfinite(p.y);
+            const int pi = fin ? integr(p.x) : -1, pj = fin ? integr(p.y) : -1;
             const bool inb = pi >= 0 && pi < I && pj >= 0 && pj < J;
             if (!inb || is_sink(s.mat[(size_t)pi * J + pj])) killed = 1;
             else key = (uint32_t)(pi * J + pj);

@@ -62,7 +62,7 @@ struct fg_server {
     uint64_t resident_bytes = 0;    // scenes held (stepping or kept)
     // summary
     std::vector<double> frame_ms, admit_wait_ms;
-    int64_t frames = 0, psub = 0, submitted = 0, completed = 0, rejected = 0;
+    int64_t frames = 0, psub = 0, submitted = 0, completed = 0, rejected = 0, failed = 0, cancelled = 0;
     double busy_ms = 0;
     uint64_t peak_bytes = 0;
     int32_t peak_resident = 0;
@@ -83,6 +83,17 @@ void drop_scene(fg_server* srv, Req& r)
     srv->resident_bytes -= r.bytes;
 }
 
+// The head request cannot be admitted, for good: it leaves the queue as
+// REJECTED with the code that refused it, and the queue moves on.
+void reject_head(fg_server* srv, Req& r, int code)
+{
+    r.status.state = FG_REQ_REJECTED;
+    r.status.error = code;
+    r.status.finish_ms = srv->now_ms();
+    srv->rejected++;
+    srv->queue.pop_front();
+}
+
 int admit(fg_server* srv, int* admitted)
 {
     *admitted = 0;
@@ -92,14 +103,26 @@ int admit(fg_server* srv, int* admitted)
         if (srv->resident_bytes + r.bytes > srv->budget) break;   // head waits: no overtaking
         fg_scene* sc = nullptr;
         int rc = fg_scene_create(srv->ctx, &r.params, &sc);
-        if (rc == FG_E_BUDGET) break;                              // context budget: wait
-        if (rc != FG_OK) return rc;
+        if (rc == FG_E_BUDGET || rc == FG_E_DEVICE) {
+            // the context's budget or the device is full: wait while this
+            // server holds scenes that will free memory, else it never fits
+            if (srv->resident_bytes > 0) break;
+            reject_head(srv, r, rc);
+            continue;
+        }
+        if (rc != FG_OK) {
+            reject_head(srv, r, rc);
+            continue;
+        }
         rc = fg_scene_init_geometry(sc, r.n_solids, r.snv.data(), r.sv.data(),
                                     r.friction.empty() ? nullptr : r.friction.data(), r.n_fluids, r.fnv.data(),
                                     r.fv.data(), r.seed);
         if (rc != FG_OK) {
+            // e.g. FG_E_CAPACITY: its fluid seeds more particles than max_particles
             fg_scene_destroy(sc);
-            return rc;
+            if (rc == FG_E_DEVICE) return rc;
+            reject_head(srv, r, rc);
+            continue;
         }
         r.scene = sc;
         srv->resident_bytes += r.bytes;
@@ -180,6 +203,7 @@ int fg_server_submit(fg_server* srv, const fg_request* q, int64_t* id_out)
     srv->submitted++;
     if (bytes > srv->budget) {
         r.status.state = FG_REQ_REJECTED;
+        r.status.error = FG_E_BUDGET;
         srv->rejected++;
     } else {
         r.status.state = FG_REQ_QUEUED;
@@ -206,12 +230,21 @@ int fg_server_tick(fg_server* srv, fg_tick_stats* st)
         for (size_t k = 0; k < n; k++) scenes[k] = srv->reqs[srv->resident[k]].scene;
         std::vector<fg_stats> stats(n);
         rc = fg_batch_step_frame(srv->ctx, scenes.data(), (int32_t)n, stats.data());
-        if (rc != FG_OK) return rc;
+        if (rc != FG_OK && rc != FG_E_NUMERIC) return rc;   // FG_E_NUMERIC: see stats[k].status
         const double t_done = srv->now_ms();
         std::vector<int64_t> still;
         for (size_t k = 0; k < n; k++) {
             Req& r = srv->reqs[srv->resident[k]];
             const fg_stats& s = stats[k];
+            if (s.status != FG_OK) {   // quarantined: this request ends, the others go on
+                r.status.state = FG_REQ_FAILED;
+                r.status.error = s.status;
+                r.status.finish_ms = t_done;
+                srv->failed++;
+                out.retired++;
+                if (!r.keep) drop_scene(srv, r);
+                continue;
+            }
             r.status.frames_done++;
             r.status.particles = s.particles;
             r.status.particle_substeps += s.particle_substeps;
@@ -271,9 +304,37 @@ int fg_server_release(fg_server* srv, int64_t id)
     auto it = srv->reqs.find(id);
     if (it == srv->reqs.end()) return bad("unknown request id");
     Req& r = it->second;
-    if (r.status.state != FG_REQ_DONE) return fg::report_error(FG_E_STATE, "request has not finished");
+    if (r.status.state != FG_REQ_DONE && r.status.state != FG_REQ_FAILED)
+        return fg::report_error(FG_E_STATE, "request has not finished (fg_server_cancel stops it)");
     drop_scene(srv, r);
-    r.status.state = FG_REQ_RELEASED;
+    if (r.status.state == FG_REQ_DONE) r.status.state = FG_REQ_RELEASED;
+    return FG_OK;
+}
+
+int fg_server_cancel(fg_server* srv, int64_t id)
+{
+    if (!srv) return bad("server is NULL");
+    auto it = srv->reqs.find(id);
+    if (it == srv->reqs.end()) return bad("unknown request id");
+    Req& r = it->second;
+    switch (r.status.state) {
+    case FG_REQ_QUEUED:
+        srv->queue.erase(std::find(srv->queue.begin(), srv->queue.end(), id));
+        break;
+    case FG_REQ_RESIDENT:
+        srv->resident.erase(std::find(srv->resident.begin(), srv->resident.end(), id));
+        drop_scene(srv, r);
+        break;
+    case FG_REQ_DONE:
+    case FG_REQ_FAILED:
+        return fg_server_release(srv, id);
+    default:
+        return fg::report_error(FG_E_STATE, "request already ended");
+    }
+    r.status.state = FG_REQ_CANCELLED;
+    r.status.error = FG_E_CANCELLED;
+    r.status.finish_ms = srv->now_ms();
+    srv->cancelled++;
     return FG_OK;
 }
 
@@ -294,6 +355,8 @@ int fg_server_summary_get(fg_server* srv, fg_server_summary* out)
     s.rejected = srv->rejected;
     s.peak_resident_bytes = srv->peak_bytes;
     s.peak_resident = srv->peak_resident;
+    s.failed = srv->failed;
+    s.cancelled = srv->cancelled;
     *out = s;
     return FG_OK;
 }

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FG_ABI_VERSION 4
+#define FG_ABI_VERSION 5
 
 enum fg_status {
     FG_OK = 0,
@@ -43,8 +43,18 @@ enum fg_status {
     FG_E_BUDGET = -3,      /* HBM budget would be exceeded */
     FG_E_CAPACITY = -4,    /* particle capacity exceeded */
     FG_E_STATE = -5,       /* call not valid in the current state */
-    FG_E_NUMERIC = -6      /* NaN/Inf detected; scene quarantined */
+    FG_E_NUMERIC = -6,     /* NaN/Inf detected: the scene is quarantined (see below) */
+    FG_E_CANCELLED = -7    /* the request was cancelled */
 };
+
+/* Quarantine.  A scene whose particle velocities or pressure solve turn
+ * non-finite is quarantined: it leaves the launch group at once (the other
+ * scenes of a batch finish their frame exactly as they would without it),
+ * its fg_stats.status is FG_E_NUMERIC, and every later step call skips it
+ * and reports FG_E_NUMERIC for it until fg_scene_set_state or
+ * fg_scene_init_geometry gives it a new state.  The reference has no
+ * counterpart (FlipSolver::project only prints when its solve fails,
+ * flipsolver2d.cpp:107-111). */
 
 /* Material codes, FlipSolver2dLib/materialgrid.h:6-13 */
 enum fg_material {
@@ -111,6 +121,12 @@ typedef struct fg_scene_state {
     float    *saved_u, *saved_v;  /* get only: grid after extrapolation, before forces */
     double   *pressure;           /* I*J, get only: last pressure solve */
     double   *rhs;                /* I*J, get only: last pressure right-hand side */
+    uint8_t  *stencil;            /* I*J, get only: the pressure matrix of the last solve
+                                   * (getPressureProjectionMatrix, flipsolver2d.cpp:797-887), one
+                                   * byte per cell: bits 0-3 fluid neighbour at i-1, i+1, j-1, j+1;
+                                   * bits 4-6 nonsolid neighbour count n; bit 7 fluid row.  Row c of
+                                   * A is s*(n x_c - sum of the flagged neighbours) on fluid rows
+                                   * and x_c elsewhere, s = dt/(rho dx^2) of that substep. */
 } fg_scene_state;
 
 typedef struct fg_stats {
@@ -129,6 +145,8 @@ typedef struct fg_stats {
     float   max_velocity;
     double  stage_ms[FG_STAGE_NUM]; /* filled when timing is enabled */
     double  frame_ms;
+    int32_t status;               /* FG_OK, or FG_E_NUMERIC: the scene is quarantined */
+    int32_t reserved;
 } fg_stats;
 
 typedef struct fg_ctx fg_ctx;
@@ -179,21 +197,50 @@ int fg_batch_autotune(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, int32_t r
                       double* rates_out);
 
 /* Per-launch HIP-event timing of one kernel family, recorded on the stream
- * the kernel runs on (the context's stream).  kernel = fg_kernel_id, or -1
- * to switch off.  fg_ctx_profile_read returns the summed event time, the
- * number of timed launches and the summed work units (cells for cell
- * kernels, particles for particle kernels) of launches that did work, then
- * resets the counters.  busy_ms is the union of the launch intervals: the
- * time some timed launch was running (launches on several lanes overlap). */
+ * the kernel runs on (the launching lane's stream).  kernel = fg_kernel_id,
+ * or -1 to switch off.  fg_ctx_profile_read returns the summed event time,
+ * the number of timed launches and the summed work units of launches that
+ * did work, then resets the counters.  Work units per family: [B] the
+ * family's algorithmic bytes (counted by the library from the scenes'
+ * particle, cell and face counts), [C] cells, [P] particles.  busy_ms is the
+ * union of the launch intervals: the time some timed launch was running
+ * (launches on several lanes overlap).  Algorithmic bytes (FP32 = 4 B, flag
+ * = 1 B; P particles, C cells, F = F_U + F_V faces, C1 level-1 cells):
+ *   P2G          16 P + 5 F            (positions, velocities; faces + validity)
+ *   SDF          8 P + 6 C             (positions; fluid SDF + material)
+ *   PARTICLES    36 P + 8 F + 4 C      (k_advect + k_g2p_count: positions and
+ *                                       velocities read and written, keys; U, V,
+ *                                       saved U, saved V; solid SDF)
+ *   MG_DOWN0     5 C + 5 C1 per V-cycle (stencil code, r; level-1 activity, r1)
+ *   MG_UP0       9 C + 4 C1 per V-cycle (code, r, z; level-1 y)
+ *   MG_RESID / MG_COARSE  as described below. */
 enum fg_kernel_id {
-    FG_K_ADVECT = 0, FG_K_SORT = 1, FG_K_P2G = 2, FG_K_SDF = 3, FG_K_LEVELSET = 4, FG_K_EXTRAPOLATE = 5,
-    FG_K_PCG_A = 6, FG_K_PCG_B = 7, FG_K_G2P = 8, FG_K_RHS = 9, FG_K_APPLY = 10,
-    FG_K_VCYCLE = 11,   /* one multigrid V-cycle (all its launches) */
-    FG_K_MG_RESID = 12, /* the V-cycle's level-1 residual launch; work units are its algorithmic bytes */
-    FG_K_MG_COARSE = 13, /* the V-cycle's coarse-levels launch (one block per scene); units: algorithmic bytes */
-    FG_K_NUM = 14
+    FG_K_ADVECT = 0,      /* [P] */
+    FG_K_SORT = 1,        /* [P] the counting sort's launches */
+    FG_K_P2G = 2,         /* [B] */
+    FG_K_SDF = 3,         /* [B] */
+    FG_K_LEVELSET = 4,    /* [C] */
+    FG_K_EXTRAPOLATE = 5, /* [C] (faces) */
+    FG_K_PCG_A = 6,       /* [C] per iteration (17 B a cell) */
+    FG_K_PCG_B = 7,       /* [C] per iteration (32 B a cell) */
+    FG_K_G2P = 8,         /* [P] */
+    FG_K_RHS = 9,         /* [C] */
+    FG_K_APPLY = 10,      /* [C] */
+    FG_K_VCYCLE = 11,     /* [C] one multigrid V-cycle (all its launches) */
+    FG_K_MG_RESID = 12,   /* [B] the V-cycle's level-1 residual launch */
+    FG_K_MG_COARSE = 13,  /* [B] the V-cycle's coarse-levels launch (one block per scene) */
+    FG_K_MG_DOWN0 = 14,   /* [B] level-0 smoothing, residual and restriction */
+    FG_K_MG_UP0 = 15,     /* [B] prolongation to level 0 and post-smoothing */
+    FG_K_PARTICLES = 16,  /* [B] k_advect and k_g2p_count together (G2P + advection) */
+    FG_K_NUM = 17
 };
 int fg_ctx_profile_kernel(fg_ctx* ctx, int32_t kernel);
+/* Work units (as above) of every kernel family, summed over all launches
+ * since the context was created or last reset (reset != 0 clears them),
+ * whether or not a family is being timed: units[FG_K_NUM].  Divided into a
+ * profiler's byte counters over the same launches, they give the HBM traffic
+ * per algorithmic byte of each kernel. */
+int fg_ctx_work_read(fg_ctx* ctx, int64_t* units, int32_t reset);
 int fg_ctx_profile_read(fg_ctx* ctx, double* total_ms, double* busy_ms, int64_t* launches, int64_t* work_units);
 /* Device-to-device copy bandwidth (read + write bytes / time) of a plain
  * copy kernel over `bytes` per buffer, best of `reps`. */
@@ -218,7 +265,8 @@ int fg_scene_set_state(fg_scene* s, const fg_scene_state* st);
 int fg_scene_get_state(fg_scene* s, fg_scene_state* st);
 int fg_scene_particle_count(fg_scene* s, int64_t* out);
 
-/* One FlipSolver::step() with the given dt. */
+/* One FlipSolver::step() with the given dt.  FG_E_NUMERIC: the scene is (or
+ * has just been) quarantined. */
 int fg_scene_substep(fg_scene* s, float dt, fg_stats* stats);
 /* One FlipSolver::stepFrame(): CFL-split substeps until the frame is done. */
 int fg_scene_step_frame(fg_scene* s, fg_stats* stats);
@@ -227,8 +275,14 @@ int fg_scene_step_frame(fg_scene* s, fg_stats* stats);
 int fg_scene_stage(fg_scene* s, int32_t stage, float dt);
 
 /* Step every scene one frame; all scenes advance together, one launch per
- * phase.  stats may be NULL or an array of n. */
+ * phase.  stats may be NULL or an array of n.  Returns FG_E_NUMERIC when a
+ * scene of the batch is quarantined (stats[k].status names it); every other
+ * scene has completed its frame. */
 int fg_batch_step_frame(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, fg_stats* stats);
+/* One FlipSolver::step() of every scene, scene k with dts[k]; the scenes
+ * advance together, one launch per phase (the substep fg_batch_step_frame
+ * repeats).  Quarantine as in fg_batch_step_frame. */
+int fg_batch_substep(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, const float* dts, fg_stats* stats);
 
 /* Isolated pressure projection (LinearSolver::solve replacement).  The matrix
  * is the reference's 5-point operator assembled from `material`
@@ -274,7 +328,9 @@ typedef struct fg_request {
 } fg_request;
 
 enum fg_request_state {
-    FG_REQ_QUEUED = 0, FG_REQ_RESIDENT = 1, FG_REQ_DONE = 2, FG_REQ_REJECTED = 3, FG_REQ_RELEASED = 4
+    FG_REQ_QUEUED = 0, FG_REQ_RESIDENT = 1, FG_REQ_DONE = 2, FG_REQ_REJECTED = 3, FG_REQ_RELEASED = 4,
+    FG_REQ_FAILED = 5,            /* quarantined while resident (error = FG_E_NUMERIC) */
+    FG_REQ_CANCELLED = 6          /* fg_server_cancel */
 };
 
 typedef struct fg_request_status {
@@ -284,6 +340,8 @@ typedef struct fg_request_status {
     int64_t particle_substeps;
     double  submit_ms, admit_ms, finish_ms;  /* server clock: ms since fg_server_create */
     double  max_frame_ms;         /* slowest frame of this request */
+    int32_t error;                /* REJECTED / FAILED: the FG_E_* code that ended it */
+    int32_t reserved;
 } fg_request_status;
 
 typedef struct fg_tick_stats {
@@ -304,18 +362,32 @@ typedef struct fg_server_summary {
     int64_t submitted, completed, rejected;
     uint64_t peak_resident_bytes;
     int32_t peak_resident;
+    int32_t reserved;
+    int64_t failed, cancelled;
 } fg_server_summary;
 
 int fg_server_create(fg_ctx* ctx, uint64_t hbm_budget_bytes, int32_t max_resident, fg_server** out);
 int fg_server_destroy(fg_server* srv);
 /* Queue a request (geometry is copied).  A request whose scene alone
- * exceeds the budget is marked FG_REQ_REJECTED at once. */
+ * exceeds the budget is marked FG_REQ_REJECTED at once; one whose admission
+ * fails for another reason (its seeding exceeds max_particles, the device
+ * cannot hold it while no other scene is resident) is marked REJECTED when
+ * it reaches the head of the queue, and the queue moves on. */
 int fg_server_submit(fg_server* srv, const fg_request* req, int64_t* id_out);
+/* Admit, step one frame of every resident scene, retire.  A scene
+ * quarantined during the frame ends its request as FG_REQ_FAILED (its scene
+ * is freed unless the request keeps it); the others go on.  Returns an
+ * error only for device or state errors. */
 int fg_server_tick(fg_server* srv, fg_tick_stats* st);
 int fg_server_status(fg_server* srv, int64_t id, fg_request_status* out);
 /* The scene of a resident or kept request (for fg_scene_get_state). */
 int fg_server_scene(fg_server* srv, int64_t id, fg_scene** out);
+/* Free the scene of a finished (DONE or FAILED) request that kept it. */
 int fg_server_release(fg_server* srv, int64_t id);
+/* Cancel a queued or resident request: it leaves the queue or the batch
+ * before the next tick and its scene is freed (state FG_REQ_CANCELLED).  A
+ * finished request that kept its scene is released instead. */
+int fg_server_cancel(fg_server* srv, int64_t id);
 int fg_server_summary_get(fg_server* srv, fg_server_summary* out);
 
 #ifdef __cplusplus
