@@ -32,7 +32,8 @@ EXPORTS = [
     "fg_server_create", "fg_server_destroy", "fg_server_submit", "fg_server_tick", "fg_server_status",
     "fg_server_scene", "fg_server_release", "fg_server_summary_get", "fg_ctx_set_batch_lanes",
     "fg_ctx_batch_lanes", "fg_batch_autotune", "fg_ctx_set_solver_tiling", "fg_ctx_set_preconditioner",
-    "fg_ctx_preconditioner", "fg_batch_substep", "fg_server_cancel", "fg_ctx_work_read",
+    "fg_ctx_preconditioner", "fg_batch_substep", "fg_server_cancel", "fg_ctx_work_read", "fg_scene_init_objects", "fg_scene_set_emitters",
+    "fg_scene_max_velocity", "fg_poisson_solve_matrix",
 ]
 MAX_LANES = 4
 REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_REJECTED, REQ_RELEASED, REQ_FAILED, REQ_CANCELLED = range(7)
@@ -69,6 +70,7 @@ class SceneState(C.Structure):
         ("material", C.c_void_p), ("solid_sdf", C.c_void_p), ("fluid_sdf", C.c_void_p),
         ("solid_id", C.c_void_p), ("particle_counts", C.c_void_p),
         ("saved_u", C.c_void_p), ("saved_v", C.c_void_p), ("pressure", C.c_void_p), ("rhs", C.c_void_p),
+        ("emitter_id", C.c_void_p), ("rng_epoch", C.c_uint32), ("next_pid", C.c_uint32),
         ("stencil", C.c_void_p),
     ]
 
@@ -82,13 +84,23 @@ class Stats(C.Structure):
         ("particles_killed", C.c_int64), ("particles_erased", C.c_int64),
         ("last_dt", C.c_float), ("max_velocity", C.c_float),
         ("stage_ms", C.c_double * 12), ("frame_ms", C.c_double),
-        ("status", C.c_int32), ("reserved", C.c_int32),
+        ("status", C.c_int32), ("reserved", C.c_int32), ("particles_reseeded", C.c_int64),
     ]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "stage_ms"}
         d["stage_ms"] = {STAGE_NAMES[i]: self.stage_ms[i] for i in range(12)}
         return d
+
+
+class Polygons(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nverts", C.c_void_p), ("verts", C.c_void_p)]
+
+
+class SceneObjects(C.Structure):
+    _fields_ = [("solids", Polygons), ("friction", C.c_void_p), ("sources", Polygons),
+                ("source_velocity", C.c_void_p), ("source_transfer", C.c_void_p), ("sinks", Polygons),
+                ("fluids", Polygons), ("seed", C.c_uint32)]
 
 
 class Request(C.Structure):
@@ -151,9 +163,13 @@ def lib():
             "fg_scene_destroy": (C.c_int, [vp]),
             "fg_scene_init_geometry": (C.c_int, [vp, i32, vp, vp, vp, i32, vp, vp, C.c_uint32]),
             "fg_scene_set_friction": (C.c_int, [vp, i32, vp]),
+            "fg_scene_init_objects": (C.c_int, [vp, C.POINTER(SceneObjects)]),
+            "fg_scene_set_emitters": (C.c_int, [vp, i32, vp, vp, C.c_uint32]),
             "fg_scene_set_state": (C.c_int, [vp, C.POINTER(SceneState)]),
             "fg_scene_get_state": (C.c_int, [vp, C.POINTER(SceneState)]),
             "fg_scene_particle_count": (C.c_int, [vp, C.POINTER(i64)]),
+            "fg_scene_max_velocity": (C.c_int, [vp, C.POINTER(f32)]),
+            "fg_poisson_solve_matrix": (C.c_int, [vp, i32, i32, vp, f64, vp, vp, f64, i32, C.POINTER(SolveStats)]),
             "fg_scene_substep": (C.c_int, [vp, f32, C.POINTER(Stats)]),
             "fg_scene_step_frame": (C.c_int, [vp, C.POINTER(Stats)]),
             "fg_scene_stage": (C.c_int, [vp, i32, f32]),
@@ -311,6 +327,36 @@ class Scene:
         _check(lib().fg_scene_init_geometry(self.h, len(solids), snv.ctypes.data, sv.ctypes.data, _ptr(fr),
                                             len(fluids), fnv.ctypes.data, fv.ctypes.data, seed))
 
+    def init_objects(self, solids=(), sources=(), sinks=(), fluids=(), friction=None, emitters=(), seed=42):
+        """First-frame init on the device (fg_scene_init_objects); emitters:
+        (vx, vy, transfer) per source polygon, velocity in m/s."""
+        keep = []
+
+        def polys(ps):
+            nv = np.array([len(q) for q in ps] or [0], np.int32)
+            v = np.ascontiguousarray(np.concatenate(ps).ravel() if len(ps) else np.zeros(2), np.float32)
+            keep.extend([nv, v])
+            return Polygons(len(ps), nv.ctypes.data, v.ctypes.data)
+
+        o = SceneObjects()
+        o.solids, o.sources, o.sinks, o.fluids = polys(solids), polys(sources), polys(sinks), polys(fluids)
+        if friction is not None:
+            fr = np.ascontiguousarray(friction, np.float32)
+            keep.append(fr)
+            o.friction = fr.ctypes.data
+        if len(emitters):
+            ev = np.ascontiguousarray([[e[0], e[1]] for e in emitters], np.float32).ravel()
+            et = np.ascontiguousarray([1 if e[2] else 0 for e in emitters], np.uint8)
+            keep.extend([ev, et])
+            o.source_velocity, o.source_transfer = ev.ctypes.data, et.ctypes.data
+        o.seed = seed
+        _check(lib().fg_scene_init_objects(self.h, C.byref(o)))
+
+    def set_emitters(self, emitters, seed=0):
+        ev = np.ascontiguousarray([[e[0], e[1]] for e in emitters] or [[0.0, 0.0]], np.float32).ravel()
+        et = np.ascontiguousarray([1 if e[2] else 0 for e in emitters] or [0], np.uint8)
+        _check(lib().fg_scene_set_emitters(self.h, len(emitters), ev.ctypes.data, et.ctypes.data, seed))
+
     def set_friction(self, friction):
         fr = np.ascontiguousarray(friction, np.float32)
         _check(lib().fg_scene_set_friction(self.h, len(fr), fr.ctypes.data))
@@ -322,10 +368,13 @@ class Scene:
             if name == "n_particles":
                 continue
             if name in st and st[name] is not None:
+                if name in ("rng_epoch", "next_pid"):
+                    setattr(s, name, int(st[name]))
+                    continue
                 dt = {"px": np.float32, "py": np.float32, "vx": np.float32, "vy": np.float32, "pid": np.uint32,
                       "u": np.float32, "v": np.float32, "u_valid": np.uint8, "v_valid": np.uint8,
                       "material": np.int8, "solid_sdf": np.float32, "fluid_sdf": np.float32,
-                      "solid_id": np.int32}.get(name)
+                      "solid_id": np.int32, "emitter_id": np.int32}.get(name)
                 if dt is None:
                     continue
                 a = np.ascontiguousarray(st[name], dt)
@@ -347,6 +396,7 @@ class Scene:
             "particle_counts": np.zeros(I * J, np.int32), "saved_u": np.zeros((I + 1) * J, np.float32),
             "saved_v": np.zeros(I * (J + 1), np.float32), "pressure": np.zeros(I * J, np.float64),
             "rhs": np.zeros(I * J, np.float64), "stencil": np.zeros(I * J, np.uint8),
+            "emitter_id": np.zeros(I * J, np.int32),
         }
         s = SceneState()
         for k, a in out.items():
@@ -355,10 +405,16 @@ class Scene:
         n = s.n_particles
         for k in ("px", "py", "vx", "vy", "pid"):
             out[k] = out[k][:n]
+        out["rng_epoch"], out["next_pid"] = s.rng_epoch, s.next_pid
         return out
 
     def particle_capacity_hint(self):
         return int(self.params.max_particles)
+
+    def max_velocity(self):
+        v = C.c_float()
+        _check(lib().fg_scene_max_velocity(self.h, C.byref(v)))
+        return v.value
 
     def particle_count(self):
         v = C.c_int64()

@@ -19,7 +19,8 @@ LIB = os.path.join(LIBDIR, "libflipgpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FG_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["fg_particles.hip", "fg_grid.hip", "fg_pcg.hip", "fg_mg.hip", "fg_api.hip", "fg_serve.cpp"]
+SOURCES = ["fg_particles.hip", "fg_grid.hip", "fg_pcg.hip", "fg_mg.hip", "fg_init.hip", "fg_api.hip",
+           "fg_serve.cpp"]
 # -ffp-contract=off: every float expression rounds as written (the kernels
 # are compared bit for bit with the CPU restatement).  Packed FP32 ops are
 # disabled for all device code.
@@ -66,7 +67,8 @@ def _build_facade() -> None:
     inc = ["-I", os.path.join(ROOT, "include"), "-I", CPP]
     link = ["-L", LIBDIR, "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
     steps = [
-        [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall"] + inc + [os.path.join(CPP, "flipsolver2d_hip.cpp")]
+        [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall"] + inc
+        + [os.path.join(CPP, "flipsolver2d_hip.cpp"), os.path.join(CPP, "json_scene.cpp")]
         + link + ["-lflipgpu", "-o", FACADE],
         [CXX, "-O2", "-std=c++17", "-Wall"] + inc + [os.path.join(CPP, "facade_demo.cpp")]
         + link + ["-lflipsolver2d_hip", "-lflipgpu", "-o", DEMO],

@@ -14,11 +14,6 @@ float lerp_host(const Grid2d<float>& g, float oi, float oj, float i, float j)
 {
     const int si = (int)g.sizeI(), sj = (int)g.sizeJ();
     auto clampf = [](float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); };
-    auto at = [&](int a, int b) {
-        a = std::min(std::max(a, 0), si - 1);
-        b = std::min(std::max(b, 0), sj - 1);
-        return g.at((size_t)a, (size_t)b);
-    };
     auto mix = [](float a, float b, float f) { return (a * (1.0f - f)) + (b * f); };
     i = clampf(i + oi, 0.f, (float)(si - 1));
     j = clampf(j + oj, 0.f, (float)(sj - 1));
@@ -28,29 +23,145 @@ float lerp_host(const Grid2d<float>& g, float oi, float oj, float i, float j)
     const int ni = fi >= 0.5f ? ci + 1 : ci - 1;
     const float iF = fi < 0.5f ? 0.5f - fi : fi - 0.5f;
     const float jF = fj < 0.5f ? 0.5f - fj : fj - 0.5f;
-    const float v1 = mix(at(ci, cj), at(ni, cj), iF);
-    const float v2 = mix(at(ci, nj), at(ni, nj), iF);
+    const float v1 = mix(g.getAt(ci, cj), g.getAt(ni, cj), iF);
+    const float v2 = mix(g.getAt(ci, nj), g.getAt(ni, nj), iF);
     return mix(v1, v2, jF);
 }
 
-void flatten(const std::vector<Geometry2d>& geos, std::vector<int32_t>& nv, std::vector<float>& v)
+// Polygons as the C-ABI takes them (fg_polygons): counts and x, y pairs.
+struct PolyPack {
+    std::vector<int32_t> nv;
+    std::vector<float> v;
+    fg_polygons view() const { return fg_polygons{(int32_t)nv.size(), nv.data(), v.data()}; }
+};
+
+PolyPack pack(const std::vector<Geometry2d>& geos)
 {
-    nv.clear();
-    v.clear();
+    PolyPack p;
     for (const Geometry2d& g : geos) {
-        nv.push_back((int32_t)g.verts().size());
-        for (const Vec3& p : g.verts()) {
-            v.push_back(p.x);
-            v.push_back(p.y);
+        const std::vector<Vec3> vs = g.verts();
+        p.nv.push_back((int32_t)vs.size());
+        for (const Vec3& q : vs) {
+            p.v.push_back(q.x());
+            p.v.push_back(q.y());
         }
     }
+    return p;
 }
 
 }  // namespace
 
+// geometry2d.cpp:54-73 (the polygon distance of Inigo Quilez's formula)
+float Geometry2d::signedDistance(float x, float y) const
+{
+    const size_t N = m_verts.size();
+    if (N == 0) return 0.f;
+    auto dot = [](float ax, float ay, float bx, float by) { return (ax * bx + ay * by) + 0.f * 0.f; };
+    const float dx0 = x - m_verts[0].x(), dy0 = y - m_verts[0].y();
+    float d = dot(dx0, dy0, dx0, dy0);
+    float s = 1.0f;
+    for (size_t i = 0, j = N - 1; i < N; j = i, i++) {
+        const float ex = m_verts[j].x() - m_verts[i].x(), ey = m_verts[j].y() - m_verts[i].y();
+        const float wx = x - m_verts[i].x(), wy = y - m_verts[i].y();
+        const float h = std::clamp(dot(wx, wy, ex, ey) / dot(ex, ey, ex, ey), 0.0f, 1.0f);
+        const float bx = wx - ex * h, by = wy - ey * h;
+        d = std::min(d, dot(bx, by, bx, by));
+        const bool c1 = y >= m_verts[i].y(), c2 = y < m_verts[j].y(), c3 = ex * wy > ey * wx;
+        if ((c1 && c2 && c3) || (!c1 && !c2 && !c3)) s *= -1.0f;
+    }
+    return (float)(s * std::sqrt((double)d));
+}
+
 Vec3 StaggeredVelocityGrid::velocityAt(float i, float j) const
 {
-    return Vec3(lerp_host(u_, 0.5f, 0.f, i, j), lerp_host(v_, 0.f, 0.5f, i, j), 0.f);
+    return Vec3(lerp_host(m_u, 0.5f, 0.f, i, j), lerp_host(m_v, 0.f, 0.5f, i, j), 0.f);
+}
+
+MarkerParticleSystem::MarkerParticleSystem(size_t gridSizeI, size_t gridSizeJ, size_t binSize)
+    : m_binSize((ssize_t)binSize), m_grid(gridSizeI, gridSizeJ),
+      m_bins(gridSizeI / binSize + (gridSizeI % binSize != 0), gridSizeJ / binSize + (gridSizeJ % binSize != 0))
+{
+    for (size_t k = 0; k < m_bins.data().size(); k++) m_bins.data()[k].m_binIdx = k;
+}
+
+std::array<ssize_t, 9> MarkerParticleSystem::binsForGridCell(ssize_t i, ssize_t j) const
+{
+    std::array<ssize_t, 9> out;
+    out.fill(-1);
+    const Index2d c = binIdxForIdx(i, j);
+    int k = 0;
+    for (ssize_t di = -1; di <= 1; di++)
+        for (ssize_t dj = -1; dj <= 1; dj++, k++)
+            if (m_bins.inBounds(c.i + di, c.j + dj)) out[k] = m_bins.linearIndex(c.i + di, c.j + dj);
+    return out;
+}
+
+void MarkerParticleSystem::assign(std::vector<Vec3> pos, std::vector<Vec3> vel, std::vector<uint32_t> ids)
+{
+    m_pos = std::move(pos);
+    m_vel = std::move(vel);
+    m_ids = std::move(ids);
+    for (ParticleBin& b : m_bins.data()) {
+        b.m_positions.clear();
+        b.m_velocities.clear();
+        b.m_ids.clear();
+    }
+    for (size_t k = 0; k < m_pos.size(); k++) {
+        const ssize_t b = gridToBinIdx((ssize_t)std::floor(m_pos[k].x()), (ssize_t)std::floor(m_pos[k].y()));
+        if (b < 0) continue;
+        ParticleBin& bin = m_bins.data()[(size_t)b];
+        bin.m_positions.push_back(m_pos[k]);
+        bin.m_velocities.push_back(m_vel[k]);
+        bin.m_ids.push_back(m_ids[k]);
+    }
+}
+
+double IndexedPressureParameterUnit::multiply(const std::vector<double>& vec, const LinearIndexable2d& idx,
+                                              double scale) const
+{
+    const ssize_t c = (ssize_t)unitIndex;
+    auto at = [&](ssize_t k) { return (k >= 0 && k < (ssize_t)vec.size()) ? vec[(size_t)k] : 0.0; };
+    return (scale * nonsolidNeighborCount) * at(c) +
+           (-scale * (fluidNeighborMask & I_NEG_NEIGHBOR_BIT)) * at(idx.linearIdxOfOffset(c, -1, 0)) +
+           (-scale * ((fluidNeighborMask & I_POS_NEIGHBOR_BIT) >> 1)) * at(idx.linearIdxOfOffset(c, 1, 0)) +
+           (-scale * ((fluidNeighborMask & J_NEG_NEIGHBOR_BIT) >> 2)) * at(idx.linearIdxOfOffset(c, 0, -1)) +
+           (-scale * ((fluidNeighborMask & J_POS_NEIGHBOR_BIT) >> 3)) * at(idx.linearIdxOfOffset(c, 0, 1));
+}
+
+void IndexedPressureParameters::multiply(const std::vector<double>& in, std::vector<double>& out) const
+{
+    out = in;
+    for (const IndexedPressureParameterUnit& u : m_data) out[u.unitIndex] = u.multiply(in, m_indexer, m_scale);
+}
+
+// getPressureProjectionMatrix (flipsolver2d.cpp:797-887): per fluid cell,
+// the neighbours (clamped at the edges) that are fluid or empty count as
+// nonsolid; a fluid neighbour is flagged when its linear offset lies inside
+// the grid (so a row-end cell couples to the next row's first cell).
+IndexedPressureParameters IndexedPressureParameters::fromMaterials(const MaterialGrid& m, double scale)
+{
+    IndexedPressureParameters out(m.linearSize(), m, scale);
+    const ssize_t I = m.sizeI(), J = m.sizeJ(), C = I * J;
+    const ssize_t di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+    const NeighborMask bit[4] = {I_NEG_NEIGHBOR_BIT, I_POS_NEIGHBOR_BIT, J_NEG_NEIGHBOR_BIT, J_POS_NEIGHBOR_BIT};
+    for (ssize_t i = 0; i < I; i++)
+        for (ssize_t j = 0; j < J; j++) {
+            if (!m.isFluid(i, j)) continue;
+            IndexedPressureParameterUnit u;
+            const ssize_t c = m.linearIndex(i, j);
+            u.unitIndex = (size_t)c;
+            for (int d = 0; d < 4; d++) {
+                if (m.isFluid(i + di[d], j + dj[d])) {
+                    u.nonsolidNeighborCount++;
+                    const ssize_t l = m.linearIdxOfOffset(c, di[d], dj[d]);
+                    u.setNeighbor(bit[d], l >= 0 && l < C);
+                } else if (m.isEmpty(i + di[d], j + dj[d])) {
+                    u.nonsolidNeighborCount++;
+                }
+            }
+            out.add(u);
+        }
+    return out;
 }
 
 void FlipSolver::check(int rc)
@@ -65,7 +176,8 @@ FlipSolver::FlipSolver(const FlipSolverParameters* p, fg_ctx* sharedCtx, int dev
     if (params_.gridSizeI <= 0 || params_.gridSizeJ <= 0) throw Error(FG_E_INVALID, "grid size must be > 0");
     if (!(params_.dx > 0.0)) params_.dx = params_.domainSizeI / params_.gridSizeI;
     if (params_.viscosityEnabled) throw Error(FG_E_INVALID, "viscosity is not part of this solver");
-    if (params_.simulationMethod != SIMULATION_LIQUID) throw Error(FG_E_INVALID, "only the liquid solver is built");
+    if (params_.simulationMethod == SIMULATION_SMOKE || params_.simulationMethod == SIMULATION_FIRE)
+        throw Error(FG_E_INVALID, "only the liquid solver is built");
     if (sharedCtx) {
         ctx_ = sharedCtx;
     } else {
@@ -77,8 +189,8 @@ FlipSolver::FlipSolver(const FlipSolverParameters* p, fg_ctx* sharedCtx, int dev
     sp.size_j = params_.gridSizeJ;
     sp.dx = params_.dx;
     sp.fluid_density = params_.fluidDensity;
-    sp.gravity_i = params_.globalAcceleration.x;
-    sp.gravity_j = params_.globalAcceleration.y;
+    sp.gravity_i = params_.globalAcceleration.x();
+    sp.gravity_j = params_.globalAcceleration.y();
     sp.pic_ratio = params_.picRatio;
     sp.cfl_number = params_.cflNumber;
     sp.particle_scale = params_.particleScale;
@@ -95,6 +207,7 @@ FlipSolver::FlipSolver(const FlipSolverParameters* p, fg_ctx* sharedCtx, int dev
         if (ownCtx_) fg_ctx_destroy(ctx_);
         throw Error(rc, msg);
     }
+    particles_ = MarkerParticleSystem(gridSizeI(), gridSizeJ(), 3);
     material_ = MaterialGrid(gridSizeI(), gridSizeJ(), EMPTY);
     velocity_ = StaggeredVelocityGrid(gridSizeI(), gridSizeJ());
     fluidSdf_ = SdfGrid(gridSizeI(), gridSizeJ());
@@ -113,14 +226,16 @@ void FlipSolver::addGeometry(Obstacle& geometry)
     obstacles_.push_back(geometry);
 }
 
-void FlipSolver::addSource(Emitter&)
+void FlipSolver::addSource(Emitter& emitter)
 {
-    throw Error(FG_E_INVALID, "emitters (SOURCE cells and reseeding) are not part of this solver");
+    if (frame_ > 0) throw Error(FG_E_STATE, "sources must be added before the first frame");
+    sources_.push_back(emitter);
 }
 
-void FlipSolver::addSink(Sink&)
+void FlipSolver::addSink(Sink& sink)
 {
-    throw Error(FG_E_INVALID, "sinks are not part of this solver");
+    if (frame_ > 0) throw Error(FG_E_STATE, "sinks must be added before the first frame");
+    sinks_.push_back(sink);
 }
 
 void FlipSolver::addInitialFluid(Emitter& emitter)
@@ -131,38 +246,81 @@ void FlipSolver::addInitialFluid(Emitter& emitter)
 
 void FlipSolver::firstFrameInit()
 {
-    std::vector<Geometry2d> solids, fluids;
-    std::vector<float> friction;
+    std::vector<Geometry2d> solids, sources, sinks, fluids;
+    std::vector<float> friction, velocity;
+    std::vector<uint8_t> transfer;
     for (const Obstacle& o : obstacles_) {
         solids.push_back(o.geometry());
         friction.push_back(o.friction());
     }
+    for (const Emitter& e : sources_) {
+        sources.push_back(e.geometry());
+        velocity.push_back(e.velocity().x());
+        velocity.push_back(e.velocity().y());
+        transfer.push_back(e.velocityTransfer() ? 1 : 0);
+    }
+    for (const Sink& s : sinks_) sinks.push_back(s.geo());
     for (const Emitter& e : initialFluid_) fluids.push_back(e.geometry());
-    std::vector<int32_t> snv, fnv;
-    std::vector<float> sv, fv;
-    flatten(solids, snv, sv);
-    flatten(fluids, fnv, fv);
-    if (snv.empty()) snv.push_back(0);
-    if (fnv.empty()) fnv.push_back(0);
-    if (sv.empty()) sv.assign(2, 0.f);
-    if (fv.empty()) fv.assign(2, 0.f);
-    check(fg_scene_init_geometry(scene_, (int32_t)solids.size(), snv.data(), sv.data(),
-                                 friction.empty() ? nullptr : friction.data(), (int32_t)fluids.size(), fnv.data(),
-                                 fv.data(), params_.seed));
+    const PolyPack ps = pack(solids), pr = pack(sources), pk = pack(sinks), pf = pack(fluids);
+    fg_scene_objects o{};
+    o.solids = ps.view();
+    o.friction = friction.empty() ? nullptr : friction.data();
+    o.sources = pr.view();
+    o.source_velocity = velocity.empty() ? nullptr : velocity.data();
+    o.source_transfer = transfer.empty() ? nullptr : transfer.data();
+    o.sinks = pk.view();
+    o.fluids = pf.view();
+    o.seed = params_.seed;
+    check(fg_scene_init_objects(scene_, &o));
     initAdditionalParameters();
     dirty_ = true;
 }
 
-void FlipSolver::step()
+void FlipSolver::runStage(int stage)
 {
-    fg_stats st{};
-    check(fg_scene_substep(scene_, stepDt_, &st));
+    check(fg_scene_stage(scene_, stage, stepDt_));
     dirty_ = true;
 }
 
-void FlipSolver::stepFrame()
+// The stage hooks: step() runs them in FlipSolver::step's order
+// (flipsolver2d.cpp:412-462).
+void FlipSolver::advect() { runStage(FG_STAGE_ADVECT); }
+void FlipSolver::rebinParticles() { runStage(FG_STAGE_REBIN); }
+void FlipSolver::densityCorrectionStage() { runStage(FG_STAGE_DENSITY); }
+void FlipSolver::particleToGrid() { runStage(FG_STAGE_P2G); }
+void FlipSolver::updateSdf() { runStage(FG_STAGE_SDF); }
+void FlipSolver::gridUpdate()
 {
-    if (frame_ == 0) firstFrameInit();
+    particleToGrid();
+    updateSdf();
+}
+void FlipSolver::extrapolateLevelset() { runStage(FG_STAGE_LEVELSET); }
+void FlipSolver::extrapolateVelocity() { runStage(FG_STAGE_EXTRAPOLATE); }
+void FlipSolver::applyBodyForces() { runStage(FG_STAGE_FORCES); }
+void FlipSolver::project()
+{
+    runStage(FG_STAGE_PROJECT);
+    runStage(FG_STAGE_SOLIDS);
+}
+void FlipSolver::particleUpdate() { runStage(FG_STAGE_G2P); }
+void FlipSolver::countParticles() { runStage(FG_STAGE_COUNT); }
+
+void FlipSolver::step()
+{
+    advect();
+    rebinParticles();
+    densityCorrectionStage();
+    gridUpdate();
+    extrapolateLevelset();
+    extrapolateVelocity();
+    applyBodyForces();
+    project();
+    particleUpdate();
+    countParticles();
+}
+
+void FlipSolver::stepFrameOnDevice()
+{
     fg_stats st{};
     check(fg_scene_step_frame(scene_, &st));
     // fg_stage -> SolverStage (flipsolver2d.cpp:412-462 stage boundaries)
@@ -178,6 +336,39 @@ void FlipSolver::stepFrame()
     stats_.resid_ = st.pressure_rel_resid;
     stepDt_ = st.last_dt;
     totalFrameMs_ += st.frame_ms;
+}
+
+// FlipSolver::stepFrame, flipsolver2d.cpp:464-500
+void FlipSolver::stepFrame()
+{
+    if (frame_ == 0) firstFrameInit();
+    if (!stagesOverridden()) {
+        stepFrameOnDevice();
+    } else {
+        // the reference's loop: a CFL dt from the fastest particle, step()
+        const float frameDt = 1.f / params_.fps;
+        float t = 0.f;
+        int count = 0;
+        bool finished = false;
+        stats_ = SolverStats();
+        while (!finished) {
+            float vel = 0.f;
+            check(fg_scene_max_velocity(scene_, &vel));
+            float dt = params_.cflNumber / (vel + 1e-15f);
+            if (t + dt >= frameDt || count == params_.maxSubsteps - 1) {
+                dt = frameDt - t;
+                finished = true;
+            } else if (t + 2.f * dt >= frameDt) {
+                dt = 0.5f * (frameDt - t);
+            }
+            stepDt_ = dt;
+            step();
+            stats_.substeps_++;
+            t += dt;
+            count++;
+            if (count > 50) break;
+        }
+    }
     frame_++;
     dirty_ = true;
 }
@@ -196,7 +387,7 @@ void FlipSolver::sync() const
     const size_t cap = I * J * (size_t)params_.particlesPerCell;
     std::vector<float> px(cap), py(cap), vx(cap), vy(cap);
     std::vector<uint32_t> id(cap);
-    std::vector<float> fsdf(I * J), ssdf(I * J);
+    std::vector<uint8_t> uval((I + 1) * J), vval(I * (J + 1));
     fg_scene_state st{};
     st.px = px.data();
     st.py = py.data();
@@ -205,20 +396,21 @@ void FlipSolver::sync() const
     st.pid = id.data();
     st.u = velocity_.velocityGridU().data().data();
     st.v = velocity_.velocityGridV().data().data();
-    st.u_valid = velocity_.uSampleValidityGrid().data().data();
-    st.v_valid = velocity_.vSampleValidityGrid().data().data();
+    st.u_valid = uval.data();
+    st.v_valid = vval.data();
     st.material = material_.data().data();
     st.fluid_sdf = fluidSdf_.data().data();
     st.solid_sdf = solidSdf_.data().data();
     check(fg_scene_get_state(scene_, &st));
+    for (size_t k = 0; k < uval.size(); k++) velocity_.uSampleValidityGrid().data()[k] = uval[k] != 0;
+    for (size_t k = 0; k < vval.size(); k++) velocity_.vSampleValidityGrid().data()[k] = vval[k] != 0;
     const size_t n = (size_t)st.n_particles;
-    particles_.pos_.resize(n);
-    particles_.vel_.resize(n);
-    particles_.ids_.assign(id.begin(), id.begin() + (ptrdiff_t)n);
+    std::vector<Vec3> pos(n), vel(n);
     for (size_t k = 0; k < n; k++) {
-        particles_.pos_[k] = Vec3(px[k], py[k], 0.f);
-        particles_.vel_[k] = Vec3(vx[k], vy[k], 0.f);
+        pos[k] = Vec3(px[k], py[k], 0.f);
+        vel[k] = Vec3(vx[k], vy[k], 0.f);
     }
+    particles_.assign(std::move(pos), std::move(vel), std::vector<uint32_t>(id.begin(), id.begin() + (ptrdiff_t)n));
     dirty_ = false;
 }
 
@@ -268,18 +460,42 @@ LinearSolver::~LinearSolver()
     if (ownCtx_ && ctx_) fg_ctx_destroy(ctx_);
 }
 
+// LinearSolver::solve (linearsolver.cpp:25-73) on the GPU: the unit list
+// becomes one matrix byte per cell (fg_poisson_solve_matrix); x starts at
+// zero as there.
 int LinearSolver::solve(const IndexedPressureParameters& m, const IndexedIPPCoefficients&, std::vector<double>& result,
                         const std::vector<double>& vec, int iterLimit, double tol)
 {
-    const size_t C = m.sizeI * m.sizeJ;
-    if (m.materials.size() != C || vec.size() != C) throw Error(FG_E_INVALID, "system size mismatch");
+    const LinearIndexable2d& ix = m.indexer();
+    const size_t C = ix.linearSize();
+    if (vec.size() != C) throw Error(FG_E_INVALID, "system size mismatch");
+    std::vector<uint8_t> code(C, 0);
+    for (const IndexedPressureParameterUnit& u : m.data()) {
+        if (u.unitIndex >= C) throw Error(FG_E_INVALID, "unit index out of range");
+        code[u.unitIndex] = (uint8_t)(0x80 | ((u.nonsolidNeighborCount & 7) << 4) | (u.fluidNeighborMask & 15));
+    }
+    double rel = tol;
+    if (mode_ == MAX_ABS) {
+        // ||r||_2 <= tol bounds every |r_k| by tol
+        double bb = 0.0;
+        for (double b : vec) bb += b * b;
+        rel = bb > 0 ? tol / std::sqrt(bb) : 1.0;
+        if (rel >= 1.0) {   // x = 0 already meets it (linearsolver.cpp:35-40)
+            result.assign(C, 0.0);
+            resid_ = 0.0;
+            iters_ = 0;
+            return 0;
+        }
+    }
     result.assign(C, 0.0);
     fg_solve_stats st{};
-    const int rc = fg_poisson_solve(ctx_, (int32_t)m.sizeI, (int32_t)m.sizeJ, m.materials.data(), m.scale, vec.data(),
-                                    result.data(), tol, iterLimit, &st);
-    if (rc != FG_OK) throw Error(rc, fg_last_error());
+    const int rc = fg_poisson_solve_matrix(ctx_, (int32_t)ix.sizeI(), (int32_t)ix.sizeJ(), code.data(), m.scale(),
+                                           vec.data(), result.data(), rel, iterLimit, &st);
+    if (rc != FG_OK && rc != FG_E_NUMERIC) throw Error(rc, fg_last_error());
     resid_ = st.rel_residual;
-    return st.iterations;
+    iters_ = st.iterations;
+    if (rc != FG_OK || !st.converged) return iterLimit;
+    return std::min(st.iterations, std::max(iterLimit - 1, 0));
 }
 
 }  // namespace fghip

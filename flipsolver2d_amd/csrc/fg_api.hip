@@ -81,6 +81,8 @@ struct Lane {
     int32_t* bflags_host = nullptr;
     int2* bmap = nullptr;           // block maps of the packed scenes
     int64_t bmap_cap = 0;
+    void* bgeo = nullptr;           // polygons of a scene set-up
+    size_t bgeo_cap = 0;
     // per-launch kernel timing
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -130,6 +132,8 @@ struct fg_scene {
     int next_stage = 0;
     int frame = 0;
     int status = FG_OK;           // FG_E_NUMERIC: quarantined until the next set_state
+    uint32_t epoch = 0;           // substeps taken (the reseed RNG's counter)
+    uint32_t next_pid = 0;        // id of the next reseeded particle
 };
 
 namespace {
@@ -297,6 +301,8 @@ void carve(fg_scene* s, Carver& cv)
     h.vvalS = cv.take<uint8_t>(FV);
     h.vvalF = cv.take<uint8_t>(FV);
     h.friction = cv.take<float>(256);
+    h.eid = cv.take<int32_t>(C);
+    h.emitters = cv.take<float4>(kMaxEmitters);
 }
 
 uint64_t arena_bytes(fg_scene* s)
@@ -373,6 +379,9 @@ int setup_scene_memory(fg_scene* s)
     h.friction_any = 0;
     h.pic = p->pic_ratio;
     h.pscale = p->particle_scale;
+    // every particle of a cell is within sqrt(0.5) = 0.70710677f of its
+    // centre, so above that radius a cell holding a particle is inside
+    h.sdf_short = p->particle_scale > 0.70710677f ? 1 : 0;
     h.g_i = p->gravity_i;
     h.g_j = p->gravity_j;
     h.dx = p->dx;
@@ -396,6 +405,7 @@ int setup_scene_memory(fg_scene* s)
         // empty, all-EMPTY scene until state arrives
         HIPCHK(hipMemsetAsync(h.mat, (int)(int8_t)FG_EMPTY, s->C, st));
         HIPCHK(hipMemsetAsync(h.sid, 0xff, sizeof(int32_t) * s->C, st));
+        HIPCHK(hipMemsetAsync(h.eid, 0xff, sizeof(int32_t) * s->C, st));
     }
     HIPCHK(hipMemcpyAsync(h.mg, &s->mgh, sizeof(MgLevels), hipMemcpyHostToDevice, st));
     FGCHK(upload_desc(s));
@@ -500,6 +510,8 @@ void set_substep_scalars(fg_scene* s, float dt)
     h.sRhs = 1.f / s->p.dx;                                            // flipsolver2d.cpp:964
     h.dens_scale = (float)((dt * dt) / (s->p.fluid_density * s->p.dx * s->p.dx));  // flipsolver2d.cpp:263
     h.dens_rhs_scale = 1.0 / dt;                                       // flipsolver2d.cpp:995
+    h.epoch = s->epoch;
+    h.next_pid = s->next_pid;
 }
 
 // ---------------------------------------------------------------------------
@@ -512,6 +524,9 @@ __global__ void k_substep_begin(const SceneDev* scenes)
     s.sc->killed = 0;
     s.sc->erased = 0;
     s.sc->density_applied = 0;
+    s.sc->reseeded = 0;
+    s.sc->reseed_dropped = 0;
+    s.sc->sdf_known = 0;
 }
 __global__ void k_set_count(const SceneDev* scenes, int n)
 {
@@ -666,6 +681,7 @@ struct Group {
     int64_t cells = 0, faces = 0, parts = 0;
     int max_inner = 0;
     int max_bands = 0;
+    bool sources = false;               // some scene has SOURCE cells (afterTransfer, reseeding)
     int mg_levels = 0;                  // most multigrid levels of a scene using the V-cycle
     int map_kinds = 0;                  // block-map kinds the solve uses (0 when maps are off)
     int64_t map_off[kMapKinds]{};       // offset of each kind's map in the lane's map buffer
@@ -697,6 +713,7 @@ void lane_destroy(Lane* L)
     if (L->bflags) hipFree(L->bflags);
     if (L->bflags_host) hipHostFree(L->bflags_host);
     if (L->bmap) hipFree(L->bmap);
+    if (L->bgeo) hipFree(L->bgeo);
     for (hipEvent_t e : L->ev_pool) hipEventDestroy(e);
     if (L->own_stream && L->stream) hipStreamDestroy(L->stream);
     delete L;
@@ -798,6 +815,7 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
         G.faces += s->FU + s->FV;
         G.parts += s->n_bound;
         G.max_inner = std::max(G.max_inner, s->h.max_inner);
+        G.sources = G.sources || s->h.has_sources;
         G.max_bands = std::max(G.max_bands, (int)std::min<int64_t>(kMaxBands, cdiv(std::max(s->I, s->J), 16) + 1));
     }
     geo_set_y(G.g, G.n);
@@ -1322,14 +1340,20 @@ int run_stage(Group& G, int stage)
         break;
     }
     case FG_STAGE_SDF: {
-        KProf kp(G, FG_K_SDF, true);
-        launch_sdf_materials(g.bins, st, d);
-        kp.done(0);
+        {
+            KProf kp(G, FG_K_SDF, true);
+            launch_sdf_materials(g.bins, st, d);
+            kp.done(0);
+        }
+        // afterTransfer (flipsolver2d.cpp:390-410), before the level set and
+        // the extrapolation
+        if (G.sources) launch_after_transfer(g.cells, st, d);
         break;
     }
     case FG_STAGE_LEVELSET: {
         KProf kp(G, FG_K_LEVELSET);
         launch_levelset_init(g.cells, st, d);
+        launch_sdf_exact(g.bins, st, d);   // only where no cell is outside
         // a band launch after the last one needed exits at once but still
         // dispatches its blocks: the first poll comes after one band more
         // than the group's scenes ran last substep, later ones every kChunk
@@ -1389,8 +1413,16 @@ int run_stage(Group& G, int stage)
         break;
     }
     case FG_STAGE_COUNT:
-        // countParticles runs inside k_g2p_count; reseeding fills SOURCE
-        // cells only (flipsolver2d.cpp:627-680), which this build has none of
+        // countParticles runs inside k_g2p_count; reseedParticles
+        // (flipsolver2d.cpp:627-680) fills the SOURCE cells of the scenes
+        // that have them: per-cell additions, their scan, the writes
+        if (!G.sources) break;
+        launch_reseed_count(g.cells, st, d);
+        launch_scan_local(g.scan, st, d);
+        launch_scan_top(g.scan1, st, d);
+        launch_scan_add(g.scan, st, d);
+        launch_reseed_write(g.cells, st, d);
+        launch_reseed_commit(g.one, st, d);
         break;
     default:
         return fail(FG_E_INVALID, "bad stage");
@@ -1432,6 +1464,8 @@ void commit_substep(Group& G, float* dts)
         const SceneScalars& sc = *s->hsc;
         if (sc.numeric_bad) quarantine(s, G.st[k]);
         const int64_t n_start = s->n_live;
+        s->epoch++;
+        s->next_pid += (uint32_t)sc.reseeded;
         s->ls_hint = sc.ls_bands;
         s->n_live = sc.n_particles - sc.erased;
         s->n_bound = sc.n_particles;
@@ -1443,6 +1477,7 @@ void commit_substep(Group& G, float* dts)
             stats->particles_killed += sc.killed;
             stats->particles_erased += sc.erased;
             stats->density_applied += sc.density_applied;
+            stats->particles_reseeded += sc.reseeded;
             stats->last_dt = dts[k];
         }
     }
@@ -1655,44 +1690,6 @@ int step_frames_split(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const s
 }
 
 
-// counter-based uniform in [0,1): 24 bits of a murmur3-finalised hash
-inline uint32_t fmix32(uint32_t h)
-{
-    h ^= h >> 16;
-    h *= 0x85ebca6bu;
-    h ^= h >> 13;
-    h *= 0xc2b2ae35u;
-    h ^= h >> 16;
-    return h;
-}
-inline float jitter(uint32_t seed, uint32_t cell, uint32_t k, uint32_t axis)
-{
-    const uint32_t h = fmix32(seed ^ fmix32(cell * 0x9e3779b1u ^ fmix32(k * 2u + axis + 0x632be5abu)));
-    return (float)(h >> 8) * (1.0f / 16777216.0f);
-}
-
-// Geometry2d::signedDistance, geometry2d.cpp:54-73 (iq polygon SDF).  The
-// zero z term of Vec3::dot is kept; sqrt is taken in double as there.
-float polygon_sdf(const float* vt, int N, float px, float py)
-{
-    auto dot = [](float ax, float ay, float bx, float by) { return (ax * bx + ay * by) + 0.f * 0.f; };
-    const float dx0 = px - vt[0], dy0 = py - vt[1];
-    float sq = dot(dx0, dy0, dx0, dy0);
-    float sgn = 1.0f;
-    for (int i = 0, j = N - 1; i < N; j = i, i++) {
-        const float ex = vt[2 * j] - vt[2 * i], ey = vt[2 * j + 1] - vt[2 * i + 1];
-        const float wx = px - vt[2 * i], wy = py - vt[2 * i + 1];
-        const float h = std::clamp(dot(wx, wy, ex, ey) / dot(ex, ey, ex, ey), 0.0f, 1.0f);
-        const float bx = wx - ex * h, by = wy - ey * h;
-        sq = std::min(sq, dot(bx, by, bx, by));
-        const bool xf = py >= vt[2 * i + 1];
-        const bool yf = py < vt[2 * j + 1];
-        const bool zf = ex * wy > ey * wx;
-        if ((xf && yf && zf) || (!xf && !yf && !zf)) sgn *= -1.0f;
-    }
-    return (float)(sgn * std::sqrt((double)sq));
-}
-
 // The solver-only scene of an isolated solve (LinearSolver::solve replacement).
 fg_scene_params solve_params(int32_t size_i, int32_t size_j, double rel_tol, int32_t max_iters)
 {
@@ -1707,9 +1704,11 @@ fg_scene_params solve_params(int32_t size_i, int32_t size_j, double rel_tol, int
     return p;
 }
 
-// Upload the system into a solver-only scene, solve, download x.
-int solve_on_scene(fg_scene* s, const int8_t* material, double scale, const double* b, double* x,
-                   fg_solve_stats* stats)
+// Upload the system into a solver-only scene, solve, download x.  The
+// matrix comes as materials (assembled on the device, k_stencil) or as its
+// bytes (one per cell, the stencil format of fg_scene_state).
+int solve_on_scene(fg_scene* s, const int8_t* material, const uint8_t* stencil, double scale, const double* b,
+                   double* x, fg_solve_stats* stats)
 {
     fg_ctx* ctx = s->ctx;
     hipStream_t q = ctx->stream;
@@ -1718,7 +1717,8 @@ int solve_on_scene(fg_scene* s, const int8_t* material, double scale, const doub
     int rc = upload_desc(s);
     const auto t0 = std::chrono::steady_clock::now();
     if (rc == FG_OK) {
-        hipError_t e = hipMemcpyAsync(s->h.mat, material, C, hipMemcpyHostToDevice, q);
+        hipError_t e = material ? hipMemcpyAsync(s->h.mat, material, C, hipMemcpyHostToDevice, q)
+                                : hipMemcpyAsync(s->h.stencil, stencil, C, hipMemcpyHostToDevice, q);
         if (e == hipSuccess) e = hipMemcpyAsync(s->h.b, b, sizeof(double) * C, hipMemcpyHostToDevice, q);
         if (e != hipSuccess) rc = fail(FG_E_DEVICE, hipGetErrorString(e));
     }
@@ -1727,7 +1727,7 @@ int solve_on_scene(fg_scene* s, const int8_t* material, double scale, const doub
         Group G;
         rc = make_group(ctx, {s}, {}, G);
         if (rc == FG_OK) {
-            launch_stencil(G.g.cells, q, G.d);
+            if (material) launch_stencil(G.g.cells, q, G.d);
             rc = run_solve(G, false);
             it = s->hsc->total_iters;
             sw = s->hsc->sweeps;
@@ -1767,6 +1767,50 @@ int batch_args(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, fg_stats* stats,
             if (scenes[q] == s) return fail(FG_E_INVALID, "a scene appears twice in the batch");
         if (stats) st[k] = &stats[k];
     }
+    return FG_OK;
+}
+
+// Polygons of a set-up packed for k_init_cells: vertex offsets, then the
+// vertices as float2, in the set order sinks, sources, solids, fluids.
+int upload_geometry(fg_scene* s, const fg_polygons* sets[4], InitGeo* geo)
+{
+    std::vector<int32_t> voff(1, 0);
+    std::vector<float2> verts;
+    int first = 0;
+    for (int q = 0; q < 4; q++) {
+        const fg_polygons& P = *sets[q];
+        if (P.n < 0 || (P.n > 0 && (!P.nverts || !P.verts))) return fail(FG_E_INVALID, "bad polygon set");
+        geo->set[q] = PolySet{first, P.n};
+        const float* v = P.verts;
+        for (int32_t k = 0; k < P.n; k++) {
+            if (P.nverts[k] < 3) return fail(FG_E_INVALID, "a polygon needs 3 vertices");
+            for (int32_t e = 0; e < P.nverts[k]; e++) verts.push_back(make_float2(v[2 * e], v[2 * e + 1]));
+            v += 2 * P.nverts[k];
+            voff.push_back((int32_t)verts.size());
+        }
+        first += P.n;
+    }
+    const size_t off_bytes = (sizeof(int32_t) * voff.size() + 255) & ~size_t(255);
+    const size_t bytes = off_bytes + sizeof(float2) * std::max<size_t>(verts.size(), 1);
+    Lane* L = s->ctx->lanes[0];
+    if (bytes > L->bgeo_cap) {
+        HIPCHK(hipStreamSynchronize(L->stream));
+        if (L->bgeo) hipFree(L->bgeo);
+        L->bgeo = nullptr;
+        L->bgeo_cap = 0;
+        HIPCHK(hipMalloc(&L->bgeo, bytes));
+        L->bgeo_cap = bytes;
+    }
+    char* base = static_cast<char*>(L->bgeo);
+    HIPCHK(hipMemcpyAsync(base, voff.data(), sizeof(int32_t) * voff.size(), hipMemcpyHostToDevice, L->stream));
+    if (!verts.empty())
+        HIPCHK(hipMemcpyAsync(base + off_bytes, verts.data(), sizeof(float2) * verts.size(), hipMemcpyHostToDevice,
+                              L->stream));
+    geo->voff = reinterpret_cast<const int32_t*>(base);
+    geo->verts = reinterpret_cast<const float2*>(base + off_bytes);
+    geo->dx = (float)s->p.dx;
+    // the host vectors are read by the copies above: wait for them
+    HIPCHK(hipStreamSynchronize(L->stream));
     return FG_OK;
 }
 
@@ -2032,6 +2076,15 @@ int fg_scene_set_state(fg_scene* s, const fg_scene_state* st)
     if (st->fluid_sdf) HIPCHK(hipMemcpyAsync(h.fsdf, st->fluid_sdf, sizeof(float) * s->C, hipMemcpyHostToDevice, q));
     if (st->solid_id) HIPCHK(hipMemcpyAsync(h.sid, st->solid_id, sizeof(int32_t) * s->C, hipMemcpyHostToDevice, q));
     else HIPCHK(hipMemsetAsync(h.sid, 0xff, sizeof(int32_t) * s->C, q));
+    if (st->emitter_id)
+        HIPCHK(hipMemcpyAsync(h.eid, st->emitter_id, sizeof(int32_t) * s->C, hipMemcpyHostToDevice, q));
+    else HIPCHK(hipMemsetAsync(h.eid, 0xff, sizeof(int32_t) * s->C, q));
+    // SOURCE cells never change class, so whether the scene has any is fixed here
+    int has_sources = 0;
+    for (int64_t c = 0; c < s->C && !has_sources; c++) has_sources = st->material[c] == (int8_t)FG_SOURCE;
+    uint32_t next_pid = st->next_pid;
+    if (next_pid == 0)
+        for (int64_t k = 0; k < n; k++) next_pid = std::max(next_pid, pid[k] + 1u);
     // the pressure iterate (the next solve's warm start) belongs to the old state
     HIPCHK(hipMemsetAsync(h.x64, 0, sizeof(double) * s->C, q));
     hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, q, s->d, (int)n);
@@ -2041,6 +2094,11 @@ int fg_scene_set_state(fg_scene* s, const fg_scene_state* st)
     s->vmax_ready = false;
     s->next_stage = 0;
     s->status = FG_OK;
+    s->epoch = st->rng_epoch;
+    s->next_pid = next_pid;
+    s->h.has_sources = has_sources;
+    FGCHK(upload_desc(s));
+    HIPCHK(hipStreamSynchronize(q));
     return FG_OK;
 }
 
@@ -2078,6 +2136,9 @@ int fg_scene_get_state(fg_scene* s, fg_scene_state* st)
     FGCHK(dl(st->pressure, h.x64, sizeof(double) * s->C));
     FGCHK(dl(st->rhs, h.b, sizeof(double) * s->C));
     FGCHK(dl(st->stencil, h.stencil, s->C));
+    FGCHK(dl(st->emitter_id, h.eid, sizeof(int32_t) * s->C));
+    st->rng_epoch = s->epoch;
+    st->next_pid = s->next_pid;
     HIPCHK(hipStreamSynchronize(q));
     int64_t w = 0;
     for (int64_t k = 0; k < n; k++) {
@@ -2093,10 +2154,89 @@ int fg_scene_get_state(fg_scene* s, fg_scene_state* st)
     return FG_OK;
 }
 
+int fg_scene_max_velocity(fg_scene* s, float* out)
+{
+    if (!s || !out) return fail(FG_E_INVALID, "NULL argument");
+    if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
+    FGCHK(ensure_vmax(s->ctx, s->ctx->lanes[0], {s}));
+    float v2;
+    std::memcpy(&v2, &s->hsc->max_v2_bits, 4);
+    *out = std::sqrt(v2);
+    return FG_OK;
+}
+
 int fg_scene_particle_count(fg_scene* s, int64_t* out)
 {
     if (!s || !out) return fail(FG_E_INVALID, "NULL argument");
     *out = s->n_live;
+    return FG_OK;
+}
+
+int fg_scene_set_emitters(fg_scene* s, int32_t n, const float* velocity, const uint8_t* transfer, uint32_t seed)
+{
+    if (!s || (n > 0 && !velocity)) return fail(FG_E_INVALID, "NULL argument");
+    if (n < 0 || n > kMaxEmitters) return fail(FG_E_INVALID, "at most 256 emitters");
+    if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
+    // per emitter the face values afterTransfer writes: velocity / dx in
+    // double, rounded to float (flipsolver2d.cpp:402-403)
+    std::vector<float4> tab(kMaxEmitters, make_float4(0.f, 0.f, 0.f, 0.f));
+    for (int32_t k = 0; k < n; k++)
+        tab[k] = make_float4((float)((double)velocity[2 * k] / s->p.dx), (float)((double)velocity[2 * k + 1] / s->p.dx),
+                             transfer && transfer[k] ? 1.f : 0.f, 0.f);
+    HIPCHK(hipMemcpyAsync(s->h.emitters, tab.data(), sizeof(float4) * kMaxEmitters, hipMemcpyHostToDevice,
+                          s->ctx->stream));
+    s->h.n_emitters = n;
+    s->h.rseed = seed;
+    FGCHK(upload_desc(s));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return FG_OK;
+}
+
+int fg_scene_init_objects(fg_scene* s, const fg_scene_objects* o)
+{
+    if (!s || !o) return fail(FG_E_INVALID, "NULL argument");
+    if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
+    if (o->solids.n > 256) return fail(FG_E_INVALID, "at most 256 obstacles");
+    if (o->sources.n > kMaxEmitters) return fail(FG_E_INVALID, "at most 256 sources");
+    const fg_polygons* sets[4] = {&o->sinks, &o->sources, &o->solids, &o->fluids};
+    InitGeo geo{};
+    FGCHK(upload_geometry(s, sets, &geo));
+    hipStream_t q = s->ctx->stream;
+    SceneDev& h = s->h;
+    // grids of a scene at rest; the cell pass writes materials, ids, SDFs
+    HIPCHK(hipMemsetAsync(h.uf, 0, sizeof(float) * s->FU, q));
+    HIPCHK(hipMemsetAsync(h.vf, 0, sizeof(float) * s->FV, q));
+    HIPCHK(hipMemsetAsync(h.uvalF, 0, s->FU, q));
+    HIPCHK(hipMemsetAsync(h.vvalF, 0, s->FV, q));
+    const dim3 cells((unsigned)cdiv(s->C, kBlock));
+    const dim3 scan((unsigned)cdiv(s->C, kScanTileCells));
+    launch_init_cells(cells, q, s->d, geo);
+    launch_scan_local(scan, q, s->d);
+    launch_scan_top(dim3(1), q, s->d);
+    launch_scan_add(scan, q, s->d);
+    int32_t total = 0;
+    HIPCHK(hipMemcpyAsync(&total, h.cellStart + s->C, sizeof(int32_t), hipMemcpyDeviceToHost, q));
+    HIPCHK(hipStreamSynchronize(q));
+    const bool fits = total <= s->cap;
+    if (fits) launch_seed(cells, q, s->d, o->seed);
+    else launch_clear_counts(cells, q, s->d);
+    const int32_t n = fits ? total : 0;
+    hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, q, s->d, (int)n);
+    HIPCHK(hipGetLastError());
+    s->n_bound = n;
+    s->n_live = n;
+    s->vmax_ready = false;
+    s->next_stage = 0;
+    s->status = FG_OK;
+    s->epoch = 0;
+    s->next_pid = (uint32_t)n;
+    s->h.has_sources = o->sources.n > 0;   // SOURCE cells can only come from sources
+    if (o->friction && o->solids.n > 0) FGCHK(fg_scene_set_friction(s, o->solids.n, o->friction));
+    else FGCHK(fg_scene_set_friction(s, 0, nullptr));
+    std::vector<float> vel(2 * std::max(o->sources.n, 0), 0.f);
+    if (o->source_velocity) std::copy(o->source_velocity, o->source_velocity + vel.size(), vel.begin());
+    FGCHK(fg_scene_set_emitters(s, o->sources.n, vel.data(), o->source_transfer, o->seed));
+    if (!fits) return fail(FG_E_CAPACITY, "seeded particles exceed max_particles");
     return FG_OK;
 }
 
@@ -2106,84 +2246,12 @@ int fg_scene_init_geometry(fg_scene* s, int32_t n_solids, const int32_t* snv, co
 {
     if (!s || (n_solids > 0 && (!snv || !sv)) || (n_fluids > 0 && (!fnv || !fv)))
         return fail(FG_E_INVALID, "NULL argument");
-    if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
-    const int I = s->I, J = s->J;
-    const int64_t C = s->C;
-    std::vector<int8_t> mat(C, (int8_t)FG_EMPTY);
-    std::vector<int32_t> sid(C, -1);
-    std::vector<float> ssdf(C, 0.f);
-    const float dxf = (float)s->p.dx;
-    // FlipSolver::updateSolids, flipsolver2d.cpp:502-530
-    for (int i = 0; i < I; i++)
-        for (int j = 0; j < J; j++) {
-            float dist = std::numeric_limits<float>::max();
-            int min_idx = 0;
-            const float* v = sv;
-            for (int k = 0; k < n_solids; k++) {
-                const float x = (float)(((float)i + 0.5) * dxf);
-                const float y = (float)(((float)j + 0.5) * dxf);
-                const float sd = polygon_sdf(v, snv[k], x, y) / dxf;
-                if (sd < dist) {
-                    min_idx = k;
-                    dist = sd;
-                }
-                v += 2 * snv[k];
-            }
-            ssdf[(size_t)i * J + j] = dist;
-            if (dist < 0) {
-                mat[(size_t)i * J + j] = (int8_t)FG_SOLID;
-                sid[(size_t)i * J + j] = min_idx;
-            }
-        }
-    // FlipSolver::updateInitialFluid, flipsolver2d.cpp:573-590
-    for (int i = 0; i < I; i++)
-        for (int j = 0; j < J; j++) {
-            const float* v = fv;
-            for (int k = 0; k < n_fluids; k++) {
-                const float x = ((float)i + 0.5f) * dxf;
-                const float y = ((float)j + 0.5f) * dxf;
-                if (polygon_sdf(v, fnv[k], x, y) <= 0.f) mat[(size_t)i * J + j] = (int8_t)FG_FLUID;
-                v += 2 * fnv[k];
-            }
-        }
-    // seedInitialFluid (flipsolver2d.cpp:682-707): particle k of a strict-fluid
-    // cell is jittered uniformly inside quadrant k%4 of the cell
-    const int ppc = s->p.particles_per_cell;
-    std::vector<float> px, py, zero;
-    std::vector<uint32_t> pid;
-    for (int i = 0; i < I; i++)
-        for (int j = 0; j < J; j++) {
-            if (mat[(size_t)i * J + j] != (int8_t)FG_FLUID) continue;
-            const uint32_t cell = (uint32_t)((size_t)i * J + j);
-            for (int k = 0; k < ppc; k++) {
-                const int q = k & 3;
-                const float qi = (float)(q >> 1), qj = (float)(q & 1);
-                pid.push_back((uint32_t)px.size());
-                px.push_back((float)i + 0.5f * (qi + jitter(seed, cell, (uint32_t)k, 0)));
-                py.push_back((float)j + 0.5f * (qj + jitter(seed, cell, (uint32_t)k, 1)));
-            }
-        }
-    const int64_t n = (int64_t)px.size();
-    if (n > s->cap) return fail(FG_E_CAPACITY, "seeded particles exceed max_particles");
-    zero.assign(n, 0.f);
-    std::vector<float> u(s->FU, 0.f), v(s->FV, 0.f), fsdf(C, 0.f);
-    fg_scene_state st{};
-    st.n_particles = n;
-    st.px = px.data();
-    st.py = py.data();
-    st.vx = zero.data();
-    st.vy = zero.data();
-    st.pid = pid.data();
-    st.u = u.data();
-    st.v = v.data();
-    st.material = mat.data();
-    st.solid_sdf = ssdf.data();
-    st.fluid_sdf = fsdf.data();
-    st.solid_id = sid.data();
-    FGCHK(fg_scene_set_state(s, &st));
-    if (friction && n_solids > 0) FGCHK(fg_scene_set_friction(s, std::min(n_solids, 256), friction));
-    else FGCHK(fg_scene_set_friction(s, 0, nullptr));
-    return FG_OK;
+    fg_scene_objects o{};
+    o.solids = fg_polygons{n_solids, snv, sv};
+    o.friction = friction;
+    o.fluids = fg_polygons{n_fluids, fnv, fv};
+    o.seed = seed;
+    return fg_scene_init_objects(s, &o);
 }
 
 int fg_scene_substep(fg_scene* s, float dt, fg_stats* stats)
@@ -2229,10 +2297,14 @@ int fg_scene_stage(fg_scene* s, int32_t stage, float dt)
         quarantine(s, nullptr);
         return fail(FG_E_NUMERIC, "NaN/Inf in the pressure solve: scene quarantined");
     }
-    if (stage == FG_STAGE_G2P) {
+    if (stage == FG_STAGE_G2P || stage == FG_STAGE_COUNT) {
         s->n_live = s->hsc->n_particles - s->hsc->erased;
         s->n_bound = s->hsc->n_particles;
         s->vmax_ready = true;
+    }
+    if (stage == FG_STAGE_COUNT) {   // the substep is complete
+        s->epoch++;
+        s->next_pid += (uint32_t)s->hsc->reseeded;
     }
     return FG_OK;
 }
@@ -2352,7 +2424,20 @@ int fg_poisson_solve(fg_ctx* ctx, int32_t size_i, int32_t size_j, const int8_t* 
     const fg_scene_params p = solve_params(size_i, size_j, rel_tol, max_iters);
     fg_scene* s = nullptr;
     FGCHK(create_scene(ctx, &p, true, &s));
-    const int rc = solve_on_scene(s, material, scale, b, x, stats);
+    const int rc = solve_on_scene(s, material, nullptr, scale, b, x, stats);
+    fg_scene_destroy(s);
+    return rc;
+}
+
+int fg_poisson_solve_matrix(fg_ctx* ctx, int32_t size_i, int32_t size_j, const uint8_t* matrix, double scale,
+                            const double* b, double* x, double rel_tol, int32_t max_iters, fg_solve_stats* stats)
+{
+    if (!ctx || !matrix || !b || !x) return fail(FG_E_INVALID, "NULL argument");
+    if (!(scale > 0)) return fail(FG_E_INVALID, "scale must be > 0");
+    const fg_scene_params p = solve_params(size_i, size_j, rel_tol, max_iters);
+    fg_scene* s = nullptr;
+    FGCHK(create_scene(ctx, &p, true, &s));
+    const int rc = solve_on_scene(s, nullptr, matrix, scale, b, x, stats);
     fg_scene_destroy(s);
     return rc;
 }

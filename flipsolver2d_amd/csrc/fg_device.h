@@ -70,6 +70,9 @@ struct SceneScalars {
     int32_t ls_bands;       // level-set BFS bands that ran this substep
     int32_t density_applied;  // density correction moved the particles this substep
     int32_t mg_stored[kMgMaxLevels];  // active rows per level that read stored coefficients
+    int32_t reseeded;       // particles reseedParticles added this substep
+    int32_t reseed_dropped; // reseeds beyond the particle capacity
+    int32_t sdf_known;      // some cell's fluid SDF is positive (extrapolateLevelsetInside has a known cell)
 };
 
 // One level of the multigrid hierarchy (levels >= 1; level 0 is the
@@ -114,6 +117,7 @@ struct SceneDev {
     double tol;             // FP64 relative residual target
     int32_t max_inner;      // inner iteration cap per sweep
     int32_t active;         // 0: skipped by every kernel of a batched launch
+    int32_t sdf_short;      // particleScale > sqrt(0.5): a cell holding a particle is inside
     int32_t density_mode;   // 0 off, 1 gated on convergence, 2 on
     float cell_vol;         // (float)(dx^3): updateDensityGridThread's cell volume
     float pmass;            // rho * cell_vol / ppc as a float
@@ -162,7 +166,39 @@ struct SceneDev {
     float* fpart;           // per-block max |v|^2 [max_pblocks]
     float* friction;
     SceneScalars* sc;
+    // sources: emitter id per cell (-1 none), per emitter the source
+    // velocity as face values (vx/dx, vy/dx) and its transfer flag
+    int32_t* eid;
+    float4* emitters;       // [kMaxEmitters]: (u, v, transfer, 0)
+    int32_t n_emitters;
+    int32_t has_sources;    // some cell is a SOURCE cell (fixed after set-up)
+    uint32_t rseed;         // reseed RNG seed
+    uint32_t epoch;         // substeps taken before this one (the reseed RNG's counter)
+    uint32_t next_pid;      // id of this substep's first reseeded particle
 };
+constexpr int kMaxEmitters = 256;
+
+// counter-based uniform in [0,1): 24 bits of a murmur3-finalised hash of
+// (seed, cell, k, axis); the same function seeds and reseeds particles
+__host__ __device__ __forceinline__ uint32_t fmix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+__host__ __device__ __forceinline__ float jitter(uint32_t seed, uint32_t cell, uint32_t k, uint32_t axis)
+{
+    const uint32_t h = fmix32(seed ^ fmix32(cell * 0x9e3779b1u ^ fmix32(k * 2u + axis + 0x632be5abu)));
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+// the reseed RNG's seed for one substep
+__host__ __device__ __forceinline__ uint32_t reseed_key(uint32_t seed, uint32_t epoch)
+{
+    return fmix32(seed ^ fmix32(epoch * 0x9e3779b1u + 0x85ebca77u));
+}
 
 // ---------------------------------------------------------------------------
 // Arithmetic helpers (FlipSolver2dLib/mathfuncs.cpp, grid2d.h)

@@ -225,11 +225,13 @@ __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
 // and combine the minima by lane shuffles (min is exact and order-free, so
 // the result equals the per-cell loop bit for bit).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_sdf_materials(const SceneDev* scenes)
+// kExact: the fallback pass (k_sdf_exact) -- every bin scans its window and
+// only the SDF is written.  Otherwise a bin whose cells all hold particles
+// (with the shortcut on) skips its scan: each of its cells is inside.
+template <bool kExact>
+__device__ __forceinline__ void sdf_bins(const SceneDev& s)
 {
     static_assert(kSdfLanes == 4, "the shuffle reduction assumes 4 lanes per bin");
-    const SceneDev& s = scenes[blockIdx.y];
-    if (!s.active) return;
     const int I = s.I, J = s.J;
     const int nbI = (I + 2) / 3, nbJ = (J + 2) / 3;
     const int sub = threadIdx.x & (kSdfLanes - 1);
@@ -260,6 +262,29 @@ __global__ void __launch_bounds__(kBlock) k_sdf_materials(const SceneDev* scenes
         const int b = cs[rc * J + c0], e = cs[rc * J + c1];
         rb[q] = r < r1 ? b : 0;
         re[q] = r < r1 ? e : 0;
+    }
+    // Every particle of a cell lies within sqrt(0.5) of its centre, so with
+    // particleScale above that (sdf_short) a cell holding a particle has
+    // sdf < 0 whatever its window holds; extrapolateLevelsetInside then
+    // overwrites its value (when some cell is outside, which k_sdf_exact
+    // checks).  A bin whose cells all hold particles skips its scan.
+    bool full = false;
+    if (!kExact && s.sdf_short) {
+        int have = 0;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {   // clamped cells: loads unconditional
+            const int i = min(i0 + a, I - 1);
+            const int j1 = min(j0 + 3, J);
+            const int cb = cs[i * J + j0], e0 = cs[i * J + min(j0 + 1, J)], e1 = cs[i * J + min(j0 + 2, J)],
+                      e2 = cs[i * J + j1];
+            have += (e0 > cb || i0 + a >= I) + (e1 > e0 || j0 + 1 >= J || i0 + a >= I) +
+                    (e2 > e1 || j0 + 2 >= J || i0 + a >= I);
+        }
+        full = have == 9;
+        if (full) {
+#pragma unroll
+            for (int q = 0; q < NR; q++) re[q] = rb[q];
+        }
     }
 #pragma unroll
     for (int q = 0; q < NR; q++) {
@@ -305,7 +330,12 @@ __global__ void __launch_bounds__(kBlock) k_sdf_materials(const SceneDev* scenes
         const int i = i0 + a, j = j0 + b;
         if (i >= I || j >= J) continue;
         const int c = i * J + j;
-        const float sdf = sqrtf(d2[a][b]) - s.pscale;
+        // a skipped bin's cells are inside: any negative value holds their place
+        const float sdf = full ? -s.pscale : sqrtf(d2[a][b]) - s.pscale;
+        if (kExact) {
+            if (s.cellStart[c + 1] > s.cellStart[c]) s.fsdf[c] = sdf;   // the cells the fast pass skipped
+            continue;
+        }
         s.fsdf[c] = sdf;
         int8_t m = s.mat[c];
         if (sdf < 0.f) {
@@ -316,6 +346,24 @@ __global__ void __launch_bounds__(kBlock) k_sdf_materials(const SceneDev* scenes
         s.mat[c] = m;
         s.pcount[c] = min(s.cellStart[c + 1] - s.cellStart[c], 2 * s.ppc);
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_sdf_materials(const SceneDev* scenes)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active) return;
+    sdf_bins<false>(s);
+}
+
+// The exact SDF of every cell that holds particles, for a scene in which no
+// cell is outside (sdf > 0): extrapolateLevelsetInside then has no known
+// cell and keeps every value, so the values the fast pass skipped must be
+// the true ones.  Runs after k_levelset_init; otherwise exits at once.
+__global__ void __launch_bounds__(kBlock) k_sdf_exact(const SceneDev* scenes)
+{
+    const SceneDev& s = scenes[blockIdx.y];
+    if (!s.active || !s.sdf_short || cst(s.sc)->sdf_known) return;
+    sdf_bins<true>(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -595,8 +643,10 @@ __global__ void __launch_bounds__(kBlock) k_levelset_init(const SceneDev* scenes
     if (!s.active) return;
     const int C = s.I * s.J;
     const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c < C) s.marker[c] = s.fsdf[c] > 0.f ? 0 : kMaxLevel;
+    const bool known = c < C && s.fsdf[c] > 0.f;
+    if (c < C) s.marker[c] = known ? 0 : kMaxLevel;
     if (c < kMaxBands) s.lsFlags[c] = 0;
+    if (__syncthreads_or(known) && threadIdx.x == 0) s.sc->sdf_known = 1;   // some cell is outside
 }
 
 // One band of extrapolateLevelsetInside: levels L0+1 .. L0+16.  Runs only if
@@ -851,6 +901,7 @@ namespace fg {
 FG_LAUNCH0(stencil)
 FG_LAUNCH0(p2g)
 FG_LAUNCH0(sdf_materials)
+FG_LAUNCH0(sdf_exact)
 FG_LAUNCH1(extrapolate, int, which)
 FG_LAUNCH0(levelset_init)
 FG_LAUNCH1(levelset_band, int, band)

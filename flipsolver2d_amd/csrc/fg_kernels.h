@@ -66,6 +66,7 @@ constexpr int kScanTileCells = kBlock * 16;
 FG_DECLARE_LAUNCH(stencil);
 FG_DECLARE_LAUNCH(p2g);
 FG_DECLARE_LAUNCH(sdf_materials);
+FG_DECLARE_LAUNCH(sdf_exact);
 FG_DECLARE_LAUNCH(extrapolate, int which);
 FG_DECLARE_LAUNCH(levelset_init);
 FG_DECLARE_LAUNCH(levelset_band, int band);
@@ -75,6 +76,27 @@ FG_DECLARE_LAUNCH(apply_solids);
 FG_DECLARE_LAUNCH(density_rhs);
 FG_DECLARE_LAUNCH(density_adjust);
 constexpr int kBfsTile = 32;
+
+// fg_init.hip
+// Polygons of a scene's set-up, by set: 0 sinks, 1 sources, 2 solids,
+// 3 initial fluid (firstFrameInit's pass order); polygon k has the vertices
+// verts[voff[k] .. voff[k+1]) in metres.
+struct PolySet {
+    int32_t first, n;
+};
+struct InitGeo {
+    PolySet set[4];
+    const int32_t* voff;
+    const float2* verts;
+    float dx;
+};
+void launch_init_cells(dim3 grid, hipStream_t st, const SceneDev* scenes, const InitGeo& geo);
+void launch_seed(dim3 grid, hipStream_t st, const SceneDev* scenes, uint32_t seed);
+FG_DECLARE_LAUNCH(clear_counts);
+FG_DECLARE_LAUNCH(after_transfer);
+FG_DECLARE_LAUNCH(reseed_count);
+FG_DECLARE_LAUNCH(reseed_write);
+FG_DECLARE_LAUNCH(reseed_commit);
 
 // fg_pcg.hip
 FG_DECLARE_LAUNCH(refine_residual, int spec);

@@ -12,8 +12,10 @@
  * Reference interfaces replaced (paths relative to the reference repo):
  *   FlipSolver::FlipSolver(const FlipSolverParameters*)  FlipSolver2dLib/flipsolver2d.cpp:27-86
  *       -> fg_scene_create
- *   FlipSolver::firstFrameInit / updateSolids / updateInitialFluid / seedInitialFluid
- *       FlipSolver2dLib/flipsolver2d.cpp:502-590, 682-707, 788-795  -> fg_scene_init_geometry
+ *   FlipSolver::firstFrameInit / updateSinks / updateSources / updateSolids /
+ *       updateInitialFluid / seedInitialFluid, FlipSolver::addGeometry / addSource / addSink /
+ *       addInitialFluid  FlipSolver2dLib/flipsolver2d.cpp:502-625, 682-707, 788-795
+ *       -> fg_scene_init_objects (fg_scene_init_geometry: solids and fluid only)
  *   FlipSolver::stepFrame()          FlipSolver2dLib/flipsolver2d.cpp:464-500 -> fg_scene_step_frame
  *   FlipSolver::step() (protected)   FlipSolver2dLib/flipsolver2d.cpp:412-462 -> fg_scene_substep
  *   FlipSolver::advect/particleToGrid/updateSdf/project/particleUpdate/...
@@ -121,6 +123,10 @@ typedef struct fg_scene_state {
     float    *saved_u, *saved_v;  /* get only: grid after extrapolation, before forces */
     double   *pressure;           /* I*J, get only: last pressure solve */
     double   *rhs;                /* I*J, get only: last pressure right-hand side */
+    int32_t  *emitter_id;         /* I*J, -1 = none: the source (index into the emitter table)
+                                   * of each SOURCE cell; NULL on set: none */
+    uint32_t  rng_epoch;          /* substeps taken: the reseed RNG's counter */
+    uint32_t  next_pid;           /* id of the next reseeded particle (0 on set: max id + 1) */
     uint8_t  *stencil;            /* I*J, get only: the pressure matrix of the last solve
                                    * (getPressureProjectionMatrix, flipsolver2d.cpp:797-887), one
                                    * byte per cell: bits 0-3 fluid neighbour at i-1, i+1, j-1, j+1;
@@ -147,6 +153,7 @@ typedef struct fg_stats {
     double  frame_ms;
     int32_t status;               /* FG_OK, or FG_E_NUMERIC: the scene is quarantined */
     int32_t reserved;
+    int64_t particles_reseeded;   /* reseedParticles: added in SOURCE cells */
 } fg_stats;
 
 typedef struct fg_ctx fg_ctx;
@@ -251,19 +258,53 @@ int fg_scene_bytes(const fg_scene_params* p, uint64_t* out);
 int fg_scene_create(fg_ctx* ctx, const fg_scene_params* p, fg_scene** out);
 int fg_scene_destroy(fg_scene* s);
 
-/* First-frame init from polygons (vertices in metres, x along I): solid SDF,
- * solid ids and materials, then quadrant-jittered seeding of strict-fluid
- * cells from a counter-based RNG.  friction[k] belongs to solid k. */
+/* Polygons: polygon k has nverts[k] >= 3 vertices, (x, y) pairs in metres
+ * (x along I), concatenated in verts. */
+typedef struct fg_polygons {
+    int32_t n;
+    const int32_t* nverts;
+    const float* verts;
+} fg_polygons;
+
+/* A scene's objects (Obstacle, Emitter, Sink, initial fluid; the reference's
+ * addGeometry / addSource / addSink / addInitialFluid). */
+typedef struct fg_scene_objects {
+    fg_polygons solids;
+    const float* friction;        /* [solids.n] or NULL: obstacle friction */
+    fg_polygons sources;
+    const float* source_velocity; /* [2 * sources.n] m/s or NULL: Emitter::velocity */
+    const uint8_t* source_transfer; /* [sources.n] or NULL: Emitter::velocityTransfer */
+    fg_polygons sinks;
+    fg_polygons fluids;
+    uint32_t seed;                /* seeding and reseeding RNG */
+} fg_scene_objects;
+
+/* First-frame init on the device (FlipSolver::firstFrameInit): the cell
+ * passes updateSinks, updateSources, updateSolids, updateInitialFluid in the
+ * reference's order (materials, solid SDF and ids, emitter ids), then
+ * seedInitialFluid: ppc particles per strict-fluid cell, particle k jittered
+ * in quadrant k % 4 by a counter-based RNG, ids in (cell, k) order.  Resets
+ * the grids, the quarantine and the reseed counter.  FG_E_CAPACITY if the
+ * seeding exceeds max_particles (the scene is then empty). */
+int fg_scene_init_objects(fg_scene* s, const fg_scene_objects* objects);
+/* fg_scene_init_objects with solids and initial fluid only. */
 int fg_scene_init_geometry(fg_scene* s,
                            int32_t n_solids, const int32_t* solid_nverts, const float* solid_verts,
                            const float* friction,
                            int32_t n_fluids, const int32_t* fluid_nverts, const float* fluid_verts,
                            uint32_t seed);
+/* The emitter table of the SOURCE cells (afterTransfer, flipsolver2d.cpp:390-410;
+ * reseedParticles, 627-680): per emitter its velocity (m/s, 2 floats) and
+ * whether it is transferred; the reseed RNG seed.  At most 256 emitters. */
+int fg_scene_set_emitters(fg_scene* s, int32_t n, const float* velocity, const uint8_t* transfer, uint32_t seed);
 /* Obstacle friction table (updateVelocityFromSolids, flipsolver2d.cpp:1077-1104). */
 int fg_scene_set_friction(fg_scene* s, int32_t n, const float* friction);
 int fg_scene_set_state(fg_scene* s, const fg_scene_state* st);
 int fg_scene_get_state(fg_scene* s, fg_scene_state* st);
 int fg_scene_particle_count(fg_scene* s, int64_t* out);
+/* FlipSolver::maxParticleVelocity (flipsolver2d.cpp:1560-1574): the largest
+ * particle speed (cells/s), as the next substep's CFL split reads it. */
+int fg_scene_max_velocity(fg_scene* s, float* out);
 
 /* One FlipSolver::step() with the given dt.  FG_E_NUMERIC: the scene is (or
  * has just been) quarantined. */
@@ -297,6 +338,13 @@ typedef struct fg_solve_stats {
 } fg_solve_stats;
 int fg_poisson_solve(fg_ctx* ctx, int32_t size_i, int32_t size_j, const int8_t* material, double scale,
                      const double* b, double* x, double rel_tol, int32_t max_iters, fg_solve_stats* stats);
+/* The same solve with the matrix given as its bytes, one per cell in the
+ * stencil format of fg_scene_state (bits 0-3 fluid neighbours at i-1, i+1,
+ * j-1, j+1 by linear offset; bits 4-6 nonsolid count; bit 7 fluid row): the
+ * rows of the reference's IndexedPressureParameters unit list
+ * (pressuredata.h:97-151), for LinearSolver::solve callers that hold one. */
+int fg_poisson_solve_matrix(fg_ctx* ctx, int32_t size_i, int32_t size_j, const uint8_t* matrix, double scale,
+                            const double* b, double* x, double rel_tol, int32_t max_iters, fg_solve_stats* stats);
 
 /* ------------------------------------------------------------------------
  * Serving: continuous batching of scene simulations on one device.

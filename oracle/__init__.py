@@ -39,6 +39,8 @@ class OrcParams(C.Structure):
         ("solver_mode", C.c_int32), ("density_mode", C.c_int32), ("threads", C.c_int32),
         ("project_tol", C.c_double),
         ("n_obstacles", C.c_int32), ("friction", C.POINTER(C.c_float)),
+        ("n_emitters", C.c_int32), ("emitter_velocity", C.c_void_p), ("emitter_transfer", C.c_void_p),
+        ("reseed_seed", C.c_uint32),
     ]
 
 
@@ -53,6 +55,7 @@ class OrcState(C.Structure):
         ("solid_id", C.c_void_p), ("particle_counts", C.c_void_p),
         ("saved_u", C.c_void_p), ("saved_v", C.c_void_p),
         ("pressure", C.c_void_p), ("rhs", C.c_void_p), ("density", C.c_void_p),
+        ("emitter_id", C.c_void_p), ("rng_epoch", C.c_uint32), ("next_pid", C.c_uint32),
     ]
 
 
@@ -151,6 +154,11 @@ def _declare_oracle(L):
         "orc_init_geometry": (None, [P, S, C.c_int32, _i32p, _f32p, C.c_int32, _i32p, _f32p]),
         "orc_seed_particles": (C.c_int64, [P, S, C.c_uint32]),
         "orc_jitter": (C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+        "orc_init_objects": (None, [P, S, C.c_int32, _i32p, _f32p, C.c_int32, _i32p, _f32p, C.c_int32, _i32p, _f32p,
+                                    C.c_int32, _i32p, _f32p]),
+        "orc_after_transfer": (None, [P, S]),
+        "orc_reseed": (C.c_int64, [P, S]),
+        "orc_reseed_jitter": (C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -194,7 +202,9 @@ def _declare_ref(L):
 
 def make_params(n_i, n_j=None, *, dx=None, ppc=4, density=1000.0, gravity=(9.81, 0.0), pic=0.03, cfl=5.0,
                 particle_scale=0.8, fps=30, max_substeps=10, pcg_iter_limit=200,
-                solver=SOLVER_CONVERGED, density_mode=DENSITY_OFF, threads=1, tol=None, friction=(0.0,)):
+                solver=SOLVER_CONVERGED, density_mode=DENSITY_OFF, threads=1, tol=None, friction=(0.0,),
+                emitters=(), reseed_seed=0):
+    """emitters: (vx, vy, transfer) per source, velocity in m/s."""
     n_j = n_i if n_j is None else n_j
     p = OrcParams()
     p.size_i, p.size_j = n_i, n_j
@@ -213,6 +223,13 @@ def make_params(n_i, n_j=None, *, dx=None, ppc=4, density=1000.0, gravity=(9.81,
     p._friction_keep = fr
     p.n_obstacles = len(fr)
     p.friction = fr.ctypes.data_as(C.POINTER(C.c_float))
+    ev = np.ascontiguousarray([[e[0], e[1]] for e in emitters] or [[0.0, 0.0]], np.float32).ravel()
+    et = np.ascontiguousarray([1 if e[2] else 0 for e in emitters] or [0], np.uint8)
+    p._emitter_keep = (ev, et)
+    p.n_emitters = len(emitters)
+    p.emitter_velocity = ev.ctypes.data
+    p.emitter_transfer = et.ctypes.data
+    p.reseed_seed = reseed_seed
     return p
 
 
@@ -241,12 +258,13 @@ class State:
         self.pressure = np.zeros(I * J, np.float64)
         self.rhs = np.zeros(I * J, np.float64)
         self.density = np.zeros(I * J, np.float32)
+        self.emitter_id = np.full(I * J, -1, np.int32)
         self.c = OrcState()
         self.c.capacity = capacity
         self.c.n_particles = 0
         for name in ("px", "py", "vx", "vy", "pid", "u", "v", "u_valid", "v_valid", "material", "solid_sdf",
                      "fluid_sdf", "solid_id", "particle_counts", "saved_u", "saved_v", "pressure", "rhs",
-                     "density"):
+                     "density", "emitter_id"):
             setattr(self.c, name, getattr(self, name).ctypes.data)
 
     @property
@@ -264,9 +282,10 @@ class State:
         o = State(self.I, self.J, len(self.px))
         for name in ("px", "py", "vx", "vy", "pid", "u", "v", "u_valid", "v_valid", "material", "solid_sdf",
                      "fluid_sdf", "solid_id", "particle_counts", "saved_u", "saved_v", "pressure", "rhs",
-                     "density"):
+                     "density", "emitter_id"):
             getattr(o, name)[...] = getattr(self, name)
         o.n = self.n
+        o.c.rng_epoch, o.c.next_pid = self.c.rng_epoch, self.c.next_pid
         return o
 
 
@@ -302,6 +321,27 @@ def init_scene(params, walls, fluids, seed=42, capacity=None):
     L.orc_init_geometry(C.byref(params), st.ref(), len(walls), snv, sv, len(fluids), fnv, fv)
     n = L.orc_seed_particles(C.byref(params), st.ref(), seed)
     assert n >= 0, "particle capacity exceeded"
+    st.c.next_pid = n
+    return st
+
+
+def _polys(ps):
+    nv = np.array([len(q) for q in ps] or [0], np.int32)
+    v = np.ascontiguousarray(np.concatenate(ps).ravel() if len(ps) else np.zeros(2), np.float32)
+    return len(ps), nv, v
+
+
+def init_scene_objects(params, solids=(), sources=(), sinks=(), fluids=(), seed=42, capacity=None):
+    """firstFrameInit (sinks, sources, solids, initial fluid) + seeding through the oracle."""
+    L = lib()
+    I, J = params.size_i, params.size_j
+    if capacity is None:
+        capacity = I * J * params.particles_per_cell
+    st = State(I, J, capacity)
+    L.orc_init_objects(C.byref(params), st.ref(), *_polys(solids), *_polys(sources), *_polys(sinks), *_polys(fluids))
+    n = L.orc_seed_particles(C.byref(params), st.ref(), seed)
+    assert n >= 0, "particle capacity exceeded"
+    st.c.next_pid = n
     return st
 
 

@@ -1288,7 +1288,8 @@ int32_t orc_substep(const OrcParams* p, OrcState* s, float dt, OrcStats* st)
     orc_fluid_sdf(p, s, cs.data());
     orc_update_materials(p, s);
     stage(5);
-    /* 6. afterTransfer (no sources), level set, extrapolate */
+    /* 6. afterTransfer, level set, extrapolate */
+    orc_after_transfer(p, s);
     orc_extrapolate_levelset_inside(s->fluid_sdf, I, J);
     orc_extrapolate(s->u, s->u_valid, I + 1, J, 10);
     orc_extrapolate(s->v, s->v_valid, I, J + 1, 10);
@@ -1323,6 +1324,8 @@ int32_t orc_substep(const OrcParams* p, OrcState* s, float dt, OrcStats* st)
     stage(10);
     /* 10. count + reseed (PARTICLE_RESEED); reseeding only fills SOURCE cells */
     const int32_t erased = orc_count_particles(p, s, cs.data());
+    orc_reseed(p, s);
+    s->rng_epoch++;
     stage(11);
     if (st) {
         st->particles_killed += killed;
@@ -1422,6 +1425,150 @@ float orc_jitter(uint32_t seed, uint32_t cell, uint32_t k, uint32_t axis)
 {
     const uint32_t h = fmix32(seed ^ fmix32(cell * 0x9e3779b1u ^ fmix32(k * 2u + axis + 0x632be5abu)));
     return static_cast<float>(h >> 8) * (1.0f / 16777216.0f);
+}
+
+/* firstFrameInit's cell passes (flipsolver2d.cpp:788-795) in the reference's
+ * order: updateSinks (555-571), updateSources (532-553), updateSolids
+ * (502-530), updateInitialFluid (573-590).  Each pass's coordinate
+ * arithmetic is the reference's own (float, or double where it mixes in
+ * 0.5 as a double). */
+void orc_init_objects(const OrcParams* p, OrcState* s, int32_t n_solids, const int32_t* snv, const float* sv,
+                      int32_t n_sources, const int32_t* srnv, const float* srv, int32_t n_sinks,
+                      const int32_t* sknv, const float* skv, int32_t n_fluids, const int32_t* fnv, const float* fv)
+{
+    const long I = p->size_i, J = p->size_j;
+    const int64_t C = I * J;
+    for (int64_t c = 0; c < C; c++) {
+        s->material[c] = (int8_t)ORC_EMPTY;
+        s->solid_id[c] = -1;
+        s->solid_sdf[c] = 0.f;
+        if (s->emitter_id) s->emitter_id[c] = -1;
+    }
+    const float dx = static_cast<float>(p->dx);
+    /* updateSinks */
+    for (long i = 0; i < I; i++)
+        for (long j = 0; j < J; j++) {
+            const float* v = skv;
+            for (int32_t k = 0; k < n_sinks; k++) {
+                if (orc_polygon_sdf(v, sknv[k], (static_cast<float>(i) + 0.5f) * dx,
+                                    (static_cast<float>(j) + 0.5f) * dx) <= 0.f)
+                    s->material[i * J + j] = (int8_t)ORC_SINK;
+                v += 2 * sknv[k];
+            }
+        }
+    /* updateSources */
+    const float hdx = static_cast<float>(dx / 2.0);
+    for (long i = 0; i < I; i++)
+        for (long j = 0; j < J; j++) {
+            const float* v = srv;
+            for (int32_t k = 0; k < n_sources; k++) {
+                if (orc_polygon_sdf(v, srnv[k], static_cast<float>(i) * dx + hdx, static_cast<float>(j) * dx + hdx) <=
+                    0.f) {
+                    s->material[i * J + j] = (int8_t)ORC_SOURCE;
+                    if (s->emitter_id) s->emitter_id[i * J + j] = k;
+                }
+                v += 2 * srnv[k];
+            }
+        }
+    /* updateSolids */
+    for (long i = 0; i < I; i++)
+        for (long j = 0; j < J; j++) {
+            float dist = std::numeric_limits<float>::max();
+            int min_idx = 0;
+            const float* v = sv;
+            for (int32_t k = 0; k < n_solids; k++) {
+                const float x = static_cast<float>((static_cast<float>(i) + 0.5) * dx);
+                const float y = static_cast<float>((static_cast<float>(j) + 0.5) * dx);
+                const float sd = orc_polygon_sdf(v, snv[k], x, y) / dx;
+                if (sd < dist) {
+                    min_idx = k;
+                    dist = sd;
+                }
+                v += 2 * snv[k];
+            }
+            s->solid_sdf[i * J + j] = dist;
+            if (dist < 0) {
+                s->material[i * J + j] = (int8_t)ORC_SOLID;
+                s->solid_id[i * J + j] = min_idx;
+            }
+        }
+    /* updateInitialFluid */
+    for (long i = 0; i < I; i++)
+        for (long j = 0; j < J; j++) {
+            const float* v = fv;
+            for (int32_t k = 0; k < n_fluids; k++) {
+                if (orc_polygon_sdf(v, fnv[k], (static_cast<float>(i) + 0.5f) * dx, (static_cast<float>(j) + 0.5f) * dx) <=
+                    0.f)
+                    s->material[i * J + j] = (int8_t)ORC_FLUID;
+                v += 2 * fnv[k];
+            }
+        }
+    std::fill(s->u, s->u + (I + 1) * J, 0.f);
+    std::fill(s->v, s->v + I * (J + 1), 0.f);
+    std::fill(s->u_valid, s->u_valid + (I + 1) * J, 0);
+    std::fill(s->v_valid, s->v_valid + I * (J + 1), 0);
+    std::fill(s->fluid_sdf, s->fluid_sdf + C, 0.f);
+}
+
+/* FlipSolver::afterTransfer, flipsolver2d.cpp:390-410 */
+void orc_after_transfer(const OrcParams* p, OrcState* s)
+{
+    if (!s->emitter_id) return;
+    const long I = p->size_i, J = p->size_j;
+    for (long i = 0; i < I; i++)
+        for (long j = 0; j < J; j++) {
+            if (!is_source(s->material[i * J + j])) continue;
+            const int e = s->emitter_id[i * J + j];
+            if (e < 0 || e >= p->n_emitters || !p->emitter_transfer || !p->emitter_transfer[e]) continue;
+            s->u[i * J + j] = static_cast<float>(p->emitter_velocity[2 * e] / p->dx);
+            s->v[i * (J + 1) + j] = static_cast<float>(p->emitter_velocity[2 * e + 1] / p->dx);
+            s->u_valid[i * J + j] = 1;
+            s->v_valid[i * (J + 1) + j] = 1;
+        }
+}
+
+float orc_reseed_jitter(uint32_t seed, uint32_t epoch, uint32_t cell, uint32_t k, uint32_t axis)
+{
+    return orc_jitter(fmix32(seed ^ fmix32(epoch * 0x9e3779b1u + 0x85ebca77u)), cell, k, axis);
+}
+
+/* FlipSolver::reseedParticles, flipsolver2d.cpp:627-680, with
+ * jitteredPosInCell (1013-1019): whole-cell uniform positions from the
+ * counter-based RNG of substep rng_epoch, velocity from the final grid when
+ * the source transfers it (velocityAt, staggeredvelocitygrid.cpp:158-167),
+ * appended after the survivors in cell order. */
+int64_t orc_reseed(const OrcParams* p, OrcState* s)
+{
+    if (!s->emitter_id) return 0;
+    const long I = p->size_i, J = p->size_j;
+    const int32_t half = p->particles_per_cell / 2;
+    int64_t added = 0;
+    for (long i = 0; i < I; i++)
+        for (long j = 0; j < J; j++) {
+            const long c = i * J + j;
+            const int32_t more = half - s->particle_counts[c];
+            if (more <= 0 || !is_source(s->material[c])) continue;
+            const int e = s->emitter_id[c];
+            const bool transfer = e >= 0 && e < p->n_emitters && p->emitter_transfer && p->emitter_transfer[e];
+            for (int32_t k = 0; k < more; k++) {
+                if (s->n_particles >= s->capacity) return added;
+                const float x = static_cast<float>(i) + orc_reseed_jitter(p->reseed_seed, s->rng_epoch, (uint32_t)c, k, 0);
+                const float y = static_cast<float>(j) + orc_reseed_jitter(p->reseed_seed, s->rng_epoch, (uint32_t)c, k, 1);
+                float vx = 0.f, vy = 0.f;
+                if (transfer) {
+                    vx = lerp_grid(s->u, I + 1, J, 0.5f, 0.f, x, y);
+                    vy = lerp_grid(s->v, I, J + 1, 0.f, 0.5f, x, y);
+                }
+                const int64_t n = s->n_particles++;
+                s->px[n] = x;
+                s->py[n] = y;
+                s->vx[n] = vx;
+                s->vy[n] = vy;
+                s->pid[n] = s->next_pid++;
+                added++;
+            }
+        }
+    return added;
 }
 
 /* seedInitialFluid (flipsolver2d.cpp:682-707) for strict-fluid cells, with

@@ -58,6 +58,12 @@ typedef struct OrcParams {
     double  project_tol;           /* replica: max-abs tol; converged: relative L2 residual */
     int32_t n_obstacles;
     const float* friction;         /* [n_obstacles] */
+    /* sources (Emitter, emitter.h): velocity (m/s, x along I) and whether
+     * it is transferred to the grid and to reseeded particles */
+    int32_t n_emitters;
+    const float* emitter_velocity; /* [2 * n_emitters] */
+    const uint8_t* emitter_transfer; /* [n_emitters] */
+    uint32_t reseed_seed;          /* counter-based RNG seed of reseedParticles */
 } OrcParams;
 
 typedef struct OrcState {
@@ -75,6 +81,9 @@ typedef struct OrcState {
     double   *pressure;            /* I*J out: last pressure solve */
     double   *rhs;                 /* I*J out: last pressure rhs */
     float    *density;             /* I*J out: density grid (density mode != off) */
+    int32_t  *emitter_id;          /* I*J, -1 = none: the source of a SOURCE cell (may be NULL) */
+    uint32_t  rng_epoch;           /* substeps taken: the reseed RNG's counter */
+    uint32_t  next_pid;            /* id of the next reseeded particle */
 } OrcState;
 
 typedef struct OrcStats {
@@ -170,7 +179,23 @@ int32_t orc_step_frame(const OrcParams* p, OrcState* s, OrcStats* st);
 void  orc_init_geometry(const OrcParams* p, OrcState* s,
                         int32_t n_solids, const int32_t* solid_nverts, const float* solid_verts,
                         int32_t n_fluids, const int32_t* fluid_nverts, const float* fluid_verts);
-/* Quadrant-jittered seeding with the counter RNG shared with the product's host init. */
+/* FlipSolver::firstFrameInit's cell passes in its order (flipsolver2d.cpp:788-795):
+ * updateSinks, updateSources, updateSolids, updateInitialFluid (502-590). */
+void  orc_init_objects(const OrcParams* p, OrcState* s,
+                       int32_t n_solids, const int32_t* solid_nverts, const float* solid_verts,
+                       int32_t n_sources, const int32_t* source_nverts, const float* source_verts,
+                       int32_t n_sinks, const int32_t* sink_nverts, const float* sink_verts,
+                       int32_t n_fluids, const int32_t* fluid_nverts, const float* fluid_verts);
+/* FlipSolver::afterTransfer (flipsolver2d.cpp:390-410): the source velocity
+ * on the faces of SOURCE cells whose emitter transfers it. */
+void  orc_after_transfer(const OrcParams* p, OrcState* s);
+/* FlipSolver::reseedParticles (flipsolver2d.cpp:627-680): ppc/2 - count new
+ * particles in each SOURCE cell, appended in cell order; returns how many. */
+int64_t orc_reseed(const OrcParams* p, OrcState* s);
+/* Uniform in [0,1) of the reseed RNG for substep `epoch` (replaces
+ * jitteredPosInCell's std::mt19937 stream, flipsolver2d.cpp:1013-1019). */
+float orc_reseed_jitter(uint32_t seed, uint32_t epoch, uint32_t cell, uint32_t k, uint32_t axis);
+/* Quadrant-jittered seeding with the counter RNG shared with the product's init. */
 int64_t orc_seed_particles(const OrcParams* p, OrcState* s, uint32_t seed);
 float   orc_jitter(uint32_t seed, uint32_t cell, uint32_t k, uint32_t axis);
 

@@ -202,3 +202,34 @@ def test_substep_parity_with_friction(fg, orc, ctx):
     assert rel_l2(g["pressure"], st.pressure) <= 1e-5
     assert max(rel_l2(g["u"], st.u), rel_l2(g["v"], st.v)) <= 1e-5
     assert max(rel_l2(g["vx"], st.vx[:N]), rel_l2(g["vy"], st.vy[:N])) <= 1e-5
+
+
+def test_full_grid_keeps_exact_sdf(fg, orc, ctx):
+    """A grid whose every cell holds particles has no cell outside the
+    liquid, so extrapolateLevelsetInside keeps every fluid-SDF value: the
+    values the fast SDF pass skips (cells holding particles are inside) must
+    come back exact from the fallback pass."""
+    n, ppc = 24, 4
+    p = orc.make_params(n, ppc=ppc, gravity=(0.0, 0.0), tol=1e-12)
+    st = orc.State(n, n, n * n * ppc)
+    st.material[:] = orc.FLUID
+    rng = np.random.default_rng(12)
+    k = 0
+    for c in range(n * n):
+        i, j = divmod(c, n)
+        for q in range(ppc):
+            st.px[k], st.py[k] = i + 0.5 * ((q >> 1) + rng.random()), j + 0.5 * ((q & 1) + rng.random())
+            st.pid[k] = k
+            k += 1
+    st.n = k
+    st.solid_sdf[:] = 1.0
+    dt = np.float32(1.0 / 60)
+    sc = fg.Scene(ctx, fg.make_params(n, ppc=ppc, gravity=(0.0, 0.0), tol=1e-10))
+    sc.set_state(oracle_state_dict(st))
+    sc.substep(float(dt))
+    g = sc.get_state()
+    sc.close()
+    orc.lib().orc_substep(C.byref(p), st.ref(), C.c_float(dt), C.byref(orc.OrcStats()))
+    assert np.all(st.fluid_sdf < 0)   # no cell is outside
+    assert np.array_equal(g["fluid_sdf"].view(np.uint32), st.fluid_sdf.view(np.uint32))
+    assert np.array_equal(g["material"], st.material)

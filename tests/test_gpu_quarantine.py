@@ -46,7 +46,7 @@ def test_nan_scene_is_quarantined_others_unchanged(fg, ctx, what):
     _log(f"quarantine ({what} NaN/Inf): {[s['n'] for s in specs]}")
     solo = [fg.make_scene(ctx, sp, tol=1e-6) for sp in specs]
     batch = [fg.make_scene(ctx, sp, tol=1e-6) for sp in specs]
-    bad = 2
+    bad = int(np.argmax([sc.particle_count() for sc in batch]))   # a scene with particles to poison
     try:
         for sc in solo:
             sc.step_frame()
