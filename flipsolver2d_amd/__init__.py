@@ -34,6 +34,7 @@ EXPORTS = [
     "fg_ctx_batch_lanes", "fg_batch_autotune", "fg_ctx_set_solver_tiling", "fg_ctx_set_preconditioner",
     "fg_ctx_preconditioner", "fg_batch_substep", "fg_server_cancel", "fg_ctx_work_read", "fg_scene_init_objects", "fg_scene_set_emitters",
     "fg_scene_max_velocity", "fg_poisson_solve_matrix",
+    "fg_poisson_solve_noalloc",
 ]
 MAX_LANES = 4
 REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_REJECTED, REQ_RELEASED, REQ_FAILED, REQ_CANCELLED = range(7)
@@ -176,6 +177,7 @@ def lib():
             "fg_batch_step_frame": (C.c_int, [vp, vp, i32, vp]),
             "fg_batch_substep": (C.c_int, [vp, vp, i32, vp, vp]),
             "fg_poisson_solve": (C.c_int, [vp, i32, i32, vp, f64, vp, vp, f64, i32, C.POINTER(SolveStats)]),
+            "fg_poisson_solve_noalloc": (C.c_int, [vp, i32, i32, vp, f64, vp, vp, f64, i32, C.POINTER(SolveStats)]),
             "fg_ctx_profile_kernel": (C.c_int, [vp, i32]),
             "fg_ctx_work_read": (C.c_int, [vp, vp, i32]),
             "fg_ctx_profile_read": (C.c_int, [vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(i64), C.POINTER(i64)]),
@@ -289,14 +291,17 @@ class Context:
         _check(lib().fg_ctx_bandwidth_probe(self.h, nbytes, reps, C.byref(g), C.byref(ms)))
         return g.value, ms.value
 
-    def poisson_solve(self, material, scale, b, tol=1e-6, max_iters=0):
+    def poisson_solve(self, material, scale, b, tol=1e-6, max_iters=0, noalloc=False):
+        """fg_poisson_solve, or with noalloc fg_poisson_solve_noalloc (no
+        device allocation in the call; systems up to its scratch size)."""
         material = np.ascontiguousarray(material, np.int8)
         b = np.ascontiguousarray(b, np.float64)
         I, J = material.shape
         x = np.zeros(I * J, np.float64)
         st = SolveStats()
-        _check(lib().fg_poisson_solve(self.h, I, J, material.ctypes.data, scale, b.ctypes.data, x.ctypes.data, tol,
-                                      max_iters, C.byref(st)))
+        fn = lib().fg_poisson_solve_noalloc if noalloc else lib().fg_poisson_solve
+        _check(fn(self.h, I, J, material.ctypes.data, scale, b.ctypes.data, x.ctypes.data, tol, max_iters,
+                  C.byref(st)))
         return x, st
 
     def __enter__(self):

@@ -136,6 +136,46 @@ struct fg_scene {
     uint32_t next_pid = 0;        // id of the next reseeded particle
 };
 
+// Device scratch of fg_poisson_solve_noalloc: the solver-only scene's arena
+// and its descriptor.  Each device that loads the library holds its own copy
+// until the process ends; its address on a device comes from
+// hipGetSymbolAddress (resolved once per device when a context is created
+// there) and only that address is handed to kernels and copies.
+constexpr uint64_t kNoAllocScratchBytes = 48ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+// Per device: the scratch's device address, the host-side scene that lives
+// in it and the descriptor / scalar staging of its copies (ordinary host
+// memory: nothing is allocated per call).  The mutex serialises the calls
+// on one device.
+struct NoAllocSlot {
+    std::mutex mu;
+    void* dev = nullptr;
+    fg_scene scene;
+    SceneDev desc_stage;
+    SceneScalars scalars;
+};
+NoAllocSlot g_noalloc[kNoAllocDevices];
+
+// Resolve the scratch's address on `device` (current) once.
+int noalloc_resolve(int device)
+{
+    if (device < 0 || device >= kNoAllocDevices) return FG_OK;   // such a device has no slot
+    NoAllocSlot& slot = g_noalloc[device];
+    std::lock_guard<std::mutex> lock(slot.mu);
+    if (slot.dev) return FG_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return fail(FG_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    slot.dev = ptr;
+    return FG_OK;
+}
+
+}  // namespace
+
 namespace {
 
 // --------------------------------------------------------------------------
@@ -1834,6 +1874,7 @@ int fg_ctx_create(int32_t device, uint64_t hbm_budget_bytes, fg_ctx** out)
     HIPCHK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(FG_E_INVALID, "no such device");
     HIPCHK(hipSetDevice(device));
+    FGCHK(noalloc_resolve(device));
     auto* c = new fg_ctx();
     c->device = device;
     c->budget = hbm_budget_bytes;
@@ -2440,6 +2481,38 @@ int fg_poisson_solve_matrix(fg_ctx* ctx, int32_t size_i, int32_t size_j, const u
     const int rc = solve_on_scene(s, nullptr, matrix, scale, b, x, stats);
     fg_scene_destroy(s);
     return rc;
+}
+
+int fg_poisson_solve_noalloc(fg_ctx* ctx, int32_t size_i, int32_t size_j, const int8_t* material, double scale,
+                             const double* b, double* x, double rel_tol, int32_t max_iters, fg_solve_stats* stats)
+{
+    if (!ctx || !material || !b || !x) return fail(FG_E_INVALID, "NULL argument");
+    if (!(scale > 0)) return fail(FG_E_INVALID, "scale must be > 0");
+    const fg_scene_params p = solve_params(size_i, size_j, rel_tol, max_iters);
+    FGCHK(validate_params(&p));
+    if (ctx->device < 0 || ctx->device >= kNoAllocDevices) return fail(FG_E_INVALID, "device index out of range");
+    NoAllocSlot& slot = g_noalloc[ctx->device];
+    std::lock_guard<std::mutex> lock(slot.mu);
+    if (!slot.dev) return fail(FG_E_STATE, "scratch address not resolved for this device");
+    HIPCHK(hipSetDevice(ctx->device));
+    fg_scene& sc = slot.scene;
+    sc = fg_scene{};
+    sc.ctx = ctx;
+    sc.p = p;
+    sc.solver_only = true;
+    init_geometry_fields(&sc);
+    sc.bytes = arena_bytes(&sc);
+    const uint64_t desc_off = (sc.bytes + 255) & ~uint64_t(255);
+    if (desc_off + sizeof(SceneDev) > kNoAllocScratchBytes)
+        return fail(FG_E_CAPACITY, "the grid needs " + std::to_string(desc_off + sizeof(SceneDev)) +
+                                       " bytes of device scratch; the allocation-free solve has " +
+                                       std::to_string(kNoAllocScratchBytes) + " (use fg_poisson_solve)");
+    sc.arena = slot.dev;
+    sc.d = reinterpret_cast<SceneDev*>(static_cast<char*>(slot.dev) + desc_off);
+    sc.hpin = &slot.desc_stage;
+    sc.hsc = &slot.scalars;
+    FGCHK(setup_scene_memory(&sc));
+    return solve_on_scene(&sc, material, nullptr, scale, b, x, stats);
 }
 
 }  // extern "C"

@@ -345,6 +345,19 @@ int fg_poisson_solve(fg_ctx* ctx, int32_t size_i, int32_t size_j, const int8_t* 
  * (pressuredata.h:97-151), for LinearSolver::solve callers that hold one. */
 int fg_poisson_solve_matrix(fg_ctx* ctx, int32_t size_i, int32_t size_j, const uint8_t* matrix, double scale,
                             const double* b, double* x, double rel_tol, int32_t max_iters, fg_solve_stats* stats);
+/* The same solve with no allocation in the call.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The solver's arrays live in a fixed 48 MiB device scratch array that the
+ * library holds on each device it runs on until the process ends (its
+ * address is resolved when the first context is created on the device); a
+ * system whose arrays do not fit (about 600,000 cells) is refused with
+ * FG_E_CAPACITY.  Calls on one device share that scratch and are serialised;
+ * the scratch is not counted against the context's HBM budget.  Results are
+ * those of fg_poisson_solve bit for bit. */
+int fg_poisson_solve_noalloc(fg_ctx* ctx, int32_t size_i, int32_t size_j, const int8_t* material, double scale,
+                             const double* b, double* x, double rel_tol, int32_t max_iters,
+                             fg_solve_stats* stats);
 
 /* ------------------------------------------------------------------------
  * Serving: continuous batching of scene simulations on one device.
