@@ -56,8 +56,9 @@ HBM_PEAK_GBS = 8000.0
 #              (timed at level 1, present where a scene has a level below
 #              the coarse block's)
 #   mg_coarse  per scene and V-cycle: 4 B per cell of the levels' dinv, of its
-#              input and of its output, once each the 100 B of every row that
-#              is not its level's canonical interior stencil
+#              input and of its output, once each the 100 B of every row above
+#              the bottom that is not its level's canonical interior stencil,
+#              and the 100 B of every bottom row
 KERNEL_BYTES = {
     "p2g": ("K_P2G", 1.0, "bytes", ["k_p2g"]),
     "particles": ("K_PARTICLES", 1.0, "bytes", ["k_advect", "k_g2p_count"]),
@@ -93,7 +94,8 @@ def parse():
                     help="frames (solves) each kernel family is timed over, after the timed ones (0: skip)")
     ap.add_argument("--skip-plain", action="store_true", help="batched leg only (profiling runs)")
     ap.add_argument("--lanes", type=int, default=0, help="lanes of the batched leg (0: the autotuner's choice)")
-    ap.add_argument("--tune-rounds", type=int, default=1, help="autotuner rounds (each steps 4 frames)")
+    ap.add_argument("--tune-rounds", type=int, default=1,
+                    help="autotuner rounds (each steps 4 frames; 0: no tuning, --lanes or 1 lane)")
     ap.add_argument("--dam-frames", type=int, default=10, help="timed frames of the 1024^2 leg (0: skip)")
     ap.add_argument("--proj-res", type=int, default=4096, help="isolated projection grid (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="served-stream window (0: skip)")
@@ -212,11 +214,15 @@ def batched_leg(fg, ctx, a, copy_gbs):
         log(f"warmup frame {w}: max substeps {max(s.substeps for s in st)}")
     # the lane schedule (size classes on separate streams) is picked by
     # measurement on live frames; the plain set advances the same frames
-    lanes, rates = fg.batch_autotune(ctx, batch, rounds=a.tune_rounds)
-    for _ in range(a.tune_rounds * fg.MAX_LANES):
-        for sc in plain:
-            sc.step_frame()
-    log(f"autotuned lanes: {lanes} (rates {[round(r / 1e6, 1) for r in rates]} M)")
+    if a.tune_rounds > 0:
+        lanes, rates = fg.batch_autotune(ctx, batch, rounds=a.tune_rounds)
+        for _ in range(a.tune_rounds * fg.MAX_LANES):
+            for sc in plain:
+                sc.step_frame()
+        log(f"autotuned lanes: {lanes} (rates {[round(r / 1e6, 1) for r in rates]} M)")
+    else:   # no tuning: the lane count given (default 1)
+        lanes, rates = max(a.lanes, 1), []
+        ctx.set_batch_lanes(lanes)
     if a.lanes > 0:   # a fixed lane count (profiling runs compare like with like)
         lanes = a.lanes
         ctx.set_batch_lanes(lanes)

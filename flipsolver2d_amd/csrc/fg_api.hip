@@ -1168,16 +1168,17 @@ void launch_vcycle(const Group& G, hipStream_t st, const SolveLaunch& S, int whi
 
 // Algorithmic bytes of one k_mg_coarse block (one scene, one V-cycle): the
 // dinv of its levels (4 B a cell), its input (r_1, or the finer level's y it
-// restricts), the y it writes (4 B a level-t cell), and once each the
-// 25 coefficients of the rows that read stored coefficients (100 B a row).
+// restricts), the y it writes (4 B a level-t cell), once each the 25
+// coefficients of the rows above the bottom that read stored coefficients,
+// and the bottom's coefficients (100 B a row: its sweeps read every row's).
 int64_t mg_coarse_bytes(const fg_scene* s)
 {
     const MgLevels& M = s->mgh;
-    const int t = M.t;
+    const int t = M.t, nb = M.n - 1;
     auto cells = [&](int l) { return (int64_t)M.lv[l].I * M.lv[l].J; };
     int64_t b = 4 * (int64_t)M.tcells + 4 * cells(t) + 4 * cells(t == 1 ? 1 : t - 1);
-    for (int l = t; l < M.n; l++) b += 4 * kMgStencil * (int64_t)s->hsc->mg_stored[l];
-    return b;
+    for (int l = t; l < nb; l++) b += 4 * kMgStencil * (int64_t)s->hsc->mg_stored[l];
+    return b + 4 * kMgStencil * cells(nb);
 }
 
 int run_solve(Group& G, bool rhs_from_grid)

@@ -29,12 +29,14 @@ constexpr int kMgBottomMax = 16;         // the bottom level has at most this ma
 constexpr int kMgBottomSweeps = 8;       // L1-Jacobi sweeps on the bottom level
 constexpr float kMgOmega = 1.5f;         // L1-Jacobi weight: x += w (r - A x) / sum_k |a_k| (SPD for w < 2)
 constexpr int kMgStencil = 25;           // 5x5 Galerkin stencil of the coarse levels
-// k_mg_coarse runs levels t .. n-1 of a scene in one block: t is the first
-// level >= 1 with at most kMgCoarseTop cells whose levels t .. n-1 hold at
-// most kMgCoarseCells cells together (their r stays in LDS)
+// k_mg_coarse runs levels t .. n-1 of a scene in one block, a thread per
+// cell: t is the first level >= 1 with at most kMgCoarseTop cells whose
+// levels t .. n-1 hold at most kMgCoarseCells cells together (their r stays
+// in LDS).  A larger top (4096 cells, four per thread) left the block
+// waiting on one global-load round trip per cell in turn.
 constexpr int kMgCoarseThreads = 1024;
-constexpr int kMgCoarseTop = 9216;
-constexpr int kMgCoarseCells = 12288;
+constexpr int kMgCoarseTop = kMgCoarseThreads;
+constexpr int kMgCoarseCells = 2 * kMgCoarseThreads;
 constexpr int kFixupLongCap = 512;       // longest cell the wave-cooperative id sort takes
 
 // Material codes and tests, FlipSolver2dLib/materialgrid.h:6-20.

@@ -527,40 +527,46 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(const SceneDev* scenes, 
 }
 
 // Levels t .. n-1 of one scene in one block (t = MgLevels::t, chosen so
-// that they fit in LDS): r_t = R y_{t-1} (r_1 comes from k_mg_down0 when
-// t = 1), the down steps of k_mg_resid and k_mg_restrict to the bottom,
-// kMgBottomSweeps L1-Jacobi sweeps from zero there, then the up steps of
-// k_mg_prolong and k_mg_post back to level t, whose y it writes for the
-// level above.  The arithmetic and its order are those of the per-level
-// kernels.  LDS: every level's r and dinv (a cell is active exactly where
-// its dinv's bit pattern is nonzero) and one work array for the level at
-// hand: its residual y on the way down (dead once the coarser r is made),
-// its x on the way up.  On the way up each level's y overwrites its r in
-// place (a thread reads only its own cell's r there), where the finer level
-// reads it.  The bottom's iterate has its own array with a two-cell zero
-// halo, so its stencil needs no bounds tests.
+// that level t has at most kMgCoarseTop = kMgCoarseThreads cells and levels
+// t .. n-1 at most kMgCoarseCells together): r_t = R y_{t-1} (r_1 comes
+// from k_mg_down0 when t = 1), the down steps of k_mg_resid and
+// k_mg_restrict to the bottom, kMgBottomSweeps L1-Jacobi sweeps from zero
+// there, then the up steps of k_mg_prolong and k_mg_post back to level t,
+// whose y it writes for the level above.  The arithmetic and its order are
+// those of the per-level kernels, so where t falls does not change z.
+// Thread c owns cell c of every level.  The block is latency-bound, so its
+// global loads come in two rounds: every level's dinv with level t's input
+// (its 4x4 patch of y_{t-1}), then the stored coefficient rows of the
+// thread's level-t and bottom cells, held in registers for the way down
+// and the way up.  LDS: every level's r and dinv (a cell is active exactly
+// where its dinv's bit pattern is nonzero) and one work array for the level
+// at hand: its residual y on the way down (dead once the coarser r is
+// made), its x on the way up.  On the way up each level's y overwrites its
+// r in place (a thread reads only its own cell's r there), where the finer
+// level reads it.  On the way down each level's x = r dinv is made once
+// into its own array before the residual reads 25 of them.  The bottom's
+// iterate has its own array with a two-cell zero halo, so its stencil needs
+// no bounds tests.
 __device__ __forceinline__ bool mg_active(float dinv) { return __float_as_uint(dinv) != 0u; }
 
-// row_dot with the canonical stencil in LDS and v_k = V(k) made as it is
-// summed (same terms, same order).  The stored coefficients are addressed by
-// 32-bit byte offsets from the level's uniform base.
-template <typename F>
-__device__ __forceinline__ float coarse_row_dot(const float* cn, gptr<const float> coef, int CL, int c, float dinv_c,
-                                                F V)
+// The 25 stored coefficients of row c (k-major layout), loaded together.
+__device__ __forceinline__ void coarse_row_load(gptr<const float> coef, int CL, int c, float (&a)[kMgStencil])
 {
-    float a[kMgStencil];
-    if (dinv_c > 0.f) {
-        gptr<const char> base = (gptr<const char>)coef;
+    gptr<const char> base = (gptr<const char>)coef;
 #pragma unroll
-        for (int k = 0; k < kMgStencil; k++)
-            a[k] = *(gptr<const float>)(base + (uint32_t)(k * CL + c) * 4u);
-    } else {
-#pragma unroll
-        for (int k = 0; k < kMgStencil; k++) a[k] = cn[k];
-    }
+    for (int k = 0; k < kMgStencil; k++) a[k] = *(gptr<const float>)(base + (uint32_t)(k * CL + c) * 4u);
+}
+
+// row_dot with v_k = V(k) made as it is summed (same terms, same order):
+// the canonical stencil (LDS) on canonical rows (dinv < 0), else the
+// coefficients in a.
+template <typename F>
+__device__ __forceinline__ float coarse_row_dot(const float* cn, const float (&a)[kMgStencil], float dinv_c, F V)
+{
+    const bool stored = dinv_c > 0.f;
     float ax = 0.f;
 #pragma unroll
-    for (int k = 0; k < kMgStencil; k++) ax += a[k] * V(k);
+    for (int k = 0; k < kMgStencil; k++) ax += (stored ? a[k] : cn[k]) * V(k);
     return ax;
 }
 
@@ -569,64 +575,93 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
     constexpr int NT = kMgCoarseThreads;
     constexpr int W = kMgBottomMax + 4;
     static_assert(kMgBottomMax * kMgBottomMax <= NT, "one bottom cell per thread");
+    static_assert(kMgCoarseTop <= NT, "one level-t cell per thread");
     __shared__ float rs[kMgCoarseCells], ds[kMgCoarseCells];
-    __shared__ float tx[kMgCoarseTop];
+    __shared__ float tx[kMgCoarseTop], xw[kMgCoarseTop];
     __shared__ float xb[2][W * W];
     __shared__ float cns[kMgMaxLevels][kMgStencil];
+    __shared__ float cbs[kMgStencil * kMgBottomMax * kMgBottomMax];   // bottom coefficients, k-major
     const SceneDev& s = scenes[blockIdx.y];
     const auto& M = *cst(s.mg);
     const int t = M.t, nb = M.n - 1;
     if (!mg_iterating(s)) return;
     const int tid = threadIdx.x;
     if (tid < kMgMaxLevels * kMgStencil) (&cns[0][0])[tid] = (&M.canon[0][0])[tid];
-    {   // dinv of levels t .. n-1 (contiguous), r_1 when t = 1
-        gptr<const float> dg = gbl<const float>(M.lv[t].dinv);
+    const auto& T = M.lv[t];
+    const int TI = T.I, TJ = T.J, CT = TI * TJ;   // roff[t] = 0
+    const bool tin = tid < CT;
+    const int tc = tin ? tid : CT - 1;
+    const int tci = tc / TJ, tcj = tc - tci * TJ;
+    // round 1: dinv of levels t .. n-1 (contiguous) and level t's input
+    {
+        gptr<const float> dg = gbl<const float>(T.dinv);
         const int TC = M.tcells;
         constexpr int Q = (kMgCoarseCells + NT - 1) / NT;
         float dv[Q];
 #pragma unroll
         for (int q = 0; q < Q; q++) dv[q] = dg[min(tid + q * NT, TC - 1)];
+        float rt;
+        if (t > 1) {   // r_t = R y_{t-1} from the launch before
+            const auto& F = M.lv[t - 1];
+            rt = restrict_at(gbl<const float>(F.y), F.I, F.J, tci, tcj);
+        } else {
+            rt = gbl<const float>(M.lv[1].r)[tc];
+        }
 #pragma unroll
         for (int q = 0; q < Q; q++)
             if (tid + q * NT < TC) ds[tid + q * NT] = dv[q];
-        if (t == 1) {
-            const int CL = M.lv[1].I * M.lv[1].J;
-            gptr<const float> r1 = gbl<const float>(M.lv[1].r);
-            constexpr int Q1 = (kMgCoarseTop + NT - 1) / NT;
-            float rv[Q1];
-#pragma unroll
-            for (int q = 0; q < Q1; q++) rv[q] = r1[min(tid + q * NT, CL - 1)];
-#pragma unroll
-            for (int q = 0; q < Q1; q++)
-                if (tid + q * NT < CL) rs[tid + q * NT] = rv[q];
-        }
+        if (tin) rs[tc] = (t == 1 || mg_active(dv[0])) ? rt : 0.f;   // dv[0] = level t's dinv of cell tc
     }
     __syncthreads();
-    if (t > 1) {   // r_t = R y_{t-1} from the launch before
-        const auto& L = M.lv[t];
-        const auto& F = M.lv[t - 1];
-        const int LJ = L.J, CL = L.I * LJ;
-        gptr<const float> fy = gbl<const float>(F.y);
-        for (int c = tid; c < CL; c += NT) {
-            const int ci = c / LJ, cj = c - ci * LJ;
-            rs[c] = mg_active(ds[c]) ? restrict_at(fy, F.I, F.J, ci, cj) : 0.f;
-        }
-        __syncthreads();
+    // round 2: the coefficients of this thread's level-t row where it reads
+    // stored ones (level t above the bottom), and the bottom's, whose sweeps
+    // read the stored coefficients of every row
+    const auto& B = M.lv[nb];
+    const int CB = B.I * B.J, ob = M.roff[nb];   // CB <= kMgBottomMax^2 (the hierarchy stops there)
+    const bool bin = tid < CB;
+    const int bc = bin ? tid : 0;
+    const float dct = tin ? ds[tc] : 0.f;
+    float ct[kMgStencil];
+#pragma unroll
+    for (int k = 0; k < kMgStencil; k++) ct[k] = 0.f;
+    if (tin && nb != t && dct > 0.f) coarse_row_load(gbl<const float>(T.coef), CT, tc, ct);
+    {   // the bottom's coefficients (k-major, contiguous) to LDS by the whole block
+        gptr<const float> bcoef = gbl<const float>(B.coef);
+        const int NBC = kMgStencil * CB;
+        constexpr int QB = (kMgStencil * kMgBottomMax * kMgBottomMax + NT - 1) / NT;
+        float v[QB];
+#pragma unroll
+        for (int q = 0; q < QB; q++) v[q] = bcoef[min(tid + q * NT, NBC - 1)];
+#pragma unroll
+        for (int q = 0; q < QB; q++)
+            if (tid + q * NT < NBC) cbs[tid + q * NT] = v[q];
     }
     for (int l = t; l < nb; l++) {   // down: x = r dinv, y = r - A x, r_{l+1} = R y
         const auto& L = M.lv[l];
         const int LI = L.I, LJ = L.J, CL = LI * LJ, o = M.roff[l];
-        for (int c = tid; c < CL; c += NT) {
+        if (tid < CL) xw[tid] = rs[o + tid] * fabsf(ds[o + tid]);   // x = r dinv
+        __syncthreads();
+        if (tid < CL) {
+            const int c = tid;
             const float dc = ds[o + c];
             float y = 0.f;   // inactive row: r = 0 and no coefficients
             if (dc != 0.f) {
                 const int ci = c / LJ, cj = c - ci * LJ;
-                const float ax = coarse_row_dot(cns[l], gbl<const float>(L.coef), CL, c, dc, [&](int k) {
+                auto V = [&](int k) {
                     const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
                     const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
-                    const int n = o + min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1);
-                    return (rs[n] * fabsf(ds[n])) * mask01(ok);
-                });
+                    return xw[min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1)] * mask01(ok);
+                };
+                float ax;
+                if (l == t) {
+                    ax = coarse_row_dot(cns[l], ct, dc, V);
+                } else {
+                    float a[kMgStencil];
+#pragma unroll
+                    for (int k = 0; k < kMgStencil; k++) a[k] = 0.f;
+                    if (dc > 0.f) coarse_row_load(gbl<const float>(L.coef), CL, c, a);
+                    ax = coarse_row_dot(cns[l], a, dc, V);
+                }
                 y = rs[o + c] - ax;
             }
             tx[c] = y;
@@ -634,41 +669,34 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
         __syncthreads();
         const auto& N = M.lv[l + 1];
         const int NJ = N.J, NC = N.I * NJ, on = M.roff[l + 1];
-        for (int c = tid; c < NC; c += NT) {
-            const int ci = c / NJ, cj = c - ci * NJ;
+        if (tid < NC) {
+            const int c = tid, ci = c / NJ, cj = c - ci * NJ;
             rs[on + c] = mg_active(ds[on + c]) ? restrict_at(static_cast<const float*>(tx), LI, LJ, ci, cj) : 0.f;
         }
         __syncthreads();
     }
     {   // bottom: y = kMgBottomSweeps sweeps of x += dinv (r - A x) from 0
-        const auto& L = M.lv[nb];
-        const int CL = L.I * L.J, o = M.roff[nb];   // CL <= kMgBottomMax^2 (the hierarchy stops there)
-        const bool in = tid < CL;
-        const int c = in ? tid : 0;
-        const int ci = c / L.J, cj = c - ci * L.J;
-        gptr<const float> coef = gbl<const float>(L.coef);
-        float cf[kMgStencil];
-#pragma unroll
-        for (int k = 0; k < kMgStencil; k++) cf[k] = coef[(size_t)k * CL + c];
-        const float dv = fabsf(ds[o + c]), rv = rs[o + c];
+        const int ci = bc / B.J, cj = bc - ci * B.J;
+        const float dv = fabsf(ds[ob + bc]), rv = rs[ob + bc];
         for (int k = tid; k < 2 * W * W; k += NT) (&xb[0][0])[k] = 0.f;
         __syncthreads();
         const int home = (ci + 2) * W + (cj + 2);
         int cur = 0;
         for (int sw = 0; sw < kMgBottomSweeps; sw++) {
-            if (in) {
+            if (bin) {
                 float ax = 0.f;
 #pragma unroll
-                for (int k = 0; k < kMgStencil; k++) ax += cf[k] * xb[cur][home + (k / 5 - 2) * W + (k % 5 - 2)];
+                for (int k = 0; k < kMgStencil; k++)
+                    ax += cbs[k * CB + bc] * xb[cur][home + (k / 5 - 2) * W + (k % 5 - 2)];
                 xb[cur ^ 1][home] = xb[cur][home] + dv * (rv - ax);
             }
             __syncthreads();
             cur ^= 1;
         }
-        if (in) {
+        if (bin) {
             const float y = xb[cur][home];
-            rs[o + c] = y;
-            if (nb == t) gbl(L.y)[c] = y;
+            rs[ob + bc] = y;
+            if (nb == t) gbl(B.y)[bc] = y;
         }
         __syncthreads();
     }
@@ -677,7 +705,8 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
         const auto& N = M.lv[l + 1];
         const int LI = L.I, LJ = L.J, CL = LI * LJ, o = M.roff[l];
         const float* yn = rs + M.roff[l + 1];   // y_{l+1}, in place of r_{l+1}
-        for (int c = tid; c < CL; c += NT) {
+        if (tid < CL) {
+            const int c = tid;
             const float dc = ds[o + c];
             float x = 0.f;
             if (mg_active(dc)) {
@@ -687,21 +716,31 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
             tx[c] = x;
         }
         __syncthreads();
-        gptr<float> yg = gbl(L.y);
-        for (int c = tid; c < CL; c += NT) {
+        if (tid < CL) {
+            const int c = tid;
             const float dc = ds[o + c];
             float y = 0.f;   // inactive row: x = 0
             if (dc != 0.f) {
                 const int ci = c / LJ, cj = c - ci * LJ;
-                const float ax = coarse_row_dot(cns[l], gbl<const float>(L.coef), CL, c, dc, [&](int k) {
+                auto V = [&](int k) {
                     const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
                     const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
                     return tx[min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1)] * mask01(ok);
-                });
+                };
+                float ax;
+                if (l == t) {
+                    ax = coarse_row_dot(cns[l], ct, dc, V);
+                } else {
+                    float a[kMgStencil];
+#pragma unroll
+                    for (int k = 0; k < kMgStencil; k++) a[k] = 0.f;
+                    if (dc > 0.f) coarse_row_load(gbl<const float>(L.coef), CL, c, a);
+                    ax = coarse_row_dot(cns[l], a, dc, V);
+                }
                 y = tx[c] + fabsf(dc) * (rs[o + c] - ax);   // tx[c] = the centre term x_12
             }
             rs[o + c] = y;
-            if (l == t) yg[c] = y;
+            if (l == t) gbl(L.y)[c] = y;
         }
         __syncthreads();
     }

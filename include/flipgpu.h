@@ -220,7 +220,12 @@ int fg_batch_autotune(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, int32_t r
  *                                       saved U, saved V; solid SDF)
  *   MG_DOWN0     5 C + 5 C1 per V-cycle (stencil code, r; level-1 activity, r1)
  *   MG_UP0       9 C + 4 C1 per V-cycle (code, r, z; level-1 y)
- *   MG_RESID / MG_COARSE  as described below. */
+ *   MG_RESID     12 C1 + 100 S1 per V-cycle (level-1 r, dinv, y; S1 the
+ *                level-1 rows that read stored coefficients)
+ *   MG_COARSE    per scene and V-cycle: 4 B per cell of the block's levels
+ *                (dinv), its input (r_1 or the finer level's y), 4 B per
+ *                top-level cell (y written), 100 B per row above the bottom
+ *                that reads stored coefficients, 100 B per bottom row. */
 enum fg_kernel_id {
     FG_K_ADVECT = 0,      /* [P] */
     FG_K_SORT = 1,        /* [P] the counting sort's launches */
