@@ -21,7 +21,7 @@ constexpr int kPcgMaxBlocks = 1024;      // default solver blocks per scene (con
 constexpr int kPcgMinChunk = 1024;       // default cells per solver block at least (4 per thread)
 constexpr int kPcgTilingMaxBlocks = 4096;  // finest tiling a context may select
 constexpr int kPcgTilingMinChunk = 256;
-constexpr int kSdfLanes = 4;             // lanes per 3x3-cell bin in the fluid SDF
+constexpr int kSdfLanes = 4;             // lanes per 3x3-cell bin in the fluid SDF (two or three window rows each)
 // multigrid preconditioner
 constexpr int kPrecondMG = 0, kPrecondIPP = 1;
 constexpr int kMgMaxLevels = 12;
@@ -254,15 +254,23 @@ __device__ __forceinline__ float2 velocity_at(const float* u, const float* v, in
 // simmath::bSpline / quadraticBSpline, mathfuncs.cpp:32-43.  The reference
 // multiplies each piece by a 0/1 factor and adds them; exactly one piece
 // survives (beyond 1.5 both are zero and the sum is +0), so selecting the
-// piece gives the same float bit for bit for every |value| < 2^64 (checked
-// exhaustively on the host; beyond, value^2 overflows and the reference's
-// form gives NaN) at a third of the instructions.  bSpline(0) = 0.75.
+// piece gives the same float bit for bit for every finite |value| < 2^64
+// (beyond, value^2 overflows and the reference's form gives NaN, as it does
+// for NaN; the select gives +0 there).  The outer piece
+// is (0.5 b) b with b = max(1.5 - a, 0): b = 1.5 - a below 1.5, and +0 from
+// 1.5 up and for NaN (fmaxf returns the other operand), where the reference
+// gives +0 as well; so one select remains.  Both pieces are made before it
+// (the empty asm pins them): a select, not divergent branches.  Checked
+// against the CPU restatement of the reference on the host.
+// bSpline(0) = 0.75.
 __device__ __forceinline__ float bspline(float value)
 {
-    value = fabsf(value);
-    const float t1 = 0.75f - value * value;
-    const float t2 = (0.5f * (1.5f - value)) * (1.5f - value);
-    return value < 0.5f ? t1 : (value < 1.5f ? t2 : 0.0f);
+    const float a = fabsf(value);
+    const float b = fmaxf(1.5f - a, 0.0f);
+    float t1 = 0.75f - a * a;
+    float t2 = (0.5f * b) * b;
+    asm volatile("" : "+v"(t1), "+v"(t2));
+    return a < 0.5f ? t1 : t2;
 }
 __device__ __forceinline__ float qbspline(float x, float y) { return (bspline(x) * bspline(y)) * 0.75f; }
 

@@ -91,6 +91,23 @@ __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
     const int nrows = wr1 - wr0 + 1;   // <= 18
     __shared__ float2 spos[kP2gCap], svel[kP2gCap];
     __shared__ int seg_beg[kP2gTile + 2], seg_off[kP2gTile + 3];
+    // this thread's face and its three rows' particle ranges, loaded now with
+    // the window's bounds (clamped rows; a row outside the grid gets an
+    // empty range)
+    const int i = i0 + (threadIdx.x >> 4), j = j0 + (threadIdx.x & 15);
+    const bool face = i < I && j < J;
+    const int ic0 = min(i, I - 1), jc = min(j, J - 1);
+    const int jl = jc > 0 ? jc - 1 : 0, jh = jc < J - 1 ? jc + 1 : J - 1;
+    gptr<const int32_t> cs = gbl<const int32_t>(s.cellStart);
+    int ra[3], re[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int ii = ic0 + q - 1, icl = min(max(ii, 0), I - 1);
+        const int a = cs[icl * J + jl], e = cs[icl * J + jh + 1];
+        const bool in = ii >= 0 && ii < I;
+        ra[q] = in ? a : 0;
+        re[q] = in ? e : 0;
+    }
     if (threadIdx.x < 64) {
         const int r = threadIdx.x;
         int beg = 0, cnt = 0;
@@ -140,24 +157,10 @@ __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
         }
     }
     __syncthreads();
-    const int i = i0 + (threadIdx.x >> 4), j = j0 + (threadIdx.x & 15);
-    if (i >= I || j >= J) return;
+    if (!face) return;
     const float fi = (float)i, fj = (float)j;
+    const float fih = fi + 0.5f, fjh = fj + 0.5f;
     float uw = 1e-10f, vw = 1e-10f, uv = 0.0f, vv = 0.0f;
-    bool uval = false, vval = false;
-    const int jl = j > 0 ? j - 1 : 0, jh = j < J - 1 ? j + 1 : J - 1;
-    // the three rows' particle ranges, loaded together (clamped rows; a row
-    // outside the grid gets an empty range)
-    gptr<const int32_t> cs = gbl<const int32_t>(s.cellStart);
-    int ra[3], re[3];
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const int ii = i + q - 1, ic = min(max(ii, 0), I - 1);
-        const int a = cs[ic * J + jl], e = cs[ic * J + jh + 1];
-        const bool in = ii >= 0 && ii < I;
-        ra[q] = in ? a : 0;
-        re[q] = in ? e : 0;
-    }
     for (int di = -1; di <= 1; di++) {
         const int ii = i + di;
         if (ii < 0 || ii >= I) continue;
@@ -184,27 +187,29 @@ __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
             }
 #pragma unroll
             for (int u = 0; u < U; u++) asm volatile("" : "+v"(pp[u].x), "+v"(pp[u].y), "+v"(vel[u].x), "+v"(vel[u].y));
+            // branch-free: a particle that does not pass adds +0, which
+            // leaves every sum as it was (the weight sums stay positive and
+            // the velocity sums, starting at +0, are never -0)
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                if (k0 + u >= e) continue;
                 const float2 p = pp[u];
-                const float wu = qbspline(p.x - fi, p.y - (fj + 0.5f));
-                const float wv = qbspline(p.x - (fi + 0.5f), p.y - fj);
-                if (wu > 1e-9f && wv > 1e-9f) {
-                    uw += wu;
-                    uv += wu * vel[u].x;
-                    uval = true;
-                    vw += wv;
-                    vv += wv * vel[u].y;
-                    vval = true;
-                }
+                const float wu = qbspline(p.x - fi, p.y - fjh);
+                const float wv = qbspline(p.x - fih, p.y - fj);
+                const bool g = (k0 + u < e) & (wu > 1e-9f) & (wv > 1e-9f);
+                const float cu = wu * vel[u].x, cv = wv * vel[u].y;
+                uw += g ? wu : 0.f;
+                uv += g ? cu : 0.f;
+                vw += g ? wv : 0.f;
+                vv += g ? cv : 0.f;
             }
         }
     }
+    // a face was reached iff its weight sum left 1e-10 (each weight that
+    // passes exceeds 1e-9)
     s.u[(size_t)i * J + j] = uv / uw;
-    s.uval[(size_t)i * J + j] = uval;
+    s.uval[(size_t)i * J + j] = uw > 1e-10f;
     s.v[(size_t)i * (J + 1) + j] = vv / vw;
-    s.vval[(size_t)i * (J + 1) + j] = vval;
+    s.vval[(size_t)i * (J + 1) + j] = vw > 1e-10f;
     if (i == I - 1) {
         s.u[(size_t)I * J + j] = 0.f;
         s.uval[(size_t)I * J + j] = 0;
@@ -228,10 +233,15 @@ __global__ void __launch_bounds__(kBlock) k_p2g(const SceneDev* scenes)
 // kExact: the fallback pass (k_sdf_exact) -- every bin scans its window and
 // only the SDF is written.  Otherwise a bin whose cells all hold particles
 // (with the shortcut on) skips its scan: each of its cells is inside.
+// Most bins scan nothing (inside, or far from the liquid), so every load a
+// lane needs besides the particles is issued at the top, in one round: the
+// window rows' bounds, the bin's 3 x 4 cell boundaries (its cells' particle
+// counts) and the materials of the cells the lane writes.
 template <bool kExact>
 __device__ __forceinline__ void sdf_bins(const SceneDev& s)
 {
-    static_assert(kSdfLanes == 4, "the shuffle reduction assumes 4 lanes per bin");
+    static_assert(kSdfLanes >= 4 && kSdfLanes <= 64 && (kSdfLanes & (kSdfLanes - 1)) == 0,
+                  "a bin's lanes are a power-of-two group inside one wave");
     const int I = s.I, J = s.J;
     const int nbI = (I + 2) / 3, nbJ = (J + 2) / 3;
     const int sub = threadIdx.x & (kSdfLanes - 1);
@@ -241,6 +251,36 @@ __device__ __forceinline__ void sdf_bins(const SceneDev& s)
     const int r0 = max(0, 3 * bi - 3), r1 = min(I, 3 * bi + 6);
     const int c0 = max(0, 3 * bj - 3), c1 = min(J, 3 * bj + 6);
     const int i0 = 3 * bi, j0 = 3 * bj;
+    // this lane's window rows r0+sub, r0+sub+kSdfLanes, ... and cells
+    // q = sub, sub+kSdfLanes, ... of the bin (q = 3a + b)
+    constexpr int NR = (9 + kSdfLanes - 1) / kSdfLanes;
+    gptr<const int32_t> cs = gbl<const int32_t>(s.cellStart);
+    int rb[NR], re[NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) {   // clamped rows: every load unconditional
+        const int r = r0 + sub + q * kSdfLanes, rc = min(r, r1 - 1);
+        rb[q] = cs[rc * J + c0];
+        re[q] = cs[rc * J + c1];
+    }
+    int bnd[3][4];   // cellStart at (i0+a, j0+k), clamped to the grid (index i*J + J is row i's end)
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) bnd[a][k] = cs[min(i0 + a, I - 1) * J + min(j0 + k, J)];
+    int8_t mq[NR];
+    if (!kExact) {
+        gptr<const int8_t> mat = gbl<const int8_t>(s.mat);
+#pragma unroll
+        for (int q = 0; q < NR; q++) {   // cell sub + q kSdfLanes (clamped: loads unconditional)
+            const int cq = min(sub + q * kSdfLanes, 8), a = cq / 3, b = cq - a * 3;
+            mq[q] = mat[min(i0 + a, I - 1) * J + min(j0 + b, J - 1)];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        const int r = r0 + sub + q * kSdfLanes;
+        if (r >= r1) re[q] = rb[q];
+    }
     float cx[3], cy[3];
 #pragma unroll
     for (int q = 0; q < 3; q++) {
@@ -252,17 +292,6 @@ __device__ __forceinline__ void sdf_bins(const SceneDev& s)
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int b = 0; b < 3; b++) d2[a][b] = 3.402823466e+38f;
-    // this lane's rows r0+sub, r0+sub+4, r0+sub+8: bounds loaded together
-    constexpr int NR = (9 + kSdfLanes - 1) / kSdfLanes;
-    gptr<const int32_t> cs = gbl<const int32_t>(s.cellStart);
-    int rb[NR], re[NR];
-#pragma unroll
-    for (int q = 0; q < NR; q++) {   // clamped rows: every load unconditional
-        const int r = r0 + sub + q * kSdfLanes, rc = min(r, r1 - 1);
-        const int b = cs[rc * J + c0], e = cs[rc * J + c1];
-        rb[q] = r < r1 ? b : 0;
-        re[q] = r < r1 ? e : 0;
-    }
     // Every particle of a cell lies within sqrt(0.5) of its centre, so with
     // particleScale above that (sdf_short) a cell holding a particle has
     // sdf < 0 whatever its window holds; extrapolateLevelsetInside then
@@ -272,14 +301,10 @@ __device__ __forceinline__ void sdf_bins(const SceneDev& s)
     if (!kExact && s.sdf_short) {
         int have = 0;
 #pragma unroll
-        for (int a = 0; a < 3; a++) {   // clamped cells: loads unconditional
-            const int i = min(i0 + a, I - 1);
-            const int j1 = min(j0 + 3, J);
-            const int cb = cs[i * J + j0], e0 = cs[i * J + min(j0 + 1, J)], e1 = cs[i * J + min(j0 + 2, J)],
-                      e2 = cs[i * J + j1];
-            have += (e0 > cb || i0 + a >= I) + (e1 > e0 || j0 + 1 >= J || i0 + a >= I) +
-                    (e2 > e1 || j0 + 2 >= J || i0 + a >= I);
-        }
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+                have += bnd[a][b + 1] > bnd[a][b] || i0 + a >= I || j0 + b >= J;
         full = have == 9;
         if (full) {
 #pragma unroll
@@ -322,7 +347,8 @@ __device__ __forceinline__ void sdf_bins(const SceneDev& s)
                 if (t < d2[a][b]) d2[a][b] = t;
             }
         }
-    // cell q of the bin is written by lane q mod kSdfLanes
+    // cell q of the bin is written by lane q mod kSdfLanes (q, a, b
+    // compile-time, so the arrays stay in registers)
 #pragma unroll
     for (int q = 0; q < 9; q++) {
         if ((q & (kSdfLanes - 1)) != sub) continue;
@@ -330,21 +356,22 @@ __device__ __forceinline__ void sdf_bins(const SceneDev& s)
         const int i = i0 + a, j = j0 + b;
         if (i >= I || j >= J) continue;
         const int c = i * J + j;
+        const int n = bnd[a][b + 1] - bnd[a][b];
         // a skipped bin's cells are inside: any negative value holds their place
         const float sdf = full ? -s.pscale : sqrtf(d2[a][b]) - s.pscale;
         if (kExact) {
-            if (s.cellStart[c + 1] > s.cellStart[c]) s.fsdf[c] = sdf;   // the cells the fast pass skipped
+            if (n > 0) s.fsdf[c] = sdf;   // the cells the fast pass skipped
             continue;
         }
         s.fsdf[c] = sdf;
-        int8_t m = s.mat[c];
+        int8_t m = mq[q / kSdfLanes];
         if (sdf < 0.f) {
             if (is_empty(m)) m = M_FLUID;
         } else {
             if (is_strict_fluid(m)) m = M_EMPTY;
         }
         s.mat[c] = m;
-        s.pcount[c] = min(s.cellStart[c + 1] - s.cellStart[c], 2 * s.ppc);
+        s.pcount[c] = min(n, 2 * s.ppc);
     }
 }
 
