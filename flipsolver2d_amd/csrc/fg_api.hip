@@ -328,6 +328,7 @@ void carve(fg_scene* s, Carver& cv)
     h.pcount = cv.take<int32_t>(C);
     h.marker = cv.take<int32_t>(C);
     h.lsFlags = cv.take<int32_t>(kMaxBands);
+    h.lsTile = cv.take<uint8_t>(2 * cdiv(s->I, kBfsTile) * cdiv(s->J, kBfsTile));
     h.u = cv.take<float>(FU);
     h.su = cv.take<float>(FU);
     h.uf = cv.take<float>(FU);
@@ -1393,7 +1394,7 @@ int run_stage(Group& G, int stage)
     }
     case FG_STAGE_LEVELSET: {
         KProf kp(G, FG_K_LEVELSET);
-        launch_levelset_init(g.cells, st, d);
+        launch_levelset_init(g.ls, st, d);
         launch_sdf_exact(g.bins, st, d);   // only where no cell is outside
         // a band launch after the last one needed exits at once but still
         // dispatches its blocks: the first poll comes after one band more

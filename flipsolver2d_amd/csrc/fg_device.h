@@ -155,6 +155,8 @@ struct SceneDev {
     int32_t* pcount;
     int32_t* marker;        // level-set BFS levels
     int32_t* lsFlags;       // [kMaxBands] band reached its last level
+    uint8_t* lsTile;        // [2][BFS tiles] by band parity: the tile reached its band's last level
+                            // (band 0 reads the tiles holding a known cell)
     // faces: working (P2G/project), saved (extrapolated, pre-force), final
     float *u, *v, *su, *sv, *uf, *vf;
     uint8_t *uval, *vval, *uvalS, *vvalS, *uvalF, *vvalF;

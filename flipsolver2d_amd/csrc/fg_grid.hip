@@ -407,6 +407,7 @@ __global__ void __launch_bounds__(kBlock) k_sdf_exact(const SceneDev* scenes)
 // (linearindexable2d.h:55-71) exactly as the reference accumulates them.
 // ---------------------------------------------------------------------------
 constexpr int kTile = 32;
+static_assert(kTile == kBfsTile, "the host sizes the BFS launches and tile flags by kBfsTile");
 constexpr int16_t kNone = 0x7fff;     // outside the grid
 constexpr int16_t kUnreached = 0x7ffe;
 
@@ -417,7 +418,7 @@ constexpr int16_t kUnreached = 0x7ffe;
 // barrier.  The value pass then visits only the cells of each level.
 template <int H, bool LEVELSET>
 __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mkin, float* gout, uint8_t* vout,
-                         int32_t* mkout, int si, int sj, int L0, int* more)
+                         int32_t* mkout, int si, int sj, int L0, int* more, uint8_t* tile_more)
 {
     constexpr int R = kTile + 2 * H;
     static_assert(R <= 64, "a region row must fit one 64-bit mask");
@@ -517,6 +518,8 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
                     gout[g] = val[r];
                     vout[g] = mk[r] != kUnreached;
                 }
+        } else if (threadIdx.x == 0) {
+            *tile_more = 0;
         }
         return;
     }
@@ -536,7 +539,7 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
     }
     __syncthreads();
     // expand levels L0+1 .. L0+H: wave 0, lane = region row; per level also
-    // the row offsets of its cells (exclusive scan of the row counts)
+    // the row offsets of its cells
     if (threadIdx.x < 64) {
         const int r = threadIdx.x;
         unsigned long long open = r < R ? open_s[r] : 0ull, f = r < R ? front[0][r] : 0ull;
@@ -551,15 +554,19 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
             if (r >= R) nf = 0;
             open &= ~nf;
             if (r < R) front[st][r] = nf;
+            // exclusive scan of the row counts by bit slices: one ballot per
+            // bit of the count (<= 64), no lane shuffles
             const int cnt = __popcll(nf);
-            int incl = cnt;
+            const unsigned long long below = (1ull << r) - 1ull;
+            int excl = 0, total = 0;
 #pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int t = __shfl_up(incl, o, 64);
-                if (r >= o) incl += t;
+            for (int b = 0; b < 7; b++) {
+                const unsigned long long m = __ballot((cnt >> b) & 1);
+                excl += __popcll(m & below) << b;
+                total += __popcll(m) << b;
             }
-            pre[st][r] = (int16_t)(incl - cnt);
-            if (r == 63) tot[st] = (int16_t)incl;
+            pre[st][r] = (int16_t)excl;
+            if (r == 0) tot[st] = (int16_t)total;
             f = nf;
             if (!__any(nf != 0ull)) break;
             top = st;
@@ -643,7 +650,13 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
                 vout[g] = (m != kUnreached) ? 1 : 0;
             }
         }
-    if (LEVELSET && reached_end) *more = 1;
+    if (LEVELSET) {
+        const bool any = __syncthreads_or(reached_end);
+        if (threadIdx.x == 0) {
+            *tile_more = any ? 1 : 0;
+            if (any) *more = 1;
+        }
+    }
 }
 
 // Velocity extrapolation, radius 10 (StaggeredVelocityGrid::extrapolate,
@@ -661,34 +674,72 @@ __global__ void __launch_bounds__(kBlock) k_extrapolate(const SceneDev* scenes, 
     const uint8_t* vin = isU ? (which == 0 ? s.uval : s.uvalS) : (which == 0 ? s.vval : s.vvalS);
     float* gout = isU ? (which == 0 ? s.su : s.uf) : (which == 0 ? s.sv : s.vf);
     uint8_t* vout = isU ? (which == 0 ? s.uvalS : s.uvalF) : (which == 0 ? s.vvalS : s.vvalF);
-    bfs_band<11, false>(gin, vin, nullptr, gout, vout, nullptr, si, sj, 0, nullptr);
+    bfs_band<11, false>(gin, vin, nullptr, gout, vout, nullptr, si, sj, 0, nullptr, nullptr);
 }
 
+// Markers (known cells: sdf > 0, level 0), one block per BFS tile, which
+// records whether it holds a known cell (band 0's front); clears the band
+// flags.
 __global__ void __launch_bounds__(kBlock) k_levelset_init(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
-    const int C = s.I * s.J;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    const bool known = c < C && s.fsdf[c] > 0.f;
-    if (c < C) s.marker[c] = known ? 0 : kMaxLevel;
-    if (c < kMaxBands) s.lsFlags[c] = 0;
-    if (__syncthreads_or(known) && threadIdx.x == 0) s.sc->sdf_known = 1;   // some cell is outside
+    const int I = s.I, J = s.J;
+    const int tilesJ = (J + kTile - 1) / kTile;
+    const int tiles = ((I + kTile - 1) / kTile) * tilesJ;
+    const int tile = blockIdx.x;
+    if (tile >= tiles) return;
+    if (tile == 0)
+        for (int k = threadIdx.x; k < kMaxBands; k += kBlock) s.lsFlags[k] = 0;
+    const int ti = tile / tilesJ, tj = tile - ti * tilesJ;
+    bool known = false;
+    for (int q = threadIdx.x; q < kTile * kTile; q += kBlock) {
+        const int i = ti * kTile + q / kTile, j = tj * kTile + q % kTile;
+        if (i >= I || j >= J) continue;
+        const int c = i * J + j;
+        const bool k = s.fsdf[c] > 0.f;
+        s.marker[c] = k ? 0 : kMaxLevel;
+        known |= k;
+    }
+    if (__syncthreads_or(known) && threadIdx.x == 0) {
+        s.lsTile[tile] = 1;
+        s.sc->sdf_known = 1;   // some cell is outside
+    } else if (threadIdx.x == 0) {
+        s.lsTile[tile] = 0;
+    }
 }
 
 // One band of extrapolateLevelsetInside: levels L0+1 .. L0+16.  Runs only if
 // the previous band reached its last level somewhere (lsFlags[band - 1]);
-// records the bands that ran in sc->ls_bands.
+// records the bands that ran in sc->ls_bands.  A tile has work only if a
+// level-L0 cell lies within its 16-cell halo, i.e. in it or one of its eight
+// neighbours (the halo is narrower than a tile), and such a tile reached
+// L0 in the band before: the others only record that they did not reach
+// the band's end.
 __global__ void __launch_bounds__(kBlock) k_levelset_band(const SceneDev* scenes, int band)
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
     if (band > 0 && cst(s.lsFlags)[band - 1] == 0) return;   // written by the previous band's launch
-    const int tiles = ((s.I + kTile - 1) / kTile) * ((s.J + kTile - 1) / kTile);
-    if ((int)blockIdx.x >= tiles) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) s.sc->ls_bands = band + 1;
+    const int tilesI = (s.I + kTile - 1) / kTile, tilesJ = (s.J + kTile - 1) / kTile;
+    const int tiles = tilesI * tilesJ;
+    const int tile = blockIdx.x;
+    if (tile >= tiles) return;
+    if (tile == 0 && threadIdx.x == 0) s.sc->ls_bands = band + 1;
+    const uint8_t* fin = s.lsTile + (band & 1) * tiles;
+    uint8_t* fout = s.lsTile + ((band + 1) & 1) * tiles;
+    const int ti = tile / tilesJ, tj = tile - ti * tilesJ;
+    bool near = false;
+    if (threadIdx.x < 9) {
+        const int ni = ti + threadIdx.x / 3 - 1, nj = tj + threadIdx.x % 3 - 1;
+        near = ni >= 0 && ni < tilesI && nj >= 0 && nj < tilesJ && fin[ni * tilesJ + nj] != 0;
+    }
+    if (!__syncthreads_or(near)) {
+        if (threadIdx.x == 0) fout[tile] = 0;
+        return;
+    }
     bfs_band<16, true>(s.fsdf, nullptr, s.marker, s.fsdf, nullptr, s.marker, s.I, s.J, band * 16,
-                       &s.lsFlags[band]);
+                       &s.lsFlags[band], &fout[tile]);
 }
 
 // ---------------------------------------------------------------------------
