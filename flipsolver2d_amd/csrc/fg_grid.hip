@@ -422,9 +422,15 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
 {
     constexpr int R = kTile + 2 * H;
     static_assert(R <= 64, "a region row must fit one 64-bit mask");
+    // LDS row strides chosen so that the cells of a region column (a level's
+    // vertical side, visited by consecutive threads) fall in different banks:
+    // an odd number of words per row, for the floats and for the int16
+    // markers (two to a word)
+    constexpr int RS = R + 1;
+    constexpr int RSM = ((R + 2) / 2) % 2 == 1 ? R + 2 : R + 4;
     constexpr int T = 16;   // threads as a 16 x 16 layout for the load and store passes
-    __shared__ int16_t mk[R * R];
-    __shared__ float val[R * R];
+    __shared__ int16_t mk[R * RSM];
+    __shared__ float val[R * RS];
     __shared__ unsigned long long front[H + 1][R];   // front[s]: region cells at level L0 + s
     __shared__ unsigned long long open_s[R];
     __shared__ int16_t pre[H + 1][64];                // row offsets of level L0+s's cells
@@ -480,8 +486,8 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
             }
             has_front |= m == L0;
             has_open |= m == kUnreached;
-            mk[ri * R + rj] = m;
-            val[ri * R + rj] = xr[a][b];
+            mk[ri * RSM + rj] = m;
+            val[ri * RS + rj] = xr[a][b];
         }
     // a tile cell reaches level L0+s only through a level-L0 cell within
     // distance s <= H, i.e. inside this region
@@ -504,7 +510,7 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
             for (int b = 0; b < NA; b++) {
                 const int ri = ty + a * T, rj = tx + b * T;
                 const bool inner = ri >= H && ri < H + kTile && rj >= H && rj < H + kTile;
-                if (ri < R && rj < R && !inner) val[ri * R + rj] = hv[a][b];
+                if (ri < R && rj < R && !inner) val[ri * RS + rj] = hv[a][b];
             }
     }
     if (!work) {
@@ -514,9 +520,9 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
                     const int gi = ti * kTile + li, gj = tj * kTile + lj;
                     if (gi >= si || gj >= sj) continue;
                     const size_t g = (size_t)gi * sj + gj;
-                    const int r = (li + H) * R + (lj + H);
+                    const int r = (li + H) * RS + (lj + H), rm = (li + H) * RSM + (lj + H);
                     gout[g] = val[r];
-                    vout[g] = mk[r] != kUnreached;
+                    vout[g] = mk[rm] != kUnreached;
                 }
         } else if (threadIdx.x == 0) {
             *tile_more = 0;
@@ -528,7 +534,7 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
     {
         const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
         for (int r = w; r < R; r += kBlock / 64) {
-            const int16_t m = lane < R ? mk[r * R + lane] : kNone;
+            const int16_t m = lane < R ? mk[r * RSM + lane] : kNone;
             const unsigned long long o = __ballot(m == kUnreached);
             const unsigned long long f = __ballot(m == (int16_t)L0);
             if (lane == 0) {
@@ -613,8 +619,8 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
                 const int dj = (e < 3) ? e - 1 : (e < 5 ? (e == 3 ? -1 : 1) : e - 6);
                 const int qi = ri + di, qj = rj + dj;
                 const bool ok = qi >= 0 && qi < R && qj >= 0 && qj < R;
-                qm[e] = ok ? mk[qi * R + qj] : kNone;
-                qv[e] = ok ? val[qi * R + qj] : 0.f;
+                qm[e] = ok ? mk[qi * RSM + qj] : kNone;
+                qv[e] = ok ? val[qi * RS + qj] : 0.f;
             }
             double avg = 0;
             int count = 0;
@@ -624,9 +630,9 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
                     avg += (double)qv[e];
                     count++;
                 }
-            if (LEVELSET) val[ri * R + rj] = (float)(avg / count - 1.f);
-            else val[ri * R + rj] = (float)(avg / count);
-            mk[ri * R + rj] = (int16_t)L;   // same-level neighbours read L or unreached: not below L either way
+            if (LEVELSET) val[ri * RS + rj] = (float)(avg / count - 1.f);
+            else val[ri * RS + rj] = (float)(avg / count);
+            mk[ri * RSM + rj] = (int16_t)L;   // same-level neighbours read L or unreached: not below L either way
         }
         __syncthreads();
     }
@@ -636,9 +642,9 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
         for (int lj = tx; lj < kTile; lj += T) {
             const int gi = ti * kTile + li, gj = tj * kTile + lj;
             if (gi >= si || gj >= sj) continue;
-            const int r = (li + H) * R + (lj + H);
+            const int r = (li + H) * RS + (lj + H), rm = (li + H) * RSM + (lj + H);
             const size_t g = (size_t)gi * sj + gj;
-            const int16_t m = mk[r];
+            const int16_t m = mk[rm];
             if (LEVELSET) {
                 if (m > L0 && m <= L0 + H) {
                     gout[g] = val[r];
