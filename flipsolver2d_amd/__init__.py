@@ -130,7 +130,8 @@ class ServerSummary(C.Structure):
                 ("p50_admit_wait_ms", C.c_double), ("p99_admit_wait_ms", C.c_double),
                 ("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
                 ("peak_resident_bytes", C.c_uint64), ("peak_resident", C.c_int32), ("reserved", C.c_int32),
-                ("failed", C.c_int64), ("cancelled", C.c_int64)]
+                ("failed", C.c_int64), ("cancelled", C.c_int64), ("budget_waits", C.c_int64),
+                ("cap_waits", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -63,6 +63,7 @@ struct fg_server {
     // summary
     std::vector<double> frame_ms, admit_wait_ms;
     int64_t frames = 0, psub = 0, submitted = 0, completed = 0, rejected = 0, failed = 0, cancelled = 0;
+    int64_t budget_waits = 0, cap_waits = 0;
     double busy_ms = 0;
     uint64_t peak_bytes = 0;
     int32_t peak_resident = 0;
@@ -99,8 +100,14 @@ int admit(fg_server* srv, int* admitted)
     *admitted = 0;
     while (!srv->queue.empty()) {
         Req& r = srv->reqs[srv->queue.front()];
-        if ((int32_t)srv->resident.size() >= srv->max_resident) break;
-        if (srv->resident_bytes + r.bytes > srv->budget) break;   // head waits: no overtaking
+        if ((int32_t)srv->resident.size() >= srv->max_resident) {
+            srv->cap_waits++;
+            break;
+        }
+        if (srv->resident_bytes + r.bytes > srv->budget) {   // head waits: no overtaking
+            srv->budget_waits++;
+            break;
+        }
         fg_scene* sc = nullptr;
         int rc = fg_scene_create(srv->ctx, &r.params, &sc);
         if (rc == FG_E_BUDGET || rc == FG_E_DEVICE) {
@@ -357,6 +364,8 @@ int fg_server_summary_get(fg_server* srv, fg_server_summary* out)
     s.peak_resident = srv->peak_resident;
     s.failed = srv->failed;
     s.cancelled = srv->cancelled;
+    s.budget_waits = srv->budget_waits;
+    s.cap_waits = srv->cap_waits;
     *out = s;
     return FG_OK;
 }

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define FG_ABI_VERSION 5
+#define FG_ABI_VERSION 6
 
 enum fg_status {
     FG_OK = 0,
@@ -430,6 +430,10 @@ typedef struct fg_server_summary {
     int32_t peak_resident;
     int32_t reserved;
     int64_t failed, cancelled;
+    /* ticks whose admission stopped with requests still queued because the
+     * head did not fit the HBM budget (budget_waits) or the resident cap
+     * (cap_waits) */
+    int64_t budget_waits, cap_waits;
 } fg_server_summary;
 
 int fg_server_create(fg_ctx* ctx, uint64_t hbm_budget_bytes, int32_t max_resident, fg_server** out);

@@ -1015,11 +1015,10 @@ void orc_ipp_ref_apply(const int8_t* m, const uint8_t* fluid, int32_t I, int32_t
  *       + [j-(c) and i-(c+J-1)] r_{c+J-1} / n_{c-1}^2
  * the last two being the fill-in at the anti-diagonal neighbours (rows c
  * and d share the lower neighbour e). */
-void orc_ipp_apply(const uint8_t* n, const uint8_t* mask, const uint8_t* fluid, int32_t I, int32_t J,
-                   const double* x, double* y)
+static void ipp_rows(const uint8_t* n, const uint8_t* mask, const uint8_t* fluid, long J, long C, const double* x,
+                     double* y, long from, long to)
 {
-    const long C = (long)I * J;
-    for (long l = 0; l < C; l++) {
+    for (long l = from; l < to; l++) {
         if (!fluid[l]) {
             y[l] = x[l];
             continue;
@@ -1049,6 +1048,12 @@ void orc_ipp_apply(const uint8_t* n, const uint8_t* mask, const uint8_t* fluid, 
         }
         y[l] = diag * x[l] + off;
     }
+}
+
+void orc_ipp_apply(const uint8_t* n, const uint8_t* mask, const uint8_t* fluid, int32_t I, int32_t J,
+                   const double* x, double* y)
+{
+    ipp_rows(n, mask, fluid, J, (long)I * J, x, y, 0, (long)I * J);
 }
 
 /* LinearSolver::solve, linearsolver.cpp:25-73, with VOps (vmath.cpp:8-149):
@@ -1191,6 +1196,87 @@ int32_t orc_pcg_converged(const int8_t* m, int32_t I, int32_t J, double scale, c
     return it;
 }
 
+/* orc_pcg_converged on `threads` pool threads, for the CPU baseline at the
+ * GPU path's accuracy: the products split by split_range, the dot products
+ * summed as per-range partials in range order (so a thread count gives one
+ * result; another count can differ in the last bits, and converges to the
+ * same tolerance). */
+static int32_t pcg_converged_mt(int threads, const int8_t* m, int32_t I, int32_t J, double scale, const double* b,
+                                double* x, int32_t iter_limit, double rel_tol, double* rel_resid_out)
+{
+    const long C = (long)I * J;
+    std::vector<uint8_t> n(C), mask(C), fluid(C);
+    orc_build_matrix(m, I, J, n.data(), mask.data(), fluid.data());
+    std::fill(x, x + C, 0.0);
+    const auto rs = split_range(C, 1, 1, threads);
+    std::vector<double> part(rs.size());
+    Pool& pool = Pool::get(std::max(1, threads));
+    auto reduce = [&](const std::function<double(long, long)>& f) {
+        pool.run(rs, [&](size_t k, Range r) { part[k] = f(r.start, r.end); });
+        double t = 0;
+        for (double v : part) t += v;
+        return t;
+    };
+    const double bb = reduce([&](long a, long e) {
+        double t = 0;
+        for (long i = a; i < e; i++) t += b[i] * b[i];
+        return t;
+    });
+    if (bb == 0) {
+        if (rel_resid_out) *rel_resid_out = 0;
+        return 0;
+    }
+    const double bn = std::sqrt(bb);
+    std::vector<double> r(b, b + C), z(C), pp(C), q(C);
+    auto ipp = [&](const double* in, double* out) {
+        parallel_for(threads, C, 1,
+                     [&](int64_t a, int64_t e) { ipp_rows(n.data(), mask.data(), fluid.data(), J, C, in, out, a, e); });
+    };
+    ipp(r.data(), z.data());
+    parallel_for(threads, C, 1, [&](int64_t a, int64_t e) { std::copy(z.begin() + a, z.begin() + e, pp.begin() + a); });
+    double rz = reduce([&](long a, long e) {
+        double t = 0;
+        for (long i = a; i < e; i++) t += r[i] * z[i];
+        return t;
+    });
+    int it = 0;
+    for (; it < iter_limit; it++) {
+        spmv_mt(threads, n.data(), mask.data(), fluid.data(), I, J, scale, pp.data(), q.data());
+        const double pq = reduce([&](long a, long e) {
+            double t = 0;
+            for (long i = a; i < e; i++) t += pp[i] * q[i];
+            return t;
+        });
+        const double alpha = rz / pq;
+        const double rr = reduce([&](long a, long e) {
+            double t = 0;
+            for (long i = a; i < e; i++) {
+                x[i] += alpha * pp[i];
+                r[i] -= alpha * q[i];
+                t += r[i] * r[i];
+            }
+            return t;
+        });
+        if (std::sqrt(rr) <= rel_tol * bn) {
+            it++;
+            break;
+        }
+        ipp(r.data(), z.data());
+        const double rz2 = reduce([&](long a, long e) {
+            double t = 0;
+            for (long i = a; i < e; i++) t += r[i] * z[i];
+            return t;
+        });
+        const double beta = rz2 / rz;
+        rz = rz2;
+        parallel_for(threads, C, 1, [&](int64_t a, int64_t e) {
+            for (long i = a; i < e; i++) pp[i] = z[i] + beta * pp[i];
+        });
+    }
+    if (rel_resid_out) *rel_resid_out = orc_rel_residual(m, I, J, scale, b, x);
+    return it;
+}
+
 /* ================================================================== */
 /* Composition: FlipSolver::step / stepFrame                              */
 /* ================================================================== */
@@ -1222,7 +1308,9 @@ static int32_t solve(const OrcParams* p, const int8_t* mat_a, double scale, cons
         return it;
     }
     const int32_t cap = std::max<int32_t>(20000, 20 * std::max(I, J));
-    const int32_t it = orc_pcg_converged(mat_a, I, J, scale, b, x, cap, p->project_tol, rel_resid);
+    const int32_t it = p->threads > 1
+                           ? pcg_converged_mt(p->threads, mat_a, I, J, scale, b, x, cap, p->project_tol, rel_resid)
+                           : orc_pcg_converged(mat_a, I, J, scale, b, x, cap, p->project_tol, rel_resid);
     *converged = *rel_resid <= p->project_tol * 10;
     return it;
 }

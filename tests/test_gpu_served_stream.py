@@ -46,7 +46,7 @@ def test_served_stream_under_binding_budget_vs_oracle(fg, orc, ctx):
     ids = {}
     try:
         t0 = time.time()
-        nxt, ticks, waited_at_budget = 0, 0, False
+        nxt, ticks = 0, 0
         while True:
             now = time.time() - t0
             while nxt < len(reqs) and reqs[nxt]["t"] <= now:
@@ -55,7 +55,6 @@ def test_served_stream_under_binding_budget_vs_oracle(fg, orc, ctx):
             t = srv.tick()
             ticks += 1
             assert t.resident_bytes <= budget
-            waited_at_budget |= t.queued > 0 and t.resident_bytes > budget // 2
             if nxt == len(reqs) and t.queued == 0 and t.resident == 0:
                 break
             if t.queued == 0 and t.resident == 0:   # idle until the next arrival
@@ -66,9 +65,9 @@ def test_served_stream_under_binding_budget_vs_oracle(fg, orc, ctx):
         _log(f"  {ticks} ticks in {time.time() - t0:.1f} s: completed {summ.completed}, rejected {summ.rejected}, "
              f"peak {summ.peak_resident_bytes / 2**30:.2f} GiB / {summ.peak_resident} scenes, "
              f"p50/p99 frame {summ.p50_frame_ms:.1f}/{summ.p99_frame_ms:.1f} ms, "
-             f"p99 admit wait {summ.p99_admit_wait_ms:.0f} ms")
+             f"p99 admit wait {summ.p99_admit_wait_ms:.0f} ms, {summ.budget_waits} ticks waited on the budget")
         assert summ.completed == len(reqs) and summ.rejected == 0 and summ.failed == 0
-        assert summ.peak_resident_bytes <= budget and waited_at_budget
+        assert summ.peak_resident_bytes <= budget and summ.budget_waits > 0 and summ.cap_waits == 0
         for k, rid in ids.items():
             s = srv.status(rid)
             assert s.state == fg.REQ_DONE and s.frames_done == reqs[k]["frames"], k
