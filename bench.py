@@ -77,7 +77,14 @@ SOLVE_FAMILIES = ["pcg_a", "pcg_b", "mg_down0", "mg_up0", "mg_resid", "mg_coarse
 # family's kernels in a run of that workload alone, divided by the library's
 # algorithmic bytes over the same launches (profiles/hbm_traffic.md; the
 # factors are what the same counters read for a 1 GiB copy kernel).
-MEASURED_TRAFFIC_PER_UNIT = {"mixed256": {}, "dam1024": {}, "proj4096": {}}
+MEASURED_TRAFFIC_PER_UNIT = {
+    "mixed256": {"p2g": 1.368, "particles": 0.695, "sdf": 2.13, "pcg_a": 1.09, "pcg_b": 1.014, "mg_down0": 0.345,
+                 "mg_up0": 0.951, "mg_resid": 1.835, "mg_coarse": 1.474},
+    "dam1024": {"p2g": 1.108, "particles": 2.042, "sdf": 0.638, "pcg_a": 1.96, "pcg_b": 1.005, "mg_down0": 0.47,
+                "mg_up0": 1.393, "mg_resid": 2.307, "mg_coarse": 2.632},
+    "proj4096": {"pcg_a": 1.863, "pcg_b": 1.0, "mg_down0": 0.973, "mg_up0": 2.41, "mg_resid": 2.846,
+                 "mg_coarse": 1.317},
+}
 
 
 def parse():
@@ -261,9 +268,12 @@ def batched_leg(fg, ctx, a, copy_gbs):
     # the named kernels: HIP events attached to each of their launches on the
     # stream they run on, over further frames of the same batch (events on
     # every launch slow the launch pipeline by ~7 %, so the timed frames above
-    # run without them)
+    # run without them), with the batch on one lane so that other lanes'
+    # kernels do not stretch the intervals
+    ctx.set_batch_lanes(1)
     table, dom = kernel_table(fg, ctx, STEP_FAMILIES, lambda: fg.batch_step_frame(ctx, batch), a.kernel_frames,
-                              "mixed256", copy_gbs, "frames of the same batch")
+                              "mixed256", copy_gbs, "frames of the same batch on one lane")
+    ctx.set_batch_lanes(lanes)
     for sc in batch + plain:
         sc.close()
     out = {
