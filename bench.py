@@ -78,12 +78,12 @@ SOLVE_FAMILIES = ["pcg_a", "pcg_b", "mg_down0", "mg_up0", "mg_resid", "mg_coarse
 # algorithmic bytes over the same launches (profiles/hbm_traffic.md; the
 # factors are what the same counters read for a 1 GiB copy kernel).
 MEASURED_TRAFFIC_PER_UNIT = {
-    "mixed256": {"p2g": 1.368, "particles": 0.695, "sdf": 2.13, "pcg_a": 1.09, "pcg_b": 1.014, "mg_down0": 0.345,
-                 "mg_up0": 0.951, "mg_resid": 1.835, "mg_coarse": 1.474},
-    "dam1024": {"p2g": 1.108, "particles": 2.042, "sdf": 0.638, "pcg_a": 1.96, "pcg_b": 1.005, "mg_down0": 0.47,
-                "mg_up0": 1.393, "mg_resid": 2.307, "mg_coarse": 2.632},
-    "proj4096": {"pcg_a": 1.863, "pcg_b": 1.0, "mg_down0": 0.973, "mg_up0": 2.41, "mg_resid": 2.846,
-                 "mg_coarse": 1.317},
+    "mixed256": {"p2g": 1.368, "particles": 0.695, "sdf": 2.13, "pcg_a": 1.089, "pcg_b": 1.014,
+                 "mg_down0": 0.345, "mg_up0": 0.951, "mg_resid": 1.085, "mg_coarse": 1.282},
+    "dam1024": {"p2g": 1.108, "particles": 2.042, "sdf": 0.638, "pcg_a": 1.96, "pcg_b": 1.005,
+                "mg_down0": 0.471, "mg_up0": 1.395, "mg_resid": 1.208, "mg_coarse": 1.865},
+    "proj4096": {"pcg_a": 1.863, "pcg_b": 1.0, "mg_down0": 0.973, "mg_up0": 2.41, "mg_resid": 1.877,
+                 "mg_coarse": 1.55},
 }
 
 

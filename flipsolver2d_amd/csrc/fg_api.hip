@@ -300,7 +300,7 @@ void carve(fg_scene* s, Carver& cv)
             MgLevel& L = M.lv[l];
             const int64_t CL = (int64_t)L.I * L.J;
             L.act = cv.take<uint8_t>(CL);
-            L.coef = cv.take<float>(kMgStencil * CL);
+            L.coef = cv.take<float>(kMgRowStride * CL);
             L.dinv = dinv ? dinv + doff : nullptr;
             doff += CL;
             L.r = cv.take<float>(CL);
@@ -728,6 +728,7 @@ struct Group {
     int64_t map_off[kMapKinds]{};       // offset of each kind's map in the lane's map buffer
     int mg_top = 0;                     // largest first level of k_mg_coarse (levels below: one launch each)
     unsigned mg_grid[kMgMaxLevels]{};   // blocks per scene over level l, one cell per thread
+    unsigned sym_grid[kMgMaxLevels]{};  // 16 x 16-row tiles per scene over level l (k_mg_sym)
     unsigned vc_grid[kMgMaxLevels]{};   // the same over the scenes whose V-cycle launches level l (l < t)
     unsigned mg_grid_d0 = 0;            // blocks per scene of k_mg_down0 (one level-1 cell per thread)
 };
@@ -843,6 +844,7 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
             for (int l = 1; l < s->mgh.n; l++) {
                 const unsigned b = (unsigned)cdiv((int64_t)s->mgh.lv[l].I * s->mgh.lv[l].J, kBlock);
                 G.mg_grid[l] = std::max(G.mg_grid[l], b);
+                G.sym_grid[l] = std::max(G.sym_grid[l], (unsigned)(cdiv(s->mgh.lv[l].I, 16) * cdiv(s->mgh.lv[l].J, 16)));
                 if (l < s->mgh.t) G.vc_grid[l] = std::max(G.vc_grid[l], b);
             }
             G.mg_grid_d0 = std::max(G.mg_grid_d0, (unsigned)cdiv((int64_t)s->mgh.lv[1].I * s->mgh.lv[1].J, kBlock));
@@ -1203,9 +1205,9 @@ int run_solve(Group& G, bool rhs_from_grid)
         const dim3 gl(G.mg_grid[l], (unsigned)G.n);
         launch_mg_galerkin(gl, st, G.d, l);
         if (l + 1 < G.mg_levels)
-            launch_mg_symmark(dim3(G.mg_grid[l] + G.mg_grid[l + 1], (unsigned)G.n), st, G.d, l, (int)G.mg_grid[l]);
+            launch_mg_symmark(dim3(G.sym_grid[l] + G.mg_grid[l + 1], (unsigned)G.n), st, G.d, l, (int)G.sym_grid[l]);
         else
-            launch_mg_sym(gl, st, G.d, l);
+            launch_mg_sym(dim3(G.sym_grid[l], (unsigned)G.n), st, G.d, l);
     }
     // A poll is a host round trip; launches after every scene of the group
     // has stopped run empty.  The first poll of a solve comes after as many

@@ -29,6 +29,7 @@ constexpr int kMgBottomMax = 16;         // the bottom level has at most this ma
 constexpr int kMgBottomSweeps = 8;       // L1-Jacobi sweeps on the bottom level
 constexpr float kMgOmega = 1.5f;         // L1-Jacobi weight: x += w (r - A x) / sum_k |a_k| (SPD for w < 2)
 constexpr int kMgStencil = 25;           // 5x5 Galerkin stencil of the coarse levels
+constexpr int kMgRowStride = 32;         // floats per stored coefficient row: one 128-byte line, 25 used
 // k_mg_coarse runs levels t .. n-1 of a scene in one block, a thread per
 // cell: t is the first level >= 1 with at most kMgCoarseTop cells whose
 // levels t .. n-1 hold at most kMgCoarseCells cells together (their r stays
@@ -82,7 +83,7 @@ struct SceneScalars {
 struct MgLevel {
     int32_t I, J;
     uint8_t* act;     // the cell takes part: one of its 2x2 finer cells does
-    float* coef;      // [kMgStencil][I*J]: entry k couples the cell to (i + k/5 - 2, j + k%5 - 2)
+    float* coef;      // [I*J][kMgRowStride], row by row: entry k couples the cell to (i + k/5 - 2, j + k%5 - 2)
     float* dinv;      // kMgOmega / sum_k |coef_k| on active cells, +0 on inactive ones (-0 on an
                       // active cell whose row is all zero, so that a nonzero bit
                       // pattern marks exactly the active cells); negative on rows

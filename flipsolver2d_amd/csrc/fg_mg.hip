@@ -95,22 +95,32 @@ __device__ __forceinline__ float prolong_at(P y, int I, int J, int fi, int fj)
     return acc;
 }
 
+// The 25 stored coefficients of row c: one 128-byte line (the row's
+// kMgRowStride floats), seven 16-byte loads issued together.
+typedef float coef4 __attribute__((ext_vector_type(4)));   // a builtin vector: loads from any address space
+__device__ __forceinline__ void load_coef_row(gptr<const float> coef, int c, float (&a)[kMgStencil])
+{
+    gptr<const coef4> r4 = (gptr<const coef4>)(coef + (size_t)c * kMgRowStride);
+    coef4 q[7];
+#pragma unroll
+    for (int u = 0; u < 7; u++) q[u] = r4[u];
+#pragma unroll
+    for (int k = 0; k < kMgStencil; k++) a[k] = q[k >> 2][k & 3];
+}
+
 // sum_k a_k v_k over an active level-l row (dinv != 0) with the level's 25
 // stencil entries: the canonical stencil on canonical rows (dinv < 0), else
-// the stored coefficients, loaded (all 25 together, one wait) only by the
-// lanes whose rows need them.
+// the stored coefficients, loaded only by the lanes whose rows need them.
 template <typename MT>
 __device__ __forceinline__ float row_dot(const MT& M, int l, gptr<const float> coef, int CL, int c,
                                          float dinv_c, const float (&v)[kMgStencil])
 {
+    (void)CL;
     const auto& cn = M.canon[l];
     float a[kMgStencil];
 #pragma unroll
     for (int k = 0; k < kMgStencil; k++) a[k] = cn[k];
-    if (dinv_c > 0.f) {
-#pragma unroll
-        for (int k = 0; k < kMgStencil; k++) a[k] = coef[(size_t)k * CL + c];
-    }
+    if (dinv_c > 0.f) load_coef_row(coef, c, a);
     float ax = 0.f;
 #pragma unroll
     for (int k = 0; k < kMgStencil; k++) ax += a[k] * v[k];
@@ -296,7 +306,7 @@ __device__ __forceinline__ void galerkin_row(const SceneDev& s, const MT& M, int
                 const uint32_t f = (uint32_t)(min(max(fi, 0), FI - 1) * FJ + min(max(fj, 0), FJ - 1));
 #pragma unroll
                 for (int e = 0; e < kMgStencil; e++)
-                    v[h][e] = *(gptr<const float>)(fb + ((uint32_t)e * (uint32_t)CF + f) * 4u);
+                    v[h][e] = *(gptr<const float>)(fb + (f * (uint32_t)kMgRowStride + (uint32_t)e) * 4u);
             }
             static_for<2>([&](auto h2) {
                 constexpr int h = decltype(h2)::value, b = b0 + h;
@@ -308,7 +318,7 @@ __device__ __forceinline__ void galerkin_row(const SceneDev& s, const MT& M, int
         });
     }
 #pragma unroll
-    for (int k = 0; k < kMgStencil; k++) L.coef[(size_t)k * CL + c] = (float)(0.25 * acc[k]);
+    for (int k = 0; k < kMgStencil; k++) L.coef[(size_t)c * kMgRowStride + k] = (float)(0.25 * acc[k]);
 }
 
 // An inactive row must hold zeros; its dinv, not yet rewritten for this
@@ -328,7 +338,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_galerkin(const SceneDev* scenes, 
     if (!L.act[c]) {
         if (__float_as_uint(gbl<const float>(L.dinv)[c]) != 0u) {
 #pragma unroll
-            for (int k = 0; k < kMgStencil; k++) L.coef[(size_t)k * CL + c] = 0.f;
+            for (int k = 0; k < kMgStencil; k++) L.coef[(size_t)c * kMgRowStride + k] = 0.f;
         }
         return;
     }
@@ -342,35 +352,67 @@ __global__ void __launch_bounds__(kBlock) k_mg_galerkin(const SceneDev* scenes, 
 // Rows equal to the level's canonical interior stencil (every entry within
 // 1e-5 of the canonical diagonal; other rows differ by more than 1 %) are
 // flagged by a negative dinv and read the canonical stencil later.
+// A block takes a 16 x 16 tile of rows: the upper entries (13..24) of the
+// tile's rows and of its two-row halo are staged in LDS with row-contiguous
+// loads (a row's entries share one line), so the mirrored reads, which
+// touch twelve neighbouring rows each, come from LDS.
+constexpr int kSymTile = 16, kSymR = kSymTile + 4, kSymUp = 13;   // 12 entries, padded to an odd stride
+__device__ __forceinline__ int sym_tiles(int LI, int LJ)
+{
+    return ((LI + kSymTile - 1) / kSymTile) * ((LJ + kSymTile - 1) / kSymTile);
+}
+
 __device__ __forceinline__ void mg_sym_block(const SceneDev& s, int l, int bx)
 {
+    static_assert(kBlock == kSymTile * kSymTile, "one tile row per thread");
     if (!s.active || s.precond != kPrecondMG) return;
     const auto& M = *cst(s.mg);
     if (l >= M.n) return;
     const auto& L = M.lv[l];
-    const int LI = L.I, LJ = L.J, CL = LI * LJ;
-    if (bx * kBlock >= CL) return;   // the whole block is past the level
-    const int c0 = bx * kBlock + threadIdx.x;
-    const bool valid = c0 < CL;
-    const int c = valid ? c0 : CL - 1;   // past the level: recompute the last row, store nothing
-    const int ci = c / LJ, cj = c - ci * LJ;
+    const int LI = L.I, LJ = L.J;
+    if (bx >= sym_tiles(LI, LJ)) return;   // the whole block is past the level
+    __shared__ float up[kSymR * kSymR][kSymUp];   // entries 13..24 of the region's rows
+    const int tilesJ = (LJ + kSymTile - 1) / kSymTile;
+    const int ti = bx / tilesJ, tj = bx - ti * tilesJ;
+    const int ri0 = ti * kSymTile - 2, rj0 = tj * kSymTile - 2;
     gptr<float> coef = gbl(L.coef);
-    gptr<const char> cb = (gptr<const char>)coef;
-    float v[kMgStencil];
-    const uint8_t av = gbl<const uint8_t>(L.act)[c];
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    // stage: region row q's floats 12..27 as four 16-byte loads, entries 13..24 kept
+    for (int w = threadIdx.x; w < kSymR * kSymR * 4; w += kBlock) {
+        const int q = w >> 2, part = w & 3;
+        const int gi = ri0 + q / kSymR, gj = rj0 + q % kSymR;
+        const bool in = gi >= 0 && gi < LI && gj >= 0 && gj < LJ;
+        const int n = min(max(gi, 0), LI - 1) * LJ + min(max(gj, 0), LJ - 1);
+        const v4 x = *(gptr<const v4>)(coef + (size_t)n * kMgRowStride + 12 + 4 * part);
+        const float m = mask01(in);
 #pragma unroll
-    for (int k = 0; k < kMgStencil; k++) {   // every load unconditional: clamped neighbour, 0/1 factor
-        if (k < 12) {
-            const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
-            const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
-            const uint32_t n = (uint32_t)(min(max(ni, 0), LI - 1) * LJ + min(max(nj, 0), LJ - 1));
-            const float m = *(gptr<const float>)(cb + ((uint32_t)(24 - k) * (uint32_t)CL + n) * 4u);
-            v[k] = m * mask01(ok);
-        } else {
-            v[k] = *(gptr<const float>)(cb + ((uint32_t)k * (uint32_t)CL + (uint32_t)c) * 4u);
+        for (int e = 0; e < 4; e++) {
+            const int k = 12 + 4 * part + e;
+            if (k >= 13 && k <= 24) up[q][k - 13] = x[e] * m;
         }
     }
-    float cn[kMgStencil];   // read before the stores below (which the compiler cannot tell apart from them)
+    __syncthreads();
+    const int li = threadIdx.x / kSymTile, lj = threadIdx.x % kSymTile;
+    const int ci0 = ti * kSymTile + li, cj0 = tj * kSymTile + lj;
+    const bool valid = ci0 < LI && cj0 < LJ;
+    const int ci = min(ci0, LI - 1), cj = min(cj0, LJ - 1), c = ci * LJ + cj;
+    float v[kMgStencil];
+    {   // own row: entries 12..24 (and the next three floats) in four loads
+        v4 o[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) o[u] = *(gptr<const v4>)(coef + (size_t)c * kMgRowStride + 12 + 4 * u);
+#pragma unroll
+        for (int k = 12; k < kMgStencil; k++) v[k] = o[(k - 12) >> 2][(k - 12) & 3];
+    }
+    const uint8_t av = gbl<const uint8_t>(L.act)[c];
+    // lower entries: the mirrored upper entry 24 - k of neighbour k (outside
+    // the level: 0, as staged)
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        const int q = (li + 2 + k / 5 - 2) * kSymR + (lj + 2 + k % 5 - 2);
+        v[k] = up[q][(24 - k) - 13];
+    }
+    float cn[kMgStencil];
 #pragma unroll
     for (int k = 0; k < kMgStencil; k++) cn[k] = M.canon[l][k];
     const float tol = 1e-5f * cn[12];
@@ -382,10 +424,15 @@ __device__ __forceinline__ void mg_sym_block(const SceneDev& s, int l, int bx)
         canon_row = canon_row && fabsf(v[k] - cn[k]) <= tol;
     }
     const bool act = av != 0;
-    // an inactive row is all zero already (k_mg_galerkin), mirrored entries included
+    // an inactive row is all zero already (k_mg_galerkin), mirrored entries
+    // included; the lower entries go out as three 16-byte stores
+    if (valid && act) {
 #pragma unroll
-    for (int k = 0; k < 12; k++)
-        if (valid && act) coef[(size_t)k * CL + c] = v[k];
+        for (int u = 0; u < 3; u++) {
+            const v4 x = {v[4 * u], v[4 * u + 1], v[4 * u + 2], v[4 * u + 3]};
+            *(gptr<v4>)(coef + (size_t)c * kMgRowStride + 4 * u) = x;
+        }
+    }
     const float di = (act && sum > 0.f) ? kMgOmega / sum : 0.f;
     if (valid) gbl(L.dinv)[c] = (canon_row && di > 0.f) ? -di : (act && di == 0.f ? -0.f : di);
     // active rows that will read their stored coefficients (for the kernels'
@@ -549,12 +596,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(const SceneDev* scenes, 
 // no bounds tests.
 __device__ __forceinline__ bool mg_active(float dinv) { return __float_as_uint(dinv) != 0u; }
 
-// The 25 stored coefficients of row c (k-major layout), loaded together.
+// The 25 stored coefficients of row c, loaded together.
 __device__ __forceinline__ void coarse_row_load(gptr<const float> coef, int CL, int c, float (&a)[kMgStencil])
 {
-    gptr<const char> base = (gptr<const char>)coef;
-#pragma unroll
-    for (int k = 0; k < kMgStencil; k++) a[k] = *(gptr<const float>)(base + (uint32_t)(k * CL + c) * 4u);
+    (void)CL;
+    load_coef_row(coef, c, a);
 }
 
 // row_dot with v_k = V(k) made as it is summed (same terms, same order):
@@ -580,7 +626,8 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
     __shared__ float tx[kMgCoarseTop], xw[kMgCoarseTop];
     __shared__ float xb[2][W * W];
     __shared__ float cns[kMgMaxLevels][kMgStencil];
-    __shared__ float cbs[kMgStencil * kMgBottomMax * kMgBottomMax];   // bottom coefficients, k-major
+    constexpr int kBS = kMgBottomMax * kMgBottomMax + 1;   // the bottom's entry stride in LDS (max)
+    __shared__ float cbs[kMgStencil * kBS];   // bottom coefficients, entry-major
     const SceneDev& s = scenes[blockIdx.y];
     const auto& M = *cst(s.mg);
     const int t = M.t, nb = M.n - 1;
@@ -625,16 +672,23 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
 #pragma unroll
     for (int k = 0; k < kMgStencil; k++) ct[k] = 0.f;
     if (tin && nb != t && dct > 0.f) coarse_row_load(gbl<const float>(T.coef), CT, tc, ct);
-    {   // the bottom's coefficients (k-major, contiguous) to LDS by the whole block
+    // the bottom's entry stride in LDS: a multiple of the 32 banks plus one,
+    // so that the copy below (consecutive threads: consecutive entries of a
+    // row) and a sweep (consecutive cells: one entry) are both conflict-free
+    const int SB = ((CB + 31) & ~31) + 1;
+    {   // the bottom's coefficient rows (contiguous) to LDS by the whole block,
+        // transposed to entry-major there
         gptr<const float> bcoef = gbl<const float>(B.coef);
-        const int NBC = kMgStencil * CB;
-        constexpr int QB = (kMgStencil * kMgBottomMax * kMgBottomMax + NT - 1) / NT;
+        const int NBC = kMgRowStride * CB;
+        constexpr int QB = (kMgRowStride * kMgBottomMax * kMgBottomMax + NT - 1) / NT;
         float v[QB];
 #pragma unroll
         for (int q = 0; q < QB; q++) v[q] = bcoef[min(tid + q * NT, NBC - 1)];
 #pragma unroll
-        for (int q = 0; q < QB; q++)
-            if (tid + q * NT < NBC) cbs[tid + q * NT] = v[q];
+        for (int q = 0; q < QB; q++) {
+            const int i = tid + q * NT, c = i / kMgRowStride, k = i % kMgRowStride;
+            if (i < NBC && k < kMgStencil) cbs[k * SB + c] = v[q];
+        }
     }
     for (int l = t; l < nb; l++) {   // down: x = r dinv, y = r - A x, r_{l+1} = R y
         const auto& L = M.lv[l];
@@ -687,7 +741,7 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
                 float ax = 0.f;
 #pragma unroll
                 for (int k = 0; k < kMgStencil; k++)
-                    ax += cbs[k * CB + bc] * xb[cur][home + (k / 5 - 2) * W + (k % 5 - 2)];
+                    ax += cbs[k * SB + bc] * xb[cur][home + (k / 5 - 2) * W + (k % 5 - 2)];
                 xb[cur ^ 1][home] = xb[cur][home] + dv * (rv - ax);
             }
             __syncthreads();
