@@ -34,7 +34,7 @@ EXPORTS = [
     "fg_ctx_batch_lanes", "fg_batch_autotune", "fg_ctx_set_solver_tiling", "fg_ctx_set_preconditioner",
     "fg_ctx_preconditioner", "fg_batch_substep", "fg_server_cancel", "fg_ctx_work_read", "fg_scene_init_objects", "fg_scene_set_emitters",
     "fg_scene_max_velocity", "fg_poisson_solve_matrix",
-    "fg_poisson_solve_noalloc",
+    "fg_poisson_solve_noalloc", "fg_ctx_set_solver_graphs",
 ]
 MAX_LANES = 4
 REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_REJECTED, REQ_RELEASED, REQ_FAILED, REQ_CANCELLED = range(7)
@@ -187,6 +187,7 @@ def lib():
             "fg_ctx_batch_lanes": (C.c_int, [vp, C.POINTER(i32)]),
             "fg_ctx_set_solver_tiling": (C.c_int, [vp, i32, i32]),
             "fg_ctx_set_preconditioner": (C.c_int, [vp, i32]),
+            "fg_ctx_set_solver_graphs": (C.c_int, [vp, i32]),
             "fg_ctx_preconditioner": (C.c_int, [vp, C.POINTER(i32)]),
             "fg_batch_autotune": (C.c_int, [vp, vp, i32, i32, C.POINTER(i32), vp]),
             "fg_server_create": (C.c_int, [vp, C.c_uint64, i32, C.POINTER(vp)]),
@@ -255,6 +256,10 @@ class Context:
     def set_preconditioner(self, kind):
         """PRECOND_MG (multigrid V-cycle, default) or PRECOND_IPP for later pressure solves."""
         _check(lib().fg_ctx_set_preconditioner(self.h, kind))
+
+    def set_solver_graphs(self, on=True):
+        """Replay each PCG iteration's launches from a captured HIP graph (bit-identical results)."""
+        _check(lib().fg_ctx_set_solver_graphs(self.h, 1 if on else 0))
 
     def preconditioner(self):
         v = C.c_int32()

@@ -54,8 +54,8 @@ int fail(int code, const std::string& msg)
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Block maps of the solve's per-iteration launches, by kind: 0 = solver
-// chunks, 1 = level-1 cells one per thread (k_mg_down0), 1 + l = level-l
-// cells one per thread (l >= 1).
+// chunks, 1 = level-1 tiles (k_mg_down0), 1 + l = level-l cells one per
+// thread (l >= 1).
 constexpr int kMapKinds = 1 + kMgMaxLevels;
 constexpr int kMapMaxScenes = 4096;   // scenes one map build handles (LDS prefix)
 constexpr int kFlagWords = 4 + kMapKinds;
@@ -89,6 +89,19 @@ struct Lane {
     int64_t prof_units = 0;
     // work units of every family since the last fg_ctx_work_read reset
     int64_t work[FG_K_NUM] = {};
+    // PCG iterations captured as graphs (fg_ctx_set_solver_graphs), by
+    // iteration index and launch shape
+    struct IterGraph {
+        const SceneDev* d;
+        const int2* map[kMapKinds];
+        unsigned blocks[kMapKinds];
+        unsigned ny;
+        bool flat;
+        int it, mg_top, mg_levels;
+        hipGraphExec_t exec;
+    };
+    std::vector<IterGraph> graphs;
+    bool graphs_off = false;   // the stream cannot be captured: launch directly
 };
 
 struct fg_ctx {
@@ -103,6 +116,7 @@ struct fg_ctx {
     int pcg_target_blocks = kPcgMaxBlocks;   // solver tiling: blocks per scene at most
     int pcg_min_chunk = kPcgMinChunk;        // and cells per block at least
     int precond = kPrecondMG;                // pressure PCG preconditioner
+    bool solver_graphs = false;              // PCG iterations replayed from HIP graphs
     std::vector<Lane*> lanes;
 };
 
@@ -645,7 +659,7 @@ __device__ int map_blocks(const SceneDev& s, int kind)
     if (kind == 0) return s.pcg_blocks;
     if (s.precond != kPrecondMG) return 0;
     const MgLevels& M = *s.mg;
-    if (kind == 1) return (M.lv[1].I * M.lv[1].J + kBlock - 1) / kBlock;
+    if (kind == 1) return mg_d0_tiles(M.lv[1].I, M.lv[1].J);
     const int l = kind - 1;
     if (l >= M.t) return 0;   // level l runs inside k_mg_coarse
     return (M.lv[l].I * M.lv[l].J + kBlock - 1) / kBlock;
@@ -730,7 +744,7 @@ struct Group {
     unsigned mg_grid[kMgMaxLevels]{};   // blocks per scene over level l, one cell per thread
     unsigned sym_grid[kMgMaxLevels]{};  // 16 x 16-row tiles per scene over level l (k_mg_sym)
     unsigned vc_grid[kMgMaxLevels]{};   // the same over the scenes whose V-cycle launches level l (l < t)
-    unsigned mg_grid_d0 = 0;            // blocks per scene of k_mg_down0 (one level-1 cell per thread)
+    unsigned mg_grid_d0 = 0;            // blocks per scene of k_mg_down0 (level-1 tiles)
 };
 
 void lane_free_buffers(Lane* L)
@@ -757,6 +771,7 @@ void lane_destroy(Lane* L)
     if (L->bmap) hipFree(L->bmap);
     if (L->bgeo) hipFree(L->bgeo);
     for (hipEvent_t e : L->ev_pool) hipEventDestroy(e);
+    for (auto& g : L->graphs) hipGraphExecDestroy(g.exec);
     if (L->own_stream && L->stream) hipStreamDestroy(L->stream);
     delete L;
 }
@@ -847,7 +862,7 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
                 G.sym_grid[l] = std::max(G.sym_grid[l], (unsigned)(cdiv(s->mgh.lv[l].I, 16) * cdiv(s->mgh.lv[l].J, 16)));
                 if (l < s->mgh.t) G.vc_grid[l] = std::max(G.vc_grid[l], b);
             }
-            G.mg_grid_d0 = std::max(G.mg_grid_d0, (unsigned)cdiv((int64_t)s->mgh.lv[1].I * s->mgh.lv[1].J, kBlock));
+            G.mg_grid_d0 = std::max(G.mg_grid_d0, (unsigned)mg_d0_tiles(s->mgh.lv[1].I, s->mgh.lv[1].J));
         }
         s->h.pblocks = (int)std::max<int64_t>(1, cdiv(std::max<int64_t>(s->n_bound, 1), kBlock));
         const Geo gs = geometry(s);
@@ -871,7 +886,7 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
             cap[0] += s->pcg_blocks;
             if (s->h.precond != kPrecondMG) continue;
             const MgLevels& M = s->mgh;
-            cap[1] += cdiv((int64_t)M.lv[1].I * M.lv[1].J, kBlock);
+            cap[1] += mg_d0_tiles(M.lv[1].I, M.lv[1].J);
             for (int l = 1; l < M.t; l++) cap[1 + l] += cdiv((int64_t)M.lv[l].I * M.lv[l].J, kBlock);
         }
         int64_t total = 0;
@@ -953,6 +968,7 @@ struct SolveLaunch {
     const SceneDev* d = nullptr;
     unsigned ny = 0;                    // scenes in d
     bool flat = false;
+    bool stable = false;                // the same shape at every solve (every scene of the group)
     const int2* map[kMapKinds] = {};
     unsigned blocks[kMapKinds] = {};    // flat: total blocks of the kind; else blocks per scene
     dim3 grid(int kind) const { return flat ? dim3(blocks[kind]) : dim3(blocks[kind], ny); }
@@ -964,6 +980,7 @@ SolveLaunch group_launch(const Group& G, const SceneDev* d, unsigned ny)
     SolveLaunch S;
     S.d = d;
     S.ny = ny;
+    S.stable = d == G.d;
     S.blocks[0] = G.g.pcg.x;
     S.blocks[1] = G.mg_grid_d0;
     for (int l = 1; l < kMgMaxLevels; l++) S.blocks[1 + l] = G.vc_grid[l];
@@ -1012,8 +1029,9 @@ int64_t map_blocks_host(const fg_scene* s, int kind)
     if (kind == 0) return s->pcg_blocks;
     if (s->h.precond != kPrecondMG) return 0;
     const MgLevels& M = s->mgh;
-    const int l = kind == 1 ? 1 : kind - 1;
-    if (kind > 1 && l >= M.t) return 0;
+    if (kind == 1) return mg_d0_tiles(M.lv[1].I, M.lv[1].J);
+    const int l = kind - 1;
+    if (l >= M.t) return 0;
     return cdiv((int64_t)M.lv[l].I * M.lv[l].J, kBlock);
 }
 
@@ -1031,6 +1049,7 @@ void all_scene_launch(Group& G, SolveLaunch* S)
     S->d = G.d;
     S->ny = (unsigned)G.n;
     S->flat = true;
+    S->stable = true;
     for (int k = 0; k < kinds; k++) {
         int64_t total = 0;
         for (const fg_scene* s : G.sc) total += map_blocks_host(s, k);
@@ -1169,6 +1188,88 @@ void launch_vcycle(const Group& G, hipStream_t st, const SolveLaunch& S, int whi
     kp.done(0);
 }
 
+// One PCG iteration it: p, q = A'p and p.q; r, x; z = M r (the V-cycle, when
+// the group uses it).
+void launch_iteration(const Group& G, hipStream_t st, const SolveLaunch& S, int it)
+{
+    {
+        KProf kp(G, FG_K_PCG_A, true);
+        launch_pcg_a(S.grid(0), st, S.d, S.map[0], it);
+        kp.done(0);   // units: at the end of the solve
+    }
+    {
+        KProf kp(G, FG_K_PCG_B, true);
+        launch_pcg_b(S.grid(0), st, S.d, S.map[0], it);
+        kp.done(0);
+    }
+    if (G.mg_levels) {
+        // the new residual: iteration it writes cr2 for even it, cr for odd
+        KProf kp(G, FG_K_VCYCLE);
+        launch_vcycle(G, st, S, (it & 1) ? 0 : 1);
+        kp.done(0);
+    }
+}
+
+// launch_iteration replayed from the lane's graph of (it, S), captured on
+// first use.  *used false: launch directly (graphs off, a kernel being
+// profiled, a stream that cannot be captured, or a launch over a packed
+// subset of the group, whose shape changes from poll to poll: capturing
+// those costs more than replaying saves).
+constexpr size_t kMaxIterGraphs = 512;   // per lane; the cache restarts beyond
+int iteration_graph(const Group& G, hipStream_t st, const SolveLaunch& S, int it, bool* used)
+{
+    *used = false;
+    Lane* L = G.L;
+    if (!G.ctx->solver_graphs || G.ctx->prof_kernel >= 0 || L->graphs_off || !S.stable) return FG_OK;
+    auto same = [&](const Lane::IterGraph& g) {
+        if (g.d != S.d || g.ny != S.ny || g.flat != S.flat || g.it != it || g.mg_top != G.mg_top ||
+            g.mg_levels != G.mg_levels)
+            return false;
+        for (int k = 0; k < kMapKinds; k++)
+            if (g.map[k] != S.map[k] || g.blocks[k] != S.blocks[k]) return false;
+        return true;
+    };
+    for (const auto& g : L->graphs)
+        if (same(g)) {
+            HIPCHK(hipGraphLaunch(g.exec, st));
+            *used = true;
+            return FG_OK;
+        }
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        (void)hipGetLastError();
+        L->graphs_off = true;
+        return FG_OK;
+    }
+    launch_iteration(G, st, S, it);
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamEndCapture(st, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(FG_E_DEVICE, std::string("graph: ") + hipGetErrorString(e));
+    if (L->graphs.size() >= kMaxIterGraphs) {
+        HIPCHK(hipStreamSynchronize(st));
+        for (auto& g : L->graphs) hipGraphExecDestroy(g.exec);
+        L->graphs.clear();
+    }
+    Lane::IterGraph g{};
+    g.d = S.d;
+    for (int k = 0; k < kMapKinds; k++) {
+        g.map[k] = S.map[k];
+        g.blocks[k] = S.blocks[k];
+    }
+    g.ny = S.ny;
+    g.flat = S.flat;
+    g.it = it;
+    g.mg_top = G.mg_top;
+    g.mg_levels = G.mg_levels;
+    g.exec = exec;
+    L->graphs.push_back(g);
+    HIPCHK(hipGraphLaunch(exec, st));
+    *used = true;
+    return FG_OK;
+}
+
 // Algorithmic bytes of one k_mg_coarse block (one scene, one V-cycle): the
 // dinv of its levels (4 B a cell), its input (r_1, or the finer level's y it
 // restricts), the y it writes (4 B a level-t cell), once each the 25
@@ -1260,22 +1361,9 @@ int run_solve(Group& G, bool rhs_from_grid)
         int chunk = sweep > 0 ? kPollNext : hint > 0 ? std::min(hint + 1, kPollMax) : kPollFirst;
         while (count > 0) {
             for (int k = 0; k < chunk; k++, it++) {
-                {
-                    KProf kp(G, FG_K_PCG_A, true);
-                    launch_pcg_a(S.grid(0), st, S.d, S.map[0], it);
-                    kp.done(0);   // units: at the end of the solve
-                }
-                {
-                    KProf kp(G, FG_K_PCG_B, true);
-                    launch_pcg_b(S.grid(0), st, S.d, S.map[0], it);
-                    kp.done(0);
-                }
-                if (G.mg_levels) {
-                    // the new residual: iteration it writes cr2 for even it, cr for odd
-                    KProf kp(G, FG_K_VCYCLE);
-                    launch_vcycle(G, st, S, (it & 1) ? 0 : 1);
-                    kp.done(0);
-                }
+                bool graphed = false;
+                FGCHK(iteration_graph(G, st, S, it, &graphed));
+                if (!graphed) launch_iteration(G, st, S, it);
             }
             launch_refine_residual(g.pcg, st, G.d, 1);
             launch_cg_init(g.pcg, st, G.d, sweep + 1, 1);
@@ -2405,6 +2493,13 @@ int fg_ctx_set_preconditioner(fg_ctx* ctx, int32_t kind)
     if (!ctx) return fail(FG_E_INVALID, "ctx is NULL");
     if (kind != FG_PRECOND_MG && kind != FG_PRECOND_IPP) return fail(FG_E_INVALID, "bad preconditioner");
     ctx->precond = kind == FG_PRECOND_MG ? kPrecondMG : kPrecondIPP;
+    return FG_OK;
+}
+
+int fg_ctx_set_solver_graphs(fg_ctx* ctx, int32_t enabled)
+{
+    if (!ctx) return fail(FG_E_INVALID, "ctx is NULL");
+    ctx->solver_graphs = enabled != 0;
     return FG_OK;
 }
 

@@ -26,6 +26,12 @@ constexpr int kSdfLanes = 4;             // lanes per 3x3-cell bin in the fluid 
 constexpr int kPrecondMG = 0, kPrecondIPP = 1;
 constexpr int kMgMaxLevels = 12;
 constexpr int kMgBottomMax = 16;         // the bottom level has at most this many cells per side
+constexpr int kD0Tile = 16;              // k_mg_down0: level-1 cells a block, per side
+// blocks a scene of k_mg_down0: kD0Tile x kD0Tile tiles of the level-1 grid
+__host__ __device__ inline int mg_d0_tiles(int I1, int J1)
+{
+    return ((I1 + kD0Tile - 1) / kD0Tile) * ((J1 + kD0Tile - 1) / kD0Tile);
+}
 constexpr int kMgBottomSweeps = 8;       // L1-Jacobi sweeps on the bottom level
 constexpr float kMgOmega = 1.5f;         // L1-Jacobi weight: x += w (r - A x) / sum_k |a_k| (SPD for w < 2)
 constexpr int kMgStencil = 25;           // 5x5 Galerkin stencil of the coarse levels
@@ -346,6 +352,11 @@ __device__ __forceinline__ float mask01(bool ok)
     asm volatile("" : "+v"(m));
     return m;
 }
+
+// The solver chunk a block of a (blocks, scenes) grid works on: chunks in
+// XCD-tiled order, so the rows a chunk shares with its neighbours (the
+// stencil's reach) are read by blocks of one L2.
+__device__ __forceinline__ int solver_block() { return xcd_tile(blockIdx.x, gridDim.x); }
 
 struct SceneBlock {
     int scene, block;

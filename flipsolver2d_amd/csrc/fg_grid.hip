@@ -873,10 +873,11 @@ __global__ void __launch_bounds__(kBlock) k_forces(const SceneDev* scenes)
 __global__ void __launch_bounds__(kBlock) k_rhs(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
+    const int blk = solver_block();
     if (!s.active) return;
-    if ((int)blockIdx.x >= s.pcg_blocks) return;
+    if (blk >= s.pcg_blocks) return;
     const int I = s.I, J = s.J;
-    const Chunk ch = my_chunk(s, blockIdx.x);
+    const Chunk ch = my_chunk(s, blk);
     double bb = 0.0;
     for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
         const int i = c / J, j = c - (c / J) * J;
@@ -904,7 +905,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(const SceneDev* scenes)
         bb += val * val;
     }
     double v[1] = {bb};
-    put_partials<1>(s, kSlotBB, v, blockIdx.x);
+    put_partials<1>(s, kSlotBB, v, blk);
 }
 
 // ---------------------------------------------------------------------------

@@ -465,59 +465,88 @@ __global__ void __launch_bounds__(kBlock) k_mg_symmark(const SceneDev* scenes, i
 // ---------------------------------------------------------------------------
 
 // Level 0 -> 1: x0 = w r / D1 on the fine level, res0 = r - A' x0, and
-// r1 = R res0, one level-1 cell per thread over its 4x4 fine support (a 6x6
-// fine patch with the stencil's reach).
+// r1 = R res0 over each level-1 cell's 4x4 fine support.  A block owns a
+// kD0Tile x kD0Tile tile of level-1 cells (mg_d0_tiles blocks a scene) and
+// stages its fine patch (the supports with the stencil's reach, kD0R cells
+// a side) in LDS: code and r, then x0 once per fine cell, then res0 once
+// per fine cell of the supports, then each thread sums its cell's 16
+// weighted res0.  Each value is formed by the same expression as it would
+// be per coarse cell, so r1 does not depend on the tiling.
+constexpr int kD0R = 2 * kD0Tile + 4;
 __global__ void __launch_bounds__(kBlock) k_mg_down0(const SceneDev* scenes, const int2* map, int which)
 {
+    static_assert(kD0Tile * kD0Tile == kBlock, "one level-1 cell per thread");
+    __shared__ float xs[kD0R][kD0R + 1];   // x0
+    __shared__ float rs[kD0R][kD0R + 1];   // r, then res0 (own cell only)
+    __shared__ uint8_t cs[kD0R][kD0R + 4];
     const SceneBlock sb = scene_block(map, true);
     const SceneDev& s = scenes[sb.scene];
     const auto& L1 = cst(s.mg)->lv[1];
-    const int J1 = L1.J;
-    const int c = sb.block * kBlock + threadIdx.x;
-    // one test after the uniform loads, so that they issue together
-    if (!mg_iterating(s) | (c >= L1.I * J1)) return;
+    const int I1 = L1.I, J1 = L1.J;
+    const int tjn = (J1 + kD0Tile - 1) / kD0Tile;
+    // one test after the uniform loads, so that they issue together; the
+    // whole block takes it
+    if (!mg_iterating(s) | (sb.block >= mg_d0_tiles(I1, J1))) return;
+    const int ti = sb.block / tjn, tj = sb.block - ti * tjn;
     const int I = s.I, J = s.J;
+    const int fi0 = 2 * kD0Tile * ti - 2, fj0 = 2 * kD0Tile * tj - 2;   // the patch's first fine cell
     gptr<const float> r = gbl<const float>(which ? s.cr2 : s.cr);
     gptr<const uint8_t> code = gbl<const uint8_t>(s.stencil);
+    {   // code, r and x0 of the patch (outside the grid: code 0, r 0)
+        constexpr int Q = (kD0R * kD0R + kBlock - 1) / kBlock;
+        uint8_t kq[Q];
+        float vq[Q];
+#pragma unroll
+        for (int u = 0; u < Q; u++) {
+            const int q = min((int)threadIdx.x + u * kBlock, kD0R * kD0R - 1);
+            const int fi = fi0 + q / kD0R, fj = fj0 + q % kD0R;
+            const int f = min(max(fi, 0), I - 1) * J + min(max(fj, 0), J - 1);   // clamped, not selected
+            kq[u] = code[f];
+            vq[u] = r[f];
+        }
+#pragma unroll
+        for (int u = 0; u < Q; u++) {
+            const int q = threadIdx.x + u * kBlock;
+            if (q >= kD0R * kD0R) break;
+            const int a = q / kD0R, b = q % kD0R;
+            const int fi = fi0 + a, fj = fj0 + b;
+            const bool in = fi >= 0 && fi < I && fj >= 0 && fj < J;
+            const uint8_t k = in ? kq[u] : 0;
+            const float rv = vq[u] * mask01(in);
+            const float d = l1_diag0(k);
+            xs[a][b] = ((k & 0x80) && d > 0.f) ? kMgOmega * rv / d : 0.f;
+            rs[a][b] = rv;
+            cs[a][b] = k;
+        }
+    }
+    __syncthreads();
+    // res0 on the supports (patch cells 1 .. kD0R-2 each way); 0 off fluid rows
+    for (int q = threadIdx.x; q < (kD0R - 2) * (kD0R - 2); q += kBlock) {
+        const int a = 1 + q / (kD0R - 2), b = 1 + q % (kD0R - 2);
+        const uint8_t k = cs[a][b];
+        float ax = (float)((k >> 4) & 7) * xs[a][b];
+        ax -= (k & 1) ? xs[a - 1][b] : 0.f;
+        ax -= (k & 2) ? xs[a + 1][b] : 0.f;
+        ax -= (k & 4) ? xs[a][b - 1] : 0.f;
+        ax -= (k & 8) ? xs[a][b + 1] : 0.f;
+        rs[a][b] = (k & 0x80) ? rs[a][b] - ax : 0.f;
+    }
+    __syncthreads();
+    const int li = threadIdx.x / kD0Tile, lj = threadIdx.x % kD0Tile;
+    const int ci = ti * kD0Tile + li, cj = tj * kD0Tile + lj;
+    if ((ci >= I1) | (cj >= J1)) return;
+    const int c = ci * J1 + cj;
     if (!gbl<const uint8_t>(L1.act)[c]) {   // inactive coarse cell: nothing to restrict
         gbl(L1.r)[c] = 0.f;
         return;
     }
-    const int ci = c / J1, cj = c - ci * J1;
-    uint8_t cd[6][6];
-    float rv[6][6];
-#pragma unroll
-    for (int a = 0; a < 6; a++)
-#pragma unroll
-        for (int b = 0; b < 6; b++) {
-            const int fi = 2 * ci - 2 + a, fj = 2 * cj - 2 + b;
-            const bool in = fi >= 0 && fi < I && fj >= 0 && fj < J;
-            const int f = min(max(fi, 0), I - 1) * J + min(max(fj, 0), J - 1);   // clamped, not selected
-            const uint8_t k = code[f];
-            const float v = r[f];
-            cd[a][b] = in ? k : 0;
-            rv[a][b] = v * mask01(in);
-        }
-    float x0[6][6];
-#pragma unroll
-    for (int a = 0; a < 6; a++)
-#pragma unroll
-        for (int b = 0; b < 6; b++) {
-            const float d = l1_diag0(cd[a][b]);
-            x0[a][b] = ((cd[a][b] & 0x80) && d > 0.f) ? kMgOmega * rv[a][b] / d : 0.f;
-        }
     float acc = 0.f;
 #pragma unroll
     for (int a = 1; a < 5; a++)
 #pragma unroll
         for (int b = 1; b < 5; b++) {
-            const uint8_t k = cd[a][b];
-            float ax = (float)((k >> 4) & 7) * x0[a][b];
-            ax -= (k & 1) ? x0[a - 1][b] : 0.f;
-            ax -= (k & 2) ? x0[a + 1][b] : 0.f;
-            ax -= (k & 4) ? x0[a][b - 1] : 0.f;
-            ax -= (k & 8) ? x0[a][b + 1] : 0.f;
-            acc += (k & 0x80) ? (w1(a - 1) * w1(b - 1)) * (rv[a][b] - ax) : 0.f;
+            const uint8_t k = cs[2 * li + a][2 * lj + b];
+            acc += (k & 0x80) ? (w1(a - 1) * w1(b - 1)) * rs[2 * li + a][2 * lj + b] : 0.f;
         }
     gbl(L1.r)[c] = 0.25f * acc;
 }
@@ -850,7 +879,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_post(const SceneDev* scenes, cons
 // partials of r.z and r.r the next PCG iteration reduces (solver chunks).
 __global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, const int2* map, int which)
 {
-    const SceneBlock sb = scene_block(map, false);
+    const SceneBlock sb = scene_block(map, true);
     const SceneDev& s = scenes[sb.scene];
     const auto& L1 = cst(s.mg)->lv[1];
     if (!mg_iterating(s) | (sb.block >= s.pcg_blocks)) return;

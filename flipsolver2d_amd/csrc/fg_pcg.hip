@@ -57,13 +57,13 @@ __global__ void __launch_bounds__(kBlock) k_refine_residual(const SceneDev* scen
     const SceneDev& s = scenes[blockIdx.y];
     cptr<SceneScalars> scr = cst(s.sc);
     const int od = scr->outer_done, idn = scr->inner_done;
-    const int nb = s.pcg_blocks;
-    if (!s.active | (od != 0) | ((spec != 0) & (idn == 0)) | ((int)blockIdx.x >= nb)) return;
+    const int nb = s.pcg_blocks, blk = solver_block();
+    if (!s.active | (od != 0) | ((spec != 0) & (idn == 0)) | (blk >= nb)) return;
     double bb[1];
     sum_partials<1>(s, kSlotBB, nb, bb);
-    if (blockIdx.x == 0 && threadIdx.x == 0) s.sc->bb = bb[0];
+    if (blk == 0 && threadIdx.x == 0) s.sc->bb = bb[0];
     const int J = s.J;
-    const Chunk ch = my_chunk(s, blockIdx.x);
+    const Chunk ch = my_chunk(s, blk);
     double rr = 0.0;
     for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
         const uint8_t code = s.stencil[c];
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(kBlock) k_refine_residual(const SceneDev* scen
         s.cr[c] = (float)((code & 0x80) ? r64 / s.sA : r64);
     }
     double v[1] = {rr};
-    put_partials<1>(s, kSlotRR64, v, blockIdx.x);
+    put_partials<1>(s, kSlotRR64, v, blk);
 }
 
 constexpr int kItems = 2;                  // cells per thread per pass (4: 1.7 % slower on the batch, 8: 7 %)
@@ -101,14 +101,14 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const SceneDev* scenes, int 
     SceneScalars* sc = s.sc;
     cptr<SceneScalars> scr = cst(sc);
     const int od = scr->outer_done, idn = scr->inner_done;
-    const int nb = s.pcg_blocks;
-    if (!s.active | (od != 0) | ((spec != 0) & (idn == 0)) | ((int)blockIdx.x >= nb)) return;
+    const int nb = s.pcg_blocks, blk = solver_block();
+    if (!s.active | (od != 0) | ((spec != 0) & (idn == 0)) | (blk >= nb)) return;
     double rr64[1];
     sum_partials<1>(s, kSlotRR64, nb, rr64);
     const double bb = sc->bb;
     if (spec) {
         const double bn = sqrt(bb), rn = sqrt(rr64[0]);
-        if (blockIdx.x == 0 && threadIdx.x == 0 && (!finite_d(rr64[0]) || rn <= s.tol * bn || bn == 0.0)) {
+        if (blk == 0 && threadIdx.x == 0 && (!finite_d(rr64[0]) || rn <= s.tol * bn || bn == 0.0)) {
             sc->rr64 = rr64[0];
             if (!finite_d(rr64[0])) sc->numeric_bad = 1;
             sc->outer_done = 1;
@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const SceneDev* scenes, int 
         // the warm start (see k_rhs) is far off (residual above 5 ||b||, or
         // not finite): start from zero, r = b (every block resets its
         // chunk; all see the same test)
-        const Chunk ch = my_chunk(s, blockIdx.x);
+        const Chunk ch = my_chunk(s, blk);
         for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
             const double b = s.b[c];
             s.x64[c] = 0.0;
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const SceneDev* scenes, int 
         rr64[0] = bb;
     }
     const double bn = sqrt(bb), rn = sqrt(rr64[0]);
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const bool lead = blk == 0 && threadIdx.x == 0;
     if (!finite_d(rr64[0]) || rn <= s.tol * bn || bn == 0.0 || sweep >= kMaxSweeps) {
         if (lead) {
             sc->rr64 = rr64[0];
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const SceneDev* scenes, int 
     }
     if (s.precond == kPrecondMG) return;   // the V-cycle forms z and the partials
     const int J = s.J;
-    const Chunk ch = my_chunk(s, blockIdx.x);
+    const Chunk ch = my_chunk(s, blk);
     const float* __restrict__ r = s.cr;
     const uint8_t* __restrict__ code = s.stencil;
     double rz = 0.0, rr = 0.0;
@@ -180,13 +180,13 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const SceneDev* scenes, int 
         rr += (double)rc * (double)rc;
     }
     double v[2] = {rz, rr};
-    put_partials<2>(s, kSlotCG, v, blockIdx.x);
+    put_partials<2>(s, kSlotCG, v, blk);
 }
 
 // KA: p = z + beta p_old; q = A' p; p.q        (z lives in cu)
 __global__ void __launch_bounds__(kBlock) k_pcg_a(const SceneDev* scenes, const int2* map, int it)
 {
-    const SceneBlock sb = scene_block(map, false);
+    const SceneBlock sb = scene_block(map, true);
     const SceneDev& s = scenes[sb.scene];
     SceneScalars* sc = s.sc;
     cptr<SceneScalars> scr = cst(sc);   // values earlier kernels wrote: scalar loads
@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_a(const SceneDev* scenes, const 
 // IPP; with the V-cycle, r and x only)
 __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const int2* map, int it)
 {
-    const SceneBlock sb = scene_block(map, false);
+    const SceneBlock sb = scene_block(map, true);
     const SceneDev& s = scenes[sb.scene];
     SceneScalars* sc = s.sc;
     cptr<SceneScalars> scr = cst(sc);   // values earlier kernels wrote: scalar loads
@@ -443,9 +443,10 @@ __global__ void __launch_bounds__(kBlock) k_pcg_b(const SceneDev* scenes, const 
 __global__ void __launch_bounds__(kBlock) k_solve_begin(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
+    const int blk = solver_block();
     if (!s.active) return;
-    if ((int)blockIdx.x >= s.pcg_blocks) return;
-    const Chunk ch = my_chunk(s, blockIdx.x);
+    if (blk >= s.pcg_blocks) return;
+    const Chunk ch = my_chunk(s, blk);
     double bb = 0.0;
     for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
         const double b = s.b[c];
@@ -453,7 +454,7 @@ __global__ void __launch_bounds__(kBlock) k_solve_begin(const SceneDev* scenes)
         s.x64[c] = 0.0;
     }
     double v[1] = {bb};
-    put_partials<1>(s, kSlotBB, v, blockIdx.x);
+    put_partials<1>(s, kSlotBB, v, blk);
 }
 
 // reset the per-solve scalars (one thread per scene)

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define FG_ABI_VERSION 6
+#define FG_ABI_VERSION 7
 
 enum fg_status {
     FG_OK = 0,
@@ -194,6 +194,19 @@ int fg_ctx_set_solver_tiling(fg_ctx* ctx, int32_t target_blocks, int32_t min_chu
 #define FG_PRECOND_IPP 1
 int fg_ctx_set_preconditioner(fg_ctx* ctx, int32_t kind);
 int fg_ctx_preconditioner(fg_ctx* ctx, int32_t* kind);
+/* Replay each PCG iteration of later solves (the direction update, the
+ * residual update and the V-cycle: about 20 kernel launches) from a HIP
+ * graph instead of launching its kernels one by one.  A lane captures one
+ * graph per iteration index and launch shape on first use and keeps it
+ * until the context is destroyed; the kernels and their arguments are the
+ * same, so results are bit-identical either way.  Only launches over every
+ * scene of a group are replayed (a single scene, a batch's first sweep up
+ * to its first poll); launches over the packed subset of a batch's scenes
+ * that still iterate change shape at every poll and stay direct, as does
+ * per-launch kernel profiling (fg_ctx_profile_kernel).  Default off.  No
+ * counterpart in the reference (its PCG loop runs on the host,
+ * linearsolver.cpp:25-73). */
+int fg_ctx_set_solver_graphs(fg_ctx* ctx, int32_t enabled);
 /* Autotuner for that schedule: steps `rounds` frames of the batch with each
  * lane count 1..FG_MAX_LANES (order alternating per round; the simulation
  * advances normally), measures particle-substeps per second for each, and
