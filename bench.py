@@ -66,7 +66,7 @@ KERNEL_BYTES = {
     "pcg_a": ("K_PCG_A", 17.0, "cells", ["k_pcg_a"]),
     "pcg_b": ("K_PCG_B", 32.0, "cells", ["k_pcg_b"]),
     "mg_down0": ("K_MG_DOWN0", 1.0, "bytes", ["k_mg_down0"]),
-    "mg_up0": ("K_MG_UP0", 1.0, "bytes", ["k_mg_up0"]),
+    "mg_up0": ("K_MG_UP0", 1.0, "bytes", ["k_mg_up0", "k_mg_up0_wide"]),
     "mg_resid": ("K_MG_RESID", 1.0, "bytes", ["k_mg_resid"]),
     "mg_coarse": ("K_MG_COARSE", 1.0, "bytes", ["k_mg_coarse"]),
 }
@@ -104,8 +104,6 @@ def parse():
     ap.add_argument("--tune-rounds", type=int, default=1,
                     help="autotuner rounds (each steps 4 frames; 0: no tuning, --lanes or 1 lane)")
     ap.add_argument("--dam-frames", type=int, default=10, help="timed frames of the 1024^2 leg (0: skip)")
-    ap.add_argument("--graph-frames", type=int, default=4,
-                    help="frames of the graph-vs-stream comparison in the batched and 1024^2 legs (0: skip)")
     ap.add_argument("--proj-res", type=int, default=4096, help="isolated projection grid (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="served-stream window (0: skip)")
     ap.add_argument("--serve-max-resident", type=int, default=4096,
@@ -278,12 +276,6 @@ def batched_leg(fg, ctx, a, copy_gbs):
     ctx.set_batch_lanes(lanes)
     for sc in batch + plain:
         sc.close()
-    graphs = graph_compare(fg, ctx, lambda: [fg.make_scene(ctx, sp, tol=a.tol, pcg_iter_limit=4000) for sp in specs],
-                           lambda bt: fg.batch_step_frame(ctx, bt), lambda bt: [sc.close() for sc in bt],
-                           a.graph_frames)
-    if graphs:
-        log(f"batched graphs vs stream ({lanes} lanes): {graphs['graph_ms_per_frame']:.2f} vs "
-            f"{graphs['stream_ms_per_frame']:.2f} ms")
     out = {
         "value": ps_b / el_b,
         "ms_per_step": 1e3 * el_b / a.steps,
@@ -300,7 +292,6 @@ def batched_leg(fg, ctx, a, copy_gbs):
         "algorithmic_bytes_total": work_totals(fg, ctx),
         "lanes": lanes,
         "autotune_rates": [float(r) for r in rates],
-        "graph_vs_stream": graphs,
     }
     if plain:
         assert ps_p == ps_b, "plain and batched sets diverged"
@@ -312,34 +303,6 @@ def batched_leg(fg, ctx, a, copy_gbs):
         }
         out["batched_over_plain"] = out["value"] / out["plain"]["value"]
     return out
-
-
-def graph_compare(fg, ctx, make, step, close, frames):
-    """Two copies of one workload stepped in lock step from the same state,
-    one launching each PCG iteration's kernels directly and one replaying
-    them from captured HIP graphs (fg_ctx_set_solver_graphs); the same
-    frames in the same order, so they cost the same work.  Two untimed
-    frames each first (the graphs are captured there)."""
-    if frames <= 0:
-        return None
-    copies = (make(), make())
-    ms = ([], [])
-    try:
-        for k in range(2 + frames):
-            for g in ((0, 1) if k % 2 == 0 else (1, 0)):
-                ctx.set_solver_graphs(g == 1)
-                t0 = time.perf_counter()
-                step(copies[g])
-                if k >= 2:
-                    ms[g].append(1e3 * (time.perf_counter() - t0))
-    finally:
-        ctx.set_solver_graphs(False)
-        for c in copies:
-            close(c)
-    s, g = sum(ms[0]) / frames, sum(ms[1]) / frames
-    return {"frames": frames, "stream_ms_per_frame": s, "graph_ms_per_frame": g, "graph_over_stream": g / s,
-            "note": "whole frames (all stages); graphs replay each PCG iteration's ~20 launches over every "
-                    "scene of the group; identical copies, alternating order"}
 
 
 def dam_leg(fg, ctx, a, copy_gbs):
@@ -363,10 +326,6 @@ def dam_leg(fg, ctx, a, copy_gbs):
                               "frames")
     scene.close()
     log(f"dam break {res}^2: {a.dam_frames} frames, {subs} substeps, {el:.2f} s")
-    graphs = graph_compare(fg, ctx, lambda: fg.dam_break_scene(ctx, res, ppc=ppc, tol=a.tol, pcg_iter_limit=4000),
-                           lambda sc: sc.step_frame(), lambda sc: sc.close(), a.graph_frames)
-    if graphs:
-        log(f"dam break graphs vs stream: {graphs['graph_ms_per_frame']:.2f} vs {graphs['stream_ms_per_frame']:.2f} ms")
     return {
         "workload": f"plain single-scene dam break {res}x{res}, {ppc} ppc ({n0} particles)",
         "value": psub / el,
@@ -380,7 +339,6 @@ def dam_leg(fg, ctx, a, copy_gbs):
         "roofline": table.get(dom),
         "kernels": table,
         "algorithmic_bytes_total": work_totals(fg, ctx),
-        "graph_vs_stream": graphs,
     }
 
 
@@ -592,7 +550,6 @@ def main():
         "roofline": b["kernels"].get(dom),
         "kernels": b["kernels"],
         "algorithmic_bytes_total": b["algorithmic_bytes_total"],
-        "graph_vs_stream": b["graph_vs_stream"],
         "cpu_baseline": cpu,
         "dam_break_1024": dam,
         "isolated_projection": proj,

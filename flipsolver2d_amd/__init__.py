@@ -258,7 +258,7 @@ class Context:
         _check(lib().fg_ctx_set_preconditioner(self.h, kind))
 
     def set_solver_graphs(self, on=True):
-        """Replay each PCG iteration's launches from a captured HIP graph (bit-identical results)."""
+        """Graph replay of PCG iterations is switched off: on=True raises (FG_E_INVALID)."""
         _check(lib().fg_ctx_set_solver_graphs(self.h, 1 if on else 0))
 
     def preconditioner(self):

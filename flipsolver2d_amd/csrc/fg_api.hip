@@ -89,19 +89,6 @@ struct Lane {
     int64_t prof_units = 0;
     // work units of every family since the last fg_ctx_work_read reset
     int64_t work[FG_K_NUM] = {};
-    // PCG iterations captured as graphs (fg_ctx_set_solver_graphs), by
-    // iteration index and launch shape
-    struct IterGraph {
-        const SceneDev* d;
-        const int2* map[kMapKinds];
-        unsigned blocks[kMapKinds];
-        unsigned ny;
-        bool flat;
-        int it, mg_top, mg_levels;
-        hipGraphExec_t exec;
-    };
-    std::vector<IterGraph> graphs;
-    bool graphs_off = false;   // the stream cannot be captured: launch directly
 };
 
 struct fg_ctx {
@@ -116,7 +103,6 @@ struct fg_ctx {
     int pcg_target_blocks = kPcgMaxBlocks;   // solver tiling: blocks per scene at most
     int pcg_min_chunk = kPcgMinChunk;        // and cells per block at least
     int precond = kPrecondMG;                // pressure PCG preconditioner
-    bool solver_graphs = false;              // PCG iterations replayed from HIP graphs
     std::vector<Lane*> lanes;
 };
 
@@ -736,6 +722,7 @@ struct Group {
     int64_t cells = 0, faces = 0, parts = 0;
     int max_inner = 0;
     int max_bands = 0;
+    int max_j = 0;                      // the widest scene's J
     bool sources = false;               // some scene has SOURCE cells (afterTransfer, reseeding)
     int mg_levels = 0;                  // most multigrid levels of a scene using the V-cycle
     int map_kinds = 0;                  // block-map kinds the solve uses (0 when maps are off)
@@ -771,7 +758,6 @@ void lane_destroy(Lane* L)
     if (L->bmap) hipFree(L->bmap);
     if (L->bgeo) hipFree(L->bgeo);
     for (hipEvent_t e : L->ev_pool) hipEventDestroy(e);
-    for (auto& g : L->graphs) hipGraphExecDestroy(g.exec);
     if (L->own_stream && L->stream) hipStreamDestroy(L->stream);
     delete L;
 }
@@ -870,6 +856,7 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
         else geo_max(G.g, gs);
         first = false;
         G.cells += s->C;
+        G.max_j = std::max(G.max_j, s->J);
         G.faces += s->FU + s->FV;
         G.parts += s->n_bound;
         G.max_inner = std::max(G.max_inner, s->h.max_inner);
@@ -968,7 +955,6 @@ struct SolveLaunch {
     const SceneDev* d = nullptr;
     unsigned ny = 0;                    // scenes in d
     bool flat = false;
-    bool stable = false;                // the same shape at every solve (every scene of the group)
     const int2* map[kMapKinds] = {};
     unsigned blocks[kMapKinds] = {};    // flat: total blocks of the kind; else blocks per scene
     dim3 grid(int kind) const { return flat ? dim3(blocks[kind]) : dim3(blocks[kind], ny); }
@@ -980,7 +966,6 @@ SolveLaunch group_launch(const Group& G, const SceneDev* d, unsigned ny)
     SolveLaunch S;
     S.d = d;
     S.ny = ny;
-    S.stable = d == G.d;
     S.blocks[0] = G.g.pcg.x;
     S.blocks[1] = G.mg_grid_d0;
     for (int l = 1; l < kMgMaxLevels; l++) S.blocks[1 + l] = G.vc_grid[l];
@@ -1049,7 +1034,6 @@ void all_scene_launch(Group& G, SolveLaunch* S)
     S->d = G.d;
     S->ny = (unsigned)G.n;
     S->flat = true;
-    S->stable = true;
     for (int k = 0; k < kinds; k++) {
         int64_t total = 0;
         for (const fg_scene* s : G.sc) total += map_blocks_host(s, k);
@@ -1184,7 +1168,10 @@ void launch_vcycle(const Group& G, hipStream_t st, const SolveLaunch& S, int whi
         launch_mg_post(S.grid(1 + l), st, S.d, S.map[1 + l], l);
     }
     KProf kp(G, FG_K_MG_UP0, true);
-    launch_mg_up0(S.grid(0), st, S.d, S.map[0], which);
+    // the LDS window of k_mg_up0 pays for a single scene whose solver
+    // chunks span four rows or more
+    const bool ring = G.n == 1 && G.sc[0]->pcg_chunk >= 4 * (int64_t)G.sc[0]->J;
+    launch_mg_up0(S.grid(0), st, S.d, S.map[0], which, G.max_j, ring);
     kp.done(0);
 }
 
@@ -1208,66 +1195,6 @@ void launch_iteration(const Group& G, hipStream_t st, const SolveLaunch& S, int 
         launch_vcycle(G, st, S, (it & 1) ? 0 : 1);
         kp.done(0);
     }
-}
-
-// launch_iteration replayed from the lane's graph of (it, S), captured on
-// first use.  *used false: launch directly (graphs off, a kernel being
-// profiled, a stream that cannot be captured, or a launch over a packed
-// subset of the group, whose shape changes from poll to poll: capturing
-// those costs more than replaying saves).
-constexpr size_t kMaxIterGraphs = 512;   // per lane; the cache restarts beyond
-int iteration_graph(const Group& G, hipStream_t st, const SolveLaunch& S, int it, bool* used)
-{
-    *used = false;
-    Lane* L = G.L;
-    if (!G.ctx->solver_graphs || G.ctx->prof_kernel >= 0 || L->graphs_off || !S.stable) return FG_OK;
-    auto same = [&](const Lane::IterGraph& g) {
-        if (g.d != S.d || g.ny != S.ny || g.flat != S.flat || g.it != it || g.mg_top != G.mg_top ||
-            g.mg_levels != G.mg_levels)
-            return false;
-        for (int k = 0; k < kMapKinds; k++)
-            if (g.map[k] != S.map[k] || g.blocks[k] != S.blocks[k]) return false;
-        return true;
-    };
-    for (const auto& g : L->graphs)
-        if (same(g)) {
-            HIPCHK(hipGraphLaunch(g.exec, st));
-            *used = true;
-            return FG_OK;
-        }
-    if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) {
-        (void)hipGetLastError();
-        L->graphs_off = true;
-        return FG_OK;
-    }
-    launch_iteration(G, st, S, it);
-    hipGraph_t graph = nullptr;
-    HIPCHK(hipStreamEndCapture(st, &graph));
-    hipGraphExec_t exec = nullptr;
-    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
-    if (e != hipSuccess) return fail(FG_E_DEVICE, std::string("graph: ") + hipGetErrorString(e));
-    if (L->graphs.size() >= kMaxIterGraphs) {
-        HIPCHK(hipStreamSynchronize(st));
-        for (auto& g : L->graphs) hipGraphExecDestroy(g.exec);
-        L->graphs.clear();
-    }
-    Lane::IterGraph g{};
-    g.d = S.d;
-    for (int k = 0; k < kMapKinds; k++) {
-        g.map[k] = S.map[k];
-        g.blocks[k] = S.blocks[k];
-    }
-    g.ny = S.ny;
-    g.flat = S.flat;
-    g.it = it;
-    g.mg_top = G.mg_top;
-    g.mg_levels = G.mg_levels;
-    g.exec = exec;
-    L->graphs.push_back(g);
-    HIPCHK(hipGraphLaunch(exec, st));
-    *used = true;
-    return FG_OK;
 }
 
 // Algorithmic bytes of one k_mg_coarse block (one scene, one V-cycle): the
@@ -1360,11 +1287,7 @@ int run_solve(Group& G, bool rhs_from_grid)
         int it = 0;
         int chunk = sweep > 0 ? kPollNext : hint > 0 ? std::min(hint + 1, kPollMax) : kPollFirst;
         while (count > 0) {
-            for (int k = 0; k < chunk; k++, it++) {
-                bool graphed = false;
-                FGCHK(iteration_graph(G, st, S, it, &graphed));
-                if (!graphed) launch_iteration(G, st, S, it);
-            }
+            for (int k = 0; k < chunk; k++, it++) launch_iteration(G, st, S, it);
             launch_refine_residual(g.pcg, st, G.d, 1);
             launch_cg_init(g.pcg, st, G.d, sweep + 1, 1);
             if (G.n > 1) {
@@ -2499,7 +2422,7 @@ int fg_ctx_set_preconditioner(fg_ctx* ctx, int32_t kind)
 int fg_ctx_set_solver_graphs(fg_ctx* ctx, int32_t enabled)
 {
     if (!ctx) return fail(FG_E_INVALID, "ctx is NULL");
-    ctx->solver_graphs = enabled != 0;
+    if (enabled) return fail(FG_E_INVALID, "graph replay of PCG iterations is switched off");
     return FG_OK;
 }
 

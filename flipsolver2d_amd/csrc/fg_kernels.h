@@ -34,6 +34,19 @@ inline void fg_launch(F kernel, dim3 grid, dim3 block, hipStream_t st, Args... a
     }
 }
 
+// The same with `shm` bytes of dynamic LDS.
+template <typename F, typename... Args>
+inline void fg_launch_shm(F kernel, dim3 grid, dim3 block, size_t shm, hipStream_t st, Args... args)
+{
+    LaunchEvents& ev = next_launch_events();
+    if (ev.start) {
+        hipExtLaunchKernelGGL(kernel, grid, block, shm, st, ev.start, ev.stop, 0, args...);
+        ev = LaunchEvents{};
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shm, st, args...);
+    }
+}
+
 #define FG_DECLARE_LAUNCH(name, ...) void launch_##name(dim3 grid, hipStream_t st, const SceneDev* scenes, ##__VA_ARGS__)
 
 // fg_particles.hip
@@ -48,7 +61,9 @@ FG_DECLARE_LAUNCH(mg_restrict, const int2* map, int l);
 FG_DECLARE_LAUNCH(mg_coarse);
 FG_DECLARE_LAUNCH(mg_prolong, const int2* map, int l);
 FG_DECLARE_LAUNCH(mg_post, const int2* map, int l);
-FG_DECLARE_LAUNCH(mg_up0, const int2* map, int which);
+// max_j: the largest J of the launch's scenes (sizes the LDS of k_mg_up0);
+// ring: k_mg_up0 (its LDS window) rather than k_mg_up0_wide
+FG_DECLARE_LAUNCH(mg_up0, const int2* map, int which, int max_j, bool ring);
 
 FG_DECLARE_LAUNCH(max_v2);
 FG_DECLARE_LAUNCH(advect, int alive_from_key);

@@ -877,7 +877,124 @@ __global__ void __launch_bounds__(kBlock) k_mg_post(const SceneDev* scenes, cons
 // Level 0, up: z0 = w r / D1 + P y1 at the cell and its fluid neighbours, then
 // z = z0 + w (r - A' z0) / D1; z = r on non-fluid rows.  Writes z and the
 // partials of r.z and r.r the next PCG iteration reduces (solver chunks).
+// z0 of a fine cell depends on that cell alone, so a block forms it once
+// per cell into an LDS ring over the flat cell index (slot f mod RB, RB =
+// 2J + 2 kBlock floats of dynamic LDS) instead of once per neighbour: pass
+// p (cells p .. p + kBlock - 1, thread t taking cell p + t as a plain
+// kBlock stride would, so the partials keep their order) reads the ring
+// over [p - J, p + J + kBlock); the prologue fills that window of the
+// first pass, and each pass fills the kBlock cells the next one adds (their
+// loads issued before this pass's reads, their slots those of the cells
+// just behind this pass's window).  The window costs a prologue of 2J +
+// kBlock cells per block and a barrier per pass: it pays where chunks span
+// several rows of a grid too large for the L2 to keep the neighbouring
+// rows (4096^2: 115.8 -> 90.7 us a launch); on one-row chunks (1024^2) and
+// on batches of small scenes k_mg_up0_wide is faster (14.3 against 18.3 us,
+// 19.4 against 23.8 us), and the host launches that there.
+constexpr int kUpMaxJ = (16384 - 2 * kBlock) / 2;   // the ring within 64 KB
+constexpr int kUpFill = 4;                          // prologue cells a thread per round of loads
 __global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, const int2* map, int which)
+{
+    extern __shared__ float zr[];
+    const SceneBlock sb = scene_block(map, true);
+    const SceneDev& s = scenes[sb.scene];
+    const auto& L1 = cst(s.mg)->lv[1];
+    if (!mg_iterating(s) | (sb.block >= s.pcg_blocks)) return;
+    const int I = s.I, J = s.J, C = I * J;
+    gptr<const float> y1 = gbl<const float>(L1.y);
+    const int I1 = L1.I, J1 = L1.J;
+    gptr<const float> r = gbl<const float>(which ? s.cr2 : s.cr);
+    gptr<const uint8_t> code = gbl<const uint8_t>(s.stencil);
+    gptr<float> z = gbl(s.cu);
+    const Chunk ch = my_chunk(s, sb.block);
+    const int RB = 2 * J + 2 * kBlock;
+    // z0 of cell f from its code, r and the four level-1 values of P y1 at it
+    auto z0_of = [&](uint8_t kf, float rf, float pg) {
+        const float dg = l1_diag0(kf);
+        return (dg > 0.f ? kMgOmega * rf / dg : 0.f) + pg;
+    };
+    {   // prologue: the first pass's window
+        const int lo = max(ch.begin - J, 0), hi = min(ch.begin + J + kBlock, C);
+        for (int f0 = lo; f0 < hi; f0 += kUpFill * kBlock) {
+            uint8_t kq[kUpFill];
+            float rq[kUpFill], pq[kUpFill];
+#pragma unroll
+            for (int u = 0; u < kUpFill; u++) {
+                const int f = min(f0 + (int)threadIdx.x + u * kBlock, hi - 1);   // clamped, not selected
+                const int fi = f / J, fj = f - fi * J;
+                kq[u] = code[f];
+                rq[u] = r[f];
+                pq[u] = prolong_at(y1, I1, J1, fi, fj);
+            }
+#pragma unroll
+            for (int u = 0; u < kUpFill; u++) {
+                const int f = f0 + threadIdx.x + u * kBlock;
+                if (f < hi) zr[f % RB] = z0_of(kq[u], rq[u], pq[u]);
+            }
+        }
+    }
+    __syncthreads();
+    const int di[5] = {0, -1, 1, 0, 0}, dj[5] = {0, 0, 0, -1, 1};
+    double rz = 0.0, rr = 0.0;
+    int sp = ch.begin % RB;   // the ring slot of cell p
+    for (int p = ch.begin; p < ch.end; p += kBlock) {
+        // the next pass's new cell of this thread: its loads first
+        const int f = p + J + kBlock + threadIdx.x;
+        const bool fill = (p + kBlock < ch.end) & (f < C);
+        const int fc = min(f, C - 1);
+        const int fi = fc / J, fj = fc - fi * J;
+        const uint8_t kf = code[fc];
+        const float rf = r[fc];
+        const float pf = prolong_at(y1, I1, J1, fi, fj);
+        const int c = p + threadIdx.x;
+        if (c < ch.end) {
+            const uint8_t k = code[c];
+            const float rc = r[c];
+            float zc = rc;
+            if (k & 0x80) {
+                const int ci = c / J, cj = c - ci * J;
+                // the fluid neighbours that enter A' (not the row-wrapping coupling)
+                bool use[5];
+                use[0] = true;
+#pragma unroll
+                for (int e = 1; e < 5; e++) use[e] = (k & (1 << (e - 1))) != 0;
+                use[3] = use[3] && cj > 0;
+                use[4] = use[4] && cj < J - 1;
+                int sc = sp + (int)threadIdx.x;
+                sc -= sc >= RB ? RB : 0;
+                float z0[5];
+#pragma unroll
+                for (int e = 0; e < 5; e++) {
+                    const int gi = min(max(ci + di[e], 0), I - 1), gj = min(max(cj + dj[e], 0), J - 1);
+                    int sg = sc + (gi - ci) * J + (gj - cj);
+                    sg += sg < 0 ? RB : 0;
+                    sg -= sg >= RB ? RB : 0;
+                    z0[e] = use[e] ? zr[sg] : 0.f;
+                }
+                const float dc = l1_diag0(k);
+                float ax = (float)((k >> 4) & 7) * z0[0];
+#pragma unroll
+                for (int e = 1; e < 5; e++) ax -= z0[e];
+                zc = ((k & 0x80) && dc > 0.f) ? z0[0] + kMgOmega * (rc - ax) / dc : rc;
+                rz += (double)rc * (double)zc;
+            } else {   // identity row: z = r
+                rz += (double)rc * (double)rc;
+            }
+            z[c] = zc;
+            rr += (double)rc * (double)rc;
+        }
+        if (fill) zr[f % RB] = z0_of(kf, rf, pf);
+        __syncthreads();
+        sp += kBlock;
+        sp -= sp >= RB ? RB : 0;
+    }
+    double v[2] = {rz, rr};
+    put_partials<2>(s, kSlotCG, v, sb.block);
+}
+
+// k_mg_up0 for grids wider than its LDS window allows: each cell forms the
+// smoothing start z0 of its fluid neighbours itself (the same values).
+__global__ void __launch_bounds__(kBlock) k_mg_up0_wide(const SceneDev* scenes, const int2* map, int which)
 {
     const SceneBlock sb = scene_block(map, true);
     const SceneDev& s = scenes[sb.scene];
@@ -961,7 +1078,13 @@ FG_MG_LAUNCH_MAP(mg_resid, int, l)
 FG_MG_LAUNCH_MAP(mg_restrict, int, l)
 FG_MG_LAUNCH_MAP(mg_prolong, int, l)
 FG_MG_LAUNCH_MAP(mg_post, int, l)
-FG_MG_LAUNCH_MAP(mg_up0, int, which)
+void launch_mg_up0(dim3 g, hipStream_t st, const SceneDev* sc, const int2* map, int which, int max_j, bool ring)
+{
+    if (ring && max_j <= kUpMaxJ)
+        fg_launch_shm(k_mg_up0, g, dim3(kBlock), sizeof(float) * (2 * max_j + 2 * kBlock), st, sc, map, which);
+    else
+        fg_launch(k_mg_up0_wide, g, dim3(kBlock), st, sc, map, which);
+}
 void launch_mg_coarse(dim3 g, hipStream_t st, const SceneDev* sc)
 {
     fg_launch(k_mg_coarse, g, dim3(kMgCoarseThreads), st, sc);

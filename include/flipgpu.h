@@ -194,18 +194,14 @@ int fg_ctx_set_solver_tiling(fg_ctx* ctx, int32_t target_blocks, int32_t min_chu
 #define FG_PRECOND_IPP 1
 int fg_ctx_set_preconditioner(fg_ctx* ctx, int32_t kind);
 int fg_ctx_preconditioner(fg_ctx* ctx, int32_t* kind);
-/* Replay each PCG iteration of later solves (the direction update, the
- * residual update and the V-cycle: about 20 kernel launches) from a HIP
- * graph instead of launching its kernels one by one.  A lane captures one
- * graph per iteration index and launch shape on first use and keeps it
- * until the context is destroyed; the kernels and their arguments are the
- * same, so results are bit-identical either way.  Only launches over every
- * scene of a group are replayed (a single scene, a batch's first sweep up
- * to its first poll); launches over the packed subset of a batch's scenes
- * that still iterate change shape at every poll and stay direct, as does
- * per-launch kernel profiling (fg_ctx_profile_kernel).  Default off.  No
- * counterpart in the reference (its PCG loop runs on the host,
- * linearsolver.cpp:25-73). */
+/* Graph replay of PCG iterations: switched off.  enabled = 0 returns FG_OK;
+ * any other value returns FG_E_INVALID.  Replaying each PCG iteration's
+ * launches (the direction update, the residual update and the V-cycle,
+ * about 20 kernels) from a captured HIP graph gave bit-identical results
+ * but frames 2-3 % slower than direct launches on the batched and 1024^2
+ * dam-break workloads, and a run under the profiler ended in a host-side
+ * fault inside it whose cause was not found.  No counterpart in the
+ * reference (its PCG loop runs on the host, linearsolver.cpp:25-73). */
 int fg_ctx_set_solver_graphs(fg_ctx* ctx, int32_t enabled);
 /* Autotuner for that schedule: steps `rounds` frames of the batch with each
  * lane count 1..FG_MAX_LANES (order alternating per round; the simulation

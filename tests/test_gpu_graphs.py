@@ -1,5 +1,8 @@
 """PCG iterations replayed from HIP graphs against direct launches.
 
+Graph replay is switched off (fg_ctx_set_solver_graphs refuses it): the
+comparisons below are kept, skipped, for the day it is switched on again.
+
 With fg_ctx_set_solver_graphs a lane captures each PCG iteration's launches
 (direction update, residual update, V-cycle) once per iteration index and
 launch shape and replays the graph in later solves.  The kernels and their
@@ -16,6 +19,9 @@ import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
+
+SWITCHED_OFF = pytest.mark.skip(reason="graph replay of PCG iterations is switched off: a 1024^2 run of it under "
+                                       "rocprofv3 ended in a host-side fault whose cause was not found")
 
 FIELDS = ("px", "py", "vx", "vy", "pid", "u", "v", "material", "fluid_sdf", "pressure", "rhs")
 
@@ -38,6 +44,7 @@ def graphs_off(ctx):
     ctx.set_batch_lanes(1)
 
 
+@SWITCHED_OFF
 def test_single_scene_graph_equals_direct(fg, graphs_off):
     ctx = graphs_off
     sp = fg.mixed_scene_specs(1, master_seed=5, resolutions=(160,))[0]
@@ -57,6 +64,7 @@ def test_single_scene_graph_equals_direct(fg, graphs_off):
         graphed.close()
 
 
+@SWITCHED_OFF
 @pytest.mark.parametrize("lanes", [1, 2])
 def test_batch_graph_equals_direct(fg, graphs_off, lanes):
     ctx = graphs_off
@@ -79,6 +87,7 @@ def test_batch_graph_equals_direct(fg, graphs_off, lanes):
             s.close()
 
 
+@SWITCHED_OFF
 def test_poisson_solve_graph_equals_direct(fg, graphs_off):
     ctx = graphs_off
     mat, _, _, b, scale = fg.isolated_projection_system(256, 3)
@@ -90,3 +99,10 @@ def test_poisson_solve_graph_equals_direct(fg, graphs_off):
         _log(f"  solve {rep}: {s1.iterations} iterations (direct {s0.iterations})")
         assert s1.iterations == s0.iterations
         assert np.array_equal(x0.view(np.uint8), x1.view(np.uint8)), rep
+
+
+def test_solver_graphs_refused(fg, ctx):
+    ctx.set_solver_graphs(False)
+    with pytest.raises(fg.FgError) as e:
+        ctx.set_solver_graphs(True)
+    assert e.value.code == fg.E_INVALID and "switched off" in str(e.value)
