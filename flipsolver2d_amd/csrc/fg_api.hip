@@ -1134,6 +1134,15 @@ struct KProf {
 // scenes of the group run the same sweep and iteration index in lock step;
 // a scene whose solve has stopped skips the launches that remain.
 // ---------------------------------------------------------------------------
+// Whether k_mg_up0 forms the neighbours' smoothing starts in an LDS window
+// over the chunk's rows rather than per cell: for a single scene whose
+// chunks span four rows or more (on one-row chunks and on batches of small
+// scenes the per-cell kernel is faster).  The same window for k_pcg_a's
+// direction (a 4096^2 projection) took 70 us a launch against 51 us per
+// cell: there the neighbours' rows it saves come back from the last-level
+// cache, and the window's prologue and barriers cost more.
+bool ring_window(const Group& G) { return G.n == 1 && G.sc[0]->pcg_chunk >= 4 * (int64_t)G.sc[0]->J; }
+
 // One V-cycle z = M r over the scenes of S whose solve uses it; which: the
 // residual buffer (0: cr, 1: cr2).  Level-l launches cover the scenes that
 // have level l; a kind with no blocks is not launched.
@@ -1168,10 +1177,7 @@ void launch_vcycle(const Group& G, hipStream_t st, const SolveLaunch& S, int whi
         launch_mg_post(S.grid(1 + l), st, S.d, S.map[1 + l], l);
     }
     KProf kp(G, FG_K_MG_UP0, true);
-    // the LDS window of k_mg_up0 pays for a single scene whose solver
-    // chunks span four rows or more
-    const bool ring = G.n == 1 && G.sc[0]->pcg_chunk >= 4 * (int64_t)G.sc[0]->J;
-    launch_mg_up0(S.grid(0), st, S.d, S.map[0], which, G.max_j, ring);
+    launch_mg_up0(S.grid(0), st, S.d, S.map[0], which, G.max_j, ring_window(G));
     kp.done(0);
 }
 

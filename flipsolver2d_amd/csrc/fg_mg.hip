@@ -879,20 +879,21 @@ __global__ void __launch_bounds__(kBlock) k_mg_post(const SceneDev* scenes, cons
 // partials of r.z and r.r the next PCG iteration reduces (solver chunks).
 // z0 of a fine cell depends on that cell alone, so a block forms it once
 // per cell into an LDS ring over the flat cell index (slot f mod RB, RB =
-// 2J + 2 kBlock floats of dynamic LDS) instead of once per neighbour: pass
-// p (cells p .. p + kBlock - 1, thread t taking cell p + t as a plain
-// kBlock stride would, so the partials keep their order) reads the ring
-// over [p - J, p + J + kBlock); the prologue fills that window of the
-// first pass, and each pass fills the kBlock cells the next one adds (their
-// loads issued before this pass's reads, their slots those of the cells
-// just behind this pass's window).  The window costs a prologue of 2J +
-// kBlock cells per block and a barrier per pass: it pays where chunks span
-// several rows of a grid too large for the L2 to keep the neighbouring
-// rows (4096^2: 115.8 -> 90.7 us a launch); on one-row chunks (1024^2) and
-// on batches of small scenes k_mg_up0_wide is faster (14.3 against 18.3 us,
-// 19.4 against 23.8 us), and the host launches that there.
-constexpr int kUpMaxJ = (16384 - 2 * kBlock) / 2;   // the ring within 64 KB
-constexpr int kUpFill = 4;                          // prologue cells a thread per round of loads
+// 2J + 2 kUpPass floats of dynamic LDS) instead of once per neighbour.
+// Pass p covers cells p .. p + kUpPass - 1, thread t taking p + t + u kBlock
+// (u = 0 .. kUpItems - 1: the cells and their order of a plain kBlock
+// stride, so the partials keep their order), and reads the ring over
+// [p - J, p + J + kUpPass); the prologue fills the first pass's window,
+// each pass the cells the next one adds (their loads issued first; their
+// slots are those of the cells just behind this pass's window).  The
+// window costs a prologue of 2J + kUpPass cells per block and a barrier per
+// pass: it pays where chunks span several rows of a grid too large for the
+// L2 to keep the neighbouring rows (4096^2); on one-row chunks (1024^2) and
+// on batches of small scenes k_mg_up0_wide is faster, and the host
+// launches that there.
+constexpr int kUpItems = 4;
+constexpr int kUpPass = kUpItems * kBlock;
+constexpr int kUpMaxJ = (16384 - 2 * kUpPass) / 2;   // the ring within 64 KB
 __global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, const int2* map, int which)
 {
     extern __shared__ float zr[];
@@ -907,19 +908,19 @@ __global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, const
     gptr<const uint8_t> code = gbl<const uint8_t>(s.stencil);
     gptr<float> z = gbl(s.cu);
     const Chunk ch = my_chunk(s, sb.block);
-    const int RB = 2 * J + 2 * kBlock;
-    // z0 of cell f from its code, r and the four level-1 values of P y1 at it
+    const int RB = 2 * J + 2 * kUpPass;
+    // z0 of cell f from its code, r and P y1 at it
     auto z0_of = [&](uint8_t kf, float rf, float pg) {
         const float dg = l1_diag0(kf);
         return (dg > 0.f ? kMgOmega * rf / dg : 0.f) + pg;
     };
     {   // prologue: the first pass's window
-        const int lo = max(ch.begin - J, 0), hi = min(ch.begin + J + kBlock, C);
-        for (int f0 = lo; f0 < hi; f0 += kUpFill * kBlock) {
-            uint8_t kq[kUpFill];
-            float rq[kUpFill], pq[kUpFill];
+        const int lo = max(ch.begin - J, 0), hi = min(ch.begin + J + kUpPass, C);
+        for (int f0 = lo; f0 < hi; f0 += kUpPass) {
+            uint8_t kq[kUpItems];
+            float rq[kUpItems], pq[kUpItems];
 #pragma unroll
-            for (int u = 0; u < kUpFill; u++) {
+            for (int u = 0; u < kUpItems; u++) {
                 const int f = min(f0 + (int)threadIdx.x + u * kBlock, hi - 1);   // clamped, not selected
                 const int fi = f / J, fj = f - fi * J;
                 kq[u] = code[f];
@@ -927,7 +928,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, const
                 pq[u] = prolong_at(y1, I1, J1, fi, fj);
             }
 #pragma unroll
-            for (int u = 0; u < kUpFill; u++) {
+            for (int u = 0; u < kUpItems; u++) {
                 const int f = f0 + threadIdx.x + u * kBlock;
                 if (f < hi) zr[f % RB] = z0_of(kq[u], rq[u], pq[u]);
             }
@@ -937,19 +938,29 @@ __global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, const
     const int di[5] = {0, -1, 1, 0, 0}, dj[5] = {0, 0, 0, -1, 1};
     double rz = 0.0, rr = 0.0;
     int sp = ch.begin % RB;   // the ring slot of cell p
-    for (int p = ch.begin; p < ch.end; p += kBlock) {
-        // the next pass's new cell of this thread: its loads first
-        const int f = p + J + kBlock + threadIdx.x;
-        const bool fill = (p + kBlock < ch.end) & (f < C);
-        const int fc = min(f, C - 1);
-        const int fi = fc / J, fj = fc - fi * J;
-        const uint8_t kf = code[fc];
-        const float rf = r[fc];
-        const float pf = prolong_at(y1, I1, J1, fi, fj);
-        const int c = p + threadIdx.x;
-        if (c < ch.end) {
-            const uint8_t k = code[c];
-            const float rc = r[c];
+    for (int p = ch.begin; p < ch.end; p += kUpPass) {
+        // the cells this thread adds for the next pass, and its own cells'
+        // code and r: loads first
+        const bool more = p + kUpPass < ch.end;
+        uint8_t kf[kUpItems], kc[kUpItems];
+        float rf[kUpItems], pf[kUpItems], rcv[kUpItems];
+#pragma unroll
+        for (int u = 0; u < kUpItems; u++) {
+            const int fc = min(p + J + kUpPass + (int)threadIdx.x + u * kBlock, C - 1);
+            const int fi = fc / J, fj = fc - fi * J;
+            kf[u] = code[fc];
+            rf[u] = r[fc];
+            pf[u] = prolong_at(y1, I1, J1, fi, fj);
+            const int c = min(p + (int)threadIdx.x + u * kBlock, ch.end - 1);
+            kc[u] = code[c];
+            rcv[u] = r[c];
+        }
+#pragma unroll
+        for (int u = 0; u < kUpItems; u++) {
+            const int c = p + threadIdx.x + u * kBlock;
+            if (c >= ch.end) break;
+            const uint8_t k = kc[u];
+            const float rc = rcv[u];
             float zc = rc;
             if (k & 0x80) {
                 const int ci = c / J, cj = c - ci * J;
@@ -960,13 +971,13 @@ __global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, const
                 for (int e = 1; e < 5; e++) use[e] = (k & (1 << (e - 1))) != 0;
                 use[3] = use[3] && cj > 0;
                 use[4] = use[4] && cj < J - 1;
-                int sc = sp + (int)threadIdx.x;
-                sc -= sc >= RB ? RB : 0;
+                int sc0 = sp + (int)threadIdx.x + u * kBlock;
+                sc0 -= sc0 >= RB ? RB : 0;
                 float z0[5];
 #pragma unroll
                 for (int e = 0; e < 5; e++) {
                     const int gi = min(max(ci + di[e], 0), I - 1), gj = min(max(cj + dj[e], 0), J - 1);
-                    int sg = sc + (gi - ci) * J + (gj - cj);
+                    int sg = sc0 + (gi - ci) * J + (gj - cj);
                     sg += sg < 0 ? RB : 0;
                     sg -= sg >= RB ? RB : 0;
                     z0[e] = use[e] ? zr[sg] : 0.f;
@@ -983,9 +994,13 @@ __global__ void __launch_bounds__(kBlock) k_mg_up0(const SceneDev* scenes, const
             z[c] = zc;
             rr += (double)rc * (double)rc;
         }
-        if (fill) zr[f % RB] = z0_of(kf, rf, pf);
+#pragma unroll
+        for (int u = 0; u < kUpItems; u++) {
+            const int f = p + J + kUpPass + threadIdx.x + u * kBlock;
+            if (more && f < C) zr[f % RB] = z0_of(kf[u], rf[u], pf[u]);
+        }
         __syncthreads();
-        sp += kBlock;
+        sp += kUpPass;
         sp -= sp >= RB ? RB : 0;
     }
     double v[2] = {rz, rr};
@@ -1081,7 +1096,7 @@ FG_MG_LAUNCH_MAP(mg_post, int, l)
 void launch_mg_up0(dim3 g, hipStream_t st, const SceneDev* sc, const int2* map, int which, int max_j, bool ring)
 {
     if (ring && max_j <= kUpMaxJ)
-        fg_launch_shm(k_mg_up0, g, dim3(kBlock), sizeof(float) * (2 * max_j + 2 * kBlock), st, sc, map, which);
+        fg_launch_shm(k_mg_up0, g, dim3(kBlock), sizeof(float) * (2 * max_j + 2 * kUpPass), st, sc, map, which);
     else
         fg_launch(k_mg_up0_wide, g, dim3(kBlock), st, sc, map, which);
 }
