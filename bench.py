@@ -72,18 +72,19 @@ KERNEL_BYTES = {
 }
 STEP_FAMILIES = ["p2g", "particles", "sdf", "pcg_a", "pcg_b", "mg_down0", "mg_up0", "mg_resid", "mg_coarse"]
 SOLVE_FAMILIES = ["pcg_a", "pcg_b", "mg_down0", "mg_up0", "mg_resid", "mg_coarse"]
-# HBM bytes moved per algorithmic byte, per workload and family: rocprofv3
-# FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) summed over every dispatch of the
+# Bytes moved past the L2 per algorithmic byte, per workload and family:
+# rocprofv3's read requests by size (TCC_EA0_RDREQ_32B/64B/128B, times 32,
+# 64 and 128 bytes) plus WRITE_SIZE, summed over every dispatch of the
 # family's kernels in a run of that workload alone, divided by the library's
-# algorithmic bytes over the same launches (profiles/hbm_traffic.md; the
-# factors are what the same counters read for a 1 GiB copy kernel).
+# algorithmic bytes over the same launches (profiles/hbm_traffic.md; on a
+# 1 GiB copy kernel the same counters read 1.000 GiB each way).
 MEASURED_TRAFFIC_PER_UNIT = {
-    "mixed256": {"p2g": 1.368, "particles": 0.695, "sdf": 2.13, "pcg_a": 1.089, "pcg_b": 1.014,
-                 "mg_down0": 0.345, "mg_up0": 0.951, "mg_resid": 1.085, "mg_coarse": 1.282},
-    "dam1024": {"p2g": 1.108, "particles": 2.042, "sdf": 0.638, "pcg_a": 1.96, "pcg_b": 1.005,
-                "mg_down0": 0.471, "mg_up0": 1.395, "mg_resid": 1.208, "mg_coarse": 1.865},
-    "proj4096": {"pcg_a": 1.863, "pcg_b": 1.0, "mg_down0": 0.973, "mg_up0": 2.41, "mg_resid": 1.877,
-                 "mg_coarse": 1.55},
+    "mixed256": {"p2g": 1.367, "particles": 0.656, "sdf": 2.114, "pcg_a": 1.088, "pcg_b": 1.012,
+                 "mg_down0": 1.064, "mg_up0": 0.946, "mg_resid": 1.062, "mg_coarse": 1.246},
+    "dam1024": {"p2g": 1.107, "particles": 2.041, "sdf": 0.63, "pcg_a": 1.022, "pcg_b": 1.003,
+                "mg_down0": 1.029, "mg_up0": 0.94, "mg_resid": 1.173, "mg_coarse": 1.528},
+    "proj4096": {"pcg_a": 1.863, "pcg_b": 1.0, "mg_down0": 1.006, "mg_up0": 1.982, "mg_resid": 1.875,
+                 "mg_coarse": 1.396},
 }
 
 
