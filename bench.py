@@ -235,6 +235,17 @@ def batched_leg(fg, ctx, a, copy_gbs):
     else:   # no tuning: the lane count given (default 1)
         lanes, rates = max(a.lanes, 1), []
         ctx.set_batch_lanes(lanes)
+    # the solver tiling, tuned on a copy of the batch: a tiling sets the FP64
+    # sums' order, and the timed batch and plain sets must see one order
+    tiling, tiling_rates = None, []
+    if a.tune_rounds > 0:
+        copy = [fg.make_scene(ctx, sp, tol=a.tol, pcg_iter_limit=4000) for sp in specs]
+        fg.batch_step_frame(ctx, copy)
+        k, rates_t = fg.batch_autotune_tiling(ctx, copy, rounds=2 * a.tune_rounds)   # both orders
+        for sc in copy:
+            sc.close()
+        tiling, tiling_rates = list(fg.TILING_CANDIDATES[k]), [float(r) for r in rates_t]
+        log(f"autotuned solver tiling: {tiling} (rates {[round(r / 1e6, 1) for r in rates_t]} M)")
     if a.lanes > 0:   # a fixed lane count (profiling runs compare like with like)
         lanes = a.lanes
         ctx.set_batch_lanes(lanes)
@@ -293,6 +304,8 @@ def batched_leg(fg, ctx, a, copy_gbs):
         "algorithmic_bytes_total": work_totals(fg, ctx),
         "lanes": lanes,
         "autotune_rates": [float(r) for r in rates],
+        "solver_tiling": tiling,
+        "tiling_rates": tiling_rates,
     }
     if plain:
         assert ps_p == ps_b, "plain and batched sets diverged"
@@ -540,6 +553,8 @@ def main():
             "scenes": a.scenes, "master_seed": a.master_seed, "solver_tol": a.tol,
             "substeps": b["substeps"], "lanes": b["lanes"],
             "autotune_rates": b["autotune_rates"],
+            "solver_tiling": b["solver_tiling"], "tiling_candidates": [list(c) for c in fg.TILING_CANDIDATES],
+            "tiling_rates": b["tiling_rates"],
         },
         "p50_frame_ms": b["p50_frame_ms"],
         "p99_frame_ms": b["p99_frame_ms"],

@@ -34,7 +34,7 @@ EXPORTS = [
     "fg_ctx_batch_lanes", "fg_batch_autotune", "fg_ctx_set_solver_tiling", "fg_ctx_set_preconditioner",
     "fg_ctx_preconditioner", "fg_batch_substep", "fg_server_cancel", "fg_ctx_work_read", "fg_scene_init_objects", "fg_scene_set_emitters",
     "fg_scene_max_velocity", "fg_poisson_solve_matrix",
-    "fg_poisson_solve_noalloc", "fg_ctx_set_solver_graphs",
+    "fg_poisson_solve_noalloc", "fg_ctx_set_solver_graphs", "fg_batch_autotune_tiling",
 ]
 MAX_LANES = 4
 REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_REJECTED, REQ_RELEASED, REQ_FAILED, REQ_CANCELLED = range(7)
@@ -190,6 +190,7 @@ def lib():
             "fg_ctx_set_solver_graphs": (C.c_int, [vp, i32]),
             "fg_ctx_preconditioner": (C.c_int, [vp, C.POINTER(i32)]),
             "fg_batch_autotune": (C.c_int, [vp, vp, i32, i32, C.POINTER(i32), vp]),
+            "fg_batch_autotune_tiling": (C.c_int, [vp, vp, i32, i32, vp, i32, C.POINTER(i32), vp]),
             "fg_server_create": (C.c_int, [vp, C.c_uint64, i32, C.POINTER(vp)]),
             "fg_server_destroy": (C.c_int, [vp]),
             "fg_server_submit": (C.c_int, [vp, C.POINTER(Request), C.POINTER(i64)]),
@@ -505,6 +506,24 @@ def stencil_matrix(stencil):
     arrays of the pressure matrix (bits 4-6, bits 0-3, bit 7)."""
     st = np.asarray(stencil, np.uint8)
     return ((st >> 4) & 7).astype(np.uint8), (st & 15).astype(np.uint8), (st >> 7).astype(np.uint8)
+
+
+TILING_CANDIDATES = ((1024, 256), (1024, 512), (1024, 1024), (1024, 2048), (1024, 4096))
+
+
+def batch_autotune_tiling(ctx, scenes, rounds=1, candidates=TILING_CANDIDATES):
+    """Measure each (target_blocks, min_chunk) solver tiling on `rounds`
+    frames of the batch (the scenes advance, and their FP64 sums follow each
+    tiling's order: tune on a copy), set the fastest on the context;
+    returns (index, rates)."""
+    n = len(scenes)
+    arr = (C.c_void_p * n)(*[s.h.value for s in scenes])
+    cand = np.asarray(candidates, np.int32).reshape(-1)
+    best = C.c_int32()
+    rates = np.zeros(len(candidates), np.float64)
+    _check(lib().fg_batch_autotune_tiling(ctx.h, arr, n, rounds, cand.ctypes.data, len(candidates), C.byref(best),
+                                          rates.ctypes.data))
+    return best.value, rates
 
 
 def batch_autotune(ctx, scenes, rounds=1):

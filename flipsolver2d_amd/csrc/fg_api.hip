@@ -2485,6 +2485,52 @@ int fg_batch_autotune(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, int32_t r
     return FG_OK;
 }
 
+int fg_batch_autotune_tiling(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, int32_t rounds,
+                             const int32_t* candidates, int32_t n_cand, int32_t* best_out, double* rates_out)
+{
+    static const int32_t kDefault[2 * FG_TILING_CANDIDATES] = {1024, 256, 1024, 512, 1024, 1024, 1024, 2048,
+                                                               1024, 4096};
+    if (!candidates) {
+        candidates = kDefault;
+        n_cand = FG_TILING_CANDIDATES;
+    }
+    if (!ctx || !scenes || n < 1 || rounds < 1 || n_cand < 1) return fail(FG_E_INVALID, "bad argument");
+    for (int k = 0; k < n_cand; k++) {
+        const int tb = candidates[2 * k], mc = candidates[2 * k + 1];
+        if (tb < 1 || tb > kPcgTilingMaxBlocks || mc < kPcgTilingMinChunk || mc % kBlock)
+            return fail(FG_E_INVALID, "bad tiling candidate");
+    }
+    const int saved_tb = ctx->pcg_target_blocks, saved_mc = ctx->pcg_min_chunk;
+    std::vector<double> ms(n_cand, 0.0), work(n_cand, 0.0);
+    std::vector<fg_stats> st(n);
+    for (int r = 0; r < rounds; r++)
+        for (int q = 0; q < n_cand; q++) {
+            const int k = (r & 1) ? n_cand - 1 - q : q;   // alternate the order
+            ctx->pcg_target_blocks = candidates[2 * k];
+            ctx->pcg_min_chunk = candidates[2 * k + 1];
+            const auto t0 = std::chrono::steady_clock::now();
+            const int rc = fg_batch_step_frame(ctx, scenes, n, st.data());
+            if (rc != FG_OK) {
+                ctx->pcg_target_blocks = saved_tb;
+                ctx->pcg_min_chunk = saved_mc;
+                return rc;
+            }
+            ms[k] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            for (const fg_stats& s : st) work[k] += (double)s.particle_substeps;
+        }
+    int best = 0;
+    for (int k = 0; k < n_cand; k++) {
+        const double rate = ms[k] > 0 ? work[k] / (ms[k] * 1e-3) : 0.0;
+        if (rates_out) rates_out[k] = rate;
+        const double rb = ms[best] > 0 ? work[best] / (ms[best] * 1e-3) : 0.0;
+        if (rate > rb) best = k;
+    }
+    ctx->pcg_target_blocks = candidates[2 * best];
+    ctx->pcg_min_chunk = candidates[2 * best + 1];
+    if (best_out) *best_out = best;
+    return FG_OK;
+}
+
 int fg_poisson_solve(fg_ctx* ctx, int32_t size_i, int32_t size_j, const int8_t* material, double scale,
                      const double* b, double* x, double rel_tol, int32_t max_iters, fg_solve_stats* stats)
 {

@@ -211,6 +211,20 @@ int fg_ctx_set_solver_graphs(fg_ctx* ctx, int32_t enabled);
  * NULL) gets FG_MAX_LANES measured rates. */
 int fg_batch_autotune(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, int32_t rounds, int32_t* lanes_out,
                       double* rates_out);
+/* Autotuner for the solver tiling (fg_ctx_set_solver_tiling): steps `rounds`
+ * frames of the batch with each of n_cand (target_blocks, min_chunk) pairs
+ * (candidates: 2 * n_cand int32; NULL: the built-in set FG_TILING_CANDIDATES,
+ * min_chunk 256 .. 4096 at 1024 target blocks), order alternating per round,
+ * measures particle-substeps per second for each and sets the fastest on
+ * the context.  *best_out gets its index; rates_out (may be NULL) gets the
+ * n_cand rates.  The tiling sets the FP64 reductions' summation order, so
+ * these frames round differently from frames under one fixed tiling: tune
+ * on a copy of the batch where results are compared.  No counterpart in
+ * the reference (its thread pool splits loops by splitRange,
+ * threading/threadpool.cpp:36-70). */
+#define FG_TILING_CANDIDATES 5
+int fg_batch_autotune_tiling(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, int32_t rounds,
+                             const int32_t* candidates, int32_t n_cand, int32_t* best_out, double* rates_out);
 
 /* Per-launch HIP-event timing of one kernel family, recorded on the stream
  * the kernel runs on (the launching lane's stream).  kernel = fg_kernel_id,

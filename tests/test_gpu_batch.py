@@ -113,3 +113,32 @@ def test_autotune_keeps_results(fg, ctx):
         ctx.set_batch_lanes(1)
     for sc in solo + batch:
         sc.close()
+
+
+def test_tiling_autotune_sets_the_fastest_and_keeps_batch_equal_to_solo(fg, ctx):
+    """The tiling autotuner measures each candidate on live frames of a copy
+    and sets the fastest; with that tiling a batch still equals its scenes
+    stepped alone."""
+    specs = fg.mixed_scene_specs(6, master_seed=91, resolutions=(48, 64, 96))
+    copy = _scenes(fg, ctx, specs)
+    cand = ((1024, 256), (1024, 1024), (64, 2048))
+    try:
+        k, rates = fg.batch_autotune_tiling(ctx, copy, rounds=2, candidates=cand)
+        _log(f"tiling autotune: {cand[k]}, rates {[round(r / 1e6, 2) for r in rates]} M")
+        assert 0 <= k < len(cand) and np.all(rates > 0) and rates[k] == rates.max()
+        solo = _scenes(fg, ctx, specs)
+        batch = _scenes(fg, ctx, specs)
+        for f in range(2):
+            s_stats = [sc.step_frame() for sc in solo]
+            b_stats = fg.batch_step_frame(ctx, batch)
+            for j in range(len(specs)):
+                assert s_stats[j].pressure_iters == b_stats[j].pressure_iters, (f, j)
+                _same(solo[j].get_state(), batch[j].get_state(), (f, j))
+        for sc in solo + batch:
+            sc.close()
+    finally:
+        ctx.set_solver_tiling(1024, 1024)
+        for sc in copy:
+            sc.close()
+    with pytest.raises(fg.FgError):
+        fg.batch_autotune_tiling(ctx, [], rounds=1)
