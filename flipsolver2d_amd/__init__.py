@@ -35,6 +35,7 @@ EXPORTS = [
     "fg_ctx_preconditioner", "fg_batch_substep", "fg_server_cancel", "fg_ctx_work_read", "fg_scene_init_objects", "fg_scene_set_emitters",
     "fg_scene_max_velocity", "fg_poisson_solve_matrix",
     "fg_poisson_solve_noalloc", "fg_ctx_set_solver_graphs", "fg_batch_autotune_tiling",
+    "fg_ctx_set_vcycle_fused",
 ]
 MAX_LANES = 4
 REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_REJECTED, REQ_RELEASED, REQ_FAILED, REQ_CANCELLED = range(7)
@@ -188,6 +189,7 @@ def lib():
             "fg_ctx_set_solver_tiling": (C.c_int, [vp, i32, i32]),
             "fg_ctx_set_preconditioner": (C.c_int, [vp, i32]),
             "fg_ctx_set_solver_graphs": (C.c_int, [vp, i32]),
+            "fg_ctx_set_vcycle_fused": (C.c_int, [vp, i32]),
             "fg_ctx_preconditioner": (C.c_int, [vp, C.POINTER(i32)]),
             "fg_batch_autotune": (C.c_int, [vp, vp, i32, i32, C.POINTER(i32), vp]),
             "fg_batch_autotune_tiling": (C.c_int, [vp, vp, i32, i32, vp, i32, C.POINTER(i32), vp]),
@@ -257,6 +259,10 @@ class Context:
     def set_preconditioner(self, kind):
         """PRECOND_MG (multigrid V-cycle, default) or PRECOND_IPP for later pressure solves."""
         _check(lib().fg_ctx_set_preconditioner(self.h, kind))
+
+    def set_vcycle_fused(self, on=True):
+        """Fused V-cycle levels (default) or the per-level kernels (bit-identical solves)."""
+        _check(lib().fg_ctx_set_vcycle_fused(self.h, 1 if on else 0))
 
     def set_solver_graphs(self, on=True):
         """Graph replay of PCG iterations is switched off: on=True raises (FG_E_INVALID)."""

@@ -54,9 +54,13 @@ int fail(int code, const std::string& msg)
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Block maps of the solve's per-iteration launches, by kind: 0 = solver
-// chunks, 1 = level-1 tiles (k_mg_down0), 1 + l = level-l cells one per
-// thread (l >= 1).
-constexpr int kMapKinds = 1 + kMgMaxLevels;
+// chunks, 1 = level-1 tiles of kVcTile (k_mg_down0, and k_mg_pp of level 1);
+// with the fused V-cycle l = level-l tiles of kRRTile (2 <= l <= t: k_mg_rr
+// of level l-1) and kMgMaxLevels + l = level-l tiles of kVcTile (2 <= l < t:
+// k_mg_pp of level l), else 1 + l = level-l cells one per thread (1 <= l < t).
+constexpr int kMapKinds = 2 * kMgMaxLevels;
+// the kind of k_mg_pp's launch over level l (fused)
+inline int pp_kind(int l) { return l == 1 ? 1 : kMgMaxLevels + l; }
 constexpr int kMapMaxScenes = 4096;   // scenes one map build handles (LDS prefix)
 constexpr int kFlagWords = 4 + kMapKinds;
 
@@ -103,6 +107,7 @@ struct fg_ctx {
     int pcg_target_blocks = kPcgMaxBlocks;   // solver tiling: blocks per scene at most
     int pcg_min_chunk = kPcgMinChunk;        // and cells per block at least
     int precond = kPrecondMG;                // pressure PCG preconditioner
+    bool vcycle_fused = true;                // V-cycle levels as k_mg_rr / k_mg_pp (else per-level kernels)
     std::vector<Lane*> lanes;
 };
 
@@ -640,15 +645,37 @@ __global__ void k_pack_iterating(const SceneDev* in, int n, SceneDev* out, int32
 }
 
 // Blocks of kind `kind` (see kMapKinds) a packed scene launches.
-__device__ int map_blocks(const SceneDev& s, int kind)
+__device__ int map_blocks(const SceneDev& s, int kind, bool fused)
 {
     if (kind == 0) return s.pcg_blocks;
     if (s.precond != kPrecondMG) return 0;
     const MgLevels& M = *s.mg;
     if (kind == 1) return mg_d0_tiles(M.lv[1].I, M.lv[1].J);
+    if (fused) {
+        if (kind <= kMgMaxLevels) return kind <= M.t ? mg_tiles(M.lv[kind].I, M.lv[kind].J, kRRTile) : 0;
+        const int l = kind - kMgMaxLevels;
+        return l < M.t ? mg_tiles(M.lv[l].I, M.lv[l].J) : 0;
+    }
     const int l = kind - 1;
     if (l >= M.t) return 0;   // level l runs inside k_mg_coarse
     return (M.lv[l].I * M.lv[l].J + kBlock - 1) / kBlock;
+}
+
+// Blocks of map kind `kind` for one scene (the host's copy of map_blocks).
+int64_t map_blocks_host(const fg_scene* s, int kind, bool fused)
+{
+    if (kind == 0) return s->pcg_blocks;
+    if (s->h.precond != kPrecondMG) return 0;
+    const MgLevels& M = s->mgh;
+    if (kind == 1) return mg_d0_tiles(M.lv[1].I, M.lv[1].J);
+    if (fused) {
+        if (kind <= kMgMaxLevels) return kind <= M.t ? mg_tiles(M.lv[kind].I, M.lv[kind].J, kRRTile) : 0;
+        const int l = kind - kMgMaxLevels;
+        return l < M.t ? mg_tiles(M.lv[l].I, M.lv[l].J) : 0;
+    }
+    const int l = kind - 1;
+    if (l >= M.t) return 0;
+    return cdiv((int64_t)M.lv[l].I * M.lv[l].J, kBlock);
 }
 
 struct MapOffsets {
@@ -660,7 +687,7 @@ struct MapOffsets {
 // natural order (consecutive tiles of a scene on one XCD).  info[0] = the
 // packed count; info[4 + kind] gets the kind's total.
 __global__ void __launch_bounds__(1024) k_build_maps(const SceneDev* packed, int32_t* info, int2* maps, MapOffsets offs,
-                                                     int n_all)
+                                                     int n_all, bool fused)
 {
     __shared__ int pre[kMapMaxScenes + 1];
     __shared__ int wsum[1024 / 64];
@@ -672,7 +699,7 @@ __global__ void __launch_bounds__(1024) k_build_maps(const SceneDev* packed, int
     const int b0 = min(n, t * per), b1 = min(n, b0 + per);
     int run = 0;
     for (int i = b0; i < b1; i++) {
-        const int c = map_blocks(packed[i], kind);
+        const int c = map_blocks(packed[i], kind, fused);
         pre[i + 1] = c;
         run += c;
     }
@@ -731,6 +758,9 @@ struct Group {
     unsigned mg_grid[kMgMaxLevels]{};   // blocks per scene over level l, one cell per thread
     unsigned sym_grid[kMgMaxLevels]{};  // 16 x 16-row tiles per scene over level l (k_mg_sym)
     unsigned vc_grid[kMgMaxLevels]{};   // the same over the scenes whose V-cycle launches level l (l < t)
+    unsigned rr_grid[kMgMaxLevels]{};   // kRRTile tiles of level l (2 <= l <= t): k_mg_rr's blocks a scene
+    unsigned pp_grid[kMgMaxLevels]{};   // kVcTile tiles of level l (l < t): k_mg_pp's blocks a scene
+    bool fused = true;                  // the context's vcycle_fused when the group was made
     unsigned mg_grid_d0 = 0;            // blocks per scene of k_mg_down0 (level-1 tiles)
 };
 
@@ -832,6 +862,7 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
     G.st.resize(scenes.size(), nullptr);
     G.n = (int)scenes.size();
     if (G.n == 0) return FG_OK;
+    G.fused = ctx->vcycle_fused;
     // the pinned staging buffers may still feed an earlier copy
     HIPCHK(hipStreamSynchronize(L->stream));
     bool first = true;
@@ -847,6 +878,10 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
                 G.mg_grid[l] = std::max(G.mg_grid[l], b);
                 G.sym_grid[l] = std::max(G.sym_grid[l], (unsigned)(cdiv(s->mgh.lv[l].I, 16) * cdiv(s->mgh.lv[l].J, 16)));
                 if (l < s->mgh.t) G.vc_grid[l] = std::max(G.vc_grid[l], b);
+                if (l <= s->mgh.t)
+                    G.rr_grid[l] = std::max(G.rr_grid[l], (unsigned)mg_tiles(s->mgh.lv[l].I, s->mgh.lv[l].J, kRRTile));
+                if (l < s->mgh.t)
+                    G.pp_grid[l] = std::max(G.pp_grid[l], (unsigned)mg_tiles(s->mgh.lv[l].I, s->mgh.lv[l].J));
             }
             G.mg_grid_d0 = std::max(G.mg_grid_d0, (unsigned)mg_d0_tiles(s->mgh.lv[1].I, s->mgh.lv[1].J));
         }
@@ -867,15 +902,11 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
     // block maps for the solve's per-iteration launches (several scenes):
     // capacity = every scene's blocks, the bound for any packed subset
     if (G.n > 1 && G.n <= kMapMaxScenes) {
-        G.map_kinds = G.mg_top + 1;   // pcg, down0, and levels 1 .. t-1
+        // pcg, down0, and levels 1 .. t-1 (fused: through k_mg_pp of level t-1)
+        G.map_kinds = G.fused ? kMgMaxLevels + G.mg_top : G.mg_top + 1;
         int64_t cap[kMapKinds] = {};
-        for (fg_scene* s : scenes) {
-            cap[0] += s->pcg_blocks;
-            if (s->h.precond != kPrecondMG) continue;
-            const MgLevels& M = s->mgh;
-            cap[1] += mg_d0_tiles(M.lv[1].I, M.lv[1].J);
-            for (int l = 1; l < M.t; l++) cap[1 + l] += cdiv((int64_t)M.lv[l].I * M.lv[l].J, kBlock);
-        }
+        for (fg_scene* s : scenes)
+            for (int k = 0; k < G.map_kinds; k++) cap[k] += map_blocks_host(s, k, G.fused);
         int64_t total = 0;
         for (int k = 0; k < G.map_kinds; k++) {
             G.map_off[k] = total;
@@ -968,7 +999,13 @@ SolveLaunch group_launch(const Group& G, const SceneDev* d, unsigned ny)
     S.ny = ny;
     S.blocks[0] = G.g.pcg.x;
     S.blocks[1] = G.mg_grid_d0;
-    for (int l = 1; l < kMgMaxLevels; l++) S.blocks[1 + l] = G.vc_grid[l];
+    if (G.fused)
+        for (int l = 2; l < kMgMaxLevels; l++) {
+            S.blocks[l] = G.rr_grid[l];
+            S.blocks[pp_kind(l)] = G.pp_grid[l];
+        }
+    else
+        for (int l = 1; l < kMgMaxLevels; l++) S.blocks[1 + l] = G.vc_grid[l];
     return S;
 }
 
@@ -984,7 +1021,8 @@ int pack_iterating(Group& G, SolveLaunch* S, int* count, int* open)
     if (kinds > 0) {
         MapOffsets offs{};
         for (int k = 0; k < kinds; k++) offs.off[k] = G.map_off[k];
-        hipLaunchKernelGGL(k_build_maps, dim3(kinds), dim3(1024), 0, L->stream, L->bpack, L->bflags, L->bmap, offs, -1);
+        hipLaunchKernelGGL(k_build_maps, dim3(kinds), dim3(1024), 0, L->stream, L->bpack, L->bflags, L->bmap, offs, -1,
+                           G.fused);
     }
     HIPCHK(hipMemcpyAsync(L->bflags_host, L->bflags, sizeof(int32_t) * (4 + kinds), hipMemcpyDeviceToHost, L->stream));
     FGCHK(queue_group_scalars(G));
@@ -1008,17 +1046,6 @@ int pack_iterating(Group& G, SolveLaunch* S, int* count, int* open)
     return FG_OK;
 }
 
-// Blocks of map kind `kind` for one scene (the host's copy of map_blocks).
-int64_t map_blocks_host(const fg_scene* s, int kind)
-{
-    if (kind == 0) return s->pcg_blocks;
-    if (s->h.precond != kPrecondMG) return 0;
-    const MgLevels& M = s->mgh;
-    if (kind == 1) return mg_d0_tiles(M.lv[1].I, M.lv[1].J);
-    const int l = kind - 1;
-    if (l >= M.t) return 0;
-    return cdiv((int64_t)M.lv[l].I * M.lv[l].J, kBlock);
-}
 
 // Flat launches over every scene of the group (a solve's first sweep, where
 // all of them iterate): the maps are built on the device from the group's
@@ -1029,14 +1056,15 @@ void all_scene_launch(Group& G, SolveLaunch* S)
     const int kinds = G.map_kinds;
     MapOffsets offs{};
     for (int k = 0; k < kinds; k++) offs.off[k] = G.map_off[k];
-    hipLaunchKernelGGL(k_build_maps, dim3(kinds), dim3(1024), 0, L->stream, G.d, L->bflags, L->bmap, offs, G.n);
+    hipLaunchKernelGGL(k_build_maps, dim3(kinds), dim3(1024), 0, L->stream, G.d, L->bflags, L->bmap, offs, G.n,
+                       G.fused);
     *S = SolveLaunch{};
     S->d = G.d;
     S->ny = (unsigned)G.n;
     S->flat = true;
     for (int k = 0; k < kinds; k++) {
         int64_t total = 0;
-        for (const fg_scene* s : G.sc) total += map_blocks_host(s, k);
+        for (const fg_scene* s : G.sc) total += map_blocks_host(s, k, G.fused);
         S->map[k] = L->bmap + G.map_off[k];
         S->blocks[k] = (unsigned)total;
     }
@@ -1149,32 +1177,49 @@ bool ring_window(const Group& G) { return G.n == 1 && G.sc[0]->pcg_chunk >= 4 * 
 void launch_vcycle(const Group& G, hipStream_t st, const SolveLaunch& S, int which)
 {
     // levels 1 .. t-1 one launch per step, levels t .. n-1 in k_mg_coarse
-    // (t per scene; the maps and vc_grid hold a level's blocks only for the
-    // scenes with l < t)
+    // (t per scene; the maps and grids hold a level's blocks only for the
+    // scenes that launch it)
     const int top = G.mg_top;
     if (S.blocks[1]) {
         KProf kp(G, FG_K_MG_DOWN0, true);
         launch_mg_down0(S.grid(1), st, S.d, S.map[1], which);
         kp.done(0);
     }
-    for (int l = 1; l < top; l++) {
-        if (S.blocks[1 + l]) {
-            // the profiling family times the largest residual launch (level 1)
+    if (G.fused) {
+        // level l's residual restricted to level l+1 (written out for the
+        // coarse block at l+1 = t): kRRTile tiles of level l+1
+        for (int l = 1; l < top; l++) {
+            if (!S.blocks[l + 1]) continue;
             KProf kp(G, l == 1 ? FG_K_MG_RESID : -2, true);
-            launch_mg_resid(S.grid(1 + l), st, S.d, S.map[1 + l], l);
+            launch_mg_rr(S.grid(l + 1), st, S.d, S.map[l + 1], l);
             kp.done(0);
         }
-        if (l + 1 < top && S.blocks[2 + l]) launch_mg_restrict(S.grid(2 + l), st, S.d, S.map[2 + l], l + 1);
+    } else {
+        for (int l = 1; l < top; l++) {
+            if (S.blocks[1 + l]) {
+                // the profiling family times the largest residual launch (level 1)
+                KProf kp(G, l == 1 ? FG_K_MG_RESID : -2, true);
+                launch_mg_resid(S.grid(1 + l), st, S.d, S.map[1 + l], l);
+                kp.done(0);
+            }
+            if (l + 1 < top && S.blocks[2 + l]) launch_mg_restrict(S.grid(2 + l), st, S.d, S.map[2 + l], l + 1);
+        }
     }
     {
         KProf kp(G, FG_K_MG_COARSE, true);
         launch_mg_coarse(dim3(1, S.ny), st, S.d);
         kp.done(0);
     }
-    for (int l = top - 1; l >= 1; l--) {
-        if (!S.blocks[1 + l]) continue;
-        launch_mg_prolong(S.grid(1 + l), st, S.d, S.map[1 + l], l);
-        launch_mg_post(S.grid(1 + l), st, S.d, S.map[1 + l], l);
+    if (G.fused) {
+        // prolongation and post-smoothing of level l: tiles of level l
+        for (int l = top - 1; l >= 1; l--)
+            if (S.blocks[pp_kind(l)]) launch_mg_pp(S.grid(pp_kind(l)), st, S.d, S.map[pp_kind(l)], l);
+    } else {
+        for (int l = top - 1; l >= 1; l--) {
+            if (!S.blocks[1 + l]) continue;
+            launch_mg_prolong(S.grid(1 + l), st, S.d, S.map[1 + l], l);
+            launch_mg_post(S.grid(1 + l), st, S.d, S.map[1 + l], l);
+        }
     }
     KProf kp(G, FG_K_MG_UP0, true);
     launch_mg_up0(S.grid(0), st, S.d, S.map[0], which, G.max_j, ring_window(G));
@@ -1327,7 +1372,10 @@ int run_solve(Group& G, bool rhs_from_grid)
         const int64_t C1 = (int64_t)L1.I * L1.J;
         // level-1 residual launch (scenes with a level below the coarse
         // block's): r, dinv, y per cell + stored rows' coefficients
-        if (s->mgh.t >= 2) add_work(G.L, timed, FG_K_MG_RESID, cycles * (12 * C1 + 4 * kMgStencil * (int64_t)h.mg_stored[1]));
+        // (fused: r and dinv per cell, r_2 written: 1 B a level-1 cell)
+        if (s->mgh.t >= 2)
+            add_work(G.L, timed, FG_K_MG_RESID,
+                     cycles * ((G.fused ? 9 : 12) * C1 + 4 * kMgStencil * (int64_t)h.mg_stored[1]));
         add_work(G.L, timed, FG_K_MG_COARSE, cycles * mg_coarse_bytes(s));
         add_work(G.L, timed, FG_K_MG_DOWN0, cycles * (5 * s->C + 5 * C1));
         add_work(G.L, timed, FG_K_MG_UP0, cycles * (9 * s->C + 4 * C1));
@@ -2422,6 +2470,13 @@ int fg_ctx_set_preconditioner(fg_ctx* ctx, int32_t kind)
     if (!ctx) return fail(FG_E_INVALID, "ctx is NULL");
     if (kind != FG_PRECOND_MG && kind != FG_PRECOND_IPP) return fail(FG_E_INVALID, "bad preconditioner");
     ctx->precond = kind == FG_PRECOND_MG ? kPrecondMG : kPrecondIPP;
+    return FG_OK;
+}
+
+int fg_ctx_set_vcycle_fused(fg_ctx* ctx, int32_t enabled)
+{
+    if (!ctx) return fail(FG_E_INVALID, "ctx is NULL");
+    ctx->vcycle_fused = enabled != 0;
     return FG_OK;
 }
 

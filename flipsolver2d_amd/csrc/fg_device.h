@@ -32,6 +32,13 @@ __host__ __device__ inline int mg_d0_tiles(int I1, int J1)
 {
     return ((I1 + kD0Tile - 1) / kD0Tile) * ((J1 + kD0Tile - 1) / kD0Tile);
 }
+constexpr int kVcTile = 16;              // k_mg_pp: level-l cells a block, per side
+constexpr int kRRTile = 8;               // k_mg_rr: level-(l+1) cells a block, per side
+// blocks of a fused V-cycle launch over a level of I x J cells in tiles of t x t
+__host__ __device__ inline int mg_tiles(int I, int J, int t = kVcTile)
+{
+    return ((I + t - 1) / t) * ((J + t - 1) / t);
+}
 constexpr int kMgBottomSweeps = 8;       // L1-Jacobi sweeps on the bottom level
 constexpr float kMgOmega = 1.5f;         // L1-Jacobi weight: x += w (r - A x) / sum_k |a_k| (SPD for w < 2)
 constexpr int kMgStencil = 25;           // 5x5 Galerkin stencil of the coarse levels

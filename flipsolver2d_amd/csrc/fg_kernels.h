@@ -61,6 +61,8 @@ FG_DECLARE_LAUNCH(mg_restrict, const int2* map, int l);
 FG_DECLARE_LAUNCH(mg_coarse);
 FG_DECLARE_LAUNCH(mg_prolong, const int2* map, int l);
 FG_DECLARE_LAUNCH(mg_post, const int2* map, int l);
+FG_DECLARE_LAUNCH(mg_rr, const int2* map, int l);
+FG_DECLARE_LAUNCH(mg_pp, const int2* map, int l);
 // max_j: the largest J of the launch's scenes (sizes the LDS of k_mg_up0);
 // ring: k_mg_up0 (its LDS window) rather than k_mg_up0_wide
 FG_DECLARE_LAUNCH(mg_up0, const int2* map, int which, int max_j, bool ring);

@@ -829,6 +829,156 @@ __global__ void __launch_bounds__(kMgCoarseThreads) k_mg_coarse(const SceneDev* 
     }
 }
 
+// The fused V-cycle levels (the default; the per-level kernels above and
+// below stay for comparison, fg_ctx_set_vcycle_fused).  A block owns a
+// kTile x kTile tile of cells and stages what its cells' stencils reach in
+// LDS, so a level's two steps take one launch instead of two.  Every value
+// is formed by the expression the per-level kernels use, at the same clamped
+// coordinates and with the same 0/1 masks, so z keeps its bits.
+static_assert(kVcTile * kVcTile == kBlock, "one cell of the tile per thread");
+constexpr int kRRY = 2 * kRRTile + 2;   // fine cells a side whose residual the restriction reads
+constexpr int kRRX = kRRY + 4;          // and the x their stencils read
+static_assert(4 * kRRTile * kRRTile <= kBlock, "one fine cell of the tile per thread");
+constexpr int kPPX = kVcTile + 4;       // level-l cells a side whose x the post-smoothing reads
+
+// Level l (1 <= l < t), down: x = r dinv and y = r - A_l x over the fine
+// patch of a kRRTile x kRRTile tile of level-(l+1) cells, then r_{l+1} = R y
+// for the tile's cells (l + 1 < t) or y written out for k_mg_coarse to
+// restrict (l + 1 = t).  The tile is smaller than k_mg_pp's: each fine
+// residual is a 25-term row with its own coefficient loads, and the small
+// tile keeps those to one or two rounds a thread.
+// Positions outside level l hold the values of their clamped cells (the
+// per-level kernels load those and mask them).
+__global__ void __launch_bounds__(kBlock) k_mg_rr(const SceneDev* scenes, const int2* map, int l)
+{
+    __shared__ float xs[kRRX][kRRX + 1];
+    __shared__ float ys[kRRY][kRRY + 1];
+    const SceneBlock sb = scene_block(map, true);
+    const SceneDev& s = scenes[sb.scene];
+    const auto& M = *cst(s.mg);
+    const auto& L = M.lv[l];
+    const auto& N = M.lv[l + 1];
+    const int LI = L.I, LJ = L.J, CL = LI * LJ, NI = N.I, NJ = N.J;
+    const int tjn = (NJ + kRRTile - 1) / kRRTile;
+    if (!mg_iterating(s) | (l + 1 > M.t) | (sb.block >= ((NI + kRRTile - 1) / kRRTile) * tjn)) return;
+    const int ti = sb.block / tjn, tj = sb.block - ti * tjn;
+    const int yi0 = 2 * kRRTile * ti - 1, yj0 = 2 * kRRTile * tj - 1;   // the residual patch's first fine cell
+    const int xi0 = yi0 - 2, xj0 = yj0 - 2;
+    gptr<const float> r = gbl<const float>(L.r), dinv = gbl<const float>(L.dinv);
+    {   // x at the clamped cell of every patch position
+        constexpr int Q = (kRRX * kRRX + kBlock - 1) / kBlock;
+        float rq[Q], dq[Q];
+#pragma unroll
+        for (int u = 0; u < Q; u++) {
+            const int q = min((int)threadIdx.x + u * kBlock, kRRX * kRRX - 1);
+            const int fi = min(max(xi0 + q / kRRX, 0), LI - 1), fj = min(max(xj0 + q % kRRX, 0), LJ - 1);
+            rq[u] = r[fi * LJ + fj];
+            dq[u] = dinv[fi * LJ + fj];
+        }
+#pragma unroll
+        for (int u = 0; u < Q; u++) {
+            const int q = threadIdx.x + u * kBlock;
+            if (q < kRRX * kRRX) xs[q / kRRX][q % kRRX] = rq[u] * fabsf(dq[u]);
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < kRRY * kRRY; q += kBlock) {   // y at the clamped cell of every position
+        const int ci = min(max(yi0 + q / kRRY, 0), LI - 1), cj = min(max(yj0 + q % kRRY, 0), LJ - 1);
+        const int c = ci * LJ + cj;
+        const float dc = dinv[c];
+        float y = 0.f;   // inactive row: r = 0 and no coefficients
+        if (dc != 0.f) {
+            float xv[kMgStencil];
+#pragma unroll
+            for (int k = 0; k < kMgStencil; k++) {
+                const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
+                const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
+                xv[k] = xs[ni - xi0][nj - xj0] * mask01(ok);
+            }
+            const float rc = r[c];
+            const float ax = row_dot(M, l, gbl<const float>(L.coef), CL, c, dc, xv);
+            y = rc - ax;
+        }
+        ys[q / kRRY][q % kRRY] = y;
+    }
+    __syncthreads();
+    if (l + 1 < M.t) {   // r_{l+1} = R y (restrict_at on the patch)
+        if ((int)threadIdx.x >= kRRTile * kRRTile) return;
+        const int li = threadIdx.x / kRRTile, lj = threadIdx.x % kRRTile;
+        const int ci = kRRTile * ti + li, cj = kRRTile * tj + lj;
+        if ((ci >= NI) | (cj >= NJ)) return;
+        const int c = ci * NJ + cj;
+        if (!gbl<const uint8_t>(N.act)[c]) {
+            gbl(N.r)[c] = 0.f;
+            return;
+        }
+        float acc = 0.f;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int fi = 2 * ci - 1 + a, fj = 2 * cj - 1 + b;
+                const bool ok = fi >= 0 && fi < LI && fj >= 0 && fj < LJ;
+                acc += ((w1(a) * w1(b)) * ys[2 * li + a][2 * lj + b]) * mask01(ok);
+            }
+        gbl(N.r)[c] = 0.25f * acc;
+    } else {   // level t's input: y of the tile's own fine cells
+        const int q = threadIdx.x;
+        if (q >= 4 * kRRTile * kRRTile) return;
+        const int a = q / (2 * kRRTile), b = q % (2 * kRRTile);
+        const int fi = 2 * kRRTile * ti + a, fj = 2 * kRRTile * tj + b;
+        if ((fi < LI) & (fj < LJ)) gbl(L.y)[fi * LJ + fj] = ys[a + 1][b + 1];
+    }
+}
+
+// Level l (1 <= l < t), up: x = r dinv + P y_{l+1} over a tile of level-l
+// cells and its two-cell halo, then y = x + dinv (r - A_l x) for the tile.
+__global__ void __launch_bounds__(kBlock) k_mg_pp(const SceneDev* scenes, const int2* map, int l)
+{
+    __shared__ float xs[kPPX][kPPX + 1];
+    const SceneBlock sb = scene_block(map, true);
+    const SceneDev& s = scenes[sb.scene];
+    const auto& M = *cst(s.mg);
+    const auto& L = M.lv[l];
+    const auto& Cl = M.lv[l + 1];
+    const int LI = L.I, LJ = L.J, CL = LI * LJ;
+    const int tjn = (LJ + kVcTile - 1) / kVcTile;
+    if (!mg_iterating(s) | (l >= M.t) | (sb.block >= ((LI + kVcTile - 1) / kVcTile) * tjn)) return;
+    const int ti = sb.block / tjn, tj = sb.block - ti * tjn;
+    const int xi0 = kVcTile * ti - 2, xj0 = kVcTile * tj - 2;
+    gptr<const float> r = gbl<const float>(L.r), dinv = gbl<const float>(L.dinv);
+    gptr<const uint8_t> act = gbl<const uint8_t>(L.act);
+    gptr<const float> yc = gbl<const float>(Cl.y);
+    for (int q = threadIdx.x; q < kPPX * kPPX; q += kBlock) {   // x at the clamped cell of every position
+        const int ci = min(max(xi0 + q / kPPX, 0), LI - 1), cj = min(max(xj0 + q % kPPX, 0), LJ - 1);
+        const int c = ci * LJ + cj;
+        const uint8_t a = act[c];
+        const float rc = r[c], dc = dinv[c];
+        const float pc = prolong_at(yc, Cl.I, Cl.J, ci, cj);
+        xs[q / kPPX][q % kPPX] = a ? rc * fabsf(dc) + pc : 0.f;
+    }
+    __syncthreads();
+    const int li = threadIdx.x / kVcTile, lj = threadIdx.x % kVcTile;
+    const int ci = kVcTile * ti + li, cj = kVcTile * tj + lj;
+    if ((ci >= LI) | (cj >= LJ)) return;
+    const int c = ci * LJ + cj;
+    const float dc = dinv[c];
+    if (dc == 0.f) {   // inactive row: x = 0
+        gbl(L.y)[c] = 0.f;
+        return;
+    }
+    float xv[kMgStencil];
+#pragma unroll
+    for (int k = 0; k < kMgStencil; k++) {
+        const int ni = ci + k / 5 - 2, nj = cj + k % 5 - 2;
+        const bool ok = ni >= 0 && ni < LI && nj >= 0 && nj < LJ;
+        xv[k] = xs[li + k / 5][lj + k % 5] * mask01(ok);
+    }
+    const float rc = r[c];
+    const float ax = row_dot(M, l, gbl<const float>(L.coef), CL, c, dc, xv);
+    gbl(L.y)[c] = xv[12] + fabsf(dc) * (rc - ax);
+}
+
 // Level l (1 <= l < t), up: x = r * dinv + P y_{l+1}.
 __global__ void __launch_bounds__(kBlock) k_mg_prolong(const SceneDev* scenes, const int2* map, int l)
 {
@@ -1093,6 +1243,8 @@ FG_MG_LAUNCH_MAP(mg_resid, int, l)
 FG_MG_LAUNCH_MAP(mg_restrict, int, l)
 FG_MG_LAUNCH_MAP(mg_prolong, int, l)
 FG_MG_LAUNCH_MAP(mg_post, int, l)
+FG_MG_LAUNCH_MAP(mg_rr, int, l)
+FG_MG_LAUNCH_MAP(mg_pp, int, l)
 void launch_mg_up0(dim3 g, hipStream_t st, const SceneDev* sc, const int2* map, int which, int max_j, bool ring)
 {
     if (ring && max_j <= kUpMaxJ)

@@ -194,6 +194,14 @@ int fg_ctx_set_solver_tiling(fg_ctx* ctx, int32_t target_blocks, int32_t min_chu
 #define FG_PRECOND_IPP 1
 int fg_ctx_set_preconditioner(fg_ctx* ctx, int32_t kind);
 int fg_ctx_preconditioner(fg_ctx* ctx, int32_t* kind);
+/* How later solves run the V-cycle's levels between level 1 and the coarse
+ * block.  Fused (default, enabled != 0): per level one launch for the
+ * residual and its restriction and one for the prolongation and the
+ * post-smoothing, each over LDS tiles; otherwise two per-cell launches for
+ * each.  The values and their arithmetic are the same, so solves are
+ * bit-identical either way; the per-level form is kept for that check.  No
+ * counterpart in the reference (its preconditioner is the IPP). */
+int fg_ctx_set_vcycle_fused(fg_ctx* ctx, int32_t enabled);
 /* Graph replay of PCG iterations: switched off.  enabled = 0 returns FG_OK;
  * any other value returns FG_E_INVALID.  Replaying each PCG iteration's
  * launches (the direction update, the residual update and the V-cycle,

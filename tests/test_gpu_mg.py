@@ -151,3 +151,42 @@ def test_substep_parity_with_ipp(fg, orc, ctx):
     assert gs.pressure_rel_resid <= 1e-10
     assert rel_l2(g["pressure"], st.pressure) <= 1e-5
     assert max(rel_l2(g["vx"], st.vx[:N]), rel_l2(g["vy"], st.vy[:N])) <= 1e-5
+
+
+@pytest.mark.parametrize("n,frames", [(128, 3), (300, 2), (384, 2)])
+def test_fused_vcycle_equals_per_level(fg, orc, ctx, n, frames):
+    """The fused V-cycle levels (residual + restriction, prolongation +
+    post-smoothing, each one launch over LDS tiles) form every value by the
+    per-level kernels' expression: the solves must be bit-identical."""
+    mat, b, scale = _dam_system(fg, orc, n, frames)
+    out = {}
+    try:
+        for fused in (True, False):
+            ctx.set_vcycle_fused(fused)
+            out[fused] = ctx.poisson_solve(mat, scale, b, tol=1e-6)
+    finally:
+        ctx.set_vcycle_fused(True)
+    (x1, s1), (x0, s0) = out[True], out[False]
+    _log(f"  {n}^2: fused {s1.iterations} iterations, per-level {s0.iterations}")
+    assert s1.iterations == s0.iterations and s1.sweeps == s0.sweeps
+    assert np.array_equal(x1.view(np.uint8), x0.view(np.uint8))
+
+
+def test_fused_vcycle_batch_equals_per_level(fg, ctx):
+    specs = fg.mixed_scene_specs(7, master_seed=41, resolutions=(64, 96, 128, 192))
+    fields = ("px", "py", "vx", "vy", "pid", "u", "v", "pressure")
+    got = {}
+    try:
+        for fused in (True, False):
+            ctx.set_vcycle_fused(fused)
+            scenes = [fg.make_scene(ctx, sp, tol=1e-6) for sp in specs]
+            for _ in range(2):
+                fg.batch_step_frame(ctx, scenes)
+            got[fused] = [sc.get_state() for sc in scenes]
+            for sc in scenes:
+                sc.close()
+    finally:
+        ctx.set_vcycle_fused(True)
+    for k in range(len(specs)):
+        for f in fields:
+            assert np.array_equal(got[True][k][f].view(np.uint8), got[False][k][f].view(np.uint8)), (k, f)
