@@ -1190,7 +1190,7 @@ void launch_vcycle(const Group& G, hipStream_t st, const SolveLaunch& S, int whi
         // coarse block at l+1 = t): kRRTile tiles of level l+1
         for (int l = 1; l < top; l++) {
             if (!S.blocks[l + 1]) continue;
-            KProf kp(G, l == 1 ? FG_K_MG_RESID : -2, true);
+            KProf kp(G, FG_K_MG_RESID, true);
             launch_mg_rr(S.grid(l + 1), st, S.d, S.map[l + 1], l);
             kp.done(0);
         }
@@ -1212,8 +1212,12 @@ void launch_vcycle(const Group& G, hipStream_t st, const SolveLaunch& S, int whi
     }
     if (G.fused) {
         // prolongation and post-smoothing of level l: tiles of level l
-        for (int l = top - 1; l >= 1; l--)
-            if (S.blocks[pp_kind(l)]) launch_mg_pp(S.grid(pp_kind(l)), st, S.d, S.map[pp_kind(l)], l);
+        for (int l = top - 1; l >= 1; l--) {
+            if (!S.blocks[pp_kind(l)]) continue;
+            KProf kp(G, FG_K_MG_RESID, true);
+            launch_mg_pp(S.grid(pp_kind(l)), st, S.d, S.map[pp_kind(l)], l);
+            kp.done(0);
+        }
     } else {
         for (int l = top - 1; l >= 1; l--) {
             if (!S.blocks[1 + l]) continue;
@@ -1370,12 +1374,21 @@ int run_solve(Group& G, bool rhs_from_grid)
         add_work(G.L, timed, FG_K_VCYCLE, cycles * s->C);
         const MgLevel& L1 = s->mgh.lv[1];
         const int64_t C1 = (int64_t)L1.I * L1.J;
-        // level-1 residual launch (scenes with a level below the coarse
-        // block's): r, dinv, y per cell + stored rows' coefficients
-        // (fused: r and dinv per cell, r_2 written: 1 B a level-1 cell)
-        if (s->mgh.t >= 2)
-            add_work(G.L, timed, FG_K_MG_RESID,
-                     cycles * ((G.fused ? 9 : 12) * C1 + 4 * kMgStencil * (int64_t)h.mg_stored[1]));
+        // the per-level launches (scenes with a level below the coarse
+        // block's).  Fused, every level l < t: k_mg_rr reads r and dinv and
+        // writes r_{l+1} (9 B a level-l cell), k_mg_pp reads act, r, dinv,
+        // y_{l+1} and writes y (14 B), and each reads a stored row's 100 B.
+        // Per-level kernels: the level-1 residual launch alone (r, dinv, y).
+        if (s->mgh.t >= 2) {
+            int64_t b = 0;
+            if (G.fused) {
+                for (int l = 1; l < s->mgh.t; l++)
+                    b += 23 * (int64_t)s->mgh.lv[l].I * s->mgh.lv[l].J + 8 * kMgStencil * (int64_t)h.mg_stored[l];
+            } else {
+                b = 12 * C1 + 4 * kMgStencil * (int64_t)h.mg_stored[1];
+            }
+            add_work(G.L, timed, FG_K_MG_RESID, cycles * b);
+        }
         add_work(G.L, timed, FG_K_MG_COARSE, cycles * mg_coarse_bytes(s));
         add_work(G.L, timed, FG_K_MG_DOWN0, cycles * (5 * s->C + 5 * C1));
         add_work(G.L, timed, FG_K_MG_UP0, cycles * (9 * s->C + 4 * C1));

@@ -270,7 +270,9 @@ enum fg_kernel_id {
     FG_K_RHS = 9,         /* [C] */
     FG_K_APPLY = 10,      /* [C] */
     FG_K_VCYCLE = 11,     /* [C] one multigrid V-cycle (all its launches) */
-    FG_K_MG_RESID = 12,   /* [B] the V-cycle's level-1 residual launch */
+    FG_K_MG_RESID = 12,   /* [B] the V-cycle's launches between level 1 and the coarse block:
+                           * k_mg_rr and k_mg_pp of every level (per-level kernels, see
+                           * fg_ctx_set_vcycle_fused: the level-1 residual launch) */
     FG_K_MG_COARSE = 13,  /* [B] the V-cycle's coarse-levels launch (one block per scene) */
     FG_K_MG_DOWN0 = 14,   /* [B] level-0 smoothing, residual and restriction */
     FG_K_MG_UP0 = 15,     /* [B] prolongation to level 0 and post-smoothing */
