@@ -62,27 +62,48 @@ __global__ void __launch_bounds__(kBlock) k_refine_residual(const SceneDev* scen
     double bb[1];
     sum_partials<1>(s, kSlotBB, nb, bb);
     if (blk == 0 && threadIdx.x == 0) s.sc->bb = bb[0];
-    const int J = s.J;
+    const int J = s.J, C = s.I * s.J;
     const Chunk ch = my_chunk(s, blk);
+    gptr<const double> x = gbl<const double>(s.x64), bv = gbl<const double>(s.b);
+    gptr<const uint8_t> codes = gbl<const uint8_t>(s.stencil);
+    const double sc = s.sA;
     double rr = 0.0;
-    for (int c = ch.begin + threadIdx.x; c < ch.end; c += kBlock) {
-        const uint8_t code = s.stencil[c];
-        const double* x = s.x64;
-        double ax;
-        if (code & 0x80) {
-            const double sc = s.sA;
-            const int n = (code >> 4) & 7;
-            ax = (sc * n) * x[c];
-            if (code & 1) ax += (-sc) * x[c - J];
-            if (code & 2) ax += (-sc) * x[c + J];
-            if (code & 4) ax += (-sc) * x[c - 1];
-            if (code & 8) ax += (-sc) * x[c + 1];
-        } else {
-            ax = x[c];
+    // kRefItems cells a thread a pass (a plain kBlock stride's cells in its
+    // order), every load issued before the stencil code is looked at
+    // (clamped indices; the code selects which neighbours enter)
+    constexpr int kRefItems = 4;
+    for (int base = ch.begin; base < ch.end; base += kRefItems * kBlock) {
+        uint8_t cd[kRefItems];
+        double xc[kRefItems][5], bc[kRefItems];
+#pragma unroll
+        for (int u = 0; u < kRefItems; u++) {
+            const int c = min(base + (int)threadIdx.x + u * kBlock, ch.end - 1);
+            const int offs[5] = {0, -J, J, -1, 1};
+            cd[u] = codes[c];
+            bc[u] = bv[c];
+#pragma unroll
+            for (int e = 0; e < 5; e++) xc[u][e] = x[min(max(c + offs[e], 0), C - 1)];
         }
-        const double r64 = s.b[c] - ax;
-        rr += r64 * r64;
-        s.cr[c] = (float)((code & 0x80) ? r64 / s.sA : r64);
+#pragma unroll
+        for (int u = 0; u < kRefItems; u++) {
+            const int c = base + threadIdx.x + u * kBlock;
+            if (c >= ch.end) break;
+            const uint8_t code = cd[u];
+            double ax;
+            if (code & 0x80) {
+                const int n = (code >> 4) & 7;
+                ax = (sc * n) * xc[u][0];
+                if (code & 1) ax += (-sc) * xc[u][1];
+                if (code & 2) ax += (-sc) * xc[u][2];
+                if (code & 4) ax += (-sc) * xc[u][3];
+                if (code & 8) ax += (-sc) * xc[u][4];
+            } else {
+                ax = xc[u][0];
+            }
+            const double r64 = bc[u] - ax;
+            rr += r64 * r64;
+            s.cr[c] = (float)((code & 0x80) ? r64 / sc : r64);
+        }
     }
     double v[1] = {rr};
     put_partials<1>(s, kSlotRR64, v, blk);
