@@ -52,10 +52,11 @@ HBM_PEAK_GBS = 8000.0
 #              in FP64)
 #   mg_down0   5 C + 5 C1 per V-cycle (code, r; level-1 activity, r1)
 #   mg_up0     9 C + 4 C1 per V-cycle (code, r, z; level-1 y)
-#   mg_resid   the V-cycle's launches between level 1 and the coarse block,
-#              k_mg_rr and k_mg_pp of every level l below it: 23 B per
-#              level-l cell (r, dinv, r_{l+1}; act, r, dinv, y_{l+1}, y) and
-#              twice the 100 B of each stored row, per V-cycle
+#   mg_resid   k_mg_rr of every level l between level 1 and the coarse
+#              block: 9 B per level-l cell (r, dinv; r_{l+1}) and the 100 B of
+#              each stored row, per V-cycle
+#   mg_pp      k_mg_pp of the same levels: 14 B per level-l cell (act, r,
+#              dinv, y_{l+1}; y) and the 100 B of each stored row
 #   mg_coarse  per scene and V-cycle: 4 B per cell of the levels' dinv, of its
 #              input and of its output, once each the 100 B of every row above
 #              the bottom that is not its level's canonical interior stencil,
@@ -68,11 +69,13 @@ KERNEL_BYTES = {
     "pcg_b": ("K_PCG_B", 32.0, "cells", ["k_pcg_b"]),
     "mg_down0": ("K_MG_DOWN0", 1.0, "bytes", ["k_mg_down0"]),
     "mg_up0": ("K_MG_UP0", 1.0, "bytes", ["k_mg_up0", "k_mg_up0_wide"]),
-    "mg_resid": ("K_MG_RESID", 1.0, "bytes", ["k_mg_rr", "k_mg_pp"]),
+    "mg_resid": ("K_MG_RESID", 1.0, "bytes", ["k_mg_rr"]),
+    "mg_pp": ("K_MG_PP", 1.0, "bytes", ["k_mg_pp"]),
     "mg_coarse": ("K_MG_COARSE", 1.0, "bytes", ["k_mg_coarse"]),
 }
-STEP_FAMILIES = ["p2g", "particles", "sdf", "pcg_a", "pcg_b", "mg_down0", "mg_up0", "mg_resid", "mg_coarse"]
-SOLVE_FAMILIES = ["pcg_a", "pcg_b", "mg_down0", "mg_up0", "mg_resid", "mg_coarse"]
+STEP_FAMILIES = ["p2g", "particles", "sdf", "pcg_a", "pcg_b", "mg_down0", "mg_up0", "mg_resid", "mg_pp",
+                 "mg_coarse"]
+SOLVE_FAMILIES = ["pcg_a", "pcg_b", "mg_down0", "mg_up0", "mg_resid", "mg_pp", "mg_coarse"]
 # Bytes moved past the L2 per algorithmic byte, per workload and family:
 # rocprofv3's read requests by size (TCC_EA0_RDREQ_32B/64B/128B, times 32,
 # 64 and 128 bytes) plus WRITE_SIZE, summed over every dispatch of the
@@ -80,12 +83,12 @@ SOLVE_FAMILIES = ["pcg_a", "pcg_b", "mg_down0", "mg_up0", "mg_resid", "mg_coarse
 # algorithmic bytes over the same launches (profiles/hbm_traffic.md; on a
 # 1 GiB copy kernel the same counters read 1.000 GiB each way).
 MEASURED_TRAFFIC_PER_UNIT = {
-    "mixed256": {"p2g": 1.367, "particles": 0.656, "sdf": 2.114, "pcg_a": 1.088, "pcg_b": 1.012,
-                 "mg_down0": 1.063, "mg_up0": 0.948, "mg_resid": 1.209, "mg_coarse": 1.243},
+    "mixed256": {"p2g": 1.367, "particles": 0.655, "sdf": 2.114, "pcg_a": 1.089, "pcg_b": 1.012,
+                 "mg_down0": 1.063, "mg_up0": 0.948, "mg_resid": 1.306, "mg_pp": 1.143, "mg_coarse": 1.243},
     "dam1024": {"p2g": 1.107, "particles": 2.041, "sdf": 0.63, "pcg_a": 1.022, "pcg_b": 1.003,
-                "mg_down0": 1.029, "mg_up0": 0.939, "mg_resid": 1.152, "mg_coarse": 1.513},
-    "proj4096": {"pcg_a": 1.863, "pcg_b": 1.0, "mg_down0": 1.006, "mg_up0": 1.982, "mg_resid": 1.239,
-                 "mg_coarse": 1.386},
+                "mg_down0": 1.029, "mg_up0": 0.939, "mg_resid": 1.21, "mg_pp": 1.113, "mg_coarse": 1.513},
+    "proj4096": {"pcg_a": 1.863, "pcg_b": 1.0, "mg_down0": 1.006, "mg_up0": 1.983, "mg_resid": 1.259,
+                 "mg_pp": 1.222, "mg_coarse": 1.386},
 }
 
 

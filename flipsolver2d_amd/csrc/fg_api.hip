@@ -1214,7 +1214,7 @@ void launch_vcycle(const Group& G, hipStream_t st, const SolveLaunch& S, int whi
         // prolongation and post-smoothing of level l: tiles of level l
         for (int l = top - 1; l >= 1; l--) {
             if (!S.blocks[pp_kind(l)]) continue;
-            KProf kp(G, FG_K_MG_RESID, true);
+            KProf kp(G, FG_K_MG_PP, true);
             launch_mg_pp(S.grid(pp_kind(l)), st, S.d, S.map[pp_kind(l)], l);
             kp.done(0);
         }
@@ -1380,14 +1380,19 @@ int run_solve(Group& G, bool rhs_from_grid)
         // y_{l+1} and writes y (14 B), and each reads a stored row's 100 B.
         // Per-level kernels: the level-1 residual launch alone (r, dinv, y).
         if (s->mgh.t >= 2) {
-            int64_t b = 0;
             if (G.fused) {
-                for (int l = 1; l < s->mgh.t; l++)
-                    b += 23 * (int64_t)s->mgh.lv[l].I * s->mgh.lv[l].J + 8 * kMgStencil * (int64_t)h.mg_stored[l];
+                int64_t brr = 0, bpp = 0;
+                for (int l = 1; l < s->mgh.t; l++) {
+                    const int64_t cl = (int64_t)s->mgh.lv[l].I * s->mgh.lv[l].J;
+                    const int64_t st = 4 * kMgStencil * (int64_t)h.mg_stored[l];
+                    brr += 9 * cl + st;
+                    bpp += 14 * cl + st;
+                }
+                add_work(G.L, timed, FG_K_MG_RESID, cycles * brr);
+                add_work(G.L, timed, FG_K_MG_PP, cycles * bpp);
             } else {
-                b = 12 * C1 + 4 * kMgStencil * (int64_t)h.mg_stored[1];
+                add_work(G.L, timed, FG_K_MG_RESID, cycles * (12 * C1 + 4 * kMgStencil * (int64_t)h.mg_stored[1]));
             }
-            add_work(G.L, timed, FG_K_MG_RESID, cycles * b);
         }
         add_work(G.L, timed, FG_K_MG_COARSE, cycles * mg_coarse_bytes(s));
         add_work(G.L, timed, FG_K_MG_DOWN0, cycles * (5 * s->C + 5 * C1));

@@ -251,8 +251,13 @@ int fg_batch_autotune_tiling(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, in
  *                                       saved U, saved V; solid SDF)
  *   MG_DOWN0     5 C + 5 C1 per V-cycle (stencil code, r; level-1 activity, r1)
  *   MG_UP0       9 C + 4 C1 per V-cycle (code, r, z; level-1 y)
- *   MG_RESID     12 C1 + 100 S1 per V-cycle (level-1 r, dinv, y; S1 the
- *                level-1 rows that read stored coefficients)
+ *   MG_RESID     per V-cycle, over every level l between level 1 and the
+ *                coarse block: 9 C_l + 100 S_l (r, dinv; r_{l+1} written,
+ *                1 B a level-l cell; S_l the rows that read stored
+ *                coefficients).  Per-level kernels: 12 C1 + 100 S1 (the
+ *                level-1 residual launch: r, dinv, y)
+ *   MG_PP        per V-cycle, over the same levels: 14 C_l + 100 S_l (act,
+ *                r, dinv, y_{l+1}: 1 B a level-l cell; y written)
  *   MG_COARSE    per scene and V-cycle: 4 B per cell of the block's levels
  *                (dinv), its input (r_1 or the finer level's y), 4 B per
  *                top-level cell (y written), 100 B per row above the bottom
@@ -270,14 +275,16 @@ enum fg_kernel_id {
     FG_K_RHS = 9,         /* [C] */
     FG_K_APPLY = 10,      /* [C] */
     FG_K_VCYCLE = 11,     /* [C] one multigrid V-cycle (all its launches) */
-    FG_K_MG_RESID = 12,   /* [B] the V-cycle's launches between level 1 and the coarse block:
-                           * k_mg_rr and k_mg_pp of every level (per-level kernels, see
-                           * fg_ctx_set_vcycle_fused: the level-1 residual launch) */
+    FG_K_MG_RESID = 12,   /* [B] the V-cycle's residual + restriction launches, k_mg_rr of
+                           * every level between level 1 and the coarse block (per-level
+                           * kernels, see fg_ctx_set_vcycle_fused: the level-1 residual) */
     FG_K_MG_COARSE = 13,  /* [B] the V-cycle's coarse-levels launch (one block per scene) */
     FG_K_MG_DOWN0 = 14,   /* [B] level-0 smoothing, residual and restriction */
     FG_K_MG_UP0 = 15,     /* [B] prolongation to level 0 and post-smoothing */
     FG_K_PARTICLES = 16,  /* [B] k_advect and k_g2p_count together (G2P + advection) */
-    FG_K_NUM = 17
+    FG_K_MG_PP = 17,      /* [B] the V-cycle's prolongation + post-smoothing launches, k_mg_pp
+                           * of the same levels (not timed with the per-level kernels) */
+    FG_K_NUM = 18
 };
 int fg_ctx_profile_kernel(fg_ctx* ctx, int32_t kernel);
 /* Work units (as above) of every kernel family, summed over all launches
