@@ -69,8 +69,8 @@ def test_product_does_not_link_the_oracle():
 
 
 def test_cpp_api_builds_and_links():
-    """The reference-named C++ API library and its driver are built and
-    resolve their libraries (the driver's usage path makes no HIP call)."""
+    """The reference-named C++ API library and its demo program are built and
+    resolve their libraries (the demo's usage path makes no HIP call)."""
     import subprocess
 
     from flipsolver2d_amd import build as b
