@@ -73,7 +73,7 @@ class SceneState(C.Structure):
         ("solid_id", C.c_void_p), ("particle_counts", C.c_void_p),
         ("saved_u", C.c_void_p), ("saved_v", C.c_void_p), ("pressure", C.c_void_p), ("rhs", C.c_void_p),
         ("emitter_id", C.c_void_p), ("rng_epoch", C.c_uint32), ("next_pid", C.c_uint32),
-        ("stencil", C.c_void_p),
+        ("stencil", C.c_void_p), ("work_u", C.c_void_p), ("work_v", C.c_void_p),
     ]
 
 
@@ -415,6 +415,7 @@ class Scene:
             "saved_v": np.zeros(I * (J + 1), np.float32), "pressure": np.zeros(I * J, np.float64),
             "rhs": np.zeros(I * J, np.float64), "stencil": np.zeros(I * J, np.uint8),
             "emitter_id": np.zeros(I * J, np.int32),
+            "work_u": np.zeros((I + 1) * J, np.float32), "work_v": np.zeros(I * (J + 1), np.float32),
         }
         s = SceneState()
         for k, a in out.items():

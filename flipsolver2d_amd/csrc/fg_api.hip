@@ -2265,6 +2265,8 @@ int fg_scene_get_state(fg_scene* s, fg_scene_state* st)
     FGCHK(dl(st->rhs, h.b, sizeof(double) * s->C));
     FGCHK(dl(st->stencil, h.stencil, s->C));
     FGCHK(dl(st->emitter_id, h.eid, sizeof(int32_t) * s->C));
+    FGCHK(dl(st->work_u, h.u, sizeof(float) * s->FU));
+    FGCHK(dl(st->work_v, h.v, sizeof(float) * s->FV));
     st->rng_epoch = s->epoch;
     st->next_pid = s->next_pid;
     HIPCHK(hipStreamSynchronize(q));

@@ -233,3 +233,73 @@ def test_full_grid_keeps_exact_sdf(fg, orc, ctx):
     assert np.all(st.fluid_sdf < 0)   # no cell is outside
     assert np.array_equal(g["fluid_sdf"].view(np.uint32), st.fluid_sdf.view(np.uint32))
     assert np.array_equal(g["material"], st.material)
+
+
+def test_post_projection_divergence_vs_oracle(fg, orc, ctx):
+    """The projection's own error against the oracle: the GPU runs a
+    substep's stages up to the grid it projects (the state's work grid), the
+    oracle projects that same grid as step() does (rhs from the current
+    materials, the matrix from the materials at the substep's start,
+    flipsolver2d.cpp:417 and 446, FP64 PCG to 1e-12, pressure applied), and
+    the two post-projection grids and their divergence on fluid cells are
+    compared.  With the matrix one material update behind, the projected
+    grid is divergence-free only where the materials around a cell did not
+    change; the divergence left there is bounded relative to the grid's
+    divergence before the projection."""
+    n, ppc = 128, 4
+    p, st = _prepared(orc, n, ppc, 4)
+    mat_a = st.material.copy()   # the materials the substep's matrix is built from
+    vel = orc.lib().orc_max_particle_velocity(st.ref())
+    dt = np.float32(min(5.0 / (vel + 1e-15), 1.0 / 30))
+    sc = fg.Scene(ctx, fg.make_params(n, ppc=ppc, tol=1e-10))
+    sc.set_friction(np.zeros(4, np.float32))
+    sc.set_state(oracle_state_dict(st))
+    for stage in (fg.STAGE_ADVECT, fg.STAGE_REBIN, fg.STAGE_DENSITY, fg.STAGE_P2G, fg.STAGE_SDF,
+                  fg.STAGE_LEVELSET, fg.STAGE_EXTRAPOLATE, fg.STAGE_FORCES):
+        sc.stage(stage, float(dt))
+    pre = sc.get_state()
+    sc.stage(fg.STAGE_PROJECT, float(dt))
+    post = sc.get_state()
+    sc.close()
+    L = orc.lib()
+    o = orc.State(n, n, 1)
+    o.u[:] = pre["work_u"]
+    o.v[:] = pre["work_v"]
+    o.material[:] = pre["material"]
+    rhs = np.zeros(n * n)
+    L.orc_pressure_rhs(C.byref(p), o.ref(), rhs)
+    assert np.array_equal(rhs.view(np.uint64), post["rhs"].view(np.uint64))
+    x = np.zeros(n * n)
+    rr = C.c_double()
+    scale = float(dt) / (p.fluid_density * p.dx * p.dx)
+    L.orc_pcg_converged(mat_a, n, n, scale, rhs, x, 400000, 1e-12, C.byref(rr))
+    L.orc_apply_pressure(C.byref(p), o.ref(), C.c_float(dt), x)
+
+    def div(u, v):
+        u = np.asarray(u, np.float64).reshape(n + 1, n)
+        v = np.asarray(v, np.float64).reshape(n, n + 1)
+        return (u[1:, :] - u[:-1, :]) + (v[:, 1:] - v[:, :-1])
+
+    m_now = pre["material"].reshape(n, n)
+    fluid = (m_now & 0x40) != 0
+    changed = np.pad(m_now != mat_a.reshape(n, n), 1)
+    near = np.zeros((n, n), bool)   # a material in the cell's 3x3 neighbourhood changed
+    for di in range(3):
+        for dj in range(3):
+            near |= changed[di:di + n, dj:dj + n]
+    steady = fluid & ~near
+    d_pre = div(pre["work_u"], pre["work_v"])
+    d_gpu, d_orc = div(post["work_u"], post["work_v"]), div(o.u, o.v)
+    scale_div = np.abs(d_pre[fluid]).max()
+    e_div = np.abs(d_gpu - d_orc)[fluid].max() / scale_div
+    e_u, e_v = rel_l2(post["work_u"], o.u), rel_l2(post["work_v"], o.v)
+    e_p = rel_l2(post["pressure"], x)
+    left_gpu = np.abs(d_gpu[steady]).max() / scale_div
+    left_orc = np.abs(d_orc[steady]).max() / scale_div
+    _log(f"  {int(fluid.sum())} fluid cells ({int(steady.sum())} with unchanged materials around): max|div| before "
+         f"{scale_div:.3e}; left on those, relative: gpu {left_gpu:.2e}, oracle {left_orc:.2e}; gpu - oracle: "
+         f"div {e_div:.2e} (max, relative), u {e_u:.2e}, v {e_v:.2e}, p {e_p:.2e} (rel L2)")
+    assert steady.sum() > 0.5 * fluid.sum()
+    assert left_gpu <= 1e-5 and left_orc <= 1e-5
+    assert e_div <= 1e-5
+    assert max(e_u, e_v) <= 1e-5 and e_p <= 1e-5
