@@ -394,8 +394,10 @@ void FlipSolver::sync() const
     st.vx = vx.data();
     st.vy = vy.data();
     st.pid = id.data();
-    st.u = velocity_.velocityGridU().data().data();
-    st.v = velocity_.velocityGridV().data().data();
+    // the grid as step() holds it after the last stage hook (between hooks
+    // of an unfinished step, e.g. the projected grid after project())
+    st.work_u = velocity_.velocityGridU().data().data();
+    st.work_v = velocity_.velocityGridV().data().data();
     st.u_valid = uval.data();
     st.v_valid = vval.data();
     st.material = material_.data().data();

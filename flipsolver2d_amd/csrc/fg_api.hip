@@ -134,6 +134,7 @@ struct fg_scene {
     int64_t n_bound = 0;          // upper bound of live particles (host mirror)
     int64_t n_live = 0;           // live particles at the last readback
     bool vmax_ready = false;
+    int last_stage = -1;          // the last fg_scene_stage of an unfinished substep, else -1
     int next_stage = 0;
     int frame = 0;
     int status = FG_OK;           // FG_E_NUMERIC: quarantined until the next set_state
@@ -1614,7 +1615,10 @@ void commit_substep(Group& G, float* dts)
 int substep_group(fg_ctx* ctx, Lane* L, const std::vector<fg_scene*>& scenes, const std::vector<float>& dts,
                   const std::vector<fg_stats*>& stats)
 {
-    for (size_t k = 0; k < scenes.size(); k++) set_substep_scalars(scenes[k], dts[k]);
+    for (size_t k = 0; k < scenes.size(); k++) {
+        set_substep_scalars(scenes[k], dts[k]);
+        scenes[k]->last_stage = -1;
+    }
     Group G;
     FGCHK(make_group(ctx, scenes, stats, G, false, L));
     if (G.n == 0) return FG_OK;
@@ -2174,6 +2178,7 @@ int fg_scene_set_state(fg_scene* s, const fg_scene_state* st)
     if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
     const int64_t n = st->n_particles;
     if (n < 0 || n > s->cap) return fail(FG_E_CAPACITY, "n_particles exceeds the scene's max_particles");
+    s->last_stage = -1;
     if (n > 0 && (!st->px || !st->py || !st->vx || !st->vy))
         return fail(FG_E_INVALID, "particle arrays are required");
     if (!st->u || !st->v || !st->material || !st->solid_sdf)
@@ -2265,8 +2270,18 @@ int fg_scene_get_state(fg_scene* s, fg_scene_state* st)
     FGCHK(dl(st->rhs, h.b, sizeof(double) * s->C));
     FGCHK(dl(st->stencil, h.stencil, s->C));
     FGCHK(dl(st->emitter_id, h.eid, sizeof(int32_t) * s->C));
-    FGCHK(dl(st->work_u, h.u, sizeof(float) * s->FU));
-    FGCHK(dl(st->work_v, h.v, sizeof(float) * s->FV));
+    {   // the grid fluidVelocityGrid holds after the last stage hook
+        const float *wu = h.u, *wv = h.v;   // P2G .. SDF, FORCES, PROJECT (+ SOLIDS)
+        if (s->last_stage == FG_STAGE_EXTRAPOLATE) {   // extrapolated into the saved grid
+            wu = h.su;
+            wv = h.sv;
+        } else if (s->last_stage < FG_STAGE_P2G || s->last_stage >= FG_STAGE_G2P) {   // the substep's grid
+            wu = h.uf;
+            wv = h.vf;
+        }
+        FGCHK(dl(st->work_u, wu, sizeof(float) * s->FU));
+        FGCHK(dl(st->work_v, wv, sizeof(float) * s->FV));
+    }
     st->rng_epoch = s->epoch;
     st->next_pid = s->next_pid;
     HIPCHK(hipStreamSynchronize(q));
@@ -2328,6 +2343,7 @@ int fg_scene_init_objects(fg_scene* s, const fg_scene_objects* o)
     if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
     if (o->solids.n > 256) return fail(FG_E_INVALID, "at most 256 obstacles");
     if (o->sources.n > kMaxEmitters) return fail(FG_E_INVALID, "at most 256 sources");
+    s->last_stage = -1;
     const fg_polygons* sets[4] = {&o->sinks, &o->sources, &o->solids, &o->fluids};
     InitGeo geo{};
     FGCHK(upload_geometry(s, sets, &geo));
@@ -2436,6 +2452,7 @@ int fg_scene_stage(fg_scene* s, int32_t stage, float dt)
         s->epoch++;
         s->next_pid += (uint32_t)s->hsc->reseeded;
     }
+    s->last_stage = stage == FG_STAGE_COUNT ? -1 : stage;
     return FG_OK;
 }
 

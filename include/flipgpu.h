@@ -133,10 +133,13 @@ typedef struct fg_scene_state {
                                    * bits 4-6 nonsolid neighbour count n; bit 7 fluid row.  Row c of
                                    * A is s*(n x_c - sum of the flagged neighbours) on fluid rows
                                    * and x_c elsewhere, s = dt/(rho dx^2) of that substep. */
-    float    *work_u, *work_v;    /* get only: the grid the stages work on (fluidVelocityGrid between
-                                   * the hooks): after FG_STAGE_FORCES the grid the projection reads,
-                                   * after FG_STAGE_PROJECT the projected one.  u and v above are the
-                                   * substep's final, extrapolated grid. */
+    float    *work_u, *work_v;    /* get only: the grid fluidVelocityGrid holds after the last
+                                   * fg_scene_stage of an unfinished substep: after FG_STAGE_P2G the
+                                   * transferred grid, after FG_STAGE_EXTRAPOLATE the extrapolated one,
+                                   * after FG_STAGE_FORCES the one the projection reads, after
+                                   * FG_STAGE_PROJECT the projected one (updateVelocityFromSolids
+                                   * applied too: the two stages share a kernel); otherwise, like u
+                                   * and v above, the substep's final, extrapolated grid. */
 } fg_scene_state;
 
 typedef struct fg_stats {

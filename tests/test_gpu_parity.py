@@ -303,3 +303,35 @@ def test_post_projection_divergence_vs_oracle(fg, orc, ctx):
     assert left_gpu <= 1e-5 and left_orc <= 1e-5
     assert e_div <= 1e-5
     assert max(e_u, e_v) <= 1e-5 and e_p <= 1e-5
+
+
+def test_work_grid_follows_the_stages(fg, orc, ctx):
+    """work_u / work_v are fluidVelocityGrid as step() holds it after each
+    stage hook: the extrapolated grid after FG_STAGE_EXTRAPOLATE (the copy
+    the saved grid keeps), saved + gravity after FG_STAGE_FORCES, and the
+    substep's final grid once the substep is complete."""
+    n, ppc = 64, 4
+    p, st = _prepared(orc, n, ppc, 2)
+    vel = orc.lib().orc_max_particle_velocity(st.ref())
+    dt = np.float32(min(5.0 / (vel + 1e-15), 1.0 / 30))
+    sc = fg.Scene(ctx, fg.make_params(n, ppc=ppc, tol=1e-8))
+    sc.set_friction(np.zeros(4, np.float32))
+    sc.set_state(oracle_state_dict(st))
+    g0 = sc.get_state()
+    assert np.array_equal(g0["work_u"], g0["u"]) and np.array_equal(g0["work_v"], g0["v"])
+    for stage in (fg.STAGE_ADVECT, fg.STAGE_REBIN, fg.STAGE_DENSITY, fg.STAGE_P2G, fg.STAGE_SDF,
+                  fg.STAGE_LEVELSET, fg.STAGE_EXTRAPOLATE):
+        sc.stage(stage, float(dt))
+    g1 = sc.get_state()
+    assert np.array_equal(g1["work_u"], g1["saved_u"]) and np.array_equal(g1["work_v"], g1["saved_v"])
+    sc.stage(fg.STAGE_FORCES, float(dt))
+    g2 = sc.get_state()
+    du, dv = g2["work_u"] - g2["saved_u"], g2["work_v"] - g2["saved_v"]
+    assert np.ptp(du) <= 1e-3 * max(1.0, abs(du).max()) and abs(du).max() > 0   # gravity along i
+    assert abs(dv).max() <= 1e-3 * abs(du).max()
+    for stage in (fg.STAGE_PROJECT, fg.STAGE_SOLIDS, fg.STAGE_G2P, fg.STAGE_COUNT):
+        sc.stage(stage, float(dt))
+    g3 = sc.get_state()
+    sc.close()
+    assert np.array_equal(g3["work_u"], g3["u"]) and np.array_equal(g3["work_v"], g3["v"])
+    assert not np.array_equal(g3["u"], g0["u"])
