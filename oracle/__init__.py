@@ -18,6 +18,7 @@ REF_SO = os.path.join(HERE, "_ref", "libfsref.so")
 SOLVER_REF_REPLICA = 0
 SOLVER_CONVERGED = 1
 DENSITY_OFF, DENSITY_GATED, DENSITY_ON = 0, 1, 2
+SIM_FLIP, SIM_NBFLIP = 0, 1
 FLUID, SOURCE, SOLID, SINK, EMPTY = 0x40, 0x41, 0x20, 0x12, 0x10
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
@@ -41,6 +42,7 @@ class OrcParams(C.Structure):
         ("n_obstacles", C.c_int32), ("friction", C.POINTER(C.c_float)),
         ("n_emitters", C.c_int32), ("emitter_velocity", C.c_void_p), ("emitter_transfer", C.c_void_p),
         ("reseed_seed", C.c_uint32),
+        ("sim_method", C.c_int32), ("source_sdf", C.c_void_p),
     ]
 
 
@@ -125,6 +127,15 @@ def _declare_oracle(L):
         "orc_fluid_sdf": (None, [P, S, _i32p]),
         "orc_update_materials": (None, [P, S]),
         "orc_extrapolate_levelset_inside": (None, [_f32p, C.c_int32, C.c_int32]),
+        "orc_extrapolate_levelset_outside": (None, [_f32p, C.c_int32, C.c_int32]),
+        "orc_euler_advect": (None, [_f32p, _f32p, C.c_int32, C.c_int32, C.c_float, _f32p, C.c_int32, C.c_int32,
+                                    C.c_float, C.c_float, _f32p]),
+        "orc_nb_prune": (None, [P, S, _u8p]),
+        "orc_nb_combine": (None, [P, S, _f32p, _f32p, _f32p]),
+        "orc_nb_reseed": (C.c_int64, [P, S]),
+        "orc_nb_substep": (C.c_int32, [P, S, C.c_float, T]),
+        "orc_nb_init_objects": (None, [P, S, C.c_int32, _i32p, _f32p, C.c_int32, _i32p, _f32p, C.c_int32, _i32p,
+                                       _f32p, C.c_int32, _i32p, _f32p, _f32p]),
         "orc_extrapolate": (None, [_f32p, _u8p, C.c_int32, C.c_int32, C.c_int32]),
         "orc_body_forces": (None, [P, S, C.c_float]),
         "orc_pressure_rhs": (None, [P, S, _f64p]),
@@ -203,8 +214,9 @@ def _declare_ref(L):
 def make_params(n_i, n_j=None, *, dx=None, ppc=4, density=1000.0, gravity=(9.81, 0.0), pic=0.03, cfl=5.0,
                 particle_scale=0.8, fps=30, max_substeps=10, pcg_iter_limit=200,
                 solver=SOLVER_CONVERGED, density_mode=DENSITY_OFF, threads=1, tol=None, friction=(0.0,),
-                emitters=(), reseed_seed=0):
-    """emitters: (vx, vy, transfer) per source, velocity in m/s."""
+                emitters=(), reseed_seed=0, sim=SIM_FLIP):
+    """emitters: (vx, vy, transfer) per source, velocity in m/s; sim: SIM_FLIP
+    (FlipSolver) or SIM_NBFLIP (NBFlipSolver)."""
     n_j = n_i if n_j is None else n_j
     p = OrcParams()
     p.size_i, p.size_j = n_i, n_j
@@ -230,6 +242,8 @@ def make_params(n_i, n_j=None, *, dx=None, ppc=4, density=1000.0, gravity=(9.81,
     p.emitter_velocity = ev.ctypes.data
     p.emitter_transfer = et.ctypes.data
     p.reseed_seed = reseed_seed
+    p.sim_method = sim
+    p.source_sdf = None
     return p
 
 
@@ -349,4 +363,29 @@ def dam_break(n, ppc=4, seed=42, **kw):
     params = make_params(n, ppc=ppc, **kw)
     walls, fluids = dam_break_geometry(n)
     st = init_scene(params, walls, fluids, seed=seed)
+    return params, st
+
+
+def init_nb_scene(params, solids=(), sources=(), sinks=(), fluids=(), capacity=None):
+    """NBFlipSolver::firstFrameInit through the oracle: materials, the fluid
+    SDF of the initial fluid and no particles.  Sets params.source_sdf (kept
+    alive on params) to the per-cell source distance the substeps read."""
+    L = lib()
+    I, J = params.size_i, params.size_j
+    if capacity is None:
+        capacity = I * J * params.particles_per_cell * 2
+    st = State(I, J, capacity)
+    src = np.zeros(I * J, np.float32)
+    L.orc_nb_init_objects(C.byref(params), st.ref(), *_polys(solids), *_polys(sources), *_polys(sinks),
+                          *_polys(fluids), src)
+    params._source_sdf_keep = src
+    params.source_sdf = src.ctypes.data
+    return st
+
+
+def nb_dam_break(n, ppc=8, **kw):
+    """The dam break run by NBFlipSolver (the demo scene's solver)."""
+    params = make_params(n, ppc=ppc, sim=SIM_NBFLIP, **kw)
+    walls, fluids = dam_break_geometry(n)
+    st = init_nb_scene(params, solids=walls, fluids=fluids)
     return params, st

@@ -642,15 +642,19 @@ static inline void neighborhood(long si, long sj, long idx, long out[8])
         }
 }
 
-/* FlipSolver::extrapolateLevelsetInside, flipsolver2d.cpp:1433-1494 (FIFO BFS) */
-void orc_extrapolate_levelset_inside(float* g, int32_t I, int32_t J)
+/* FlipSolver::extrapolateLevelsetInside / extrapolateLevelsetOutside,
+ * flipsolver2d.cpp:1433-1494 / 1496-1557 (FIFO BFS).  They differ only in
+ * the known set (sdf > 0 inside; sdf < I*J as a float outside) and in the
+ * step each level adds (-1 inside, +1 outside). */
+static void levelset_bfs(float* g, int32_t I, int32_t J, bool outside)
 {
     const long C = (long)I * J;
     const int MAXI = std::numeric_limits<int>::max();
+    const float max_sdf = static_cast<float>(static_cast<size_t>(I) * static_cast<size_t>(J));
     std::vector<int> mk(C, MAXI);
     std::queue<long> wf;
     for (long c = 0; c < C; c++)
-        if (g[c] > 0.f) mk[c] = 0;
+        if (outside ? g[c] < max_sdf : g[c] > 0.f) mk[c] = 0;
     long nb[8];
     for (long c = 0; c < C; c++) {
         if (mk[c] != 0) {
@@ -681,10 +685,13 @@ void orc_extrapolate_levelset_inside(float* g, int32_t I, int32_t J)
                 wf.push(q);
             }
         }
-        g[c] = static_cast<float>(avg / count - 1.f);
+        g[c] = outside ? static_cast<float>(avg / count + 1.f) : static_cast<float>(avg / count - 1.f);
         wf.pop();
     }
 }
+
+void orc_extrapolate_levelset_inside(float* g, int32_t I, int32_t J) { levelset_bfs(g, I, J, false); }
+void orc_extrapolate_levelset_outside(float* g, int32_t I, int32_t J) { levelset_bfs(g, I, J, true); }
 
 /* simmath::breadthFirstExtrapolate, mathfuncs.cpp:152-215 (FIFO BFS) */
 void orc_extrapolate(float* g, uint8_t* valid, int32_t si, int32_t sj, int32_t radius)
@@ -1435,7 +1442,8 @@ int32_t orc_step_frame(const OrcParams* p, OrcState* s, OrcStats* st)
         const float vel = orc_max_particle_velocity(s);
         const float dt = orc_frame_dt(vel, p->cfl_number, frame_dt, t, count, p->max_substeps, &finished);
         if (st) st->max_velocity = vel;
-        orc_substep(p, s, dt, st);
+        if (p->sim_method == ORC_SIM_NBFLIP) orc_nb_substep(p, s, dt, st);
+        else orc_substep(p, s, dt, st);
         t += dt;
         count++;
         if (count > 50) break;
@@ -1683,6 +1691,232 @@ int64_t orc_seed_particles(const OrcParams* p, OrcState* s, uint32_t seed)
         }
     s->n_particles = n;
     return n;
+}
+
+
+/* ================================================================== */
+/* NB-FLIP: NBFlipSolver (nbflipsolver.cpp)                             */
+/* ================================================================== */
+/* m_narrowBand, m_combinationBand, m_resamplingBand (nbflipsolver.cpp:18-20) */
+static constexpr float kNarrowBand = -3.f, kCombinationBand = -2.f, kResamplingBand = -1.f;
+
+void orc_euler_advect(const float* u, const float* v, int32_t I, int32_t J, float dt, const float* in, int32_t si,
+                      int32_t sj, float off_i, float off_j, float* out)
+{
+    for (long idx = 0; idx < (long)si * sj; idx++) {
+        const long i = idx / sj, j = idx - (idx / sj) * sj;
+        const V2 q = rk4(u, v, I, J, V2{static_cast<float>(i), static_cast<float>(j)}, -dt);
+        out[idx] = lerp_grid(in, si, sj, off_i, off_j, q.x, q.y);
+    }
+}
+
+void orc_nb_prune(const OrcParams* p, const OrcState* s, uint8_t* alive)
+{
+    const long I = p->size_i, J = p->size_j;
+    const int32_t most = 2 * p->particles_per_cell;
+    for (int64_t k = 0; k < s->n_particles; k++) {
+        if (!alive[k]) continue;
+        const float x = s->px[k], y = s->py[k];
+        /* ssize_t of the position: a live particle is inside the grid, where
+         * the conversion equals the floor */
+        const long c = static_cast<long>(x) * J + static_cast<long>(y);
+        const float sdf = lerp_grid(s->fluid_sdf, I, J, 0.f, 0.f, x, y);
+        const bool src = is_source(s->material[c]);
+        if (!src && sdf < kNarrowBand) {
+            alive[k] = 0;
+            continue;
+        }
+        if ((src || sdf < kResamplingBand) && s->particle_counts[c] > most) alive[k] = 0;
+    }
+}
+
+void orc_nb_combine(const OrcParams* p, OrcState* s, const float* adv_u, const float* adv_v, const float* adv_sdf)
+{
+    const long I = p->size_i, J = p->size_j;
+    const float flt_max = std::numeric_limits<float>::max();
+    for (long c = 0; c < I * J; c++) {
+        /* updateGridFromSources: std::min(dist, sdf) */
+        const float dist = p->source_sdf ? p->source_sdf[c] : flt_max;
+        float f = s->fluid_sdf[c];
+        f = (f < dist) ? f : dist;
+        /* combineLevelset: std::min(advected + h, sdf), h = 1 */
+        const float a = adv_sdf[c] + 1.f;
+        s->fluid_sdf[c] = (f < a) ? f : a;
+    }
+    /* combineVelocityGrid: nbCombine(advected, particle, sdf at the face) */
+    for (long i = 0; i < I + 1; i++)
+        for (long j = 0; j < J; j++) {
+            const float sdf = lerp_grid(s->fluid_sdf, I, J, 0.f, 0.f, static_cast<float>(i), 0.5f + static_cast<float>(j));
+            if (!(sdf > kCombinationBand)) s->u[i * J + j] = adv_u[i * J + j];
+        }
+    for (long i = 0; i < I; i++)
+        for (long j = 0; j < J + 1; j++) {
+            const float sdf = lerp_grid(s->fluid_sdf, I, J, 0.f, 0.f, 0.5f + static_cast<float>(i), static_cast<float>(j));
+            if (!(sdf > kCombinationBand)) s->v[i * (J + 1) + j] = adv_v[i * (J + 1) + j];
+        }
+}
+
+int64_t orc_nb_reseed(const OrcParams* p, OrcState* s)
+{
+    const long I = p->size_i, J = p->size_j;
+    const int32_t ppc = p->particles_per_cell;
+    int64_t added = 0;
+    for (long i = 0; i < I; i++)
+        for (long j = 0; j < J; j++) {
+            const long c = i * J + j;
+            const bool src = is_source(s->material[c]);
+            const float sd = s->fluid_sdf[c];
+            if (!(src || (sd < kResamplingBand && sd > kNarrowBand))) continue;
+            const int32_t more = ppc - s->particle_counts[c];
+            if (more <= 0) continue;
+            const int e = s->emitter_id ? s->emitter_id[c] : -1;
+            const bool transfer = e >= 0 && e < p->n_emitters && p->emitter_transfer && p->emitter_transfer[e];
+            for (int32_t k = 0; k < more; k++) {
+                const float x = static_cast<float>(i) + orc_reseed_jitter(p->reseed_seed, s->rng_epoch, (uint32_t)c, k, 0);
+                const float y = static_cast<float>(j) + orc_reseed_jitter(p->reseed_seed, s->rng_epoch, (uint32_t)c, k, 1);
+                const float ns = lerp_grid(s->fluid_sdf, I, J, 0.f, 0.f, x, y);
+                if ((src && sd > kResamplingBand) || ns < kNarrowBand) continue;
+                if (s->n_particles >= s->capacity) return added;
+                float vx = 0.f, vy = 0.f;
+                if (transfer) {
+                    vx = lerp_grid(s->u, I + 1, J, 0.5f, 0.f, x, y);
+                    vy = lerp_grid(s->v, I, J + 1, 0.f, 0.5f, x, y);
+                }
+                const int64_t n = s->n_particles++;
+                s->px[n] = x;
+                s->py[n] = y;
+                s->vx[n] = vx;
+                s->vy[n] = vy;
+                s->pid[n] = s->next_pid++;
+                added++;
+            }
+        }
+    return added;
+}
+
+int32_t orc_nb_substep(const OrcParams* p, OrcState* s, float dt, OrcStats* st)
+{
+    const long I = p->size_i, J = p->size_j;
+    const int64_t C = (int64_t)I * J;
+    double t0 = now_ms();
+    auto stage = [&](int k) {
+        const double t1 = now_ms();
+        if (st) st->stage_ms[k] += t1 - t0;
+        t0 = t1;
+    };
+    /* 1. advect: FlipSolver::advect, pruneNarrowBand, then the Euler pass of
+     * U, V and the level set over the grid as it is now (66-113) */
+    std::vector<uint8_t> alive(s->n_particles);
+    orc_advect(p, s, dt, alive.data());
+    orc_nb_prune(p, s, alive.data());
+    int64_t killed = 0;
+    for (uint8_t a : alive) killed += a == 0;
+    std::vector<float> adv_u((I + 1) * J), adv_v(I * (J + 1)), adv_sdf(C);
+    orc_euler_advect(s->u, s->v, I, J, dt, s->u, I + 1, J, 0.5f, 0.f, adv_u.data());
+    orc_euler_advect(s->u, s->v, I, J, dt, s->v, I, J + 1, 0.f, 0.5f, adv_v.data());
+    orc_euler_advect(s->u, s->v, I, J, dt, s->fluid_sdf, I, J, 0.f, 0.f, adv_sdf.data());
+    stage(0);
+    /* 2. prune + rebin */
+    std::vector<int32_t> cs(C + 1);
+    orc_sort_particles(p, s, alive.data(), cs.data());
+    stage(3);
+    /* 3. particleToGrid, extrapolate(10), the saved grid */
+    orc_p2g(p, s, cs.data());
+    orc_extrapolate(s->u, s->u_valid, I + 1, J, 10);
+    orc_extrapolate(s->v, s->v_valid, I, J + 1, 10);
+    std::memcpy(s->saved_u, s->u, sizeof(float) * (I + 1) * J);
+    std::memcpy(s->saved_v, s->v, sizeof(float) * I * (J + 1));
+    stage(4);
+    /* 4. gridUpdate (213-226): updateSdf, extrapolateLevelsetOutside,
+     * afterTransfer (FlipSolver's, updateGridFromSources, combine),
+     * extrapolateLevelsetInside, updateMaterials, applyBodyForces */
+    orc_fluid_sdf(p, s, cs.data());
+    orc_extrapolate_levelset_outside(s->fluid_sdf, I, J);
+    orc_after_transfer(p, s);
+    orc_nb_combine(p, s, adv_u.data(), adv_v.data(), adv_sdf.data());
+    orc_extrapolate_levelset_inside(s->fluid_sdf, I, J);
+    orc_update_materials(p, s);
+    orc_body_forces(p, s, dt);
+    stage(5);
+    /* 5. the matrix from the materials as they are now, then project */
+    std::vector<int8_t> mat_a(s->material, s->material + C);
+    const double sA = dt / (p->fluid_density * p->dx * p->dx);
+    stage(1);
+    {
+        std::vector<double> rhs_local, pr_local;
+        double* rhs = s->rhs;
+        double* pr = s->pressure;
+        if (!rhs) { rhs_local.resize(C); rhs = rhs_local.data(); }
+        if (!pr) { pr_local.resize(C); pr = pr_local.data(); }
+        orc_pressure_rhs(p, s, rhs);
+        double rel = 0;
+        bool conv = false;
+        const int32_t it = solve(p, mat_a.data(), sA, rhs, pr, &rel, &conv);
+        if (st) {
+            st->pressure_iters = std::max(st->pressure_iters, it);
+            st->pressure_rel_resid = rel;
+        }
+        orc_apply_pressure(p, s, dt, pr);
+    }
+    stage(7);
+    /* 6. solids, extrapolate, G2P */
+    orc_velocity_from_solids(p, s);
+    orc_extrapolate(s->u, s->u_valid, I + 1, J, 10);
+    orc_extrapolate(s->v, s->v_valid, I, J + 1, 10);
+    orc_g2p(p, s);
+    stage(10);
+    /* 7. countParticles + the band reseed */
+    const int32_t erased = orc_count_particles(p, s, cs.data());
+    orc_nb_reseed(p, s);
+    s->rng_epoch++;
+    stage(11);
+    if (st) {
+        st->particles_killed += killed;
+        st->particles_erased += erased;
+        st->last_dt = dt;
+        st->substeps++;
+    }
+    return 0;
+}
+
+void orc_nb_init_objects(const OrcParams* p, OrcState* s, int32_t n_solids, const int32_t* snv, const float* sv,
+                         int32_t n_sources, const int32_t* srnv, const float* srv, int32_t n_sinks,
+                         const int32_t* sknv, const float* skv, int32_t n_fluids, const int32_t* fnv, const float* fv,
+                         float* source_sdf)
+{
+    const long I = p->size_i, J = p->size_j;
+    /* updateSinks, updateSources, updateSolids (no updateInitialFluid) */
+    orc_init_objects(p, s, n_solids, snv, sv, n_sources, srnv, srv, n_sinks, sknv, skv, 0, nullptr, nullptr);
+    const float dx = static_cast<float>(p->dx);
+    const float flt_max = std::numeric_limits<float>::max();
+    for (long i = 0; i < I; i++)
+        for (long j = 0; j < J; j++) {
+            /* fluidSdfFromInitialFluid: centre in double, as updateSolids */
+            float dist = flt_max;
+            const float* v = fv;
+            for (int32_t k = 0; k < n_fluids; k++) {
+                const float x = static_cast<float>((static_cast<float>(i) + 0.5) * dx);
+                const float y = static_cast<float>((static_cast<float>(j) + 0.5) * dx);
+                const float sd = orc_polygon_sdf(v, fnv[k], x, y) / dx;
+                if (sd < dist) dist = sd;
+                v += 2 * fnv[k];
+            }
+            s->fluid_sdf[i * J + j] = dist;
+            if (dist < 0) s->material[i * J + j] = (int8_t)ORC_FLUID;
+            /* updateGridFromSources' distance: the cell corner (i dx, j dx) */
+            float sdist = flt_max;
+            v = srv;
+            for (int32_t k = 0; k < n_sources; k++) {
+                const float sd = orc_polygon_sdf(v, srnv[k], static_cast<float>(i) * dx, static_cast<float>(j) * dx) / dx;
+                if (sd < sdist) sdist = sd;
+                v += 2 * srnv[k];
+            }
+            if (source_sdf) source_sdf[i * J + j] = sdist;
+        }
+    std::fill(s->particle_counts, s->particle_counts + I * J, 0);
+    s->n_particles = 0;
+    s->next_pid = 0;
+    s->rng_epoch = 0;
 }
 
 } /* extern "C" */

@@ -33,6 +33,7 @@ extern "C" {
 
 enum { ORC_SOLVER_REF_REPLICA = 0, ORC_SOLVER_CONVERGED = 1 };
 enum { ORC_DENSITY_OFF = 0, ORC_DENSITY_GATED = 1, ORC_DENSITY_ON = 2 };
+enum { ORC_SIM_FLIP = 0, ORC_SIM_NBFLIP = 1 };
 
 /* Material codes, materialgrid.h:6-13. */
 enum {
@@ -64,6 +65,9 @@ typedef struct OrcParams {
     const float* emitter_velocity; /* [2 * n_emitters] */
     const uint8_t* emitter_transfer; /* [n_emitters] */
     uint32_t reseed_seed;          /* counter-based RNG seed of reseedParticles */
+    int32_t sim_method;            /* ORC_SIM_FLIP (FlipSolver) or ORC_SIM_NBFLIP (NBFlipSolver) */
+    const float* source_sdf;       /* NB-FLIP: I*J distance (cells) of the nearest source at the cell
+                                    * corner, FLT_MAX without sources (orc_nb_init_objects); may be NULL */
 } OrcParams;
 
 typedef struct OrcState {
@@ -129,6 +133,7 @@ void  orc_p2g(const OrcParams* p, OrcState* s, const int32_t* cell_start);
 void  orc_fluid_sdf(const OrcParams* p, OrcState* s, const int32_t* cell_start);
 void  orc_update_materials(const OrcParams* p, OrcState* s);
 void  orc_extrapolate_levelset_inside(float* sdf, int32_t I, int32_t J);
+void  orc_extrapolate_levelset_outside(float* sdf, int32_t I, int32_t J);
 void  orc_extrapolate(float* g, uint8_t* valid, int32_t si, int32_t sj, int32_t radius);
 void  orc_body_forces(const OrcParams* p, OrcState* s, float dt);
 void  orc_pressure_rhs(const OrcParams* p, const OrcState* s, double* rhs);
@@ -198,6 +203,32 @@ float orc_reseed_jitter(uint32_t seed, uint32_t epoch, uint32_t cell, uint32_t k
 /* Quadrant-jittered seeding with the counter RNG shared with the product's init. */
 int64_t orc_seed_particles(const OrcParams* p, OrcState* s, uint32_t seed);
 float   orc_jitter(uint32_t seed, uint32_t cell, uint32_t k, uint32_t axis);
+
+/* ---- NB-FLIP (NBFlipSolver, nbflipsolver.cpp) ---- */
+/* FlipSolver::eulerAdvectionThread (flipsolver2d.cpp:340-351): out[i,j] =
+ * in lerped (with its offset) at RK4(-dt) from the grid index (i, j). */
+void  orc_euler_advect(const float* u, const float* v, int32_t I, int32_t J, float dt,
+                       const float* in, int32_t si, int32_t sj, float off_i, float off_j, float* out);
+/* NBFlipSolver::pruneNarrowBand (nbflipsolver.cpp:228-252) over the particles
+ * advect left alive (positions already advected). */
+void  orc_nb_prune(const OrcParams* p, const OrcState* s, uint8_t* alive);
+/* updateGridFromSources' distance + combineAdvectedGrids (nbflipsolver.cpp:
+ * 320-419): level set, then the faces by the combined level set. */
+void  orc_nb_combine(const OrcParams* p, OrcState* s, const float* adv_u, const float* adv_v,
+                     const float* adv_sdf);
+/* NBFlipSolver::reseedParticles (nbflipsolver.cpp:115-201); returns how many. */
+int64_t orc_nb_reseed(const OrcParams* p, OrcState* s);
+/* NBFlipSolver::step (nbflipsolver.cpp:26-64). */
+int32_t orc_nb_substep(const OrcParams* p, OrcState* s, float dt, OrcStats* st);
+/* NBFlipSolver::firstFrameInit (nbflipsolver.cpp:203-211): sinks, sources,
+ * solids, then the fluid SDF of the initial fluid (291-318); no particles.
+ * source_sdf_out (I*J) gets updateGridFromSources' per-cell distance. */
+void  orc_nb_init_objects(const OrcParams* p, OrcState* s,
+                          int32_t n_solids, const int32_t* solid_nverts, const float* solid_verts,
+                          int32_t n_sources, const int32_t* source_nverts, const float* source_verts,
+                          int32_t n_sinks, const int32_t* sink_nverts, const float* sink_verts,
+                          int32_t n_fluids, const int32_t* fluid_nverts, const float* fluid_verts,
+                          float* source_sdf_out);
 
 #ifdef __cplusplus
 }
