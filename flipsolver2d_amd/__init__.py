@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libflipgpu.so")
 FG_OK = 0
 FLUID, SOURCE, SOLID, SINK, EMPTY = 0x40, 0x41, 0x20, 0x12, 0x10
 DENSITY_OFF, DENSITY_GATED, DENSITY_ON = 0, 1, 2
+SIM_FLIP, SIM_NBFLIP = 0, 1
 (STAGE_ADVECT, STAGE_REBIN, STAGE_DENSITY, STAGE_P2G, STAGE_SDF, STAGE_LEVELSET, STAGE_EXTRAPOLATE,
  STAGE_FORCES, STAGE_PROJECT, STAGE_SOLIDS, STAGE_G2P, STAGE_COUNT) = range(12)
 STAGE_NAMES = ["advect", "rebin", "density", "p2g", "sdf", "levelset", "extrapolate", "forces", "project",
@@ -61,6 +62,7 @@ class SceneParams(C.Structure):
         ("fps", C.c_int32), ("max_substeps", C.c_int32), ("particles_per_cell", C.c_int32),
         ("pcg_iter_limit", C.c_int32), ("density_mode", C.c_int32),
         ("solver_tolerance", C.c_double), ("max_particles", C.c_int64),
+        ("sim_method", C.c_int32), ("reserved0", C.c_int32),
     ]
 
 
@@ -222,7 +224,10 @@ def _ptr(a):
 
 def make_params(n_i, n_j=None, *, dx=None, ppc=4, density=1000.0, gravity=(9.81, 0.0), pic=0.03, cfl=5.0,
                 particle_scale=0.8, fps=30, max_substeps=10, pcg_iter_limit=2000, density_mode=DENSITY_OFF,
-                tol=1e-6, max_particles=None):
+                tol=1e-6, max_particles=None, sim=SIM_FLIP):
+    """FlipSolverParameters as fg_scene_params; sim: SIM_FLIP (FlipSolver) or
+    SIM_NBFLIP (NBFlipSolver, whose band cells fill to ppc and crowded cells
+    keep up to 2 ppc: its default capacity is twice FLIP's)."""
     n_j = n_i if n_j is None else n_j
     p = SceneParams()
     p.size_i, p.size_j = n_i, n_j
@@ -232,7 +237,8 @@ def make_params(n_i, n_j=None, *, dx=None, ppc=4, density=1000.0, gravity=(9.81,
     p.pic_ratio, p.cfl_number, p.particle_scale = pic, cfl, particle_scale
     p.fps, p.max_substeps, p.particles_per_cell = fps, max_substeps, ppc
     p.pcg_iter_limit, p.density_mode, p.solver_tolerance = pcg_iter_limit, density_mode, tol
-    p.max_particles = n_i * n_j * ppc if max_particles is None else max_particles
+    p.max_particles = n_i * n_j * ppc * (2 if sim == SIM_NBFLIP else 1) if max_particles is None else max_particles
+    p.sim_method = sim
     return p
 
 
@@ -392,7 +398,7 @@ class Scene:
                 dt = {"px": np.float32, "py": np.float32, "vx": np.float32, "vy": np.float32, "pid": np.uint32,
                       "u": np.float32, "v": np.float32, "u_valid": np.uint8, "v_valid": np.uint8,
                       "material": np.int8, "solid_sdf": np.float32, "fluid_sdf": np.float32,
-                      "solid_id": np.int32, "emitter_id": np.int32}.get(name)
+                      "solid_id": np.int32, "emitter_id": np.int32, "particle_counts": np.int32}.get(name)
                 if dt is None:
                     continue
                 a = np.ascontiguousarray(st[name], dt)
@@ -478,6 +484,17 @@ def dam_break_scene(ctx, n, ppc=4, seed=42, fluid_box=(0.3, 1.0, 0.0, 0.4), **kw
     sc = Scene(ctx, p)
     walls, fluid = dam_break_geometry(n, fluid_box)
     sc.init_geometry(walls, fluid, friction=np.zeros(len(walls), np.float32), seed=seed)
+    return sc
+
+
+def nb_dam_break_scene(ctx, n, ppc=8, seed=0, fluid_box=(0.3, 1.0, 0.0, 0.4), **kw):
+    """The dam break run by NBFlipSolver (the solver the reference's demo
+    scene file names): first frame on the device, no particles yet; seed is
+    the reseed RNG's."""
+    p = make_params(n, ppc=ppc, sim=SIM_NBFLIP, **kw)
+    sc = Scene(ctx, p)
+    walls, fluid = dam_break_geometry(n, fluid_box)
+    sc.init_objects(solids=walls, fluids=fluid, friction=np.zeros(len(walls), np.float32), seed=seed)
     return sc
 
 

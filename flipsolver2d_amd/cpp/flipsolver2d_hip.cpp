@@ -201,6 +201,8 @@ FlipSolver::FlipSolver(const FlipSolverParameters* p, fg_ctx* sharedCtx, int dev
     sp.density_mode = FG_DENSITY_OFF;
     sp.solver_tolerance = 1e-6;
     sp.max_particles = (int64_t)params_.gridSizeI * params_.gridSizeJ * params_.particlesPerCell;
+    sp.sim_method = params_.simulationMethod == SIMULATION_NBFLIP ? FG_SIM_NBFLIP : FG_SIM_FLIP;
+    if (sp.sim_method == FG_SIM_NBFLIP) sp.max_particles *= 2;   // band cells fill to ppc, crowded ones hold 2 ppc
     const int rc = fg_scene_create(ctx_, &sp, &scene_);
     if (rc != FG_OK) {
         const std::string msg = fg_last_error();
@@ -212,6 +214,34 @@ FlipSolver::FlipSolver(const FlipSolverParameters* p, fg_ctx* sharedCtx, int dev
     velocity_ = StaggeredVelocityGrid(gridSizeI(), gridSizeJ());
     fluidSdf_ = SdfGrid(gridSizeI(), gridSizeJ());
     solidSdf_ = SdfGrid(gridSizeI(), gridSizeJ());
+}
+
+namespace {
+// the parameters of an NBFlipSolver (a temporary that lives through the base
+// constructor's initialiser)
+struct NBParams {
+    FlipSolverParameters p;
+    const FlipSolverParameters* get() const { return &p; }
+};
+NBParams nbflip_params(const NBFlipParameters* p)
+{
+    if (!p) throw Error(FG_E_INVALID, "NBFlipParameters is NULL");
+    NBParams r{*p};
+    r.p.simulationMethod = SIMULATION_NBFLIP;
+    return r;
+}
+}  // namespace
+
+NBFlipSolver::NBFlipSolver(const NBFlipParameters* p, fg_ctx* sharedCtx, int device)
+    : FlipSolver(nbflip_params(p).get(), sharedCtx, device)
+{
+}
+
+// NBFlipSolver::step (nbflipsolver.cpp:26-64) as one device substep
+void NBFlipSolver::step()
+{
+    fg_stats st{};
+    check(fg_scene_substep(deviceScene(), stepDt(), &st));
 }
 
 FlipSolver::~FlipSolver()

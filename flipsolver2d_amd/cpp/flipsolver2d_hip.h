@@ -549,10 +549,10 @@ protected:
     // step() substep by substep)
     virtual bool stagesOverridden() const { return false; }
     void runStage(int stage);
+    static void check(int rc);
 
 private:
     void sync() const;
-    static void check(int rc);
     void stepFrameOnDevice();
 
     FlipSolverParameters params_;
@@ -573,6 +573,23 @@ private:
     mutable MaterialGrid material_;
     mutable StaggeredVelocityGrid velocity_;
     mutable SdfGrid fluidSdf_, solidSdf_;
+};
+
+// NBFlipParameters (nbflipsolver.h:8-11): no fields of its own.
+struct NBFlipParameters : FlipSolverParameters {
+};
+
+// NBFlipSolver (nbflipsolver.h:13-80, nbflipsolver.cpp): the narrow-band
+// variant.  Its step(), first frame and band rules run on the device
+// (fg_scene_params.sim_method = FG_SIM_NBFLIP); the solve tolerance is the
+// constructor's 1e-6 (nbflipsolver.cpp:22).  Its step() is whole: the stage
+// hooks follow FlipSolver::step's order and refuse an NB-FLIP scene.
+class NBFlipSolver : public FlipSolver {
+public:
+    explicit NBFlipSolver(const NBFlipParameters* p, fg_ctx* sharedCtx = nullptr, int device = 0);
+
+protected:
+    void step() override;
 };
 
 }  // namespace fghip

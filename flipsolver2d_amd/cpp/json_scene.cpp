@@ -338,7 +338,15 @@ std::shared_ptr<FlipSolver> JsonSceneReader::loadJson(const std::string& fileNam
         SceneDescription d = readScene(fileName);
         if (d.params.simulationMethod == SIMULATION_SMOKE || d.params.simulationMethod == SIMULATION_FIRE)
             throw Error(FG_E_INVALID, "smoke and fire scenes are not supported (liquid solver only)");
-        auto solver = std::make_shared<FlipSolver>(&d.params);
+        // jsonscenereader.cpp:50-62: an "nbflip" scene builds NBFlipSolver
+        std::shared_ptr<FlipSolver> solver;
+        if (d.params.simulationMethod == SIMULATION_NBFLIP) {
+            NBFlipParameters np;
+            static_cast<FlipSolverParameters&>(np) = d.params;
+            solver = std::make_shared<NBFlipSolver>(&np);
+        } else {
+            solver = std::make_shared<FlipSolver>(&d.params);
+        }
         solver->initAdditionalParameters();
         for (Obstacle& o : d.solids) solver->addGeometry(o);
         for (Emitter& e : d.sources) solver->addSource(e);

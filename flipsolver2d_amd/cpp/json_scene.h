@@ -50,8 +50,9 @@ class JsonSceneReader {
 public:
     // The reference's entry point (jsonscenereader.cpp:8-77): a solver with
     // the scene's parameters and objects, or an empty pointer (with the
-    // error printed) when the file cannot be read.  "flip" and "nbflip"
-    // scenes build the FLIP liquid solver; smoke and fire scenes are refused.
+    // error printed) when the file cannot be read.  "flip" scenes build the
+    // FLIP liquid solver, "nbflip" ones NBFlipSolver; smoke and fire scenes
+    // are refused.
     static std::shared_ptr<FlipSolver> loadJson(const std::string& fileName);
     // The same file as a description (throws Error).
     static SceneDescription readScene(const std::string& fileName);

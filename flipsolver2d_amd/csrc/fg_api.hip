@@ -350,6 +350,12 @@ void carve(fg_scene* s, Carver& cv)
     h.friction = cv.take<float>(256);
     h.eid = cv.take<int32_t>(C);
     h.emitters = cv.take<float4>(kMaxEmitters);
+    if (s->p.sim_method == FG_SIM_NBFLIP) {
+        h.adv_u = cv.take<float>(FU);
+        h.adv_v = cv.take<float>(FV);
+        h.adv_sdf = cv.take<float>(C);
+        h.src_sdf = cv.take<float>(C);
+    }
 }
 
 uint64_t arena_bytes(fg_scene* s)
@@ -371,6 +377,9 @@ int validate_params(const fg_scene_params* p)
     if (p->max_particles < 0 || p->max_particles > ((int64_t)1 << 31) - 1)
         return fail(FG_E_INVALID, "max_particles out of range");
     if (p->density_mode < 0 || p->density_mode > 2) return fail(FG_E_INVALID, "bad density_mode");
+    if (p->sim_method != FG_SIM_FLIP && p->sim_method != FG_SIM_NBFLIP) return fail(FG_E_INVALID, "bad sim_method");
+    if (p->sim_method == FG_SIM_NBFLIP && p->density_mode != FG_DENSITY_OFF)
+        return fail(FG_E_INVALID, "NBFlipSolver::step has no density correction: density_mode must be OFF");
     return FG_OK;
 }
 
@@ -437,6 +446,7 @@ int setup_scene_memory(fg_scene* s)
     h.max_inner = p->pcg_iter_limit > 0 ? p->pcg_iter_limit : 2000;
     h.active = 1;
     h.density_mode = p->density_mode;
+    h.nb = p->sim_method == FG_SIM_NBFLIP ? 1 : 0;
     // flipsolver2d.cpp:207-208
     h.cell_vol = (float)(p->dx * p->dx * p->dx);
     h.pmass = (float)((p->fluid_density * h.cell_vol) / (float)p->particles_per_cell);
@@ -453,6 +463,7 @@ int setup_scene_memory(fg_scene* s)
         HIPCHK(hipMemsetAsync(h.mat, (int)(int8_t)FG_EMPTY, s->C, st));
         HIPCHK(hipMemsetAsync(h.sid, 0xff, sizeof(int32_t) * s->C, st));
         HIPCHK(hipMemsetAsync(h.eid, 0xff, sizeof(int32_t) * s->C, st));
+        if (h.nb) HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(h.src_sdf), 0x7f7fffff, s->C, st));
     }
     HIPCHK(hipMemcpyAsync(h.mg, &s->mgh, sizeof(MgLevels), hipMemcpyHostToDevice, st));
     FGCHK(upload_desc(s));
@@ -510,7 +521,7 @@ void stats_reset(fg_stats* st)
 // launch geometry
 // --------------------------------------------------------------------------
 struct Geo {
-    dim3 cells, parts, scan, scan1, faces, extrap, ls, one, pcg, bins, p2g;
+    dim3 cells, parts, scan, scan1, faces, extrap, ls, one, pcg, bins, p2g, nodes;
 };
 
 // x extents of one scene's launches (y is set by the launch group)
@@ -530,20 +541,23 @@ Geo geometry(const fg_scene* s)
     g.pcg = dim3((unsigned)s->pcg_blocks);
     g.bins = dim3((unsigned)cdiv(cdiv(s->I, 3) * cdiv(s->J, 3), kBlock / kSdfLanes));
     g.p2g = dim3((unsigned)(cdiv(s->I, 16) * cdiv(s->J, 16)));
+    g.nodes = dim3((unsigned)cdiv((int64_t)(s->I + 1) * (s->J + 1), kBlock));
     return g;
 }
 
 void geo_max(Geo& a, const Geo& b)
 {
-    dim3* pa[] = {&a.cells, &a.parts, &a.scan, &a.scan1, &a.faces, &a.extrap, &a.ls, &a.one, &a.pcg, &a.bins, &a.p2g};
-    const dim3* pb[] = {&b.cells, &b.parts, &b.scan, &b.scan1, &b.faces, &b.extrap, &b.ls, &b.one, &b.pcg, &b.bins,
-                        &b.p2g};
-    for (int k = 0; k < 11; k++) pa[k]->x = std::max(pa[k]->x, pb[k]->x);
+    dim3* pa[] = {&a.cells, &a.parts, &a.scan, &a.scan1, &a.faces, &a.extrap,
+                  &a.ls, &a.one, &a.pcg, &a.bins, &a.p2g, &a.nodes};
+    const dim3* pb[] = {&b.cells, &b.parts, &b.scan, &b.scan1, &b.faces, &b.extrap,
+                        &b.ls, &b.one, &b.pcg, &b.bins, &b.p2g, &b.nodes};
+    for (int k = 0; k < 12; k++) pa[k]->x = std::max(pa[k]->x, pb[k]->x);
 }
 
 void geo_set_y(Geo& a, int n)
 {
-    dim3* pa[] = {&a.cells, &a.parts, &a.scan, &a.scan1, &a.faces, &a.extrap, &a.ls, &a.one, &a.pcg, &a.bins, &a.p2g};
+    dim3* pa[] = {&a.cells, &a.parts, &a.scan, &a.scan1, &a.faces, &a.extrap,
+                  &a.ls, &a.one, &a.pcg, &a.bins, &a.p2g, &a.nodes};
     for (dim3* d : pa) d->y = (unsigned)n;
 }
 
@@ -752,6 +766,7 @@ struct Group {
     int max_bands = 0;
     int max_j = 0;                      // the widest scene's J
     bool sources = false;               // some scene has SOURCE cells (afterTransfer, reseeding)
+    bool nb = false;                    // NB-FLIP scenes (a group holds one solver class)
     int mg_levels = 0;                  // most multigrid levels of a scene using the V-cycle
     int map_kinds = 0;                  // block-map kinds the solve uses (0 when maps are off)
     int64_t map_off[kMapKinds]{};       // offset of each kind's map in the lane's map buffer
@@ -897,6 +912,7 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
         G.parts += s->n_bound;
         G.max_inner = std::max(G.max_inner, s->h.max_inner);
         G.sources = G.sources || s->h.has_sources;
+        G.nb = G.nb || s->h.nb;
         G.max_bands = std::max(G.max_bands, (int)std::min<int64_t>(kMaxBands, cdiv(std::max(s->I, s->J), 16) + 1));
     }
     geo_set_y(G.g, G.n);
@@ -1407,6 +1423,26 @@ int run_solve(Group& G, bool rhs_from_grid)
 // ---------------------------------------------------------------------------
 // stages of FlipSolver::step
 // ---------------------------------------------------------------------------
+// The band launches of a level-set extrapolation whose markers
+// k_levelset_init has set.  A band launch after the last one needed exits
+// at once but still dispatches its blocks: the first poll comes after one
+// band more than the group's scenes ran last substep, later ones every kChunk.
+int levelset_bands(Group& G, bool outside)
+{
+    constexpr int kChunk = 8;
+    int hint = 0;
+    for (fg_scene* s : G.sc) hint = std::max(hint, s->ls_hint);
+    int chunk = hint > 0 ? hint + 1 : kChunk;
+    int32_t flags[3];
+    for (int band = 0; band < G.max_bands; band += chunk, chunk = kChunk) {
+        const int end = std::min(G.max_bands, band + chunk);
+        for (int k = band; k < end; k++) launch_levelset_band(G.g.ls, G.L->stream, G.d, k, outside);
+        FGCHK(poll_group(G, end - 1, flags));
+        if (!flags[2]) break;
+    }
+    return FG_OK;
+}
+
 int run_stage(Group& G, int stage)
 {
     fg_ctx* ctx = G.ctx;
@@ -1480,22 +1516,9 @@ int run_stage(Group& G, int stage)
     }
     case FG_STAGE_LEVELSET: {
         KProf kp(G, FG_K_LEVELSET);
-        launch_levelset_init(g.ls, st, d);
+        launch_levelset_init(g.ls, st, d, false);
         launch_sdf_exact(g.bins, st, d);   // only where no cell is outside
-        // a band launch after the last one needed exits at once but still
-        // dispatches its blocks: the first poll comes after one band more
-        // than the group's scenes ran last substep, later ones every kChunk
-        constexpr int kChunk = 8;
-        int hint = 0;
-        for (fg_scene* s : G.sc) hint = std::max(hint, s->ls_hint);
-        int chunk = hint > 0 ? hint + 1 : kChunk;
-        int32_t flags[3];
-        for (int band = 0; band < G.max_bands; band += chunk, chunk = kChunk) {
-            const int end = std::min(G.max_bands, band + chunk);
-            for (int k = band; k < end; k++) launch_levelset_band(g.ls, st, d, k);
-            FGCHK(poll_group(G, end - 1, flags));
-            if (!flags[2]) break;
-        }
+        FGCHK(levelset_bands(G, false));
         kp.done(G.cells);
         break;
     }
@@ -1543,8 +1566,9 @@ int run_stage(Group& G, int stage)
     case FG_STAGE_COUNT:
         // countParticles runs inside k_g2p_count; reseedParticles
         // (flipsolver2d.cpp:627-680) fills the SOURCE cells of the scenes
-        // that have them: per-cell additions, their scan, the writes
-        if (!G.sources) break;
+        // that have them, NB-FLIP's (nbflipsolver.cpp:115-201) its band
+        // cells: per-cell additions, their scan, the writes
+        if (!G.sources && !G.nb) break;
         launch_reseed_count(g.cells, st, d);
         launch_scan_local(g.scan, st, d);
         launch_scan_top(g.scan1, st, d);
@@ -1611,9 +1635,80 @@ void commit_substep(Group& G, float* dts)
     }
 }
 
-// One FlipSolver::step for every scene of `scenes`, each with its own dt.
-int substep_group(fg_ctx* ctx, Lane* L, const std::vector<fg_scene*>& scenes, const std::vector<float>& dts,
-                  const std::vector<fg_stats*>& stats)
+// NBFlipSolver::step (nbflipsolver.cpp:26-64) for a group of NB-FLIP
+// scenes.  Stage times are charged to the FlipSolver stage of the same work.
+int run_nb_substep(Group& G, Timer& tm)
+{
+    hipStream_t st = G.L->stream;
+    const Geo& g = G.g;
+    const SceneDev* d = G.d;
+    // advect(): FlipSolver::advect with the band prune, then the Euler pass
+    // over the grid and the level set as they are now (66-113)
+    {
+        KProf kp(G, FG_K_ADVECT, true);
+        launch_advect(g.parts, st, d, 1);
+        kp.done(G.parts);
+    }
+    launch_nb_euler(g.nodes, st, d);
+    tm.lap(FG_STAGE_ADVECT);
+    {   // pruneParticles + rebin
+        KProf kp(G, FG_K_SORT);
+        launch_scan_local(g.scan, st, d);
+        launch_scan_top(g.scan1, st, d);
+        launch_scan_add(g.scan, st, d);
+        launch_scatter(g.parts, st, d);
+        launch_fixup(g.cells, st, d);
+        launch_gather(g.parts, st, d);
+        launch_finalize(g.one, st, d, 1);
+        kp.done(G.parts);
+    }
+    tm.lap(FG_STAGE_REBIN);
+    {   // particleToGrid, extrapolate(10): the saved grid
+        KProf kp(G, FG_K_P2G, true);
+        launch_p2g(g.p2g, st, d);
+        kp.done(0);
+    }
+    tm.lap(FG_STAGE_P2G);
+    {
+        KProf kp(G, FG_K_EXTRAPOLATE);
+        launch_extrapolate(g.extrap, st, d, 0);
+        kp.done(G.faces);
+    }
+    tm.lap(FG_STAGE_EXTRAPOLATE);
+    // gridUpdate (213-226): updateSdf, extrapolateLevelsetOutside,
+    // afterTransfer (the source faces, updateGridFromSources, the
+    // combination), extrapolateLevelsetInside, updateMaterials, body forces
+    {
+        KProf kp(G, FG_K_SDF, true);
+        launch_sdf_materials(g.bins, st, d);
+        kp.done(0);
+    }
+    tm.lap(FG_STAGE_SDF);
+    {
+        KProf kp(G, FG_K_LEVELSET);
+        launch_levelset_init(g.ls, st, d, true);
+        FGCHK(levelset_bands(G, true));
+        launch_nb_combine_sdf(g.cells, st, d);
+        launch_nb_combine_vel(g.faces, st, d);
+        launch_levelset_init(g.ls, st, d, false);
+        FGCHK(levelset_bands(G, false));
+        launch_nb_materials(g.cells, st, d);
+        kp.done(2 * G.cells);
+    }
+    tm.lap(FG_STAGE_LEVELSET);
+    // the pressure matrix from the materials as they are now (45-47)
+    launch_stencil(g.cells, st, d);
+    tm.lap(FG_STAGE_FORCES);
+    for (int stage : {FG_STAGE_PROJECT, FG_STAGE_SOLIDS, FG_STAGE_G2P, FG_STAGE_COUNT}) {
+        FGCHK(run_stage(G, stage));
+        tm.lap(stage);
+    }
+    return FG_OK;
+}
+
+// One step for every scene of `scenes` (one solver class), each with its own dt.
+int substep_one_class(fg_ctx* ctx, Lane* L, const std::vector<fg_scene*>& scenes, const std::vector<float>& dts,
+                      const std::vector<fg_stats*>& stats)
 {
     for (size_t k = 0; k < scenes.size(); k++) {
         set_substep_scalars(scenes[k], dts[k]);
@@ -1624,18 +1719,46 @@ int substep_group(fg_ctx* ctx, Lane* L, const std::vector<fg_scene*>& scenes, co
     if (G.n == 0) return FG_OK;
     hipLaunchKernelGGL(k_substep_begin, G.g.one, dim3(64), 0, G.L->stream, G.d);
     Timer tm(ctx, &G);
-    static const int order[] = {FG_STAGE_ADVECT, FG_STAGE_REBIN, FG_STAGE_DENSITY, FG_STAGE_P2G,
-                                FG_STAGE_SDF, FG_STAGE_LEVELSET, FG_STAGE_EXTRAPOLATE, FG_STAGE_FORCES,
-                                FG_STAGE_PROJECT, FG_STAGE_SOLIDS, FG_STAGE_G2P, FG_STAGE_COUNT};
-    for (int stage : order) {
-        FGCHK(run_stage(G, stage));
-        tm.lap(stage);
+    if (G.nb) {
+        FGCHK(run_nb_substep(G, tm));
+    } else {
+        static const int order[] = {FG_STAGE_ADVECT, FG_STAGE_REBIN, FG_STAGE_DENSITY, FG_STAGE_P2G,
+                                    FG_STAGE_SDF, FG_STAGE_LEVELSET, FG_STAGE_EXTRAPOLATE, FG_STAGE_FORCES,
+                                    FG_STAGE_PROJECT, FG_STAGE_SOLIDS, FG_STAGE_G2P, FG_STAGE_COUNT};
+        for (int stage : order) {
+            FGCHK(run_stage(G, stage));
+            tm.lap(stage);
+        }
     }
     HIPCHK(hipGetLastError());
     FGCHK(read_group_scalars(G));
     stage_bytes(G);
     std::vector<float> d(dts);
     commit_substep(G, d.data());
+    return FG_OK;
+}
+
+// One FlipSolver::step (NBFlipSolver::step for NB-FLIP scenes) for every
+// scene of `scenes`, each with its own dt; the two solver classes run as
+// separate groups, one after the other on the lane.
+int substep_group(fg_ctx* ctx, Lane* L, const std::vector<fg_scene*>& scenes, const std::vector<float>& dts,
+                  const std::vector<fg_stats*>& stats)
+{
+    bool mixed = false;
+    for (fg_scene* s : scenes) mixed = mixed || (s->h.nb != scenes[0]->h.nb);
+    if (!mixed) return substep_one_class(ctx, L, scenes, dts, stats);
+    for (int cls = 0; cls < 2; cls++) {
+        std::vector<fg_scene*> sc;
+        std::vector<float> dt;
+        std::vector<fg_stats*> st;
+        for (size_t k = 0; k < scenes.size(); k++)
+            if (scenes[k]->h.nb == cls) {
+                sc.push_back(scenes[k]);
+                dt.push_back(dts[k]);
+                st.push_back(k < stats.size() ? stats[k] : nullptr);
+            }
+        FGCHK(substep_one_class(ctx, L, sc, dt, st));
+    }
     return FG_OK;
 }
 
@@ -2183,6 +2306,8 @@ int fg_scene_set_state(fg_scene* s, const fg_scene_state* st)
         return fail(FG_E_INVALID, "particle arrays are required");
     if (!st->u || !st->v || !st->material || !st->solid_sdf)
         return fail(FG_E_INVALID, "u, v, material and solid_sdf are required");
+    if (s->h.nb && !st->fluid_sdf)
+        return fail(FG_E_INVALID, "an NB-FLIP scene needs fluid_sdf (its next substep advects the level set)");
     hipStream_t q = s->ctx->stream;
     const SceneDev& h = s->h;
     std::vector<float2> pos(n), vel(n);
@@ -2207,6 +2332,9 @@ int fg_scene_set_state(fg_scene* s, const fg_scene_state* st)
     HIPCHK(hipMemcpyAsync(h.mat, st->material, s->C, hipMemcpyHostToDevice, q));
     HIPCHK(hipMemcpyAsync(h.ssdf, st->solid_sdf, sizeof(float) * s->C, hipMemcpyHostToDevice, q));
     if (st->fluid_sdf) HIPCHK(hipMemcpyAsync(h.fsdf, st->fluid_sdf, sizeof(float) * s->C, hipMemcpyHostToDevice, q));
+    if (st->particle_counts)
+        HIPCHK(hipMemcpyAsync(h.pcount, st->particle_counts, sizeof(int32_t) * s->C, hipMemcpyHostToDevice, q));
+    else HIPCHK(hipMemsetAsync(h.pcount, 0, sizeof(int32_t) * s->C, q));
     if (st->solid_id) HIPCHK(hipMemcpyAsync(h.sid, st->solid_id, sizeof(int32_t) * s->C, hipMemcpyHostToDevice, q));
     else HIPCHK(hipMemsetAsync(h.sid, 0xff, sizeof(int32_t) * s->C, q));
     if (st->emitter_id)
@@ -2426,6 +2554,7 @@ int fg_scene_stage(fg_scene* s, int32_t stage, float dt)
     if (!s) return fail(FG_E_INVALID, "scene is NULL");
     if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
     if (stage < 0 || stage >= FG_STAGE_NUM) return fail(FG_E_INVALID, "bad stage");
+    if (s->h.nb) return fail(FG_E_INVALID, "stage hooks follow FlipSolver::step; an NB-FLIP scene steps whole");
     if (s->status != FG_OK) return fail(s->status, "scene is quarantined (set_state gives it a new state)");
     if (stage == FG_STAGE_ADVECT) {
         if (!(dt > 0.f)) return fail(FG_E_INVALID, "dt must be positive");

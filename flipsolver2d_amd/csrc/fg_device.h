@@ -193,8 +193,15 @@ struct SceneDev {
     uint32_t rseed;         // reseed RNG seed
     uint32_t epoch;         // substeps taken before this one (the reseed RNG's counter)
     uint32_t next_pid;      // id of this substep's first reseeded particle
+    // NB-FLIP (NBFlipSolver): the step's semi-Lagrangian copies of the grid
+    // and the level set, and per cell the distance (cells) of the nearest
+    // source at the cell corner (FLT_MAX without sources)
+    int32_t nb;
+    float *adv_u, *adv_v, *adv_sdf, *src_sdf;
 };
 constexpr int kMaxEmitters = 256;
+// NBFlipSolver's bands (nbflipsolver.cpp:18-20), in cells of level set
+constexpr float kNbNarrowBand = -3.f, kNbCombinationBand = -2.f, kNbResamplingBand = -1.f;
 
 // counter-based uniform in [0,1): 24 bits of a murmur3-finalised hash of
 // (seed, cell, k, axis); the same function seeds and reseeds particles
@@ -265,6 +272,24 @@ __device__ __forceinline__ float lerp_grid(const float* g, int si, int sj, float
 __device__ __forceinline__ float2 velocity_at(const float* u, const float* v, int I, int J, float x, float y)
 {
     return make_float2(lerp_grid(u, I + 1, J, 0.5f, 0.f, x, y), lerp_grid(v, I, J + 1, 0.f, 0.5f, x, y));
+}
+
+// FlipSolver::rk4Integrate, flipsolver2d.cpp:1195-1203 (Vec3 arithmetic of
+// geometry2d.cpp:3-27: a scalar times a vector scales each component)
+__device__ __forceinline__ float2 rk4(const float* u, const float* v, int I, int J, float2 p, float dt)
+{
+    float2 a = velocity_at(u, v, I, J, p.x, p.y);
+    const float k1x = a.x * dt, k1y = a.y * dt;
+    a = velocity_at(u, v, I, J, p.x + k1x * 0.5f, p.y + k1y * 0.5f);
+    const float k2x = a.x * dt, k2y = a.y * dt;
+    a = velocity_at(u, v, I, J, p.x + k2x * 0.5f, p.y + k2y * 0.5f);
+    const float k3x = a.x * dt, k3y = a.y * dt;
+    a = velocity_at(u, v, I, J, p.x + k3x, p.y + k3y);
+    const float k4x = a.x * dt, k4y = a.y * dt;
+    const float sx = ((k1x + k2x * 2.f) + k3x * 2.f) + k4x;
+    const float sy = ((k1y + k2y * 2.f) + k3y * 2.f) + k4y;
+    const float sixth = 1.0f / 6.0f;
+    return make_float2(p.x + sx * sixth, p.y + sy * sixth);
 }
 
 // simmath::bSpline / quadraticBSpline, mathfuncs.cpp:32-43.  The reference

@@ -297,8 +297,10 @@ __device__ __forceinline__ void sdf_bins(const SceneDev& s)
     // sdf < 0 whatever its window holds; extrapolateLevelsetInside then
     // overwrites its value (when some cell is outside, which k_sdf_exact
     // checks).  A bin whose cells all hold particles skips its scan.
+    // (NB-FLIP combines the level set with the advected one before the inside
+    // extrapolation, so it needs every value: no shortcut there)
     bool full = false;
-    if (!kExact && s.sdf_short) {
+    if (!kExact && s.sdf_short && !s.nb) {
         int have = 0;
 #pragma unroll
         for (int a = 0; a < 3; a++)
@@ -364,6 +366,8 @@ __device__ __forceinline__ void sdf_bins(const SceneDev& s)
             continue;
         }
         s.fsdf[c] = sdf;
+        s.pcount[c] = min(n, 2 * s.ppc);
+        if (s.nb) continue;   // NB-FLIP updates the materials from its final level set
         int8_t m = mq[q / kSdfLanes];
         if (sdf < 0.f) {
             if (is_empty(m)) m = M_FLUID;
@@ -371,7 +375,6 @@ __device__ __forceinline__ void sdf_bins(const SceneDev& s)
             if (is_strict_fluid(m)) m = M_EMPTY;
         }
         s.mat[c] = m;
-        s.pcount[c] = min(n, 2 * s.ppc);
     }
 }
 
@@ -416,7 +419,7 @@ constexpr int16_t kUnreached = 0x7ffe;
 // level L with shifts and lane shuffles, and removes the reached set, so the
 // expansion costs a few instructions per row and level and needs no
 // barrier.  The value pass then visits only the cells of each level.
-template <int H, bool LEVELSET>
+template <int H, bool LEVELSET, bool OUT = false>
 __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mkin, float* gout, uint8_t* vout,
                          int32_t* mkout, int si, int sj, int L0, int* more, uint8_t* tile_more)
 {
@@ -630,7 +633,8 @@ __device__ void bfs_band(const float* gin, const uint8_t* vin, const int32_t* mk
                     avg += (double)qv[e];
                     count++;
                 }
-            if (LEVELSET) val[ri * RS + rj] = (float)(avg / count - 1.f);
+            if (LEVELSET && OUT) val[ri * RS + rj] = (float)(avg / count + 1.f);
+            else if (LEVELSET) val[ri * RS + rj] = (float)(avg / count - 1.f);
             else val[ri * RS + rj] = (float)(avg / count);
             mk[ri * RSM + rj] = (int16_t)L;   // same-level neighbours read L or unreached: not below L either way
         }
@@ -685,7 +689,9 @@ __global__ void __launch_bounds__(kBlock) k_extrapolate(const SceneDev* scenes, 
 
 // Markers (known cells: sdf > 0, level 0), one block per BFS tile, which
 // records whether it holds a known cell (band 0's front); clears the band
-// flags.
+// flags.  OUT: extrapolateLevelsetOutside (flipsolver2d.cpp:1496-1557),
+// whose known cells hold less than I*J (as a float) and whose levels add 1.
+template <bool OUT>
 __global__ void __launch_bounds__(kBlock) k_levelset_init(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
@@ -703,13 +709,14 @@ __global__ void __launch_bounds__(kBlock) k_levelset_init(const SceneDev* scenes
         const int i = ti * kTile + q / kTile, j = tj * kTile + q % kTile;
         if (i >= I || j >= J) continue;
         const int c = i * J + j;
-        const bool k = s.fsdf[c] > 0.f;
+        const float f = s.fsdf[c];
+        const bool k = OUT ? f < (float)((uint64_t)I * (uint64_t)J) : f > 0.f;
         s.marker[c] = k ? 0 : kMaxLevel;
         known |= k;
     }
     if (__syncthreads_or(known) && threadIdx.x == 0) {
         s.lsTile[tile] = 1;
-        s.sc->sdf_known = 1;   // some cell is outside
+        if (!OUT) s.sc->sdf_known = 1;   // some cell is outside
     } else if (threadIdx.x == 0) {
         s.lsTile[tile] = 0;
     }
@@ -722,6 +729,7 @@ __global__ void __launch_bounds__(kBlock) k_levelset_init(const SceneDev* scenes
 // neighbours (the halo is narrower than a tile), and such a tile reached
 // L0 in the band before: the others only record that they did not reach
 // the band's end.
+template <bool OUT>
 __global__ void __launch_bounds__(kBlock) k_levelset_band(const SceneDev* scenes, int band)
 {
     const SceneDev& s = scenes[blockIdx.y];
@@ -744,8 +752,8 @@ __global__ void __launch_bounds__(kBlock) k_levelset_band(const SceneDev* scenes
         if (threadIdx.x == 0) fout[tile] = 0;
         return;
     }
-    bfs_band<16, true>(s.fsdf, nullptr, s.marker, s.fsdf, nullptr, s.marker, s.I, s.J, band * 16,
-                       &s.lsFlags[band], &fout[tile]);
+    bfs_band<16, true, OUT>(s.fsdf, nullptr, s.marker, s.fsdf, nullptr, s.marker, s.I, s.J, band * 16,
+                            &s.lsFlags[band], &fout[tile]);
 }
 
 // ---------------------------------------------------------------------------
@@ -988,8 +996,16 @@ FG_LAUNCH0(p2g)
 FG_LAUNCH0(sdf_materials)
 FG_LAUNCH0(sdf_exact)
 FG_LAUNCH1(extrapolate, int, which)
-FG_LAUNCH0(levelset_init)
-FG_LAUNCH1(levelset_band, int, band)
+void launch_levelset_init(dim3 g, hipStream_t st, const SceneDev* sc, bool outside)
+{
+    if (outside) fg_launch(k_levelset_init<true>, g, dim3(kBlock), st, sc);
+    else fg_launch(k_levelset_init<false>, g, dim3(kBlock), st, sc);
+}
+void launch_levelset_band(dim3 g, hipStream_t st, const SceneDev* sc, int band, bool outside)
+{
+    if (outside) fg_launch(k_levelset_band<true>, g, dim3(kBlock), st, sc, band);
+    else fg_launch(k_levelset_band<false>, g, dim3(kBlock), st, sc, band);
+}
 FG_LAUNCH0(forces)
 FG_LAUNCH0(rhs)
 FG_LAUNCH0(apply_solids)

@@ -91,19 +91,43 @@ __global__ void __launch_bounds__(kBlock) k_init_cells(const SceneDev* scenes, I
         mat = M_SOLID;
         sid = min_idx;
     }
-    // updateInitialFluid: float centre
-    {
+    float fsdf = 0.f;
+    if (!s.nb) {
+        // updateInitialFluid: float centre
         const float x = ((float)i + 0.5f) * dx, y = ((float)j + 0.5f) * dx;
         for (int k = geo.set[3].first; k < geo.set[3].first + geo.set[3].n; k++)
             if (polygon_sdf(poly(k), nv(k), x, y) <= 0.f) mat = M_FLUID;
+    } else {
+        // NBFlipSolver::fluidSdfFromInitialFluid (nbflipsolver.cpp:291-318):
+        // centre in double as updateSolids, the nearest fluid's distance in
+        // cells; FLUID where negative, whatever the cell was
+        const float x = (float)(((float)i + 0.5) * (double)dx), y = (float)(((float)j + 0.5) * (double)dx);
+        fsdf = 3.40282347e+38f;
+        for (int k = geo.set[3].first; k < geo.set[3].first + geo.set[3].n; k++) {
+            const float sd = polygon_sdf(poly(k), nv(k), x, y) / dx;
+            if (sd < fsdf) fsdf = sd;
+        }
+        if (fsdf < 0) mat = M_FLUID;
+        // updateGridFromSources' distance (320-349) at the cell corner
+        // (i dx, j dx); the sources never move, so it is taken once here
+        const float cx = (float)i * dx, cy = (float)j * dx;
+        float sdist = 3.40282347e+38f;
+        for (int k = geo.set[1].first; k < geo.set[1].first + geo.set[1].n; k++) {
+            const float sd = polygon_sdf(poly(k), nv(k), cx, cy) / dx;
+            if (sd < sdist) sdist = sd;
+        }
+        s.src_sdf[c] = sdist;
+        s.pcount[c] = 0;   // m_fluidParticleCounts.fill(0)
     }
     s.mat[c] = mat;
     s.sid[c] = sid;
     s.eid[c] = eid;
     s.ssdf[c] = dist;
-    s.fsdf[c] = 0.f;
+    s.fsdf[c] = fsdf;
     s.x64[c] = 0.0;
-    s.cellCount[c] = is_strict_fluid(mat) ? s.ppc : 0;   // seeding: this cell's particles
+    // seeding: this cell's particles (NB-FLIP's initialFluidSeed adds none,
+    // nbflipsolver.cpp:254-289)
+    s.cellCount[c] = (!s.nb && is_strict_fluid(mat)) ? s.ppc : 0;
 }
 
 // seedInitialFluid (flipsolver2d.cpp:682-707): particle k of a strict-fluid
@@ -164,15 +188,51 @@ __global__ void __launch_bounds__(kBlock) k_after_transfer(const SceneDev* scene
     s.vval[(size_t)i * (J + 1) + j] = 1;
 }
 
-// reseedParticles, part 1: the particles each SOURCE cell gains (the cell
-// counts of countParticles are this substep's, k_sdf_materials).
+// NBFlipSolver::reseedParticles (nbflipsolver.cpp:115-201): a SOURCE cell or
+// a cell whose level set lies inside the band (narrow, resampling) offers
+// ppc - count candidates (nb_candidates); candidate k, the counter RNG's k-th
+// position, is skipped in a SOURCE cell above the resampling band or where
+// the level set at the candidate is below the narrow band (nb_keep).
+__device__ __forceinline__ int nb_candidates(const SceneDev& s, int c)
+{
+    const float sd = s.fsdf[c];
+    const bool src = s.mat[c] == M_SOURCE;
+    if (!(src || (sd < kNbResamplingBand && sd > kNbNarrowBand))) return 0;
+    const int more = s.ppc - s.pcount[c];
+    return more > 0 ? more : 0;
+}
+__device__ __forceinline__ bool nb_keep(const SceneDev& s, int c, float x, float y)
+{
+    const bool src = s.mat[c] == M_SOURCE;
+    const float ns = lerp_grid(s.fsdf, s.I, s.J, 0.f, 0.f, x, y);
+    return !((src && s.fsdf[c] > kNbResamplingBand) || ns < kNbNarrowBand);
+}
+
+// reseedParticles, part 1: the particles each cell gains -- FlipSolver's
+// SOURCE cells, or NB-FLIP's band cells (the cell counts of countParticles
+// are this substep's, k_sdf_materials).
 __global__ void __launch_bounds__(kBlock) k_reseed_count(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
-    if (!s.active || !s.has_sources) return;
+    if (!s.active || !(s.has_sources || s.nb)) return;
     const int C = s.I * s.J;
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= C) return;
+    if (s.nb) {
+        const int more = nb_candidates(s, c);
+        int kept = 0;
+        if (more > 0) {
+            const uint32_t key = reseed_key(s.rseed, s.epoch);
+            const int i = c / s.J, j = c - (c / s.J) * s.J;
+            for (int k = 0; k < more; k++) {
+                const float x = (float)i + jitter(key, (uint32_t)c, (uint32_t)k, 0);
+                const float y = (float)j + jitter(key, (uint32_t)c, (uint32_t)k, 1);
+                kept += nb_keep(s, c, x, y) ? 1 : 0;
+            }
+        }
+        s.cellCount[c] = kept;
+        return;
+    }
     const int more = s.ppc / 2 - s.pcount[c];
     s.cellCount[c] = (s.mat[c] == M_SOURCE && more > 0) ? more : 0;
 }
@@ -184,7 +244,7 @@ __global__ void __launch_bounds__(kBlock) k_reseed_count(const SceneDev* scenes)
 __global__ void __launch_bounds__(kBlock) k_reseed_write(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
-    if (!s.active || !s.has_sources) return;
+    if (!s.active || !(s.has_sources || s.nb)) return;
     const int I = s.I, J = s.J, C = I * J;
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= C) return;
@@ -198,18 +258,22 @@ __global__ void __launch_bounds__(kBlock) k_reseed_write(const SceneDev* scenes)
     const int e = s.eid[c];
     const bool transfer = e >= 0 && e < s.n_emitters && s.emitters[e].z != 0.f;
     float m = 0.f;
-    for (int k = 0; k < more; k++) {
-        const int64_t idx = n + off + k;
-        if (idx >= s.cap) break;
+    // NB-FLIP: walk the candidates again and write the kept ones
+    const int cand = s.nb ? nb_candidates(s, c) : more;
+    for (int k = 0, a = 0; k < cand && a < more; k++) {
         const float x = (float)i + jitter(key, (uint32_t)c, (uint32_t)k, 0);
         const float y = (float)j + jitter(key, (uint32_t)c, (uint32_t)k, 1);
+        if (s.nb && !nb_keep(s, c, x, y)) continue;
+        const int64_t idx = n + off + a;
+        if (idx >= s.cap) break;
         const float2 v = transfer ? velocity_at(s.uf, s.vf, I, J, x, y) : make_float2(0.f, 0.f);
         s.pos[idx] = make_float2(x, y);
         s.vel[idx] = v;
-        s.pid[idx] = s.next_pid + (uint32_t)(off + k);
+        s.pid[idx] = s.next_pid + (uint32_t)(off + a);
         s.key[idx] = 0;
         const float v2 = v.x * v.x + v.y * v.y;
         m = v2 > m ? v2 : m;
+        a++;
     }
     if (m > 0.f) atomicMax(&s.sc->max_v2_bits, __float_as_uint(m));
 }
@@ -218,7 +282,7 @@ __global__ void __launch_bounds__(kBlock) k_reseed_write(const SceneDev* scenes)
 __global__ void k_reseed_commit(const SceneDev* scenes)
 {
     const SceneDev& s = scenes[blockIdx.y];
-    if (!s.active || !s.has_sources || threadIdx.x != 0) return;
+    if (!s.active || !(s.has_sources || s.nb) || threadIdx.x != 0) return;
     SceneScalars* sc = s.sc;
     const int64_t n = sc->n_particles, total = s.cellStart[s.I * s.J];
     const int64_t added = total < s.cap - n ? total : s.cap - n;

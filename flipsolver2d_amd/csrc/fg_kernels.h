@@ -85,8 +85,9 @@ FG_DECLARE_LAUNCH(p2g);
 FG_DECLARE_LAUNCH(sdf_materials);
 FG_DECLARE_LAUNCH(sdf_exact);
 FG_DECLARE_LAUNCH(extrapolate, int which);
-FG_DECLARE_LAUNCH(levelset_init);
-FG_DECLARE_LAUNCH(levelset_band, int band);
+// outside: extrapolateLevelsetOutside; else extrapolateLevelsetInside
+FG_DECLARE_LAUNCH(levelset_init, bool outside);
+FG_DECLARE_LAUNCH(levelset_band, int band, bool outside);
 FG_DECLARE_LAUNCH(forces);
 FG_DECLARE_LAUNCH(rhs);
 FG_DECLARE_LAUNCH(apply_solids);
@@ -114,6 +115,13 @@ FG_DECLARE_LAUNCH(after_transfer);
 FG_DECLARE_LAUNCH(reseed_count);
 FG_DECLARE_LAUNCH(reseed_write);
 FG_DECLARE_LAUNCH(reseed_commit);
+
+// fg_nbflip.hip: NBFlipSolver's grid passes (nb_euler over the (I+1) x (J+1)
+// lattice points, combine_vel over faces, the others over cells)
+FG_DECLARE_LAUNCH(nb_euler);
+FG_DECLARE_LAUNCH(nb_combine_sdf);
+FG_DECLARE_LAUNCH(nb_combine_vel);
+FG_DECLARE_LAUNCH(nb_materials);
 
 // fg_pcg.hip
 FG_DECLARE_LAUNCH(refine_residual, int spec);

@@ -60,25 +60,9 @@ __global__ void __launch_bounds__(kBlock) k_max_v2(const SceneDev* scenes)
 
 // ---------------------------------------------------------------------------
 // Advection: FlipSolver::advectThread (flipsolver2d.cpp:305-338) with
-// rk4Integrate (1195-1203) and SdfGrid::closestSurfacePoint
+// rk4Integrate (1195-1203, fg_device.h) and SdfGrid::closestSurfacePoint
 // (sdfgrid.cpp:15-46).  Fused with the cell histogram of the counting sort.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float2 rk4(const float* u, const float* v, int I, int J, float2 p, float dt)
-{
-    float2 a = velocity_at(u, v, I, J, p.x, p.y);
-    const float k1x = a.x * dt, k1y = a.y * dt;
-    a = velocity_at(u, v, I, J, p.x + k1x * 0.5f, p.y + k1y * 0.5f);
-    const float k2x = a.x * dt, k2y = a.y * dt;
-    a = velocity_at(u, v, I, J, p.x + k2x * 0.5f, p.y + k2y * 0.5f);
-    const float k3x = a.x * dt, k3y = a.y * dt;
-    a = velocity_at(u, v, I, J, p.x + k3x, p.y + k3y);
-    const float k4x = a.x * dt, k4y = a.y * dt;
-    const float sx = ((k1x + k2x * 2.f) + k3x * 2.f) + k4x;
-    const float sy = ((k1y + k2y * 2.f) + k3y * 2.f) + k4y;
-    const float sixth = 1.0f / 6.0f;
-    return make_float2(p.x + sx * sixth, p.y + sy * sixth);
-}
-
 // simmath::gradCenteredGrid with truncating (ssize_t) arguments
 __device__ __forceinline__ float2 grad_trunc(const float* g, int I, int J, float x, float y)
 {
@@ -182,6 +166,20 @@ __device__ float2 closest_surface_point(const float* sdf, int I, int J, float2 p
     return cp;
 }
 
+// NBFlipSolver::pruneNarrowBand (nbflipsolver.cpp:228-252) for a particle
+// advect left alive in cell c (the position's truncation equals its floor
+// inside the grid): below the narrow band it dies unless its cell is a
+// SOURCE; in a SOURCE cell or below the resampling band it dies when its cell
+// held more than 2*ppc particles at the last count.  The level set, the
+// materials and the counts are the previous substep's.
+__device__ __forceinline__ bool nb_pruned(const SceneDev& s, int c, float2 p)
+{
+    const float sdf = lerp_grid(s.fsdf, s.I, s.J, 0.f, 0.f, p.x, p.y);
+    const bool src = s.mat[c] == M_SOURCE;
+    if (!src && sdf < kNbNarrowBand) return true;
+    return (src || sdf < kNbResamplingBand) && s.pcount[c] > 2 * s.ppc;
+}
+
 __global__ void __launch_bounds__(kBlock) k_advect(const SceneDev* scenes, int alive_from_key)
 {
     const SceneDev& s = scenes[blockIdx.y];
@@ -204,8 +202,13 @@ __global__ void __launch_bounds__(kBlock) k_advect(const SceneDev* scenes, int a
             const bool fin = isfinite(p.x) && isfinite(p.y);
             const int pi = fin ? integr(p.x) : -1, pj = fin ? integr(p.y) : -1;
             const bool inb = pi >= 0 && pi < I && pj >= 0 && pj < J;
-            if (!inb || is_sink(s.mat[(size_t)pi * J + pj])) killed = 1;
-            else key = (uint32_t)(pi * J + pj);
+            if (!inb || is_sink(s.mat[(size_t)pi * J + pj])) {
+                killed = 1;
+            } else if (s.nb && nb_pruned(s, pi * J + pj, p)) {
+                killed = 1;
+            } else {
+                key = (uint32_t)(pi * J + pj);
+            }
         }
         s.posA[k] = p;
         s.velA[k] = vel;

@@ -18,6 +18,8 @@
  *       -> fg_scene_init_objects (fg_scene_init_geometry: solids and fluid only)
  *   FlipSolver::stepFrame()          FlipSolver2dLib/flipsolver2d.cpp:464-500 -> fg_scene_step_frame
  *   FlipSolver::step() (protected)   FlipSolver2dLib/flipsolver2d.cpp:412-462 -> fg_scene_substep
+ *   NBFlipSolver (constructor, step, firstFrameInit)  FlipSolver2dLib/nbflipsolver.cpp:9-64, 203-211
+ *       -> fg_scene_params.sim_method = FG_SIM_NBFLIP, then the same entry points
  *   FlipSolver::advect/particleToGrid/updateSdf/project/particleUpdate/...
  *       (protected virtual hooks, FlipSolver2dLib/flipsolver2d.h:280-367) -> fg_scene_stage
  *   FlipSolver accessors markerParticles/materialGrid/fluidVelocityGrid/fluidSdf/solidSdf
@@ -36,7 +38,7 @@
 extern "C" {
 #endif
 
-#define FG_ABI_VERSION 7
+#define FG_ABI_VERSION 8
 
 enum fg_status {
     FG_OK = 0,
@@ -70,6 +72,16 @@ enum fg_material {
  * tolerance; ON always applies it. */
 enum fg_density_mode { FG_DENSITY_OFF = 0, FG_DENSITY_GATED = 1, FG_DENSITY_ON = 2 };
 
+/* Which solver class steps the scene (SimulationMethod SIMULATION_LIQUID /
+ * SIMULATION_NBFLIP, flipsolver2d.h; chosen by a scene file's simType,
+ * Utils/jsonscenereader.cpp:50-62, 166-186).  FG_SIM_NBFLIP is
+ * NBFlipSolver::step (nbflipsolver.cpp:26-64): particles only in a band
+ * below the surface, the deep interior carried by semi-Lagrangian advection
+ * of the grids and the level set; its first frame seeds no particles
+ * (firstFrameInit, 203-211), its solve tolerance is the constructor's 1e-6
+ * (22) and it has no density correction (density_mode must be OFF). */
+enum fg_sim_method { FG_SIM_FLIP = 0, FG_SIM_NBFLIP = 1 };
+
 /* Stages, in FlipSolver::step order (flipsolver2d.cpp:412-462). */
 enum fg_stage {
     FG_STAGE_ADVECT = 0,          /* advect(): RK4 + solid projection + kill   */
@@ -102,13 +114,17 @@ typedef struct fg_scene_params {
     int32_t density_mode;         /* fg_density_mode */
     double  solver_tolerance;     /* FP64 relative residual ||b-Ax||2/||b||2 target */
     int64_t max_particles;        /* particle capacity */
+    int32_t sim_method;           /* fg_sim_method */
+    int32_t reserved0;            /* 0 */
 } fg_scene_params;
 
 /* Host-side view of a scene's state.  Arrays are row-major, index i*J+j for
  * cells; U is (I+1) x J, V is I x (J+1) (staggeredvelocitygrid.cpp:6-14).
  * Particle positions/velocities are in cell units (cells, cells/s).
- * Any pointer may be NULL on get (skipped).  On set, particles, u/v (+valid),
- * material, solid_sdf and solid_id are required. */
+ * Any pointer may be NULL on get (skipped).  On set, particles, u/v, material
+ * and solid_sdf are required, and for an FG_SIM_NBFLIP scene fluid_sdf too
+ * (its next substep advects the level set); u_valid/v_valid, solid_id and
+ * emitter_id default as noted. */
 typedef struct fg_scene_state {
     int64_t   n_particles;
     float    *px, *py, *vx, *vy;  /* [n_particles] */
@@ -119,7 +135,9 @@ typedef struct fg_scene_state {
     float    *solid_sdf;          /* I*J */
     float    *fluid_sdf;          /* I*J */
     int32_t  *solid_id;           /* I*J, -1 = none */
-    int32_t  *particle_counts;    /* I*J, get only */
+    int32_t  *particle_counts;    /* I*J: countParticles' per-cell counts (at most 2*ppc).  On set
+                                   * optional (NULL: zero, as firstFrameInit leaves them): an
+                                   * NB-FLIP substep's band prune reads the previous ones. */
     float    *saved_u, *saved_v;  /* get only: grid after extrapolation, before forces */
     double   *pressure;           /* I*J, get only: last pressure solve */
     double   *rhs;                /* I*J, get only: last pressure right-hand side */
@@ -337,7 +355,11 @@ typedef struct fg_scene_objects {
  * seedInitialFluid: ppc particles per strict-fluid cell, particle k jittered
  * in quadrant k % 4 by a counter-based RNG, ids in (cell, k) order.  Resets
  * the grids, the quarantine and the reseed counter.  FG_E_CAPACITY if the
- * seeding exceeds max_particles (the scene is then empty). */
+ * seeding exceeds max_particles (the scene is then empty).  An FG_SIM_NBFLIP
+ * scene runs NBFlipSolver::firstFrameInit instead (nbflipsolver.cpp:203-211,
+ * 291-349): sinks, sources, solids, then the fluid SDF of the initial fluid
+ * (the cell centre's distance over dx, FLUID where negative); no particles;
+ * the per-cell source distance its substeps take is computed here. */
 int fg_scene_init_objects(fg_scene* s, const fg_scene_objects* objects);
 /* fg_scene_init_objects with solids and initial fluid only. */
 int fg_scene_init_geometry(fg_scene* s,
@@ -364,7 +386,8 @@ int fg_scene_substep(fg_scene* s, float dt, fg_stats* stats);
 /* One FlipSolver::stepFrame(): CFL-split substeps until the frame is done. */
 int fg_scene_step_frame(fg_scene* s, fg_stats* stats);
 /* Run a single stage (fg_stage) of step() with dt; stages must be called in
- * order for a full step.  For parity tests of the reference's hooks. */
+ * order for a full step.  For parity tests of the reference's hooks.
+ * FlipSolver's stage order only: an FG_SIM_NBFLIP scene returns FG_E_INVALID. */
 int fg_scene_stage(fg_scene* s, int32_t stage, float dt);
 
 /* Step every scene one frame; all scenes advance together, one launch per

@@ -18,6 +18,7 @@ def oracle_state_dict(st):
         "pid": st.pid[:n].copy(), "u": st.u.copy(), "v": st.v.copy(), "u_valid": st.u_valid.copy(),
         "v_valid": st.v_valid.copy(), "material": st.material.copy(), "solid_sdf": st.solid_sdf.copy(),
         "fluid_sdf": st.fluid_sdf.copy(), "solid_id": st.solid_id.copy(), "emitter_id": st.emitter_id.copy(),
+        "particle_counts": st.particle_counts.copy(),
         "rng_epoch": int(st.c.rng_epoch), "next_pid": int(st.c.next_pid),
     }
 
@@ -36,7 +37,8 @@ def oracle_state_from_gpu(orc, g, I, J, capacity=None):
     st = orc.State(I, J, max(capacity or 0, n, 1))
     for k in ("px", "py", "vx", "vy", "pid"):
         getattr(st, k)[:n] = g[k]
-    for k in ("u", "v", "u_valid", "v_valid", "material", "solid_sdf", "fluid_sdf", "solid_id", "emitter_id"):
+    for k in ("u", "v", "u_valid", "v_valid", "material", "solid_sdf", "fluid_sdf", "solid_id", "emitter_id",
+              "particle_counts"):
         getattr(st, k)[...] = g[k]
     st.n = n
     st.c.rng_epoch, st.c.next_pid = int(g["rng_epoch"]), int(g["next_pid"])

@@ -120,7 +120,7 @@ def test_facade_linear_solver_vs_oracle(fg, orc, ctx, tmp_path, maxabs):
 
 def test_facade_json_scene_equals_c_abi(fg, orc, ctx, tmp_path):
     """waterfall.json: 128 x 85 cells, nine obstacles, a source and a sink
-    (an "nbflip" scene, run by the FLIP solver)."""
+    (an "nbflip" scene: the loader builds NBFlipSolver)."""
     scene = os.path.join(HERE, "golden", "scenes", "waterfall.json")
     frames = 3
     out = str(tmp_path / "wf.bin")
@@ -146,7 +146,9 @@ def test_facade_json_scene_equals_c_abi(fg, orc, ctx, tmp_path):
     p = fg.make_params(I, J, dx=info["dx"], ppc=int(info["particlesPerCell"]), density=info["fluidDensity"],
                        gravity=(float(g[0]), float(g[1])), pic=info["picRatio"], cfl=info["cflNumber"],
                        particle_scale=info["particleScale"], fps=int(info["fps"]),
-                       max_substeps=int(info["maxSubsteps"]), pcg_iter_limit=0)
+                       max_substeps=int(info["maxSubsteps"]), pcg_iter_limit=0,
+                       sim=fg.SIM_NBFLIP if int(info["simulationMethod"]) == 3 else fg.SIM_FLIP)
+    assert p.sim_method == fg.SIM_NBFLIP
     sc = fg.Scene(ctx, p)
     try:
         sc.init_objects(polys["solid"], polys["source"], polys["sink"], polys["fluid"], friction=fric, emitters=emit,
