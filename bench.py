@@ -12,7 +12,9 @@ frames, give the plain-GPU number; both sets do identical work (their
 results are bit-identical).  State stays resident in HBM.
 
 Also reported on the same line: the plain single-scene 1024^2 dam break
-(8 particles per cell), the isolated 4096^2 pressure projection, a served
+(8 particles per cell), the same scene under NBFlipSolver (narrow-band
+FLIP, with the oracle's NB substep timed on the GPU's state beside it), the
+isolated 4096^2 pressure projection, a served
 stream (Poisson arrivals, HBM-budget
 admission, continuous batching), `roofline` for the dominant kernel measured
 with HIP events attached to its launches on the stream they run on (over as
@@ -57,6 +59,8 @@ HBM_PEAK_GBS = 8000.0
 #              each stored row, per V-cycle
 #   mg_pp      k_mg_pp of the same levels: 14 B per level-l cell (act, r,
 #              dinv, y_{l+1}; y) and the 100 B of each stored row
+#   nb_euler   NB-FLIP's Euler pass: U, V and the level set read once and
+#              their advected copies written, 8 (F + C) per substep
 #   mg_coarse  per scene and V-cycle: 4 B per cell of the levels' dinv, of its
 #              input and of its output, once each the 100 B of every row above
 #              the bottom that is not its level's canonical interior stencil,
@@ -72,6 +76,7 @@ KERNEL_BYTES = {
     "mg_resid": ("K_MG_RESID", 1.0, "bytes", ["k_mg_rr"]),
     "mg_pp": ("K_MG_PP", 1.0, "bytes", ["k_mg_pp"]),
     "mg_coarse": ("K_MG_COARSE", 1.0, "bytes", ["k_mg_coarse"]),
+    "nb_euler": ("K_NB_EULER", 1.0, "bytes", ["k_nb_euler"]),
 }
 STEP_FAMILIES = ["p2g", "particles", "sdf", "pcg_a", "pcg_b", "mg_down0", "mg_up0", "mg_resid", "mg_pp",
                  "mg_coarse"]
@@ -87,6 +92,7 @@ MEASURED_TRAFFIC_PER_UNIT = {
                  "mg_down0": 1.063, "mg_up0": 0.948, "mg_resid": 1.306, "mg_pp": 1.143, "mg_coarse": 1.243},
     "dam1024": {"p2g": 1.107, "particles": 2.041, "sdf": 0.63, "pcg_a": 1.022, "pcg_b": 1.003,
                 "mg_down0": 1.029, "mg_up0": 0.939, "mg_resid": 1.21, "mg_pp": 1.113, "mg_coarse": 1.513},
+    "nb1024": {},
     "proj4096": {"pcg_a": 1.863, "pcg_b": 1.0, "mg_down0": 1.006, "mg_up0": 1.983, "mg_resid": 1.259,
                  "mg_pp": 1.222, "mg_coarse": 1.386},
 }
@@ -100,8 +106,8 @@ def parse():
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--master-seed", type=int, default=2024)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--legs", default="batched,plain,dam,proj,served,cpu",
-                    help="comma list of batched, plain, dam, proj, served, cpu (profiling runs pick one)")
+    ap.add_argument("--legs", default="batched,plain,dam,nb,proj,served,cpu",
+                    help="comma list of batched, plain, dam, nb, proj, served, cpu (profiling runs pick one)")
     ap.add_argument("--kernel-frames", type=int, default=2,
                     help="frames (solves) each kernel family is timed over, after the timed ones (0: skip)")
     ap.add_argument("--skip-plain", action="store_true", help="batched leg only (profiling runs)")
@@ -109,6 +115,7 @@ def parse():
     ap.add_argument("--tune-rounds", type=int, default=1,
                     help="autotuner rounds (each steps 4 frames; 0: no tuning, --lanes or 1 lane)")
     ap.add_argument("--dam-frames", type=int, default=10, help="timed frames of the 1024^2 leg (0: skip)")
+    ap.add_argument("--nb-frames", type=int, default=10, help="timed frames of the NB-FLIP 1024^2 leg (0: skip)")
     ap.add_argument("--proj-res", type=int, default=4096, help="isolated projection grid (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="served-stream window (0: skip)")
     ap.add_argument("--serve-max-resident", type=int, default=4096,
@@ -360,6 +367,78 @@ def dam_leg(fg, ctx, a, copy_gbs):
     }
 
 
+def nb_cpu_sample(scene, res, ppc, dt, a):
+    """One NB-FLIP substep of the oracle from the GPU's state after the timed
+    frames (same scene, same dt), its converged FP64 solve to the same
+    tolerance on `threads` threads."""
+    import oracle as orc
+
+    threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+    g = scene.get_state()
+    p = orc.make_params(res, ppc=ppc, solver=orc.SOLVER_CONVERGED, threads=threads, tol=a.tol, sim=orc.SIM_NBFLIP)
+    n = len(g["px"])
+    st = orc.State(res, res, 2 * res * res * ppc)
+    for k in ("px", "py", "vx", "vy", "pid"):
+        getattr(st, k)[:n] = g[k]
+    for k in ("u", "v", "u_valid", "v_valid", "material", "solid_sdf", "fluid_sdf", "solid_id", "emitter_id",
+              "particle_counts"):
+        getattr(st, k)[...] = g[k]
+    st.n = n
+    st.c.rng_epoch, st.c.next_pid = int(g["rng_epoch"]), int(g["next_pid"])
+    t0 = time.perf_counter()
+    orc.lib().orc_nb_substep(C.byref(p), st.ref(), C.c_float(dt), C.byref(orc.OrcStats()))
+    el = time.perf_counter() - t0
+    return {"value": n / el, "unit": "particle-substeps/s", "ms_per_substep": 1e3 * el, "cores": threads,
+            "kind": "port", "sample": f"one NB-FLIP substep of the {res}x{res} scene ({n} particles) from the "
+                                      "GPU's state after the timed frames, dt of the last GPU substep"}
+
+
+def nb_leg(fg, ctx, a, copy_gbs):
+    """NBFlipSolver (nbflipsolver.cpp) on the 1024^2 dam break, 8 ppc: the
+    first frame seeds no particle, later frames keep particles in the band
+    below the surface (the timed frames follow --warmup frames)."""
+    res, ppc = 1024, 8
+    ctx.work_read(reset=True)
+    scene = fg.nb_dam_break_scene(ctx, res, ppc=ppc, tol=a.tol, pcg_iter_limit=4000)
+    for _ in range(a.warmup):
+        scene.step_frame()
+    lat, psub, subs, resid, conv, last_dt = [], 0, 0, 0.0, True, 0.0
+    t0 = time.perf_counter()
+    for _ in range(a.nb_frames):
+        s = scene.step_frame()
+        lat.append(s.frame_ms)
+        psub += s.particle_substeps
+        subs += s.substeps
+        resid = max(resid, s.pressure_rel_resid)
+        conv = conv and bool(s.converged)
+        last_dt = s.last_dt
+    el = time.perf_counter() - t0
+    n_end = scene.particle_count()
+    cpu = nb_cpu_sample(scene, res, ppc, last_dt, a) if "cpu" in a.legs and a.cpu_seconds > 0 else None
+    table, dom = kernel_table(fg, ctx, STEP_FAMILIES + ["nb_euler"], scene.step_frame, a.kernel_frames, "nb1024",
+                              copy_gbs, "frames")
+    scene.close()
+    log(f"NB-FLIP dam break {res}^2: {a.nb_frames} frames, {subs} substeps, {el:.2f} s")
+    return {
+        "workload": f"plain single-scene NB-FLIP dam break {res}x{res}, {ppc} ppc ({n_end} particles after the "
+                    f"timed frames: the band below the surface)",
+        "value": psub / el,
+        "cell_substeps_per_s": res * res * subs / el,
+        "ms_per_step": 1e3 * el / a.nb_frames,
+        "ms_per_substep": 1e3 * el / max(subs, 1),
+        "frames": a.nb_frames,
+        "substeps": subs,
+        "p50_frame_ms": pctl(lat, 0.5),
+        "p99_frame_ms": pctl(lat, 0.99),
+        "max_rel_residual": resid,
+        "all_converged": conv,
+        "roofline": table.get(dom),
+        "kernels": table,
+        "algorithmic_bytes_total": work_totals(fg, ctx),
+        "cpu_sample": cpu,
+    }
+
+
 def projection_leg(fg, ctx, a, copy_gbs):
     n = a.proj_res
     mat, _, _, b, scale = fg.isolated_projection_system(n, 1)
@@ -519,12 +598,13 @@ def main():
     a.legs = set(a.legs.split(","))
     if a.skip_plain:
         a.legs.discard("plain")
-    b = dam = proj = served = None
-    if a.legs & {"batched", "plain", "dam", "proj", "served"}:
+    b = dam = nb = proj = served = None
+    if a.legs & {"batched", "plain", "dam", "nb", "proj", "served"}:
         ctx = fg.Context(0)
         copy_gbs, _ = ctx.bandwidth_probe(1 << 30, 5)
         b = batched_leg(fg, ctx, a, copy_gbs) if "batched" in a.legs else None
         dam = dam_leg(fg, ctx, a, copy_gbs) if a.dam_frames > 0 and "dam" in a.legs else None
+        nb = nb_leg(fg, ctx, a, copy_gbs) if a.nb_frames > 0 and "nb" in a.legs else None
         proj = projection_leg(fg, ctx, a, copy_gbs) if a.proj_res > 0 and "proj" in a.legs else None
         served = served_leg(fg, ctx, a) if a.serve_seconds > 0 and "served" in a.legs else None
         ctx.close()
@@ -532,8 +612,8 @@ def main():
     if cpu:
         log(f"cpu baseline: {cpu['value']:.3e} particle-substeps/s")
     if b is None:   # a profiling run of other legs: their results alone
-        print(json.dumps({"dam_break_1024": dam, "isolated_projection": proj, "served": served,
-                          "cpu_baseline": cpu}), flush=True)
+        print(json.dumps({"dam_break_1024": dam, "nbflip_dam_break_1024": nb, "isolated_projection": proj,
+                          "served": served, "cpu_baseline": cpu}), flush=True)
         return
 
     dom = b.pop("dominant")
@@ -572,6 +652,7 @@ def main():
         "algorithmic_bytes_total": b["algorithmic_bytes_total"],
         "cpu_baseline": cpu,
         "dam_break_1024": dam,
+        "nbflip_dam_break_1024": nb,
         "isolated_projection": proj,
         "served": served,
     }

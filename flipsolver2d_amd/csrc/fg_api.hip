@@ -1649,7 +1649,13 @@ int run_nb_substep(Group& G, Timer& tm)
         launch_advect(g.parts, st, d, 1);
         kp.done(G.parts);
     }
-    launch_nb_euler(g.nodes, st, d);
+    {
+        KProf kp(G, FG_K_NB_EULER, true);
+        launch_nb_euler(g.nodes, st, d);
+        int64_t bytes = 0;
+        for (fg_scene* s : G.sc) bytes += 8 * (s->FU + s->FV + s->C);
+        kp.done(bytes);
+    }
     tm.lap(FG_STAGE_ADVECT);
     {   // pruneParticles + rebin
         KProf kp(G, FG_K_SORT);

@@ -78,8 +78,9 @@ enum fg_density_mode { FG_DENSITY_OFF = 0, FG_DENSITY_GATED = 1, FG_DENSITY_ON =
  * NBFlipSolver::step (nbflipsolver.cpp:26-64): particles only in a band
  * below the surface, the deep interior carried by semi-Lagrangian advection
  * of the grids and the level set; its first frame seeds no particles
- * (firstFrameInit, 203-211), its solve tolerance is the constructor's 1e-6
- * (22) and it has no density correction (density_mode must be OFF). */
+ * (firstFrameInit, 203-211) and it has no density correction (density_mode
+ * must be OFF).  Its constructor sets the solve tolerance to 1e-6 (22):
+ * pass that as solver_tolerance to follow it. */
 enum fg_sim_method { FG_SIM_FLIP = 0, FG_SIM_NBFLIP = 1 };
 
 /* Stages, in FlipSolver::step order (flipsolver2d.cpp:412-462). */
@@ -309,7 +310,9 @@ enum fg_kernel_id {
     FG_K_PARTICLES = 16,  /* [B] k_advect and k_g2p_count together (G2P + advection) */
     FG_K_MG_PP = 17,      /* [B] the V-cycle's prolongation + post-smoothing launches, k_mg_pp
                            * of the same levels (not timed with the per-level kernels) */
-    FG_K_NUM = 18
+    FG_K_NB_EULER = 18,   /* [B] NB-FLIP's Euler pass (k_nb_euler): U, V and the level set
+                           * read once and their advected copies written, 8 (F_U + F_V + C) */
+    FG_K_NUM = 19
 };
 int fg_ctx_profile_kernel(fg_ctx* ctx, int32_t kernel);
 /* Work units (as above) of every kernel family, summed over all launches
