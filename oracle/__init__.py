@@ -43,6 +43,7 @@ class OrcParams(C.Structure):
         ("n_emitters", C.c_int32), ("emitter_velocity", C.c_void_p), ("emitter_transfer", C.c_void_p),
         ("reseed_seed", C.c_uint32),
         ("sim_method", C.c_int32), ("source_sdf", C.c_void_p),
+        ("viscosity_enabled", C.c_int32), ("emitter_viscosity", C.c_void_p), ("fluid_viscosity", C.c_void_p),
     ]
 
 
@@ -58,6 +59,7 @@ class OrcState(C.Structure):
         ("saved_u", C.c_void_p), ("saved_v", C.c_void_p),
         ("pressure", C.c_void_p), ("rhs", C.c_void_p), ("density", C.c_void_p),
         ("emitter_id", C.c_void_p), ("rng_epoch", C.c_uint32), ("next_pid", C.c_uint32),
+        ("pvisc", C.c_void_p), ("viscosity", C.c_void_p),
     ]
 
 
@@ -131,6 +133,9 @@ def _declare_oracle(L):
         "orc_euler_advect": (None, [_f32p, _f32p, C.c_int32, C.c_int32, C.c_float, _f32p, C.c_int32, C.c_int32,
                                     C.c_float, C.c_float, _f32p]),
         "orc_nb_prune": (None, [P, S, _u8p]),
+        "orc_viscosity_grid": (None, [P, S, _i32p]),
+        "orc_viscosity_matrix": (None, [_i8p, _f32p, C.c_int32, C.c_int32, C.c_float, _f64p, _f64p, _f64p]),
+        "orc_viscosity_apply": (C.c_int32, [P, S, C.c_float, C.c_double]),
         "orc_nb_combine": (None, [P, S, _f32p, _f32p, _f32p]),
         "orc_nb_reseed": (C.c_int64, [P, S]),
         "orc_nb_substep": (C.c_int32, [P, S, C.c_float, T]),
@@ -214,9 +219,11 @@ def _declare_ref(L):
 def make_params(n_i, n_j=None, *, dx=None, ppc=4, density=1000.0, gravity=(9.81, 0.0), pic=0.03, cfl=5.0,
                 particle_scale=0.8, fps=30, max_substeps=10, pcg_iter_limit=200,
                 solver=SOLVER_CONVERGED, density_mode=DENSITY_OFF, threads=1, tol=None, friction=(0.0,),
-                emitters=(), reseed_seed=0, sim=SIM_FLIP):
-    """emitters: (vx, vy, transfer) per source, velocity in m/s; sim: SIM_FLIP
-    (FlipSolver) or SIM_NBFLIP (NBFlipSolver)."""
+                emitters=(), reseed_seed=0, sim=SIM_FLIP, viscosity=False, fluid_viscosity=()):
+    """emitters: (vx, vy, transfer[, viscosity]) per source, velocity in m/s;
+    sim: SIM_FLIP (FlipSolver) or SIM_NBFLIP (NBFlipSolver); viscosity:
+    viscosityEnabled (the light model); fluid_viscosity: per initial-fluid
+    polygon."""
     n_j = n_i if n_j is None else n_j
     p = OrcParams()
     p.size_i, p.size_j = n_i, n_j
@@ -244,6 +251,12 @@ def make_params(n_i, n_j=None, *, dx=None, ppc=4, density=1000.0, gravity=(9.81,
     p.reseed_seed = reseed_seed
     p.sim_method = sim
     p.source_sdf = None
+    p.viscosity_enabled = 1 if viscosity else 0
+    evis = np.ascontiguousarray([e[3] if len(e) > 3 else 0.0 for e in emitters] or [0.0], np.float32)
+    fvis = np.ascontiguousarray(list(fluid_viscosity) or [0.0], np.float32)
+    p._visc_keep = (evis, fvis)
+    p.emitter_viscosity = evis.ctypes.data
+    p.fluid_viscosity = fvis.ctypes.data
     return p
 
 
@@ -273,12 +286,14 @@ class State:
         self.rhs = np.zeros(I * J, np.float64)
         self.density = np.zeros(I * J, np.float32)
         self.emitter_id = np.full(I * J, -1, np.int32)
+        self.pvisc = np.zeros(capacity, np.float32)
+        self.viscosity = np.zeros(I * J, np.float32)
         self.c = OrcState()
         self.c.capacity = capacity
         self.c.n_particles = 0
         for name in ("px", "py", "vx", "vy", "pid", "u", "v", "u_valid", "v_valid", "material", "solid_sdf",
                      "fluid_sdf", "solid_id", "particle_counts", "saved_u", "saved_v", "pressure", "rhs",
-                     "density", "emitter_id"):
+                     "density", "emitter_id", "pvisc", "viscosity"):
             setattr(self.c, name, getattr(self, name).ctypes.data)
 
     @property
@@ -296,7 +311,7 @@ class State:
         o = State(self.I, self.J, len(self.px))
         for name in ("px", "py", "vx", "vy", "pid", "u", "v", "u_valid", "v_valid", "material", "solid_sdf",
                      "fluid_sdf", "solid_id", "particle_counts", "saved_u", "saved_v", "pressure", "rhs",
-                     "density", "emitter_id"):
+                     "density", "emitter_id", "pvisc", "viscosity"):
             getattr(o, name)[...] = getattr(self, name)
         o.n = self.n
         o.c.rng_epoch, o.c.next_pid = self.c.rng_epoch, self.c.next_pid

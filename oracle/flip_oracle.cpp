@@ -457,15 +457,17 @@ void orc_sort_particles(const OrcParams* p, OrcState* s, const uint8_t* alive, i
         return a < b; /* a caller may repeat an id: input order breaks the tie */
     });
     const int64_t n = (int64_t)idx.size();
-    std::vector<float> px(n), py(n), vx(n), vy(n);
+    std::vector<float> px(n), py(n), vx(n), vy(n), pv(s->pvisc ? n : 0);
     std::vector<uint32_t> id(n);
     for (int64_t k = 0; k < n; k++) {
         const int64_t q = idx[k];
         px[k] = s->px[q]; py[k] = s->py[q]; vx[k] = s->vx[q]; vy[k] = s->vy[q]; id[k] = s->pid[q];
+        if (s->pvisc) pv[k] = s->pvisc[q];
     }
     std::memcpy(s->px, px.data(), n * 4); std::memcpy(s->py, py.data(), n * 4);
     std::memcpy(s->vx, vx.data(), n * 4); std::memcpy(s->vy, vy.data(), n * 4);
     std::memcpy(s->pid, id.data(), n * 4);
+    if (s->pvisc) std::memcpy(s->pvisc, pv.data(), n * 4);
     s->n_particles = n;
     if (cell_start) {
         std::fill(cell_start, cell_start + C + 1, 0);
@@ -516,6 +518,7 @@ int32_t orc_count_particles(const OrcParams* p, OrcState* s, const int32_t* cs)
             if (!keep[k]) continue;
             s->px[w] = s->px[k]; s->py[w] = s->py[k]; s->vx[w] = s->vx[k]; s->vy[w] = s->vy[k];
             s->pid[w] = s->pid[k];
+            if (s->pvisc) s->pvisc[w] = s->pvisc[k];
             w++;
         }
         s->n_particles = w;
@@ -1349,7 +1352,7 @@ int32_t orc_substep(const OrcParams* p, OrcState* s, float dt, OrcStats* st)
     orc_sort_particles(p, s, alive.data(), cs.data());
     stage(3);
     /* 4. density correction (DENSITY) */
-    if (p->density_mode != ORC_DENSITY_OFF) {
+    if (p->density_mode != ORC_DENSITY_OFF && !p->viscosity_enabled) {   /* flipsolver2d.cpp:425 */
         std::vector<float> dens_local;
         float* dens = s->density;
         if (!dens) {
@@ -1377,8 +1380,10 @@ int32_t orc_substep(const OrcParams* p, OrcState* s, float dt, OrcStats* st)
         }
     }
     stage(2);
-    /* 5. gridUpdate: P2G, SDF, materials (PARTICLE_TO_GRID, GRID_UPDATE) */
+    /* 5. gridUpdate: P2G (+ centeredParamsToGrid's viscosity), SDF, materials
+     * (PARTICLE_TO_GRID, GRID_UPDATE) */
     orc_p2g(p, s, cs.data());
+    if (p->viscosity_enabled && s->pvisc && s->viscosity) orc_viscosity_grid(p, s, cs.data());
     stage(4);
     orc_fluid_sdf(p, s, cs.data());
     orc_update_materials(p, s);
@@ -1411,8 +1416,27 @@ int32_t orc_substep(const OrcParams* p, OrcState* s, float dt, OrcStats* st)
         orc_apply_pressure(p, s, dt, pr);
     }
     stage(7);
-    /* 9. solids, extrapolate, G2P (PARTICLE_UPDATE) */
+    /* 9. solids; with viscosity the light model and a second projection with
+     * the same matrix (VISCOSITY, REPRESSURE: flipsolver2d.cpp:446-453);
+     * extrapolate, G2P (PARTICLE_UPDATE) */
     orc_velocity_from_solids(p, s);
+    if (p->viscosity_enabled && s->viscosity) {
+        orc_viscosity_apply(p, s, dt, 1e-12);
+        std::vector<double> rhs_local, pr_local;
+        double* rhs = s->rhs;
+        double* pr = s->pressure;
+        if (!rhs) { rhs_local.resize(C); rhs = rhs_local.data(); }
+        if (!pr) { pr_local.resize(C); pr = pr_local.data(); }
+        orc_pressure_rhs(p, s, rhs);
+        double rel = 0;
+        bool conv = false;
+        const int32_t it = solve(p, mat_a.data(), sA, rhs, pr, &rel, &conv);
+        if (st) {
+            st->pressure_iters = std::max(st->pressure_iters, it);
+            st->pressure_rel_resid = rel;
+        }
+        orc_apply_pressure(p, s, dt, pr);
+    }
     orc_extrapolate(s->u, s->u_valid, I + 1, J, 10);
     orc_extrapolate(s->v, s->v_valid, I, J + 1, 10);
     orc_g2p(p, s);
@@ -1588,14 +1612,17 @@ void orc_init_objects(const OrcParams* p, OrcState* s, int32_t n_solids, const i
                 s->solid_id[i * J + j] = min_idx;
             }
         }
-    /* updateInitialFluid */
+    /* updateInitialFluid: also the viscosity grid of the fluid cells */
+    if (s->viscosity) std::fill(s->viscosity, s->viscosity + C, 0.f);
     for (long i = 0; i < I; i++)
         for (long j = 0; j < J; j++) {
             const float* v = fv;
             for (int32_t k = 0; k < n_fluids; k++) {
                 if (orc_polygon_sdf(v, fnv[k], (static_cast<float>(i) + 0.5f) * dx, (static_cast<float>(j) + 0.5f) * dx) <=
-                    0.f)
+                    0.f) {
                     s->material[i * J + j] = (int8_t)ORC_FLUID;
+                    if (s->viscosity) s->viscosity[i * J + j] = p->fluid_viscosity ? p->fluid_viscosity[k] : 0.f;
+                }
                 v += 2 * fnv[k];
             }
         }
@@ -1661,6 +1688,7 @@ int64_t orc_reseed(const OrcParams* p, OrcState* s)
                 s->vx[n] = vx;
                 s->vy[n] = vy;
                 s->pid[n] = s->next_pid++;
+                if (s->pvisc) s->pvisc[n] = p->emitter_viscosity ? p->emitter_viscosity[e] : 0.f;
                 added++;
             }
         }
@@ -1686,6 +1714,9 @@ int64_t orc_seed_particles(const OrcParams* p, OrcState* s, uint32_t seed)
                 s->vx[n] = 0.f;
                 s->vy[n] = 0.f;
                 s->pid[n] = (uint32_t)n;
+                /* m_viscosityGrid.interpolateAt(pos): offset (0.5, 0.5) (flipsolver2d.cpp:37, 697-702) */
+                if (s->pvisc)
+                    s->pvisc[n] = s->viscosity ? lerp_grid(s->viscosity, I, J, 0.5f, 0.5f, s->px[n], s->py[n]) : 0.f;
                 n++;
             }
         }
@@ -1693,6 +1724,142 @@ int64_t orc_seed_particles(const OrcParams* p, OrcState* s, uint32_t seed)
     return n;
 }
 
+
+/* ================================================================== */
+/* Viscosity: the light model (viscositymodel.cpp:4-140)               */
+/* ================================================================== */
+void orc_viscosity_grid(const OrcParams* p, OrcState* s, const int32_t* cs)
+{
+    const long I = p->size_i, J = p->size_j;
+    for (long i = 0; i < I; i++)
+        for (long j = 0; j < J; j++) {
+            /* the quadratic B-spline at (x - i, y - j) reaches cells i-2 .. i+1
+             * (and j-2 .. j+1); their particles in (cell, id) order */
+            float wt = 1e-10f, vv = 0.0f;
+            const long j0 = std::max<long>(0, j - 2), j1 = std::min<long>(J - 1, j + 1);
+            for (long a = std::max<long>(0, i - 2); a <= std::min<long>(I - 1, i + 1); a++)
+                for (int32_t k = cs[a * J + j0]; k < cs[a * J + j1 + 1]; k++) {
+                    const float w = qbspline(s->px[k] - static_cast<float>(i), s->py[k] - static_cast<float>(j));
+                    if (w > 1e-9f) {
+                        wt += w;
+                        vv += w * s->pvisc[k];
+                    }
+                }
+            float v = vv / wt;
+            /* afterTransfer: a SOURCE cell takes its emitter's viscosity */
+            const long c = i * J + j;
+            if (is_source(s->material[c]) && s->emitter_id) {
+                const int e = s->emitter_id[c];
+                if (e >= 0 && e < p->n_emitters) v = p->emitter_viscosity ? p->emitter_viscosity[e] : 0.f;
+            }
+            s->viscosity[c] = v;
+        }
+}
+
+void orc_viscosity_matrix(const int8_t* m, const float* visc, int32_t I, int32_t J, float dt, double* diag,
+                          double* east, double* north)
+{
+    const double scale = dt;
+    for (long i = 0; i < I; i++)
+        for (long j = 0; j < J; j++) {
+            const long c = i * J + j;
+            const bool sc = is_solid(m[c]);
+            diag[c] = sc ? 1.0 : 4.0 * (visc[c] * scale) + 1.0;
+            /* a pair's entry is written by each non-solid end; the later one
+             * in the (i, j) loop wins: the far end's value if it is not
+             * solid, else the near end's */
+            east[c] = 0.0;
+            if (i + 1 < I) {
+                const long b = c + J;
+                const bool sb = is_solid(m[b]);
+                if (!(sc && sb)) east[c] = (sc ? visc[b] : visc[c]) * scale;
+            }
+            north[c] = 0.0;
+            if (j + 1 < J) {
+                const long b = c + 1;
+                const bool sb = is_solid(m[b]);
+                if (!(sc && sb)) north[c] = (sc ? visc[b] : visc[c]) * scale;
+            }
+        }
+}
+
+static void visc_spmv(const double* d, const double* e, const double* nn, long I, long J, const double* x, double* y)
+{
+    for (long i = 0; i < I; i++)
+        for (long j = 0; j < J; j++) {
+            const long c = i * J + j;
+            double v = d[c] * x[c];
+            if (i + 1 < I) v += e[c] * x[c + J];
+            if (i > 0) v += e[c - J] * x[c - J];
+            if (j + 1 < J) v += nn[c] * x[c + 1];
+            if (j > 0) v += nn[c - 1] * x[c - 1];
+            y[c] = v;
+        }
+}
+
+/* Eigen::ConjugateGradient with its DiagonalPreconditioner, from x = 0,
+ * stopping when ||r||^2 < tol^2 ||b||^2 */
+static int32_t visc_cg(const double* d, const double* e, const double* nn, long I, long J, const double* b, double* x,
+                       double tol)
+{
+    const long C = I * J;
+    std::vector<double> r(b, b + C), z(C), pp(C), q(C);
+    std::fill(x, x + C, 0.0);
+    double bb = 0;
+    for (long c = 0; c < C; c++) bb += b[c] * b[c];
+    if (bb == 0.0) return 0;
+    const double thr = tol * tol * bb;
+    double rr = bb;
+    if (rr < thr) return 0;
+    double rz = 0;
+    for (long c = 0; c < C; c++) {
+        pp[c] = r[c] / d[c];
+        rz += r[c] * pp[c];
+    }
+    int32_t it = 0;
+    while (it < 100000) {
+        visc_spmv(d, e, nn, I, J, pp.data(), q.data());
+        double pq = 0;
+        for (long c = 0; c < C; c++) pq += pp[c] * q[c];
+        const double alpha = rz / pq;
+        rr = 0;
+        for (long c = 0; c < C; c++) {
+            x[c] += alpha * pp[c];
+            r[c] -= alpha * q[c];
+            rr += r[c] * r[c];
+        }
+        it++;
+        if (rr < thr) break;
+        double rzn = 0;
+        for (long c = 0; c < C; c++) {
+            z[c] = r[c] / d[c];
+            rzn += r[c] * z[c];
+        }
+        const double beta = rzn / rz;
+        rz = rzn;
+        for (long c = 0; c < C; c++) pp[c] = z[c] + beta * pp[c];
+    }
+    return it;
+}
+
+int32_t orc_viscosity_apply(const OrcParams* p, OrcState* s, float dt, double rel_tol)
+{
+    const long I = p->size_i, J = p->size_j, C = I * J;
+    std::vector<double> d(C), e(C), nn(C), b(C), x(C);
+    orc_viscosity_matrix(s->material, s->viscosity, I, J, dt, d.data(), e.data(), nn.data());
+    const float rho = static_cast<float>(p->fluid_density);
+    int32_t most = 0;
+    for (int which = 0; which < 2; which++) {
+        float* g = which == 0 ? s->u : s->v;
+        const long sj = which == 0 ? J : J + 1;
+        for (long i = 0; i < I; i++)
+            for (long j = 0; j < J; j++) b[i * J + j] = rho * g[i * sj + j];   /* fillRhs */
+        most = std::max(most, visc_cg(d.data(), e.data(), nn.data(), I, J, b.data(), x.data(), rel_tol));
+        for (long i = 0; i < I; i++)
+            for (long j = 0; j < J; j++) g[i * sj + j] = static_cast<float>(x[i * J + j] / rho);   /* applyResult */
+    }
+    return most;
+}
 
 /* ================================================================== */
 /* NB-FLIP: NBFlipSolver (nbflipsolver.cpp)                             */

@@ -68,6 +68,9 @@ typedef struct OrcParams {
     int32_t sim_method;            /* ORC_SIM_FLIP (FlipSolver) or ORC_SIM_NBFLIP (NBFlipSolver) */
     const float* source_sdf;       /* NB-FLIP: I*J distance (cells) of the nearest source at the cell
                                     * corner, FLT_MAX without sources (orc_nb_init_objects); may be NULL */
+    int32_t viscosity_enabled;     /* viscosityEnabled: the light model (viscositymodel.cpp:4-140) */
+    const float* emitter_viscosity; /* [n_emitters] Emitter::viscosity, or NULL (0) */
+    const float* fluid_viscosity;  /* [initial fluids] Emitter::viscosity, or NULL (0) */
 } OrcParams;
 
 typedef struct OrcState {
@@ -88,6 +91,8 @@ typedef struct OrcState {
     int32_t  *emitter_id;          /* I*J, -1 = none: the source of a SOURCE cell (may be NULL) */
     uint32_t  rng_epoch;           /* substeps taken: the reseed RNG's counter */
     uint32_t  next_pid;            /* id of the next reseeded particle */
+    float    *pvisc;               /* [capacity] the particles' viscosity property, or NULL */
+    float    *viscosity;           /* I*J the viscosity grid (centeredParamsToGrid), or NULL */
 } OrcState;
 
 typedef struct OrcStats {
@@ -203,6 +208,20 @@ float orc_reseed_jitter(uint32_t seed, uint32_t epoch, uint32_t cell, uint32_t k
 /* Quadrant-jittered seeding with the counter RNG shared with the product's init. */
 int64_t orc_seed_particles(const OrcParams* p, OrcState* s, uint32_t seed);
 float   orc_jitter(uint32_t seed, uint32_t cell, uint32_t k, uint32_t axis);
+
+/* ---- viscosity, the light model (viscositymodel.cpp:4-140) ---- */
+/* FlipSolver::centeredParamsToGrid's viscosity (flipsolver2d.cpp:1379-1431)
+ * and afterTransfer's SOURCE cells (396-397): into s->viscosity. */
+void  orc_viscosity_grid(const OrcParams* p, OrcState* s, const int32_t* cell_start);
+/* LightViscosityModel::getMatrix (viscositymodel.cpp:63-134): per cell the
+ * diagonal, and the symmetric couplings to (i+1, j) (east) and (i, j+1)
+ * (north) as the assembly's last write leaves them. */
+void  orc_viscosity_matrix(const int8_t* material, const float* viscosity, int32_t I, int32_t J, float dt,
+                           double* diag, double* east, double* north);
+/* LightViscosityModel::apply (4-61): U then V solved with the same matrix
+ * (Jacobi-preconditioned CG in FP64 to relative residual rel_tol, x0 = 0);
+ * returns the larger iteration count. */
+int32_t orc_viscosity_apply(const OrcParams* p, OrcState* s, float dt, double rel_tol);
 
 /* ---- NB-FLIP (NBFlipSolver, nbflipsolver.cpp) ---- */
 /* FlipSolver::eulerAdvectionThread (flipsolver2d.cpp:340-351): out[i,j] =
