@@ -62,7 +62,7 @@ class SceneParams(C.Structure):
         ("fps", C.c_int32), ("max_substeps", C.c_int32), ("particles_per_cell", C.c_int32),
         ("pcg_iter_limit", C.c_int32), ("density_mode", C.c_int32),
         ("solver_tolerance", C.c_double), ("max_particles", C.c_int64),
-        ("sim_method", C.c_int32), ("reserved0", C.c_int32),
+        ("sim_method", C.c_int32), ("viscosity_enabled", C.c_int32),
     ]
 
 
@@ -76,6 +76,7 @@ class SceneState(C.Structure):
         ("saved_u", C.c_void_p), ("saved_v", C.c_void_p), ("pressure", C.c_void_p), ("rhs", C.c_void_p),
         ("emitter_id", C.c_void_p), ("rng_epoch", C.c_uint32), ("next_pid", C.c_uint32),
         ("stencil", C.c_void_p), ("work_u", C.c_void_p), ("work_v", C.c_void_p),
+        ("particle_viscosity", C.c_void_p), ("viscosity", C.c_void_p),
     ]
 
 
@@ -88,7 +89,7 @@ class Stats(C.Structure):
         ("particles_killed", C.c_int64), ("particles_erased", C.c_int64),
         ("last_dt", C.c_float), ("max_velocity", C.c_float),
         ("stage_ms", C.c_double * 12), ("frame_ms", C.c_double),
-        ("status", C.c_int32), ("reserved", C.c_int32), ("particles_reseeded", C.c_int64),
+        ("status", C.c_int32), ("viscosity_iters", C.c_int32), ("particles_reseeded", C.c_int64),
     ]
 
     def as_dict(self):
@@ -104,7 +105,8 @@ class Polygons(C.Structure):
 class SceneObjects(C.Structure):
     _fields_ = [("solids", Polygons), ("friction", C.c_void_p), ("sources", Polygons),
                 ("source_velocity", C.c_void_p), ("source_transfer", C.c_void_p), ("sinks", Polygons),
-                ("fluids", Polygons), ("seed", C.c_uint32)]
+                ("fluids", Polygons), ("seed", C.c_uint32), ("source_viscosity", C.c_void_p),
+                ("fluid_viscosity", C.c_void_p)]
 
 
 class Request(C.Structure):
@@ -224,10 +226,11 @@ def _ptr(a):
 
 def make_params(n_i, n_j=None, *, dx=None, ppc=4, density=1000.0, gravity=(9.81, 0.0), pic=0.03, cfl=5.0,
                 particle_scale=0.8, fps=30, max_substeps=10, pcg_iter_limit=2000, density_mode=DENSITY_OFF,
-                tol=1e-6, max_particles=None, sim=SIM_FLIP):
+                tol=1e-6, max_particles=None, sim=SIM_FLIP, viscosity=False):
     """FlipSolverParameters as fg_scene_params; sim: SIM_FLIP (FlipSolver) or
     SIM_NBFLIP (NBFlipSolver, whose band cells fill to ppc and crowded cells
-    keep up to 2 ppc: its default capacity is twice FLIP's)."""
+    keep up to 2 ppc: its default capacity is twice FLIP's); viscosity:
+    viscosityEnabled (the light model, FlipSolver only)."""
     n_j = n_i if n_j is None else n_j
     p = SceneParams()
     p.size_i, p.size_j = n_i, n_j
@@ -239,6 +242,7 @@ def make_params(n_i, n_j=None, *, dx=None, ppc=4, density=1000.0, gravity=(9.81,
     p.pcg_iter_limit, p.density_mode, p.solver_tolerance = pcg_iter_limit, density_mode, tol
     p.max_particles = n_i * n_j * ppc * (2 if sim == SIM_NBFLIP else 1) if max_particles is None else max_particles
     p.sim_method = sim
+    p.viscosity_enabled = 1 if viscosity else 0
     return p
 
 
@@ -351,9 +355,11 @@ class Scene:
         _check(lib().fg_scene_init_geometry(self.h, len(solids), snv.ctypes.data, sv.ctypes.data, _ptr(fr),
                                             len(fluids), fnv.ctypes.data, fv.ctypes.data, seed))
 
-    def init_objects(self, solids=(), sources=(), sinks=(), fluids=(), friction=None, emitters=(), seed=42):
+    def init_objects(self, solids=(), sources=(), sinks=(), fluids=(), friction=None, emitters=(), seed=42,
+                     fluid_viscosity=()):
         """First-frame init on the device (fg_scene_init_objects); emitters:
-        (vx, vy, transfer) per source polygon, velocity in m/s."""
+        (vx, vy, transfer[, viscosity]) per source polygon, velocity in m/s;
+        fluid_viscosity: per initial-fluid polygon."""
         keep = []
 
         def polys(ps):
@@ -371,8 +377,14 @@ class Scene:
         if len(emitters):
             ev = np.ascontiguousarray([[e[0], e[1]] for e in emitters], np.float32).ravel()
             et = np.ascontiguousarray([1 if e[2] else 0 for e in emitters], np.uint8)
-            keep.extend([ev, et])
+            evis = np.ascontiguousarray([e[3] if len(e) > 3 else 0.0 for e in emitters], np.float32)
+            keep.extend([ev, et, evis])
             o.source_velocity, o.source_transfer = ev.ctypes.data, et.ctypes.data
+            o.source_viscosity = evis.ctypes.data
+        if len(fluid_viscosity):
+            fvis = np.ascontiguousarray(fluid_viscosity, np.float32)
+            keep.append(fvis)
+            o.fluid_viscosity = fvis.ctypes.data
         o.seed = seed
         _check(lib().fg_scene_init_objects(self.h, C.byref(o)))
 
@@ -398,7 +410,8 @@ class Scene:
                 dt = {"px": np.float32, "py": np.float32, "vx": np.float32, "vy": np.float32, "pid": np.uint32,
                       "u": np.float32, "v": np.float32, "u_valid": np.uint8, "v_valid": np.uint8,
                       "material": np.int8, "solid_sdf": np.float32, "fluid_sdf": np.float32,
-                      "solid_id": np.int32, "emitter_id": np.int32, "particle_counts": np.int32}.get(name)
+                      "solid_id": np.int32, "emitter_id": np.int32, "particle_counts": np.int32,
+                      "particle_viscosity": np.float32}.get(name)
                 if dt is None:
                     continue
                 a = np.ascontiguousarray(st[name], dt)
@@ -422,13 +435,14 @@ class Scene:
             "rhs": np.zeros(I * J, np.float64), "stencil": np.zeros(I * J, np.uint8),
             "emitter_id": np.zeros(I * J, np.int32),
             "work_u": np.zeros((I + 1) * J, np.float32), "work_v": np.zeros(I * (J + 1), np.float32),
+            "particle_viscosity": np.zeros(n, np.float32), "viscosity": np.zeros(I * J, np.float32),
         }
         s = SceneState()
         for k, a in out.items():
             setattr(s, k, a.ctypes.data)
         _check(lib().fg_scene_get_state(self.h, C.byref(s)))
         n = s.n_particles
-        for k in ("px", "py", "vx", "vy", "pid"):
+        for k in ("px", "py", "vx", "vy", "pid", "particle_viscosity"):
             out[k] = out[k][:n]
         out["rng_epoch"], out["next_pid"] = s.rng_epoch, s.next_pid
         return out

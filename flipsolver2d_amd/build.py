@@ -19,7 +19,7 @@ LIB = os.path.join(LIBDIR, "libflipgpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FG_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["fg_particles.hip", "fg_grid.hip", "fg_pcg.hip", "fg_mg.hip", "fg_init.hip", "fg_nbflip.hip",
+SOURCES = ["fg_particles.hip", "fg_grid.hip", "fg_pcg.hip", "fg_mg.hip", "fg_init.hip", "fg_nbflip.hip", "fg_visc.hip",
            "fg_api.hip", "fg_serve.cpp"]
 # -ffp-contract=off: every float expression rounds as written (the kernels
 # are compared bit for bit with the CPU restatement).  Packed FP32 ops are

@@ -175,7 +175,12 @@ FlipSolver::FlipSolver(const FlipSolverParameters* p, fg_ctx* sharedCtx, int dev
     params_ = *p;
     if (params_.gridSizeI <= 0 || params_.gridSizeJ <= 0) throw Error(FG_E_INVALID, "grid size must be > 0");
     if (!(params_.dx > 0.0)) params_.dx = params_.domainSizeI / params_.gridSizeI;
-    if (params_.viscosityEnabled) throw Error(FG_E_INVALID, "viscosity is not part of this solver");
+    if (params_.viscosityEnabled && params_.useHeavyViscosity)
+        throw Error(FG_E_INVALID, "the heavy viscosity model is not built (the light one is)");
+    if (params_.viscosityEnabled && params_.simulationMethod == SIMULATION_NBFLIP)
+        throw Error(FG_E_INVALID, "viscosity is built for FlipSolver's step only");
+    if (params_.viscosityEnabled && params_.parameterHandlingMethod != PARTICLE)
+        throw Error(FG_E_INVALID, "viscosity is built for particle parameter handling only");
     if (params_.simulationMethod == SIMULATION_SMOKE || params_.simulationMethod == SIMULATION_FIRE)
         throw Error(FG_E_INVALID, "only the liquid solver is built");
     if (sharedCtx) {
@@ -202,6 +207,7 @@ FlipSolver::FlipSolver(const FlipSolverParameters* p, fg_ctx* sharedCtx, int dev
     sp.solver_tolerance = 1e-6;
     sp.max_particles = (int64_t)params_.gridSizeI * params_.gridSizeJ * params_.particlesPerCell;
     sp.sim_method = params_.simulationMethod == SIMULATION_NBFLIP ? FG_SIM_NBFLIP : FG_SIM_FLIP;
+    sp.viscosity_enabled = params_.viscosityEnabled ? 1 : 0;
     if (sp.sim_method == FG_SIM_NBFLIP) sp.max_particles *= 2;   // band cells fill to ppc, crowded ones hold 2 ppc
     const int rc = fg_scene_create(ctx_, &sp, &scene_);
     if (rc != FG_OK) {
@@ -277,7 +283,7 @@ void FlipSolver::addInitialFluid(Emitter& emitter)
 void FlipSolver::firstFrameInit()
 {
     std::vector<Geometry2d> solids, sources, sinks, fluids;
-    std::vector<float> friction, velocity;
+    std::vector<float> friction, velocity, source_visc, fluid_visc;
     std::vector<uint8_t> transfer;
     for (const Obstacle& o : obstacles_) {
         solids.push_back(o.geometry());
@@ -288,9 +294,13 @@ void FlipSolver::firstFrameInit()
         velocity.push_back(e.velocity().x());
         velocity.push_back(e.velocity().y());
         transfer.push_back(e.velocityTransfer() ? 1 : 0);
+        source_visc.push_back(e.viscosity());
     }
     for (const Sink& s : sinks_) sinks.push_back(s.geo());
-    for (const Emitter& e : initialFluid_) fluids.push_back(e.geometry());
+    for (const Emitter& e : initialFluid_) {
+        fluids.push_back(e.geometry());
+        fluid_visc.push_back(e.viscosity());
+    }
     const PolyPack ps = pack(solids), pr = pack(sources), pk = pack(sinks), pf = pack(fluids);
     fg_scene_objects o{};
     o.solids = ps.view();
@@ -301,6 +311,8 @@ void FlipSolver::firstFrameInit()
     o.sinks = pk.view();
     o.fluids = pf.view();
     o.seed = params_.seed;
+    o.source_viscosity = source_visc.empty() ? nullptr : source_visc.data();
+    o.fluid_viscosity = fluid_visc.empty() ? nullptr : fluid_visc.data();
     check(fg_scene_init_objects(scene_, &o));
     initAdditionalParameters();
     dirty_ = true;

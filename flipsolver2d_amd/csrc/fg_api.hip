@@ -350,6 +350,21 @@ void carve(fg_scene* s, Carver& cv)
     h.friction = cv.take<float>(256);
     h.eid = cv.take<int32_t>(C);
     h.emitters = cv.take<float4>(kMaxEmitters);
+    if (s->p.viscosity_enabled) {
+        const int64_t cb = cdiv(C, kBlock);
+        h.pvisc = cv.take<float>(P);
+        h.pviscA = cv.take<float>(P);
+        h.cvisc = cv.take<float>(C);
+        h.vd = cv.take<double>(C);
+        h.ve = cv.take<double>(C);
+        h.vn = cv.take<double>(C);
+        h.vx = cv.take<double>(2 * C);
+        h.vr = cv.take<double>(2 * C);
+        h.vp = cv.take<double>(2 * C);
+        h.vq = cv.take<double>(2 * C);
+        h.vpart = cv.take<double>(kViscSlots * cb);
+        h.vsc = cv.take<double>(kViscScalars);
+    }
     if (s->p.sim_method == FG_SIM_NBFLIP) {
         h.adv_u = cv.take<float>(FU);
         h.adv_v = cv.take<float>(FV);
@@ -380,6 +395,11 @@ int validate_params(const fg_scene_params* p)
     if (p->sim_method != FG_SIM_FLIP && p->sim_method != FG_SIM_NBFLIP) return fail(FG_E_INVALID, "bad sim_method");
     if (p->sim_method == FG_SIM_NBFLIP && p->density_mode != FG_DENSITY_OFF)
         return fail(FG_E_INVALID, "NBFlipSolver::step has no density correction: density_mode must be OFF");
+    if (p->viscosity_enabled != 0 && p->viscosity_enabled != 1) return fail(FG_E_INVALID, "bad viscosity_enabled");
+    if (p->viscosity_enabled && p->sim_method != FG_SIM_FLIP)
+        return fail(FG_E_INVALID, "viscosity is built for FlipSolver's step only");
+    if (p->viscosity_enabled && p->density_mode != FG_DENSITY_OFF)
+        return fail(FG_E_INVALID, "FlipSolver::step skips density correction with viscosity: density_mode must be OFF");
     return FG_OK;
 }
 
@@ -447,6 +467,7 @@ int setup_scene_memory(fg_scene* s)
     h.active = 1;
     h.density_mode = p->density_mode;
     h.nb = p->sim_method == FG_SIM_NBFLIP ? 1 : 0;
+    h.visc_on = p->viscosity_enabled ? 1 : 0;
     // flipsolver2d.cpp:207-208
     h.cell_vol = (float)(p->dx * p->dx * p->dx);
     h.pmass = (float)((p->fluid_density * h.cell_vol) / (float)p->particles_per_cell);
@@ -767,6 +788,7 @@ struct Group {
     int max_j = 0;                      // the widest scene's J
     bool sources = false;               // some scene has SOURCE cells (afterTransfer, reseeding)
     bool nb = false;                    // NB-FLIP scenes (a group holds one solver class)
+    bool visc = false;                  // some scene has viscosity enabled
     int mg_levels = 0;                  // most multigrid levels of a scene using the V-cycle
     int map_kinds = 0;                  // block-map kinds the solve uses (0 when maps are off)
     int64_t map_off[kMapKinds]{};       // offset of each kind's map in the lane's map buffer
@@ -913,6 +935,7 @@ int make_group(fg_ctx* ctx, const std::vector<fg_scene*>& scenes, const std::vec
         G.max_inner = std::max(G.max_inner, s->h.max_inner);
         G.sources = G.sources || s->h.has_sources;
         G.nb = G.nb || s->h.nb;
+        G.visc = G.visc || s->h.visc_on;
         G.max_bands = std::max(G.max_bands, (int)std::min<int64_t>(kMaxBands, cdiv(std::max(s->I, s->J), 16) + 1));
     }
     geo_set_y(G.g, G.n);
@@ -1423,6 +1446,71 @@ int run_solve(Group& G, bool rhs_from_grid)
 // ---------------------------------------------------------------------------
 // stages of FlipSolver::step
 // ---------------------------------------------------------------------------
+// applyViscosity and the second project() of FlipSolver::step
+// (flipsolver2d.cpp:446-453) for the scenes of G with viscosity, as a
+// sub-group: the light model's matrix, its Jacobi-preconditioned CG on U and
+// V together (polled every kViscPoll iterations), the result, then the
+// pressure solve again with the substep's matrix and its plain application.
+constexpr int kViscPoll = 8;
+constexpr int kViscMaxIters = 20000;
+constexpr double kViscTol = 1e-12;   // relative residual of each system (FP64)
+int run_viscosity(Group& G)
+{
+    std::vector<fg_scene*> vs;
+    std::vector<fg_stats*> vst;
+    for (int k = 0; k < G.n; k++)
+        if (G.sc[k]->h.visc_on) {
+            vs.push_back(G.sc[k]);
+            vst.push_back(G.st[k]);
+        }
+    Group V;
+    FGCHK(make_group(G.ctx, vs, vst, V, true, G.L));
+    hipStream_t st = V.L->stream;
+    const dim3 one(1, V.n);
+    const double tol2 = kViscTol * kViscTol;
+    launch_visc_setup(V.g.cells, st, V.d);
+    launch_visc_red(one, st, V.d, 0, tol2);
+    int it = 0, open = 1;
+    while (open && it < kViscMaxIters) {
+        for (int k = 0; k < kViscPoll; k++) {
+            launch_visc_spmv(V.g.cells, st, V.d);
+            launch_visc_red(one, st, V.d, 1, tol2);
+            launch_visc_update(V.g.cells, st, V.d);
+            launch_visc_red(one, st, V.d, 2, tol2);
+            launch_visc_dir(V.g.cells, st, V.d);
+        }
+        it += kViscPoll;
+        launch_visc_flags(st, V.d, V.n, V.L->bflags);
+        HIPCHK(hipMemcpyAsync(V.L->bflags_host, V.L->bflags, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        open = V.L->bflags_host[0];
+    }
+    launch_visc_apply(V.g.cells, st, V.d);
+    for (int k = 0; k < V.n; k++) {
+        double sc[kViscScalars];
+        HIPCHK(hipMemcpyAsync(sc, V.sc[k]->h.vsc, sizeof(sc), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (fg_stats* stats = V.st[k]) {
+            stats->viscosity_iters = std::max(stats->viscosity_iters, (int32_t)std::max(sc[14], sc[15]));
+            if (sc[12] == 0.0 || sc[13] == 0.0) stats->converged = 0;
+        }
+    }
+    // project() again: rhs from the viscous grid, the same matrix snapshot
+    const int r = run_solve(V, true);
+    for (int k = 0; k < V.n; k++) {
+        fg_stats* stats = V.st[k];
+        if (!stats) continue;
+        const SceneScalars& sc = *V.sc[k]->hsc;
+        stats->pressure_iters = std::max(stats->pressure_iters, sc.total_iters);
+        stats->pressure_sweeps = std::max(stats->pressure_sweeps, sc.sweeps);
+        stats->pressure_rel_resid = sc.bb > 0 ? std::sqrt(sc.rr64 / sc.bb) : 0.0;
+        if (stats->pressure_rel_resid > V.sc[k]->h.tol) stats->converged = 0;
+    }
+    FGCHK(r);
+    launch_apply_solids(V.g.cells, st, V.d, 0);
+    return FG_OK;
+}
+
 // The band launches of a level-set extrapolation whose markers
 // k_levelset_init has set.  A band launch after the last one needed exits
 // at once but still dispatches its blocks: the first poll comes after one
@@ -1498,9 +1586,13 @@ int run_stage(Group& G, int stage)
         break;
     }
     case FG_STAGE_P2G: {
-        KProf kp(G, FG_K_P2G, true);
-        launch_p2g(g.p2g, st, d);
-        kp.done(0);   // bytes: stage_bytes, once the substep's counts are back
+        {
+            KProf kp(G, FG_K_P2G, true);
+            launch_p2g(g.p2g, st, d);
+            kp.done(0);   // bytes: stage_bytes, once the substep's counts are back
+        }
+        // particleToGrid's centeredParamsToGrid: the viscosity grid
+        if (G.visc) launch_visc_grid(g.cells, st, d);
         break;
     }
     case FG_STAGE_SDF: {
@@ -1543,9 +1635,12 @@ int run_stage(Group& G, int stage)
             if (stats->pressure_rel_resid > G.sc[k]->h.tol) stats->converged = 0;
         }
         FGCHK(r);
-        KProf kp(G, FG_K_APPLY, true);
-        launch_apply_solids(g.cells, st, d);
-        kp.done(G.cells);
+        {
+            KProf kp(G, FG_K_APPLY, true);
+            launch_apply_solids(g.cells, st, d, 1);
+            kp.done(G.cells);
+        }
+        if (G.visc) FGCHK(run_viscosity(G));
         break;
     }
     case FG_STAGE_SOLIDS:
@@ -2032,10 +2127,11 @@ int batch_args(fg_ctx* ctx, fg_scene* const* scenes, int32_t n, fg_stats* stats,
 
 // Polygons of a set-up packed for k_init_cells: vertex offsets, then the
 // vertices as float2, in the set order sinks, sources, solids, fluids.
-int upload_geometry(fg_scene* s, const fg_polygons* sets[4], InitGeo* geo)
+int upload_geometry(fg_scene* s, const fg_polygons* sets[4], const float* const visc[4], InitGeo* geo)
 {
     std::vector<int32_t> voff(1, 0);
     std::vector<float2> verts;
+    std::vector<float> pvisc;
     int first = 0;
     for (int q = 0; q < 4; q++) {
         const fg_polygons& P = *sets[q];
@@ -2043,6 +2139,7 @@ int upload_geometry(fg_scene* s, const fg_polygons* sets[4], InitGeo* geo)
         geo->set[q] = PolySet{first, P.n};
         const float* v = P.verts;
         for (int32_t k = 0; k < P.n; k++) {
+            pvisc.push_back(visc[q] ? visc[q][k] : 0.f);
             if (P.nverts[k] < 3) return fail(FG_E_INVALID, "a polygon needs 3 vertices");
             for (int32_t e = 0; e < P.nverts[k]; e++) verts.push_back(make_float2(v[2 * e], v[2 * e + 1]));
             v += 2 * P.nverts[k];
@@ -2051,7 +2148,8 @@ int upload_geometry(fg_scene* s, const fg_polygons* sets[4], InitGeo* geo)
         first += P.n;
     }
     const size_t off_bytes = (sizeof(int32_t) * voff.size() + 255) & ~size_t(255);
-    const size_t bytes = off_bytes + sizeof(float2) * std::max<size_t>(verts.size(), 1);
+    const size_t visc_bytes = (sizeof(float) * std::max<size_t>(pvisc.size(), 1) + 255) & ~size_t(255);
+    const size_t bytes = off_bytes + visc_bytes + sizeof(float2) * std::max<size_t>(verts.size(), 1);
     Lane* L = s->ctx->lanes[0];
     if (bytes > L->bgeo_cap) {
         HIPCHK(hipStreamSynchronize(L->stream));
@@ -2063,14 +2161,41 @@ int upload_geometry(fg_scene* s, const fg_polygons* sets[4], InitGeo* geo)
     }
     char* base = static_cast<char*>(L->bgeo);
     HIPCHK(hipMemcpyAsync(base, voff.data(), sizeof(int32_t) * voff.size(), hipMemcpyHostToDevice, L->stream));
-    if (!verts.empty())
-        HIPCHK(hipMemcpyAsync(base + off_bytes, verts.data(), sizeof(float2) * verts.size(), hipMemcpyHostToDevice,
+    if (!pvisc.empty())
+        HIPCHK(hipMemcpyAsync(base + off_bytes, pvisc.data(), sizeof(float) * pvisc.size(), hipMemcpyHostToDevice,
                               L->stream));
+    if (!verts.empty())
+        HIPCHK(hipMemcpyAsync(base + off_bytes + visc_bytes, verts.data(), sizeof(float2) * verts.size(),
+                              hipMemcpyHostToDevice, L->stream));
     geo->voff = reinterpret_cast<const int32_t*>(base);
-    geo->verts = reinterpret_cast<const float2*>(base + off_bytes);
+    geo->visc = reinterpret_cast<const float*>(base + off_bytes);
+    geo->verts = reinterpret_cast<const float2*>(base + off_bytes + visc_bytes);
     geo->dx = (float)s->p.dx;
     // the host vectors are read by the copies above: wait for them
     HIPCHK(hipStreamSynchronize(L->stream));
+    return FG_OK;
+}
+
+// The emitter table (fg_scene_set_emitters; with the sources' viscosity
+// from fg_scene_init_objects).
+int set_emitters(fg_scene* s, int32_t n, const float* velocity, const uint8_t* transfer, const float* viscosity,
+                 uint32_t seed)
+{
+    if (!s || (n > 0 && !velocity)) return fail(FG_E_INVALID, "NULL argument");
+    if (n < 0 || n > kMaxEmitters) return fail(FG_E_INVALID, "at most 256 emitters");
+    if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
+    // per emitter the face values afterTransfer writes: velocity / dx in
+    // double, rounded to float (flipsolver2d.cpp:402-403)
+    std::vector<float4> tab(kMaxEmitters, make_float4(0.f, 0.f, 0.f, 0.f));
+    for (int32_t k = 0; k < n; k++)
+        tab[k] = make_float4((float)((double)velocity[2 * k] / s->p.dx), (float)((double)velocity[2 * k + 1] / s->p.dx),
+                             transfer && transfer[k] ? 1.f : 0.f, viscosity ? viscosity[k] : 0.f);
+    HIPCHK(hipMemcpyAsync(s->h.emitters, tab.data(), sizeof(float4) * kMaxEmitters, hipMemcpyHostToDevice,
+                          s->ctx->stream));
+    s->h.n_emitters = n;
+    s->h.rseed = seed;
+    FGCHK(upload_desc(s));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
     return FG_OK;
 }
 
@@ -2328,6 +2453,11 @@ int fg_scene_set_state(fg_scene* s, const fg_scene_state* st)
         HIPCHK(hipMemcpyAsync(h.vel, vel.data(), sizeof(float2) * n, hipMemcpyHostToDevice, q));
         HIPCHK(hipMemcpyAsync(h.pid, pid.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, q));
         HIPCHK(hipMemsetAsync(h.key, 0, sizeof(uint32_t) * n, q));
+        if (h.visc_on) {
+            if (st->particle_viscosity)
+                HIPCHK(hipMemcpyAsync(h.pvisc, st->particle_viscosity, sizeof(float) * n, hipMemcpyHostToDevice, q));
+            else HIPCHK(hipMemsetAsync(h.pvisc, 0, sizeof(float) * n, q));
+        }
     }
     HIPCHK(hipMemcpyAsync(h.uf, st->u, sizeof(float) * s->FU, hipMemcpyHostToDevice, q));
     HIPCHK(hipMemcpyAsync(h.vf, st->v, sizeof(float) * s->FV, hipMemcpyHostToDevice, q));
@@ -2379,7 +2509,9 @@ int fg_scene_get_state(fg_scene* s, fg_scene_state* st)
     const int64_t n = s->hsc->n_particles;
     std::vector<float2> pos(n), vel(n);
     std::vector<uint32_t> pid(n), key(n);
+    std::vector<float> pv(h.visc_on ? n : 0);
     if (n > 0) {
+        if (h.visc_on) HIPCHK(hipMemcpyAsync(pv.data(), h.pvisc, sizeof(float) * n, hipMemcpyDeviceToHost, q));
         HIPCHK(hipMemcpyAsync(pos.data(), h.pos, sizeof(float2) * n, hipMemcpyDeviceToHost, q));
         HIPCHK(hipMemcpyAsync(vel.data(), h.vel, sizeof(float2) * n, hipMemcpyDeviceToHost, q));
         HIPCHK(hipMemcpyAsync(pid.data(), h.pid, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, q));
@@ -2404,6 +2536,8 @@ int fg_scene_get_state(fg_scene* s, fg_scene_state* st)
     FGCHK(dl(st->rhs, h.b, sizeof(double) * s->C));
     FGCHK(dl(st->stencil, h.stencil, s->C));
     FGCHK(dl(st->emitter_id, h.eid, sizeof(int32_t) * s->C));
+    if (h.visc_on) FGCHK(dl(st->viscosity, h.cvisc, sizeof(float) * s->C));
+    else if (st->viscosity) std::fill(st->viscosity, st->viscosity + s->C, 0.f);
     {   // the grid fluidVelocityGrid holds after the last stage hook
         const float *wu = h.u, *wv = h.v;   // P2G .. SDF, FORCES, PROJECT (+ SOLIDS)
         if (s->last_stage == FG_STAGE_EXTRAPOLATE) {   // extrapolated into the saved grid
@@ -2427,6 +2561,7 @@ int fg_scene_get_state(fg_scene* s, fg_scene_state* st)
         if (st->vx) st->vx[w] = vel[k].x;
         if (st->vy) st->vy[w] = vel[k].y;
         if (st->pid) st->pid[w] = pid[k];
+        if (st->particle_viscosity) st->particle_viscosity[w] = h.visc_on ? pv[k] : 0.f;
         w++;
     }
     st->n_particles = w;
@@ -2453,22 +2588,7 @@ int fg_scene_particle_count(fg_scene* s, int64_t* out)
 
 int fg_scene_set_emitters(fg_scene* s, int32_t n, const float* velocity, const uint8_t* transfer, uint32_t seed)
 {
-    if (!s || (n > 0 && !velocity)) return fail(FG_E_INVALID, "NULL argument");
-    if (n < 0 || n > kMaxEmitters) return fail(FG_E_INVALID, "at most 256 emitters");
-    if (s->solver_only) return fail(FG_E_STATE, "solver-only scene");
-    // per emitter the face values afterTransfer writes: velocity / dx in
-    // double, rounded to float (flipsolver2d.cpp:402-403)
-    std::vector<float4> tab(kMaxEmitters, make_float4(0.f, 0.f, 0.f, 0.f));
-    for (int32_t k = 0; k < n; k++)
-        tab[k] = make_float4((float)((double)velocity[2 * k] / s->p.dx), (float)((double)velocity[2 * k + 1] / s->p.dx),
-                             transfer && transfer[k] ? 1.f : 0.f, 0.f);
-    HIPCHK(hipMemcpyAsync(s->h.emitters, tab.data(), sizeof(float4) * kMaxEmitters, hipMemcpyHostToDevice,
-                          s->ctx->stream));
-    s->h.n_emitters = n;
-    s->h.rseed = seed;
-    FGCHK(upload_desc(s));
-    HIPCHK(hipStreamSynchronize(s->ctx->stream));
-    return FG_OK;
+    return set_emitters(s, n, velocity, transfer, nullptr, seed);
 }
 
 int fg_scene_init_objects(fg_scene* s, const fg_scene_objects* o)
@@ -2479,8 +2599,9 @@ int fg_scene_init_objects(fg_scene* s, const fg_scene_objects* o)
     if (o->sources.n > kMaxEmitters) return fail(FG_E_INVALID, "at most 256 sources");
     s->last_stage = -1;
     const fg_polygons* sets[4] = {&o->sinks, &o->sources, &o->solids, &o->fluids};
+    const float* const visc[4] = {nullptr, o->source_viscosity, nullptr, o->fluid_viscosity};
     InitGeo geo{};
-    FGCHK(upload_geometry(s, sets, &geo));
+    FGCHK(upload_geometry(s, sets, visc, &geo));
     hipStream_t q = s->ctx->stream;
     SceneDev& h = s->h;
     // grids of a scene at rest; the cell pass writes materials, ids, SDFs
@@ -2515,7 +2636,7 @@ int fg_scene_init_objects(fg_scene* s, const fg_scene_objects* o)
     else FGCHK(fg_scene_set_friction(s, 0, nullptr));
     std::vector<float> vel(2 * std::max(o->sources.n, 0), 0.f);
     if (o->source_velocity) std::copy(o->source_velocity, o->source_velocity + vel.size(), vel.begin());
-    FGCHK(fg_scene_set_emitters(s, o->sources.n, vel.data(), o->source_transfer, o->seed));
+    FGCHK(set_emitters(s, o->sources.n, vel.data(), o->source_transfer, o->source_viscosity, o->seed));
     if (!fits) return fail(FG_E_CAPACITY, "seeded particles exceed max_particles");
     return FG_OK;
 }

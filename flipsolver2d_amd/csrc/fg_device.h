@@ -198,7 +198,20 @@ struct SceneDev {
     // source at the cell corner (FLT_MAX without sources)
     int32_t nb;
     float *adv_u, *adv_v, *adv_sdf, *src_sdf;
+    // viscosity (the light model, viscositymodel.cpp:4-140): the particles'
+    // viscosity (sorted, advected copy), the cell grid; the matrix (diagonal,
+    // couplings to i+1 and j+1) and the CG vectors of the U and V systems
+    // ([2][C]: x, r, p, q), per-block partials and the CG scalars
+    int32_t visc_on;
+    float *pvisc, *pviscA, *cvisc;
+    double *vd, *ve, *vn, *vx, *vr, *vp, *vq;
+    double* vpart;          // [kViscSlots][max cell blocks]
+    double* vsc;            // kViscScalars
 };
+// viscosity CG scalars (per system s = 0 (U), 1 (V)): bb, rz, pq, rr, alpha,
+// beta, done at index 2 k + s; the iteration counts at 14, 15
+constexpr int kViscSlots = 4;         // per-block partials: two dots x two systems
+constexpr int kViscScalars = 16;
 constexpr int kMaxEmitters = 256;
 // NBFlipSolver's bands (nbflipsolver.cpp:18-20), in cells of level set
 constexpr float kNbNarrowBand = -3.f, kNbCombinationBand = -2.f, kNbResamplingBand = -1.f;

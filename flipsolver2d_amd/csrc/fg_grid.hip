@@ -922,7 +922,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(const SceneDev* scenes)
 // updateVelocityFromSolids (1077-1104, materialgrid.cpp:152-166).  Both
 // touch faces U(i,j), V(i,j) of cell (i,j) only, so they fuse per cell.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_apply_solids(const SceneDev* scenes)
+__global__ void __launch_bounds__(kBlock) k_apply_solids(const SceneDev* scenes, int friction)
 {
     const SceneDev& s = scenes[blockIdx.y];
     if (!s.active) return;
@@ -959,7 +959,7 @@ __global__ void __launch_bounds__(kBlock) k_apply_solids(const SceneDev* scenes)
     // friction the factor (1 - avg) is exactly 1 and the loads are skipped
     int n = 0;
     float avg = 0;
-    if (s.friction_any) {
+    if (friction && s.friction_any) {
         for (int di = -1; di < 2; di++)
             for (int dj = -1; dj < 2; dj++) {
                 const int ii = i + di, jj = j + dj;
@@ -1008,7 +1008,7 @@ void launch_levelset_band(dim3 g, hipStream_t st, const SceneDev* sc, int band, 
 }
 FG_LAUNCH0(forces)
 FG_LAUNCH0(rhs)
-FG_LAUNCH0(apply_solids)
+FG_LAUNCH1(apply_solids, int, friction)
 FG_LAUNCH0(density_rhs)
 FG_LAUNCH0(density_adjust)
 }  // namespace fg

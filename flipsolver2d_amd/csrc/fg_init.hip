@@ -91,12 +91,15 @@ __global__ void __launch_bounds__(kBlock) k_init_cells(const SceneDev* scenes, I
         mat = M_SOLID;
         sid = min_idx;
     }
-    float fsdf = 0.f;
+    float fsdf = 0.f, visc = 0.f;
     if (!s.nb) {
-        // updateInitialFluid: float centre
+        // updateInitialFluid: float centre; the fluid's viscosity into the grid
         const float x = ((float)i + 0.5f) * dx, y = ((float)j + 0.5f) * dx;
         for (int k = geo.set[3].first; k < geo.set[3].first + geo.set[3].n; k++)
-            if (polygon_sdf(poly(k), nv(k), x, y) <= 0.f) mat = M_FLUID;
+            if (polygon_sdf(poly(k), nv(k), x, y) <= 0.f) {
+                mat = M_FLUID;
+                visc = geo.visc[k];
+            }
     } else {
         // NBFlipSolver::fluidSdfFromInitialFluid (nbflipsolver.cpp:291-318):
         // centre in double as updateSolids, the nearest fluid's distance in
@@ -124,6 +127,7 @@ __global__ void __launch_bounds__(kBlock) k_init_cells(const SceneDev* scenes, I
     s.eid[c] = eid;
     s.ssdf[c] = dist;
     s.fsdf[c] = fsdf;
+    if (s.visc_on) s.cvisc[c] = visc;
     s.x64[c] = 0.0;
     // seeding: this cell's particles (NB-FLIP's initialFluidSeed adds none,
     // nbflipsolver.cpp:254-289)
@@ -156,6 +160,8 @@ __global__ void __launch_bounds__(kBlock) k_seed(const SceneDev* scenes, uint32_
         s.vel[idx] = make_float2(0.f, 0.f);
         s.pid[idx] = (uint32_t)idx;
         s.key[idx] = 0;
+        // m_viscosityGrid.interpolateAt(pos), offset (1/2, 1/2) (flipsolver2d.cpp:37, 697-702)
+        if (s.visc_on) s.pvisc[idx] = lerp_grid(s.cvisc, I, J, 0.5f, 0.5f, x, y);
     }
 }
 
@@ -271,6 +277,7 @@ __global__ void __launch_bounds__(kBlock) k_reseed_write(const SceneDev* scenes)
         s.vel[idx] = v;
         s.pid[idx] = s.next_pid + (uint32_t)(off + a);
         s.key[idx] = 0;
+        if (s.visc_on) s.pvisc[idx] = (e >= 0 && e < s.n_emitters) ? s.emitters[e].w : 0.f;
         const float v2 = v.x * v.x + v.y * v.y;
         m = v2 > m ? v2 : m;
         a++;

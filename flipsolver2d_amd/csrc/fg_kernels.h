@@ -90,7 +90,8 @@ FG_DECLARE_LAUNCH(levelset_init, bool outside);
 FG_DECLARE_LAUNCH(levelset_band, int band, bool outside);
 FG_DECLARE_LAUNCH(forces);
 FG_DECLARE_LAUNCH(rhs);
-FG_DECLARE_LAUNCH(apply_solids);
+// friction: also updateVelocityFromSolids (0: the second projection's plain application)
+FG_DECLARE_LAUNCH(apply_solids, int friction);
 FG_DECLARE_LAUNCH(density_rhs);
 FG_DECLARE_LAUNCH(density_adjust);
 constexpr int kBfsTile = 32;
@@ -106,6 +107,7 @@ struct InitGeo {
     PolySet set[4];
     const int32_t* voff;
     const float2* verts;
+    const float* visc;      // per polygon: Emitter::viscosity of sources and fluids (0 otherwise)
     float dx;
 };
 void launch_init_cells(dim3 grid, hipStream_t st, const SceneDev* scenes, const InitGeo& geo);
@@ -115,6 +117,16 @@ FG_DECLARE_LAUNCH(after_transfer);
 FG_DECLARE_LAUNCH(reseed_count);
 FG_DECLARE_LAUNCH(reseed_write);
 FG_DECLARE_LAUNCH(reseed_commit);
+
+// fg_visc.hip: the light viscosity model (cells; red: one block a scene)
+FG_DECLARE_LAUNCH(visc_grid);
+FG_DECLARE_LAUNCH(visc_setup);
+FG_DECLARE_LAUNCH(visc_spmv);
+FG_DECLARE_LAUNCH(visc_update);
+FG_DECLARE_LAUNCH(visc_dir);
+FG_DECLARE_LAUNCH(visc_apply);
+FG_DECLARE_LAUNCH(visc_red, int what, double tol2);
+void launch_visc_flags(hipStream_t st, const SceneDev* scenes, int n, int32_t* out);
 
 // fg_nbflip.hip: NBFlipSolver's grid passes (nb_euler over the (I+1) x (J+1)
 // lattice points, combine_vel over faces, the others over cells)

@@ -212,6 +212,7 @@ __global__ void __launch_bounds__(kBlock) k_advect(const SceneDev* scenes, int a
         }
         s.posA[k] = p;
         s.velA[k] = vel;
+        if (s.visc_on) s.pviscA[k] = s.pvisc[k];
         s.key[k] = key;
     }
     // histogram: one atomic per run of equal keys in the wave
@@ -473,6 +474,7 @@ __global__ void __launch_bounds__(kBlock) k_gather(const SceneDev* scenes)
     const uint32_t src = s.tmpSrc[k];
     s.pos[k] = s.posA[src];
     s.vel[k] = s.velA[src];
+    if (s.visc_on) s.pvisc[k] = s.pviscA[src];
     s.pid[k] = s.tmpPid[k];
     s.key[k] = 0;   // live
 }

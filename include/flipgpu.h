@@ -116,7 +116,11 @@ typedef struct fg_scene_params {
     double  solver_tolerance;     /* FP64 relative residual ||b-Ax||2/||b||2 target */
     int64_t max_particles;        /* particle capacity */
     int32_t sim_method;           /* fg_sim_method */
-    int32_t reserved0;            /* 0 */
+    int32_t viscosity_enabled;    /* viscosityEnabled (flipsolver2d.h:52): the light viscosity model
+                                   * (LightViscosityModel, viscositymodel.cpp:4-140) after the
+                                   * projection, then a second projection (flipsolver2d.cpp:446-453).
+                                   * FG_SIM_FLIP only, with density_mode OFF (step() skips density
+                                   * correction with viscosity, 425); the heavy model is not built. */
 } fg_scene_params;
 
 /* Host-side view of a scene's state.  Arrays are row-major, index i*J+j for
@@ -159,6 +163,10 @@ typedef struct fg_scene_state {
                                    * FG_STAGE_PROJECT the projected one (updateVelocityFromSolids
                                    * applied too: the two stages share a kernel); otherwise, like u
                                    * and v above, the substep's final, extrapolated grid. */
+    float    *particle_viscosity; /* [n_particles] the particles' viscosity property (a scene with
+                                   * viscosity_enabled; on set optional: NULL gives 0) */
+    float    *viscosity;          /* I*J, get only: the viscosity grid (centeredParamsToGrid,
+                                   * flipsolver2d.cpp:1379-1431) of the last substep */
 } fg_scene_state;
 
 typedef struct fg_stats {
@@ -178,7 +186,7 @@ typedef struct fg_stats {
     double  stage_ms[FG_STAGE_NUM]; /* filled when timing is enabled */
     double  frame_ms;
     int32_t status;               /* FG_OK, or FG_E_NUMERIC: the scene is quarantined */
-    int32_t reserved;
+    int32_t viscosity_iters;      /* max over substeps of the viscosity CG's iterations (U or V) */
     int64_t particles_reseeded;   /* reseedParticles: added in SOURCE cells */
 } fg_stats;
 
@@ -350,6 +358,10 @@ typedef struct fg_scene_objects {
     fg_polygons sinks;
     fg_polygons fluids;
     uint32_t seed;                /* seeding and reseeding RNG */
+    const float* source_viscosity; /* [sources.n] or NULL: Emitter::viscosity (reseeded particles,
+                                    * SOURCE cells' viscosity) */
+    const float* fluid_viscosity; /* [fluids.n] or NULL: Emitter::viscosity of the initial fluid
+                                   * (updateInitialFluid's grid, which seeding lerps) */
 } fg_scene_objects;
 
 /* First-frame init on the device (FlipSolver::firstFrameInit): the cell

@@ -18,7 +18,7 @@ def oracle_state_dict(st):
         "pid": st.pid[:n].copy(), "u": st.u.copy(), "v": st.v.copy(), "u_valid": st.u_valid.copy(),
         "v_valid": st.v_valid.copy(), "material": st.material.copy(), "solid_sdf": st.solid_sdf.copy(),
         "fluid_sdf": st.fluid_sdf.copy(), "solid_id": st.solid_id.copy(), "emitter_id": st.emitter_id.copy(),
-        "particle_counts": st.particle_counts.copy(),
+        "particle_counts": st.particle_counts.copy(), "particle_viscosity": st.pvisc[:n].copy(),
         "rng_epoch": int(st.c.rng_epoch), "next_pid": int(st.c.next_pid),
     }
 
