@@ -61,6 +61,8 @@ HBM_PEAK_GBS = 8000.0
 #              dinv, y_{l+1}; y) and the 100 B of each stored row
 #   nb_euler   NB-FLIP's Euler pass: U, V and the level set read once and
 #              their advected copies written, 8 (F + C) per substep
+#   visc_cg    the viscosity CG (U and V together): 216 B a cell and iteration,
+#              timed over the whole iteration stretch with its polls
 #   mg_coarse  per scene and V-cycle: 4 B per cell of the levels' dinv, of its
 #              input and of its output, once each the 100 B of every row above
 #              the bottom that is not its level's canonical interior stencil,
@@ -77,6 +79,7 @@ KERNEL_BYTES = {
     "mg_pp": ("K_MG_PP", 1.0, "bytes", ["k_mg_pp"]),
     "mg_coarse": ("K_MG_COARSE", 1.0, "bytes", ["k_mg_coarse"]),
     "nb_euler": ("K_NB_EULER", 1.0, "bytes", ["k_nb_euler"]),
+    "visc_cg": ("K_VISC", 1.0, "bytes", ["k_visc_spmv", "k_visc_update", "k_visc_dir", "k_visc_red"]),
 }
 STEP_FAMILIES = ["p2g", "particles", "sdf", "pcg_a", "pcg_b", "mg_down0", "mg_up0", "mg_resid", "mg_pp",
                  "mg_coarse"]
@@ -93,6 +96,7 @@ MEASURED_TRAFFIC_PER_UNIT = {
     "dam1024": {"p2g": 1.107, "particles": 2.041, "sdf": 0.63, "pcg_a": 1.022, "pcg_b": 1.003,
                 "mg_down0": 1.029, "mg_up0": 0.939, "mg_resid": 1.21, "mg_pp": 1.113, "mg_coarse": 1.513},
     "nb1024": {},
+    "visc256": {},
     "proj4096": {"pcg_a": 1.863, "pcg_b": 1.0, "mg_down0": 1.006, "mg_up0": 1.983, "mg_resid": 1.259,
                  "mg_pp": 1.222, "mg_coarse": 1.386},
 }
@@ -106,8 +110,8 @@ def parse():
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--master-seed", type=int, default=2024)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--legs", default="batched,plain,dam,nb,proj,served,cpu",
-                    help="comma list of batched, plain, dam, nb, proj, served, cpu (profiling runs pick one)")
+    ap.add_argument("--legs", default="batched,plain,dam,nb,visc,proj,served,cpu",
+                    help="comma list of batched, plain, dam, nb, visc, proj, served, cpu (profiling runs pick one)")
     ap.add_argument("--kernel-frames", type=int, default=2,
                     help="frames (solves) each kernel family is timed over, after the timed ones (0: skip)")
     ap.add_argument("--skip-plain", action="store_true", help="batched leg only (profiling runs)")
@@ -116,6 +120,9 @@ def parse():
                     help="autotuner rounds (each steps 4 frames; 0: no tuning, --lanes or 1 lane)")
     ap.add_argument("--dam-frames", type=int, default=10, help="timed frames of the 1024^2 leg (0: skip)")
     ap.add_argument("--nb-frames", type=int, default=10, help="timed frames of the NB-FLIP 1024^2 leg (0: skip)")
+    ap.add_argument("--visc-frames", type=int, default=5, help="timed frames of the viscous batch leg (0: skip)")
+    ap.add_argument("--visc-scenes", type=int, default=32, help="256^2 scenes of the viscous batch leg")
+    ap.add_argument("--visc", type=float, default=10.0, help="the viscous leg's fluid viscosity")
     ap.add_argument("--proj-res", type=int, default=4096, help="isolated projection grid (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="served-stream window (0: skip)")
     ap.add_argument("--serve-max-resident", type=int, default=4096,
@@ -439,6 +446,99 @@ def nb_leg(fg, ctx, a, copy_gbs):
     }
 
 
+def visc_cpu_sample(scene, res, ppc, dt, a):
+    """One viscous substep of the oracle from the GPU's state after the
+    timed frames (same scene, same dt; converged FP64 solves)."""
+    import oracle as orc
+
+    threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+    g = scene.get_state()
+    p = orc.make_params(res, ppc=ppc, solver=orc.SOLVER_CONVERGED, threads=threads, tol=a.tol, viscosity=True)
+    n = len(g["px"])
+    st = orc.State(res, res, res * res * ppc)
+    for k in ("px", "py", "vx", "vy", "pid"):
+        getattr(st, k)[:n] = g[k]
+    st.pvisc[:n] = g["particle_viscosity"]
+    for k in ("u", "v", "u_valid", "v_valid", "material", "solid_sdf", "fluid_sdf", "solid_id", "emitter_id",
+              "particle_counts"):
+        getattr(st, k)[...] = g[k]
+    st.n = n
+    st.c.rng_epoch, st.c.next_pid = int(g["rng_epoch"]), int(g["next_pid"])
+    t0 = time.perf_counter()
+    orc.lib().orc_substep(C.byref(p), st.ref(), C.c_float(dt), C.byref(orc.OrcStats()))
+    el = time.perf_counter() - t0
+    return {"value": n / el, "unit": "particle-substeps/s", "ms_per_substep": 1e3 * el, "cores": threads,
+            "kind": "port", "sample": f"one viscous substep of the {res}x{res} scene ({n} particles) from the GPU's "
+                                      "state after the timed frames, dt of the last GPU substep"}
+
+
+def visc_leg(fg, ctx, a, copy_gbs):
+    """Viscous dam breaks (viscosityEnabled: the light model and a second
+    projection every substep), a batch of a.visc_scenes 256^2 scenes with 8
+    ppc and fluid rectangles of varied size, stepped together by
+    fg_batch_step_frame.  (A single 1024^2 viscous dam break runs away from
+    its first substep from rest, on the oracle exactly as here: the light
+    model's rows of air cells beside the liquid are not diagonally dominant
+    once 4 nu dt > 1 and the second projection reuses the substep's stale
+    matrix; at 256^2 the flow stays bounded.)"""
+    import numpy as np
+
+    res, ppc = 256, 8
+    rng = np.random.default_rng(a.master_seed)
+    ctx.work_read(reset=True)
+    scenes = []
+    walls, _ = fg.dam_break_geometry(res)
+    for k in range(a.visc_scenes):
+        x1, y1 = rng.uniform(0.4, 0.9), rng.uniform(0.25, 0.6)
+        _, fluid = fg.dam_break_geometry(res, (x1 - rng.uniform(0.2, 0.4), x1, 0.0, y1))
+        sc = fg.Scene(ctx, fg.make_params(res, ppc=ppc, tol=a.tol, pcg_iter_limit=4000, viscosity=True))
+        sc.init_objects(solids=walls, fluids=fluid, friction=np.zeros(len(walls), np.float32),
+                        fluid_viscosity=[a.visc], seed=k)
+        scenes.append(sc)
+    n0 = sum(sc.particle_count() for sc in scenes)
+    for _ in range(a.warmup):
+        fg.batch_step_frame(ctx, scenes)
+    lat, psub, subs, resid, conv, vit, vmax = [], 0, 0, 0.0, True, 0, 0.0
+    t0 = time.perf_counter()
+    for _ in range(a.visc_frames):
+        tf = time.perf_counter()
+        st = fg.batch_step_frame(ctx, scenes)
+        lat.append(1e3 * (time.perf_counter() - tf))
+        for s in st:
+            psub += s.particle_substeps
+            subs = max(subs, s.substeps)
+            resid = max(resid, s.pressure_rel_resid)
+            conv = conv and bool(s.converged)
+            vit = max(vit, s.viscosity_iters)
+            vmax = max(vmax, s.max_velocity)
+    el = time.perf_counter() - t0
+    last_dt = st[0].last_dt
+    cpu = visc_cpu_sample(scenes[0], res, ppc, last_dt, a) if "cpu" in a.legs and a.cpu_seconds > 0 else None
+    table, dom = kernel_table(fg, ctx, STEP_FAMILIES + ["visc_cg"], lambda: fg.batch_step_frame(ctx, scenes),
+                              a.kernel_frames, "visc256", copy_gbs, "frames")
+    for sc in scenes:
+        sc.close()
+    log(f"viscous batch {a.visc_scenes} x {res}^2: {a.visc_frames} frames, {el:.2f} s")
+    return {
+        "workload": f"batched: {a.visc_scenes} viscous dam-break scenes {res}x{res}, {ppc} ppc ({n0} particles), "
+                    f"fluid viscosity {a.visc:g}: the light model + a second projection every substep",
+        "value": psub / el,
+        "ms_per_step": 1e3 * el / a.visc_frames,
+        "frames": a.visc_frames,
+        "max_substeps_per_frame": subs,
+        "viscosity_iters_max": vit,
+        "max_particle_speed": vmax,
+        "p50_frame_ms": pctl(lat, 0.5),
+        "p99_frame_ms": pctl(lat, 0.99),
+        "max_rel_residual": resid,
+        "all_converged": conv,
+        "roofline": table.get(dom),
+        "kernels": table,
+        "algorithmic_bytes_total": work_totals(fg, ctx),
+        "cpu_sample": cpu,
+    }
+
+
 def projection_leg(fg, ctx, a, copy_gbs):
     n = a.proj_res
     mat, _, _, b, scale = fg.isolated_projection_system(n, 1)
@@ -598,13 +698,14 @@ def main():
     a.legs = set(a.legs.split(","))
     if a.skip_plain:
         a.legs.discard("plain")
-    b = dam = nb = proj = served = None
-    if a.legs & {"batched", "plain", "dam", "nb", "proj", "served"}:
+    b = dam = nb = visc = proj = served = None
+    if a.legs & {"batched", "plain", "dam", "nb", "visc", "proj", "served"}:
         ctx = fg.Context(0)
         copy_gbs, _ = ctx.bandwidth_probe(1 << 30, 5)
         b = batched_leg(fg, ctx, a, copy_gbs) if "batched" in a.legs else None
         dam = dam_leg(fg, ctx, a, copy_gbs) if a.dam_frames > 0 and "dam" in a.legs else None
         nb = nb_leg(fg, ctx, a, copy_gbs) if a.nb_frames > 0 and "nb" in a.legs else None
+        visc = visc_leg(fg, ctx, a, copy_gbs) if a.visc_frames > 0 and "visc" in a.legs else None
         proj = projection_leg(fg, ctx, a, copy_gbs) if a.proj_res > 0 and "proj" in a.legs else None
         served = served_leg(fg, ctx, a) if a.serve_seconds > 0 and "served" in a.legs else None
         ctx.close()
@@ -612,8 +713,8 @@ def main():
     if cpu:
         log(f"cpu baseline: {cpu['value']:.3e} particle-substeps/s")
     if b is None:   # a profiling run of other legs: their results alone
-        print(json.dumps({"dam_break_1024": dam, "nbflip_dam_break_1024": nb, "isolated_projection": proj,
-                          "served": served, "cpu_baseline": cpu}), flush=True)
+        print(json.dumps({"dam_break_1024": dam, "nbflip_dam_break_1024": nb, "viscous_batch_256": visc,
+                          "isolated_projection": proj, "served": served, "cpu_baseline": cpu}), flush=True)
         return
 
     dom = b.pop("dominant")
@@ -653,6 +754,7 @@ def main():
         "cpu_baseline": cpu,
         "dam_break_1024": dam,
         "nbflip_dam_break_1024": nb,
+        "viscous_batch_256": visc,
         "isolated_projection": proj,
         "served": served,
     }

@@ -42,8 +42,8 @@ MAX_LANES = 4
 REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_REJECTED, REQ_RELEASED, REQ_FAILED, REQ_CANCELLED = range(7)
 E_INVALID, E_DEVICE, E_BUDGET, E_CAPACITY, E_STATE, E_NUMERIC, E_CANCELLED = -1, -2, -3, -4, -5, -6, -7
 (K_ADVECT, K_SORT, K_P2G, K_SDF, K_LEVELSET, K_EXTRAPOLATE, K_PCG_A, K_PCG_B, K_G2P, K_RHS, K_APPLY,
- K_VCYCLE, K_MG_RESID, K_MG_COARSE, K_MG_DOWN0, K_MG_UP0, K_PARTICLES, K_MG_PP, K_NB_EULER) = range(19)
-K_NUM = 19
+ K_VCYCLE, K_MG_RESID, K_MG_COARSE, K_MG_DOWN0, K_MG_UP0, K_PARTICLES, K_MG_PP, K_NB_EULER, K_VISC) = range(20)
+K_NUM = 20
 PRECOND_MG, PRECOND_IPP = 0, 1
 
 

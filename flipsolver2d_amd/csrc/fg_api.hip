@@ -1471,6 +1471,7 @@ int run_viscosity(Group& G)
     launch_visc_setup(V.g.cells, st, V.d);
     launch_visc_red(one, st, V.d, 0, tol2);
     int it = 0, open = 1;
+    KProf kp(V, FG_K_VISC);
     while (open && it < kViscMaxIters) {
         for (int k = 0; k < kViscPoll; k++) {
             launch_visc_spmv(V.g.cells, st, V.d);
@@ -1485,6 +1486,10 @@ int run_viscosity(Group& G)
         HIPCHK(hipStreamSynchronize(st));
         open = V.L->bflags_host[0];
     }
+    // per cell and iteration, U and V together: spmv 24 (d, e, n) + 16 (p)
+    // + 16 (q); update 8 (d) + 16 x 4 read (p, x, r, q) + 16 x 2 written;
+    // direction 8 + 16 x 2 read + 16 written
+    kp.done(216 * V.cells * (int64_t)it);
     launch_visc_apply(V.g.cells, st, V.d);
     for (int k = 0; k < V.n; k++) {
         double sc[kViscScalars];

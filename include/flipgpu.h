@@ -320,7 +320,10 @@ enum fg_kernel_id {
                            * of the same levels (not timed with the per-level kernels) */
     FG_K_NB_EULER = 18,   /* [B] NB-FLIP's Euler pass (k_nb_euler): U, V and the level set
                            * read once and their advected copies written, 8 (F_U + F_V + C) */
-    FG_K_NUM = 19
+    FG_K_VISC = 19,       /* [B] the viscosity CG's iterations (k_visc_spmv, k_visc_update,
+                           * k_visc_dir, k_visc_red; timed as one stretch with its polls):
+                           * 216 B a cell and iteration for the U and V systems together */
+    FG_K_NUM = 20
 };
 int fg_ctx_profile_kernel(fg_ctx* ctx, int32_t kernel);
 /* Work units (as above) of every kernel family, summed over all launches
