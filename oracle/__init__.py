@@ -44,6 +44,7 @@ class OrcParams(C.Structure):
         ("reseed_seed", C.c_uint32),
         ("sim_method", C.c_int32), ("source_sdf", C.c_void_p),
         ("viscosity_enabled", C.c_int32), ("emitter_viscosity", C.c_void_p), ("fluid_viscosity", C.c_void_p),
+        ("source_id", C.c_void_p),
     ]
 
 
@@ -140,7 +141,9 @@ def _declare_oracle(L):
         "orc_nb_reseed": (C.c_int64, [P, S]),
         "orc_nb_substep": (C.c_int32, [P, S, C.c_float, T]),
         "orc_nb_init_objects": (None, [P, S, C.c_int32, _i32p, _f32p, C.c_int32, _i32p, _f32p, C.c_int32, _i32p,
-                                       _f32p, C.c_int32, _i32p, _f32p, _f32p]),
+                                       _f32p, C.c_int32, _i32p, _f32p, _f32p, _i32p]),
+        "orc_nb_viscosity_grid": (None, [P, S, _i32p]),
+        "orc_nb_combine_viscosity": (None, [P, S, _f32p]),
         "orc_extrapolate": (None, [_f32p, _u8p, C.c_int32, C.c_int32, C.c_int32]),
         "orc_body_forces": (None, [P, S, C.c_float]),
         "orc_pressure_rhs": (None, [P, S, _f64p]),
@@ -257,6 +260,7 @@ def make_params(n_i, n_j=None, *, dx=None, ppc=4, density=1000.0, gravity=(9.81,
     p._visc_keep = (evis, fvis)
     p.emitter_viscosity = evis.ctypes.data
     p.fluid_viscosity = fvis.ctypes.data
+    p.source_id = None
     return p
 
 
@@ -391,10 +395,12 @@ def init_nb_scene(params, solids=(), sources=(), sinks=(), fluids=(), capacity=N
         capacity = I * J * params.particles_per_cell * 2
     st = State(I, J, capacity)
     src = np.zeros(I * J, np.float32)
+    sid = np.zeros(I * J, np.int32)
     L.orc_nb_init_objects(C.byref(params), st.ref(), *_polys(solids), *_polys(sources), *_polys(sinks),
-                          *_polys(fluids), src)
-    params._source_sdf_keep = src
+                          *_polys(fluids), src, sid)
+    params._source_sdf_keep = (src, sid)
     params.source_sdf = src.ctypes.data
+    params.source_id = sid.ctypes.data
     return st
 
 

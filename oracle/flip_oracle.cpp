@@ -1897,6 +1897,7 @@ void orc_nb_prune(const OrcParams* p, const OrcState* s, uint8_t* alive)
     }
 }
 
+
 void orc_nb_combine(const OrcParams* p, OrcState* s, const float* adv_u, const float* adv_v, const float* adv_sdf)
 {
     const long I = p->size_i, J = p->size_j;
@@ -1955,10 +1956,60 @@ int64_t orc_nb_reseed(const OrcParams* p, OrcState* s)
                 s->vx[n] = vx;
                 s->vy[n] = vy;
                 s->pid[n] = s->next_pid++;
+                /* the emitter's viscosity, else the grid's at the position */
+                if (s->pvisc) {
+                    if (e != -1) s->pvisc[n] = (e >= 0 && e < p->n_emitters && p->emitter_viscosity) ? p->emitter_viscosity[e] : 0.f;
+                    else s->pvisc[n] = s->viscosity ? lerp_grid(s->viscosity, I, J, 0.5f, 0.5f, x, y) : 0.f;
+                }
                 added++;
             }
         }
     return added;
+}
+
+/* NBFlipSolver::centeredParamsToGridThread (nbflipsolver.cpp:441-504): the
+ * cells i-2 .. i+1 x j-2 .. j+1 (where the spline reaches) in (cell, id) order */
+void orc_nb_viscosity_grid(const OrcParams* p, OrcState* s, const int32_t* cs)
+{
+    const long I = p->size_i, J = p->size_j;
+    for (long i = 0; i < I; i++)
+        for (long j = 0; j < J; j++) {
+            float cw = 1e-10f, vv = 0.f;
+            bool known = false;
+            const long j0 = std::max<long>(0, j - 2), j1 = std::min<long>(J - 1, j + 1);
+            for (long a = std::max<long>(0, i - 2); a <= std::min<long>(I - 1, i + 1); a++)
+                for (int32_t k = cs[a * J + j0]; k < cs[a * J + j1 + 1]; k++) {
+                    const float w = qbspline(s->px[k] - static_cast<float>(i), s->py[k] - static_cast<float>(j));
+                    if (std::abs(w) > 1e-6f) {
+                        cw += w;
+                        vv += w * s->pvisc[k];
+                        known = true;
+                    }
+                }
+            s->viscosity[i * J + j] = known ? vv / cw : vv;
+        }
+}
+
+/* FlipSolver::afterTransfer's SOURCE cells (flipsolver2d.cpp:396-397),
+ * updateGridFromSources' viscosity (nbflipsolver.cpp:338-346) and
+ * combineCenteredGrids (400-419) */
+void orc_nb_combine_viscosity(const OrcParams* p, OrcState* s, float* adv_visc)
+{
+    const long I = p->size_i, J = p->size_j;
+    auto ev = [&](int e) { return (e >= 0 && e < p->n_emitters && p->emitter_viscosity) ? p->emitter_viscosity[e] : 0.f; };
+    for (long c = 0; c < I * J; c++) {
+        if (is_source(s->material[c]) && s->emitter_id && s->emitter_id[c] >= 0) s->viscosity[c] = ev(s->emitter_id[c]);
+        if (p->source_id && p->source_id[c] >= 0) {
+            s->viscosity[c] = ev(p->source_id[c]);
+            adv_visc[c] = ev(p->source_id[c]);
+        }
+    }
+    for (long i = 0; i < I; i++)
+        for (long j = 0; j < J; j++) {
+            const float sdf = lerp_grid(s->fluid_sdf, I, J, 0.f, 0.f, 0.5f + static_cast<float>(i),
+                                        static_cast<float>(j + 0.5));
+            if (!(sdf > kCombinationBand)) s->viscosity[i * J + j] = adv_visc[i * J + j];
+        }
 }
 
 int32_t orc_nb_substep(const OrcParams* p, OrcState* s, float dt, OrcStats* st)
@@ -1982,6 +2033,9 @@ int32_t orc_nb_substep(const OrcParams* p, OrcState* s, float dt, OrcStats* st)
     orc_euler_advect(s->u, s->v, I, J, dt, s->u, I + 1, J, 0.5f, 0.f, adv_u.data());
     orc_euler_advect(s->u, s->v, I, J, dt, s->v, I, J + 1, 0.f, 0.5f, adv_v.data());
     orc_euler_advect(s->u, s->v, I, J, dt, s->fluid_sdf, I, J, 0.f, 0.f, adv_sdf.data());
+    const bool visc = p->viscosity_enabled && s->viscosity && s->pvisc;
+    std::vector<float> adv_visc(visc ? C : 0);
+    if (visc) orc_euler_advect(s->u, s->v, I, J, dt, s->viscosity, I, J, 0.5f, 0.5f, adv_visc.data());
     stage(0);
     /* 2. prune + rebin */
     std::vector<int32_t> cs(C + 1);
@@ -1989,6 +2043,7 @@ int32_t orc_nb_substep(const OrcParams* p, OrcState* s, float dt, OrcStats* st)
     stage(3);
     /* 3. particleToGrid, extrapolate(10), the saved grid */
     orc_p2g(p, s, cs.data());
+    if (visc) orc_nb_viscosity_grid(p, s, cs.data());
     orc_extrapolate(s->u, s->u_valid, I + 1, J, 10);
     orc_extrapolate(s->v, s->v_valid, I, J + 1, 10);
     std::memcpy(s->saved_u, s->u, sizeof(float) * (I + 1) * J);
@@ -2001,6 +2056,7 @@ int32_t orc_nb_substep(const OrcParams* p, OrcState* s, float dt, OrcStats* st)
     orc_extrapolate_levelset_outside(s->fluid_sdf, I, J);
     orc_after_transfer(p, s);
     orc_nb_combine(p, s, adv_u.data(), adv_v.data(), adv_sdf.data());
+    if (visc) orc_nb_combine_viscosity(p, s, adv_visc.data());
     orc_extrapolate_levelset_inside(s->fluid_sdf, I, J);
     orc_update_materials(p, s);
     orc_body_forces(p, s, dt);
@@ -2026,8 +2082,25 @@ int32_t orc_nb_substep(const OrcParams* p, OrcState* s, float dt, OrcStats* st)
         orc_apply_pressure(p, s, dt, pr);
     }
     stage(7);
-    /* 6. solids, extrapolate, G2P */
+    /* 6. solids; viscosity and the second projection (46-53); extrapolate, G2P */
     orc_velocity_from_solids(p, s);
+    if (visc) {
+        orc_viscosity_apply(p, s, dt, 1e-12);
+        double* rhs = s->rhs;
+        double* pr = s->pressure;
+        std::vector<double> rhs_local, pr_local;
+        if (!rhs) { rhs_local.resize(C); rhs = rhs_local.data(); }
+        if (!pr) { pr_local.resize(C); pr = pr_local.data(); }
+        orc_pressure_rhs(p, s, rhs);
+        double rel = 0;
+        bool conv = false;
+        const int32_t it = solve(p, mat_a.data(), sA, rhs, pr, &rel, &conv);
+        if (st) {
+            st->pressure_iters = std::max(st->pressure_iters, it);
+            st->pressure_rel_resid = rel;
+        }
+        orc_apply_pressure(p, s, dt, pr);
+    }
     orc_extrapolate(s->u, s->u_valid, I + 1, J, 10);
     orc_extrapolate(s->v, s->v_valid, I, J + 1, 10);
     orc_g2p(p, s);
@@ -2049,7 +2122,7 @@ int32_t orc_nb_substep(const OrcParams* p, OrcState* s, float dt, OrcStats* st)
 void orc_nb_init_objects(const OrcParams* p, OrcState* s, int32_t n_solids, const int32_t* snv, const float* sv,
                          int32_t n_sources, const int32_t* srnv, const float* srv, int32_t n_sinks,
                          const int32_t* sknv, const float* skv, int32_t n_fluids, const int32_t* fnv, const float* fv,
-                         float* source_sdf)
+                         float* source_sdf, int32_t* source_id)
 {
     const long I = p->size_i, J = p->size_j;
     /* updateSinks, updateSources, updateSolids (no updateInitialFluid) */
@@ -2060,25 +2133,37 @@ void orc_nb_init_objects(const OrcParams* p, OrcState* s, int32_t n_solids, cons
         for (long j = 0; j < J; j++) {
             /* fluidSdfFromInitialFluid: centre in double, as updateSolids */
             float dist = flt_max;
+            int fid = -1;
             const float* v = fv;
             for (int32_t k = 0; k < n_fluids; k++) {
                 const float x = static_cast<float>((static_cast<float>(i) + 0.5) * dx);
                 const float y = static_cast<float>((static_cast<float>(j) + 0.5) * dx);
                 const float sd = orc_polygon_sdf(v, fnv[k], x, y) / dx;
-                if (sd < dist) dist = sd;
+                if (sd < dist) {
+                    dist = sd;
+                    fid = k;
+                }
                 v += 2 * fnv[k];
             }
             s->fluid_sdf[i * J + j] = dist;
-            if (dist < 0) s->material[i * J + j] = (int8_t)ORC_FLUID;
+            if (dist < 0) {
+                s->material[i * J + j] = (int8_t)ORC_FLUID;
+                if (s->viscosity && fid != -1) s->viscosity[i * J + j] = p->fluid_viscosity ? p->fluid_viscosity[fid] : 0.f;
+            }
             /* updateGridFromSources' distance: the cell corner (i dx, j dx) */
             float sdist = flt_max;
+            int sid = -1;
             v = srv;
             for (int32_t k = 0; k < n_sources; k++) {
                 const float sd = orc_polygon_sdf(v, srnv[k], static_cast<float>(i) * dx, static_cast<float>(j) * dx) / dx;
-                if (sd < sdist) sdist = sd;
+                if (sd < sdist) {
+                    sdist = sd;
+                    sid = k;
+                }
                 v += 2 * srnv[k];
             }
             if (source_sdf) source_sdf[i * J + j] = sdist;
+            if (source_id) source_id[i * J + j] = sdist < 0 ? sid : -1;
         }
     std::fill(s->particle_counts, s->particle_counts + I * J, 0);
     s->n_particles = 0;

@@ -71,6 +71,8 @@ typedef struct OrcParams {
     int32_t viscosity_enabled;     /* viscosityEnabled: the light model (viscositymodel.cpp:4-140) */
     const float* emitter_viscosity; /* [n_emitters] Emitter::viscosity, or NULL (0) */
     const float* fluid_viscosity;  /* [initial fluids] Emitter::viscosity, or NULL (0) */
+    const int32_t* source_id;      /* NB-FLIP: I*J the source whose corner distance is nearest and
+                                    * negative, else -1 (orc_nb_init_objects); may be NULL */
 } OrcParams;
 
 typedef struct OrcState {
@@ -247,7 +249,13 @@ void  orc_nb_init_objects(const OrcParams* p, OrcState* s,
                           int32_t n_sources, const int32_t* source_nverts, const float* source_verts,
                           int32_t n_sinks, const int32_t* sink_nverts, const float* sink_verts,
                           int32_t n_fluids, const int32_t* fluid_nverts, const float* fluid_verts,
-                          float* source_sdf_out);
+                          float* source_sdf_out, int32_t* source_id_out);
+/* NBFlipSolver::centeredParamsToGridThread (nbflipsolver.cpp:441-504): the
+ * viscosity grid from the particles, |w| > 1e-6, 0 where no weight. */
+void  orc_nb_viscosity_grid(const OrcParams* p, OrcState* s, const int32_t* cell_start);
+/* afterTransfer's SOURCE cells, updateGridFromSources' viscosity (to both
+ * grids) and combineCenteredGrids (nbflipsolver.cpp:338-346, 400-419). */
+void  orc_nb_combine_viscosity(const OrcParams* p, OrcState* s, float* adv_visc);
 
 #ifdef __cplusplus
 }

@@ -366,3 +366,86 @@ def test_nb_init_objects_fluid_and_source_distance():
                                              np.float32(j) * dx)) / dx
             assert ssdf[i, j] == s, (i, j)
     assert (st.material == O.SOLID).any() and (st.material == O.SOURCE).any()
+
+
+# ---------------------------------------------------------------------------
+# NB-FLIP with viscosity (nbflipsolver.cpp:66-113, 115-201, 291-349, 400-504)
+# ---------------------------------------------------------------------------
+@needs_ref
+def test_nb_viscosity_grid_of_one_particle():
+    """NBFlipSolver::centeredParamsToGridThread: |w| > 1e-6 (FlipSolver's
+    uses w > 1e-9), v w / (w + 1e-10) where known, 0 elsewhere."""
+    I, J = 9, 10
+    p = O.make_params(I, J)
+    st = O.State(I, J, 4)
+    st.n = 1
+    st.px[0], st.py[0] = 4.3, 5.8
+    st.pvisc[0] = 2.5
+    cs = np.zeros(I * J + 1, np.int32)
+    cs[4 * J + 5 + 1:] = 1
+    L.orc_nb_viscosity_grid(C.byref(p), st.ref(), cs)
+    g = st.viscosity.reshape(I, J)
+    for i in range(I):
+        for j in range(J):
+            w = np.float32(R.ref_quadratic_bspline(np.float32(4.3) - np.float32(i), np.float32(5.8) - np.float32(j)))
+            want = np.float32(np.float32(w * np.float32(2.5)) / np.float32(np.float32(1e-10) + w)) if abs(w) > 1e-6 \
+                else np.float32(0.0)
+            assert g[i, j] == want, (i, j)
+
+
+def test_nb_init_viscosity_of_the_nearest_fluid():
+    n = 32
+    p = O.make_params(n, sim=O.SIM_NBFLIP, viscosity=True, fluid_viscosity=[2.0, 9.0])
+    walls, _ = O.dam_break_geometry(n)
+    fluids = [O.box_polys(0.1, 0.45, 0.1, 0.5), O.box_polys(0.55, 0.9, 0.1, 0.5)]
+    st = O.init_nb_scene(p, solids=walls, fluids=fluids)
+    g = st.viscosity.reshape(n, n)
+    fl = st.fluid_sdf.reshape(n, n) < 0
+    xs = (np.arange(n) + 0.5) / n
+    left = xs[:, None] < 0.5
+    assert np.all(g[fl & left] == 2.0) and np.all(g[fl & ~left] == 9.0) and np.all(g[~fl] == 0.0)
+
+
+@needs_ref
+def test_nb_reseed_viscosity_from_the_grid():
+    I, J, ppc = 6, 5, 4
+    p, st = _reseed_setup(I, J, ppc, -2.0)
+    rng = np.random.default_rng(2)
+    st.viscosity[:] = rng.uniform(0, 5, I * J).astype(np.float32)
+    st.particle_counts[:] = 1
+    n = L.orc_nb_reseed(C.byref(p), st.ref())
+    want = np.zeros(n, np.float32)
+    R.ref_lerp(st.viscosity, I, J, 0.5, 0.5, np.ascontiguousarray(st.px[:n]), np.ascontiguousarray(st.py[:n]), n, want)
+    assert n == I * J * (ppc - 1) and np.array_equal(st.pvisc[:n], want)
+
+
+def test_nb_combine_viscosity_band_and_sources():
+    I, J = 12, 7
+    p = O.make_params(I, J, emitters=[(0.0, 0.0, False, 4.0), (0.0, 0.0, False, 6.0)])
+    st = O.State(I, J, 4)
+    # combined level set i - 5.5 at the lerp's samples: the cell centre
+    # (i + 1/2, j + 1/2) reads i - 5.5 -> above -2 from i = 4
+    st.fluid_sdf[:] = np.repeat(np.arange(I, dtype=np.float32), J) - np.float32(5.5)
+    st.viscosity[:] = 1.0
+    adv = np.full(I * J, 3.0, np.float32)
+    st.material[:] = O.FLUID
+    st.material[2 * J + 3] = O.SOURCE
+    st.emitter_id[:] = -1
+    st.emitter_id[2 * J + 3] = 0
+    sid = np.full(I * J, -1, np.int32)
+    sid[9 * J + 1] = 1            # a cell inside source 1 (corner distance < 0)
+    p.source_id = sid.ctypes.data
+    L.orc_nb_combine_viscosity(C.byref(p), st.ref(), adv)
+    g = st.viscosity.reshape(I, J)
+    for i in range(I):
+        for j in range(J):
+            if (i, j) == (9, 1):
+                want = 6.0                       # the source's viscosity, to both grids
+            elif i >= 4:
+                want = 1.0                       # the particle grid above the band
+            elif (i, j) == (2, 3):
+                want = 3.0                       # SOURCE cell's emitter value, then the advected one below the band
+            else:
+                want = 3.0
+            assert g[i, j] == want, (i, j)
+    assert adv[9 * J + 1] == 6.0
