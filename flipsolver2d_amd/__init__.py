@@ -411,7 +411,7 @@ class Scene:
                       "u": np.float32, "v": np.float32, "u_valid": np.uint8, "v_valid": np.uint8,
                       "material": np.int8, "solid_sdf": np.float32, "fluid_sdf": np.float32,
                       "solid_id": np.int32, "emitter_id": np.int32, "particle_counts": np.int32,
-                      "particle_viscosity": np.float32}.get(name)
+                      "particle_viscosity": np.float32, "viscosity": np.float32}.get(name)
                 if dt is None:
                     continue
                 a = np.ascontiguousarray(st[name], dt)

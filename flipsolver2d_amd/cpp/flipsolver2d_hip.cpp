@@ -177,8 +177,6 @@ FlipSolver::FlipSolver(const FlipSolverParameters* p, fg_ctx* sharedCtx, int dev
     if (!(params_.dx > 0.0)) params_.dx = params_.domainSizeI / params_.gridSizeI;
     if (params_.viscosityEnabled && params_.useHeavyViscosity)
         throw Error(FG_E_INVALID, "the heavy viscosity model is not built (the light one is)");
-    if (params_.viscosityEnabled && params_.simulationMethod == SIMULATION_NBFLIP)
-        throw Error(FG_E_INVALID, "viscosity is built for FlipSolver's step only");
     if (params_.viscosityEnabled && params_.parameterHandlingMethod != PARTICLE)
         throw Error(FG_E_INVALID, "viscosity is built for particle parameter handling only");
     if (params_.simulationMethod == SIMULATION_SMOKE || params_.simulationMethod == SIMULATION_FIRE)

@@ -370,6 +370,10 @@ void carve(fg_scene* s, Carver& cv)
         h.adv_v = cv.take<float>(FV);
         h.adv_sdf = cv.take<float>(C);
         h.src_sdf = cv.take<float>(C);
+        if (s->p.viscosity_enabled) {
+            h.adv_visc = cv.take<float>(C);
+            h.src_id = cv.take<int32_t>(C);
+        }
     }
 }
 
@@ -396,8 +400,6 @@ int validate_params(const fg_scene_params* p)
     if (p->sim_method == FG_SIM_NBFLIP && p->density_mode != FG_DENSITY_OFF)
         return fail(FG_E_INVALID, "NBFlipSolver::step has no density correction: density_mode must be OFF");
     if (p->viscosity_enabled != 0 && p->viscosity_enabled != 1) return fail(FG_E_INVALID, "bad viscosity_enabled");
-    if (p->viscosity_enabled && p->sim_method != FG_SIM_FLIP)
-        return fail(FG_E_INVALID, "viscosity is built for FlipSolver's step only");
     if (p->viscosity_enabled && p->density_mode != FG_DENSITY_OFF)
         return fail(FG_E_INVALID, "FlipSolver::step skips density correction with viscosity: density_mode must be OFF");
     return FG_OK;
@@ -485,6 +487,7 @@ int setup_scene_memory(fg_scene* s)
         HIPCHK(hipMemsetAsync(h.sid, 0xff, sizeof(int32_t) * s->C, st));
         HIPCHK(hipMemsetAsync(h.eid, 0xff, sizeof(int32_t) * s->C, st));
         if (h.nb) HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(h.src_sdf), 0x7f7fffff, s->C, st));
+        if (h.nb && h.visc_on) HIPCHK(hipMemsetAsync(h.src_id, 0xff, sizeof(int32_t) * s->C, st));
     }
     HIPCHK(hipMemcpyAsync(h.mg, &s->mgh, sizeof(MgLevels), hipMemcpyHostToDevice, st));
     FGCHK(upload_desc(s));
@@ -1769,11 +1772,12 @@ int run_nb_substep(Group& G, Timer& tm)
         kp.done(G.parts);
     }
     tm.lap(FG_STAGE_REBIN);
-    {   // particleToGrid, extrapolate(10): the saved grid
+    {   // particleToGrid (with NBFlipSolver's centeredParamsToGrid), extrapolate(10): the saved grid
         KProf kp(G, FG_K_P2G, true);
         launch_p2g(g.p2g, st, d);
         kp.done(0);
     }
+    if (G.visc) launch_visc_grid(g.cells, st, d);
     tm.lap(FG_STAGE_P2G);
     {
         KProf kp(G, FG_K_EXTRAPOLATE);
@@ -2476,6 +2480,10 @@ int fg_scene_set_state(fg_scene* s, const fg_scene_state* st)
     if (st->particle_counts)
         HIPCHK(hipMemcpyAsync(h.pcount, st->particle_counts, sizeof(int32_t) * s->C, hipMemcpyHostToDevice, q));
     else HIPCHK(hipMemsetAsync(h.pcount, 0, sizeof(int32_t) * s->C, q));
+    if (h.visc_on) {   // the viscosity grid (an NB-FLIP substep advects it)
+        if (st->viscosity) HIPCHK(hipMemcpyAsync(h.cvisc, st->viscosity, sizeof(float) * s->C, hipMemcpyHostToDevice, q));
+        else HIPCHK(hipMemsetAsync(h.cvisc, 0, sizeof(float) * s->C, q));
+    }
     if (st->solid_id) HIPCHK(hipMemcpyAsync(h.sid, st->solid_id, sizeof(int32_t) * s->C, hipMemcpyHostToDevice, q));
     else HIPCHK(hipMemsetAsync(h.sid, 0xff, sizeof(int32_t) * s->C, q));
     if (st->emitter_id)

@@ -207,6 +207,10 @@ struct SceneDev {
     double *vd, *ve, *vn, *vx, *vr, *vp, *vq;
     double* vpart;          // [kViscSlots][max cell blocks]
     double* vsc;            // kViscScalars
+    // NB-FLIP with viscosity: the advected viscosity grid, and per cell the
+    // source whose corner distance is nearest and negative (-1: none)
+    float* adv_visc;
+    int32_t* src_id;
 };
 // viscosity CG scalars (per system s = 0 (U), 1 (V)): bb, rz, pq, rr, alpha,
 // beta, done at index 2 k + s; the iteration counts at 14, 15

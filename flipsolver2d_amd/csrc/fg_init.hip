@@ -106,20 +106,32 @@ __global__ void __launch_bounds__(kBlock) k_init_cells(const SceneDev* scenes, I
         // cells; FLUID where negative, whatever the cell was
         const float x = (float)(((float)i + 0.5) * (double)dx), y = (float)(((float)j + 0.5) * (double)dx);
         fsdf = 3.40282347e+38f;
+        int fid = -1;
         for (int k = geo.set[3].first; k < geo.set[3].first + geo.set[3].n; k++) {
             const float sd = polygon_sdf(poly(k), nv(k), x, y) / dx;
-            if (sd < fsdf) fsdf = sd;
+            if (sd < fsdf) {
+                fsdf = sd;
+                fid = k;
+            }
         }
-        if (fsdf < 0) mat = M_FLUID;
+        if (fsdf < 0) {
+            mat = M_FLUID;
+            if (fid >= 0) visc = geo.visc[fid];   // the nearest fluid's viscosity
+        }
         // updateGridFromSources' distance (320-349) at the cell corner
         // (i dx, j dx); the sources never move, so it is taken once here
         const float cx = (float)i * dx, cy = (float)j * dx;
         float sdist = 3.40282347e+38f;
-        for (int k = geo.set[1].first; k < geo.set[1].first + geo.set[1].n; k++) {
+        int sid = -1;
+        for (int k = geo.set[1].first, e = 0; k < geo.set[1].first + geo.set[1].n; k++, e++) {
             const float sd = polygon_sdf(poly(k), nv(k), cx, cy) / dx;
-            if (sd < sdist) sdist = sd;
+            if (sd < sdist) {
+                sdist = sd;
+                sid = e;
+            }
         }
         s.src_sdf[c] = sdist;
+        if (s.visc_on) s.src_id[c] = sdist < 0 ? sid : -1;
         s.pcount[c] = 0;   // m_fluidParticleCounts.fill(0)
     }
     s.mat[c] = mat;
@@ -277,7 +289,10 @@ __global__ void __launch_bounds__(kBlock) k_reseed_write(const SceneDev* scenes)
         s.vel[idx] = v;
         s.pid[idx] = s.next_pid + (uint32_t)(off + a);
         s.key[idx] = 0;
-        if (s.visc_on) s.pvisc[idx] = (e >= 0 && e < s.n_emitters) ? s.emitters[e].w : 0.f;
+        // the emitter's viscosity; NB-FLIP: else the grid's at the position
+        if (s.visc_on)
+            s.pvisc[idx] = (e >= 0 && e < s.n_emitters) ? s.emitters[e].w
+                           : (s.nb && e == -1 ? lerp_grid(s.cvisc, I, J, 0.5f, 0.5f, x, y) : 0.f);
         const float v2 = v.x * v.x + v.y * v.y;
         m = v2 > m ? v2 : m;
         a++;

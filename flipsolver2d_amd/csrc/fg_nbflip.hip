@@ -38,6 +38,8 @@ __global__ void __launch_bounds__(kBlock) k_nb_euler(const SceneDev* scenes)
     if (j < J) s.adv_u[(size_t)i * J + j] = lerp_grid(s.uf, I + 1, J, 0.5f, 0.f, q.x, q.y);
     if (i < I) s.adv_v[(size_t)i * (J + 1) + j] = lerp_grid(s.vf, I, J + 1, 0.f, 0.5f, q.x, q.y);
     if (i < I && j < J) s.adv_sdf[(size_t)i * J + j] = lerp_grid(s.fsdf, I, J, 0.f, 0.f, q.x, q.y);
+    // with viscosity the viscosity grid too (offset (1/2, 1/2))
+    if (s.visc_on && i < I && j < J) s.adv_visc[(size_t)i * J + j] = lerp_grid(s.cvisc, I, J, 0.5f, 0.5f, q.x, q.y);
 }
 
 __global__ void __launch_bounds__(kBlock) k_nb_combine_sdf(const SceneDev* scenes)
@@ -92,6 +94,22 @@ __global__ void __launch_bounds__(kBlock) k_nb_combine_vel(const SceneDev* scene
         }
         const float sdf = lerp_grid(s.fsdf, I, J, 0.f, 0.f, 0.5f + (float)i, (float)j);
         s.v[k] = (sdf > kNbCombinationBand ? part : s.adv_v[k]) + dv;
+    }
+    // the viscosity grid: afterTransfer's SOURCE cells (flipsolver2d.cpp:396-397),
+    // updateGridFromSources' (nbflipsolver.cpp:338-346, to both grids),
+    // combineCenteredGrids at the cell centre (400-419)
+    if (s.visc_on && k < I * J) {
+        const int i = k / J, j = k - (k / J) * J;
+        float vis = s.cvisc[k], adv = s.adv_visc[k];
+        auto ev = [&](int e) { return (e >= 0 && e < s.n_emitters) ? s.emitters[e].w : 0.f; };
+        if (s.mat[k] == M_SOURCE && s.eid[k] >= 0) vis = ev(s.eid[k]);
+        const int sid = s.src_id[k];
+        if (sid >= 0) {
+            vis = ev(sid);
+            adv = ev(sid);
+        }
+        const float sdf = lerp_grid(s.fsdf, I, J, 0.f, 0.f, 0.5f + (float)i, (float)((double)j + 0.5));
+        s.cvisc[k] = sdf > kNbCombinationBand ? vis : adv;
     }
 }
 

@@ -37,17 +37,25 @@ __global__ void __launch_bounds__(kBlock) k_visc_grid(const SceneDev* scenes)
     if (c >= I * J) return;
     const int i = c / J, j = c - (c / J) * J;
     float wt = 1e-10f, vv = 0.0f;
+    bool known = false;
     const int j0 = max(0, j - 2), j1 = min(J - 1, j + 1);
     for (int a = max(0, i - 2); a <= min(I - 1, i + 1); a++) {
         const int k0 = s.cellStart[a * J + j0], k1 = s.cellStart[a * J + j1 + 1];
         for (int k = k0; k < k1; k++) {
             const float2 p = ldg2(s.pos, k);
             const float w = qbspline(p.x - (float)i, p.y - (float)j);
-            if (w > 1e-9f) {
+            // NBFlipSolver's own (nbflipsolver.cpp:441-504): |w| > 1e-6, and
+            // the sum divided only where some weight counted
+            if (s.nb ? fabsf(w) > 1e-6f : w > 1e-9f) {
                 wt += w;
                 vv += w * s.pvisc[k];
+                known = true;
             }
         }
+    }
+    if (s.nb) {   // its SOURCE cells are set in the combination (k_nb_combine_vel)
+        s.cvisc[c] = known ? vv / wt : vv;
+        return;
     }
     float v = vv / wt;
     if (s.mat[c] == M_SOURCE) {

@@ -165,8 +165,9 @@ typedef struct fg_scene_state {
                                    * and v above, the substep's final, extrapolated grid. */
     float    *particle_viscosity; /* [n_particles] the particles' viscosity property (a scene with
                                    * viscosity_enabled; on set optional: NULL gives 0) */
-    float    *viscosity;          /* I*J, get only: the viscosity grid (centeredParamsToGrid,
-                                   * flipsolver2d.cpp:1379-1431) of the last substep */
+    float    *viscosity;          /* I*J: the viscosity grid (centeredParamsToGrid,
+                                   * flipsolver2d.cpp:1379-1431) of the last substep; on set
+                                   * optional (NULL: 0): an NB-FLIP substep advects it */
 } fg_scene_state;
 
 typedef struct fg_stats {

@@ -19,6 +19,7 @@ def oracle_state_dict(st):
         "v_valid": st.v_valid.copy(), "material": st.material.copy(), "solid_sdf": st.solid_sdf.copy(),
         "fluid_sdf": st.fluid_sdf.copy(), "solid_id": st.solid_id.copy(), "emitter_id": st.emitter_id.copy(),
         "particle_counts": st.particle_counts.copy(), "particle_viscosity": st.pvisc[:n].copy(),
+        "viscosity": st.viscosity.copy(),
         "rng_epoch": int(st.c.rng_epoch), "next_pid": int(st.c.next_pid),
     }
 

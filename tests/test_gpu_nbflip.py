@@ -220,3 +220,73 @@ def test_nb_refusals(fg, ctx):
     with pytest.raises(fg.FgError) as e:
         fg.Scene(ctx, fg.make_params(32, sim=fg.SIM_NBFLIP, density_mode=fg.DENSITY_ON))
     assert e.value.code == fg.E_INVALID
+
+
+# ---------------------------------------------------------------------------
+# NB-FLIP with viscosity: the advected viscosity grid, its band combination,
+# the sources' viscosity and the reseed's (nbflipsolver.cpp:66-113, 115-201,
+# 338-346, 400-504), then the light model and the second projection
+# ---------------------------------------------------------------------------
+def _nb_visc_pair(fg, orc, ctx, n, ppc, seed=3):
+    walls, srcs, sinks, fluids = _objects(fg, n)
+    fluids = fluids + [fg.box(0.1, 0.25, 0.1, 0.3)]
+    emit = [(0.5, -0.25, True, 7.5)]
+    p = orc.make_params(n, ppc=ppc, tol=1e-12, sim=orc.SIM_NBFLIP, emitters=emit, reseed_seed=seed, viscosity=True,
+                        fluid_viscosity=[2.0, 20.0])
+    st = orc.init_nb_scene(p, solids=walls, sources=srcs, sinks=sinks, fluids=fluids)
+    sc = fg.Scene(ctx, fg.make_params(n, ppc=ppc, tol=1e-10, sim=fg.SIM_NBFLIP, viscosity=True))
+    sc.init_objects(solids=walls, sources=srcs, sinks=sinks, fluids=fluids, friction=np.zeros(len(walls), np.float32),
+                    emitters=emit, seed=seed, fluid_viscosity=[2.0, 20.0])
+    return p, st, sc
+
+
+def test_nb_viscous_substeps_vs_oracle(fg, orc, ctx):
+    n, ppc = 64, 8
+    p, st, sc = _nb_visc_pair(fg, orc, ctx, n, ppc)
+    L = orc.lib()
+    try:
+        g0 = sc.get_state()
+        assert np.array_equal(bits(g0["viscosity"]), bits(st.viscosity)) and st.viscosity.max() == 20.0
+        for k in range(4):
+            vel = L.orc_max_particle_velocity(st.ref())
+            dt = np.float32(min(5.0 / (vel + 1e-15), 1.0 / 60))
+            sc.set_state(oracle_state_dict(st))
+            gs = sc.substep(float(dt))
+            g = sc.get_state()
+            L.orc_nb_substep(C.byref(p), st.ref(), C.c_float(dt), C.byref(orc.OrcStats()))
+            N = st.n
+            _log(f"  NB viscous substep {k}: particles {N}, viscosity iters {gs.viscosity_iters}")
+            assert len(g["px"]) == N and np.array_equal(g["pid"], st.pid[:N])
+            assert np.array_equal(bits(g["px"]), bits(st.px[:N]))
+            assert np.array_equal(bits(g["particle_viscosity"]), bits(st.pvisc[:N]))
+            assert np.array_equal(bits(g["viscosity"]), bits(st.viscosity))
+            assert np.array_equal(bits(g["fluid_sdf"]), bits(st.fluid_sdf))
+            assert np.array_equal(g["material"], st.material)
+            e = {"p": rel_l2(g["pressure"], st.pressure), "u": rel_l2(g["u"], st.u), "v": rel_l2(g["v"], st.v),
+                 "vx": rel_l2(g["vx"], st.vx[:N]), "vy": rel_l2(g["vy"], st.vy[:N])}
+            assert max(e.values()) <= 1e-5, e
+        assert st.n > 0 and (st.pvisc[:st.n] == np.float32(7.5)).any()
+    finally:
+        sc.close()
+
+
+def test_nb_viscous_frames_vs_oracle(fg, orc, ctx):
+    n, ppc = 48, 8
+    p, st, sc = _nb_visc_pair(fg, orc, ctx, n, ppc)
+    L = orc.lib()
+    try:
+        for f in range(3):
+            gs = sc.step_frame()
+            os_ = orc.OrcStats()
+            subs = L.orc_step_frame(C.byref(p), st.ref(), C.byref(os_))
+            _log(f"  frame {f}: substeps gpu={gs.substeps} oracle={subs} particles gpu={gs.particles} oracle={st.n}")
+            assert gs.converged and gs.substeps == subs and gs.particles == st.n
+        g = sc.get_state()
+    finally:
+        sc.close()
+    N = st.n
+    assert np.array_equal(g["pid"], st.pid[:N])
+    e = {"px": rel_l2(g["px"], st.px[:N]), "u": rel_l2(g["u"], st.u), "v": rel_l2(g["v"], st.v),
+         "visc": rel_l2(g["viscosity"], st.viscosity), "pvisc": rel_l2(g["particle_viscosity"], st.pvisc[:N])}
+    _log("  after the frames, rel L2: " + " ".join(f"{k}={v:.2e}" for k, v in e.items()))
+    assert max(e.values()) <= 1e-5, e

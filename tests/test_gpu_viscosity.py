@@ -158,7 +158,7 @@ def test_viscous_source_reseeds_with_its_viscosity(fg, orc, ctx):
 
 
 def test_viscosity_refusals(fg, ctx):
-    for kw in ({"sim": fg.SIM_NBFLIP}, {"density_mode": fg.DENSITY_GATED}):
+    for kw in ({"density_mode": fg.DENSITY_GATED}, {"sim": fg.SIM_NBFLIP, "density_mode": fg.DENSITY_ON}):
         with pytest.raises(fg.FgError) as e:
             fg.Scene(ctx, fg.make_params(32, viscosity=True, **kw))
         assert e.value.code == fg.E_INVALID
