@@ -239,6 +239,7 @@ static int run_jsoninfo(const char* scene)
     g.clear();
     for (const Emitter& e : d.sources) {
         std::printf("source_velocity %.9g %.9g %d\n", e.velocity().x(), e.velocity().y(), (int)e.velocityTransfer());
+        std::printf("source_viscosity %.9g\n", e.viscosity());
         g.push_back(e.geometry());
     }
     print_polys("source", g);
@@ -246,7 +247,10 @@ static int run_jsoninfo(const char* scene)
     for (const Sink& s : d.sinks) g.push_back(s.geo());
     print_polys("sink", g);
     g.clear();
-    for (const Emitter& e : d.fluids) g.push_back(e.geometry());
+    for (const Emitter& e : d.fluids) {
+        std::printf("fluid_viscosity %.9g\n", e.viscosity());
+        g.push_back(e.geometry());
+    }
     print_polys("fluid", g);
     return 0;
 }

@@ -118,18 +118,23 @@ def test_facade_linear_solver_vs_oracle(fg, orc, ctx, tmp_path, maxabs):
     assert np.linalg.norm(x - xo) <= 1e-5 * np.linalg.norm(xo)
 
 
-def test_facade_json_scene_equals_c_abi(fg, orc, ctx, tmp_path):
-    """waterfall.json: 128 x 85 cells, nine obstacles, a source and a sink
-    (an "nbflip" scene: the loader builds NBFlipSolver)."""
-    scene = os.path.join(HERE, "golden", "scenes", "waterfall.json")
-    frames = 3
-    out = str(tmp_path / "wf.bin")
+@pytest.mark.parametrize("name,frames", [("waterfall.json", 3), ("viscosity_test.json", 3),
+                                         ("double_dam_break.json", 2)])
+def test_facade_json_scene_equals_c_abi(fg, orc, ctx, tmp_path, name, frames):
+    """A scene file of the reference through JsonSceneReader::loadJson and the
+    C++ API, against the same objects stepped through the C-ABI:
+    waterfall.json (128 x 85 cells, nine obstacles, a source and a sink; an
+    "nbflip" scene: the loader builds NBFlipSolver), viscosity_test.json
+    (NBFlipSolver with viscosity) and double_dam_break.json (FlipSolver with
+    viscosity)."""
+    scene = os.path.join(HERE, "golden", "scenes", name)
+    out = str(tmp_path / "scene.bin")
     _run(["json", scene, str(frames), out])
     info = {}
     lines = subprocess.run([fgbuild.DEMO, "jsoninfo", scene], capture_output=True, text=True,
                            check=True).stdout.splitlines()
     polys = {"solid": [], "source": [], "sink": [], "fluid": []}
-    fric, emit = [], []
+    fric, emit, svisc, fvisc = [], [], [], []
     for line in lines:
         k, *v = line.split()
         if k in polys:
@@ -138,29 +143,38 @@ def test_facade_json_scene_equals_c_abi(fg, orc, ctx, tmp_path):
             fric.append(float(v[0]))
         elif k == "source_velocity":
             emit.append((float(v[0]), float(v[1]), bool(int(v[2]))))
+        elif k == "source_viscosity":
+            svisc.append(float(v[0]))
+        elif k == "fluid_viscosity":
+            fvisc.append(float(v[0]))
         else:
             info[k] = float(v[0]) if len(v) == 1 else v
+    emit = [e + (sv,) for e, sv in zip(emit, svisc)]
     I, J = int(info["gridSizeI"]), int(info["gridSizeJ"])
     got = _read_state(out, I, J)
     g = info["globalAcceleration"]
+    visc = bool(int(info["viscosityEnabled"]))
     p = fg.make_params(I, J, dx=info["dx"], ppc=int(info["particlesPerCell"]), density=info["fluidDensity"],
                        gravity=(float(g[0]), float(g[1])), pic=info["picRatio"], cfl=info["cflNumber"],
                        particle_scale=info["particleScale"], fps=int(info["fps"]),
                        max_substeps=int(info["maxSubsteps"]), pcg_iter_limit=0,
-                       sim=fg.SIM_NBFLIP if int(info["simulationMethod"]) == 3 else fg.SIM_FLIP)
-    assert p.sim_method == fg.SIM_NBFLIP
+                       sim=fg.SIM_NBFLIP if int(info["simulationMethod"]) == 3 else fg.SIM_FLIP, viscosity=visc)
+    if name != "double_dam_break.json":
+        assert p.sim_method == fg.SIM_NBFLIP
     sc = fg.Scene(ctx, p)
     try:
         sc.init_objects(polys["solid"], polys["source"], polys["sink"], polys["fluid"], friction=fric, emitters=emit,
-                        seed=int(info["seed"]))
+                        seed=int(info["seed"]), fluid_viscosity=fvisc)
         for _ in range(frames):
             st = sc.step_frame()
         ref = sc.get_state()
     finally:
         sc.close()
-    _log(f"  waterfall {I}x{J}: {len(ref['px'])} particles after {frames} frames, "
-         f"{st.particles_reseeded} reseeded in the last")
-    assert len(ref["px"]) > 0 and (ref["material"] == fg.SOURCE).any() and (ref["material"] == fg.SINK).any()
+    _log(f"  {name} {I}x{J}: {len(ref['px'])} particles after {frames} frames, "
+         f"{st.particles_reseeded} reseeded in the last, viscosity iters {st.viscosity_iters}")
+    assert len(ref["px"]) > 0
+    if visc:
+        assert st.viscosity_iters > 0
     assert got["substeps"] == st.substeps
     for k in ("px", "py", "vx", "vy", "pid", "material", "u", "v"):
         assert np.array_equal(got[k].view(np.uint8), ref[k].view(np.uint8)), k
