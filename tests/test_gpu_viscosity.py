@@ -162,3 +162,37 @@ def test_viscosity_refusals(fg, ctx):
         with pytest.raises(fg.FgError) as e:
             fg.Scene(ctx, fg.make_params(32, viscosity=True, **kw))
         assert e.value.code == fg.E_INVALID
+
+
+def test_mixed_batch_with_viscous_scenes_equals_solo(fg, ctx):
+    """A batch of FLIP, viscous FLIP and viscous NB-FLIP scenes: the viscous
+    ones solve as a sub-group inside their class's group; every scene ends
+    bit-identical to its solo run."""
+    def make():
+        out = []
+        for k, (n, sim, visc) in enumerate([(48, fg.SIM_FLIP, False), (40, fg.SIM_FLIP, True),
+                                            (56, fg.SIM_NBFLIP, True), (33, fg.SIM_FLIP, True)]):
+            walls, fluids = fg.dam_break_geometry(n)
+            sc = fg.Scene(ctx, fg.make_params(n, ppc=4, sim=sim, viscosity=visc))
+            sc.init_objects(solids=walls, fluids=fluids, friction=np.zeros(4, np.float32), seed=k,
+                            fluid_viscosity=[3.0 + k])
+            out.append(sc)
+        return out
+    fields = ("px", "py", "vx", "vy", "pid", "u", "v", "viscosity", "particle_viscosity", "pressure")
+    solo = []
+    for sc in make():
+        for _ in range(3):
+            sc.step_frame()
+        solo.append(sc.get_state())
+        sc.close()
+    batch = make()
+    try:
+        for _ in range(3):
+            fg.batch_step_frame(ctx, batch)
+        got = [sc.get_state() for sc in batch]
+    finally:
+        for sc in batch:
+            sc.close()
+    for k in range(len(solo)):
+        for f in fields:
+            assert np.array_equal(bits(got[k][f]), bits(solo[k][f])), (k, f)
