@@ -95,8 +95,10 @@ MEASURED_TRAFFIC_PER_UNIT = {
                  "mg_down0": 1.063, "mg_up0": 0.948, "mg_resid": 1.306, "mg_pp": 1.143, "mg_coarse": 1.243},
     "dam1024": {"p2g": 1.107, "particles": 2.041, "sdf": 0.63, "pcg_a": 1.022, "pcg_b": 1.003,
                 "mg_down0": 1.029, "mg_up0": 0.939, "mg_resid": 1.21, "mg_pp": 1.113, "mg_coarse": 1.513},
-    "nb1024": {},
-    "visc256": {},
+    "nb1024": {"p2g": 1.3, "particles": 0.261, "sdf": 2.155, "pcg_a": 1.022, "pcg_b": 1.003, "mg_down0": 1.018,
+               "mg_up0": 0.939, "mg_resid": 1.211, "mg_pp": 1.114, "mg_coarse": 1.516, "nb_euler": 1.646},
+    "visc256": {"p2g": 1.224, "particles": 1.704, "sdf": 1.285, "pcg_a": 1.054, "pcg_b": 1.005, "mg_down0": 1.024,
+                "mg_up0": 0.937, "mg_resid": 1.203, "mg_pp": 1.093, "mg_coarse": 1.311, "visc_cg": 1.193},
     "proj4096": {"pcg_a": 1.863, "pcg_b": 1.0, "mg_down0": 1.006, "mg_up0": 1.983, "mg_resid": 1.259,
                  "mg_pp": 1.222, "mg_coarse": 1.386},
 }
