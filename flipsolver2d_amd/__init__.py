@@ -113,7 +113,7 @@ class Request(C.Structure):
     _fields_ = [("params", SceneParams), ("n_solids", C.c_int32), ("solid_nverts", C.c_void_p),
                 ("solid_verts", C.c_void_p), ("friction", C.c_void_p), ("n_fluids", C.c_int32),
                 ("fluid_nverts", C.c_void_p), ("fluid_verts", C.c_void_p), ("seed", C.c_uint32),
-                ("frames", C.c_int32), ("keep", C.c_int32)]
+                ("frames", C.c_int32), ("keep", C.c_int32), ("fluid_viscosity", C.c_void_p)]
 
 
 class RequestStatus(C.Structure):
@@ -626,7 +626,7 @@ class Server:
             lib().fg_server_destroy(self.h)
             self.h = C.c_void_p()
 
-    def submit(self, params, solids, fluids, frames, seed=42, friction=None, keep=False):
+    def submit(self, params, solids, fluids, frames, seed=42, friction=None, keep=False, fluid_viscosity=None):
         snv = np.array([len(w) for w in solids] or [0], np.int32)
         sv = np.ascontiguousarray(np.concatenate(solids).ravel() if solids else np.zeros(2), np.float32)
         fnv = np.array([len(f) for f in fluids] or [0], np.int32)
@@ -637,6 +637,8 @@ class Server:
         r.n_solids, r.solid_nverts, r.solid_verts, r.friction = len(solids), snv.ctypes.data, sv.ctypes.data, _ptr(fr)
         r.n_fluids, r.fluid_nverts, r.fluid_verts = len(fluids), fnv.ctypes.data, fv.ctypes.data
         r.seed, r.frames, r.keep = seed, frames, 1 if keep else 0
+        fvis = None if fluid_viscosity is None else np.ascontiguousarray(fluid_viscosity, np.float32)
+        r.fluid_viscosity = _ptr(fvis)
         rid = C.c_int64()
         _check(lib().fg_server_submit(self.h, C.byref(r), C.byref(rid)))
         self._params[rid.value] = params

@@ -28,7 +28,7 @@ struct Req {
     int64_t id = 0;
     fg_scene_params params{};
     std::vector<int32_t> snv, fnv;
-    std::vector<float> sv, fv, friction;
+    std::vector<float> sv, fv, friction, fvisc;
     int32_t n_solids = 0, n_fluids = 0;
     uint32_t seed = 0;
     int32_t frames = 0;
@@ -121,9 +121,15 @@ int admit(fg_server* srv, int* admitted)
             reject_head(srv, r, rc);
             continue;
         }
-        rc = fg_scene_init_geometry(sc, r.n_solids, r.snv.data(), r.sv.data(),
-                                    r.friction.empty() ? nullptr : r.friction.data(), r.n_fluids, r.fnv.data(),
-                                    r.fv.data(), r.seed);
+        {
+            fg_scene_objects o{};
+            o.solids = fg_polygons{r.n_solids, r.snv.data(), r.sv.data()};
+            o.friction = r.friction.empty() ? nullptr : r.friction.data();
+            o.fluids = fg_polygons{r.n_fluids, r.fnv.data(), r.fv.data()};
+            o.fluid_viscosity = r.fvisc.empty() ? nullptr : r.fvisc.data();
+            o.seed = r.seed;
+            rc = fg_scene_init_objects(sc, &o);
+        }
         if (rc != FG_OK) {
             // e.g. FG_E_CAPACITY: its fluid seeds more particles than max_particles
             fg_scene_destroy(sc);
@@ -198,6 +204,7 @@ int fg_server_submit(fg_server* srv, const fg_request* q, int64_t* id_out)
     if (ns) r.sv.assign(q->solid_verts, q->solid_verts + 2 * ns);
     if (nf) r.fv.assign(q->fluid_verts, q->fluid_verts + 2 * nf);
     if (q->friction && q->n_solids > 0) r.friction.assign(q->friction, q->friction + q->n_solids);
+    if (q->fluid_viscosity && q->n_fluids > 0) r.fvisc.assign(q->fluid_viscosity, q->fluid_viscosity + q->n_fluids);
     if (r.snv.empty()) r.snv.push_back(0);
     if (r.fnv.empty()) r.fnv.push_back(0);
     if (r.sv.empty()) r.sv.assign(2, 0.f);

@@ -480,6 +480,8 @@ typedef struct fg_request {
     uint32_t seed;
     int32_t frames;               /* frames to simulate (>= 1) */
     int32_t keep;                 /* 1: keep the scene after its last frame until fg_server_release */
+    const float* fluid_viscosity; /* [n_fluids] or NULL: the fluids' viscosity (params.viscosity_enabled);
+                                   * params.sim_method chooses FlipSolver or NBFlipSolver */
 } fg_request;
 
 enum fg_request_state {

@@ -92,3 +92,39 @@ def test_budget_gates_admission(fg, ctx):
     s = srv.summary()
     assert s.peak_resident_bytes <= budget and s.rejected == 1 and s.completed == 6
     srv.close()
+
+
+def test_served_nbflip_and_viscous_requests_equal_solo(fg, ctx):
+    """Requests of both solver classes, with and without viscosity, served
+    together (the classes step as separate groups, the viscous scenes' model
+    as a sub-group): each result equals the scene stepped alone."""
+    kinds = [(48, fg.SIM_NBFLIP, False), (40, fg.SIM_FLIP, True), (56, fg.SIM_NBFLIP, True), (32, fg.SIM_FLIP, False)]
+    frames = [2, 3, 2, 3]
+
+    def params(n, sim, visc):
+        return fg.make_params(n, ppc=4, tol=1e-6, sim=sim, viscosity=visc)
+
+    srv = fg.Server(ctx, hbm_budget=8 << 30, max_resident=4)
+    ids = {}
+    for k, (n, sim, visc) in enumerate(kinds):
+        walls, fluid = fg.dam_break_geometry(n)
+        ids[k] = srv.submit(params(n, sim, visc), walls, fluid, frames[k], seed=k, friction=np.zeros(4, np.float32),
+                            keep=True, fluid_viscosity=[4.0])
+    for _ in range(10):
+        t = srv.tick()
+        if t.queued == 0 and t.resident == 0:
+            break
+    for k, (n, sim, visc) in enumerate(kinds):
+        assert srv.status(ids[k]).state == fg.REQ_DONE
+        got = srv.scene(ids[k]).get_state()
+        sc = fg.Scene(ctx, params(n, sim, visc))
+        walls, fluid = fg.dam_break_geometry(n)
+        sc.init_objects(solids=walls, fluids=fluid, friction=np.zeros(4, np.float32), seed=k, fluid_viscosity=[4.0])
+        for _ in range(frames[k]):
+            sc.step_frame()
+        ref = sc.get_state()
+        sc.close()
+        _log(f"  served {n}^2 sim={sim} viscosity={visc}: {len(ref['px'])} particles")
+        for f in FIELDS + ("viscosity", "particle_viscosity"):
+            assert np.array_equal(got[f].view(np.uint8), ref[f].view(np.uint8)), (k, f)
+        srv.release(ids[k])
