@@ -93,7 +93,9 @@ enum fg_stage {
     FG_STAGE_LEVELSET = 5,        /* extrapolateLevelsetInside()               */
     FG_STAGE_EXTRAPOLATE = 6,     /* m_fluidVelocityGrid.extrapolate(10)       */
     FG_STAGE_FORCES = 7,          /* saved grid copy + applyBodyForces()       */
-    FG_STAGE_PROJECT = 8,         /* calcPressureRhs + solve + apply pressure  */
+    FG_STAGE_PROJECT = 8,         /* calcPressureRhs + solve + apply pressure
+                                   * (viscosity_enabled: then updateVelocityFromSolids,
+                                   * applyViscosity and project() again, 446-453) */
     FG_STAGE_SOLIDS = 9,          /* updateVelocityFromSolids()                */
     FG_STAGE_G2P = 10,            /* particleUpdate()                          */
     FG_STAGE_COUNT = 11,          /* countParticles() (+ reseed)               */
