@@ -131,6 +131,7 @@ struct fg_scene {
     SceneScalars* hsc = nullptr;  // pinned readback of the scalars
     int iters_hint[2] = {0, 0};   // first-sweep PCG iterations of the last solve (pressure, density)
     int ls_hint = 0;              // level-set bands the last substep ran
+    int ls_hint_out = 0;          // NB-FLIP: bands the last outside extrapolation launched
     int64_t n_bound = 0;          // upper bound of live particles (host mirror)
     int64_t n_live = 0;           // live particles at the last readback
     bool vmax_ready = false;
@@ -1526,16 +1527,22 @@ int run_viscosity(Group& G)
 int levelset_bands(Group& G, bool outside)
 {
     constexpr int kChunk = 8;
+    // the inside chain's hint is the band count the last one reached; the
+    // outside chain (NB-FLIP, far longer) keeps the count it launched
     int hint = 0;
-    for (fg_scene* s : G.sc) hint = std::max(hint, s->ls_hint);
-    int chunk = hint > 0 ? hint + 1 : kChunk;
+    for (fg_scene* s : G.sc) hint = std::max(hint, outside ? s->ls_hint_out : (s->ls_hint > 0 ? s->ls_hint + 1 : 0));
+    int chunk = hint > 0 ? hint : kChunk;
     int32_t flags[3];
+    int launched = 0;
     for (int band = 0; band < G.max_bands; band += chunk, chunk = kChunk) {
         const int end = std::min(G.max_bands, band + chunk);
         for (int k = band; k < end; k++) launch_levelset_band(G.g.ls, G.L->stream, G.d, k, outside);
+        launched = end;
         FGCHK(poll_group(G, end - 1, flags));
         if (!flags[2]) break;
     }
+    if (outside)
+        for (fg_scene* s : G.sc) s->ls_hint_out = launched;
     return FG_OK;
 }
 
